@@ -1,0 +1,226 @@
+/*
+ * kmpm.h -- C ABI of the MI355X-native explicit MPM step (libkmpm.so).
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every entry point
+ * returns an int status (KMPM_OK == 0); kmpm_status_string() maps it to the
+ * reference's error text.  A context owns every device buffer; the host
+ * passes copies in and receives copies out and no host pointer is retained.
+ *
+ * The entry points replace, one for one, the virtual step interface of the
+ * reference's Method class and the per-particle constitutive plugin calls:
+ *   Method virtuals ............ src/method.h:33-48
+ *   ULMPM implementation ....... src/ulmpm.cpp:88-563
+ *   Scheme loops ............... src/usl.cpp:43-93, src/musl.cpp:43-94
+ *   constitutive plugins ....... src/eos.h:40, src/strength.h:38-45,
+ *                                src/damage.h:35-41
+ *   plugin registry keys ....... src/material.cpp:40-66, src/style_*.h
+ * File:line citations are relative to the reference repository root.
+ */
+#ifndef KMPM_H
+#define KMPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  KMPM_OK = 0,
+  KMPM_ERR_ARG = 1,            /* invalid argument / unsupported option */
+  KMPM_ERR_HIP = 2,            /* HIP runtime error */
+  KMPM_ERR_NO_DEVICE = 3,      /* no usable gfx950 device */
+  KMPM_ERR_CAPACITY = 4,       /* output buffer too small */
+  KMPM_ERR_PARTICLE_LEFT_BOX = 10,  /* solid.cpp:617-627 */
+  KMPM_ERR_JACOBIAN = 11,           /* J<=0 with damage<1, solid.cpp:1208-1215 */
+  KMPM_ERR_DTCFL = 12,              /* dtCFL NaN or 0, ulmpm.cpp:535-544 */
+  KMPM_ERR_WAVE_SPEED = 13,         /* NaN wave speed, solid.cpp:1387-1398 */
+  KMPM_ERR_GRID = 14                /* grid too small for the stencil */
+};
+
+/* ---- options ----------------------------------------------------------- */
+/* method(ulmpm|tlmpm, ...) -> update.cpp:108-196, style_method.h */
+enum { KMPM_METHOD_ULMPM = 0, KMPM_METHOD_TLMPM = 1 };
+/* scheme(usl|musl) -> update.cpp:83-101, style_scheme.h */
+enum { KMPM_SCHEME_USL = 0, KMPM_SCHEME_MUSL = 1 };
+/* shape functions -> basis_functions.h:21-104 */
+enum { KMPM_SHAPE_LINEAR = 0, KMPM_SHAPE_CUBIC_SPLINE = 1 };
+/* arithmetic of the device step */
+enum { KMPM_PRECISION_FP64 = 0, KMPM_PRECISION_MIXED = 1 };
+
+typedef struct kmpm_config {
+  int dimension;          /* 2 or 3 (domain.cpp:496-551) */
+  int method;             /* KMPM_METHOD_* */
+  int scheme;             /* KMPM_SCHEME_* (reference default: MUSL, update.cpp:42-45) */
+  int shape;              /* KMPM_SHAPE_* */
+  int precision;          /* KMPM_PRECISION_* */
+  int reserved0;
+  double boxlo[3], boxhi[3];
+  double cellsize;        /* grid cell size h */
+  double pic_flip;        /* FLIP blending alpha (update.cpp:163-166) */
+  double dt;              /* dt of the first step (1e-16 unless set_dt, update.cpp:38) */
+  int    dt_constant;     /* nonzero after set_dt (update.cpp:63-70) */
+  int    reserved1;
+  double dt_factor;       /* dt_factor (default 0.9, update.cpp:40) */
+} kmpm_config;
+
+/* ---- materials: material(...) + eos/strength/damage styles ------------- */
+/* material types, material.cpp:250-340 */
+enum { KMPM_MAT_LINEAR = 0, KMPM_MAT_EOS_STRENGTH = 1 };
+/* eos(id, <key>, ...) keys: linear, shock, fluid (style_eos.h) */
+enum { KMPM_EOS_NONE = 0, KMPM_EOS_LINEAR = 1, KMPM_EOS_SHOCK = 2, KMPM_EOS_FLUID = 3 };
+/* strength(id, <key>, ...) keys: linear, johnson_cook, fluid (style_strength.h) */
+enum { KMPM_STRENGTH_NONE = 0, KMPM_STRENGTH_LINEAR = 1,
+       KMPM_STRENGTH_JOHNSON_COOK = 2, KMPM_STRENGTH_FLUID = 3 };
+/* damage(id, <key>, ...) keys: damage_johnson_cook (style_damage.h) */
+enum { KMPM_DAMAGE_NONE = 0, KMPM_DAMAGE_JOHNSON_COOK = 1 };
+
+typedef struct kmpm_material {
+  int type;               /* KMPM_MAT_* */
+  int eos, strength, damage;
+  /* Parameters in the order the reference constructors parse them:
+   *   linear material   lin_p      = rho0, E, nu               (material.cpp:279-289)
+   *   eos linear        eos_p      = rho0, K                   (eos_linear.cpp:52-53)
+   *   eos shock         eos_p      = rho0, K, c0, S, Gamma, cv, Tr, Q1, Q2 (eos_shock.cpp:53-61)
+   *   eos fluid         eos_p      = rho0, K, Gamma            (eos_fluid.cpp:43-45)
+   *   strength linear   strength_p = G                         (strength_linear.cpp:50)
+   *   strength JC       strength_p = G, A, B, n, epsdot0, C, m, Tr, Tm (strength_jc.cpp:55-63)
+   *   strength fluid    strength_p = mu                        (strength_fluid.cpp:35)
+   *   damage JC         damage_p   = d1, d2, d3, d4, d5, epsdot0, Tr, Tm (damage_jc.cpp:54-61)
+   */
+  double lin_p[3];
+  double eos_p[9];
+  double strength_p[9];
+  double damage_p[8];
+  /* Derived moduli, filled by kmpm_material_init (Mat ctors, material.cpp:720-769). */
+  double rho0, K, G, E, nu, lambda, signal_velocity;
+} kmpm_material;
+
+/* ---- regions used to populate solids and groups ------------------------ */
+enum { KMPM_REGION_BLOCK = 0, KMPM_REGION_CYLINDER = 1 };
+typedef struct kmpm_region {
+  int kind;               /* KMPM_REGION_* */
+  int axis;               /* cylinder axis: 0=x, 1=y, 2=z (2D disks use z) */
+  int interior;           /* 1: inside matches (default); 0: outside matches (region.cpp:65-72) */
+  int reserved;
+  /* block: xlo, xhi, ylo, yhi, zlo, zhi (region_block.cpp:51-135)
+   * cylinder: c1, c2, R, lo, hi (region_cylinder.cpp:60-110; 2D: lo=hi=0) */
+  double p[6];
+} kmpm_region;
+
+/* ---- particle interchange (host side, reference layouts) --------------- */
+/* Vectors are [np][3], tensors [np][3][3] row-major, as Eigen::Vector3d /
+ * Matrix3d per particle in solid.h:50-97.  On download any NULL pointer is
+ * skipped.  Particles are returned in ascending ptag order. */
+typedef struct kmpm_particles {
+  int64_t np;
+  int64_t *ptag;
+  int32_t *solid;         /* solid index in creation order */
+  int32_t *mat;           /* material index */
+  uint32_t *mask;         /* group bitmask (group.cpp:65-234) */
+  double *x0, *x, *v;
+  double *F, *sigma, *strain_el;
+  double *mass, *vol0, *vol, *rho0, *rho, *J;
+  double *eff_plastic_strain, *eff_plastic_strain_rate;
+  double *damage, *damage_init;
+} kmpm_particles;
+
+/* ---- host-side set-up (no device needed) -------------------------------- */
+const char *kmpm_status_string(int status);
+/* Derive rho0/K/G/E/nu/lambda/signal_velocity (Mat ctors, material.cpp:720-769). */
+int kmpm_material_init(kmpm_material *mat);
+/* Grid node counts per dimension (Grid::init, grid.cpp:151-173, one rank). */
+int kmpm_grid_size(const kmpm_config *cfg, int64_t n_out[3]);
+/* Lattice population of a region (Solid::populate, solid.cpp:1810-2336, one rank).
+ * Writes up to `capacity` particles into `out` (arrays must be allocated);
+ * *np_out receives the count (also when it exceeds capacity -> KMPM_ERR_CAPACITY).
+ * ptag = ptag_offset + i + 1 (solid.cpp:2320). */
+int kmpm_populate(const kmpm_config *cfg, const kmpm_region *region, int ppc1d,
+                  const kmpm_material *mat, int mat_index, int solid_index,
+                  int64_t ptag_offset, kmpm_particles *out, int64_t capacity,
+                  int64_t *np_out);
+/* Region::match for one point (region.cpp:65-72 + the style's inside()). */
+int kmpm_region_match(const kmpm_region *region, int dimension, double x, double y, double z);
+
+/* ---- device context ------------------------------------------------------ */
+typedef struct kmpm_ctx kmpm_ctx;
+
+/* Creates the context on the current HIP device: grid (Grid::init), material
+ * table, streams.  Replaces Update::create_method/create_scheme + Domain::set_dimension. */
+int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
+                    kmpm_ctx **out);
+int kmpm_ctx_destroy(kmpm_ctx *ctx);
+/* Upload all particles (all solids, ordered by solid).  Replaces the Solid
+ * particle vectors (solid.h:50-112). */
+int kmpm_upload_particles(kmpm_ctx *ctx, const kmpm_particles *p);
+int kmpm_download_particles(kmpm_ctx *ctx, kmpm_particles *p);
+/* Node group bitmask, one uint32 per node in tag order (Grid::mask, grid.h:94). */
+int kmpm_set_node_mask(kmpm_ctx *ctx, const uint32_t *mask, int64_t nnodes);
+/* fix(id, velocity_nodes, group, vx, vy, vz): v_update <- value(t),
+ * v <- value(t-dt) on the group's nodes (fix_velocity_nodes.cpp:130-271). */
+int kmpm_fix_velocity_nodes(kmpm_ctx *ctx, uint32_t groupbit, const int set[3],
+                            const double value[3], const double value_prev[3]);
+/* fix(id, body_force, group, gx, gy, gz): mb += m*g on nodes with m>0
+ * (fix_body_force.cpp:106-179). */
+int kmpm_fix_body_force(kmpm_ctx *ctx, uint32_t groupbit, const int set[3],
+                        const double value[3]);
+
+/* ---- step phases (Method virtuals, method.h:33-48) ------------------------ */
+/* compute_grid_weight_functions_and_gradients (ulmpm.cpp:88-337): particle
+ * cell indices + deterministic cell sort; weights are evaluated in the
+ * kernels that use them. */
+int kmpm_compute_grid_weight_functions_and_gradients(kmpm_ctx *ctx);
+/* particles_to_grid + post_particles_to_grid fixes (ulmpm.cpp:339-378). */
+int kmpm_particles_to_grid(kmpm_ctx *ctx);
+/* update_grid_state + post_update_grid_state fixes (ulmpm.cpp:436-440,
+ * grid.cpp:448-466). */
+int kmpm_update_grid_state(kmpm_ctx *ctx);
+/* USL: compute_rate_deformation_gradient(false) + grid_to_points +
+ * advance_particles + update_deformation_gradient + update_stress fused
+ * (usl.cpp:59-74). */
+int kmpm_grid_to_points_and_stress(kmpm_ctx *ctx);
+/* MUSL: grid_to_points + advance_particles (musl.cpp:61-67). */
+int kmpm_grid_to_points(kmpm_ctx *ctx);
+/* MUSL: velocities_to_grid + post_velocities_to_grid (ulmpm.cpp:476-496). */
+int kmpm_velocities_to_grid(kmpm_ctx *ctx);
+/* MUSL: compute_rate_deformation_gradient(true) + update_deformation_gradient
+ * + update_stress (musl.cpp:75-77). */
+int kmpm_update_stress(kmpm_ctx *ctx);
+/* update_time + adjust_dt (update.cpp:201-206, ulmpm.cpp:525-551). */
+int kmpm_adjust_dt(kmpm_ctx *ctx);
+/* nsteps complete steps of the configured scheme (USL::run / MUSL::run),
+ * stream-ordered, no host synchronisation inside. */
+int kmpm_step(kmpm_ctx *ctx, int64_t nsteps);
+/* Waits for the stream and returns the sticky device error (first offending
+ * particle tag in *ptag_out, may be NULL). */
+int kmpm_synchronize(kmpm_ctx *ctx, int64_t *ptag_out);
+
+/* ---- observables ---------------------------------------------------------- */
+/* dt, time, step counter (update.h) */
+int kmpm_get_time(kmpm_ctx *ctx, double *dt, double *time, int64_t *step);
+/* Ek = sum 1/2 m |v|^2, Es = sum 1/2 vol sigma:strain_el over particles whose
+ * mask has groupbit (compute_kinetic_energy.cpp:63-102, compute_strain_energy.cpp:64-117). */
+int kmpm_reduce_energy(kmpm_ctx *ctx, uint32_t groupbit, double *ek, double *es);
+/* Grid node state in tag order; NULL pointers skipped.  mass [nn], v/v_update [nn][3]. */
+int kmpm_download_grid(kmpm_ctx *ctx, double *mass, double *v, double *v_update,
+                       int64_t nnodes);
+/* Particle-cell indices (i0,j0,k0) of the last weight pass, [np][3] in ptag
+ * order, as ulmpm.cpp:163-165 / 206-208 computes them, and the cell sort
+ * permutation as the list of ptags in sorted order [np]. */
+int kmpm_download_cells(kmpm_ctx *ctx, int32_t *cell_ijk, int64_t *sorted_ptags,
+                        int64_t np);
+/* Tile shape (cells per tile per axis) of the cell sort key: particles are
+ * ordered by tile, then by cell inside the tile (x-major), then by ptag. */
+int kmpm_get_tile_shape(kmpm_ctx *ctx, int tile[3]);
+/* Dense stencil weights of the last weight pass for parity checks:
+ * per particle (ptag order) S = 2^d (linear) or 4^d (cubic) entries of
+ * node tag (-1 when the pair is dropped), w and grad w [3].  Arrays of
+ * np*S, np*S, np*S*3. */
+int kmpm_debug_weights(kmpm_ctx *ctx, int64_t *node_tag, double *w, double *wd,
+                       int64_t np);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMPM_H */

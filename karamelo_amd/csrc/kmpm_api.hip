@@ -1,0 +1,575 @@
+// kmpm_api.hip -- C ABI of the MI355X explicit MPM step: context, buffers,
+// phases and the stream-ordered step loop.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kmpm_ctx.h"
+
+using namespace kmpm;
+
+namespace kmpm {
+void tile_shape(int dim, int S1, int T[3]);
+}
+
+#define HIPCHK(x)                          \
+  do {                                     \
+    hipError_t e__ = (x);                  \
+    if (e__ != hipSuccess) return KMPM_ERR_HIP; \
+  } while (0)
+
+static int status_from_device(kmpm_ctx *c, int64_t *ptag_out) {
+  DevScalars sc;
+  if (hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    return KMPM_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return KMPM_ERR_HIP;
+  if (ptag_out) *ptag_out = (sc.err_pid >= 0 && sc.err_pid < c->a.np) ? c->h_ptag[sc.err_pid] : -1;
+  return sc.error;
+}
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void **)p, n * sizeof(T));
+}
+
+static void free_particle_buffers(kmpm_ctx *c) {
+  void *ptrs[] = {c->d_pa, c->d_pb, c->d_xo, c->d_ia, c->d_ib, c->d_key, c->d_rank,
+                  c->d_order_tmp, c->d_order, c->d_pmask};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  c->d_pa = c->d_pb = c->d_xo = nullptr;
+  c->d_ia = c->d_ib = c->d_key = c->d_rank = c->d_order_tmp = c->d_order = nullptr;
+  c->d_pmask = nullptr;
+  free(c->h_ptag);
+  free(c->h_solid);
+  free(c->h_pmask);
+  free(c->h_x0);
+  c->h_ptag = nullptr;
+  c->h_solid = nullptr;
+  c->h_pmask = nullptr;
+  c->h_x0 = nullptr;
+}
+
+static void refresh_args(kmpm_ctx *c) {
+  KArgs &a = c->a;
+  a.pa = c->d_pa;
+  a.pb = c->d_pb;
+  a.ia = c->d_ia;
+  a.ib = c->d_ib;
+  a.xo = c->d_xo;
+  a.key = c->d_key;
+  a.rank = c->d_rank;
+  a.order_tmp = c->d_order_tmp;
+  a.order = c->d_order;
+}
+
+static void swap_state(kmpm_ctx *c) {
+  std::swap(c->d_pa, c->d_pb);
+  std::swap(c->d_ia, c->d_ib);
+  refresh_args(c);
+}
+
+extern "C" {
+
+int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
+                    kmpm_ctx **out) {
+  if (!cfg || !out || nmat < 1 || nmat > 16 || !mats) return KMPM_ERR_ARG;
+  *out = nullptr;
+  if (cfg->method != KMPM_METHOD_ULMPM) return KMPM_ERR_ARG;
+  if (cfg->scheme != KMPM_SCHEME_USL && cfg->scheme != KMPM_SCHEME_MUSL) return KMPM_ERR_ARG;
+  if (cfg->shape != KMPM_SHAPE_LINEAR && cfg->shape != KMPM_SHAPE_CUBIC_SPLINE) return KMPM_ERR_ARG;
+  if (cfg->precision != KMPM_PRECISION_FP64) return KMPM_ERR_ARG;
+  if (cfg->dimension != 2 && cfg->dimension != 3) return KMPM_ERR_ARG;
+  int64_t n3[3];
+  int st = kmpm_grid_size(cfg, n3);
+  if (st != KMPM_OK) return st;
+  const int S1 = cfg->shape == KMPM_SHAPE_LINEAR ? 2 : 4;
+  for (int d = 0; d < cfg->dimension; d++)
+    if (n3[d] < S1) return KMPM_ERR_GRID;  // node tags must not wrap inside one stencil
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return KMPM_ERR_NO_DEVICE;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return KMPM_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return KMPM_ERR_NO_DEVICE;
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) return KMPM_ERR_NO_DEVICE;
+
+  kmpm_ctx *c = (kmpm_ctx *)calloc(1, sizeof(kmpm_ctx));
+  c->cfg = *cfg;
+  c->device = dev;
+  c->cu_count = prop.multiProcessorCount;
+  c->nmat = nmat;
+  memcpy(c->mats_host, mats, nmat * sizeof(kmpm_material));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    free(c);
+    return KMPM_ERR_HIP;
+  }
+  KArgs &a = c->a;
+  a.dim = cfg->dimension;
+  a.S1 = S1;
+  a.scheme = cfg->scheme;
+  for (int d = 0; d < 3; d++) {
+    a.n[d] = (int)n3[d];
+    a.lo[d] = d < cfg->dimension ? cfg->boxlo[d] : 0.0;
+    a.hi[d] = d < cfg->dimension ? cfg->boxhi[d] : 0.0;
+  }
+  a.nn = n3[0] * n3[1] * n3[2];
+  a.h = cfg->cellsize;
+  a.inv_h = 1.0 / cfg->cellsize;  // ulmpm.cpp:122
+  a.flip = cfg->pic_flip;
+  a.dt_factor = cfg->dt_factor;
+  a.dt_constant = cfg->dt_constant;
+  a.nmat = nmat;
+  TileGeom &g = a.tg;
+  tile_shape(a.dim, S1, g.T);
+  g.ntiles = 1;
+  g.cpt = 1;
+  g.bn = 1;
+  g.off = S1 == 4 ? 1 : 0;
+  for (int d = 0; d < 3; d++) {
+    g.nt[d] = (a.n[d] + g.T[d] - 1) / g.T[d];
+    g.E[d] = d < a.dim ? g.T[d] + S1 - 1 : 1;
+    g.ntiles *= g.nt[d];
+    g.cpt *= g.T[d];
+    g.bn *= g.E[d];
+  }
+  c->ncells = (int64_t)g.ntiles * g.cpt;
+  if (c->ncells >= 2147483647LL) {
+    (void)hipStreamDestroy(c->stream);
+    free(c);
+    return KMPM_ERR_ARG;
+  }
+  // material table
+  std::vector<DevMat> dm(nmat);
+  for (int i = 0; i < nmat; i++) {
+    const kmpm_material &m = mats[i];
+    DevMat &d = dm[i];
+    memset(&d, 0, sizeof(d));
+    d.type = m.type;
+    d.eos = m.eos;
+    d.strength = m.strength;
+    d.damage = m.damage;
+    d.rho0 = m.rho0;
+    d.K = m.K;
+    d.G = m.G;
+    d.lambda = m.lambda;
+    d.signal_velocity = m.signal_velocity;
+    d.cfl_k43g = m.K + FOUR_THIRD * m.G;
+    memcpy(d.eos_p, m.eos_p, sizeof(d.eos_p));
+    memcpy(d.strength_p, m.strength_p, sizeof(d.strength_p));
+    memcpy(d.damage_p, m.damage_p, sizeof(d.damage_p));
+  }
+  bool ok = dalloc(&c->d_mats, nmat) == hipSuccess &&
+            hipMemcpy(c->d_mats, dm.data(), nmat * sizeof(DevMat), hipMemcpyHostToDevice) ==
+                hipSuccess;
+  ok = ok && dalloc(&c->d_gmass, a.nn) == hipSuccess && dalloc(&c->d_gv, 3 * a.nn) == hipSuccess &&
+       dalloc(&c->d_gvu, 3 * a.nn) == hipSuccess && dalloc(&c->d_gmask, a.nn) == hipSuccess;
+  ok = ok && dalloc(&c->d_cell_count, c->ncells + 1) == hipSuccess &&
+       dalloc(&c->d_cell_start, c->ncells + 1) == hipSuccess;
+  ok = ok && dalloc(&c->d_tile_list, g.ntiles) == hipSuccess &&
+       dalloc(&c->d_tile_slot, g.ntiles) == hipSuccess &&
+       dalloc(&c->d_dil_list, g.ntiles) == hipSuccess && dalloc(&c->d_sc, 1) == hipSuccess;
+  if (ok) {
+    c->scan_tmp_bytes = scan_temp_bytes(c->ncells + 1);
+    ok = hipMalloc(&c->d_scan_tmp, c->scan_tmp_bytes ? c->scan_tmp_bytes : 16) == hipSuccess;
+  }
+  if (ok) {
+    std::vector<uint32_t> ones(a.nn, 1u);  // Grid::grow sets mask = 1 (grid.cpp:429)
+    ok = hipMemcpy(c->d_gmask, ones.data(), a.nn * sizeof(uint32_t), hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemset(c->d_gmass, 0, a.nn * sizeof(double)) == hipSuccess &&
+         hipMemset(c->d_gv, 0, 3 * a.nn * sizeof(double)) == hipSuccess &&
+         hipMemset(c->d_gvu, 0, 3 * a.nn * sizeof(double)) == hipSuccess &&
+         hipMemset(c->d_tile_slot, 0xff, g.ntiles * sizeof(int)) == hipSuccess;
+    DevScalars sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.dt = cfg->dt;
+    sc.err_pid = -1;
+    ok = ok && hipMemcpy(c->d_sc, &sc, sizeof(sc), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  a.mats = c->d_mats;
+  a.gmass = c->d_gmass;
+  a.gv = c->d_gv;
+  a.gvu = c->d_gvu;
+  a.gmask = c->d_gmask;
+  a.cell_count = c->d_cell_count;
+  a.cell_start = c->d_cell_start;
+  a.tile_list = c->d_tile_list;
+  a.tile_slot = c->d_tile_slot;
+  a.dil_list = c->d_dil_list;
+  a.sc = c->d_sc;
+  if (!ok) {
+    kmpm_ctx_destroy(c);
+    return KMPM_ERR_HIP;
+  }
+  *out = c;
+  return KMPM_OK;
+}
+
+int kmpm_ctx_destroy(kmpm_ctx *c) {
+  if (!c) return KMPM_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_particle_buffers(c);
+  void *ptrs[] = {c->d_mats, c->d_cell_count, c->d_cell_start, c->d_tile_list, c->d_tile_slot,
+                  c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
+                  c->d_sc, c->d_scan_tmp};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  free(c);
+  return KMPM_OK;
+}
+
+int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
+  if (!c || !p || p->np < 0 || p->np >= 2147483647LL) return KMPM_ERR_ARG;
+  if (!p->ptag || !p->solid || !p->mat || !p->x || !p->v || !p->F || !p->sigma ||
+      !p->strain_el || !p->mass || !p->vol0 || !p->eff_plastic_strain ||
+      !p->eff_plastic_strain_rate || !p->damage || !p->damage_init)
+    return KMPM_ERR_ARG;
+  const int64_t np = p->np;
+  for (int64_t i = 0; i < np; i++) {
+    if (p->mat[i] < 0 || p->mat[i] >= c->nmat) return KMPM_ERR_ARG;
+    if (i > 0 && (p->ptag[i] <= p->ptag[i - 1] || p->solid[i] < p->solid[i - 1]))
+      return KMPM_ERR_ARG;  // ascending tags, ordered by solid
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  free_particle_buffers(c);
+  const int cap = (int)std::max<int64_t>(np, 1);
+  KArgs &a = c->a;
+  a.np = (int)np;
+  a.cap = cap;
+  HIPCHK(dalloc(&c->d_pa, (size_t)NDF * cap));
+  HIPCHK(dalloc(&c->d_pb, (size_t)NDF * cap));
+  HIPCHK(dalloc(&c->d_xo, (size_t)3 * cap));
+  HIPCHK(dalloc(&c->d_ia, (size_t)NIF * cap));
+  HIPCHK(dalloc(&c->d_ib, (size_t)NIF * cap));
+  HIPCHK(dalloc(&c->d_key, cap));
+  HIPCHK(dalloc(&c->d_rank, cap));
+  HIPCHK(dalloc(&c->d_order_tmp, cap));
+  HIPCHK(dalloc(&c->d_order, cap));
+  HIPCHK(dalloc(&c->d_pmask, cap));
+  // scratch: one partial node block per active tile (at most min(tiles, np))
+  const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(a.tg.ntiles, np));
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  c->d_scratch = nullptr;
+  HIPCHK(dalloc(&c->d_scratch, (size_t)slots * 7 * a.tg.bn));
+  a.scratch = c->d_scratch;
+  a.scratch_slots = (int)slots;
+
+  c->h_ptag = (int64_t *)malloc(sizeof(int64_t) * cap);
+  c->h_solid = (int32_t *)malloc(sizeof(int32_t) * cap);
+  c->h_pmask = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+  c->h_x0 = (double *)malloc(sizeof(double) * 3 * cap);
+  std::vector<double> hd((size_t)NDF * cap, 0.0);
+  std::vector<int> hi((size_t)NIF * cap, 0);
+  auto D = [&](int f, int64_t i) -> double & { return hd[(size_t)f * cap + i]; };
+  for (int64_t i = 0; i < np; i++) {
+    c->h_ptag[i] = p->ptag[i];
+    c->h_solid[i] = p->solid[i];
+    c->h_pmask[i] = p->mask ? p->mask[i] : 1u;
+    for (int d = 0; d < 3; d++) {
+      c->h_x0[3 * i + d] = p->x0 ? p->x0[3 * i + d] : p->x[3 * i + d];
+      D(PX + d, i) = p->x[3 * i + d];
+      D(VX + d, i) = p->v[3 * i + d];
+    }
+    for (int e = 0; e < 9; e++) D(F00 + e, i) = p->F[9 * i + e];
+    const double *s = &p->sigma[9 * i], *e = &p->strain_el[9 * i];
+    const int sym[6] = {0, 4, 8, 1, 2, 5};
+    for (int k = 0; k < 6; k++) {
+      D(SXX + k, i) = s[sym[k]];
+      D(EXX + k, i) = e[sym[k]];
+    }
+    D(EP, i) = p->eff_plastic_strain[i];
+    D(EPDOT, i) = p->eff_plastic_strain_rate[i];
+    D(DINIT, i) = p->damage_init[i];
+    D(DMG, i) = p->damage[i];
+    D(MASS, i) = p->mass[i];
+    D(VOL0, i) = p->vol0[i];
+    D(VOL, i) = p->vol ? p->vol[i] : p->vol0[i];
+    hi[(size_t)PID * cap + i] = (int)i;
+    hi[(size_t)MATI * cap + i] = p->mat[i];
+  }
+  HIPCHK(hipMemcpy(c->d_pa, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_ia, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_pmask, c->h_pmask, np * sizeof(uint32_t), hipMemcpyHostToDevice));
+  refresh_args(c);
+  c->keys_valid = false;
+  c->sorted = false;
+  return KMPM_OK;
+}
+
+int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
+  if (!c || !p) return KMPM_ERR_ARG;
+  const int64_t np = c->a.np;
+  if (p->np < np) return KMPM_ERR_CAPACITY;
+  const int cap = c->a.cap;
+  std::vector<double> hd((size_t)NDF * cap);
+  std::vector<int> hi((size_t)NIF * cap);
+  HIPCHK(hipMemcpyAsync(hd.data(), c->d_pa, hd.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(hi.data(), c->d_ia, hi.size() * sizeof(int), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto D = [&](int f, int64_t i) { return hd[(size_t)f * cap + i]; };
+  for (int64_t i = 0; i < np; i++) {
+    const int64_t q = hi[(size_t)PID * cap + i];  // output index = upload index
+    if (q < 0 || q >= np) return KMPM_ERR_HIP;
+    const int mat = hi[(size_t)MATI * cap + i];
+    if (p->ptag) p->ptag[q] = c->h_ptag[q];
+    if (p->solid) p->solid[q] = c->h_solid[q];
+    if (p->mat) p->mat[q] = mat;
+    if (p->mask) p->mask[q] = c->h_pmask[q];
+    for (int d = 0; d < 3; d++) {
+      if (p->x0) p->x0[3 * q + d] = c->h_x0[3 * q + d];
+      if (p->x) p->x[3 * q + d] = D(PX + d, i);
+      if (p->v) p->v[3 * q + d] = D(VX + d, i);
+    }
+    if (p->F)
+      for (int e = 0; e < 9; e++) p->F[9 * q + e] = D(F00 + e, i);
+    const int full[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    for (int e = 0; e < 9; e++) {
+      if (p->sigma) p->sigma[9 * q + e] = D(SXX + full[e], i);
+      if (p->strain_el) p->strain_el[9 * q + e] = D(EXX + full[e], i);
+    }
+    const double *F = nullptr;
+    double Fl[9];
+    for (int e = 0; e < 9; e++) Fl[e] = D(F00 + e, i);
+    F = Fl;
+    const double J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                     F[2] * (F[3] * F[7] - F[4] * F[6]);
+    const double rho0 = c->mats_host[mat].rho0;
+    if (p->mass) p->mass[q] = D(MASS, i);
+    if (p->vol0) p->vol0[q] = D(VOL0, i);
+    if (p->vol) p->vol[q] = D(VOL, i);
+    if (p->rho0) p->rho0[q] = rho0;
+    if (p->rho) p->rho[q] = rho0 / J;
+    if (p->J) p->J[q] = J;
+    if (p->eff_plastic_strain) p->eff_plastic_strain[q] = D(EP, i);
+    if (p->eff_plastic_strain_rate) p->eff_plastic_strain_rate[q] = D(EPDOT, i);
+    if (p->damage) p->damage[q] = D(DMG, i);
+    if (p->damage_init) p->damage_init[q] = D(DINIT, i);
+  }
+  return KMPM_OK;
+}
+
+int kmpm_set_node_mask(kmpm_ctx *c, const uint32_t *mask, int64_t nn) {
+  if (!c || !mask || nn != c->a.nn) return KMPM_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(c->d_gmask, mask, nn * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return KMPM_OK;
+}
+
+int kmpm_fix_velocity_nodes(kmpm_ctx *c, uint32_t bit, const int set[3], const double val[3],
+                            const double prev[3]) {
+  if (!c || !set || !val || !prev || c->a.nfv >= MAX_FIX) return KMPM_ERR_ARG;
+  DevFixVel &f = c->a.fv[c->a.nfv++];
+  f.bit = bit;
+  for (int d = 0; d < 3; d++) {
+    f.set[d] = set[d] && d < c->a.dim;  // fix_velocity_nodes.cpp:81-116
+    f.val[d] = val[d];
+    f.prev[d] = prev[d];
+  }
+  return KMPM_OK;
+}
+
+int kmpm_fix_body_force(kmpm_ctx *c, uint32_t bit, const int set[3], const double val[3]) {
+  if (!c || !set || !val || c->a.nbf >= MAX_FIX) return KMPM_ERR_ARG;
+  DevFixBF &f = c->a.bf[c->a.nbf++];
+  f.bit = bit;
+  for (int d = 0; d < 3; d++) {
+    f.set[d] = set[d];
+    f.val[d] = val[d];
+  }
+  return KMPM_OK;
+}
+
+// ---- phases ---------------------------------------------------------------
+int kmpm_compute_grid_weight_functions_and_gradients(kmpm_ctx *c) {
+  if (!c) return KMPM_ERR_ARG;
+  if (!c->keys_valid) {
+    HIPCHK(launch_keys(c));
+    c->keys_valid = true;
+  }
+  HIPCHK(launch_sort(c));
+  c->sorted = true;
+  return KMPM_OK;
+}
+
+int kmpm_particles_to_grid(kmpm_ctx *c) {
+  if (!c || !c->sorted) return KMPM_ERR_ARG;
+  HIPCHK(launch_p2g(c, 0));
+  return KMPM_OK;
+}
+
+int kmpm_update_grid_state(kmpm_ctx *c) {
+  if (!c || !c->sorted) return KMPM_ERR_ARG;
+  HIPCHK(launch_grid_update(c, 0));
+  return KMPM_OK;
+}
+
+int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
+  if (!c || !c->sorted) return KMPM_ERR_ARG;
+  HIPCHK(launch_g2p_usl(c));
+  swap_state(c);
+  c->sorted = false;
+  c->keys_valid = true;
+  return KMPM_OK;
+}
+
+int kmpm_grid_to_points(kmpm_ctx *c) {
+  if (!c || !c->sorted) return KMPM_ERR_ARG;
+  HIPCHK(launch_g2p_musl_a(c));
+  return KMPM_OK;
+}
+
+int kmpm_velocities_to_grid(kmpm_ctx *c) {
+  if (!c || !c->sorted) return KMPM_ERR_ARG;
+  HIPCHK(launch_p2g(c, 1));
+  HIPCHK(launch_grid_update(c, 1));
+  return KMPM_OK;
+}
+
+int kmpm_update_stress(kmpm_ctx *c) {
+  if (!c || !c->sorted) return KMPM_ERR_ARG;
+  HIPCHK(launch_stress_musl_b(c));
+  swap_state(c);
+  c->sorted = false;
+  c->keys_valid = true;
+  return KMPM_OK;
+}
+
+int kmpm_adjust_dt(kmpm_ctx *c) {
+  if (!c) return KMPM_ERR_ARG;
+  HIPCHK(launch_adjust_dt(c));
+  return KMPM_OK;
+}
+
+int kmpm_step(kmpm_ctx *c, int64_t nsteps) {
+  if (!c || nsteps < 0) return KMPM_ERR_ARG;
+  for (int64_t k = 0; k < nsteps; k++) {
+    int s = kmpm_compute_grid_weight_functions_and_gradients(c);
+    if (s) return s;
+    if ((s = kmpm_particles_to_grid(c))) return s;
+    if ((s = kmpm_update_grid_state(c))) return s;
+    if (c->cfg.scheme == KMPM_SCHEME_USL) {
+      if ((s = kmpm_grid_to_points_and_stress(c))) return s;
+    } else {
+      if ((s = kmpm_grid_to_points(c))) return s;
+      if ((s = kmpm_velocities_to_grid(c))) return s;
+      if ((s = kmpm_update_stress(c))) return s;
+    }
+    if ((s = kmpm_adjust_dt(c))) return s;
+  }
+  return KMPM_OK;
+}
+
+int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
+  if (!c) return KMPM_ERR_ARG;
+  return status_from_device(c, ptag_out);
+}
+
+int kmpm_get_time(kmpm_ctx *c, double *dt, double *time, int64_t *step) {
+  if (!c) return KMPM_ERR_ARG;
+  DevScalars sc;
+  HIPCHK(hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (dt) *dt = sc.dt;
+  if (time) *time = sc.time;
+  if (step) *step = sc.step;
+  return KMPM_OK;
+}
+
+int kmpm_reduce_energy(kmpm_ctx *c, uint32_t bit, double *ek, double *es) {
+  if (!c || !ek || !es) return KMPM_ERR_ARG;
+  double *d_out = nullptr;
+  HIPCHK(hipMalloc(&d_out, (2 + 2 * 64) * sizeof(double)));
+  hipError_t e = launch_energy(c, bit, d_out);
+  double h[2] = {0, 0};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h, d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return KMPM_ERR_HIP;
+  *ek = h[0];
+  *es = h[1];
+  return KMPM_OK;
+}
+
+int kmpm_download_grid(kmpm_ctx *c, double *mass, double *v, double *vu, int64_t nn) {
+  if (!c || nn != c->a.nn) return KMPM_ERR_ARG;
+  HIPCHK(launch_grid_clean(c));
+  std::vector<double> t(3 * nn);
+  if (mass) HIPCHK(hipMemcpyAsync(mass, c->d_gmass, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // device layout [comp][node] -> reference layout [node][comp]
+  double *dsts[2] = {v, vu};
+  double *srcs[2] = {c->d_gv, c->d_gvu};
+  for (int k = 0; k < 2; k++) {
+    if (!dsts[k]) continue;
+    HIPCHK(hipMemcpy(t.data(), srcs[k], 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nn; i++)
+      for (int d = 0; d < 3; d++) dsts[k][3 * i + d] = t[(size_t)d * nn + i];
+  }
+  return KMPM_OK;
+}
+
+int kmpm_download_cells(kmpm_ctx *c, int32_t *cell_ijk, int64_t *sorted_ptags, int64_t np) {
+  if (!c || np != c->a.np) return KMPM_ERR_ARG;
+  if (cell_ijk) {
+    int *d = nullptr;
+    HIPCHK(hipMalloc(&d, 3 * std::max<int64_t>(np, 1) * sizeof(int)));
+    hipError_t e = launch_cells(c, d);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(cell_ijk, d, 3 * np * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return KMPM_ERR_HIP;
+  }
+  if (sorted_ptags) {
+    if (!c->sorted) return KMPM_ERR_ARG;
+    std::vector<int> order(np), pid((size_t)c->a.cap);
+    HIPCHK(hipMemcpyAsync(order.data(), c->d_order, np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(pid.data(), c->d_ia + (size_t)PID * c->a.cap, c->a.cap * sizeof(int),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t j = 0; j < np; j++) sorted_ptags[j] = c->h_ptag[pid[order[j]]];
+  }
+  return KMPM_OK;
+}
+
+int kmpm_get_tile_shape(kmpm_ctx *c, int tile[3]) {
+  if (!c || !tile) return KMPM_ERR_ARG;
+  for (int d = 0; d < 3; d++) tile[d] = c->a.tg.T[d];
+  return KMPM_OK;
+}
+
+int kmpm_debug_weights(kmpm_ctx *c, int64_t *tag, double *w, double *wd, int64_t np) {
+  if (!c || !tag || !w || !wd || np != c->a.np) return KMPM_ERR_ARG;
+  const int S = c->a.dim == 3 ? c->a.S1 * c->a.S1 * c->a.S1 : c->a.S1 * c->a.S1;
+  const size_t n = (size_t)std::max<int64_t>(np, 1) * S;
+  int64_t *dt = nullptr;
+  double *dw = nullptr, *dwd = nullptr;
+  HIPCHK(hipMalloc(&dt, n * sizeof(int64_t)));
+  if (hipMalloc(&dw, n * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dwd, 3 * n * sizeof(double)) != hipSuccess) {
+    (void)hipFree(dt);
+    if (dw) (void)hipFree(dw);
+    return KMPM_ERR_HIP;
+  }
+  hipError_t e = launch_debug_weights(c, dt, dw, dwd);
+  const size_t m = (size_t)np * S;
+  if (e == hipSuccess) e = hipMemcpyAsync(tag, dt, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(w, dw, m * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(wd, dwd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dt);
+  (void)hipFree(dw);
+  (void)hipFree(dwd);
+  return e == hipSuccess ? KMPM_OK : KMPM_ERR_HIP;
+}
+
+}  // extern "C"

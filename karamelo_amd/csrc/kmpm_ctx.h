@@ -1,0 +1,110 @@
+// kmpm_ctx.h -- the device context behind the C ABI.
+#pragma once
+
+#include "kmpm_device.h"
+
+namespace kmpm {
+
+// Tile geometry shared by the sort key, P2G, grid update and G2P kernels.
+struct TileGeom {
+  int T[3];        // tile size in cells (T[2] == 1 in 2D)
+  int nt[3];       // tiles per axis
+  int ntiles;      // nt[0]*nt[1]*nt[2]
+  int cpt;         // cells per tile
+  int E[3];        // node block extent per tile: T + S1 - 1 (1 in unused z)
+  int bn;          // nodes per block
+  int off;         // block origin = tile origin - off (1 cubic, 0 linear)
+};
+
+// Pointers and sizes handed to every kernel by value.
+struct KArgs {
+  // problem
+  int dim, S1, scheme;
+  int n[3];
+  int64_t nn;
+  double lo[3], hi[3];
+  double h, inv_h, flip, dt_factor;
+  int dt_constant;
+  int nmat;
+  const DevMat *mats;
+  TileGeom tg;
+  // particles
+  int np, cap;
+  double *pa;       // current state (NDF * cap)
+  int *ia;          // current int state (NIF * cap)
+  double *pb;       // next state
+  int *ib;
+  double *xo;       // MUSL: positions at the start of the step, sorted slots (3*cap)
+  int *key;         // cell key per current slot
+  int *rank;        // arrival rank within the cell
+  int *order_tmp;   // counting-sort scatter
+  int *order;       // sorted position -> current slot
+  // cells and tiles
+  int *cell_count;  // [ncells + 1]
+  int *cell_start;  // [ncells + 1]
+  int *tile_list;   // active tiles (by list slot)
+  int *tile_slot;   // tile -> list slot or -1
+  int *dil_list;    // tiles whose owned nodes get contributions
+  double *scratch;  // partial node blocks [slot][field][bn]
+  int scratch_slots;
+  // grid (tag order, SoA per component)
+  double *gmass, *gv, *gvu;   // gv/gvu: 3 * nn
+  uint32_t *gmask;
+  // fixes
+  int nfv, nbf;
+  DevFixVel fv[MAX_FIX];
+  DevFixBF bf[MAX_FIX];
+  // scalars
+  DevScalars *sc;
+};
+
+}  // namespace kmpm
+
+struct kmpm_ctx {
+  kmpm_config cfg;
+  kmpm::KArgs a;
+  int device;
+  int cu_count;
+  hipStream_t stream;
+  int nmat;
+  kmpm_material mats_host[16];
+  // static per-particle data in upload order (index = pid)
+  int64_t *h_ptag;
+  int32_t *h_solid;
+  uint32_t *h_pmask;
+  double *h_x0;
+  // device allocations
+  kmpm::DevMat *d_mats;
+  double *d_pa, *d_pb, *d_xo;
+  int *d_ia, *d_ib;
+  int *d_key, *d_rank, *d_order_tmp, *d_order;
+  int *d_cell_count, *d_cell_start;
+  int *d_tile_list, *d_tile_slot, *d_dil_list;
+  double *d_scratch;
+  double *d_gmass, *d_gv, *d_gvu;
+  uint32_t *d_gmask;
+  uint32_t *d_pmask;     // particle group masks (by pid)
+  kmpm::DevScalars *d_sc;
+  void *d_scan_tmp;
+  size_t scan_tmp_bytes;
+  int64_t ncells;
+  bool keys_valid;       // d_key holds keys of the current positions
+  bool sorted;           // order/tiles valid for the current step
+};
+
+namespace kmpm {
+// launchers (kmpm_kernels.hip); all stream-ordered, return hipError_t
+hipError_t launch_keys(kmpm_ctx *c);
+hipError_t launch_sort(kmpm_ctx *c);
+hipError_t launch_p2g(kmpm_ctx *c, int momentum_only);
+hipError_t launch_grid_update(kmpm_ctx *c, int remap);
+hipError_t launch_g2p_usl(kmpm_ctx *c);
+hipError_t launch_g2p_musl_a(kmpm_ctx *c);
+hipError_t launch_stress_musl_b(kmpm_ctx *c);
+hipError_t launch_adjust_dt(kmpm_ctx *c);
+hipError_t launch_grid_clean(kmpm_ctx *c);
+hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);
+hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double *d_wd);
+hipError_t launch_cells(kmpm_ctx *c, int *d_ijk);
+size_t scan_temp_bytes(int64_t n);
+}  // namespace kmpm

@@ -1,0 +1,174 @@
+// kmpm_device.h -- device-side data layout and point functions of the
+// MI355X explicit MPM step.  Built with -ffp-contract=off: every rounding in
+// the shape functions and cell indices matches the reference expression
+// order; accumulation loops use explicit fma().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/kmpm.h"
+
+namespace kmpm {
+
+// ---- particle state (FP64 plain path), structure of arrays --------------
+// Double fields, each an array of `cap` doubles at base + field * cap.
+// Symmetric tensors are stored as xx, yy, zz, xy, xz, yz.
+enum DField {
+  PX = 0, PY, PZ,            // position x
+  VX, VY, VZ,                // velocity v
+  F00, F01, F02, F10, F11, F12, F20, F21, F22,  // deformation gradient, row-major
+  SXX, SYY, SZZ, SXY, SXZ, SYZ,                  // Cauchy stress sigma
+  EXX, EYY, EZZ, EXY, EXZ, EYZ,                  // elastic strain strain_el
+  EP, EPDOT, DINIT, DMG,     // eff_plastic_strain, its rate, damage_init, damage
+  MASS, VOL0, VOL,           // mass, vol0, vol
+  NDF                        // number of double fields
+};
+// Int fields: PID = index into the static (upload-order) arrays, MAT = material.
+enum IField { PID = 0, MATI, NIF };
+
+// ---- material table entry ------------------------------------------------
+struct DevMat {
+  int type, eos, strength, damage;
+  double rho0, K, G, lambda, signal_velocity;
+  double cfl_k43g;            // K + FOUR_THIRD * G  (solid.cpp:1382)
+  double eos_p[9];
+  double strength_p[9];
+  double damage_p[8];
+};
+
+struct DevFixVel { uint32_t bit; int set[3]; double val[3], prev[3]; };
+struct DevFixBF { uint32_t bit; int set[3]; double val[3]; };
+constexpr int MAX_FIX = 8;
+
+// ---- device scalars (time, dt, CFL accumulator, sticky error) -------------
+struct DevScalars {
+  double dt, time;
+  unsigned long long max_c_bits;  // max wave speed of the current step (bits of a
+                                  // non-negative double; ordered like the value)
+  int64_t step;
+  int error;                      // first error code (sticky)
+  int pad0;
+  int64_t err_pid;                // pid of the first offending particle
+  int tile_count;                 // active tiles this step
+  int dil_count;                  // tiles whose owned nodes receive contributions
+};
+
+constexpr double SQRT_3_OVER_2 = 1.224744871;                            // strength_jc.cpp:29
+constexpr double FOUR_THIRD = 1.333333333333333333333333333333333333333; // solid.cpp:39
+
+// ---- shape functions: basis_functions.h:21-104 ------------------------------
+__device__ __host__ inline double bf_linear(double r_) {
+  double r = fabs(r_);
+  if (r >= 1.0) return 0.0;
+  return 1.0 - r;
+}
+__device__ __host__ inline double bf_dlinear(double r, double inv_h) {
+  if (r >= 1.0 || r <= -1.0 || r == 0) return 0.0;
+  else if (r > 0.0) return -inv_h;
+  else return inv_h;
+}
+__device__ __host__ inline double bf_cubic(double r, int ntype) {
+  if (r >= 1 && r < 2) {
+    if (ntype == 1) return 0;
+    return ((-1.0 / 6.0 * r + 1) * r - 2) * r + 4.0 / 3.0;
+  } else if (r >= 0 && r < 1) {
+    if (ntype == -2) return (1.0 / 6.0 * r * r - 1) * r + 1;
+    else if (ntype == 2) return 1;
+    else if (ntype == 1) return (1.0 / 3.0 * r - 1) * r * r + 2.0 / 3.0;
+    else return (0.5 * r - 1) * r * r + 2.0 / 3.0;
+  } else if (r >= -1 && r < 0) {
+    if (ntype == 2) return (-1.0 / 6.0 * r * r + 1) * r + 1;
+    else if (ntype == -1) return (-1.0 / 3.0 * r - 1) * r * r + 2.0 / 3.0;
+    else return (-0.5 * r - 1) * r * r + 2.0 / 3.0;
+  } else if (r >= -2 && r < -1) {
+    return ((1.0 / 6.0 * r + 1) * r + 2) * r + 4.0 / 3.0;
+  }
+  return 0;
+}
+__device__ __host__ inline double bf_dcubic(double r, int ntype, double ih) {
+  if (r >= 1 && r < 2) {
+    if (ntype == 1) return -ih;
+    return ih * ((-0.5 * r + 2) * r - 2);
+  } else if (r >= 0 && r < 1) {
+    if (ntype == -2) return ih * (0.5 * r * r - 1);
+    else if (ntype == 2) return ih;
+    else if (ntype == 1) return ih * r * (r - 2);
+    else return ih * (3.0 / 2.0 * r - 2) * r;
+  } else if (r >= -1 && r < 0) {
+    if (ntype == 2) return ih * (-0.5 * r * r + 1);
+    else if (ntype == -1) return ih * (-r - 2) * r;
+    else return ih * (-3.0 / 2.0 * r - 2) * r;
+  } else if (r >= -2 && r < -1) {
+    return ih * ((0.5 * r + 2) * r + 2);
+  }
+  return 0;
+}
+
+// Node type of node index i on an axis with n nodes (grid.cpp:236-239).
+__device__ __host__ inline int node_type(int i, int n) {
+  int a = i < 2 ? i : 2;
+  int b = (n - 1 - i) < 2 ? (n - 1 - i) : 2;
+  return a - b;
+}
+
+// Cell of a coordinate for the sort: trunc((x - lo) * inv_h) clamped to
+// [0, n-1]; flags NaN and out-of-range values.
+__device__ inline int cell_of(double x, double lo, double inv_h, int n, bool &bad) {
+  double u = (x - lo) * inv_h;
+  if (!(u >= 0.0)) { bad = true; return 0; }          // negative or NaN
+  if (u >= (double)(n - 1)) { if (u > (double)(n - 1)) bad = true; return n - 1; }
+  return (int)u;
+}
+
+// 1D weights and derivatives of the S1 stencil nodes base..base+S1-1 of a
+// particle at coordinate x on an axis (ulmpm.cpp:244-311 with the pair
+// filter of :252-265 applied per factor: a zero value zeroes the derivative,
+// so a dropped pair contributes nothing).  Nodes outside [0, n) get 0.
+template <int S1>
+__device__ inline void weights_1d(double x, double lo, double h, double inv_h, int n, int base,
+                                  double *w, double *dw) {
+#pragma unroll
+  for (int a = 0; a < S1; a++) {
+    int i = base + a;
+    double s = 0, d = 0;
+    if (i >= 0 && i < n) {
+      double xn = lo + (0 + i) * h;          // grid.cpp:227
+      double r = (x - xn) * inv_h;           // ulmpm.cpp:248
+      if (S1 == 2) {
+        s = bf_linear(r);
+        d = bf_dlinear(r, inv_h);
+      } else {
+        int nt = node_type(i, n);
+        s = bf_cubic(r, nt);
+        d = bf_dcubic(r, nt, inv_h);
+      }
+      if (s == 0) d = 0;
+    }
+    w[a] = s;
+    dw[a] = d;
+  }
+}
+
+// ---- 3x3 helpers ------------------------------------------------------------
+__device__ inline double det3(const double *m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+// Frobenius norm of a symmetric tensor stored xx,yy,zz,xy,xz,yz, summed in the
+// row-major order of the full matrix.
+__device__ inline double norm_sym(const double *s) {
+  double t = s[0] * s[0];
+  t += s[3] * s[3];
+  t += s[4] * s[4];
+  t += s[3] * s[3];
+  t += s[1] * s[1];
+  t += s[5] * s[5];
+  t += s[4] * s[4];
+  t += s[5] * s[5];
+  t += s[2] * s[2];
+  return sqrt(t);
+}
+
+}  // namespace kmpm

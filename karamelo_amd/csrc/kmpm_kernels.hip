@@ -1,0 +1,1073 @@
+// kmpm_kernels.hip -- gfx950 kernels of the explicit MPM step (FP64 path).
+//
+// Step structure (USL):
+//   sort   : cell key per particle -> counting sort by tile-major cell key,
+//            ties broken by particle id (deterministic permutation)
+//   P2G    : one workgroup per active tile; lanes = stencil nodes; each lane
+//            accumulates the particles of a cell in registers, flushes into an
+//            LDS node block of the tile; the block is written to a per-tile
+//            partial buffer (no global atomics)
+//   grid   : each owned node of a touched tile sums the partial blocks that
+//            cover it, then v = p/m, v_update = v + dt (f + mb)/m, fixes
+//   G2P    : one workgroup per active tile; LDS block of node velocities;
+//            lanes = particles: gather, L, x/v update, F, stress, CFL, next
+//            cell key; state is written in sorted order (the permutation is
+//            applied for free)
+#include <hipcub/hipcub.hpp>
+
+#include "kmpm_constitutive.h"
+#include "kmpm_ctx.h"
+
+namespace kmpm {
+
+#define PF(base, f, i) ((base)[(size_t)(f) * (size_t)a.cap + (size_t)(i)])
+#define IFLD(base, f, i) ((base)[(size_t)(f) * (size_t)a.cap + (size_t)(i)])
+
+__device__ inline void set_error(DevScalars *sc, int code, int64_t pid) {
+  if (atomicCAS(&sc->error, 0, code) == 0) sc->err_pid = pid;
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline int cell_of_clamped(double x, double lo, double inv_h, int n) {
+  double u = (x - lo) * inv_h;
+  if (!(u >= 0.0)) return 0;   // negative or NaN
+  if (u >= (double)(n - 1)) return n - 1;
+  return (int)u;
+}
+
+__device__ inline int tile_key(const KArgs &a, int c0, int c1, int c2) {
+  const TileGeom &g = a.tg;
+  int t = ((c0 / g.T[0]) * g.nt[1] + c1 / g.T[1]) * g.nt[2] + c2 / g.T[2];
+  int l = ((c0 % g.T[0]) * g.T[1] + c1 % g.T[1]) * g.T[2] + c2 % g.T[2];
+  return t * g.cpt + l;
+}
+
+__device__ inline int particle_key(const KArgs &a, double x, double y, double z) {
+  int c0 = cell_of_clamped(x, a.lo[0], a.inv_h, a.n[0]);
+  int c1 = cell_of_clamped(y, a.lo[1], a.inv_h, a.n[1]);
+  int c2 = a.dim == 3 ? cell_of_clamped(z, a.lo[2], a.inv_h, a.n[2]) : 0;
+  return tile_key(a, c0, c1, c2);
+}
+
+// ---------------------------------------------------------------------------
+// sort
+// ---------------------------------------------------------------------------
+__global__ void k_keys(KArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  a.key[i] = particle_key(a, PF(a.pa, PX, i), PF(a.pa, PY, i), PF(a.pa, PZ, i));
+}
+
+__global__ void k_sort_begin(KArgs a, int64_t ncells) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= ncells; q += stride)
+    a.cell_count[q] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc->tile_count = 0;
+    a.sc->dil_count = 0;
+  }
+}
+
+__global__ void k_count(KArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  a.rank[i] = atomicAdd(&a.cell_count[a.key[i]], 1);
+}
+
+__global__ void k_scatter(KArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  a.order_tmp[a.cell_start[a.key[i]] + a.rank[i]] = i;
+}
+
+// Arrival order inside a cell is not deterministic; the final position is
+// the rank of the particle id inside its cell segment.
+__global__ void k_rankfix(KArgs a) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.np) return;
+  int i = a.order_tmp[j];
+  int k = a.key[i];
+  int s = a.cell_start[k], e = a.cell_start[k + 1];
+  int pid = IFLD(a.ia, PID, i);
+  int r = 0;
+  for (int q = s; q < e; q++) r += IFLD(a.ia, PID, a.order_tmp[q]) < pid ? 1 : 0;
+  a.order[s + r] = i;
+}
+
+__global__ void k_tiles(KArgs a) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.tg.ntiles) return;
+  int p0 = a.cell_start[(int64_t)t * a.tg.cpt], p1 = a.cell_start[(int64_t)(t + 1) * a.tg.cpt];
+  if (p1 > p0) {
+    int slot = atomicAdd(&a.sc->tile_count, 1);
+    a.tile_list[slot] = t;
+    a.tile_slot[t] = slot;
+  } else {
+    a.tile_slot[t] = -1;
+  }
+}
+
+// Tiles whose owned nodes lie in the node block of an active tile: the
+// active tile itself and its neighbours at -1 (and +1 for cubic) per axis.
+__global__ void k_dilate(KArgs a) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.tg.ntiles) return;
+  const TileGeom &g = a.tg;
+  int tz = t % g.nt[2], ty = (t / g.nt[2]) % g.nt[1], tx = t / (g.nt[2] * g.nt[1]);
+  int up = a.S1 == 4 ? 1 : 0;
+  bool hit = false;
+  for (int dx = -1; dx <= up && !hit; dx++)
+    for (int dy = -1; dy <= up && !hit; dy++)
+      for (int dz = (a.dim == 3 ? -1 : 0); dz <= (a.dim == 3 ? up : 0) && !hit; dz++) {
+        int ux = tx + dx, uy = ty + dy, uz = tz + dz;
+        if (ux < 0 || uy < 0 || uz < 0 || ux >= g.nt[0] || uy >= g.nt[1] || uz >= g.nt[2]) continue;
+        if (a.tile_slot[(ux * g.nt[1] + uy) * g.nt[2] + uz] >= 0) hit = true;
+      }
+  if (hit) {
+    int slot = atomicAdd(&a.sc->dil_count, 1);
+    a.dil_list[slot] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P2G: lanes = stencil nodes, register accumulation per cell, LDS tile block
+// ---------------------------------------------------------------------------
+constexpr int SCR_FIELDS = 7;   // scratch slot layout: mass, momentum[3], force[3]
+
+template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP>
+__global__ __launch_bounds__(256) void k_p2g(KArgs a) {
+  constexpr int S = DIM == 3 ? S1 * S1 * S1 : S1 * S1;
+  constexpr int G = 64 / S;
+  constexpr int RUN = CH / G;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  constexpr int NSIG = DIM == 3 ? 6 : 3;
+  constexpr int NW = DIM * S1;                  // 1D values (then NW derivatives)
+  constexpr int NF = REMAP ? (1 + DIM) : (1 + 2 * DIM);
+  constexpr int NPAY = REMAP ? (1 + DIM) : (1 + DIM + NSIG);
+  constexpr int NV = 2 * NW + NPAY;
+  static_assert(64 % S == 0 && CH % G == 0 && CH <= 64, "stencil/chunk shape");
+
+  __shared__ double blk[NF * BN];
+  __shared__ double stg[4][NV][CH];
+  __shared__ int stg_cell[4][CH];
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+
+  // lane -> stencil node offsets
+  const int g = lane / S, s = lane % S;
+  const int sa = DIM == 3 ? s / (S1 * S1) : s / S1;
+  const int sb = DIM == 3 ? (s / S1) % S1 : s % S1;
+  const int sc = DIM == 3 ? s % S1 : 0;
+
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const int tz = DIM == 3 ? t % nt2 : 0;
+    const int ty = (t / nt2) % nt1;
+    const int tx = t / (nt2 * nt1);
+    const int ox = tx * TX, oy = ty * TY, oz = DIM == 3 ? tz * TZ : 0;
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    for (int q = tid; q < NF * BN; q += 256) blk[q] = 0;
+    __syncthreads();
+
+    const int npart = p1 - p0;
+    for (int c0 = wave * CH; c0 < npart; c0 += 4 * CH) {
+      const int cnt = min(CH, npart - c0);
+      if (lane < cnt) {
+        const int j = p0 + c0 + lane;
+        double x[3], v[3], m;
+        if (REMAP) {
+          for (int d = 0; d < DIM; d++) {
+            x[d] = a.xo[(size_t)d * a.cap + j];
+            v[d] = PF(a.pb, VX + d, j);
+          }
+          m = PF(a.pb, MASS, j);
+        } else {
+          const int slot = a.order[j];
+          for (int d = 0; d < DIM; d++) {
+            x[d] = PF(a.pa, PX + d, slot);
+            v[d] = PF(a.pa, VX + d, slot);
+          }
+          m = PF(a.pa, MASS, slot);
+          const double nvol = -PF(a.pa, VOL, slot);
+          if (DIM == 3) {
+            for (int e = 0; e < 6; e++) stg[wave][2 * NW + 1 + DIM + e][lane] = nvol * PF(a.pa, SXX + e, slot);
+          } else {
+            stg[wave][2 * NW + 1 + DIM + 0][lane] = nvol * PF(a.pa, SXX, slot);
+            stg[wave][2 * NW + 1 + DIM + 1][lane] = nvol * PF(a.pa, SYY, slot);
+            stg[wave][2 * NW + 1 + DIM + 2][lane] = nvol * PF(a.pa, SXY, slot);
+          }
+        }
+        int cl[3] = {0, 0, 0};
+        const int org[3] = {ox, oy, oz};
+        for (int d = 0; d < DIM; d++) {
+          const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
+          cl[d] = c - org[d];
+          double w[S1], dw[S1];
+          weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
+          for (int q = 0; q < S1; q++) {
+            stg[wave][d * S1 + q][lane] = w[q];
+            stg[wave][NW + d * S1 + q][lane] = dw[q];
+          }
+        }
+        stg_cell[wave][lane] = (cl[0] * EY + cl[1]) * EZ + cl[2];
+        stg[wave][2 * NW][lane] = m;
+        for (int d = 0; d < DIM; d++) stg[wave][2 * NW + 1 + d][lane] = v[d];
+      }
+      wave_sync();
+
+      double acc[NF];
+      for (int f = 0; f < NF; f++) acc[f] = 0;
+      int cur = -1;
+      const int sofs = DIM == 3 ? (sa * EY + sb) * EZ + sc : sa * EY + sb;
+      for (int r = 0; r < RUN; r++) {
+        const int j = g * RUN + r;
+        if (j >= cnt) break;
+        const int cell = stg_cell[wave][j];
+        if (cell != cur) {
+          if (cur >= 0)
+            for (int f = 0; f < NF; f++) atomicAdd(&blk[f * BN + cur + sofs], acc[f]);
+          for (int f = 0; f < NF; f++) acc[f] = 0;
+          cur = cell;
+        }
+        const double wx = stg[wave][sa][j], wy = stg[wave][S1 + sb][j];
+        const double dx = stg[wave][NW + sa][j], dy = stg[wave][NW + S1 + sb][j];
+        double w, g0, g1, g2 = 0;
+        if (DIM == 3) {
+          const double wz = stg[wave][2 * S1 + sc][j], dz = stg[wave][NW + 2 * S1 + sc][j];
+          const double wxy = wx * wy;
+          w = wxy * wz;
+          g0 = dx * wy * wz;
+          g1 = wx * dy * wz;
+          g2 = wxy * dz;
+        } else {
+          w = wx * wy;
+          g0 = dx * wy;
+          g1 = wx * dy;
+        }
+        const double m = stg[wave][2 * NW][j];
+        const double wm = w * m;
+        acc[0] += wm;
+        for (int d = 0; d < DIM; d++) acc[1 + d] = fma(wm, stg[wave][2 * NW + 1 + d][j], acc[1 + d]);
+        if (!REMAP) {
+          const double *ns = &stg[wave][2 * NW + 1 + DIM][0];
+          if (DIM == 3) {
+            // f_d += sum_e (-vol sigma)_de g_e ; sym order xx,yy,zz,xy,xz,yz
+            const double sxx = ns[0 * CH + j], syy = ns[1 * CH + j], szz = ns[2 * CH + j];
+            const double sxy = ns[3 * CH + j], sxz = ns[4 * CH + j], syz = ns[5 * CH + j];
+            acc[4] = fma(sxz, g2, fma(sxy, g1, fma(sxx, g0, acc[4])));
+            acc[5] = fma(syz, g2, fma(syy, g1, fma(sxy, g0, acc[5])));
+            acc[6] = fma(szz, g2, fma(syz, g1, fma(sxz, g0, acc[6])));
+          } else {
+            const double sxx = ns[0 * CH + j], syy = ns[1 * CH + j], sxy = ns[2 * CH + j];
+            acc[3] = fma(sxy, g1, fma(sxx, g0, acc[3]));
+            acc[4] = fma(syy, g1, fma(sxy, g0, acc[4]));
+          }
+        }
+        (void)g2;
+      }
+      if (cur >= 0)
+        for (int f = 0; f < NF; f++) atomicAdd(&blk[f * BN + cur + sofs], acc[f]);
+      wave_sync();
+    }
+    __syncthreads();
+    double *dst = a.scratch + (size_t)li * SCR_FIELDS * BN;
+    // blk fields: mass, mom[DIM], force[DIM] -> scratch fields mass, mom[3],
+    // force[3]; components a 2D run does not have are written as zeros.
+    const int f_lo = REMAP ? 1 : 0, f_hi = REMAP ? 4 : SCR_FIELDS;
+    for (int q = tid + f_lo * BN; q < f_hi * BN; q += 256) {
+      const int sf = q / BN, o = q % BN;
+      int bf;
+      if (sf == 0) bf = 0;
+      else if (sf <= 3) bf = sf <= DIM ? sf : -1;
+      else bf = (sf - 3) <= DIM ? DIM + (sf - 3) : -1;
+      dst[q] = bf >= 0 ? blk[bf * BN + o] : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// grid: sum partial blocks, v = p/m, v_update, fixes (grid.cpp:448-466,
+// fix_body_force.cpp:117-146, fix_velocity_nodes.cpp:130-271)
+// ---------------------------------------------------------------------------
+template <int DIM, int S1, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int TZZ = DIM == 3 ? TZ : 1;
+  constexpr int NO = TX * TY * TZZ;
+  const int off = S1 == 4 ? 1 : 0;
+  const int up = S1 == 4 ? 1 : 0;
+  const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int total = a.sc->dil_count * NO;
+  const double dt = a.sc->dt;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const int t = a.dil_list[q / NO], o = q % NO;
+    const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
+    const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
+    if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
+    const int64_t tag = ((int64_t)i * a.n[1] + j) * a.n[2] + k;
+    double sum[SCR_FIELDS];
+    for (int f = 0; f < SCR_FIELDS; f++) sum[f] = 0;
+    for (int dx = -1; dx <= up; dx++)
+      for (int dy = -1; dy <= up; dy++)
+        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? up : 0); dz++) {
+          const int ux = tx + dx, uy = ty + dy, uz = tz + dz;
+          if (ux < 0 || uy < 0 || uz < 0 || ux >= nt0 || uy >= nt1 || uz >= nt2) continue;
+          const int slot = a.tile_slot[(ux * nt1 + uy) * nt2 + uz];
+          if (slot < 0) continue;
+          const int bx = i - (ux * TX - off), by = j - (uy * TY - off);
+          const int bz = DIM == 3 ? k - (uz * TZ - off) : 0;
+          if (bx < 0 || by < 0 || bz < 0 || bx >= EX || by >= EY || bz >= EZ) continue;
+          const int idx = (bx * EY + by) * EZ + bz;
+          const double *src = a.scratch + (size_t)slot * SCR_FIELDS * BN + idx;
+          if (remap) {
+            for (int f = 1; f <= 3; f++) sum[f] += src[f * BN];
+          } else {
+            for (int f = 0; f < SCR_FIELDS; f++) sum[f] += src[f * BN];
+          }
+        }
+    const uint32_t mask = a.gmask[tag];
+    if (remap) {
+      // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
+      // node mass, then FixVelocityNodes::post_velocities_to_grid.
+      const double m = a.gmass[tag];
+      double v[3];
+      for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+      for (int fi = 0; fi < a.nfv; fi++)
+        if (mask & a.fv[fi].bit)
+          for (int d = 0; d < 3; d++)
+            if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
+      for (int d = 0; d < 3; d++) a.gv[d * a.nn + tag] = v[d];
+      continue;
+    }
+    const double m = sum[0];
+    double v[3], vu[3], mb[3] = {0, 0, 0};
+    for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+    for (int fi = 0; fi < a.nbf; fi++)
+      if (m > 0 && (mask & a.bf[fi].bit))
+        for (int d = 0; d < 3; d++)
+          if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
+    for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
+    for (int fi = 0; fi < a.nfv; fi++)
+      if (mask & a.fv[fi].bit)
+        for (int d = 0; d < 3; d++)
+          if (a.fv[fi].set[d]) {
+            vu[d] = a.fv[fi].val[d];
+            v[d] = a.fv[fi].prev[d];
+          }
+    a.gmass[tag] = m;
+    for (int d = 0; d < 3; d++) {
+      a.gv[d * a.nn + tag] = v[d];
+      a.gvu[d * a.nn + tag] = vu[d];
+    }
+  }
+}
+
+// Grid state for download: nodes outside every touched tile hold zeros
+// (and their velocity-fix values), as the reference's reset leaves them.
+template <int DIM, int TX, int TY, int TZ>
+__global__ void k_grid_clean(KArgs a) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int S1 = a.S1;
+  const int up = S1 == 4 ? 1 : 0;
+  for (int64_t tag = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tag < a.nn; tag += stride) {
+    const int k = (int)(tag % a.n[2]), j = (int)((tag / a.n[2]) % a.n[1]),
+              i = (int)(tag / ((int64_t)a.n[2] * a.n[1]));
+    const int tx = i / TX, ty = j / TY, tz = DIM == 3 ? k / TZ : 0;
+    bool touched = false;
+    for (int dx = -1; dx <= up && !touched; dx++)
+      for (int dy = -1; dy <= up && !touched; dy++)
+        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? up : 0) && !touched; dz++) {
+          const int ux = tx + dx, uy = ty + dy, uz = tz + dz;
+          if (ux < 0 || uy < 0 || uz < 0 || ux >= a.tg.nt[0] || uy >= nt1 || uz >= nt2) continue;
+          if (a.tile_slot[(ux * nt1 + uy) * nt2 + uz] >= 0) touched = true;
+        }
+    if (touched) continue;
+    double v[3] = {0, 0, 0}, vu[3] = {0, 0, 0};
+    const uint32_t mask = a.gmask[tag];
+    for (int fi = 0; fi < a.nfv; fi++)
+      if (mask & a.fv[fi].bit)
+        for (int d = 0; d < 3; d++)
+          if (a.fv[fi].set[d]) {
+            vu[d] = a.fv[fi].val[d];
+            v[d] = a.fv[fi].prev[d];
+          }
+    a.gmass[tag] = 0;
+    for (int d = 0; d < 3; d++) {
+      a.gv[d * a.nn + tag] = v[d];
+      a.gvu[d * a.nn + tag] = vu[d];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// G2P helpers
+// ---------------------------------------------------------------------------
+__device__ inline void block_max_to(double val, DevScalars *sc) {
+  // val >= 0 (or NaN handled by caller); wave max then LDS then atomic max
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    double other = __shfl_xor(val, o, 64);
+    val = other > val ? other : val;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) red[wave] = val;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] > m ? red[w] : m;
+    if (m > 0) atomicMax(&sc->max_c_bits, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Load the tile's node block (velocities) into LDS: comps [nc][BN].
+template <int DIM, int S1, int TX, int TY, int TZ, int MODE>
+__device__ inline void load_node_block(const KArgs &a, double *nb, int ox, int oy, int oz) {
+  // MODE 0: v_update and v_update - v (USL / MUSL part A); MODE 1: v (MUSL part B)
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  const int off = S1 == 4 ? 1 : 0;
+  for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+    const int bz = q % EZ, by = (q / EZ) % EY, bx = q / (EZ * EY);
+    const int i = ox - off + bx, j = oy - off + by, k = DIM == 3 ? oz - off + bz : 0;
+    const bool ok = i >= 0 && j >= 0 && k >= 0 && i < a.n[0] && j < a.n[1] && k < a.n[2];
+    const int64_t tag = ok ? ((int64_t)i * a.n[1] + j) * a.n[2] + k : 0;
+    for (int d = 0; d < DIM; d++) {
+      if (MODE == 0) {
+        const double vu = ok ? a.gvu[d * a.nn + tag] : 0.0;
+        const double v = ok ? a.gv[d * a.nn + tag] : 0.0;
+        nb[d * BN + q] = vu;
+        nb[(DIM + d) * BN + q] = vu - v;
+      } else {
+        nb[d * BN + q] = ok ? a.gv[d * a.nn + tag] : 0.0;
+      }
+    }
+  }
+}
+
+// Gather over the stencil: vu, acc (MODE 0 / A), L (USL / B).
+template <int DIM, int S1, int EY, int EZ, int BN, bool WANT_VA, bool WANT_L, bool L_FROM_V>
+__device__ inline void gather_stencil(const double *nb, const int *cl, const double (*w)[S1],
+                                      const double (*dw)[S1], double *vu, double *acc,
+                                      double *L) {
+#pragma unroll
+  for (int ia = 0; ia < S1; ia++) {
+#pragma unroll
+    for (int ib = 0; ib < S1; ib++) {
+      double wxy = w[0][ia] * w[1][ib];
+      double dxy = dw[0][ia] * w[1][ib];
+      double xdy = w[0][ia] * dw[1][ib];
+      // Opaque barrier: the previous row's accumulations complete here, and
+      // neither this row's LDS loads nor its weight products move above it.
+      // Without it the scheduler defers whole FMA chains and spills.
+      if (WANT_VA)
+        for (int d = 0; d < DIM; d++) asm volatile("" : "+v"(vu[d]), "+v"(acc[d]));
+      if (WANT_L)
+        for (int d = 0; d < 9; d++) asm volatile("" : "+v"(L[d]));
+      asm volatile("" : "+v"(wxy), "+v"(dxy), "+v"(xdy)::"memory");
+#pragma unroll
+      for (int ic = 0; ic < (DIM == 3 ? S1 : 1); ic++) {
+        double wt, g0, g1, g2 = 0;
+        int idx;
+        if (DIM == 3) {
+          wt = wxy * w[2][ic];
+          g0 = dxy * w[2][ic];
+          g1 = xdy * w[2][ic];
+          g2 = wxy * dw[2][ic];
+          idx = ((cl[0] + ia) * EY + (cl[1] + ib)) * EZ + (cl[2] + ic);
+        } else {
+          wt = wxy;
+          g0 = dxy;
+          g1 = xdy;
+          idx = (cl[0] + ia) * EY + (cl[1] + ib);
+        }
+        double vn[DIM];
+        for (int d = 0; d < DIM; d++) vn[d] = nb[d * BN + idx];
+        if (WANT_VA) {
+          for (int d = 0; d < DIM; d++) {
+            vu[d] = fma(wt, vn[d], vu[d]);
+            acc[d] = fma(wt, nb[(DIM + d) * BN + idx], acc[d]);
+          }
+        }
+        if (WANT_L) {
+          const double gg[3] = {g0, g1, g2};
+          for (int d = 0; d < DIM; d++)
+            for (int e = 0; e < DIM; e++) L[3 * d + e] = fma(vn[d], gg[e], L[3 * d + e]);
+        }
+        (void)L_FROM_V;
+      }
+    }
+  }
+}
+
+// F update (solid.cpp:1181-1216, UL), D = sym(L), stress, CFL speed.
+// Returns wave speed (or -1 when the particle is fully damaged).
+__device__ inline double finish_particle(const KArgs &a, double dt, const double *L, double *F,
+                                         double *sig, double *eel, double &ep, double &epdot,
+                                         double &dinit, double &dmg, double vol0, double &vol,
+                                         int mat, const double *v, int pid) {
+  double M[9], Fn[9];
+  for (int e = 0; e < 9; e++) M[e] = ((e % 4 == 0) ? 1.0 : 0.0) + dt * L[e];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++)
+      Fn[3 * r + c] = M[3 * r] * F[c] + M[3 * r + 1] * F[3 + c] + M[3 * r + 2] * F[6 + c];
+  for (int e = 0; e < 9; e++) F[e] = Fn[e];
+  const double J = det3(F);
+  vol = J * vol0;
+  if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+  const DevMat &m = a.mats[mat];
+  const double rho = m.rho0 / J;
+  double D[6];
+  D[0] = 0.5 * (L[0] + L[0]);
+  D[1] = 0.5 * (L[4] + L[4]);
+  D[2] = 0.5 * (L[8] + L[8]);
+  D[3] = 0.5 * (L[1] + L[3]);
+  D[4] = 0.5 * (L[2] + L[6]);
+  D[5] = 0.5 * (L[5] + L[7]);
+  update_stress_point(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  if (dmg >= 1.0) return -1.0;
+  const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
+  const double c = sqrt(m.cfl_k43g / rho) + vm;
+  if (isnan(c)) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
+  return c;
+}
+
+__device__ inline bool outside_box(const KArgs &a, const double *x) {
+  // Domain::inside (domain.cpp:174-182); in 2D z == boxlo[2] == boxhi[2] == 0
+  for (int d = 0; d < a.dim; d++)
+    if (!(x[d] >= a.lo[d] && x[d] <= a.hi[d])) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
+// ---------------------------------------------------------------------------
+template <int DIM, int S1, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ double nb[2 * DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double dt = a.sc->dt;
+  const double inv_dt = 1.0 / dt;
+  const double alpha = a.flip;
+  double cmax = 0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 0>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x[3], v[3];
+      for (int d = 0; d < 3; d++) {
+        x[d] = PF(a.pa, PX + d, slot);
+        v[d] = PF(a.pa, VX + d, slot);
+      }
+      double w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = c - org[d];
+        weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double vu[3] = {0, 0, 0}, acc[3] = {0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vu, acc, L);
+      const int pid = IFLD(a.ia, PID, slot);
+      for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
+      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
+      if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      double F[9], sig[6], eel[6];
+      for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+      for (int e = 0; e < 6; e++) {
+        sig[e] = PF(a.pa, SXX + e, slot);
+        eel[e] = PF(a.pa, EXX + e, slot);
+      }
+      double ep = PF(a.pa, EP, slot), epdot = PF(a.pa, EPDOT, slot);
+      double dinit = PF(a.pa, DINIT, slot), dmg = PF(a.pa, DMG, slot);
+      const double mass = PF(a.pa, MASS, slot), vol0 = PF(a.pa, VOL0, slot);
+      const int mat = IFLD(a.ia, MATI, slot);
+      double vol;
+      const double c =
+          finish_particle(a, dt, L, F, sig, eel, ep, epdot, dinit, dmg, vol0, vol, mat, v, pid);
+      if (c > cmax) cmax = c;
+      // write in sorted order
+      for (int d = 0; d < 3; d++) {
+        PF(a.pb, PX + d, j) = x[d];
+        PF(a.pb, VX + d, j) = v[d];
+      }
+      for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
+      for (int e = 0; e < 6; e++) {
+        PF(a.pb, SXX + e, j) = sig[e];
+        PF(a.pb, EXX + e, j) = eel[e];
+      }
+      PF(a.pb, EP, j) = ep;
+      PF(a.pb, EPDOT, j) = epdot;
+      PF(a.pb, DINIT, j) = dinit;
+      PF(a.pb, DMG, j) = dmg;
+      PF(a.pb, MASS, j) = mass;
+      PF(a.pb, VOL0, j) = vol0;
+      PF(a.pb, VOL, j) = vol;
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      a.key[j] = particle_key(a, x[0], x[1], x[2]);
+    }
+  }
+  block_max_to(cmax, a.sc);
+}
+
+// ---------------------------------------------------------------------------
+// MUSL part A: G2P + advance (musl.cpp:61-67); keeps start-of-step x
+// ---------------------------------------------------------------------------
+template <int DIM, int S1, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ double nb[2 * DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double dt = a.sc->dt;
+  const double inv_dt = 1.0 / dt;
+  const double alpha = a.flip;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 0>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x[3], v[3];
+      for (int d = 0; d < 3; d++) {
+        x[d] = PF(a.pa, PX + d, slot);
+        v[d] = PF(a.pa, VX + d, slot);
+      }
+      double w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = c - org[d];
+        weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double vu[3] = {0, 0, 0}, acc[3] = {0, 0, 0};
+      gather_stencil<DIM, S1, EY, EZ, BN, true, false, false>(nb, cl, w, dw, vu, acc, nullptr);
+      const int pid = IFLD(a.ia, PID, slot);
+      for (int d = 0; d < 3; d++) a.xo[(size_t)d * a.cap + j] = x[d];
+      for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
+      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
+      if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      for (int d = 0; d < 3; d++) {
+        PF(a.pb, PX + d, j) = x[d];
+        PF(a.pb, VX + d, j) = v[d];
+      }
+      PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
+      a.key[j] = particle_key(a, x[0], x[1], x[2]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MUSL part B: L from re-mapped v (old weights), F, stress, CFL (musl.cpp:75-77)
+// ---------------------------------------------------------------------------
+template <int DIM, int S1, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ double nb[DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double dt = a.sc->dt;
+  double cmax = 0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 1>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x[3];
+      for (int d = 0; d < 3; d++) x[d] = a.xo[(size_t)d * a.cap + j];
+      double w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = c - org[d];
+        weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, nullptr, nullptr, L);
+      const int pid = IFLD(a.ia, PID, slot);
+      double v[3];
+      for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
+      double F[9], sig[6], eel[6];
+      for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+      for (int e = 0; e < 6; e++) {
+        sig[e] = PF(a.pa, SXX + e, slot);
+        eel[e] = PF(a.pa, EXX + e, slot);
+      }
+      double ep = PF(a.pa, EP, slot), epdot = PF(a.pa, EPDOT, slot);
+      double dinit = PF(a.pa, DINIT, slot), dmg = PF(a.pa, DMG, slot);
+      const double vol0 = PF(a.pa, VOL0, slot);
+      const int mat = IFLD(a.ia, MATI, slot);
+      double vol;
+      const double c =
+          finish_particle(a, dt, L, F, sig, eel, ep, epdot, dinit, dmg, vol0, vol, mat, v, pid);
+      if (c > cmax) cmax = c;
+      for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
+      for (int e = 0; e < 6; e++) {
+        PF(a.pb, SXX + e, j) = sig[e];
+        PF(a.pb, EXX + e, j) = eel[e];
+      }
+      PF(a.pb, EP, j) = ep;
+      PF(a.pb, EPDOT, j) = epdot;
+      PF(a.pb, DINIT, j) = dinit;
+      PF(a.pb, DMG, j) = dmg;
+      PF(a.pb, VOL0, j) = vol0;
+      PF(a.pb, VOL, j) = vol;
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+    }
+  }
+  block_max_to(cmax, a.sc);
+}
+
+// update_time + adjust_dt (update.cpp:201-206, ulmpm.cpp:525-551)
+__global__ void k_adjust_dt(KArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DevScalars *sc = a.sc;
+  sc->time += sc->dt;
+  if (!a.dt_constant) {
+    const double maxc = __longlong_as_double((long long)sc->max_c_bits);
+    double dtcfl = a.h * 1.0 / maxc;
+    dtcfl = 1.0e22 < dtcfl ? 1.0e22 : dtcfl;
+    if ((dtcfl == 0 || isnan(dtcfl)) && sc->error == 0) {
+      sc->error = KMPM_ERR_DTCFL;
+      sc->err_pid = -1;
+    }
+    sc->dt = dtcfl * a.dt_factor;
+  }
+  sc->max_c_bits = 0;
+  sc->step += 1;
+}
+
+// ---------------------------------------------------------------------------
+// observables and debug
+// ---------------------------------------------------------------------------
+__global__ void k_energy(KArgs a, const uint32_t *pmask, uint32_t bit, double *partial) {
+  __shared__ double sk[256], ss[256];
+  double ek = 0, es = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.np; i += gridDim.x * blockDim.x) {
+    const int pid = IFLD(a.ia, PID, i);
+    if (!(pmask[pid] & bit)) continue;
+    const double v0 = PF(a.pa, VX, i), v1 = PF(a.pa, VY, i), v2 = PF(a.pa, VZ, i);
+    const double nv = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    ek += 0.5 * PF(a.pa, MASS, i) * (nv * nv);
+    // sigma : strain_el over the full tensor, row-major order
+    const double s00 = PF(a.pa, SXX, i), s11 = PF(a.pa, SYY, i), s22 = PF(a.pa, SZZ, i);
+    const double s01 = PF(a.pa, SXY, i), s02 = PF(a.pa, SXZ, i), s12 = PF(a.pa, SYZ, i);
+    const double e00 = PF(a.pa, EXX, i), e11 = PF(a.pa, EYY, i), e22 = PF(a.pa, EZZ, i);
+    const double e01 = PF(a.pa, EXY, i), e02 = PF(a.pa, EXZ, i), e12 = PF(a.pa, EYZ, i);
+    double t = s00 * e00;
+    t += s01 * e01;
+    t += s02 * e02;
+    t += s01 * e01;
+    t += s11 * e11;
+    t += s12 * e12;
+    t += s02 * e02;
+    t += s12 * e12;
+    t += s22 * e22;
+    es += 0.5 * PF(a.pa, VOL, i) * t;
+  }
+  sk[threadIdx.x] = ek;
+  ss[threadIdx.x] = es;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sk[threadIdx.x] += sk[threadIdx.x + o];
+      ss[threadIdx.x] += ss[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sk[0];
+    partial[2 * blockIdx.x + 1] = ss[0];
+  }
+}
+
+__global__ void k_energy_final(const double *partial, int nb, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double ek = 0, es = 0;
+  for (int b = 0; b < nb; b++) {
+    ek += partial[2 * b];
+    es += partial[2 * b + 1];
+  }
+  out[0] = ek;
+  out[1] = es;
+}
+
+// Dense stencil of the current positions (per pid): node tag or -1 when the
+// node does not exist or the pair is dropped, w, grad w.
+template <int S1>
+__global__ void k_debug_weights(KArgs a, int64_t *tag_out, double *w_out, double *wd_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const int S = a.dim == 3 ? S1 * S1 * S1 : S1 * S1;
+  const int off = S1 == 4 ? 1 : 0;
+  const int pid = IFLD(a.ia, PID, i);
+  double w[3][S1], dw[3][S1];
+  int base[3] = {0, 0, 0};
+  for (int d = 0; d < 3; d++)
+    for (int q = 0; q < S1; q++) { w[d][q] = 1; dw[d][q] = 0; }
+  for (int d = 0; d < a.dim; d++) {
+    const double x = PF(a.pa, PX + d, i);
+    const int c = cell_of_clamped(x, a.lo[d], a.inv_h, a.n[d]);
+    base[d] = c - off;
+    weights_1d<S1>(x, a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+  }
+  int s = 0;
+  for (int ia = 0; ia < S1; ia++)
+    for (int ib = 0; ib < S1; ib++)
+      for (int ic = 0; ic < (a.dim == 3 ? S1 : 1); ic++, s++) {
+        const int64_t o = (int64_t)pid * S + s;
+        double wt, g0, g1, g2;
+        if (a.dim == 3) {
+          const double wxy = w[0][ia] * w[1][ib];
+          wt = wxy * w[2][ic];
+          g0 = dw[0][ia] * w[1][ib] * w[2][ic];
+          g1 = w[0][ia] * dw[1][ib] * w[2][ic];
+          g2 = wxy * dw[2][ic];
+        } else {
+          wt = w[0][ia] * w[1][ib];
+          g0 = dw[0][ia] * w[1][ib];
+          g1 = w[0][ia] * dw[1][ib];
+          g2 = 0;
+        }
+        const int ni = base[0] + ia, nj = base[1] + ib, nk = a.dim == 3 ? base[2] + ic : 0;
+        const bool ok = wt != 0 && ni >= 0 && nj >= 0 && nk >= 0 && ni < a.n[0] &&
+                        nj < a.n[1] && nk < a.n[2];
+        tag_out[o] = ok ? ((int64_t)ni * a.n[1] + nj) * a.n[2] + nk : -1;
+        w_out[o] = ok ? wt : 0;
+        wd_out[3 * o] = ok ? g0 : 0;
+        wd_out[3 * o + 1] = ok ? g1 : 0;
+        wd_out[3 * o + 2] = ok ? g2 : 0;
+      }
+}
+
+// Reference particle-cell indices (ulmpm.cpp:163-165 linear, :206-208 splines), by pid.
+__global__ void k_cells(KArgs a, int *ijk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const int pid = IFLD(a.ia, PID, i);
+  for (int d = 0; d < 3; d++) {
+    int v = 0;
+    if (d < a.dim) {
+      const double x = PF(a.pa, PX + d, i);
+      if (a.S1 == 2) v = (int)((x - a.lo[d]) * a.inv_h);
+      else v = (int)((x - a.lo[d]) * a.inv_h - 1);
+    }
+    ijk[3 * (int64_t)pid + d] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tile configurations and launchers
+// ---------------------------------------------------------------------------
+template <int DIM, int S1> struct Tiles;
+template <> struct Tiles<3, 4> { static constexpr int TX = 4, TY = 4, TZ = 4, CH = 32; };
+template <> struct Tiles<3, 2> { static constexpr int TX = 4, TY = 4, TZ = 4, CH = 32; };
+template <> struct Tiles<2, 4> { static constexpr int TX = 8, TY = 8, TZ = 1, CH = 64; };
+template <> struct Tiles<2, 2> { static constexpr int TX = 8, TY = 8, TZ = 1, CH = 64; };
+
+static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+static int persistent_grid(kmpm_ctx *c, int per_cu) {
+  int64_t g = (int64_t)c->cu_count * per_cu;
+  int64_t maxt = c->a.tg.ntiles < c->a.np ? c->a.tg.ntiles : c->a.np;
+  if (maxt < 1) maxt = 1;
+  return (int)(g < maxt ? g : maxt);
+}
+
+#define DISPATCH(c, ...)                                                              \
+  do {                                                                                \
+    if ((c)->a.dim == 3 && (c)->a.S1 == 4) { constexpr int DIM = 3, S1 = 4; __VA_ARGS__; } \
+    else if ((c)->a.dim == 3) { constexpr int DIM = 3, S1 = 2; __VA_ARGS__; }               \
+    else if ((c)->a.S1 == 4) { constexpr int DIM = 2, S1 = 4; __VA_ARGS__; }                \
+    else { constexpr int DIM = 2, S1 = 2; __VA_ARGS__; }                                    \
+  } while (0)
+
+void tile_shape(int dim, int S1, int T[3]) {
+  if (dim == 3 && S1 == 4) { T[0] = Tiles<3, 4>::TX; T[1] = Tiles<3, 4>::TY; T[2] = Tiles<3, 4>::TZ; }
+  else if (dim == 3) { T[0] = Tiles<3, 2>::TX; T[1] = Tiles<3, 2>::TY; T[2] = Tiles<3, 2>::TZ; }
+  else if (S1 == 4) { T[0] = Tiles<2, 4>::TX; T[1] = Tiles<2, 4>::TY; T[2] = 1; }
+  else { T[0] = Tiles<2, 2>::TX; T[1] = Tiles<2, 2>::TY; T[2] = 1; }
+}
+
+hipError_t launch_keys(kmpm_ctx *c) {
+  if (c->a.np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keys, dim3(nblk(c->a.np, 256)), dim3(256), 0, c->stream, c->a);
+  return hipGetLastError();
+}
+
+size_t scan_temp_bytes(int64_t n) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int *)nullptr, (int *)nullptr, (int)n);
+  return bytes;
+}
+
+hipError_t launch_sort(kmpm_ctx *c) {
+  KArgs &a = c->a;
+  const int64_t nc = c->ncells;
+  int g = (int)(nc / 256 + 1);
+  if (g > c->cu_count * 16) g = c->cu_count * 16;
+  hipLaunchKernelGGL(k_sort_begin, dim3(g), dim3(256), 0, c->stream, a, nc);
+  if (a.np > 0) {
+    hipLaunchKernelGGL(k_count, dim3(nblk(a.np, 256)), dim3(256), 0, c->stream, a);
+  }
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, c->scan_tmp_bytes, a.cell_count,
+                                                  a.cell_start, (int)(nc + 1), c->stream);
+  if (e != hipSuccess) return e;
+  if (a.np > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(a.np, 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_rankfix, dim3(nblk(a.np, 256)), dim3(256), 0, c->stream, a);
+  }
+  hipLaunchKernelGGL(k_tiles, dim3(nblk(a.tg.ntiles, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_dilate, dim3(nblk(a.tg.ntiles, 256)), dim3(256), 0, c->stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_p2g(kmpm_ctx *c, int remap) {
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 2);
+  DISPATCH(c, {
+    using TT = Tiles<DIM, S1>;
+    if (remap)
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+    else
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_update(kmpm_ctx *c, int remap) {
+  int64_t total = (int64_t)c->a.tg.ntiles * c->a.tg.T[0] * c->a.tg.T[1] * c->a.tg.T[2];
+  int g = nblk(total, 256);
+  if (g > c->cu_count * 8) g = c->cu_count * 8;
+  if (g < 1) g = 1;
+  DISPATCH(c, {
+    using TT = Tiles<DIM, S1>;
+    hipLaunchKernelGGL((k_grid_update<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+                       c->stream, c->a, remap);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_clean(kmpm_ctx *c) {
+  int g = nblk(c->a.nn, 256);
+  if (g > c->cu_count * 16) g = c->cu_count * 16;
+  DISPATCH(c, {
+    using TT = Tiles<DIM, S1>;
+    hipLaunchKernelGGL((k_grid_clean<DIM, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+                       c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_g2p_usl(kmpm_ctx *c) {
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 4);
+  DISPATCH(c, {
+    using TT = Tiles<DIM, S1>;
+    hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+                       c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_g2p_musl_a(kmpm_ctx *c) {
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 4);
+  DISPATCH(c, {
+    using TT = Tiles<DIM, S1>;
+    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+                       c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_stress_musl_b(kmpm_ctx *c) {
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 4);
+  DISPATCH(c, {
+    using TT = Tiles<DIM, S1>;
+    hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+                       c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_adjust_dt(kmpm_ctx *c) {
+  hipLaunchKernelGGL(k_adjust_dt, dim3(1), dim3(64), 0, c->stream, c->a);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2) {
+  const int nb = 64;
+  hipLaunchKernelGGL(k_energy, dim3(nb), dim3(256), 0, c->stream, c->a, c->d_pmask, bit,
+                     d_out2 + 2);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, c->stream, d_out2 + 2, nb, d_out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double *d_wd) {
+  if (c->a.np == 0) return hipSuccess;
+  if (c->a.S1 == 4)
+    hipLaunchKernelGGL(k_debug_weights<4>, dim3(nblk(c->a.np, 128)), dim3(128), 0, c->stream,
+                       c->a, d_tag, d_w, d_wd);
+  else
+    hipLaunchKernelGGL(k_debug_weights<2>, dim3(nblk(c->a.np, 128)), dim3(128), 0, c->stream,
+                       c->a, d_tag, d_w, d_wd);
+  return hipGetLastError();
+}
+
+hipError_t launch_cells(kmpm_ctx *c, int *d_ijk) {
+  if (c->a.np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cells, dim3(nblk(c->a.np, 256)), dim3(256), 0, c->stream, c->a, d_ijk);
+  return hipGetLastError();
+}
+
+}  // namespace kmpm

@@ -1,0 +1,316 @@
+"""ctypes binding of libkmpm.so (include/kmpm.h).
+
+The product path is the HIP library: loading fails loudly when libkmpm.so is
+missing, and there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkmpm.so")
+
+# ---- constants (include/kmpm.h) ------------------------------------------
+OK = 0
+ERR_ARG, ERR_HIP, ERR_NO_DEVICE, ERR_CAPACITY = 1, 2, 3, 4
+ERR_PARTICLE_LEFT_BOX, ERR_JACOBIAN, ERR_DTCFL, ERR_WAVE_SPEED, ERR_GRID = 10, 11, 12, 13, 14
+METHOD_ULMPM, METHOD_TLMPM = 0, 1
+SCHEME_USL, SCHEME_MUSL = 0, 1
+SHAPE_LINEAR, SHAPE_CUBIC_SPLINE = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+MAT_LINEAR, MAT_EOS_STRENGTH = 0, 1
+EOS_NONE, EOS_LINEAR, EOS_SHOCK, EOS_FLUID = 0, 1, 2, 3
+STRENGTH_NONE, STRENGTH_LINEAR, STRENGTH_JOHNSON_COOK, STRENGTH_FLUID = 0, 1, 2, 3
+DAMAGE_NONE, DAMAGE_JOHNSON_COOK = 0, 1
+REGION_BLOCK, REGION_CYLINDER = 0, 1
+
+# style keys of the reference's plugin registries -> enum values
+EOS_KEYS = {"linear": EOS_LINEAR, "shock": EOS_SHOCK, "fluid": EOS_FLUID}
+STRENGTH_KEYS = {"linear": STRENGTH_LINEAR, "johnson_cook": STRENGTH_JOHNSON_COOK,
+                 "fluid": STRENGTH_FLUID}
+DAMAGE_KEYS = {"damage_johnson_cook": DAMAGE_JOHNSON_COOK}
+SCHEME_KEYS = {"usl": SCHEME_USL, "musl": SCHEME_MUSL}
+METHOD_KEYS = {"ulmpm": METHOD_ULMPM, "tlmpm": METHOD_TLMPM}
+SHAPE_KEYS = {"linear": SHAPE_LINEAR, "cubic-spline": SHAPE_CUBIC_SPLINE}
+
+D3 = C.c_double * 3
+I3 = C.c_int * 3
+I64_3 = C.c_int64 * 3
+
+
+class Config(C.Structure):
+    _fields_ = [("dimension", C.c_int), ("method", C.c_int), ("scheme", C.c_int),
+                ("shape", C.c_int), ("precision", C.c_int), ("reserved0", C.c_int),
+                ("boxlo", D3), ("boxhi", D3), ("cellsize", C.c_double),
+                ("pic_flip", C.c_double), ("dt", C.c_double), ("dt_constant", C.c_int),
+                ("reserved1", C.c_int), ("dt_factor", C.c_double)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int), ("eos", C.c_int), ("strength", C.c_int), ("damage", C.c_int),
+                ("lin_p", C.c_double * 3), ("eos_p", C.c_double * 9),
+                ("strength_p", C.c_double * 9), ("damage_p", C.c_double * 8),
+                ("rho0", C.c_double), ("K", C.c_double), ("G", C.c_double), ("E", C.c_double),
+                ("nu", C.c_double), ("lambda_", C.c_double), ("signal_velocity", C.c_double)]
+
+
+class Region(C.Structure):
+    _fields_ = [("kind", C.c_int), ("axis", C.c_int), ("interior", C.c_int),
+                ("reserved", C.c_int), ("p", C.c_double * 6)]
+
+
+_DP = C.POINTER(C.c_double)
+
+
+class Particles(C.Structure):
+    _fields_ = [("np", C.c_int64), ("ptag", C.POINTER(C.c_int64)),
+                ("solid", C.POINTER(C.c_int32)), ("mat", C.POINTER(C.c_int32)),
+                ("mask", C.POINTER(C.c_uint32)), ("x0", _DP), ("x", _DP), ("v", _DP),
+                ("F", _DP), ("sigma", _DP), ("strain_el", _DP), ("mass", _DP), ("vol0", _DP),
+                ("vol", _DP), ("rho0", _DP), ("rho", _DP), ("J", _DP),
+                ("eff_plastic_strain", _DP), ("eff_plastic_strain_rate", _DP),
+                ("damage", _DP), ("damage_init", _DP)]
+
+
+# particle arrays: name -> (dtype, components)
+PARTICLE_FIELDS = {
+    "ptag": (np.int64, 1), "solid": (np.int32, 1), "mat": (np.int32, 1), "mask": (np.uint32, 1),
+    "x0": (np.float64, 3), "x": (np.float64, 3), "v": (np.float64, 3), "F": (np.float64, 9),
+    "sigma": (np.float64, 9), "strain_el": (np.float64, 9), "mass": (np.float64, 1),
+    "vol0": (np.float64, 1), "vol": (np.float64, 1), "rho0": (np.float64, 1),
+    "rho": (np.float64, 1), "J": (np.float64, 1), "eff_plastic_strain": (np.float64, 1),
+    "eff_plastic_strain_rate": (np.float64, 1), "damage": (np.float64, 1),
+    "damage_init": (np.float64, 1),
+}
+
+
+def empty_particles(n):
+    """Dict of zeroed particle arrays in the reference layouts."""
+    out = {}
+    for k, (dt, c) in PARTICLE_FIELDS.items():
+        out[k] = np.zeros((n, c) if c > 1 else (n,), dtype=dt)
+    return out
+
+
+def particles_struct(arrs, n=None):
+    """Build a Particles struct pointing at the numpy arrays in `arrs`
+    (the arrays must stay alive while the struct is used)."""
+    p = Particles()
+    p.np = int(n if n is not None else len(arrs["ptag"]))
+    ctmap = {np.int64: C.c_int64, np.int32: C.c_int32, np.uint32: C.c_uint32,
+             np.float64: C.c_double}
+    for k, (dt, _) in PARTICLE_FIELDS.items():
+        a = arrs.get(k)
+        if a is None:
+            setattr(p, k, None)
+            continue
+        assert a.dtype == dt and a.flags["C_CONTIGUOUS"], k
+        setattr(p, k, a.ctypes.data_as(C.POINTER(ctmap[dt])))
+    return p
+
+
+def concat_particles(parts):
+    return {k: np.concatenate([p[k] for p in parts]) for k in PARTICLE_FIELDS}
+
+
+def _declare(lib):
+    vp = C.c_void_p
+    P = C.POINTER
+    sigs = {
+        "kmpm_status_string": (C.c_char_p, [C.c_int]),
+        "kmpm_material_init": (C.c_int, [P(Material)]),
+        "kmpm_grid_size": (C.c_int, [P(Config), I64_3]),
+        "kmpm_populate": (C.c_int, [P(Config), P(Region), C.c_int, P(Material), C.c_int, C.c_int,
+                                    C.c_int64, P(Particles), C.c_int64, P(C.c_int64)]),
+        "kmpm_region_match": (C.c_int, [P(Region), C.c_int, C.c_double, C.c_double, C.c_double]),
+        "kmpm_ctx_create": (C.c_int, [P(Config), P(Material), C.c_int, P(vp)]),
+        "kmpm_ctx_destroy": (C.c_int, [vp]),
+        "kmpm_upload_particles": (C.c_int, [vp, P(Particles)]),
+        "kmpm_download_particles": (C.c_int, [vp, P(Particles)]),
+        "kmpm_set_node_mask": (C.c_int, [vp, P(C.c_uint32), C.c_int64]),
+        "kmpm_fix_velocity_nodes": (C.c_int, [vp, C.c_uint32, I3, D3, D3]),
+        "kmpm_fix_body_force": (C.c_int, [vp, C.c_uint32, I3, D3]),
+        "kmpm_compute_grid_weight_functions_and_gradients": (C.c_int, [vp]),
+        "kmpm_particles_to_grid": (C.c_int, [vp]),
+        "kmpm_update_grid_state": (C.c_int, [vp]),
+        "kmpm_grid_to_points_and_stress": (C.c_int, [vp]),
+        "kmpm_grid_to_points": (C.c_int, [vp]),
+        "kmpm_velocities_to_grid": (C.c_int, [vp]),
+        "kmpm_update_stress": (C.c_int, [vp]),
+        "kmpm_adjust_dt": (C.c_int, [vp]),
+        "kmpm_step": (C.c_int, [vp, C.c_int64]),
+        "kmpm_synchronize": (C.c_int, [vp, P(C.c_int64)]),
+        "kmpm_get_time": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_int64)]),
+        "kmpm_reduce_energy": (C.c_int, [vp, C.c_uint32, P(C.c_double), P(C.c_double)]),
+        "kmpm_download_grid": (C.c_int, [vp, _DP, _DP, _DP, C.c_int64]),
+        "kmpm_download_cells": (C.c_int, [vp, P(C.c_int32), P(C.c_int64), C.c_int64]),
+        "kmpm_debug_weights": (C.c_int, [vp, P(C.c_int64), _DP, _DP, C.c_int64]),
+        "kmpm_get_tile_shape": (C.c_int, [vp, I3]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    """Load libkmpm.so (built by karamelo_amd.build); raises if it is missing."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libkmpm.so is not built (run karamelo_amd.build.build()); "
+                               "there is no CPU fallback for the MPM step")
+        _LIB = _declare(C.CDLL(LIB_PATH))
+    return _LIB
+
+
+class KmpmError(RuntimeError):
+    def __init__(self, code, what=""):
+        msg = lib().kmpm_status_string(code).decode()
+        super().__init__("%s: status %d (%s)" % (what, code, msg))
+        self.code = code
+
+
+def check(code, what=""):
+    if code != OK:
+        raise KmpmError(code, what)
+    return code
+
+
+def material_init(m):
+    check(lib().kmpm_material_init(C.byref(m)), "kmpm_material_init")
+    return m
+
+
+def grid_size(cfg):
+    n = I64_3()
+    check(lib().kmpm_grid_size(C.byref(cfg), n), "kmpm_grid_size")
+    return tuple(int(v) for v in n)
+
+
+def populate(cfg, region, ppc1d, mat, mat_index, solid_index, ptag_offset, fn=None):
+    """Solid::populate through the C ABI (or an equivalent `fn` with the same
+    signature, e.g. the oracle's); returns a dict of particle arrays."""
+    fn = fn or lib().kmpm_populate
+    npo = C.c_int64(0)
+    empty = particles_struct(empty_particles(0), 0)
+    st = fn(C.byref(cfg), C.byref(region), ppc1d, C.byref(mat), mat_index, solid_index,
+            ptag_offset, C.byref(empty), 0, C.byref(npo))
+    if st not in (OK, ERR_CAPACITY):
+        raise KmpmError(st, "populate (count)")
+    n = npo.value
+    arrs = empty_particles(n)
+    ps = particles_struct(arrs, n)
+    check(fn(C.byref(cfg), C.byref(region), ppc1d, C.byref(mat), mat_index, solid_index,
+             ptag_offset, C.byref(ps), n, C.byref(npo)), "populate")
+    return arrs
+
+
+class Context:
+    """One device context (kmpm_ctx) on the current HIP device."""
+
+    def __init__(self, cfg, mats):
+        self.L = lib()
+        self.cfg = cfg
+        arr = (Material * len(mats))(*mats)
+        self._mats = arr
+        h = C.c_void_p()
+        check(self.L.kmpm_ctx_create(C.byref(cfg), arr, len(mats), C.byref(h)), "kmpm_ctx_create")
+        self.h = h
+        self.nn = int(np.prod(grid_size(cfg)))
+        self.np = 0
+
+    def close(self):
+        if self.h:
+            self.L.kmpm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, arrs):
+        self._upload_keep = arrs
+        ps = particles_struct(arrs)
+        check(self.L.kmpm_upload_particles(self.h, C.byref(ps)), "upload")
+        self.np = ps.np
+
+    def download(self):
+        arrs = empty_particles(self.np)
+        ps = particles_struct(arrs, self.np)
+        check(self.L.kmpm_download_particles(self.h, C.byref(ps)), "download")
+        return arrs
+
+    def set_node_mask(self, mask):
+        mask = np.ascontiguousarray(mask, dtype=np.uint32)
+        check(self.L.kmpm_set_node_mask(self.h, mask.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        len(mask)), "set_node_mask")
+
+    def fix_velocity_nodes(self, bit, setv, val, prev):
+        check(self.L.kmpm_fix_velocity_nodes(self.h, bit, I3(*setv), D3(*val), D3(*prev)),
+              "fix_velocity_nodes")
+
+    def fix_body_force(self, bit, setv, val):
+        check(self.L.kmpm_fix_body_force(self.h, bit, I3(*setv), D3(*val)), "fix_body_force")
+
+    def phase(self, name):
+        check(getattr(self.L, "kmpm_" + name)(self.h), name)
+
+    def step(self, n):
+        check(self.L.kmpm_step(self.h, int(n)), "step")
+
+    def synchronize(self):
+        t = C.c_int64(-1)
+        st = self.L.kmpm_synchronize(self.h, C.byref(t))
+        return st, t.value
+
+    def time(self):
+        dt, t, s = C.c_double(), C.c_double(), C.c_int64()
+        check(self.L.kmpm_get_time(self.h, C.byref(dt), C.byref(t), C.byref(s)), "get_time")
+        return dt.value, t.value, s.value
+
+    def energy(self, bit=1):
+        ek, es = C.c_double(), C.c_double()
+        check(self.L.kmpm_reduce_energy(self.h, bit, C.byref(ek), C.byref(es)), "energy")
+        return ek.value, es.value
+
+    def grid(self):
+        m = np.zeros(self.nn)
+        v = np.zeros((self.nn, 3))
+        vu = np.zeros((self.nn, 3))
+        check(self.L.kmpm_download_grid(self.h, m.ctypes.data_as(_DP), v.ctypes.data_as(_DP),
+                                        vu.ctypes.data_as(_DP), self.nn), "download_grid")
+        return m, v, vu
+
+    def cells(self, sorted_ptags=True):
+        ijk = np.zeros((self.np, 3), dtype=np.int32)
+        sp = np.zeros(self.np, dtype=np.int64)
+        check(self.L.kmpm_download_cells(
+            self.h, ijk.ctypes.data_as(C.POINTER(C.c_int32)),
+            sp.ctypes.data_as(C.POINTER(C.c_int64)) if sorted_ptags else None, self.np),
+            "download_cells")
+        return ijk, (sp if sorted_ptags else None)
+
+    def tile_shape(self):
+        t = I3()
+        check(self.L.kmpm_get_tile_shape(self.h, t), "get_tile_shape")
+        return tuple(t)
+
+    def debug_weights(self):
+        d = self.cfg.dimension
+        S1 = 2 if self.cfg.shape == SHAPE_LINEAR else 4
+        S = S1 ** d
+        tag = np.zeros((self.np, S), dtype=np.int64)
+        w = np.zeros((self.np, S))
+        wd = np.zeros((self.np, S, 3))
+        check(self.L.kmpm_debug_weights(self.h, tag.ctypes.data_as(C.POINTER(C.c_int64)),
+                                        w.ctypes.data_as(_DP), wd.ctypes.data_as(_DP), self.np),
+              "debug_weights")
+        return tag, w, wd

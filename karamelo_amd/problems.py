@@ -1,0 +1,242 @@
+"""Problem descriptions for the explicit MPM step: the configurations the
+input scripts of the reference would build (method/dimension/region/
+material/solid/fix commands), expressed through the C ABI's POD types.
+
+Numeric literals follow the reference's parser: plain decimals are read
+with stof (float32) and widened (src/input.cpp:407,411); integer ratios
+such as 1/20 are evaluated in double; `aeb` is a * 10^b (input.cpp:191).
+"""
+import dataclasses
+from typing import List, Optional
+
+import numpy as np
+
+from . import native as N
+
+
+def lit(x):
+    """A decimal literal as the reference's parser stores it (stof, then double)."""
+    return float(np.float32(x))
+
+
+def lit_e(mant, exp):
+    """`<mant>e<exp>` literal: mant * pow(10, exp) (input.cpp:191, powv)."""
+    return float(np.float32(mant)) * float(10.0 ** exp)
+
+
+@dataclasses.dataclass
+class Fix:
+    kind: str                 # "velocity_nodes" | "body_force"
+    bit: int
+    set: tuple
+    value: tuple
+    prev: Optional[tuple] = None
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    cfg: N.Config
+    mats: List[N.Material]
+    particles: dict
+    node_mask: Optional[np.ndarray] = None
+    fixes: List[Fix] = dataclasses.field(default_factory=list)
+    steps: int = 0
+    notes: str = ""
+
+    @property
+    def np(self):
+        return len(self.particles["ptag"])
+
+
+def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-16,
+                dt_constant=False, dt_factor=0.9, method="ulmpm"):
+    c = N.Config()
+    c.dimension = dim
+    c.method = N.METHOD_KEYS[method]
+    c.scheme = N.SCHEME_KEYS[scheme]
+    c.shape = N.SHAPE_KEYS[shape]
+    c.precision = N.PRECISION_FP64
+    for d in range(3):
+        c.boxlo[d] = lo[d] if d < dim else 0.0
+        c.boxhi[d] = hi[d] if d < dim else 0.0
+    c.cellsize = h
+    c.pic_flip = flip
+    c.dt = dt
+    c.dt_constant = 1 if dt_constant else 0
+    c.dt_factor = dt_factor
+    return c
+
+
+def linear_material(rho, E, nu):
+    m = N.Material()
+    m.type = N.MAT_LINEAR
+    m.lin_p[:] = [rho, E, nu]
+    return N.material_init(m)
+
+
+def eos_strength_material(eos, eos_p, strength, strength_p, damage=None, damage_p=()):
+    m = N.Material()
+    m.type = N.MAT_EOS_STRENGTH
+    m.eos = N.EOS_KEYS[eos]
+    m.strength = N.STRENGTH_KEYS[strength]
+    m.damage = N.DAMAGE_KEYS[damage] if damage else N.DAMAGE_NONE
+    m.eos_p[:len(eos_p)] = list(eos_p)
+    m.strength_p[:len(strength_p)] = list(strength_p)
+    m.damage_p[:len(damage_p)] = list(damage_p)
+    return N.material_init(m)
+
+
+def block(xlo, xhi, ylo, yhi, zlo=0.0, zhi=0.0):
+    r = N.Region()
+    r.kind = N.REGION_BLOCK
+    r.interior = 1
+    r.p[:] = [xlo, xhi, ylo, yhi, zlo, zhi]
+    return r
+
+
+def cylinder(c1, c2, R, lo=0.0, hi=0.0, axis=2):
+    r = N.Region()
+    r.kind = N.REGION_CYLINDER
+    r.axis = axis
+    r.interior = 1
+    r.p[:] = [c1, c2, R, lo, hi, 0.0]
+    return r
+
+
+def node_coords(cfg):
+    """Grid node positions in tag order (grid.cpp:227-231)."""
+    n = N.grid_size(cfg)
+    idx = [np.arange(n[d], dtype=np.float64) for d in range(3)]
+    out = []
+    for d in range(3):
+        if d < cfg.dimension:
+            out.append(cfg.boxlo[d] + idx[d] * cfg.cellsize)
+        else:
+            out.append(np.zeros(1))
+    X, Y, Z = np.meshgrid(out[0], out[1], out[2], indexing="ij")
+    return X.ravel(), Y.ravel(), Z.ravel()
+
+
+def region_match_nodes(region, X, Y, Z):
+    """Region::match on arrays (region_block.cpp:148-154, region_cylinder.cpp:147-168)."""
+    p = list(region.p)
+    if region.kind == N.REGION_BLOCK:
+        inside = (X >= p[0]) & (X <= p[1]) & (Y >= p[2]) & (Y <= p[3]) & (Z >= p[4]) & (Z <= p[5])
+    else:
+        c1, c2, R, lo, hi = p[:5]
+        a, b, al = {0: (Y, Z, X), 1: (X, Z, Y), 2: (X, Y, Z)}[region.axis]
+        d2 = (a - c1) * (a - c1) + (b - c2) * (b - c2)
+        inside = (al >= lo) & (al <= hi) & (d2 <= R * R)
+    return inside if region.interior else ~inside
+
+
+def populate_solids(cfg, specs, fn=None):
+    """specs: list of (region, ppc1d, mat, mat_index) in solid order."""
+    parts, off = [], 0
+    for s, (region, ppc, mat, mi) in enumerate(specs):
+        arrs = N.populate(cfg, region, ppc, mat, mi, s, off, fn=fn)
+        off += len(arrs["ptag"])
+        parts.append(arrs)
+    return N.concat_particles(parts)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def two_disks(steps=3000, ppc1d=2, flip=1.0, populate_fn=None):
+    """Two elastic disks colliding (2D ULMPM USL, linear shape, FP64):
+    unit square, cell 1/20, disks R=0.2 at (0.25,0.25) and (0.75,0.75),
+    E=1000, nu=0.3, rho=1000, v0 = +-0.1 along the diagonal, dt = 1e-3."""
+    h = 1.0 / 20.0
+    cfg = make_config(2, (0.0, 0.0), (1.0, 1.0), h, scheme="usl", shape="linear", flip=flip,
+                      dt=lit_e(1, -3), dt_constant=True)
+    mat = linear_material(1000.0, 1000.0, lit(0.3))
+    R = lit(0.2)
+    specs = [(cylinder(0.25, 0.25, R), ppc1d, mat, 0), (cylinder(0.75, 0.75, R), ppc1d, mat, 0)]
+    p = populate_solids(cfg, specs, fn=populate_fn)
+    v = lit(0.1)
+    s0 = p["solid"] == 0
+    p["v"][s0, 0] = v
+    p["v"][s0, 1] = v
+    p["v"][~s0, 0] = -v
+    p["v"][~s0, 1] = -v
+    return Problem("two_disks", cfg, [mat], p, steps=steps,
+                   notes="fix initial_velocity_particles applied at set-up (step-1 hook)")
+
+
+def steel_jc(damage=True, d_params=(0.05, 3.44, -2.12, 0.002, 0.61)):
+    """Steel with Johnson-Cook strength + shock EOS (+ JC damage), SI units.
+    E = 200 GPa, nu = 0.3 -> K, G; JC A=792 MPa, B=510 MPa, n=0.26, C=0.014,
+    epsdot0 = 1/s, m = 1, Tr = 0, Tm = 1800; shock rho0=7850, c0 = sqrt(K/rho0),
+    S = 1.49, Gamma = 2, cv = 0, Q1 = Q2 = 0."""
+    E, nu, rho = 200e9, lit(0.3), 7850.0
+    K = E / (3 * (1 - 2 * nu))
+    G = E / (2 * (1 + nu))
+    c0 = float(np.sqrt(K / rho))
+    eos_p = (rho, K, c0, lit(1.49), 2.0, 0.0, 0.0, 0.0, 0.0)
+    jc_p = (G, 792e6, 510e6, lit(0.26), 1.0, lit(0.014), 1.0, 0.0, 1800.0)
+    dmg_p = tuple(lit(d) for d in d_params) + (1.0, 0.0, 1800.0)
+    if damage:
+        return eos_strength_material("shock", eos_p, "johnson_cook", jc_p,
+                                     "damage_johnson_cook", dmg_p)
+    return eos_strength_material("shock", eos_p, "johnson_cook", jc_p)
+
+
+def tensile_bar(h=6e-5, ppc1d=2, v_pull=10.0, shape="cubic-spline", scheme="usl",
+                damage=True, d_params=(0.05, 3.44, -2.12, 0.002, 0.61), steps=0,
+                populate_fn=None, length=0.02, width=0.0024):
+    """3D ULMPM tensile bar with JC strength, shock EOS and JC damage.
+    Bar length x width x width centred at the origin in a box twice its
+    size; nodes within 0.5 mm of each end are driven at -+v_pull along x
+    (fix velocity_nodes).  h = 0.06 mm, 2x2x2 particles per cell gives
+    4.27M particles."""
+    L2, W2 = length / 2, width / 2
+    cfg = make_config(3, (-2 * L2, -2 * W2, -2 * W2), (2 * L2, 2 * W2, 2 * W2), h,
+                      scheme=scheme, shape=shape, flip=0.99)
+    mat = steel_jc(damage, d_params)
+    p = populate_solids(cfg, [(block(-L2, L2, -W2, W2, -W2, W2), ppc1d, mat, 0)],
+                        fn=populate_fn)
+    X, Y, Z = node_coords(cfg)
+    xbc = L2 - 0.0005
+    left = region_match_nodes(block(-1e20, -xbc, -1e20, 1e20, -1e20, 1e20), X, Y, Z)
+    right = region_match_nodes(block(xbc, 1e20, -1e20, 1e20, -1e20, 1e20), X, Y, Z)
+    mask = np.ones(len(X), dtype=np.uint32)
+    mask[left] |= 2
+    mask[right] |= 4
+    fixes = [Fix("velocity_nodes", 2, (1, 0, 0), (-v_pull, 0, 0), (-v_pull, 0, 0)),
+             Fix("velocity_nodes", 4, (1, 0, 0), (v_pull, 0, 0), (v_pull, 0, 0))]
+    return Problem("tensile_bar", cfg, [mat], p, node_mask=mask, fixes=fixes, steps=steps)
+
+
+def dam_break(n_cells=32, ppc1d=2, steps=0, scheme="musl", shape="cubic-spline", jitter=0.0,
+              seed=20261202, populate_fn=None):
+    """3D ULMPM dam break: unit box, water column 0.4 x 0.4 x 0.8 in a corner,
+    eos fluid (rho0=1000, K=2e6, Gamma=7) + strength fluid (mu=1e-3),
+    gravity -9.81 along z (fix body_force on all nodes)."""
+    h = 1.0 / n_cells
+    cfg = make_config(3, (0, 0, 0), (1, 1, 1), h, scheme=scheme, shape=shape, flip=0.99)
+    mat = eos_strength_material("fluid", (1000.0, 2e6, 7.0), "fluid", (lit_e(1, -3),))
+    p = populate_solids(cfg, [(block(0, lit(0.4), 0, lit(0.4), 0, lit(0.8)), ppc1d, mat, 0)],
+                        fn=populate_fn)
+    if jitter:
+        rng = np.random.default_rng(seed)
+        p["x"] += rng.uniform(-jitter * h, jitter * h, size=p["x"].shape)
+        p["x"] = np.clip(p["x"], 1e-9, 1 - 1e-9)
+        p["x0"][:] = p["x"]
+    fixes = [Fix("body_force", 1, (0, 0, 1), (0, 0, lit(-9.81)))]
+    return Problem("dam_break", cfg, [mat], p, fixes=fixes, steps=steps)
+
+
+def context_for(problem):
+    """Device context holding the problem (particles, node groups, fixes)."""
+    ctx = N.Context(problem.cfg, problem.mats)
+    ctx.upload(problem.particles)
+    if problem.node_mask is not None:
+        ctx.set_node_mask(problem.node_mask)
+    for f in problem.fixes:
+        if f.kind == "velocity_nodes":
+            ctx.fix_velocity_nodes(f.bit, f.set, f.value, f.prev)
+        else:
+            ctx.fix_body_force(f.bit, f.set, f.value)
+    return ctx

@@ -1,0 +1,1114 @@
+/*
+ * kmpm_oracle.c -- CPU FP64 restatement of Karamelo's explicit MPM step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load this library; the product path
+ * (libkmpm.so) never does.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the reference repository root, src/).  Expression order follows the
+ * reference's C++/Eigen expressions left to right; the library is compiled
+ * with -ffp-contract=off so no fused multiply-add changes a rounding.
+ *
+ * Scope: ULMPM with the USL and MUSL schemes, linear and cubic-spline shape
+ * functions, 2D and 3D, materials linear / eos-strength with eos
+ * linear|shock|fluid, strength linear|johnson_cook|fluid, damage
+ * damage_johnson_cook; fixes velocity_nodes (constant values) and
+ * body_force (constant values); computes kinetic/strain energy.  One MPI rank.
+ */
+#include "kmpm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SQRT_3_OVER_2 1.224744871                              /* strength_jc.cpp:29 */
+#define FOUR_THIRD 1.333333333333333333333333333333333333333   /* solid.cpp:39 */
+#define MAXFIX 16
+
+static double dmax(double a, double b) { return a > b ? a : b; } /* MAX macro */
+static double dmin(double a, double b) { return a < b ? a : b; }
+static int imin(int a, int b) { return a < b ? a : b; }
+static double square(double x) { return x * x; }                /* math_special.h */
+
+/* ------------------------------------------------------------------------ */
+/* shape functions: basis_functions.h:21-104                                 */
+/* ------------------------------------------------------------------------ */
+static double bf_linear(double r_, int ntype) {                 /* :21-28 */
+  (void)ntype;
+  double r = fabs(r_);
+  if (r >= 1.0) return 0.0;
+  return 1.0 - r;
+}
+static double bf_dlinear(double r, int ntype, double inv_cellsize) { /* :30-38 */
+  (void)ntype;
+  if (r >= 1.0 || r <= -1.0 || r == 0) return 0.0;
+  else if (r > 0.0) return -inv_cellsize;
+  else return inv_cellsize;
+}
+static double bf_cubic(double r, int ntype) {                   /* :40-71 */
+  if (r >= 1 && r < 2) {
+    if (ntype == 1) return 0;
+    return ((-1.0 / 6.0 * r + 1) * r - 2) * r + 4.0 / 3.0;
+  } else if (r >= 0 && r < 1) {
+    if (ntype == -2) return (1.0 / 6.0 * r * r - 1) * r + 1;
+    else if (ntype == 2) return 1;
+    else if (ntype == 1) return (1.0 / 3.0 * r - 1) * r * r + 2.0 / 3.0;
+    else return (0.5 * r - 1) * r * r + 2.0 / 3.0;
+  } else if (r >= -1 && r < 0) {
+    if (ntype == 2) return (-1.0 / 6.0 * r * r + 1) * r + 1;
+    else if (ntype == -1) return (-1.0 / 3.0 * r - 1) * r * r + 2.0 / 3.0;
+    else return (-0.5 * r - 1) * r * r + 2.0 / 3.0;
+  } else if (r >= -2 && r < -1) {
+    return ((1.0 / 6.0 * r + 1) * r + 2) * r + 4.0 / 3.0;
+  }
+  return 0;
+}
+static double bf_dcubic(double r, int ntype, double icellsize) { /* :73-104 */
+  if (r >= 1 && r < 2) {
+    if (ntype == 1) return -icellsize;
+    return icellsize * ((-0.5 * r + 2) * r - 2);
+  } else if (r >= 0 && r < 1) {
+    if (ntype == -2) return icellsize * (0.5 * r * r - 1);
+    else if (ntype == 2) return icellsize;
+    else if (ntype == 1) return icellsize * r * (r - 2);
+    else return icellsize * (3.0 / 2.0 * r - 2) * r;
+  } else if (r >= -1 && r < 0) {
+    if (ntype == 2) return icellsize * (-0.5 * r * r + 1);
+    else if (ntype == -1) return icellsize * (-r - 2) * r;
+    else return icellsize * (-3.0 / 2.0 * r - 2) * r;
+  } else if (r >= -2 && r < -1) {
+    return icellsize * ((0.5 * r + 2) * r + 2);
+  }
+  return 0;
+}
+double orc_basis(int shape, int derivative, double r, int ntype, double inv_h) {
+  if (shape == KMPM_SHAPE_LINEAR)
+    return derivative ? bf_dlinear(r, ntype, inv_h) : bf_linear(r, ntype);
+  return derivative ? bf_dcubic(r, ntype, inv_h) : bf_cubic(r, ntype);
+}
+
+/* ------------------------------------------------------------------------ */
+/* materials: Mat ctors, material.cpp:720-769                                */
+/* ------------------------------------------------------------------------ */
+int orc_material_init(kmpm_material *m) {
+  if (m->type == KMPM_MAT_LINEAR) {                              /* :745-764 */
+    m->rho0 = m->lin_p[0];
+    m->E = m->lin_p[1];
+    m->nu = m->lin_p[2];
+    m->G = m->E / (2 * (1 + m->nu));
+    m->lambda = m->E * m->nu / ((1 + m->nu) * (1 - 2 * m->nu));
+    m->K = m->E / (3 * (1 - 2 * m->nu));
+    m->signal_velocity = sqrt(m->K / m->rho0);
+    return KMPM_OK;
+  }
+  if (m->type != KMPM_MAT_EOS_STRENGTH) return KMPM_ERR_ARG;
+  /* EOS::rho0(), EOS::K() (eos_*.cpp), Strength::G() (strength_*.cpp) */
+  if (m->eos < KMPM_EOS_LINEAR || m->eos > KMPM_EOS_FLUID) return KMPM_ERR_ARG;
+  if (m->strength < KMPM_STRENGTH_LINEAR || m->strength > KMPM_STRENGTH_FLUID) return KMPM_ERR_ARG;
+  m->rho0 = m->eos_p[0];
+  m->K = m->eos_p[1];
+  m->G = m->strength_p[0];
+  m->E = 9 * m->K * m->G / (3 * m->K + m->G);                  /* :732-736 */
+  m->nu = (3 * m->K - 2 * m->G) / (2 * (3 * m->K + m->G));
+  m->lambda = m->K - 2 * m->G / 3;
+  m->signal_velocity = sqrt((m->lambda + 2 * m->G) / m->rho0);
+  if (m->strength == KMPM_STRENGTH_JOHNSON_COOK && m->strength_p[7] == m->strength_p[8])
+    return KMPM_ERR_ARG;                                        /* strength_jc.cpp:89-93 */
+  if (m->damage == KMPM_DAMAGE_JOHNSON_COOK && m->damage_p[6] == m->damage_p[7])
+    return KMPM_ERR_ARG;                                        /* damage_jc.cpp:80-84 */
+  return KMPM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid: Grid::init for one rank, grid.cpp:68-275                            */
+/* ------------------------------------------------------------------------ */
+static void grid_global_counts(const kmpm_config *c, int n[3]) {
+  double h = c->cellsize;
+  double Lx = c->boxhi[0] - c->boxlo[0];                        /* :151-155 */
+  n[0] = ((int)(Lx / h)) + 1;
+  while (n[0] * h <= Lx + 0.5 * h) n[0]++;
+  if (c->dimension >= 2) {                                      /* :157-163 */
+    double Ly = c->boxhi[1] - c->boxlo[1];
+    n[1] = (int)(((int)Ly) / h) + 1;
+    while (n[1] * h <= Ly + 0.5 * h) n[1]++;
+  } else n[1] = 1;
+  if (c->dimension == 3) {                                      /* :165-171 */
+    double Lz = c->boxhi[2] - c->boxlo[2];
+    n[2] = (int)(((int)Lz) / h) + 1;
+    while (n[2] * h <= Lz + 0.5 * h) n[2]++;
+  } else n[2] = 1;
+}
+/* local node counts for a rank that owns the whole box (:177-206) */
+static void grid_local_counts(const kmpm_config *c, int n[3]) {
+  double h = c->cellsize;
+  for (int d = 0; d < 3; d++) {
+    if (d >= c->dimension) { n[d] = 1; continue; }
+    double L = c->boxhi[d] - c->boxlo[d];
+    n[d] = (int)ceil(L / h);
+    if (n[d] < 0) n[d] = 0;
+    while (c->boxlo[d] + h * (0 + n[d] - 0.5) < c->boxhi[d]) n[d]++;
+  }
+}
+int orc_grid_size(const kmpm_config *c, int64_t out[3]) {
+  int g[3], l[3];
+  if (!c || c->cellsize <= 0) return KMPM_ERR_ARG;
+  grid_global_counts(c, g);
+  grid_local_counts(c, l);
+  for (int d = 0; d < 3; d++) {
+    if (g[d] != l[d]) return KMPM_ERR_GRID;  /* tags would not be a bijection */
+    out[d] = g[d];
+  }
+  return KMPM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* regions: region_block.cpp:148-154, region_cylinder.cpp:147-168,           */
+/* Region::match region.cpp:65-72                                            */
+/* ------------------------------------------------------------------------ */
+static int region_inside(const kmpm_region *r, double x, double y, double z) {
+  if (r->kind == KMPM_REGION_BLOCK) {
+    if (x >= r->p[0] && x <= r->p[1] && y >= r->p[2] && y <= r->p[3] &&
+        z >= r->p[4] && z <= r->p[5])
+      return 1;
+    return 0;
+  }
+  double c1 = r->p[0], c2 = r->p[1], R = r->p[2], lo = r->p[3], hi = r->p[4];
+  double RSq = R * R, dSq;
+  if (r->axis == 0) {
+    dSq = square(y - c1) + square(z - c2);
+    if (x >= lo && x <= hi && dSq <= RSq) return 1;
+  } else if (r->axis == 1) {
+    dSq = square(x - c1) + square(z - c2);
+    if (y >= lo && y <= hi && dSq <= RSq) return 1;
+  } else {
+    dSq = square(x - c1) + square(y - c2);
+    if (z >= lo && z <= hi && dSq <= RSq) return 1;
+  }
+  return 0;
+}
+int orc_region_match(const kmpm_region *r, int dimension, double x, double y, double z) {
+  (void)dimension;
+  int in = region_inside(r, x, y, z);
+  return r->interior ? in : !in;
+}
+
+/* ------------------------------------------------------------------------ */
+/* populate: Solid::populate, solid.cpp:1810-2336 (ULMPM, one rank)          */
+/* ------------------------------------------------------------------------ */
+int orc_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
+                 const kmpm_material *mat, int mat_index, int solid_index,
+                 int64_t ptag_offset, kmpm_particles *out, int64_t capacity,
+                 int64_t *np_out) {
+  if (c->method != KMPM_METHOD_ULMPM) return KMPM_ERR_ARG;
+  const double *boundlo = c->boxlo, *boundhi = c->boxhi;       /* :1933-1936 */
+  const double *sublo = c->boxlo, *subhi = c->boxhi;
+  double delta = c->cellsize;                                   /* :1863 */
+  int dim = c->dimension;
+  int noffsetlo[3], noffsethi[3], nsub[3];
+  for (int d = 0; d < 3; d++) {                                 /* :1938-1986 */
+    double Llo = dmax(0.0, sublo[d] - boundlo[d]);
+    double Lhi = dmax(0.0, dmin(subhi[d], boundhi[d]) - boundlo[d]);
+    noffsetlo[d] = (int)floor(Llo / delta);
+    noffsethi[d] = (int)ceil(Lhi / delta);
+  }
+  nsub[0] = noffsethi[0] - noffsetlo[0] > 0 ? noffsethi[0] - noffsetlo[0] : 0;
+  nsub[1] = dim >= 2 ? (noffsethi[1] - noffsetlo[1] > 0 ? noffsethi[1] - noffsetlo[1] : 0) : 1;
+  nsub[2] = dim >= 3 ? (noffsethi[2] - noffsetlo[2] > 0 ? noffsethi[2] - noffsetlo[2] : 0) : 1;
+  for (int d = 0; d < 3; d++) {                                 /* :1988-2002 */
+    double lo = d < dim ? boundlo[d] : 0.0, hi = d < dim ? boundhi[d] : 0.0;
+    double shi = d < dim ? subhi[d] : 0.0;
+    while (lo + delta * (noffsetlo[d] + nsub[d] - 0.5) < dmin(shi, hi)) nsub[d]++;
+  }
+  double vol_ = dim == 1 ? delta : (dim == 2 ? delta * delta : delta * delta * delta);
+  double mass_ = mat->rho0 * vol_;                              /* :2019 */
+  int nip = 1;
+  double ip3[3 * 64];
+  memset(ip3, 0, sizeof(ip3));
+  if (ppc1d == 1) {                                             /* :2026-2035 */
+    nip = 1;
+  } else if (ppc1d == 2) {                                      /* :2036-2051 */
+    nip = dim == 1 ? 2 : (dim == 2 ? 4 : 8);
+    double xi = 0.25;
+    const double t[24] = {-xi, -xi, -xi, -xi, xi, -xi, xi, -xi, -xi, xi, xi, -xi,
+                          -xi, -xi, xi,  -xi, xi, xi,  xi, -xi, xi,  xi, xi, xi};
+    memcpy(ip3, t, sizeof(t));
+  } else if (ppc1d >= 4) {                                      /* :2098-2121 */
+    nip = dim == 1 ? ppc1d : (dim == 2 ? ppc1d * ppc1d : ppc1d * ppc1d * ppc1d);
+    if (nip > 64) return KMPM_ERR_ARG;
+    double dd = 1.0 / ppc1d;
+    int l = 0;
+    for (int k = 0; k < ppc1d; k++)
+      for (int i = 0; i < ppc1d; i++)
+        for (int j = 0; j < ppc1d; j++) {
+          ip3[l++] = (i + 0.5) * dd - 0.5;
+          ip3[l++] = (j + 0.5) * dd - 0.5;
+          ip3[l++] = (k + 0.5) * dd - 0.5;
+        }
+  } else {
+    return KMPM_ERR_ARG;  /* ppc1d==3 (Bernstein layout) is not on this path */
+  }
+  mass_ /= (double)nip;                                         /* :2126-2127 */
+  vol_ /= (double)nip;
+  int64_t l = 0;
+  for (int i = 0; i < nsub[0]; i++)
+    for (int j = 0; j < nsub[1]; j++)
+      for (int k = 0; k < nsub[2]; k++)
+        for (int ip = 0; ip < nip; ip++) {                      /* :2146-2183 */
+          double x = boundlo[0] + delta * (noffsetlo[0] + i + 0.5 + ip3[3 * ip + 0]);
+          double y = boundlo[1] + delta * (noffsetlo[1] + j + 0.5 + ip3[3 * ip + 1]);
+          double z = dim == 3 ? boundlo[2] + delta * (noffsetlo[2] + k + 0.5 + ip3[3 * ip + 2]) : 0;
+          /* inside_subdomain (domain.cpp:187-195) && region->inside (== 1) */
+          double sl2 = dim == 3 ? sublo[2] : 0, sh2 = dim == 3 ? subhi[2] : 0;
+          if (x < sublo[0] || x > subhi[0] || y < sublo[1] || y > subhi[1] || z < sl2 || z > sh2)
+            continue;
+          if (region_inside(region, x, y, z) != 1) continue;
+          if (l < capacity) {
+            int64_t q = l;
+            out->x0[3 * q] = out->x[3 * q] = x;
+            out->x0[3 * q + 1] = out->x[3 * q + 1] = y;
+            out->x0[3 * q + 2] = out->x[3 * q + 2] = z;
+            /* :2280-2320 */
+            for (int d = 0; d < 3; d++) out->v[3 * q + d] = 0;
+            for (int e = 0; e < 9; e++) {
+              out->F[9 * q + e] = (e % 4 == 0) ? 1.0 : 0.0;
+              out->sigma[9 * q + e] = 0;
+              out->strain_el[9 * q + e] = 0;
+            }
+            out->mass[q] = mass_;
+            out->vol0[q] = vol_;
+            if (out->vol) out->vol[q] = vol_;
+            if (out->rho0) out->rho0[q] = mat->rho0;
+            if (out->rho) out->rho[q] = mat->rho0;
+            if (out->J) out->J[q] = 1;
+            out->eff_plastic_strain[q] = 0;
+            out->eff_plastic_strain_rate[q] = 0;
+            out->damage[q] = 0;
+            out->damage_init[q] = 0;
+            out->ptag[q] = ptag_offset + q + 1;
+            out->solid[q] = solid_index;
+            out->mat[q] = mat_index;
+            out->mask[q] = 1;
+          }
+          l++;
+        }
+  *np_out = l;
+  return l > capacity ? KMPM_ERR_CAPACITY : KMPM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* simulation state                                                         */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint32_t bit; int set[3]; double val[3], prev[3]; } fixvel_t;
+typedef struct { uint32_t bit; int set[3]; double val[3]; } fixbf_t;
+
+struct orc_sim {
+  kmpm_config c;
+  int dim, S1, S;               /* S1 nodes per dim in the stencil, S per particle */
+  double h, inv_h;
+  int n[3];
+  int64_t nn;
+  /* grid (grid.h:81-96) */
+  double *gx0; int *gntype; double *gmass, *gv, *gvu, *gf, *gmb; uint32_t *gmask;
+  /* materials */
+  int nmat; kmpm_material *mats;
+  /* particles (solid.h:50-97), ordered by solid */
+  int64_t np;
+  int64_t *ptag; int32_t *solid, *mat; uint32_t *pmask;
+  double *x0, *x, *v, *vu, *a, *F, *sig, *eel, *L, *D;
+  double *mass, *vol0, *vol, *rho0, *rho, *J;
+  double *ep, *epdot, *dmg, *dinit;
+  /* neighbour lists: particle-major (neigh_pn) and node-major CSR (neigh_np) */
+  int32_t *nn_pn; int64_t *neigh_pn; double *wf_pn, *wfd_pn;
+  int64_t *np_start; int64_t *neigh_np; double *wf_np, *wfd_np; int64_t npairs_cap;
+  int32_t *cell_ijk;
+  /* time (update.h) */
+  double dt, time, dtCFL; int64_t step;
+  int nfv, nbf; fixvel_t fv[MAXFIX]; fixbf_t bf[MAXFIX];
+  int status; int64_t err_ptag;
+};
+
+static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
+
+orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
+  int64_t n3[3];
+  if (orc_grid_size(c, n3) != KMPM_OK) return NULL;
+  if (c->method != KMPM_METHOD_ULMPM || (c->dimension != 2 && c->dimension != 3)) return NULL;
+  orc_sim *s = (orc_sim *)xcalloc(1, sizeof(orc_sim));
+  s->c = *c;
+  s->dim = c->dimension;
+  s->S1 = c->shape == KMPM_SHAPE_LINEAR ? 2 : 4;
+  s->S = s->dim == 3 ? s->S1 * s->S1 * s->S1 : s->S1 * s->S1;
+  s->h = c->cellsize;
+  s->inv_h = 1.0 / s->h;                                        /* ulmpm.cpp:122 */
+  for (int d = 0; d < 3; d++) s->n[d] = (int)n3[d];
+  s->nn = n3[0] * n3[1] * n3[2];
+  s->gx0 = (double *)xcalloc(3 * s->nn, sizeof(double));
+  s->gntype = (int *)xcalloc(3 * s->nn, sizeof(int));
+  s->gmass = (double *)xcalloc(s->nn, sizeof(double));
+  s->gv = (double *)xcalloc(3 * s->nn, sizeof(double));
+  s->gvu = (double *)xcalloc(3 * s->nn, sizeof(double));
+  s->gf = (double *)xcalloc(3 * s->nn, sizeof(double));
+  s->gmb = (double *)xcalloc(3 * s->nn, sizeof(double));
+  s->gmask = (uint32_t *)xcalloc(s->nn, sizeof(uint32_t));
+  s->np_start = (int64_t *)xcalloc(s->nn + 1, sizeof(int64_t));
+  int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  int64_t l = 0;
+  for (int i = 0; i < nx; i++)                                  /* grid.cpp:218-262 */
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        s->gx0[3 * l] = c->boxlo[0] + (0 + i) * s->h;
+        s->gx0[3 * l + 1] = s->dim >= 2 ? c->boxlo[1] + (0 + j) * s->h : 0;
+        s->gx0[3 * l + 2] = s->dim == 3 ? c->boxlo[2] + (0 + k) * s->h : 0;
+        if (c->shape == KMPM_SHAPE_LINEAR) {
+          s->gntype[3 * l] = s->gntype[3 * l + 1] = s->gntype[3 * l + 2] = 0;
+        } else {
+          s->gntype[3 * l] = imin(2, i) - imin(nx - 1 - i, 2);
+          s->gntype[3 * l + 1] = imin(2, j) - imin(ny - 1 - j, 2);
+          s->gntype[3 * l + 2] = imin(2, k) - imin(nz - 1 - k, 2);
+        }
+        s->gmask[l] = 1;                                        /* grid.cpp:429 */
+        l++;
+      }
+  s->nmat = nmat;
+  s->mats = (kmpm_material *)xcalloc(nmat, sizeof(kmpm_material));
+  memcpy(s->mats, mats, nmat * sizeof(kmpm_material));
+  s->dt = c->dt;
+  s->time = 0;
+  s->step = 0;
+  return s;
+}
+
+static void free_particles(orc_sim *s) {
+  void *ptrs[] = {s->ptag, s->solid, s->mat, s->pmask, s->x0, s->x, s->v, s->vu, s->a,
+                  s->F, s->sig, s->eel, s->L, s->D, s->mass, s->vol0, s->vol, s->rho0,
+                  s->rho, s->J, s->ep, s->epdot, s->dmg, s->dinit, s->nn_pn, s->neigh_pn,
+                  s->wf_pn, s->wfd_pn, s->neigh_np, s->wf_np, s->wfd_np, s->cell_ijk};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+}
+
+void orc_destroy(orc_sim *s) {
+  if (!s) return;
+  free_particles(s);
+  free(s->gx0); free(s->gntype); free(s->gmass); free(s->gv); free(s->gvu); free(s->gf);
+  free(s->gmb); free(s->gmask); free(s->np_start); free(s->mats);
+  free(s);
+}
+
+int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
+  free_particles(s);
+  int64_t np = p->np, S = s->S;
+  for (int64_t i = 1; i < np; i++)
+    if (p->solid[i] < p->solid[i - 1]) return KMPM_ERR_ARG;  /* ordered by solid */
+  s->np = np;
+#define A(f, t, k) s->f = (t *)xcalloc((size_t)(k) * np, sizeof(t))
+  A(ptag, int64_t, 1); A(solid, int32_t, 1); A(mat, int32_t, 1); A(pmask, uint32_t, 1);
+  A(x0, double, 3); A(x, double, 3); A(v, double, 3); A(vu, double, 3); A(a, double, 3);
+  A(F, double, 9); A(sig, double, 9); A(eel, double, 9); A(L, double, 9); A(D, double, 9);
+  A(mass, double, 1); A(vol0, double, 1); A(vol, double, 1); A(rho0, double, 1);
+  A(rho, double, 1); A(J, double, 1); A(ep, double, 1); A(epdot, double, 1);
+  A(dmg, double, 1); A(dinit, double, 1);
+  A(nn_pn, int32_t, 1); A(neigh_pn, int64_t, S); A(wf_pn, double, S); A(wfd_pn, double, 3 * S);
+  A(cell_ijk, int32_t, 3);
+#undef A
+  s->npairs_cap = np * S;
+  s->neigh_np = (int64_t *)xcalloc(s->npairs_cap, sizeof(int64_t));
+  s->wf_np = (double *)xcalloc(s->npairs_cap, sizeof(double));
+  s->wfd_np = (double *)xcalloc(3 * s->npairs_cap, sizeof(double));
+  for (int64_t i = 0; i < np; i++) {
+    int m = p->mat[i];
+    if (m < 0 || m >= s->nmat) return KMPM_ERR_ARG;
+    s->ptag[i] = p->ptag[i];
+    s->solid[i] = p->solid[i];
+    s->mat[i] = m;
+    s->pmask[i] = p->mask ? p->mask[i] : 1;
+    for (int d = 0; d < 3; d++) {
+      s->x0[3 * i + d] = p->x0[3 * i + d];
+      s->x[3 * i + d] = p->x[3 * i + d];
+      s->v[3 * i + d] = p->v[3 * i + d];
+    }
+    for (int e = 0; e < 9; e++) {
+      s->F[9 * i + e] = p->F[9 * i + e];
+      s->sig[9 * i + e] = p->sigma[9 * i + e];
+      s->eel[9 * i + e] = p->strain_el[9 * i + e];
+    }
+    s->mass[i] = p->mass[i];
+    s->vol0[i] = p->vol0[i];
+    s->vol[i] = p->vol ? p->vol[i] : p->vol0[i];
+    s->rho0[i] = s->mats[m].rho0;
+    s->rho[i] = p->rho ? p->rho[i] : s->mats[m].rho0;
+    s->J[i] = p->J ? p->J[i] : 1.0;
+    s->ep[i] = p->eff_plastic_strain[i];
+    s->epdot[i] = p->eff_plastic_strain_rate[i];
+    s->dmg[i] = p->damage[i];
+    s->dinit[i] = p->damage_init[i];
+  }
+  return KMPM_OK;
+}
+
+int orc_get_particles(orc_sim *s, kmpm_particles *p) {
+  if (p->np < s->np) return KMPM_ERR_CAPACITY;
+  for (int64_t i = 0; i < s->np; i++) {
+    if (p->ptag) p->ptag[i] = s->ptag[i];
+    if (p->solid) p->solid[i] = s->solid[i];
+    if (p->mat) p->mat[i] = s->mat[i];
+    if (p->mask) p->mask[i] = s->pmask[i];
+    for (int d = 0; d < 3; d++) {
+      if (p->x0) p->x0[3 * i + d] = s->x0[3 * i + d];
+      if (p->x) p->x[3 * i + d] = s->x[3 * i + d];
+      if (p->v) p->v[3 * i + d] = s->v[3 * i + d];
+    }
+    for (int e = 0; e < 9; e++) {
+      if (p->F) p->F[9 * i + e] = s->F[9 * i + e];
+      if (p->sigma) p->sigma[9 * i + e] = s->sig[9 * i + e];
+      if (p->strain_el) p->strain_el[9 * i + e] = s->eel[9 * i + e];
+    }
+    if (p->mass) p->mass[i] = s->mass[i];
+    if (p->vol0) p->vol0[i] = s->vol0[i];
+    if (p->vol) p->vol[i] = s->vol[i];
+    if (p->rho0) p->rho0[i] = s->rho0[i];
+    if (p->rho) p->rho[i] = s->rho[i];
+    if (p->J) p->J[i] = s->J[i];
+    if (p->eff_plastic_strain) p->eff_plastic_strain[i] = s->ep[i];
+    if (p->eff_plastic_strain_rate) p->eff_plastic_strain_rate[i] = s->epdot[i];
+    if (p->damage) p->damage[i] = s->dmg[i];
+    if (p->damage_init) p->damage_init[i] = s->dinit[i];
+  }
+  return KMPM_OK;
+}
+
+int orc_set_node_mask(orc_sim *s, const uint32_t *mask, int64_t nn) {
+  if (nn != s->nn) return KMPM_ERR_ARG;
+  memcpy(s->gmask, mask, nn * sizeof(uint32_t));
+  return KMPM_OK;
+}
+
+int orc_fix_velocity_nodes(orc_sim *s, uint32_t bit, const int set[3], const double val[3],
+                           const double prev[3]) {
+  if (s->nfv >= MAXFIX) return KMPM_ERR_ARG;
+  fixvel_t *f = &s->fv[s->nfv++];
+  f->bit = bit;
+  for (int d = 0; d < 3; d++) {
+    f->set[d] = set[d] && d < s->dim;   /* fix_velocity_nodes.cpp:81,98 */
+    f->val[d] = val[d];
+    f->prev[d] = prev[d];
+  }
+  return KMPM_OK;
+}
+
+int orc_fix_body_force(orc_sim *s, uint32_t bit, const int set[3], const double val[3]) {
+  if (s->nbf >= MAXFIX) return KMPM_ERR_ARG;
+  fixbf_t *f = &s->bf[s->nbf++];
+  f->bit = bit;
+  for (int d = 0; d < 3; d++) { f->set[d] = set[d]; f->val[d] = val[d]; }
+  return KMPM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* weights: ULMPM::compute_grid_weight_functions_and_gradients,              */
+/* ulmpm.cpp:88-337 (one rank: map_ntag is the identity on [0,nnodes))       */
+/* ------------------------------------------------------------------------ */
+int orc_phase_weights(orc_sim *s) {
+  const double *lo = s->c.boxlo;
+  double inv_h = s->inv_h;
+  int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  int64_t nnodes = s->nn;
+  int lin = s->c.shape == KMPM_SHAPE_LINEAR;
+  int S1 = s->S1;
+  int64_t cand[64];
+  (void)nx;
+  for (int64_t in = 0; in <= s->nn; in++) s->np_start[in] = 0;
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    const double *xp = &s->x[3 * ip];
+    int i0, j0, k0;
+    if (lin) {                                                  /* :161-165 */
+      i0 = (int)((xp[0] - lo[0]) * inv_h);
+      j0 = (int)((xp[1] - lo[1]) * inv_h);
+      k0 = (int)((xp[2] - lo[2]) * inv_h);
+    } else {                                                    /* :204-208 */
+      i0 = (int)((xp[0] - lo[0]) * inv_h - 1);
+      j0 = (int)((xp[1] - lo[1]) * inv_h - 1);
+      k0 = (int)((xp[2] - lo[2]) * inv_h - 1);
+    }
+    s->cell_ijk[3 * ip] = i0;
+    s->cell_ijk[3 * ip + 1] = j0;
+    s->cell_ijk[3 * ip + 2] = s->dim == 3 ? k0 : 0;
+    int nc = 0;
+    for (int i = i0; i < i0 + S1; i++) {                        /* :167-242 */
+      for (int j = j0; j < j0 + S1; j++) {
+        if (nz > 1) {
+          for (int k = k0; k < k0 + S1; k++) {
+            int64_t tag = (int64_t)nz * ny * i + (int64_t)nz * j + k;
+            if (tag < nnodes && tag >= 0) cand[nc++] = tag;  /* negative tags: see note */
+          }
+        } else {
+          int64_t tag = (int64_t)ny * i + j;
+          if (tag < nnodes && tag >= 0) cand[nc++] = tag;
+        }
+      }
+    }
+    /* Note: the reference indexes map_ntag with any tag < nnodes; a negative
+     * tag (a particle exactly on boxlo with cubic splines) would read before
+     * the vector.  The restatement drops such tags instead. */
+    int cnt = 0;
+    for (int c = 0; c < nc; c++) {                              /* :244-316 */
+      int64_t in = cand[c];
+      double r[3], sv[3], sd[3] = {0, 0, 0}, wf;
+      for (int d = 0; d < 3; d++) r[d] = (xp[d] - s->gx0[3 * in + d]) * inv_h;
+      const int *nt = &s->gntype[3 * in];
+      sv[0] = orc_basis(s->c.shape, 0, r[0], nt[0], inv_h);
+      wf = sv[0];
+      if (wf != 0) {
+        if (s->dim >= 2) { sv[1] = orc_basis(s->c.shape, 0, r[1], nt[1], inv_h); wf *= sv[1]; }
+        else sv[1] = 1;
+        if (s->dim == 3 && wf != 0) { sv[2] = orc_basis(s->c.shape, 0, r[2], nt[2], inv_h); wf *= sv[2]; }
+        else sv[2] = 1;
+      }
+      if (wf == 0) continue;
+      sd[0] = orc_basis(s->c.shape, 1, r[0], nt[0], inv_h);
+      if (s->dim >= 2) sd[1] = orc_basis(s->c.shape, 1, r[1], nt[1], inv_h);
+      if (s->dim == 3) sd[2] = orc_basis(s->c.shape, 1, r[2], nt[2], inv_h);
+      int64_t q = ip * s->S + cnt;
+      s->neigh_pn[q] = in;
+      s->wf_pn[q] = wf;
+      if (s->dim == 3) {
+        s->wfd_pn[3 * q] = sd[0] * sv[1] * sv[2];
+        s->wfd_pn[3 * q + 1] = sv[0] * sd[1] * sv[2];
+        s->wfd_pn[3 * q + 2] = sv[0] * sv[1] * sd[2];
+      } else {
+        s->wfd_pn[3 * q] = sd[0] * sv[1];
+        s->wfd_pn[3 * q + 1] = sv[0] * sd[1];
+        s->wfd_pn[3 * q + 2] = 0;
+      }
+      s->np_start[in + 1]++;
+      cnt++;
+    }
+    s->nn_pn[ip] = cnt;
+  }
+  /* node-major lists (neigh_np) in ascending particle order, as the
+   * push_back at :269-275 builds them */
+  for (int64_t in = 0; in < s->nn; in++) s->np_start[in + 1] += s->np_start[in];
+  int64_t *fill = (int64_t *)xcalloc(s->nn, sizeof(int64_t));
+  for (int64_t ip = 0; ip < s->np; ip++)
+    for (int c = 0; c < s->nn_pn[ip]; c++) {
+      int64_t q = ip * s->S + c, in = s->neigh_pn[q];
+      int64_t o = s->np_start[in] + fill[in]++;
+      s->neigh_np[o] = ip;
+      s->wf_np[o] = s->wf_pn[q];
+      for (int d = 0; d < 3; d++) s->wfd_np[3 * o + d] = s->wfd_pn[3 * q + d];
+    }
+  free(fill);
+  return KMPM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* P2G: ULMPM::particles_to_grid ulmpm.cpp:339-378 ->                        */
+/*   Solid::compute_mass_nodes solid.cpp:317-335,                            */
+/*   Solid::compute_velocity_nodes :337-390,                                 */
+/*   Solid::compute_external_and_internal_forces_nodes_UL :482-522           */
+/* then Modify::post_particles_to_grid -> FixBodyforce fix_body_force.cpp:106-179 */
+/* ------------------------------------------------------------------------ */
+int orc_phase_p2g(orc_sim *s) {
+  int nsolid_max = 0;
+  for (int64_t ip = 0; ip < s->np; ip++)
+    if (s->solid[ip] + 1 > nsolid_max) nsolid_max = s->solid[ip] + 1;
+  for (int64_t in = 0; in < s->nn; in++) {
+    int64_t b = s->np_start[in], e = s->np_start[in + 1];
+    /* mass: solid 0 resets, every solid accumulates in ascending ip */
+    s->gmass[in] = 0;
+    for (int64_t o = b; o < e; o++) {
+      int64_t ip = s->neigh_np[o];
+      s->gmass[in] += s->wf_np[o] * s->mass[ip];
+    }
+  }
+  for (int64_t in = 0; in < s->nn; in++) {
+    int64_t b = s->np_start[in], e = s->np_start[in + 1];
+    double *gv = &s->gv[3 * in], *gf = &s->gf[3 * in], *gmb = &s->gmb[3 * in];
+    gv[0] = gv[1] = gv[2] = 0;
+    gf[0] = gf[1] = gf[2] = 0;
+    gmb[0] = gmb[1] = gmb[2] = 0;
+    /* velocity, per solid (:357-378): v += (sum_solid (w m) v) / mass */
+    int64_t o = b;
+    while (o < e) {
+      int sol = s->solid[s->neigh_np[o]];
+      double vt[3] = {0, 0, 0};
+      int64_t o2 = o;
+      for (; o2 < e && s->solid[s->neigh_np[o2]] == sol; o2++) {
+        int64_t ip = s->neigh_np[o2];
+        double wm = s->wf_np[o2] * s->mass[ip];
+        for (int d = 0; d < 3; d++) vt[d] += wm * s->v[3 * ip + d];
+      }
+      if (s->gmass[in] > 0) {
+        for (int d = 0; d < 3; d++) vt[d] /= s->gmass[in];
+        for (int d = 0; d < 3; d++) gv[d] += vt[d];
+      }
+      o = o2;
+    }
+    /* internal force (:510-515): f -= vol * (sigma * wfd) */
+    for (o = b; o < e; o++) {
+      int64_t ip = s->neigh_np[o];
+      const double *sg = &s->sig[9 * ip], *wd = &s->wfd_np[3 * o];
+      for (int d = 0; d < 3; d++) {
+        double t = sg[3 * d] * wd[0] + sg[3 * d + 1] * wd[1] + sg[3 * d + 2] * wd[2];
+        gf[d] -= s->vol[ip] * t;
+      }
+      /* mb += wf * mbp with mbp == 0 (ULMPM::reset, ulmpm.cpp:553-563) */
+    }
+  }
+  /* body force fix (fix_body_force.cpp:117-146) */
+  for (int k = 0; k < s->nbf; k++) {
+    fixbf_t *f = &s->bf[k];
+    for (int64_t in = 0; in < s->nn; in++) {
+      if (s->gmass[in] > 0 && (s->gmask[in] & f->bit)) {
+        double fv[3] = {0, 0, 0};
+        for (int d = 0; d < 3; d++) if (f->set[d]) fv[d] = f->val[d];
+        for (int d = 0; d < 3; d++) fv[d] *= s->gmass[in];
+        for (int d = 0; d < 3; d++) s->gmb[3 * in + d] += fv[d];
+      }
+    }
+  }
+  (void)nsolid_max;
+  return KMPM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid update: Grid::update_grid_velocities grid.cpp:448-466, then          */
+/* FixVelocityNodes::post_update_grid_state fix_velocity_nodes.cpp:130-225   */
+/* ------------------------------------------------------------------------ */
+int orc_phase_grid_update(orc_sim *s) {
+  double dt = s->dt;
+  for (int64_t in = 0; in < s->nn; in++) {
+    double m = s->gmass[in];
+    for (int d = 0; d < 3; d++) {
+      if (m != 0)
+        s->gvu[3 * in + d] = s->gv[3 * in + d] + dt * (s->gf[3 * in + d] + s->gmb[3 * in + d]) / m;
+      else
+        s->gvu[3 * in + d] = s->gv[3 * in + d];
+    }
+  }
+  for (int k = 0; k < s->nfv; k++) {
+    fixvel_t *f = &s->fv[k];
+    for (int64_t in = 0; in < s->nn; in++) {
+      if (s->gmask[in] & f->bit) {
+        for (int d = 0; d < 3; d++)
+          if (f->set[d]) { s->gvu[3 * in + d] = f->val[d]; s->gv[3 * in + d] = f->prev[d]; }
+      }
+    }
+  }
+  return KMPM_OK;
+}
+
+/* MUSL re-map: ULMPM::velocities_to_grid ulmpm.cpp:476-496 ->
+ * compute_velocity_nodes (solid.cpp:337-390), then
+ * FixVelocityNodes::post_velocities_to_grid (fix_velocity_nodes.cpp:227-271) */
+static void phase_velocities_to_grid(orc_sim *s) {
+  for (int64_t in = 0; in < s->nn; in++) {
+    int64_t b = s->np_start[in], e = s->np_start[in + 1];
+    double *gv = &s->gv[3 * in];
+    gv[0] = gv[1] = gv[2] = 0;
+    int64_t o = b;
+    while (o < e) {
+      int sol = s->solid[s->neigh_np[o]];
+      double vt[3] = {0, 0, 0};
+      int64_t o2 = o;
+      for (; o2 < e && s->solid[s->neigh_np[o2]] == sol; o2++) {
+        int64_t ip = s->neigh_np[o2];
+        double wm = s->wf_np[o2] * s->mass[ip];
+        for (int d = 0; d < 3; d++) vt[d] += wm * s->v[3 * ip + d];
+      }
+      if (s->gmass[in] > 0) {
+        for (int d = 0; d < 3; d++) vt[d] /= s->gmass[in];
+        for (int d = 0; d < 3; d++) gv[d] += vt[d];
+      }
+      o = o2;
+    }
+  }
+  for (int k = 0; k < s->nfv; k++) {
+    fixvel_t *f = &s->fv[k];
+    for (int64_t in = 0; in < s->nn; in++)
+      if (s->gmask[in] & f->bit)
+        for (int d = 0; d < 3; d++) if (f->set[d]) s->gv[3 * in + d] = f->val[d];
+  }
+}
+
+/* Solid::compute_rate_deformation_gradient_UL solid.cpp:860-936 */
+static void phase_rate_of_deformation(orc_sim *s, int doublemapping) {
+  const double *vn = doublemapping ? s->gv : s->gvu;
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    double *L = &s->L[9 * ip];
+    for (int e = 0; e < 9; e++) L[e] = 0;
+    for (int c = 0; c < s->nn_pn[ip]; c++) {
+      int64_t q = ip * s->S + c, in = s->neigh_pn[q];
+      const double *wd = &s->wfd_pn[3 * q], *v = &vn[3 * in];
+      if (s->dim == 2) {
+        L[0] += v[0] * wd[0];
+        L[1] += v[0] * wd[1];
+        L[3] += v[1] * wd[0];
+        L[4] += v[1] * wd[1];
+      } else {
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) L[3 * a + b] += v[a] * wd[b];
+      }
+    }
+  }
+}
+
+/* Solid::compute_particle_accelerations_velocities_and_positions
+ * solid.cpp:576-635; Solid::update_particle_velocities :786-790 */
+static int phase_g2p(orc_sim *s) {
+  double inv_dt = 1.0 / s->dt;
+  double alpha = s->c.pic_flip;
+  const double *lo = s->c.boxlo, *hi = s->c.boxhi;
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    double *vu = &s->vu[3 * ip], *a = &s->a[3 * ip];
+    for (int d = 0; d < 3; d++) vu[d] = a[d] = 0;
+    for (int c = 0; c < s->nn_pn[ip]; c++) {
+      int64_t q = ip * s->S + c, in = s->neigh_pn[q];
+      double w = s->wf_pn[q];
+      for (int d = 0; d < 3; d++) vu[d] += w * s->gvu[3 * in + d];
+      for (int d = 0; d < 3; d++) a[d] += w * (s->gvu[3 * in + d] - s->gv[3 * in + d]);
+    }
+    for (int d = 0; d < 3; d++) a[d] *= inv_dt;
+    for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * vu[d];
+    const double *x = &s->x[3 * ip];
+    if (!(x[0] >= lo[0] && x[0] <= hi[0] && x[1] >= lo[1] && x[1] <= hi[1] &&
+          x[2] >= (s->dim == 3 ? lo[2] : 0) && x[2] <= (s->dim == 3 ? hi[2] : 0))) {
+      if (s->status == KMPM_OK) { s->status = KMPM_ERR_PARTICLE_LEFT_BOX; s->err_ptag = s->ptag[ip]; }
+    }
+  }
+  for (int64_t ip = 0; ip < s->np; ip++)
+    for (int d = 0; d < 3; d++)
+      s->v[3 * ip + d] = (1 - alpha) * s->vu[3 * ip + d] +
+                         alpha * (s->v[3 * ip + d] + s->dt * s->a[3 * ip + d]);
+  return s->status;
+}
+
+/* eye-based helpers mirroring Eigen expressions */
+static double trace3(const double *M) { return M[0] + M[4] + M[8]; }
+/* MPM_Math::Deviator mpm_math.h:28-33: M - eye*(trace/3) */
+static void deviator(const double *M, double *out) {
+  double t = trace3(M) / 3.0;
+  for (int e = 0; e < 9; e++) out[e] = M[e] - ((e % 4 == 0) ? 1.0 : 0.0) * t;
+}
+/* Frobenius norm, row-major sequential sum */
+static double norm9(const double *M) {
+  double s = 0;
+  for (int e = 0; e < 9; e++) s += M[e] * M[e];
+  return sqrt(s);
+}
+/* 3x3 determinant, cofactor expansion along the first row */
+static double det3(const double *m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+/* StrengthJohnsonCook::update_deviatoric_stress strength_jc.cpp:100-184 (T = 0) */
+void orc_jc_strength(const kmpm_material *mt, const double sigma[9], const double D[9],
+                     double dt, double eps_p, double epsdot, double damage,
+                     double out[9], double *dps) {
+  const double *p = mt->strength_p;
+  double G_ = p[0], A = p[1], B = p[2], n = p[3], epsdot0 = p[4], C = p[5], m = p[6],
+         Tr = p[7], Tm = p[8];
+  double Tmr = Tm - Tr, T = 0;
+  if (damage >= 1.0) {
+    for (int e = 0; e < 9; e++) out[e] = 0;
+    return;                          /* *dps left untouched: caller holds 0 */
+  }
+  double epsdot_ratio = epsdot / epsdot0;
+  epsdot_ratio = dmax(epsdot_ratio, 1.0);
+  double ys;
+  if (eps_p < 1.0e-10) ys = A;
+  else ys = A + B * pow(eps_p, n);
+  if (C != 0) ys *= pow(1.0 + epsdot_ratio, C);
+  if (T < Tm) {
+    if (m != 0 && T >= Tr) ys *= 1.0 - pow((T - Tr) / Tmr, m);
+  } else ys = 0;
+  double Gd = G_;
+  if (damage > 0) { Gd *= (1 - damage); ys *= (1 - damage); }
+  double c = dt * 2.0 * Gd, trial[9], dev[9];
+  for (int e = 0; e < 9; e++) trial[e] = sigma[e] + c * D[e];
+  deviator(trial, dev);
+  double J2 = SQRT_3_OVER_2 * norm9(dev);
+  for (int e = 0; e < 9; e++) out[e] = dev[e];
+  if (J2 < ys) {
+    *dps = 0.0;
+  } else {
+    *dps = (J2 - ys) / (3.0 * Gd);
+    for (int e = 0; e < 9; e++) out[e] *= (ys / J2);
+  }
+}
+
+/* EOSShock::compute_pressure eos_shock.cpp:99-133 (T = 0) */
+double orc_shock_pressure(const kmpm_material *mt, double J, double rho, double damage,
+                          const double D[9], double cellsize) {
+  const double *p = mt->eos_p;
+  double rho0_ = p[0], c0 = p[2], S = p[3], Gamma = p[4], cv = p[5], Tr = p[6], Q1 = p[7],
+         Q2 = p[8];
+  double alpha = cv * rho0_, e0 = 0, T = 0, e;
+  double mu = rho / rho0_ - 1.0;
+  double pH = rho0_ * square(c0) * mu * (1.0 + mu) / square(1.0 - (S - 1.0) * mu);
+  if (T > Tr) e = alpha * (T - Tr);
+  else e = 0;
+  double pFinal = pH + Gamma * (e - e0);
+  if (damage > 0.0) {
+    if (pFinal < 0.0) {
+      if (damage >= 1.0) pFinal = 0;
+      else pFinal *= 1.0 - damage;
+    }
+  }
+  if (Q1 != 0 || Q2 != 0) {
+    double tr_eps = trace3(D);
+    if (tr_eps < 0) {
+      double q = rho * cellsize * (Q1 * cellsize * tr_eps * tr_eps - Q2 * c0 * sqrt(J) * tr_eps);
+      pFinal += q;
+    }
+  }
+  return pFinal;
+}
+
+/* DamageJohnsonCook::compute_damage damage_jc.cpp:91-143 (T = 0) */
+void orc_jc_damage(const kmpm_material *mt, double *damage_init, double *damage, double pH,
+                   const double sdev[9], double epsdot, double dps) {
+  const double *p = mt->damage_p;
+  double d1 = p[0], d2 = p[1], d3 = p[2], d4 = p[3], d5 = p[4], epsdot0 = p[5], Tr = p[6],
+         Tm = p[7];
+  double Tmr = Tm - Tr, T = 0;
+  if (dps == 0 && *damage >= 1.0) return;
+  double vm = SQRT_3_OVER_2 * norm9(sdev);
+  double triax = 0.0;
+  if (pH != 0.0 && vm != 0.0) triax = -pH / (vm + 0.001 * fabs(pH));
+  if (triax <= -3) { *damage_init = 0; return; }
+  double jc = d1 + d2 * exp(d3 * triax);
+  if (d4 > 0.0) {
+    if (epsdot > epsdot0) {
+      double r = epsdot / epsdot0;
+      jc *= (1.0 + d4 * log(r));
+    }
+  }
+  if (d5 > 0.0 && T >= Tr) jc *= 1 + d5 * (T - Tr) / Tmr;
+  *damage_init += dps / jc;
+  if (*damage_init >= 1.0) *damage = dmin((*damage_init - 1.0) * 10, 1.0);
+}
+
+/* Solid::update_deformation_gradient solid.cpp:1155-1244 (UL),
+ * Solid::update_stress solid.cpp:1246-1438 */
+static int phase_F_and_stress(orc_sim *s) {
+  double dt = s->dt;
+  double max_c = 0;
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    double *F = &s->F[9 * ip], *L = &s->L[9 * ip], *D = &s->D[9 * ip];
+    double M[9], Fn[9];
+    for (int e = 0; e < 9; e++) M[e] = ((e % 4 == 0) ? 1.0 : 0.0) + dt * L[e];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        Fn[3 * i + j] = M[3 * i] * F[j] + M[3 * i + 1] * F[3 + j] + M[3 * i + 2] * F[6 + j];
+    memcpy(F, Fn, sizeof(Fn));
+    s->J[ip] = det3(F);
+    s->vol[ip] = s->J[ip] * s->vol0[ip];
+    if (s->J[ip] <= 0.0 && s->dmg[ip] < 1.0) {
+      if (s->status == KMPM_OK) { s->status = KMPM_ERR_JACOBIAN; s->err_ptag = s->ptag[ip]; }
+    }
+    s->rho[ip] = s->rho0[ip] / s->J[ip];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) D[3 * i + j] = 0.5 * (L[3 * i + j] + L[3 * j + i]);
+  }
+  double tav_h = 1000 * s->h;
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    const kmpm_material *mt = &s->mats[s->mat[ip]];
+    double *sg = &s->sig[9 * ip], *D = &s->D[9 * ip], *eel = &s->eel[9 * ip];
+    if (mt->type == KMPM_MAT_LINEAR) {                         /* :1268-1280 */
+      double si[9];
+      for (int e = 0; e < 9; e++) si[e] = dt * D[e];
+      for (int e = 0; e < 9; e++) eel[e] += si[e];
+      double tr = trace3(si);
+      for (int e = 0; e < 9; e++)
+        sg[e] += 2 * mt->G * si[e] + mt->lambda * tr * ((e % 4 == 0) ? 1.0 : 0.0);
+    } else {                                                    /* :1292-1374 */
+      double pH = 0, dps = 0, sdev[9];
+      /* EOS */
+      if (mt->eos == KMPM_EOS_SHOCK) {
+        pH = orc_shock_pressure(mt, s->J[ip], s->rho[ip], s->dmg[ip], D, s->h);
+      } else if (mt->eos == KMPM_EOS_FLUID) {                   /* eos_fluid.cpp:68-73 */
+        double mu = s->rho[ip] / mt->eos_p[0];
+        pH = mt->eos_p[1] * (pow(mu, mt->eos_p[2]) - 1.0);
+      } else {                                                  /* eos_linear.cpp:71-74 */
+        pH = mt->eos_p[1] * (1 - s->J[ip]) * (1 - s->dmg[ip]);
+      }
+      /* strength */
+      if (mt->strength == KMPM_STRENGTH_JOHNSON_COOK) {
+        orc_jc_strength(mt, sg, D, dt, s->ep[ip], s->epdot[ip], s->dmg[ip], sdev, &dps);
+      } else if (mt->strength == KMPM_STRENGTH_FLUID) {         /* strength_fluid.cpp:46-58 */
+        double dv[9];
+        deviator(D, dv);
+        for (int e = 0; e < 9; e++) sdev[e] = 2.0 * mt->strength_p[0] * dv[e];
+      } else {                                                  /* strength_linear.cpp:59-71 */
+        double dv[9], ds[9];
+        deviator(D, dv);
+        deviator(sg, ds);
+        double c2 = 2.0 * mt->strength_p[0] * (1 - s->dmg[ip]);
+        for (int e = 0; e < 9; e++) sdev[e] = ds[e] + dt * (c2 * dv[e]);
+      }
+      s->ep[ip] += dps;                                         /* :1319 */
+      double tav = tav_h / mt->signal_velocity;                 /* :1324-1329 */
+      s->epdot[ip] -= s->epdot[ip] * dt / tav;
+      s->epdot[ip] += dps / tav;
+      s->epdot[ip] = dmax(0.0, s->epdot[ip]);
+      if (mt->damage == KMPM_DAMAGE_JOHNSON_COOK)               /* :1331-1341 */
+        orc_jc_damage(mt, &s->dinit[ip], &s->dmg[ip], pH, sdev, s->epdot[ip], dps);
+      double dm = s->dmg[ip];
+      if (dm == 0 || pH >= 0) {                                 /* :1353-1356 */
+        for (int e = 0; e < 9; e++) sg[e] = -pH * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e];
+      } else {
+        for (int e = 0; e < 9; e++)
+          sg[e] = -pH * (1.0 - dm) * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e];
+      }
+      double t3 = (dt * trace3(D) + trace3(eel)) / 3.0;         /* :1358-1366 */
+      double Gd = dm > 1e-10 ? mt->G * (1 - dm) : mt->G;
+      for (int e = 0; e < 9; e++) eel[e] = t3 * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e] / Gd;
+    }
+  }
+  /* CFL (:1376-1437), UL: min_h_ratio = 1 */
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    if (s->dmg[ip] >= 1.0) continue;
+    const kmpm_material *mt = &s->mats[s->mat[ip]];
+    const double *v = &s->v[3 * ip];
+    max_c = dmax(max_c, sqrt((mt->K + FOUR_THIRD * mt->G) / s->rho[ip]) +
+                            dmax(dmax(fabs(v[0]), fabs(v[1])), fabs(v[2])));
+    if (isnan(max_c)) {
+      if (s->status == KMPM_OK) { s->status = KMPM_ERR_WAVE_SPEED; s->err_ptag = s->ptag[ip]; }
+    }
+  }
+  s->dtCFL = dmin(s->dtCFL, s->h * 1.0 / max_c);
+  if (isnan(s->dtCFL) && s->status == KMPM_OK) s->status = KMPM_ERR_DTCFL;
+  return s->status;
+}
+
+/* Update::update_time update.cpp:201-206; ULMPM::adjust_dt ulmpm.cpp:525-551 */
+static int phase_time_and_dt(orc_sim *s) {
+  s->time += s->dt;
+  if (s->c.dt_constant) return s->status;
+  if (s->dtCFL == 0 && s->status == KMPM_OK) s->status = KMPM_ERR_DTCFL;
+  s->dt = s->dtCFL * s->c.dt_factor;
+  return s->status;
+}
+
+/* USL::run usl.cpp:43-93 and MUSL::run musl.cpp:43-94, one step each */
+int orc_step(orc_sim *s, int64_t nsteps) {
+  for (int64_t k = 0; k < nsteps && s->status == KMPM_OK; k++) {
+    s->step++;                                                  /* update_timestep */
+    orc_phase_weights(s);
+    s->dtCFL = 1.0e22;                                          /* reset */
+    orc_phase_p2g(s);
+    orc_phase_grid_update(s);
+    if (s->c.scheme == KMPM_SCHEME_USL) {
+      phase_rate_of_deformation(s, 0);
+      if (phase_g2p(s)) break;
+    } else {
+      if (phase_g2p(s)) break;
+      phase_velocities_to_grid(s);
+      phase_rate_of_deformation(s, 1);
+    }
+    if (phase_F_and_stress(s)) break;
+    phase_time_and_dt(s);
+  }
+  return s->status;
+}
+
+int orc_get_time(orc_sim *s, double *dt, double *time, int64_t *step) {
+  if (dt) *dt = s->dt;
+  if (time) *time = s->time;
+  if (step) *step = s->step;
+  return KMPM_OK;
+}
+
+/* FixKineticEnergy::final_integrate fix_kinetic_energy.cpp:78-118,
+ * FixStrainEnergy::final_integrate fix_strain_energy.cpp:79-133 */
+int orc_energy(orc_sim *s, uint32_t bit, double *ek, double *es) {
+  double Ek = 0, Es = 0;
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    if (!(s->pmask[ip] & bit)) continue;
+    const double *v = &s->v[3 * ip];
+    Ek += 0.5 * s->mass[ip] * square(sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    const double *sg = &s->sig[9 * ip], *e = &s->eel[9 * ip];
+    double t = sg[0] * e[0];
+    for (int q = 1; q < 9; q++) t += sg[q] * e[q];
+    Es += 0.5 * s->vol[ip] * t;
+  }
+  *ek = Ek;
+  *es = Es;
+  return KMPM_OK;
+}
+
+int orc_get_grid(orc_sim *s, double *mass, double *v, double *vu, double *f, int64_t nn) {
+  if (nn != s->nn) return KMPM_ERR_ARG;
+  if (mass) memcpy(mass, s->gmass, nn * sizeof(double));
+  if (v) memcpy(v, s->gv, 3 * nn * sizeof(double));
+  if (vu) memcpy(vu, s->gvu, 3 * nn * sizeof(double));
+  if (f) memcpy(f, s->gf, 3 * nn * sizeof(double));
+  return KMPM_OK;
+}
+
+int orc_get_neighbours(orc_sim *s, int32_t *numneigh, int64_t *tags, double *w, double *wd,
+                       int64_t np, int S) {
+  if (np != s->np || S < s->S) return KMPM_ERR_ARG;
+  for (int64_t ip = 0; ip < np; ip++) {
+    numneigh[ip] = s->nn_pn[ip];
+    for (int c = 0; c < S; c++) {
+      int64_t q = ip * s->S + c, o = ip * S + c;
+      if (c < s->nn_pn[ip]) {
+        tags[o] = s->neigh_pn[q];
+        w[o] = s->wf_pn[q];
+        for (int d = 0; d < 3; d++) wd[3 * o + d] = s->wfd_pn[3 * q + d];
+      } else {
+        tags[o] = -1;
+        w[o] = 0;
+        for (int d = 0; d < 3; d++) wd[3 * o + d] = 0;
+      }
+    }
+  }
+  return KMPM_OK;
+}
+
+int orc_cell_index(orc_sim *s, int32_t *ijk, int64_t np) {
+  if (np != s->np) return KMPM_ERR_ARG;
+  memcpy(ijk, s->cell_ijk, 3 * np * sizeof(int32_t));
+  return KMPM_OK;
+}
+
+/* Cell of a particle for the sort: trunc((x - boxlo) * inv_h) per
+ * dimension (the linear-shape base index of ulmpm.cpp:163-165), clamped to
+ * the node range; key is tile-major. */
+static int64_t cell_key(orc_sim *s, int64_t ip, const int T[3]) {
+  int c[3] = {0, 0, 0}, nt[3];
+  for (int d = 0; d < s->dim; d++) {
+    double u = (s->x[3 * ip + d] - s->c.boxlo[d]) * s->inv_h;
+    int ci = (u >= 0 && u < 2147483647.0) ? (int)u : (u >= 0 ? s->n[d] - 1 : 0);
+    if (ci > s->n[d] - 1) ci = s->n[d] - 1;
+    c[d] = ci;
+  }
+  int Tz = s->dim == 3 ? T[2] : 1;
+  int Tt[3] = {T[0], T[1], Tz};
+  for (int d = 0; d < 3; d++) nt[d] = (s->n[d] + Tt[d] - 1) / Tt[d];
+  int64_t tile = ((int64_t)(c[0] / Tt[0]) * nt[1] + c[1] / Tt[1]) * nt[2] + c[2] / Tt[2];
+  int64_t local = ((int64_t)(c[0] % Tt[0]) * Tt[1] + c[1] % Tt[1]) * Tt[2] + c[2] % Tt[2];
+  return tile * ((int64_t)Tt[0] * Tt[1] * Tt[2]) + local;
+}
+
+typedef struct { int64_t key, ptag; } kp_t;
+static int kp_cmp(const void *a, const void *b) {
+  const kp_t *x = (const kp_t *)a, *y = (const kp_t *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  if (x->ptag != y->ptag) return x->ptag < y->ptag ? -1 : 1;
+  return 0;
+}
+int orc_sort_permutation(orc_sim *s, const int tile[3], int64_t *sorted, int64_t np) {
+  if (np != s->np) return KMPM_ERR_ARG;
+  kp_t *kp = (kp_t *)xcalloc(np, sizeof(kp_t));
+  for (int64_t ip = 0; ip < np; ip++) { kp[ip].key = cell_key(s, ip, tile); kp[ip].ptag = s->ptag[ip]; }
+  qsort(kp, np, sizeof(kp_t), kp_cmp);
+  for (int64_t i = 0; i < np; i++) sorted[i] = kp[i].ptag;
+  free(kp);
+  return KMPM_OK;
+}
+
+int64_t orc_error_particle(orc_sim *s) { return s->err_ptag; }

@@ -1,0 +1,252 @@
+"""GPU parity of the HIP step against the oracle, through the C ABI.
+
+Bar: particle-cell indices and the sort permutation bit-exact; shape
+function weights and gradients bit-exact; grid and particle fields within
+1e-12 (one P2G) / 1e-10 relative L2 (after N steps); energies within 1e-10.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from karamelo_amd import native as N
+from karamelo_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-10      # relative L2 of particle fields after N steps
+TOL_GRID = 1e-12      # relative max error of grid fields after one P2G
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, float).ravel()
+    b = np.asarray(b, float).ravel()
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _randomize(prob, seed=7, vscale=1.0, sscale=1e6):
+    """Random velocities and symmetric stresses so every P2G term is exercised."""
+    rng = np.random.default_rng(seed)
+    p = prob.particles
+    dim = prob.cfg.dimension
+    p["v"][:, :dim] = rng.normal(scale=vscale, size=(len(p["v"]), dim))
+    s = rng.normal(scale=sscale, size=(len(p["v"]), 3, 3))
+    s = 0.5 * (s + s.transpose(0, 2, 1))
+    if dim == 2:
+        s[:, 2, :2] = 0
+        s[:, :2, 2] = 0
+    p["sigma"][:] = s.reshape(-1, 9)
+    return prob
+
+
+def small_problems():
+    return {
+        "two_disks": lambda: P.two_disks(),
+        "two_disks_cubic": lambda: _cubic_disks(),
+        "tensile_cubic": lambda: P.tensile_bar(h=6e-4),
+        "tensile_linear": lambda: P.tensile_bar(h=6e-4, shape="linear"),
+        "dam_cubic": lambda: P.dam_break(n_cells=16),
+    }
+
+
+def _cubic_disks():
+    prob = P.two_disks()
+    prob.cfg.shape = N.SHAPE_CUBIC_SPLINE
+    return prob
+
+
+@pytest.mark.parametrize("name", list(small_problems()))
+def test_weights_cells_and_sort_bit_exact(name):
+    say("weights/cells/sort: %s" % name)
+    prob = small_problems()[name]()
+    ctx = P.context_for(prob)
+    o = O.oracle_for(prob)
+    ctx.phase("compute_grid_weight_functions_and_gradients")
+    o.phase("weights")
+    ijk_g, sorted_g = ctx.cells()
+    assert np.array_equal(ijk_g, o.cells())
+    assert np.array_equal(sorted_g, o.sort_permutation(ctx.tile_shape()))
+    tag_g, w_g, wd_g = ctx.debug_weights()
+    nn, tag_o, w_o, wd_o = o.neighbours()
+    S = tag_g.shape[1]
+    for p in range(prob.np):
+        keep = tag_g[p] >= 0
+        tg, wg, dg = tag_g[p][keep], w_g[p][keep], wd_g[p][keep]
+        og = np.argsort(tg)
+        k = nn[p]
+        to, wo, do = tag_o[p, :k], w_o[p, :k], wd_o[p, :k]
+        oo = np.argsort(to)
+        assert np.array_equal(tg[og], to[oo]), p
+        assert np.array_equal(wg[og], wo[oo]), p
+        assert np.array_equal(dg[og], do[oo]), p
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", list(small_problems()))
+def test_p2g_and_grid_update(name):
+    say("p2g/grid: %s" % name)
+    prob = _randomize(small_problems()[name]())
+    ctx = P.context_for(prob)
+    o = O.oracle_for(prob)
+    for ph in ("compute_grid_weight_functions_and_gradients", "particles_to_grid",
+               "update_grid_state"):
+        ctx.phase(ph)
+    for ph in ("weights", "p2g", "grid_update"):
+        o.phase(ph)
+    mg, vg, vug = ctx.grid()
+    mo, vo, vuo, fo = o.grid()
+    assert np.abs(mg - mo).max() <= TOL_GRID * np.abs(mo).max()
+    assert np.abs(vg - vo).max() <= TOL_GRID * np.abs(vo).max()
+    assert np.abs(vug - vuo).max() <= TOL_GRID * np.abs(vuo).max()
+    st, _ = ctx.synchronize()
+    assert st == 0
+    ctx.close()
+
+
+FIELDS = ("x", "v", "sigma", "F", "strain_el", "eff_plastic_strain", "damage_init", "damage")
+
+
+def run_and_compare(prob, steps, chunk, tol=TOL_STEP, check_energy=True):
+    ctx = P.context_for(prob)
+    o = O.oracle_for(prob)
+    worst = {}
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        ctx.step(n)
+        assert o.step(n) == 0
+        done += n
+        st, tag = ctx.synchronize()
+        assert st == 0, (st, tag)
+        dtg, tg, sg = ctx.time()
+        dto, to, so = o.time()
+        assert sg == so == done
+        assert abs(dtg - dto) <= tol * abs(dto)
+        assert abs(tg - to) <= tol * abs(to)
+        if check_energy:
+            eg, eo = ctx.energy(), o.energy()
+            for a, b in zip(eg, eo):
+                assert abs(a - b) <= tol * max(abs(b), 1e-300) or abs(a - b) < 1e-300
+        pg, po = ctx.download(), o.download()
+        assert np.array_equal(pg["ptag"], po["ptag"])
+        for f in FIELDS:
+            e = rel_l2(pg[f], po[f])
+            worst[f] = max(worst.get(f, 0.0), e)
+            assert e <= tol, (f, done, e)
+    say("  worst rel L2: " + ", ".join("%s=%.2e" % kv for kv in worst.items()))
+    ctx.close()
+    return worst, po
+
+
+def test_two_disks_3000_steps():
+    say("two disks, 3000 steps")
+    run_and_compare(P.two_disks(), 3000, 500)
+
+
+def test_two_disks_cubic_1000_steps():
+    say("two disks cubic, 1000 steps")
+    run_and_compare(_cubic_disks(), 1000, 250)
+
+
+def test_tensile_bar_usl_cubic_plastic():
+    say("tensile bar USL cubic, 300 steps")
+    prob = P.tensile_bar(h=6e-4, v_pull=40.0)
+    w, po = run_and_compare(prob, 300, 100)
+    assert po["eff_plastic_strain"].max() > 0, "the bar must yield in this run"
+
+
+def test_tensile_bar_musl_linear():
+    say("tensile bar MUSL linear, 200 steps")
+    run_and_compare(P.tensile_bar(h=6e-4, shape="linear", scheme="musl", v_pull=40.0), 200, 100)
+
+
+def test_tensile_bar_damage_onset():
+    """Low failure strain so damage initiates and grows within the run."""
+    say("tensile bar with damage onset")
+    prob = P.tensile_bar(h=6e-4, v_pull=80.0, d_params=(0.002, 0.0, 0.0, 0.0, 0.0))
+    w, po = run_and_compare(prob, 300, 100)
+    assert po["damage"].max() > 0, "damage must initiate in this configuration"
+
+
+def test_dam_break_musl_cubic():
+    say("dam break MUSL cubic, 150 steps")
+    run_and_compare(P.dam_break(n_cells=16), 150, 50)
+
+
+def test_dam_break_usl_linear_jittered():
+    say("dam break USL linear jittered, 150 steps")
+    run_and_compare(P.dam_break(n_cells=16, scheme="usl", shape="linear", jitter=0.1), 150, 50)
+
+
+def test_particle_leaving_box_is_reported():
+    say("error: particle left the box")
+    prob = P.two_disks()
+    prob.particles["v"][5, 0] = -100.0
+    ctx = P.context_for(prob)
+    o = O.oracle_for(prob)
+    ctx.step(20)
+    st, tag = ctx.synchronize()
+    assert o.step(20) == N.ERR_PARTICLE_LEFT_BOX
+    assert st == N.ERR_PARTICLE_LEFT_BOX
+    assert tag == O.load().orc_error_particle(o.h) == prob.particles["ptag"][5]
+    ctx.close()
+
+
+def test_negative_jacobian_is_reported():
+    say("error: J <= 0")
+    prob = P.two_disks()
+    prob.particles["F"][7] = [1, 0, 0, 0, 1, 0, 0, 0, -1]
+    ctx = P.context_for(prob)
+    ctx.step(1)
+    st, tag = ctx.synchronize()
+    assert st == N.ERR_JACOBIAN and tag == prob.particles["ptag"][7]
+    ctx.close()
+
+
+def test_full_size_tensile_bar_properties():
+    """4.27M particles: cell indices of the reference rule, a complete sort
+    permutation ordered by (tile, cell, ptag), and mass and momentum
+    conservation of one P2G."""
+    say("full-size tensile bar properties")
+    prob = _randomize(P.tensile_bar(), vscale=1.0)
+    p = prob.particles
+    ctx = P.context_for(prob)
+    ctx.phase("compute_grid_weight_functions_and_gradients")
+    ijk, sp = ctx.cells()
+    lo = np.array(prob.cfg.boxlo)
+    inv_h = 1.0 / prob.cfg.cellsize
+    ref = ((p["x"] - lo) * inv_h - 1).astype(np.int64)   # ulmpm.cpp:206-208 (truncation)
+    assert np.array_equal(ijk, ref)
+    assert np.array_equal(np.sort(sp), np.sort(p["ptag"]))
+    T = ctx.tile_shape()
+    n = N.grid_size(prob.cfg)
+    c = np.minimum(((p["x"] - lo) * inv_h).astype(np.int64), np.array(n) - 1)
+    nt = [(n[d] + T[d] - 1) // T[d] for d in range(3)]
+    tile = ((c[:, 0] // T[0]) * nt[1] + c[:, 1] // T[1]) * nt[2] + c[:, 2] // T[2]
+    loc = ((c[:, 0] % T[0]) * T[1] + c[:, 1] % T[1]) * T[2] + c[:, 2] % T[2]
+    key = tile * (T[0] * T[1] * T[2]) + loc
+    order = np.lexsort((p["ptag"], key))
+    assert np.array_equal(sp, p["ptag"][order])
+    assert ctx.synchronize()[0] == 0
+    ctx.close()
+    # conservation of one P2G, without the velocity fixes that overwrite nodes
+    free = dataclasses.replace(prob, fixes=[], node_mask=None)
+    ctx = P.context_for(free)
+    for ph in ("compute_grid_weight_functions_and_gradients", "particles_to_grid",
+               "update_grid_state"):
+        ctx.phase(ph)
+    m, v, vu = ctx.grid()
+    assert abs(m.sum() - p["mass"].sum()) <= 1e-12 * p["mass"].sum()
+    mom = (m[:, None] * v).sum(axis=0)
+    mp = (p["mass"][:, None] * p["v"]).sum(axis=0)
+    assert np.abs(mom - mp).max() <= 1e-12 * np.abs(p["mass"][:, None] * p["v"]).sum()
+    assert ctx.synchronize()[0] == 0
+    ctx.close()
