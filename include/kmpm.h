@@ -196,6 +196,30 @@ int kmpm_step(kmpm_ctx *ctx, int64_t nsteps);
  * particle tag in *ptag_out, may be NULL). */
 int kmpm_synchronize(kmpm_ctx *ctx, int64_t *ptag_out);
 
+/* ---- phase timing ---------------------------------------------------------- */
+/* Phases whose launches can be timed with HIP events recorded on the
+ * context's stream (no counterpart in the reference; measurement only). */
+enum {
+  KMPM_PHASE_SORT = 0,      /* cell keys + counting sort + tile lists */
+  KMPM_PHASE_P2G = 1,       /* particles_to_grid */
+  KMPM_PHASE_GRID = 2,      /* update_grid_state */
+  KMPM_PHASE_G2P = 3,       /* USL fused G2P + F + stress */
+  KMPM_PHASE_G2P_A = 4,     /* MUSL grid_to_points */
+  KMPM_PHASE_P2G_B = 5,     /* MUSL velocities_to_grid (P2G of momentum) */
+  KMPM_PHASE_GRID_B = 6,    /* MUSL velocities_to_grid (node velocities) */
+  KMPM_PHASE_STRESS_B = 7,  /* MUSL rate of deformation + F + stress */
+  KMPM_PHASE_DT = 8,        /* update_time + adjust_dt */
+  KMPM_NPHASE = 9
+};
+/* enable != 0 starts recording (and clears totals); 0 stops. */
+int kmpm_timing_enable(kmpm_ctx *ctx, int enable);
+/* Waits for the stream; total milliseconds and launch counts per phase. */
+int kmpm_timing_read(kmpm_ctx *ctx, double ms[KMPM_NPHASE], int64_t launches[KMPM_NPHASE]);
+/* Streaming copy of `bytes` bytes between two device buffers, `reps` times,
+ * on a private stream; returns GB/s (10^9 B/s) of read+write traffic.  Used
+ * to scale hardware byte counters; no counterpart in the reference. */
+int kmpm_copy_probe(int64_t bytes, int reps, double *gbps);
+
 /* ---- observables ---------------------------------------------------------- */
 /* dt, time, step counter (update.h) */
 int kmpm_get_time(kmpm_ctx *ctx, double *dt, double *time, int64_t *step);

@@ -73,6 +73,26 @@ static void swap_state(kmpm_ctx *c) {
   refresh_args(c);
 }
 
+// Phase timing: a start/stop HIP event pair recorded on the context stream
+// around each launch of a phase while timing is enabled.
+static void tbegin(kmpm_ctx *c, int phase) {
+  if (!c->timing) return;
+  std::vector<hipEvent_t> &pool = *c->ev_pool;
+  const size_t k = c->ev_phase->size() * 2;
+  while (pool.size() < k + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    pool.push_back(e);
+  }
+  (void)hipEventRecord(pool[k], c->stream);
+  c->ev_phase->push_back(phase);
+}
+static void tend(kmpm_ctx *c) {
+  if (!c->timing || c->ev_phase->empty()) return;
+  const size_t k = c->ev_phase->size() * 2 - 1;
+  (void)hipEventRecord((*c->ev_pool)[k], c->stream);
+}
+
 extern "C" {
 
 int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
@@ -219,6 +239,11 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
                   c->d_sc, c->d_scan_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->ev_pool) {
+    for (hipEvent_t e : *c->ev_pool) (void)hipEventDestroy(e);
+    delete c->ev_pool;
+    delete c->ev_phase;
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   free(c);
   return KMPM_OK;
@@ -390,30 +415,38 @@ int kmpm_fix_body_force(kmpm_ctx *c, uint32_t bit, const int set[3], const doubl
 // ---- phases ---------------------------------------------------------------
 int kmpm_compute_grid_weight_functions_and_gradients(kmpm_ctx *c) {
   if (!c) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_SORT);
   if (!c->keys_valid) {
     HIPCHK(launch_keys(c));
     c->keys_valid = true;
   }
   HIPCHK(launch_sort(c));
+  tend(c);
   c->sorted = true;
   return KMPM_OK;
 }
 
 int kmpm_particles_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_P2G);
   HIPCHK(launch_p2g(c, 0));
+  tend(c);
   return KMPM_OK;
 }
 
 int kmpm_update_grid_state(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_GRID);
   HIPCHK(launch_grid_update(c, 0));
+  tend(c);
   return KMPM_OK;
 }
 
 int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_G2P);
   HIPCHK(launch_g2p_usl(c));
+  tend(c);
   swap_state(c);
   c->sorted = false;
   c->keys_valid = true;
@@ -422,20 +455,28 @@ int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
 
 int kmpm_grid_to_points(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_G2P_A);
   HIPCHK(launch_g2p_musl_a(c));
+  tend(c);
   return KMPM_OK;
 }
 
 int kmpm_velocities_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_P2G_B);
   HIPCHK(launch_p2g(c, 1));
+  tend(c);
+  tbegin(c, KMPM_PHASE_GRID_B);
   HIPCHK(launch_grid_update(c, 1));
+  tend(c);
   return KMPM_OK;
 }
 
 int kmpm_update_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_STRESS_B);
   HIPCHK(launch_stress_musl_b(c));
+  tend(c);
   swap_state(c);
   c->sorted = false;
   c->keys_valid = true;
@@ -444,7 +485,9 @@ int kmpm_update_stress(kmpm_ctx *c) {
 
 int kmpm_adjust_dt(kmpm_ctx *c) {
   if (!c) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_DT);
   HIPCHK(launch_adjust_dt(c));
+  tend(c);
   return KMPM_OK;
 }
 
@@ -539,6 +582,41 @@ int kmpm_download_cells(kmpm_ctx *c, int32_t *cell_ijk, int64_t *sorted_ptags, i
     for (int64_t j = 0; j < np; j++) sorted_ptags[j] = c->h_ptag[pid[order[j]]];
   }
   return KMPM_OK;
+}
+
+int kmpm_timing_enable(kmpm_ctx *c, int enable) {
+  if (!c) return KMPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->ev_pool) {
+    c->ev_pool = new std::vector<hipEvent_t>();
+    c->ev_phase = new std::vector<int>();
+  }
+  if (enable) c->ev_phase->clear();
+  c->timing = enable != 0;
+  return KMPM_OK;
+}
+
+int kmpm_timing_read(kmpm_ctx *c, double ms[KMPM_NPHASE], int64_t launches[KMPM_NPHASE]) {
+  if (!c || !ms || !launches) return KMPM_ERR_ARG;
+  for (int k = 0; k < KMPM_NPHASE; k++) {
+    ms[k] = 0;
+    launches[k] = 0;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->ev_phase) return KMPM_OK;
+  for (size_t i = 0; i < c->ev_phase->size(); i++) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, (*c->ev_pool)[2 * i], (*c->ev_pool)[2 * i + 1]));
+    const int ph = (*c->ev_phase)[i];
+    ms[ph] += t;
+    launches[ph] += 1;
+  }
+  return KMPM_OK;
+}
+
+int kmpm_copy_probe(int64_t bytes, int reps, double *gbps) {
+  if (bytes < 1024 || reps < 1 || !gbps) return KMPM_ERR_ARG;
+  return kmpm::copy_probe(bytes, reps, gbps) == hipSuccess ? KMPM_OK : KMPM_ERR_HIP;
 }
 
 int kmpm_get_tile_shape(kmpm_ctx *c, int tile[3]) {

@@ -1,6 +1,8 @@
 // kmpm_ctx.h -- the device context behind the C ABI.
 #pragma once
 
+#include <vector>
+
 #include "kmpm_device.h"
 
 namespace kmpm {
@@ -88,6 +90,10 @@ struct kmpm_ctx {
   void *d_scan_tmp;
   size_t scan_tmp_bytes;
   int64_t ncells;
+  // phase timing (HIP events on `stream`)
+  bool timing;
+  std::vector<hipEvent_t> *ev_pool;   // recorded pairs: start, stop
+  std::vector<int> *ev_phase;         // phase of each pair
   bool keys_valid;       // d_key holds keys of the current positions
   bool sorted;           // order/tiles valid for the current step
 };
@@ -107,4 +113,5 @@ hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);
 hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double *d_wd);
 hipError_t launch_cells(kmpm_ctx *c, int *d_ijk);
 size_t scan_temp_bytes(int64_t n);
+hipError_t copy_probe(int64_t bytes, int reps, double *gbps);
 }  // namespace kmpm

@@ -1070,4 +1070,42 @@ hipError_t launch_cells(kmpm_ctx *c, int *d_ijk) {
   return hipGetLastError();
 }
 
+// Streaming copy used to calibrate hardware byte counters.
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t copy_probe(int64_t bytes, int reps, double *gbps) {
+  const int64_t n = bytes / (int64_t)sizeof(double2);
+  double2 *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, n * sizeof(double2));
+  if (e == hipSuccess) e = hipMalloc(&b, n * sizeof(double2));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid = cus * 8;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a, b, n);   // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a, b, n);
+    (void)hipEventRecord(e1, s);
+    e = hipStreamSynchronize(s);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) *gbps = 2.0 * (double)n * sizeof(double2) * reps / (ms * 1e-3) / 1e9;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return e;
+}
+
 }  // namespace kmpm

@@ -24,6 +24,8 @@ EOS_NONE, EOS_LINEAR, EOS_SHOCK, EOS_FLUID = 0, 1, 2, 3
 STRENGTH_NONE, STRENGTH_LINEAR, STRENGTH_JOHNSON_COOK, STRENGTH_FLUID = 0, 1, 2, 3
 DAMAGE_NONE, DAMAGE_JOHNSON_COOK = 0, 1
 REGION_BLOCK, REGION_CYLINDER = 0, 1
+PHASES = ["sort", "p2g", "grid", "g2p", "g2p_a", "p2g_b", "grid_b", "stress_b", "dt"]
+NPHASE = len(PHASES)
 
 # style keys of the reference's plugin registries -> enum values
 EOS_KEYS = {"linear": EOS_LINEAR, "shock": EOS_SHOCK, "fluid": EOS_FLUID}
@@ -147,6 +149,9 @@ def _declare(lib):
         "kmpm_download_cells": (C.c_int, [vp, P(C.c_int32), P(C.c_int64), C.c_int64]),
         "kmpm_debug_weights": (C.c_int, [vp, P(C.c_int64), _DP, _DP, C.c_int64]),
         "kmpm_get_tile_shape": (C.c_int, [vp, I3]),
+        "kmpm_timing_enable": (C.c_int, [vp, C.c_int]),
+        "kmpm_timing_read": (C.c_int, [vp, P(C.c_double), P(C.c_int64)]),
+        "kmpm_copy_probe": (C.c_int, [C.c_int64, C.c_int, P(C.c_double)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -298,6 +303,15 @@ class Context:
             "download_cells")
         return ijk, (sp if sorted_ptags else None)
 
+    def timing_enable(self, on=True):
+        check(self.L.kmpm_timing_enable(self.h, 1 if on else 0), "timing_enable")
+
+    def timing_read(self):
+        ms = (C.c_double * NPHASE)()
+        cnt = (C.c_int64 * NPHASE)()
+        check(self.L.kmpm_timing_read(self.h, ms, cnt), "timing_read")
+        return {PHASES[k]: (ms[k], cnt[k]) for k in range(NPHASE)}
+
     def tile_shape(self):
         t = I3()
         check(self.L.kmpm_get_tile_shape(self.h, t), "get_tile_shape")
@@ -314,3 +328,10 @@ class Context:
                                         w.ctypes.data_as(_DP), wd.ctypes.data_as(_DP), self.np),
               "debug_weights")
         return tag, w, wd
+
+
+def copy_probe(nbytes, reps=20):
+    """GB/s of a streaming device copy (read + write bytes)."""
+    g = C.c_double()
+    check(lib().kmpm_copy_probe(int(nbytes), int(reps), C.byref(g)), "copy_probe")
+    return g.value

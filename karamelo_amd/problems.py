@@ -144,10 +144,12 @@ def populate_solids(cfg, specs, fn=None):
 # ---------------------------------------------------------------------------
 # workloads
 # ---------------------------------------------------------------------------
-def two_disks(steps=3000, ppc1d=2, flip=1.0, populate_fn=None):
+def two_disks(steps=3000, ppc1d=2, flip=0.99, populate_fn=None):
     """Two elastic disks colliding (2D ULMPM USL, linear shape, FP64):
     unit square, cell 1/20, disks R=0.2 at (0.25,0.25) and (0.75,0.75),
-    E=1000, nu=0.3, rho=1000, v0 = +-0.1 along the diagonal, dt = 1e-3."""
+    E=1000, nu=0.3, rho=1000, v0 = +-0.1 along the diagonal, dt = 1e-3,
+    FLIP 0.99 (the reference's default PIC_FLIP, update.cpp:50; pure FLIP
+    lets 1-ulp input differences reach J <= 0 within 3000 steps)."""
     h = 1.0 / 20.0
     cfg = make_config(2, (0.0, 0.0), (1.0, 1.0), h, scheme="usl", shape="linear", flip=flip,
                       dt=lit_e(1, -3), dt_constant=True)
@@ -210,12 +212,16 @@ def tensile_bar(h=6e-5, ppc1d=2, v_pull=10.0, shape="cubic-spline", scheme="usl"
 
 
 def dam_break(n_cells=32, ppc1d=2, steps=0, scheme="musl", shape="cubic-spline", jitter=0.0,
-              seed=20261202, populate_fn=None):
+              seed=20261202, populate_fn=None, dt_factor=0.1):
     """3D ULMPM dam break: unit box, water column 0.4 x 0.4 x 0.8 in a corner,
     eos fluid (rho0=1000, K=2e6, Gamma=7) + strength fluid (mu=1e-3),
-    gravity -9.81 along z (fix body_force on all nodes)."""
+    gravity -9.81 along z (fix body_force on all nodes), slip walls on the six
+    box faces (velocity_nodes with a zero normal component on nodes within
+    h/4 of a face, as examples/Dam-break/water-dam.mpm does in 2D),
+    dt_factor 0.1."""
     h = 1.0 / n_cells
-    cfg = make_config(3, (0, 0, 0), (1, 1, 1), h, scheme=scheme, shape=shape, flip=0.99)
+    cfg = make_config(3, (0, 0, 0), (1, 1, 1), h, scheme=scheme, shape=shape, flip=0.99,
+                      dt_factor=dt_factor)
     mat = eos_strength_material("fluid", (1000.0, 2e6, 7.0), "fluid", (lit_e(1, -3),))
     p = populate_solids(cfg, [(block(0, lit(0.4), 0, lit(0.4), 0, lit(0.8)), ppc1d, mat, 0)],
                         fn=populate_fn)
@@ -224,8 +230,25 @@ def dam_break(n_cells=32, ppc1d=2, steps=0, scheme="musl", shape="cubic-spline",
         p["x"] += rng.uniform(-jitter * h, jitter * h, size=p["x"].shape)
         p["x"] = np.clip(p["x"], 1e-9, 1 - 1e-9)
         p["x0"][:] = p["x"]
+    X, Y, Z = node_coords(cfg)
+    mask = np.ones(len(X), dtype=np.uint32)
     fixes = [Fix("body_force", 1, (0, 0, 1), (0, 0, lit(-9.81)))]
-    return Problem("dam_break", cfg, [mat], p, fixes=fixes, steps=steps)
+    big = 1e20
+    for axis, C in enumerate((X, Y, Z)):
+        for side in (0, 1):
+            bit = 1 << (1 + 2 * axis + side)
+            lo_ = [-big] * 3
+            hi_ = [big] * 3
+            if side == 0:
+                hi_[axis] = h / 4
+            else:
+                lo_[axis] = 1.0 - h / 4
+            r = block(lo_[0], hi_[0], lo_[1], hi_[1], lo_[2], hi_[2])
+            mask[region_match_nodes(r, X, Y, Z)] |= bit
+            sv = [0, 0, 0]
+            sv[axis] = 1
+            fixes.append(Fix("velocity_nodes", bit, tuple(sv), (0, 0, 0), (0, 0, 0)))
+    return Problem("dam_break", cfg, [mat], p, node_mask=mask, fixes=fixes, steps=steps)
 
 
 def context_for(problem):

@@ -261,7 +261,7 @@ def test_jc_damage_known_answers():
 def test_two_disks_conservation_and_symmetry():
     """Whole USL step on the two-disk collision: total momentum stays 0 by
     symmetry, mass is conserved, Ek + Es stays near its initial value."""
-    prob = P.two_disks(populate_fn=O.oracle_populate_fn())
+    prob = P.two_disks(flip=1.0, populate_fn=O.oracle_populate_fn())
     assert prob.np == 416
     o = O.oracle_for(prob)
     p0 = o.download()
