@@ -22,12 +22,20 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s: mixed+surrogate vs plain FP64 GPU at ≤1e-3 rel L2 vs FP64"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per particle of the fused USL G2P kernel (FP64): the
-# particle state it must read (x3 v3 F9 sigma6 eps_p epsdot_p damage_init
-# damage mass vol0 = 27 doubles = 216 B) and write (x v F sigma eps_p epsdot_p
-# damage_init damage = 25 doubles = 200 B), plus its grid share (v_update and
-# v, 6 doubles per node at 8 particles per cell = 6 B).
-G2P_ALG_BYTES_PER_PARTICLE = 216 + 200 + 6
+# Algorithmic bytes per particle of each timed phase (FP64, 3D, 8 particles
+# per cell; node data divided by 8):
+#   g2p  fused USL G2P/F/stress: reads x3 v3 F9 sigma6 eps_p epsdot_p
+#        damage_init damage mass vol0 (27 doubles = 216 B), writes x v F sigma
+#        eps_p epsdot_p damage_init damage (25 doubles = 200 B), reads v_update
+#        and v of the grid (6 doubles per node -> 6 B)                 = 422 B
+#   p2g  reads x3 v3 mass vol sigma6 (14 doubles = 112 B), writes node mass,
+#        momentum and force (7 doubles per node -> 7 B)                 = 119 B
+#   grid reads node mass, momentum, force (7 doubles), writes mass, v,
+#        v_update (7 doubles): 14 doubles per node -> 14 B               = 14 B
+#   sort reads the cell key and particle id, writes the permutation
+#        (3 x 4 B)                                                      = 12 B
+ALG_BYTES_PER_PARTICLE = {"g2p": 422, "p2g": 119, "grid": 14, "sort": 12}
+KERNEL_NAMES = {"g2p": "k_g2p_usl", "p2g": "k_p2g", "grid": "k_grid_update", "sort": "sort"}
 # Whole step, same counting (state once in, once out, grid once): 442 B.
 STEP_ALG_BYTES_PER_PARTICLE = 442
 
@@ -90,7 +98,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--h", type=float, default=6e-5, help="cell size (m)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -136,12 +144,8 @@ def main():
     dom = max(phases, key=lambda k: phases[k][0])
     ms_tot, launches = phases[dom]
     avg_s = ms_tot / max(launches, 1) * 1e-3
-    if dom == "g2p":
-        alg = G2P_ALG_BYTES_PER_PARTICLE * npart
-        kname = "k_g2p_usl"
-    else:
-        alg = None
-        kname = dom
+    alg = ALG_BYTES_PER_PARTICLE.get(dom, 0) * npart or None
+    kname = KERNEL_NAMES.get(dom, dom)
     achieved = alg / avg_s / 1e9 if alg else None
     traffic = read_traffic(kname, workload)
     roofline = {"bound": "hbm", "kernel": kname,

@@ -1,6 +1,7 @@
 // kmpm_api.hip -- C ABI of the MI355X explicit MPM step: context, buffers,
 // phases and the stream-ordered step loop.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -163,6 +164,19 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
     free(c);
     return KMPM_ERR_ARG;
   }
+  // material class of the stress kernels
+  {
+    bool all_lin = true, all_jc = true, all_fluid = true;
+    for (int i = 0; i < nmat; i++) {
+      const kmpm_material &m = mats[i];
+      all_lin = all_lin && m.type == KMPM_MAT_LINEAR;
+      all_jc = all_jc && m.type == KMPM_MAT_EOS_STRENGTH && m.eos == KMPM_EOS_SHOCK &&
+               m.strength == KMPM_STRENGTH_JOHNSON_COOK;
+      all_fluid = all_fluid && m.type == KMPM_MAT_EOS_STRENGTH && m.eos == KMPM_EOS_FLUID &&
+                  m.strength == KMPM_STRENGTH_FLUID && m.damage == KMPM_DAMAGE_NONE;
+    }
+    a.mat_class = all_lin ? MC_LINEAR : all_jc ? MC_JC_SHOCK : all_fluid ? MC_FLUID : MC_GENERIC;
+  }
   // material table
   std::vector<DevMat> dm(nmat);
   for (int i = 0; i < nmat; i++) {
@@ -179,6 +193,16 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
     d.lambda = m.lambda;
     d.signal_velocity = m.signal_velocity;
     d.cfl_k43g = m.K + FOUR_THIRD * m.G;
+    {  // JC thermal factor at T = 0 (strength_jc.cpp:127-133)
+      const double mm = m.strength_p[6], Tr = m.strength_p[7], Tm = m.strength_p[8], T = 0;
+      double f = 1.0;
+      if (T < Tm) {
+        if (mm != 0 && T >= Tr) f = 1.0 - std::pow((T - Tr) / (Tm - Tr), mm);
+      } else {
+        f = 0.0;
+      }
+      d.jc_thermal = f;
+    }
     memcpy(d.eos_p, m.eos_p, sizeof(d.eos_p));
     memcpy(d.strength_p, m.strength_p, sizeof(d.strength_p));
     memcpy(d.damage_p, m.damage_p, sizeof(d.damage_p));

@@ -31,11 +31,10 @@ __device__ inline void jc_strength(const DevMat &m, const double *sig, const dou
   if (eps_p < 1.0e-10) ys = A;
   else ys = A + B * pow(eps_p, n);
   if (C != 0) ys *= pow(1.0 + epsdot_ratio, C);
-  if (T < Tm) {
-    if (mm != 0 && T >= Tr) ys *= 1.0 - pow((T - Tr) / Tmr, mm);
-  } else {
-    ys = 0;
-  }
+  // Thermal softening: with no temperature law T = 0 and the factor
+  // (1 - ((T-Tr)/Tmr)^m, 1, or 0 above Tm) depends on the material only.
+  ys *= m.jc_thermal;
+  (void)T; (void)Tmr; (void)Tr; (void)Tm; (void)mm;
   double Gd = G_;
   if (damage > 0) {
     Gd *= (1 - damage);
@@ -119,11 +118,13 @@ __device__ inline void jc_damage(const DevMat &m, double &damage_init, double &d
 
 // Solid::update_stress per particle, solid.cpp:1262-1374, for one particle.
 // sig, eel: symmetric (in/out); D symmetric rate of deformation.
+template <int MC>
 __device__ inline void update_stress_point(const DevMat &m, double dt, double h, double J,
                                            double rho, const double *D, double *sig, double *eel,
                                            double &ep, double &epdot, double &dinit,
                                            double &dmg) {
-  if (m.type == KMPM_MAT_LINEAR) {   // :1268-1280
+  const bool linear = MC == MC_LINEAR || (MC == MC_GENERIC && m.type == KMPM_MAT_LINEAR);
+  if (linear) {   // :1268-1280
     double si[6];
     for (int e = 0; e < 6; e++) si[e] = dt * D[e];
     for (int e = 0; e < 6; e++) eel[e] += si[e];
@@ -134,20 +135,25 @@ __device__ inline void update_stress_point(const DevMat &m, double dt, double h,
     for (int e = 3; e < 6; e++) sig[e] += g2 * si[e] + lt * 0.0;
     return;
   }
+  if (MC == MC_LINEAR) return;
   // eos-strength branch, :1292-1374
+  const bool shock = MC == MC_JC_SHOCK || (MC == MC_GENERIC && m.eos == KMPM_EOS_SHOCK);
+  const bool efluid = MC == MC_FLUID || (MC == MC_GENERIC && m.eos == KMPM_EOS_FLUID);
+  const bool jc = MC == MC_JC_SHOCK || (MC == MC_GENERIC && m.strength == KMPM_STRENGTH_JOHNSON_COOK);
+  const bool sfluid = MC == MC_FLUID || (MC == MC_GENERIC && m.strength == KMPM_STRENGTH_FLUID);
   double pH;
-  if (m.eos == KMPM_EOS_SHOCK) {
+  if (shock) {
     pH = shock_pressure(m, J, rho, dmg, D, h);
-  } else if (m.eos == KMPM_EOS_FLUID) {            // eos_fluid.cpp:68-73
+  } else if (efluid) {                               // eos_fluid.cpp:68-73
     const double mu = rho / m.eos_p[0];
     pH = m.eos_p[1] * (pow(mu, m.eos_p[2]) - 1.0);
   } else {                                           // eos_linear.cpp:71-74
     pH = m.eos_p[1] * (1 - J) * (1 - dmg);
   }
   double sdev[6], dps = 0;
-  if (m.strength == KMPM_STRENGTH_JOHNSON_COOK) {
+  if (jc) {
     jc_strength(m, sig, D, dt, ep, epdot, dmg, sdev, &dps);
-  } else if (m.strength == KMPM_STRENGTH_FLUID) {    // strength_fluid.cpp:46-58
+  } else if (sfluid) {                               // strength_fluid.cpp:46-58
     const double t3 = (D[0] + D[1] + D[2]) / 3.0;
     const double c2 = 2.0 * m.strength_p[0];
     sdev[0] = c2 * (D[0] - t3);
@@ -169,7 +175,8 @@ __device__ inline void update_stress_point(const DevMat &m, double dt, double h,
   epdot -= epdot * dt / tav;
   epdot += dps / tav;
   epdot = epdot > 0.0 ? epdot : 0.0;
-  if (m.damage == KMPM_DAMAGE_JOHNSON_COOK) jc_damage(m, dinit, dmg, pH, sdev, epdot, dps);
+  if (MC != MC_FLUID && m.damage == KMPM_DAMAGE_JOHNSON_COOK)
+    jc_damage(m, dinit, dmg, pH, sdev, epdot, dps);
   const double trD = D[0] + D[1] + D[2];
   const double trE = eel[0] + eel[1] + eel[2];
   if (dmg == 0 || pH >= 0) {                        // :1353-1356

@@ -28,6 +28,7 @@ struct KArgs {
   double h, inv_h, flip, dt_factor;
   int dt_constant;
   int nmat;
+  int mat_class;    // MatClass the stress kernels are specialised for
   const DevMat *mats;
   TileGeom tg;
   // particles

@@ -32,10 +32,16 @@ struct DevMat {
   int type, eos, strength, damage;
   double rho0, K, G, lambda, signal_velocity;
   double cfl_k43g;            // K + FOUR_THIRD * G  (solid.cpp:1382)
+  double jc_thermal;          // JC thermal factor at T = 0 (strength_jc.cpp:127-133), host-computed
   double eos_p[9];
   double strength_p[9];
   double damage_p[8];
 };
+
+// Material classes the stress kernels are specialised for (chosen on the
+// host from the material table): only the plugins of the class are compiled
+// into the kernel, which keeps its register footprint small.
+enum MatClass { MC_GENERIC = 0, MC_LINEAR = 1, MC_JC_SHOCK = 2, MC_FLUID = 3 };
 
 struct DevFixVel { uint32_t bit; int set[3]; double val[3], prev[3]; };
 struct DevFixBF { uint32_t bit; int set[3]; double val[3]; };

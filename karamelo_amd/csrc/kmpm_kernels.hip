@@ -513,21 +513,25 @@ __device__ inline void gather_stencil(const double *nb, const int *cl, const dou
   }
 }
 
-// F update (solid.cpp:1181-1216, UL), D = sym(L), stress, CFL speed.
-// Returns wave speed (or -1 when the particle is fully damaged).
-__device__ inline double finish_particle(const KArgs &a, double dt, const double *L, double *F,
-                                         double *sig, double *eel, double &ep, double &epdot,
-                                         double &dinit, double &dmg, double vol0, double &vol,
-                                         int mat, const double *v, int pid) {
+// F update (solid.cpp:1181-1216, UL): F = (I + dt L) F, J = det F.
+__device__ inline double update_F(double dt, const double *L, double *F) {
   double M[9], Fn[9];
   for (int e = 0; e < 9; e++) M[e] = ((e % 4 == 0) ? 1.0 : 0.0) + dt * L[e];
   for (int r = 0; r < 3; r++)
     for (int c = 0; c < 3; c++)
       Fn[3 * r + c] = M[3 * r] * F[c] + M[3 * r + 1] * F[3 + c] + M[3 * r + 2] * F[6 + c];
   for (int e = 0; e < 9; e++) F[e] = Fn[e];
-  const double J = det3(F);
-  vol = J * vol0;
-  if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+  return det3(F);
+}
+
+// D = sym(L) (solid.cpp:1239), stress update (solid.cpp:1246-1374) and the
+// particle's CFL wave speed (:1376-1385); returns -1 for fully damaged
+// particles (skipped by the CFL maximum).
+template <int MC>
+__device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
+                                          double *sig, double *eel, double &ep, double &epdot,
+                                          double &dinit, double &dmg, int mat, const double *v,
+                                          int pid) {
   const DevMat &m = a.mats[mat];
   const double rho = m.rho0 / J;
   double D[6];
@@ -537,7 +541,7 @@ __device__ inline double finish_particle(const KArgs &a, double dt, const double
   D[3] = 0.5 * (L[1] + L[3]);
   D[4] = 0.5 * (L[2] + L[6]);
   D[5] = 0.5 * (L[5] + L[7]);
-  update_stress_point(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   if (dmg >= 1.0) return -1.0;
   const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
   const double c = sqrt(m.cfl_k43g / rho) + vm;
@@ -555,7 +559,7 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // ---------------------------------------------------------------------------
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ>
+template <int DIM, int S1, int TX, int TY, int TZ, int MC>
 __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
@@ -598,26 +602,39 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
       for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
       if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
       for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
-      double F[9], sig[6], eel[6];
-      for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+      // State is written in sorted order as soon as each part is final, so
+      // it does not stay live across the constitutive update.
+      const int mat = IFLD(a.ia, MATI, slot);
+      for (int d = 0; d < 3; d++) {
+        PF(a.pb, PX + d, j) = x[d];
+        PF(a.pb, VX + d, j) = v[d];
+      }
+      a.key[j] = particle_key(a, x[0], x[1], x[2]);
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      const double vol0 = PF(a.pa, VOL0, slot);
+      PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
+      PF(a.pb, VOL0, j) = vol0;
+      double J;
+      {
+        double F[9];
+        for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+        J = update_F(dt, L, F);
+        for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
+      }
+      PF(a.pb, VOL, j) = J * vol0;
+      double dmg = PF(a.pa, DMG, slot);
+      if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+      double sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
         sig[e] = PF(a.pa, SXX + e, slot);
         eel[e] = PF(a.pa, EXX + e, slot);
       }
       double ep = PF(a.pa, EP, slot), epdot = PF(a.pa, EPDOT, slot);
-      double dinit = PF(a.pa, DINIT, slot), dmg = PF(a.pa, DMG, slot);
-      const double mass = PF(a.pa, MASS, slot), vol0 = PF(a.pa, VOL0, slot);
-      const int mat = IFLD(a.ia, MATI, slot);
-      double vol;
+      double dinit = PF(a.pa, DINIT, slot);
       const double c =
-          finish_particle(a, dt, L, F, sig, eel, ep, epdot, dinit, dmg, vol0, vol, mat, v, pid);
+          stress_and_speed<MC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
       if (c > cmax) cmax = c;
-      // write in sorted order
-      for (int d = 0; d < 3; d++) {
-        PF(a.pb, PX + d, j) = x[d];
-        PF(a.pb, VX + d, j) = v[d];
-      }
-      for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
       for (int e = 0; e < 6; e++) {
         PF(a.pb, SXX + e, j) = sig[e];
         PF(a.pb, EXX + e, j) = eel[e];
@@ -626,12 +643,6 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
       PF(a.pb, EPDOT, j) = epdot;
       PF(a.pb, DINIT, j) = dinit;
       PF(a.pb, DMG, j) = dmg;
-      PF(a.pb, MASS, j) = mass;
-      PF(a.pb, VOL0, j) = vol0;
-      PF(a.pb, VOL, j) = vol;
-      IFLD(a.ib, PID, j) = pid;
-      IFLD(a.ib, MATI, j) = mat;
-      a.key[j] = particle_key(a, x[0], x[1], x[2]);
     }
   }
   block_max_to(cmax, a.sc);
@@ -696,7 +707,7 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
 // ---------------------------------------------------------------------------
 // MUSL part B: L from re-mapped v (old weights), F, stress, CFL (musl.cpp:75-77)
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ>
+template <int DIM, int S1, int TX, int TY, int TZ, int MC>
 __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
@@ -732,21 +743,31 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
       const int pid = IFLD(a.ia, PID, slot);
       double v[3];
       for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
-      double F[9], sig[6], eel[6];
-      for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+      const int mat = IFLD(a.ia, MATI, slot);
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      const double vol0 = PF(a.pa, VOL0, slot);
+      PF(a.pb, VOL0, j) = vol0;
+      double J;
+      {
+        double F[9];
+        for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+        J = update_F(dt, L, F);
+        for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
+      }
+      PF(a.pb, VOL, j) = J * vol0;
+      double dmg = PF(a.pa, DMG, slot);
+      if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+      double sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
         sig[e] = PF(a.pa, SXX + e, slot);
         eel[e] = PF(a.pa, EXX + e, slot);
       }
       double ep = PF(a.pa, EP, slot), epdot = PF(a.pa, EPDOT, slot);
-      double dinit = PF(a.pa, DINIT, slot), dmg = PF(a.pa, DMG, slot);
-      const double vol0 = PF(a.pa, VOL0, slot);
-      const int mat = IFLD(a.ia, MATI, slot);
-      double vol;
+      double dinit = PF(a.pa, DINIT, slot);
       const double c =
-          finish_particle(a, dt, L, F, sig, eel, ep, epdot, dinit, dmg, vol0, vol, mat, v, pid);
+          stress_and_speed<MC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
       if (c > cmax) cmax = c;
-      for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
       for (int e = 0; e < 6; e++) {
         PF(a.pb, SXX + e, j) = sig[e];
         PF(a.pb, EXX + e, j) = eel[e];
@@ -755,10 +776,6 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
       PF(a.pb, EPDOT, j) = epdot;
       PF(a.pb, DINIT, j) = dinit;
       PF(a.pb, DMG, j) = dmg;
-      PF(a.pb, VOL0, j) = vol0;
-      PF(a.pb, VOL, j) = vol;
-      IFLD(a.ib, PID, j) = pid;
-      IFLD(a.ib, MATI, j) = mat;
     }
   }
   block_max_to(cmax, a.sc);
@@ -1007,14 +1024,24 @@ hipError_t launch_grid_clean(kmpm_ctx *c) {
   return hipGetLastError();
 }
 
+#define MC_DISPATCH(c, ...)                                                         \
+  do {                                                                              \
+    switch ((c)->a.mat_class) {                                                     \
+      case MC_LINEAR: { constexpr int MC = MC_LINEAR; __VA_ARGS__; } break;         \
+      case MC_JC_SHOCK: { constexpr int MC = MC_JC_SHOCK; __VA_ARGS__; } break;     \
+      case MC_FLUID: { constexpr int MC = MC_FLUID; __VA_ARGS__; } break;           \
+      default: { constexpr int MC = MC_GENERIC; __VA_ARGS__; } break;               \
+    }                                                                               \
+  } while (0)
+
 hipError_t launch_g2p_usl(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 4);
-  DISPATCH(c, {
+  DISPATCH(c, MC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+    hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256), 0,
                        c->stream, c->a);
-  });
+  }));
   return hipGetLastError();
 }
 
@@ -1032,11 +1059,11 @@ hipError_t launch_g2p_musl_a(kmpm_ctx *c) {
 hipError_t launch_stress_musl_b(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 4);
-  DISPATCH(c, {
+  DISPATCH(c, MC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
-                       c->stream, c->a);
-  });
+    hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256),
+                       0, c->stream, c->a);
+  }));
   return hipGetLastError();
 }
 
