@@ -11,6 +11,10 @@ namespace kmpm {
 // Symmetric tensor index helpers: xx,yy,zz,xy,xz,yz
 __device__ inline double tr_sym(const double *s) { return s[0] + s[1] + s[2]; }
 
+// x^y for x > 0 as exp(y log x): a few ulp, and far fewer registers than
+// the general-case pow (used where the base is positive by construction).
+__device__ inline double pow_pos(double x, double y) { return exp(y * log(x)); }
+
 // StrengthJohnsonCook::update_deviatoric_stress, strength_jc.cpp:100-184 (T = 0).
 // out = final deviatoric stress, *dps = plastic strain increment (untouched
 // when damage >= 1, the caller holds 0).
@@ -29,8 +33,8 @@ __device__ inline void jc_strength(const DevMat &m, const double *sig, const dou
   epsdot_ratio = epsdot_ratio > 1.0 ? epsdot_ratio : 1.0;
   double ys;
   if (eps_p < 1.0e-10) ys = A;
-  else ys = A + B * pow(eps_p, n);
-  if (C != 0) ys *= pow(1.0 + epsdot_ratio, C);
+  else ys = A + B * pow_pos(eps_p, n);                 // eps_p >= 1e-10
+  if (C != 0) ys *= pow_pos(1.0 + epsdot_ratio, C);   // base >= 2
   // Thermal softening: with no temperature law T = 0 and the factor
   // (1 - ((T-Tr)/Tmr)^m, 1, or 0 above Tm) depends on the material only.
   ys *= m.jc_thermal;

@@ -73,10 +73,28 @@ __global__ void k_sort_begin(KArgs a, int64_t ncells) {
   }
 }
 
+// Counts particles per cell and gives each its arrival rank.  Particles are
+// stored in the previous step's sorted order, so equal keys come in runs:
+// one atomic per run of equal keys inside a wavefront.
 __global__ void k_count(KArgs a) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  a.rank[i] = atomicAdd(&a.cell_count[a.key[i]], 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < a.np;
+  const int key = valid ? a.key[i] : -1;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = valid && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long live = __ballot(valid);
+  // run of this lane: [start, end)
+  const unsigned long long below = heads & ((2ull << lane) - 1ull);   // heads at lanes <= lane
+  const int start = below ? 63 - __clzll(below) : 0;
+  const unsigned long long above = lane < 63 ? (heads >> (lane + 1)) : 0ull;
+  const int nlive = __popcll(live);
+  const int end = above ? lane + 1 + __ffsll(above) - 1 : nlive;
+  int base = 0;
+  if (head) base = atomicAdd(&a.cell_count[key], end - lane);
+  base = __shfl(base, start, 64);
+  if (valid) a.rank[i] = base + (lane - start);
 }
 
 __global__ void k_scatter(KArgs a) {
@@ -139,6 +157,39 @@ __global__ void k_dilate(KArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int SCR_FIELDS = 7;   // scratch slot layout: mass, momentum[3], force[3]
 
+// Raw particle data a P2G lane loads for one particle.
+template <int DIM, bool REMAP>
+struct P2GIn {
+  double x[DIM], v[DIM], m;
+  double s[REMAP ? 1 : (DIM == 3 ? 6 : 3)];   // -vol * sigma (xx,yy,zz,xy,xz,yz | xx,yy,xy)
+};
+
+template <int DIM, bool REMAP>
+__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP> &in) {
+  if (REMAP) {
+    for (int d = 0; d < DIM; d++) {
+      in.x[d] = a.xo[(size_t)d * a.cap + j];
+      in.v[d] = PF(a.pb, VX + d, j);
+    }
+    in.m = PF(a.pb, MASS, j);
+  } else {
+    const int slot = a.order[j];
+    for (int d = 0; d < DIM; d++) {
+      in.x[d] = PF(a.pa, PX + d, slot);
+      in.v[d] = PF(a.pa, VX + d, slot);
+    }
+    in.m = PF(a.pa, MASS, slot);
+    const double nvol = -PF(a.pa, VOL, slot);
+    if (DIM == 3) {
+      for (int e = 0; e < 6; e++) in.s[e] = nvol * PF(a.pa, SXX + e, slot);
+    } else {
+      in.s[0] = nvol * PF(a.pa, SXX, slot);
+      in.s[1] = nvol * PF(a.pa, SYY, slot);
+      in.s[2] = nvol * PF(a.pa, SXY, slot);
+    }
+  }
+}
+
 template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP>
 __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   constexpr int S = DIM == 3 ? S1 * S1 * S1 : S1 * S1;
@@ -148,14 +199,22 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   constexpr int BN = EX * EY * EZ;
   constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   constexpr int NSIG = DIM == 3 ? 6 : 3;
-  constexpr int NW = DIM * S1;                  // 1D values (then NW derivatives)
+  constexpr int NW = DIM * S1;                              // (w, dw) pairs per particle
   constexpr int NF = REMAP ? (1 + DIM) : (1 + 2 * DIM);
-  constexpr int NPAY = REMAP ? (1 + DIM) : (1 + DIM + NSIG);
-  constexpr int NV = 2 * NW + NPAY;
+  constexpr int NP2 = 2 + (REMAP ? 0 : (NSIG + 1) / 2);    // payload pairs: (m,v0),(v1,v2),sigma
   static_assert(64 % S == 0 && CH % G == 0 && CH <= 64, "stencil/chunk shape");
+  // LDS node block strides.  For the 3D cubic 4x4x4 tile the rows are padded
+  // (z stride 8, x stride 58 doubles) so the 64 lanes' atomic adds of one
+  // cell hit every bank at most twice, the minimum for 64 doubles.
+  constexpr bool PAD = DIM == 3 && S1 == 4 && TX == 4 && TY == 4 && TZ == 4;
+  constexpr int PSB = PAD ? 8 : EZ;
+  constexpr int PSA = PAD ? 58 : EY * EZ;
+  constexpr int PBN = EX * PSA;
+  static_assert(PSA >= EY * PSB, "block stride");
 
-  __shared__ double blk[NF * BN];
-  __shared__ double stg[4][NV][CH];
+  __shared__ double blk[NF * PBN];
+  __shared__ double2 stw[4][NW][CH + 1];   // +1: the S1 rows a lane group reads fall in distinct banks
+  __shared__ double2 spay[4][NP2][CH];
   __shared__ int stg_cell[4][CH];
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -168,115 +227,120 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   const int sa = DIM == 3 ? s / (S1 * S1) : s / S1;
   const int sb = DIM == 3 ? (s / S1) % S1 : s % S1;
   const int sc = DIM == 3 ? s % S1 : 0;
+  const int sofs = sa * PSA + sb * PSB + sc;
 
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
     const int tz = DIM == 3 ? t % nt2 : 0;
     const int ty = (t / nt2) % nt1;
     const int tx = t / (nt2 * nt1);
-    const int ox = tx * TX, oy = ty * TY, oz = DIM == 3 ? tz * TZ : 0;
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
-    for (int q = tid; q < NF * BN; q += 256) blk[q] = 0;
+    for (int q = tid; q < NF * PBN; q += 256) blk[q] = 0;
     __syncthreads();
 
     const int npart = p1 - p0;
-    for (int c0 = wave * CH; c0 < npart; c0 += 4 * CH) {
+    // The particle data of the wave's next chunk is loaded into registers
+    // while the current chunk is accumulated (hides HBM latency).
+    P2GIn<DIM, REMAP> in;
+    int c0 = wave * CH;
+    if (c0 < npart && lane < min(CH, npart - c0)) p2g_load<DIM, REMAP>(a, p0 + c0 + lane, in);
+    for (; c0 < npart; c0 += 4 * CH) {
       const int cnt = min(CH, npart - c0);
       if (lane < cnt) {
-        const int j = p0 + c0 + lane;
-        double x[3], v[3], m;
-        if (REMAP) {
-          for (int d = 0; d < DIM; d++) {
-            x[d] = a.xo[(size_t)d * a.cap + j];
-            v[d] = PF(a.pb, VX + d, j);
-          }
-          m = PF(a.pb, MASS, j);
-        } else {
-          const int slot = a.order[j];
-          for (int d = 0; d < DIM; d++) {
-            x[d] = PF(a.pa, PX + d, slot);
-            v[d] = PF(a.pa, VX + d, slot);
-          }
-          m = PF(a.pa, MASS, slot);
-          const double nvol = -PF(a.pa, VOL, slot);
-          if (DIM == 3) {
-            for (int e = 0; e < 6; e++) stg[wave][2 * NW + 1 + DIM + e][lane] = nvol * PF(a.pa, SXX + e, slot);
-          } else {
-            stg[wave][2 * NW + 1 + DIM + 0][lane] = nvol * PF(a.pa, SXX, slot);
-            stg[wave][2 * NW + 1 + DIM + 1][lane] = nvol * PF(a.pa, SYY, slot);
-            stg[wave][2 * NW + 1 + DIM + 2][lane] = nvol * PF(a.pa, SXY, slot);
-          }
-        }
         int cl[3] = {0, 0, 0};
-        const int org[3] = {ox, oy, oz};
+#pragma unroll
         for (int d = 0; d < DIM; d++) {
-          const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
+          const int c = cell_of_clamped(in.x[d], a.lo[d], a.inv_h, a.n[d]);
           cl[d] = c - org[d];
           double w[S1], dw[S1];
-          weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
-          for (int q = 0; q < S1; q++) {
-            stg[wave][d * S1 + q][lane] = w[q];
-            stg[wave][NW + d * S1 + q][lane] = dw[q];
-          }
+          weights_1d<S1>(in.x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
+#pragma unroll
+          for (int q = 0; q < S1; q++) stw[wave][d * S1 + q][lane] = make_double2(w[q], dw[q]);
         }
-        stg_cell[wave][lane] = (cl[0] * EY + cl[1]) * EZ + cl[2];
-        stg[wave][2 * NW][lane] = m;
-        for (int d = 0; d < DIM; d++) stg[wave][2 * NW + 1 + d][lane] = v[d];
+        stg_cell[wave][lane] = cl[0] * PSA + cl[1] * PSB + cl[2];
+        spay[wave][0][lane] = make_double2(in.m, in.v[0]);
+        spay[wave][1][lane] = make_double2(in.v[1], DIM == 3 ? in.v[DIM - 1] : 0.0);
+        if (!REMAP) {
+#pragma unroll
+          for (int e = 0; e < NSIG; e += 2)
+            spay[wave][2 + e / 2][lane] = make_double2(in.s[e], e + 1 < NSIG ? in.s[e + 1] : 0.0);
+        }
       }
       wave_sync();
+      const int cn = c0 + 4 * CH;
+      if (cn < npart && lane < min(CH, npart - cn)) p2g_load<DIM, REMAP>(a, p0 + cn + lane, in);
 
-      double acc[NF];
-      for (int f = 0; f < NF; f++) acc[f] = 0;
-      int cur = -1;
-      const int sofs = DIM == 3 ? (sa * EY + sb) * EZ + sc : sa * EY + sb;
-      for (int r = 0; r < RUN; r++) {
-        const int j = g * RUN + r;
-        if (j >= cnt) break;
-        const int cell = stg_cell[wave][j];
-        if (cell != cur) {
-          if (cur >= 0)
-            for (int f = 0; f < NF; f++) atomicAdd(&blk[f * BN + cur + sofs], acc[f]);
-          for (int f = 0; f < NF; f++) acc[f] = 0;
-          cur = cell;
-        }
-        const double wx = stg[wave][sa][j], wy = stg[wave][S1 + sb][j];
-        const double dx = stg[wave][NW + sa][j], dy = stg[wave][NW + S1 + sb][j];
-        double w, g0, g1, g2 = 0;
-        if (DIM == 3) {
-          const double wz = stg[wave][2 * S1 + sc][j], dz = stg[wave][NW + 2 * S1 + sc][j];
-          const double wxy = wx * wy;
-          w = wxy * wz;
-          g0 = dx * wy * wz;
-          g1 = wx * dy * wz;
-          g2 = wxy * dz;
-        } else {
-          w = wx * wy;
-          g0 = dx * wy;
-          g1 = wx * dy;
-        }
-        const double m = stg[wave][2 * NW][j];
-        const double wm = w * m;
-        acc[0] += wm;
-        for (int d = 0; d < DIM; d++) acc[1 + d] = fma(wm, stg[wave][2 * NW + 1 + d][j], acc[1 + d]);
-        if (!REMAP) {
-          const double *ns = &stg[wave][2 * NW + 1 + DIM][0];
+      // Accumulate the group's run of particles; the LDS operands of
+      // particle j+1 are loaded before the arithmetic of particle j.
+      const int jb = g * RUN, je = min(jb + RUN, cnt);
+      if (jb < je) {
+        double acc[NF];
+        for (int f = 0; f < NF; f++) acc[f] = 0;
+        constexpr int NS2 = REMAP ? 0 : (NSIG + 1) / 2;
+        int cur = stg_cell[wave][jb];
+        double2 px = stw[wave][sa][jb], py = stw[wave][S1 + sb][jb];
+        double2 pz = DIM == 3 ? stw[wave][2 * S1 + sc][jb] : make_double2(1.0, 0.0);
+        double2 mv0 = spay[wave][0][jb], v12 = spay[wave][1][jb];
+        double2 sg[NS2 > 0 ? NS2 : 1];
+        for (int e = 0; e < NS2; e++) sg[e] = spay[wave][2 + e][jb];
+        for (int j = jb; j < je; j++) {
+          const int jn = j + 1 < je ? j + 1 : j;
+          const int celln = stg_cell[wave][jn];
+          const double2 pxn = stw[wave][sa][jn], pyn = stw[wave][S1 + sb][jn];
+          const double2 pzn = DIM == 3 ? stw[wave][2 * S1 + sc][jn] : make_double2(1.0, 0.0);
+          const double2 mv0n = spay[wave][0][jn], v12n = spay[wave][1][jn];
+          double2 sgn[NS2 > 0 ? NS2 : 1];
+          for (int e = 0; e < NS2; e++) sgn[e] = spay[wave][2 + e][jn];
+
+          double w, g0, g1, g2 = 0;
           if (DIM == 3) {
-            // f_d += sum_e (-vol sigma)_de g_e ; sym order xx,yy,zz,xy,xz,yz
-            const double sxx = ns[0 * CH + j], syy = ns[1 * CH + j], szz = ns[2 * CH + j];
-            const double sxy = ns[3 * CH + j], sxz = ns[4 * CH + j], syz = ns[5 * CH + j];
-            acc[4] = fma(sxz, g2, fma(sxy, g1, fma(sxx, g0, acc[4])));
-            acc[5] = fma(syz, g2, fma(syy, g1, fma(sxy, g0, acc[5])));
-            acc[6] = fma(szz, g2, fma(syz, g1, fma(sxz, g0, acc[6])));
+            const double wxy = px.x * py.x;
+            w = wxy * pz.x;
+            g0 = px.y * py.x * pz.x;
+            g1 = px.x * py.y * pz.x;
+            g2 = wxy * pz.y;
           } else {
-            const double sxx = ns[0 * CH + j], syy = ns[1 * CH + j], sxy = ns[2 * CH + j];
-            acc[3] = fma(sxy, g1, fma(sxx, g0, acc[3]));
-            acc[4] = fma(syy, g1, fma(sxy, g0, acc[4]));
+            w = px.x * py.x;
+            g0 = px.y * py.x;
+            g1 = px.x * py.y;
           }
+          const double wm = w * mv0.x;
+          acc[0] += wm;
+          acc[1] = fma(wm, mv0.y, acc[1]);
+          acc[2] = fma(wm, v12.x, acc[2]);
+          if (DIM == 3) acc[3] = fma(wm, v12.y, acc[3]);
+          if (!REMAP) {
+            if (DIM == 3) {
+              // f_d += sum_e (-vol sigma)_de g_e ; sym order xx,yy,zz,xy,xz,yz
+              const double sxx = sg[0].x, syy = sg[0].y, szz = sg[1].x, sxy = sg[1].y,
+                           sxz = sg[2 % (NS2 > 0 ? NS2 : 1)].x, syz = sg[2 % (NS2 > 0 ? NS2 : 1)].y;
+              acc[4] = fma(sxz, g2, fma(sxy, g1, fma(sxx, g0, acc[4])));
+              acc[5] = fma(syz, g2, fma(syy, g1, fma(sxy, g0, acc[5])));
+              acc[6] = fma(szz, g2, fma(syz, g1, fma(sxz, g0, acc[6])));
+            } else {
+              const double sxx = sg[0].x, syy = sg[0].y, sxy = sg[1 % (NS2 > 0 ? NS2 : 1)].x;
+              acc[3] = fma(sxy, g1, fma(sxx, g0, acc[3]));
+              acc[4] = fma(syy, g1, fma(sxy, g0, acc[4]));
+            }
+          }
+          (void)g2;
+          if (j + 1 < je && celln != cur) {
+            for (int f = 0; f < NF; f++) {
+              atomicAdd(&blk[f * PBN + cur + sofs], acc[f]);
+              acc[f] = 0;
+            }
+            cur = celln;
+          }
+          px = pxn;
+          py = pyn;
+          pz = pzn;
+          mv0 = mv0n;
+          v12 = v12n;
+          for (int e = 0; e < NS2; e++) sg[e] = sgn[e];
         }
-        (void)g2;
+        for (int f = 0; f < NF; f++) atomicAdd(&blk[f * PBN + cur + sofs], acc[f]);
       }
-      if (cur >= 0)
-        for (int f = 0; f < NF; f++) atomicAdd(&blk[f * BN + cur + sofs], acc[f]);
       wave_sync();
     }
     __syncthreads();
@@ -290,7 +354,8 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
       if (sf == 0) bf = 0;
       else if (sf <= 3) bf = sf <= DIM ? sf : -1;
       else bf = (sf - 3) <= DIM ? DIM + (sf - 3) : -1;
-      dst[q] = bf >= 0 ? blk[bf * BN + o] : 0.0;
+      const int bz = o % EZ, by = (o / EZ) % EY, bx = o / (EZ * EY);
+      dst[q] = bf >= 0 ? blk[bf * PBN + bx * PSA + by * PSB + bz] : 0.0;
     }
     __syncthreads();
   }
@@ -923,7 +988,7 @@ __global__ void k_cells(KArgs a, int *ijk) {
 // tile configurations and launchers
 // ---------------------------------------------------------------------------
 template <int DIM, int S1> struct Tiles;
-template <> struct Tiles<3, 4> { static constexpr int TX = 4, TY = 4, TZ = 4, CH = 32; };
+template <> struct Tiles<3, 4> { static constexpr int TX = 4, TY = 4, TZ = 4, CH = 24; };
 template <> struct Tiles<3, 2> { static constexpr int TX = 4, TY = 4, TZ = 4, CH = 32; };
 template <> struct Tiles<2, 4> { static constexpr int TX = 8, TY = 8, TZ = 1, CH = 64; };
 template <> struct Tiles<2, 2> { static constexpr int TX = 8, TY = 8, TZ = 1, CH = 64; };
@@ -987,7 +1052,7 @@ hipError_t launch_sort(kmpm_ctx *c) {
 
 hipError_t launch_p2g(kmpm_ctx *c, int remap) {
   if (c->a.np == 0) return hipSuccess;
-  const int g = persistent_grid(c, 2);
+  const int g = persistent_grid(c, 8);
   DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
     if (remap)
@@ -1036,7 +1101,7 @@ hipError_t launch_grid_clean(kmpm_ctx *c) {
 
 hipError_t launch_g2p_usl(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
-  const int g = persistent_grid(c, 4);
+  const int g = persistent_grid(c, 8);
   DISPATCH(c, MC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
     hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256), 0,
@@ -1047,7 +1112,7 @@ hipError_t launch_g2p_usl(kmpm_ctx *c) {
 
 hipError_t launch_g2p_musl_a(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
-  const int g = persistent_grid(c, 4);
+  const int g = persistent_grid(c, 8);
   DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
     hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
@@ -1058,7 +1123,7 @@ hipError_t launch_g2p_musl_a(kmpm_ctx *c) {
 
 hipError_t launch_stress_musl_b(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
-  const int g = persistent_grid(c, 4);
+  const int g = persistent_grid(c, 8);
   DISPATCH(c, MC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
     hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256),
