@@ -36,7 +36,11 @@ enum {
   KMPM_ERR_JACOBIAN = 11,           /* J<=0 with damage<1, solid.cpp:1208-1215 */
   KMPM_ERR_DTCFL = 12,              /* dtCFL NaN or 0, ulmpm.cpp:535-544 */
   KMPM_ERR_WAVE_SPEED = 13,         /* NaN wave speed, solid.cpp:1387-1398 */
-  KMPM_ERR_GRID = 14                /* grid too small for the stencil */
+  KMPM_ERR_GRID = 14,               /* grid too small for the stencil */
+  KMPM_ERR_FIXED_POINT = 15         /* mixed path: a node sum left the range of
+                                       its 64-bit fixed-point scale (no
+                                       counterpart in the reference, which
+                                       accumulates in FP64) */
 };
 
 /* ---- options ----------------------------------------------------------- */

@@ -38,11 +38,12 @@ static hipError_t dalloc(T **p, size_t n) {
 }
 
 static void free_particle_buffers(kmpm_ctx *c) {
-  void *ptrs[] = {c->d_pa, c->d_pb, c->d_xo, c->d_ia, c->d_ib, c->d_key, c->d_rank,
-                  c->d_order_tmp, c->d_order, c->d_pmask};
+  void *ptrs[] = {c->d_pa, c->d_pb, c->d_fa, c->d_fb, c->d_xo, c->d_ia, c->d_ib, c->d_key,
+                  c->d_rank, c->d_order_tmp, c->d_order, c->d_pmask};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_pa = c->d_pb = c->d_xo = nullptr;
+  c->d_fa = c->d_fb = nullptr;
   c->d_ia = c->d_ib = c->d_key = c->d_rank = c->d_order_tmp = c->d_order = nullptr;
   c->d_pmask = nullptr;
   free(c->h_ptag);
@@ -59,6 +60,8 @@ static void refresh_args(kmpm_ctx *c) {
   KArgs &a = c->a;
   a.pa = c->d_pa;
   a.pb = c->d_pb;
+  a.fa = c->d_fa;
+  a.fb = c->d_fb;
   a.ia = c->d_ia;
   a.ib = c->d_ib;
   a.xo = c->d_xo;
@@ -70,6 +73,7 @@ static void refresh_args(kmpm_ctx *c) {
 
 static void swap_state(kmpm_ctx *c) {
   std::swap(c->d_pa, c->d_pb);
+  std::swap(c->d_fa, c->d_fb);
   std::swap(c->d_ia, c->d_ib);
   refresh_args(c);
 }
@@ -94,6 +98,58 @@ static void tend(kmpm_ctx *c) {
   (void)hipEventRecord((*c->ev_pool)[k], c->stream);
 }
 
+// Mixed layout <-> the FP64 field layout used on the host.
+static void split_mixed(const std::vector<double> &hd, int cap, std::vector<double> &dd,
+                        std::vector<float> &ff) {
+  for (int f = 0; f < NDF; f++)
+    for (int i = 0; i < cap; i++) {
+      const double v = hd[(size_t)f * cap + i];
+      if (mx_is_double(f)) dd[(size_t)mx_dslot(f) * cap + i] = v;
+      else ff[(size_t)mx_fslot(f) * cap + i] = (float)v;
+    }
+}
+static void join_mixed(const std::vector<double> &dd, const std::vector<float> &ff, int cap,
+                       std::vector<double> &hd) {
+  for (int f = 0; f < NDF; f++)
+    for (int i = 0; i < cap; i++)
+      hd[(size_t)f * cap + i] = mx_is_double(f) ? dd[(size_t)mx_dslot(f) * cap + i]
+                                                : (double)ff[(size_t)mx_fslot(f) * cap + i];
+}
+
+// Fixed-point scales of the mixed path's node sums: the largest power of
+// two that keeps one particle's contribution below 2^FX_CONTRIB_BITS, from
+// bounds taken at upload with a factor 4 of headroom:
+//   mass      m_max                      (weights are at most 1)
+//   momentum  m_max * v_ref,  v_ref = 4 max(|v|_max, max wave speed of the materials)
+//   force     vol_ref * sigma_ref / h,   vol_ref = 4 vol_max,
+//             sigma_ref = 4 max(|sigma|_max, max (K + 4/3 G))
+// (|grad w| <= 1/h for both shape families).
+static void set_fixed_point_scales(kmpm_ctx *c, const std::vector<double> &hd, int cap,
+                                   int64_t np) {
+  double mmax = 0, vmax = 0, smax = 0, volmax = 0;
+  for (int64_t i = 0; i < np; i++) {
+    mmax = std::max(mmax, std::fabs(hd[(size_t)MASS * cap + i]));
+    volmax = std::max(volmax, std::fabs(hd[(size_t)VOL * cap + i]));
+    for (int d = 0; d < 3; d++) vmax = std::max(vmax, std::fabs(hd[(size_t)(VX + d) * cap + i]));
+    for (int e = 0; e < 6; e++) smax = std::max(smax, std::fabs(hd[(size_t)(SXX + e) * cap + i]));
+  }
+  double cmax = 0, modmax = 0;
+  for (int k = 0; k < c->nmat; k++) {
+    const kmpm_material &m = c->mats_host[k];
+    cmax = std::max(cmax, m.signal_velocity);
+    modmax = std::max(modmax, std::fabs(m.K) + FOUR_THIRD * std::fabs(m.G));
+  }
+  const double vref = 4 * std::max(vmax, cmax);
+  const double sref = 4 * std::max(smax, modmax);
+  const double bound[FX_N] = {mmax, mmax * vref, 4 * volmax * sref * c->a.inv_h};
+  for (int k = 0; k < FX_N; k++) {
+    int e = 0;
+    std::frexp(bound[k] > 0 && std::isfinite(bound[k]) ? bound[k] : 1.0, &e);   // bound < 2^e
+    c->a.fxs[k] = std::ldexp(1.0, FX_CONTRIB_BITS - e);
+    c->a.fxi[k] = std::ldexp(1.0, e - FX_CONTRIB_BITS);
+  }
+}
+
 extern "C" {
 
 int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
@@ -103,7 +159,8 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   if (cfg->method != KMPM_METHOD_ULMPM) return KMPM_ERR_ARG;
   if (cfg->scheme != KMPM_SCHEME_USL && cfg->scheme != KMPM_SCHEME_MUSL) return KMPM_ERR_ARG;
   if (cfg->shape != KMPM_SHAPE_LINEAR && cfg->shape != KMPM_SHAPE_CUBIC_SPLINE) return KMPM_ERR_ARG;
-  if (cfg->precision != KMPM_PRECISION_FP64) return KMPM_ERR_ARG;
+  if (cfg->precision != KMPM_PRECISION_FP64 && cfg->precision != KMPM_PRECISION_MIXED)
+    return KMPM_ERR_ARG;
   if (cfg->dimension != 2 && cfg->dimension != 3) return KMPM_ERR_ARG;
   int64_t n3[3];
   int st = kmpm_grid_size(cfg, n3);
@@ -131,6 +188,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   }
   KArgs &a = c->a;
   a.dim = cfg->dimension;
+  a.prec = cfg->precision;
   a.S1 = S1;
   a.scheme = cfg->scheme;
   for (int d = 0; d < 3; d++) {
@@ -291,8 +349,13 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   KArgs &a = c->a;
   a.np = (int)np;
   a.cap = cap;
-  HIPCHK(dalloc(&c->d_pa, (size_t)NDF * cap));
-  HIPCHK(dalloc(&c->d_pb, (size_t)NDF * cap));
+  const bool mixed = a.prec == KMPM_PRECISION_MIXED;
+  HIPCHK(dalloc(&c->d_pa, (size_t)(mixed ? MX_ND : NDF) * cap));
+  HIPCHK(dalloc(&c->d_pb, (size_t)(mixed ? MX_ND : NDF) * cap));
+  if (mixed) {
+    HIPCHK(dalloc(&c->d_fa, (size_t)MX_NF * cap));
+    HIPCHK(dalloc(&c->d_fb, (size_t)MX_NF * cap));
+  }
   HIPCHK(dalloc(&c->d_xo, (size_t)3 * cap));
   HIPCHK(dalloc(&c->d_ia, (size_t)NIF * cap));
   HIPCHK(dalloc(&c->d_ib, (size_t)NIF * cap));
@@ -342,7 +405,16 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
     hi[(size_t)PID * cap + i] = (int)i;
     hi[(size_t)MATI * cap + i] = p->mat[i];
   }
-  HIPCHK(hipMemcpy(c->d_pa, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (mixed) {
+    std::vector<double> dd((size_t)MX_ND * cap);
+    std::vector<float> ff((size_t)MX_NF * cap);
+    split_mixed(hd, cap, dd, ff);
+    HIPCHK(hipMemcpy(c->d_pa, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_fa, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice));
+    set_fixed_point_scales(c, hd, cap, np);
+  } else {
+    HIPCHK(hipMemcpy(c->d_pa, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMemcpy(c->d_ia, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_pmask, c->h_pmask, np * sizeof(uint32_t), hipMemcpyHostToDevice));
   refresh_args(c);
@@ -358,11 +430,24 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
   const int cap = c->a.cap;
   std::vector<double> hd((size_t)NDF * cap);
   std::vector<int> hi((size_t)NIF * cap);
-  HIPCHK(hipMemcpyAsync(hd.data(), c->d_pa, hd.size() * sizeof(double), hipMemcpyDeviceToHost,
-                        c->stream));
+  std::vector<double> dd;
+  std::vector<float> ff;
+  const bool mixed = c->a.prec == KMPM_PRECISION_MIXED;
+  if (mixed) {
+    dd.resize((size_t)MX_ND * cap);
+    ff.resize((size_t)MX_NF * cap);
+    HIPCHK(hipMemcpyAsync(dd.data(), c->d_pa, dd.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(ff.data(), c->d_fa, ff.size() * sizeof(float), hipMemcpyDeviceToHost,
+                          c->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(hd.data(), c->d_pa, hd.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  }
   HIPCHK(hipMemcpyAsync(hi.data(), c->d_ia, hi.size() * sizeof(int), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (mixed) join_mixed(dd, ff, cap, hd);
   auto D = [&](int f, int64_t i) { return hd[(size_t)f * cap + i]; };
   for (int64_t i = 0; i < np; i++) {
     const int64_t q = hi[(size_t)PID * cap + i];  // output index = upload index
@@ -569,17 +654,27 @@ int kmpm_reduce_energy(kmpm_ctx *c, uint32_t bit, double *ek, double *es) {
 int kmpm_download_grid(kmpm_ctx *c, double *mass, double *v, double *vu, int64_t nn) {
   if (!c || nn != c->a.nn) return KMPM_ERR_ARG;
   HIPCHK(launch_grid_clean(c));
-  std::vector<double> t(3 * nn);
-  if (mass) HIPCHK(hipMemcpyAsync(mass, c->d_gmass, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  // device layout [comp][node] -> reference layout [node][comp]
+  // device layout [comp][node] (FP32 values in the mixed path) -> reference
+  // layout [node][comp]
+  const bool mixed = c->a.prec == KMPM_PRECISION_MIXED;
+  const size_t esz = mixed ? sizeof(float) : sizeof(double);
+  std::vector<char> raw(3 * nn * esz);
+  auto val = [&](size_t k) {
+    return mixed ? (double)reinterpret_cast<const float *>(raw.data())[k]
+                 : reinterpret_cast<const double *>(raw.data())[k];
+  };
+  if (mass) {
+    HIPCHK(hipMemcpy(raw.data(), c->d_gmass, nn * esz, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nn; i++) mass[i] = val(i);
+  }
   double *dsts[2] = {v, vu};
   double *srcs[2] = {c->d_gv, c->d_gvu};
   for (int k = 0; k < 2; k++) {
     if (!dsts[k]) continue;
-    HIPCHK(hipMemcpy(t.data(), srcs[k], 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(raw.data(), srcs[k], 3 * nn * esz, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < nn; i++)
-      for (int d = 0; d < 3; d++) dsts[k][3 * i + d] = t[(size_t)d * nn + i];
+      for (int d = 0; d < 3; d++) dsts[k][3 * i + d] = val((size_t)d * nn + i);
   }
   return KMPM_OK;
 }

@@ -32,11 +32,15 @@ struct KArgs {
   const DevMat *mats;
   TileGeom tg;
   // particles
+  int prec;         // KMPM_PRECISION_*
   int np, cap;
-  double *pa;       // current state (NDF * cap)
+  double *pa;       // current state: FP64 NDF * cap, mixed MX_ND * cap
+  float *fa;        // current FP32 state (mixed: MX_NF * cap)
   int *ia;          // current int state (NIF * cap)
   double *pb;       // next state
+  float *fb;
   int *ib;
+  double fxs[FX_N], fxi[FX_N];  // fixed-point scales (powers of two) and inverses
   double *xo;       // MUSL: positions at the start of the step, sorted slots (3*cap)
   int *key;         // cell key per current slot
   int *rank;        // arrival rank within the cell
@@ -48,9 +52,10 @@ struct KArgs {
   int *tile_list;   // active tiles (by list slot)
   int *tile_slot;   // tile -> list slot or -1
   int *dil_list;    // tiles whose owned nodes get contributions
-  double *scratch;  // partial node blocks [slot][field][bn]
+  double *scratch;  // partial node blocks [slot][field][bn] (int64 in the mixed path)
   int scratch_slots;
-  // grid (tag order, SoA per component)
+  // grid (tag order, SoA per component); the mixed path stores FP32 values
+  // in the same allocations
   double *gmass, *gv, *gvu;   // gv/gvu: 3 * nn
   uint32_t *gmask;
   // fixes
@@ -79,6 +84,7 @@ struct kmpm_ctx {
   // device allocations
   kmpm::DevMat *d_mats;
   double *d_pa, *d_pb, *d_xo;
+  float *d_fa, *d_fb;
   int *d_ia, *d_ib;
   int *d_key, *d_rank, *d_order_tmp, *d_order;
   int *d_cell_count, *d_cell_start;

@@ -27,6 +27,28 @@ enum DField {
 // Int fields: PID = index into the static (upload-order) arrays, MAT = material.
 enum IField { PID = 0, MATI, NIF };
 
+// Mixed-precision layout (KMPM_PRECISION_MIXED): position and deformation
+// gradient stay FP64 (a double block of 12 fields: x, y, z, F00..F22), every
+// other field is FP32 (a float block of NDF - 12 fields in DField order).
+constexpr int MX_ND = 12;
+constexpr int MX_NF = NDF - MX_ND;
+__host__ __device__ constexpr bool mx_is_double(int f) { return f <= PZ || (f >= F00 && f <= F22); }
+__host__ __device__ constexpr int mx_dslot(int f) { return f <= PZ ? f : f - F00 + 3; }
+__host__ __device__ constexpr int mx_fslot(int f) { return f < F00 ? f - VX : f - SXX + 3; }
+
+// Real type of the bulk state and of the transfer arithmetic per precision.
+template <int PREC> struct Real { using T = double; using T2 = double2; };
+template <> struct Real<KMPM_PRECISION_MIXED> { using T = float; using T2 = float2; };
+
+// Fixed-point grid accumulation (mixed path): scaled node sums are 64-bit
+// integers, so their value does not depend on the order of the adds.
+enum FxField { FX_MASS = 0, FX_MOM, FX_FORCE, FX_N };
+// A single flush (a run of particles of one cell) may reach 2^54 in scaled
+// units before the step reports KMPM_ERR_FIXED_POINT; the scales put one
+// particle's bound at 2^46, leaving room for 2^9 flush sums per node.
+constexpr double FX_FLUSH_LIMIT = 18014398509481984.0;   // 2^54
+constexpr int FX_CONTRIB_BITS = 46;
+
 // ---- material table entry ------------------------------------------------
 struct DevMat {
   int type, eos, strength, damage;
@@ -58,6 +80,8 @@ struct DevScalars {
   int64_t err_pid;                // pid of the first offending particle
   int tile_count;                 // active tiles this step
   int dil_count;                  // tiles whose owned nodes receive contributions
+  unsigned long long surr_evals;      // particle updates that used a surrogate network
+  unsigned long long surr_fallbacks;  // particle updates outside the surrogate's envelope
 };
 
 constexpr double SQRT_3_OVER_2 = 1.224744871;                            // strength_jc.cpp:29

@@ -60,6 +60,7 @@ const char *kmpm_status_string(int s) {
     case KMPM_ERR_DTCFL: return "Error: dtCFL == 0 or NaN";
     case KMPM_ERR_WAVE_SPEED: return "Error: max_p_wave_speed is nan";
     case KMPM_ERR_GRID: return "grid does not support the stencil on one device";
+    case KMPM_ERR_FIXED_POINT: return "node sum exceeded the fixed-point range of the mixed path";
     default: return "unknown status";
   }
 }

@@ -15,6 +15,8 @@
 //            applied for free)
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "kmpm_constitutive.h"
 #include "kmpm_ctx.h"
 
@@ -22,6 +24,29 @@ namespace kmpm {
 
 #define PF(base, f, i) ((base)[(size_t)(f) * (size_t)a.cap + (size_t)(i)])
 #define IFLD(base, f, i) ((base)[(size_t)(f) * (size_t)a.cap + (size_t)(i)])
+
+// Particle field access for both state layouts.  Values travel as double;
+// the mixed layout widens its FP32 fields on load and rounds them on store.
+template <int PREC>
+__device__ inline double ldp(const KArgs &a, const double *d, const float *f, int fld, int i) {
+  if (PREC == KMPM_PRECISION_FP64) return d[(size_t)fld * a.cap + i];
+  if (mx_is_double(fld)) return d[(size_t)mx_dslot(fld) * a.cap + i];
+  return (double)f[(size_t)mx_fslot(fld) * a.cap + i];
+}
+template <int PREC>
+__device__ inline void stp(const KArgs &a, double *d, float *f, int fld, int i, double v) {
+  if (PREC == KMPM_PRECISION_FP64) d[(size_t)fld * a.cap + i] = v;
+  else if (mx_is_double(fld)) d[(size_t)mx_dslot(fld) * a.cap + i] = v;
+  else f[(size_t)mx_fslot(fld) * a.cap + i] = (float)v;
+}
+#define LDA(f, i) ldp<PREC>(a, a.pa, a.fa, (f), (i))
+#define LDB(f, i) ldp<PREC>(a, a.pb, a.fb, (f), (i))
+#define STB(f, i, v) stp<PREC>(a, a.pb, a.fb, (f), (i), (v))
+
+// Grid arrays hold R = Real<PREC>::T values in the double allocations.
+template <int PREC> __device__ inline typename Real<PREC>::T *gptr(double *p) {
+  return reinterpret_cast<typename Real<PREC>::T *>(p);
+}
 
 __device__ inline void set_error(DevScalars *sc, int code, int64_t pid) {
   if (atomicCAS(&sc->error, 0, code) == 0) sc->err_pid = pid;
@@ -31,6 +56,14 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Scaled flush of a P2G register sum into a 64-bit fixed-point LDS node sum.
+__device__ inline void fx_add(unsigned long long *dst, double val, double scale, DevScalars *sc) {
+  const double q = val * scale;
+  if (!(fabs(q) < FX_FLUSH_LIMIT)) set_error(sc, KMPM_ERR_FIXED_POINT, -1);
+  const long long iv = fabs(q) < FX_FLUSH_LIMIT ? __double2ll_rn(q) : 0;
+  atomicAdd(dst, (unsigned long long)iv);
 }
 
 __device__ inline int cell_of_clamped(double x, double lo, double inv_h, int n) {
@@ -157,41 +190,49 @@ __global__ void k_dilate(KArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int SCR_FIELDS = 7;   // scratch slot layout: mass, momentum[3], force[3]
 
-// Raw particle data a P2G lane loads for one particle.
-template <int DIM, bool REMAP>
+// Raw particle data a P2G lane loads for one particle (R: the transfer type).
+template <int DIM, bool REMAP, class R>
 struct P2GIn {
-  double x[DIM], v[DIM], m;
-  double s[REMAP ? 1 : (DIM == 3 ? 6 : 3)];   // -vol * sigma (xx,yy,zz,xy,xz,yz | xx,yy,xy)
+  double x[DIM];
+  R v[DIM], m;
+  R s[REMAP ? 1 : (DIM == 3 ? 6 : 3)];   // -vol * sigma (xx,yy,zz,xy,xz,yz | xx,yy,xy)
 };
 
-template <int DIM, bool REMAP>
-__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP> &in) {
+template <int DIM, bool REMAP, int PREC, class R>
+__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, R> &in) {
   if (REMAP) {
     for (int d = 0; d < DIM; d++) {
       in.x[d] = a.xo[(size_t)d * a.cap + j];
-      in.v[d] = PF(a.pb, VX + d, j);
+      in.v[d] = (R)LDB(VX + d, j);
     }
-    in.m = PF(a.pb, MASS, j);
+    in.m = (R)LDB(MASS, j);
   } else {
     const int slot = a.order[j];
     for (int d = 0; d < DIM; d++) {
-      in.x[d] = PF(a.pa, PX + d, slot);
-      in.v[d] = PF(a.pa, VX + d, slot);
+      in.x[d] = LDA(PX + d, slot);
+      in.v[d] = (R)LDA(VX + d, slot);
     }
-    in.m = PF(a.pa, MASS, slot);
-    const double nvol = -PF(a.pa, VOL, slot);
+    in.m = (R)LDA(MASS, slot);
+    const R nvol = (R)-LDA(VOL, slot);
     if (DIM == 3) {
-      for (int e = 0; e < 6; e++) in.s[e] = nvol * PF(a.pa, SXX + e, slot);
+      for (int e = 0; e < 6; e++) in.s[e] = nvol * (R)LDA(SXX + e, slot);
     } else {
-      in.s[0] = nvol * PF(a.pa, SXX, slot);
-      in.s[1] = nvol * PF(a.pa, SYY, slot);
-      in.s[2] = nvol * PF(a.pa, SXY, slot);
+      in.s[0] = nvol * (R)LDA(SXX, slot);
+      in.s[1] = nvol * (R)LDA(SYY, slot);
+      in.s[2] = nvol * (R)LDA(SXY, slot);
     }
   }
 }
 
-template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP>
+template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP, int PREC>
 __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
+  // FP64: double transfer arithmetic, double LDS node sums.  Mixed: float
+  // transfer arithmetic, register sums flushed into 64-bit fixed-point LDS
+  // node sums (order-independent, so the scatter is deterministic).
+  using R = typename Real<PREC>::T;
+  using R2 = typename Real<PREC>::T2;
+  constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
+  using BT = typename std::conditional<FX, unsigned long long, double>::type;
   constexpr int S = DIM == 3 ? S1 * S1 * S1 : S1 * S1;
   constexpr int G = 64 / S;
   constexpr int RUN = CH / G;
@@ -212,9 +253,9 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   constexpr int PBN = EX * PSA;
   static_assert(PSA >= EY * PSB, "block stride");
 
-  __shared__ double blk[NF * PBN];
-  __shared__ double2 stw[4][NW][CH + 1];   // +1: the S1 rows a lane group reads fall in distinct banks
-  __shared__ double2 spay[4][NP2][CH];
+  __shared__ BT blk[NF * PBN];
+  __shared__ R2 stw[4][NW][CH + 1];   // +1: the S1 rows a lane group reads fall in distinct banks
+  __shared__ R2 spay[4][NP2][CH];
   __shared__ int stg_cell[4][CH];
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -242,9 +283,10 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
     const int npart = p1 - p0;
     // The particle data of the wave's next chunk is loaded into registers
     // while the current chunk is accumulated (hides HBM latency).
-    P2GIn<DIM, REMAP> in;
+    P2GIn<DIM, REMAP, R> in;
     int c0 = wave * CH;
-    if (c0 < npart && lane < min(CH, npart - c0)) p2g_load<DIM, REMAP>(a, p0 + c0 + lane, in);
+    if (c0 < npart && lane < min(CH, npart - c0))
+      p2g_load<DIM, REMAP, PREC>(a, p0 + c0 + lane, in);
     for (; c0 < npart; c0 += 4 * CH) {
       const int cnt = min(CH, npart - c0);
       if (lane < cnt) {
@@ -256,95 +298,115 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
           double w[S1], dw[S1];
           weights_1d<S1>(in.x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
 #pragma unroll
-          for (int q = 0; q < S1; q++) stw[wave][d * S1 + q][lane] = make_double2(w[q], dw[q]);
+          for (int q = 0; q < S1; q++) stw[wave][d * S1 + q][lane] = R2{(R)w[q], (R)dw[q]};
         }
         stg_cell[wave][lane] = cl[0] * PSA + cl[1] * PSB + cl[2];
-        spay[wave][0][lane] = make_double2(in.m, in.v[0]);
-        spay[wave][1][lane] = make_double2(in.v[1], DIM == 3 ? in.v[DIM - 1] : 0.0);
+        spay[wave][0][lane] = R2{in.m, in.v[0]};
+        spay[wave][1][lane] = R2{in.v[1], DIM == 3 ? in.v[DIM - 1] : (R)0};
         if (!REMAP) {
 #pragma unroll
           for (int e = 0; e < NSIG; e += 2)
-            spay[wave][2 + e / 2][lane] = make_double2(in.s[e], e + 1 < NSIG ? in.s[e + 1] : 0.0);
+            spay[wave][2 + e / 2][lane] = R2{in.s[e], e + 1 < NSIG ? in.s[e + 1] : (R)0};
         }
       }
       wave_sync();
       const int cn = c0 + 4 * CH;
-      if (cn < npart && lane < min(CH, npart - cn)) p2g_load<DIM, REMAP>(a, p0 + cn + lane, in);
+      if (cn < npart && lane < min(CH, npart - cn))
+        p2g_load<DIM, REMAP, PREC>(a, p0 + cn + lane, in);
 
-      // Accumulate the group's run of particles; the LDS operands of
-      // particle j+1 are loaded before the arithmetic of particle j.
+      // Accumulate the group's run of particles; the LDS operands of the
+      // next particle are loaded before the arithmetic of the current one
+      // (two operand sets used alternately, no register copies).
       const int jb = g * RUN, je = min(jb + RUN, cnt);
       if (jb < je) {
-        double acc[NF];
-        for (int f = 0; f < NF; f++) acc[f] = 0;
         constexpr int NS2 = REMAP ? 0 : (NSIG + 1) / 2;
-        int cur = stg_cell[wave][jb];
-        double2 px = stw[wave][sa][jb], py = stw[wave][S1 + sb][jb];
-        double2 pz = DIM == 3 ? stw[wave][2 * S1 + sc][jb] : make_double2(1.0, 0.0);
-        double2 mv0 = spay[wave][0][jb], v12 = spay[wave][1][jb];
-        double2 sg[NS2 > 0 ? NS2 : 1];
-        for (int e = 0; e < NS2; e++) sg[e] = spay[wave][2 + e][jb];
-        for (int j = jb; j < je; j++) {
-          const int jn = j + 1 < je ? j + 1 : j;
-          const int celln = stg_cell[wave][jn];
-          const double2 pxn = stw[wave][sa][jn], pyn = stw[wave][S1 + sb][jn];
-          const double2 pzn = DIM == 3 ? stw[wave][2 * S1 + sc][jn] : make_double2(1.0, 0.0);
-          const double2 mv0n = spay[wave][0][jn], v12n = spay[wave][1][jn];
-          double2 sgn[NS2 > 0 ? NS2 : 1];
-          for (int e = 0; e < NS2; e++) sgn[e] = spay[wave][2 + e][jn];
-
-          double w, g0, g1, g2 = 0;
+        struct Ops {
+          int cell;
+          R2 px, py, pz, mv0, v12;
+          R2 sg[NS2 > 0 ? NS2 : 1];
+        };
+        auto load = [&](int j, Ops &o) {
+          o.cell = stg_cell[wave][j];
+          o.px = stw[wave][sa][j];
+          o.py = stw[wave][S1 + sb][j];
+          o.pz = DIM == 3 ? stw[wave][2 * S1 + sc][j] : R2{(R)1, (R)0};
+          o.mv0 = spay[wave][0][j];
+          o.v12 = spay[wave][1][j];
+          for (int e = 0; e < NS2; e++) o.sg[e] = spay[wave][2 + e][j];
+        };
+        R acc[NF];
+        for (int f = 0; f < NF; f++) acc[f] = 0;
+        auto compute = [&](const Ops &o) {
+          R w, g0, g1, g2 = 0;
           if (DIM == 3) {
-            const double wxy = px.x * py.x;
-            w = wxy * pz.x;
-            g0 = px.y * py.x * pz.x;
-            g1 = px.x * py.y * pz.x;
-            g2 = wxy * pz.y;
+            const R wxy = o.px.x * o.py.x;
+            w = wxy * o.pz.x;
+            g0 = o.px.y * o.py.x * o.pz.x;
+            g1 = o.px.x * o.py.y * o.pz.x;
+            g2 = wxy * o.pz.y;
           } else {
-            w = px.x * py.x;
-            g0 = px.y * py.x;
-            g1 = px.x * py.y;
+            w = o.px.x * o.py.x;
+            g0 = o.px.y * o.py.x;
+            g1 = o.px.x * o.py.y;
           }
-          const double wm = w * mv0.x;
+          const R wm = w * o.mv0.x;
           acc[0] += wm;
-          acc[1] = fma(wm, mv0.y, acc[1]);
-          acc[2] = fma(wm, v12.x, acc[2]);
-          if (DIM == 3) acc[3] = fma(wm, v12.y, acc[3]);
+          acc[1] = fma(wm, o.mv0.y, acc[1]);
+          acc[2] = fma(wm, o.v12.x, acc[2]);
+          if (DIM == 3) acc[3] = fma(wm, o.v12.y, acc[3]);
           if (!REMAP) {
+            constexpr int K = NS2 > 0 ? NS2 : 1;
             if (DIM == 3) {
               // f_d += sum_e (-vol sigma)_de g_e ; sym order xx,yy,zz,xy,xz,yz
-              const double sxx = sg[0].x, syy = sg[0].y, szz = sg[1].x, sxy = sg[1].y,
-                           sxz = sg[2 % (NS2 > 0 ? NS2 : 1)].x, syz = sg[2 % (NS2 > 0 ? NS2 : 1)].y;
+              const R sxx = o.sg[0].x, syy = o.sg[0].y, szz = o.sg[1 % K].x,
+                      sxy = o.sg[1 % K].y, sxz = o.sg[2 % K].x, syz = o.sg[2 % K].y;
               acc[4] = fma(sxz, g2, fma(sxy, g1, fma(sxx, g0, acc[4])));
               acc[5] = fma(syz, g2, fma(syy, g1, fma(sxy, g0, acc[5])));
               acc[6] = fma(szz, g2, fma(syz, g1, fma(sxz, g0, acc[6])));
             } else {
-              const double sxx = sg[0].x, syy = sg[0].y, sxy = sg[1 % (NS2 > 0 ? NS2 : 1)].x;
+              const R sxx = o.sg[0].x, syy = o.sg[0].y, sxy = o.sg[1 % K].x;
               acc[3] = fma(sxy, g1, fma(sxx, g0, acc[3]));
               acc[4] = fma(syy, g1, fma(sxy, g0, acc[4]));
             }
           }
           (void)g2;
-          if (j + 1 < je && celln != cur) {
-            for (int f = 0; f < NF; f++) {
-              atomicAdd(&blk[f * PBN + cur + sofs], acc[f]);
-              acc[f] = 0;
+        };
+        auto flush = [&](int cell) {
+          for (int f = 0; f < NF; f++) {
+            if constexpr (FX) {
+              const int k = f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE;
+              fx_add(&blk[f * PBN + cell + sofs], (double)acc[f], a.fxs[k], a.sc);
+            } else {
+              atomicAdd(&blk[f * PBN + cell + sofs], acc[f]);
             }
-            cur = celln;
+            acc[f] = 0;
           }
-          px = pxn;
-          py = pyn;
-          pz = pzn;
-          mv0 = mv0n;
-          v12 = v12n;
-          for (int e = 0; e < NS2; e++) sg[e] = sgn[e];
+        };
+        Ops A, B;
+        load(jb, A);
+        int cur = A.cell;
+        for (int j = jb;; j += 2) {
+          load(j + 1 < je ? j + 1 : j, B);
+          compute(A);
+          if (j + 1 >= je) break;
+          if (B.cell != cur) {
+            flush(cur);
+            cur = B.cell;
+          }
+          load(j + 2 < je ? j + 2 : j + 1, A);
+          compute(B);
+          if (j + 2 >= je) break;
+          if (A.cell != cur) {
+            flush(cur);
+            cur = A.cell;
+          }
         }
-        for (int f = 0; f < NF; f++) atomicAdd(&blk[f * PBN + cur + sofs], acc[f]);
+        flush(cur);
       }
       wave_sync();
     }
     __syncthreads();
-    double *dst = a.scratch + (size_t)li * SCR_FIELDS * BN;
+    BT *dst = reinterpret_cast<BT *>(a.scratch) + (size_t)li * SCR_FIELDS * BN;
     // blk fields: mass, mom[DIM], force[DIM] -> scratch fields mass, mom[3],
     // force[3]; components a 2D run does not have are written as zeros.
     const int f_lo = REMAP ? 1 : 0, f_hi = REMAP ? 4 : SCR_FIELDS;
@@ -355,7 +417,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
       else if (sf <= 3) bf = sf <= DIM ? sf : -1;
       else bf = (sf - 3) <= DIM ? DIM + (sf - 3) : -1;
       const int bz = o % EZ, by = (o / EZ) % EY, bx = o / (EZ * EY);
-      dst[q] = bf >= 0 ? blk[bf * PBN + bx * PSA + by * PSB + bz] : 0.0;
+      dst[q] = bf >= 0 ? blk[bf * PBN + bx * PSA + by * PSB + bz] : (BT)0;
     }
     __syncthreads();
   }
@@ -365,8 +427,12 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
 // grid: sum partial blocks, v = p/m, v_update, fixes (grid.cpp:448-466,
 // fix_body_force.cpp:117-146, fix_velocity_nodes.cpp:130-271)
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ>
+template <int DIM, int S1, int TX, int TY, int TZ, int PREC>
 __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
+  using R = typename Real<PREC>::T;
+  constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
+  using BT = typename std::conditional<FX, long long, double>::type;
+  R *gmass = gptr<PREC>(a.gmass), *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   constexpr int TZZ = DIM == 3 ? TZ : 1;
@@ -383,8 +449,8 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
     const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
     if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
     const int64_t tag = ((int64_t)i * a.n[1] + j) * a.n[2] + k;
-    double sum[SCR_FIELDS];
-    for (int f = 0; f < SCR_FIELDS; f++) sum[f] = 0;
+    BT isum[SCR_FIELDS];
+    for (int f = 0; f < SCR_FIELDS; f++) isum[f] = 0;
     for (int dx = -1; dx <= up; dx++)
       for (int dy = -1; dy <= up; dy++)
         for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? up : 0); dz++) {
@@ -396,25 +462,29 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
           const int bz = DIM == 3 ? k - (uz * TZ - off) : 0;
           if (bx < 0 || by < 0 || bz < 0 || bx >= EX || by >= EY || bz >= EZ) continue;
           const int idx = (bx * EY + by) * EZ + bz;
-          const double *src = a.scratch + (size_t)slot * SCR_FIELDS * BN + idx;
+          const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + idx;
           if (remap) {
-            for (int f = 1; f <= 3; f++) sum[f] += src[f * BN];
+            for (int f = 1; f <= 3; f++) isum[f] += src[f * BN];
           } else {
-            for (int f = 0; f < SCR_FIELDS; f++) sum[f] += src[f * BN];
+            for (int f = 0; f < SCR_FIELDS; f++) isum[f] += src[f * BN];
           }
         }
+    double sum[SCR_FIELDS];
+    for (int f = 0; f < SCR_FIELDS; f++)
+      sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
+                  : (double)isum[f];
     const uint32_t mask = a.gmask[tag];
     if (remap) {
       // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
       // node mass, then FixVelocityNodes::post_velocities_to_grid.
-      const double m = a.gmass[tag];
+      const double m = gmass[tag];
       double v[3];
       for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
       for (int fi = 0; fi < a.nfv; fi++)
         if (mask & a.fv[fi].bit)
           for (int d = 0; d < 3; d++)
             if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
-      for (int d = 0; d < 3; d++) a.gv[d * a.nn + tag] = v[d];
+      for (int d = 0; d < 3; d++) gv[d * a.nn + tag] = (R)v[d];
       continue;
     }
     const double m = sum[0];
@@ -432,18 +502,20 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
             vu[d] = a.fv[fi].val[d];
             v[d] = a.fv[fi].prev[d];
           }
-    a.gmass[tag] = m;
+    gmass[tag] = (R)m;
     for (int d = 0; d < 3; d++) {
-      a.gv[d * a.nn + tag] = v[d];
-      a.gvu[d * a.nn + tag] = vu[d];
+      gv[d * a.nn + tag] = (R)v[d];
+      gvu[d * a.nn + tag] = (R)vu[d];
     }
   }
 }
 
 // Grid state for download: nodes outside every touched tile hold zeros
 // (and their velocity-fix values), as the reference's reset leaves them.
-template <int DIM, int TX, int TY, int TZ>
+template <int DIM, int TX, int TY, int TZ, int PREC>
 __global__ void k_grid_clean(KArgs a) {
+  using R = typename Real<PREC>::T;
+  R *gmass = gptr<PREC>(a.gmass), *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int S1 = a.S1;
@@ -470,10 +542,10 @@ __global__ void k_grid_clean(KArgs a) {
             vu[d] = a.fv[fi].val[d];
             v[d] = a.fv[fi].prev[d];
           }
-    a.gmass[tag] = 0;
+    gmass[tag] = 0;
     for (int d = 0; d < 3; d++) {
-      a.gv[d * a.nn + tag] = v[d];
-      a.gvu[d * a.nn + tag] = vu[d];
+      gv[d * a.nn + tag] = (R)v[d];
+      gvu[d * a.nn + tag] = (R)vu[d];
     }
   }
 }
@@ -499,8 +571,11 @@ __device__ inline void block_max_to(double val, DevScalars *sc) {
 }
 
 // Load the tile's node block (velocities) into LDS: comps [nc][BN].
-template <int DIM, int S1, int TX, int TY, int TZ, int MODE>
-__device__ inline void load_node_block(const KArgs &a, double *nb, int ox, int oy, int oz) {
+template <int DIM, int S1, int TX, int TY, int TZ, int MODE, int PREC>
+__device__ inline void load_node_block(const KArgs &a, typename Real<PREC>::T *nb, int ox, int oy,
+                                       int oz) {
+  using R = typename Real<PREC>::T;
+  const R *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
   // MODE 0: v_update and v_update - v (USL / MUSL part A); MODE 1: v (MUSL part B)
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
@@ -512,29 +587,29 @@ __device__ inline void load_node_block(const KArgs &a, double *nb, int ox, int o
     const int64_t tag = ok ? ((int64_t)i * a.n[1] + j) * a.n[2] + k : 0;
     for (int d = 0; d < DIM; d++) {
       if (MODE == 0) {
-        const double vu = ok ? a.gvu[d * a.nn + tag] : 0.0;
-        const double v = ok ? a.gv[d * a.nn + tag] : 0.0;
+        const R vu = ok ? gvu[d * a.nn + tag] : (R)0;
+        const R v = ok ? gv[d * a.nn + tag] : (R)0;
         nb[d * BN + q] = vu;
         nb[(DIM + d) * BN + q] = vu - v;
       } else {
-        nb[d * BN + q] = ok ? a.gv[d * a.nn + tag] : 0.0;
+        nb[d * BN + q] = ok ? gv[d * a.nn + tag] : (R)0;
       }
     }
   }
 }
 
 // Gather over the stencil: vu, acc (MODE 0 / A), L (USL / B).
-template <int DIM, int S1, int EY, int EZ, int BN, bool WANT_VA, bool WANT_L, bool L_FROM_V>
-__device__ inline void gather_stencil(const double *nb, const int *cl, const double (*w)[S1],
-                                      const double (*dw)[S1], double *vu, double *acc,
-                                      double *L) {
+template <int DIM, int S1, int EY, int EZ, int BN, bool WANT_VA, bool WANT_L, bool L_FROM_V,
+          class R>
+__device__ inline void gather_stencil(const R *nb, const int *cl, const R (*w)[S1],
+                                      const R (*dw)[S1], R *vu, R *acc, R *L) {
 #pragma unroll
   for (int ia = 0; ia < S1; ia++) {
 #pragma unroll
     for (int ib = 0; ib < S1; ib++) {
-      double wxy = w[0][ia] * w[1][ib];
-      double dxy = dw[0][ia] * w[1][ib];
-      double xdy = w[0][ia] * dw[1][ib];
+      R wxy = w[0][ia] * w[1][ib];
+      R dxy = dw[0][ia] * w[1][ib];
+      R xdy = w[0][ia] * dw[1][ib];
       // Opaque barrier: the previous row's accumulations complete here, and
       // neither this row's LDS loads nor its weight products move above it.
       // Without it the scheduler defers whole FMA chains and spills.
@@ -545,7 +620,7 @@ __device__ inline void gather_stencil(const double *nb, const int *cl, const dou
       asm volatile("" : "+v"(wxy), "+v"(dxy), "+v"(xdy)::"memory");
 #pragma unroll
       for (int ic = 0; ic < (DIM == 3 ? S1 : 1); ic++) {
-        double wt, g0, g1, g2 = 0;
+        R wt, g0, g1, g2 = 0;
         int idx;
         if (DIM == 3) {
           wt = wxy * w[2][ic];
@@ -559,7 +634,7 @@ __device__ inline void gather_stencil(const double *nb, const int *cl, const dou
           g1 = xdy;
           idx = (cl[0] + ia) * EY + (cl[1] + ib);
         }
-        double vn[DIM];
+        R vn[DIM];
         for (int d = 0; d < DIM; d++) vn[d] = nb[d * BN + idx];
         if (WANT_VA) {
           for (int d = 0; d < DIM; d++) {
@@ -568,13 +643,26 @@ __device__ inline void gather_stencil(const double *nb, const int *cl, const dou
           }
         }
         if (WANT_L) {
-          const double gg[3] = {g0, g1, g2};
+          const R gg[3] = {g0, g1, g2};
           for (int d = 0; d < DIM; d++)
             for (int e = 0; e < DIM; e++) L[3 * d + e] = fma(vn[d], gg[e], L[3 * d + e]);
         }
         (void)L_FROM_V;
       }
     }
+  }
+}
+
+// 1D weights of one axis in FP64 (reference rounding), stored as R.
+template <int S1, class R>
+__device__ inline void weights_1d_as(double x, double lo, double h, double inv_h, int n, int base,
+                                     R *w, R *dw) {
+  double wd[S1], dwd[S1];
+  weights_1d<S1>(x, lo, h, inv_h, n, base, wd, dwd);
+#pragma unroll
+  for (int q = 0; q < S1; q++) {
+    w[q] = (R)wd[q];
+    dw[q] = (R)dwd[q];
   }
 }
 
@@ -624,12 +712,13 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // ---------------------------------------------------------------------------
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
+  using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
-  __shared__ double nb[2 * DIM * BN];
+  __shared__ R nb[2 * DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -643,25 +732,33 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
     __syncthreads();
-    load_node_block<DIM, S1, TX, TY, TZ, 0>(a, nb, org[0], org[1], org[2]);
+    load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
       double x[3], v[3];
       for (int d = 0; d < 3; d++) {
-        x[d] = PF(a.pa, PX + d, slot);
-        v[d] = PF(a.pa, VX + d, slot);
+        x[d] = LDA(PX + d, slot);
+        v[d] = LDA(VX + d, slot);
       }
-      double w[3][S1], dw[3][S1];
+      R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = c - org[d];
-        weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
-      double vu[3] = {0, 0, 0}, acc[3] = {0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vu, acc, L);
+      double vu[3], acc[3], L[9];
+      {
+        R vur[3] = {0, 0, 0}, accr[3] = {0, 0, 0}, Lr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vur, accr, Lr);
+        for (int d = 0; d < 3; d++) {
+          vu[d] = vur[d];
+          acc[d] = accr[d];
+        }
+        for (int e = 0; e < 9; e++) L[e] = Lr[e];
+      }
       const int pid = IFLD(a.ia, PID, slot);
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
       for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
@@ -671,43 +768,43 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
       // it does not stay live across the constitutive update.
       const int mat = IFLD(a.ia, MATI, slot);
       for (int d = 0; d < 3; d++) {
-        PF(a.pb, PX + d, j) = x[d];
-        PF(a.pb, VX + d, j) = v[d];
+        STB(PX + d, j, x[d]);
+        STB(VX + d, j, v[d]);
       }
       a.key[j] = particle_key(a, x[0], x[1], x[2]);
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
-      const double vol0 = PF(a.pa, VOL0, slot);
-      PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
-      PF(a.pb, VOL0, j) = vol0;
+      const double vol0 = LDA(VOL0, slot);
+      STB(MASS, j, LDA(MASS, slot));
+      STB(VOL0, j, vol0);
       double J;
       {
         double F[9];
-        for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+        for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot);
         J = update_F(dt, L, F);
-        for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
+        for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
       }
-      PF(a.pb, VOL, j) = J * vol0;
-      double dmg = PF(a.pa, DMG, slot);
+      STB(VOL, j, J * vol0);
+      double dmg = LDA(DMG, slot);
       if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
       double sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
-        sig[e] = PF(a.pa, SXX + e, slot);
-        eel[e] = PF(a.pa, EXX + e, slot);
+        sig[e] = LDA(SXX + e, slot);
+        eel[e] = LDA(EXX + e, slot);
       }
-      double ep = PF(a.pa, EP, slot), epdot = PF(a.pa, EPDOT, slot);
-      double dinit = PF(a.pa, DINIT, slot);
+      double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
+      double dinit = LDA(DINIT, slot);
       const double c =
           stress_and_speed<MC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
-        PF(a.pb, SXX + e, j) = sig[e];
-        PF(a.pb, EXX + e, j) = eel[e];
+        STB(SXX + e, j, sig[e]);
+        STB(EXX + e, j, eel[e]);
       }
-      PF(a.pb, EP, j) = ep;
-      PF(a.pb, EPDOT, j) = epdot;
-      PF(a.pb, DINIT, j) = dinit;
-      PF(a.pb, DMG, j) = dmg;
+      STB(EP, j, ep);
+      STB(EPDOT, j, epdot);
+      STB(DINIT, j, dinit);
+      STB(DMG, j, dmg);
     }
   }
   block_max_to(cmax, a.sc);
@@ -716,12 +813,13 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
 // ---------------------------------------------------------------------------
 // MUSL part A: G2P + advance (musl.cpp:61-67); keeps start-of-step x
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ>
+template <int DIM, int S1, int TX, int TY, int TZ, int PREC>
 __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
+  using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
-  __shared__ double nb[2 * DIM * BN];
+  __shared__ R nb[2 * DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -734,25 +832,33 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
     __syncthreads();
-    load_node_block<DIM, S1, TX, TY, TZ, 0>(a, nb, org[0], org[1], org[2]);
+    load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
       double x[3], v[3];
       for (int d = 0; d < 3; d++) {
-        x[d] = PF(a.pa, PX + d, slot);
-        v[d] = PF(a.pa, VX + d, slot);
+        x[d] = LDA(PX + d, slot);
+        v[d] = LDA(VX + d, slot);
       }
-      double w[3][S1], dw[3][S1];
+      R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = c - org[d];
-        weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
-      double vu[3] = {0, 0, 0}, acc[3] = {0, 0, 0};
-      gather_stencil<DIM, S1, EY, EZ, BN, true, false, false>(nb, cl, w, dw, vu, acc, nullptr);
+      double vu[3], acc[3];
+      {
+        R vur[3] = {0, 0, 0}, accr[3] = {0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, true, false, false>(nb, cl, w, dw, vur, accr,
+                                                                (R *)nullptr);
+        for (int d = 0; d < 3; d++) {
+          vu[d] = vur[d];
+          acc[d] = accr[d];
+        }
+      }
       const int pid = IFLD(a.ia, PID, slot);
       for (int d = 0; d < 3; d++) a.xo[(size_t)d * a.cap + j] = x[d];
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
@@ -760,10 +866,10 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
       if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
       for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
       for (int d = 0; d < 3; d++) {
-        PF(a.pb, PX + d, j) = x[d];
-        PF(a.pb, VX + d, j) = v[d];
+        STB(PX + d, j, x[d]);
+        STB(VX + d, j, v[d]);
       }
-      PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
+      STB(MASS, j, LDA(MASS, slot));
       a.key[j] = particle_key(a, x[0], x[1], x[2]);
     }
   }
@@ -772,12 +878,13 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
 // ---------------------------------------------------------------------------
 // MUSL part B: L from re-mapped v (old weights), F, stress, CFL (musl.cpp:75-77)
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
+  using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
-  __shared__ double nb[DIM * BN];
+  __shared__ R nb[DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -789,58 +896,63 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
     __syncthreads();
-    load_node_block<DIM, S1, TX, TY, TZ, 1>(a, nb, org[0], org[1], org[2]);
+    load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
       double x[3];
       for (int d = 0; d < 3; d++) x[d] = a.xo[(size_t)d * a.cap + j];
-      double w[3][S1], dw[3][S1];
+      R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = c - org[d];
-        weights_1d<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
-      double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, nullptr, nullptr, L);
+      double L[9];
+      {
+        R Lr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, (R *)nullptr,
+                                                               (R *)nullptr, Lr);
+        for (int e = 0; e < 9; e++) L[e] = Lr[e];
+      }
       const int pid = IFLD(a.ia, PID, slot);
       double v[3];
-      for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
+      for (int d = 0; d < 3; d++) v[d] = LDB(VX + d, j);
       const int mat = IFLD(a.ia, MATI, slot);
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
-      const double vol0 = PF(a.pa, VOL0, slot);
-      PF(a.pb, VOL0, j) = vol0;
+      const double vol0 = LDA(VOL0, slot);
+      STB(VOL0, j, vol0);
       double J;
       {
         double F[9];
-        for (int e = 0; e < 9; e++) F[e] = PF(a.pa, F00 + e, slot);
+        for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot);
         J = update_F(dt, L, F);
-        for (int e = 0; e < 9; e++) PF(a.pb, F00 + e, j) = F[e];
+        for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
       }
-      PF(a.pb, VOL, j) = J * vol0;
-      double dmg = PF(a.pa, DMG, slot);
+      STB(VOL, j, J * vol0);
+      double dmg = LDA(DMG, slot);
       if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
       double sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
-        sig[e] = PF(a.pa, SXX + e, slot);
-        eel[e] = PF(a.pa, EXX + e, slot);
+        sig[e] = LDA(SXX + e, slot);
+        eel[e] = LDA(EXX + e, slot);
       }
-      double ep = PF(a.pa, EP, slot), epdot = PF(a.pa, EPDOT, slot);
-      double dinit = PF(a.pa, DINIT, slot);
+      double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
+      double dinit = LDA(DINIT, slot);
       const double c =
           stress_and_speed<MC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
-        PF(a.pb, SXX + e, j) = sig[e];
-        PF(a.pb, EXX + e, j) = eel[e];
+        STB(SXX + e, j, sig[e]);
+        STB(EXX + e, j, eel[e]);
       }
-      PF(a.pb, EP, j) = ep;
-      PF(a.pb, EPDOT, j) = epdot;
-      PF(a.pb, DINIT, j) = dinit;
-      PF(a.pb, DMG, j) = dmg;
+      STB(EP, j, ep);
+      STB(EPDOT, j, epdot);
+      STB(DINIT, j, dinit);
+      STB(DMG, j, dmg);
     }
   }
   block_max_to(cmax, a.sc);
@@ -868,20 +980,21 @@ __global__ void k_adjust_dt(KArgs a) {
 // ---------------------------------------------------------------------------
 // observables and debug
 // ---------------------------------------------------------------------------
+template <int PREC>
 __global__ void k_energy(KArgs a, const uint32_t *pmask, uint32_t bit, double *partial) {
   __shared__ double sk[256], ss[256];
   double ek = 0, es = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.np; i += gridDim.x * blockDim.x) {
     const int pid = IFLD(a.ia, PID, i);
     if (!(pmask[pid] & bit)) continue;
-    const double v0 = PF(a.pa, VX, i), v1 = PF(a.pa, VY, i), v2 = PF(a.pa, VZ, i);
+    const double v0 = LDA(VX, i), v1 = LDA(VY, i), v2 = LDA(VZ, i);
     const double nv = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-    ek += 0.5 * PF(a.pa, MASS, i) * (nv * nv);
+    ek += 0.5 * LDA(MASS, i) * (nv * nv);
     // sigma : strain_el over the full tensor, row-major order
-    const double s00 = PF(a.pa, SXX, i), s11 = PF(a.pa, SYY, i), s22 = PF(a.pa, SZZ, i);
-    const double s01 = PF(a.pa, SXY, i), s02 = PF(a.pa, SXZ, i), s12 = PF(a.pa, SYZ, i);
-    const double e00 = PF(a.pa, EXX, i), e11 = PF(a.pa, EYY, i), e22 = PF(a.pa, EZZ, i);
-    const double e01 = PF(a.pa, EXY, i), e02 = PF(a.pa, EXZ, i), e12 = PF(a.pa, EYZ, i);
+    const double s00 = LDA(SXX, i), s11 = LDA(SYY, i), s22 = LDA(SZZ, i);
+    const double s01 = LDA(SXY, i), s02 = LDA(SXZ, i), s12 = LDA(SYZ, i);
+    const double e00 = LDA(EXX, i), e11 = LDA(EYY, i), e22 = LDA(EZZ, i);
+    const double e01 = LDA(EXY, i), e02 = LDA(EXZ, i), e12 = LDA(EYZ, i);
     double t = s00 * e00;
     t += s01 * e01;
     t += s02 * e02;
@@ -891,7 +1004,7 @@ __global__ void k_energy(KArgs a, const uint32_t *pmask, uint32_t bit, double *p
     t += s02 * e02;
     t += s12 * e12;
     t += s22 * e22;
-    es += 0.5 * PF(a.pa, VOL, i) * t;
+    es += 0.5 * LDA(VOL, i) * t;
   }
   sk[threadIdx.x] = ek;
   ss[threadIdx.x] = es;
@@ -1010,6 +1123,12 @@ static int persistent_grid(kmpm_ctx *c, int per_cu) {
     else { constexpr int DIM = 2, S1 = 2; __VA_ARGS__; }                                    \
   } while (0)
 
+#define PREC_DISPATCH(c, ...)                                                             \
+  do {                                                                                    \
+    if ((c)->a.prec == KMPM_PRECISION_MIXED) { constexpr int PREC = KMPM_PRECISION_MIXED; __VA_ARGS__; } \
+    else { constexpr int PREC = KMPM_PRECISION_FP64; __VA_ARGS__; }                        \
+  } while (0)
+
 void tile_shape(int dim, int S1, int T[3]) {
   if (dim == 3 && S1 == 4) { T[0] = Tiles<3, 4>::TX; T[1] = Tiles<3, 4>::TY; T[2] = Tiles<3, 4>::TZ; }
   else if (dim == 3) { T[0] = Tiles<3, 2>::TX; T[1] = Tiles<3, 2>::TY; T[2] = Tiles<3, 2>::TZ; }
@@ -1053,15 +1172,15 @@ hipError_t launch_sort(kmpm_ctx *c) {
 hipError_t launch_p2g(kmpm_ctx *c, int remap) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
-  DISPATCH(c, {
+  DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
     if (remap)
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true>), dim3(g),
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC>), dim3(g),
                          dim3(256), 0, c->stream, c->a);
     else
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false>), dim3(g),
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC>), dim3(g),
                          dim3(256), 0, c->stream, c->a);
-  });
+  }));
   return hipGetLastError();
 }
 
@@ -1070,22 +1189,22 @@ hipError_t launch_grid_update(kmpm_ctx *c, int remap) {
   int g = nblk(total, 256);
   if (g > c->cu_count * 8) g = c->cu_count * 8;
   if (g < 1) g = 1;
-  DISPATCH(c, {
+  DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_grid_update<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
-                       c->stream, c->a, remap);
-  });
+    hipLaunchKernelGGL((k_grid_update<DIM, S1, TT::TX, TT::TY, TT::TZ, PREC>), dim3(g), dim3(256),
+                       0, c->stream, c->a, remap);
+  }));
   return hipGetLastError();
 }
 
 hipError_t launch_grid_clean(kmpm_ctx *c) {
   int g = nblk(c->a.nn, 256);
   if (g > c->cu_count * 16) g = c->cu_count * 16;
-  DISPATCH(c, {
+  DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_grid_clean<DIM, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+    hipLaunchKernelGGL((k_grid_clean<DIM, TT::TX, TT::TY, TT::TZ, PREC>), dim3(g), dim3(256), 0,
                        c->stream, c->a);
-  });
+  }));
   return hipGetLastError();
 }
 
@@ -1102,33 +1221,33 @@ hipError_t launch_grid_clean(kmpm_ctx *c) {
 hipError_t launch_g2p_usl(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
-  DISPATCH(c, MC_DISPATCH(c, {
+  DISPATCH(c, MC_DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256), 0,
-                       c->stream, c->a);
-  }));
+    hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC>), dim3(g), dim3(256),
+                       0, c->stream, c->a);
+  })));
   return hipGetLastError();
 }
 
 hipError_t launch_g2p_musl_a(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
-  DISPATCH(c, {
+  DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
-                       c->stream, c->a);
-  });
+    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, PREC>), dim3(g), dim3(256),
+                       0, c->stream, c->a);
+  }));
   return hipGetLastError();
 }
 
 hipError_t launch_stress_musl_b(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
-  DISPATCH(c, MC_DISPATCH(c, {
+  DISPATCH(c, MC_DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256),
-                       0, c->stream, c->a);
-  }));
+    hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC>), dim3(g),
+                       dim3(256), 0, c->stream, c->a);
+  })));
   return hipGetLastError();
 }
 
@@ -1139,8 +1258,8 @@ hipError_t launch_adjust_dt(kmpm_ctx *c) {
 
 hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2) {
   const int nb = 64;
-  hipLaunchKernelGGL(k_energy, dim3(nb), dim3(256), 0, c->stream, c->a, c->d_pmask, bit,
-                     d_out2 + 2);
+  PREC_DISPATCH(c, hipLaunchKernelGGL(k_energy<PREC>, dim3(nb), dim3(256), 0, c->stream, c->a,
+                                      c->d_pmask, bit, d_out2 + 2));
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, c->stream, d_out2 + 2, nb, d_out2);
   return hipGetLastError();
 }
