@@ -248,6 +248,71 @@ int kmpm_get_tile_shape(kmpm_ctx *ctx, int tile[3]);
 int kmpm_debug_weights(kmpm_ctx *ctx, int64_t *node_tag, double *w, double *wd,
                        int64_t np);
 
+/* ---- learned constitutive surrogate ----------------------------------------
+ * Replaces, for a johnson_cook (+ damage_johnson_cook) material, the two
+ * transcendental material functions of
+ *   StrengthJohnsonCook::update_deviatoric_stress (strength_jc.cpp:100-184)
+ *   DamageJohnsonCook::compute_damage             (damage_jc.cpp:91-143)
+ * with two 2 -> 32 -> 32 -> 1 tanh MLPs run on the BF16 matrix cores:
+ *   net 0: (eps_p^(1/4), ln epsdot*)              -> sigma_y / A
+ *   net 1: (triaxiality, ln(epsdot / epsdot0_d))  -> ln eps_f
+ * The radial return, damage accumulation and every branch of the reference
+ * stay exact; states outside the training envelope use the plain FP64
+ * functions.  Training runs on the device on tuples the plain functions
+ * generate.  No counterpart in the reference. */
+typedef struct kmpm_surrogate_config {
+  double eps_p_max;       /* envelope of net 0: eps_p in [1e-10, eps_p_max] */
+  double rate_max;        /* envelope of both: epsdot / epsdot0 in [1, rate_max] */
+  double triax_max;       /* envelope of net 1: triaxiality in (-3, triax_max] */
+  int64_t samples;        /* training tuples per network (device-generated) */
+  int64_t iterations;     /* Adam iterations per network */
+  int32_t batch;          /* tuples per iteration, a multiple of 256 */
+  int32_t reserved;
+  double learning_rate;   /* initial Adam step; cosine decay to 1% of it */
+  uint64_t seed;
+} kmpm_surrogate_config;
+
+typedef struct kmpm_surrogate kmpm_surrogate;
+/* number of FP32 parameters of one network (flat order: w1[32][2], b1[32],
+ * w2[32][32] ([out][in]), b2[32], w3[32], b3) */
+enum { KMPM_SURROGATE_NPARAM = 1185 };
+
+/* Generates the training tuples and initialises both networks. */
+int kmpm_surrogate_create(const kmpm_material *mat, const kmpm_surrogate_config *cfg,
+                          kmpm_surrogate **out);
+int kmpm_surrogate_destroy(kmpm_surrogate *s);
+/* Trains both networks (fused forward/backward kernels + Adam); returns the
+ * final mean squared error of each on its tuple set (normalised units). */
+int kmpm_surrogate_train(kmpm_surrogate *s, double final_loss[2]);
+/* n fresh tuples of network `net` from the plain functions:
+ * features [n][2], targets [n]. */
+int kmpm_surrogate_tuples(kmpm_surrogate *s, int net, int64_t n, uint64_t seed,
+                          float *features, float *targets);
+/* Network outputs (target units) for features [n][2]. */
+int kmpm_surrogate_eval(kmpm_surrogate *s, int net, const float *features, int64_t n,
+                        float *out);
+/* Gradient of L = (1/n) sum (z - (t - base(x) - out_shift)/out_scale)^2
+ * with respect to the FP32 parameters (flat order above), z the raw
+ * network output. */
+int kmpm_surrogate_gradient(kmpm_surrogate *s, int net, const float *features,
+                            const float *targets, int64_t n, float *grad, double *loss);
+/* Read (set = 0) or replace (set = 1) the FP32 parameters of a network. */
+int kmpm_surrogate_params(kmpm_surrogate *s, int net, float *params, int nparam, int set);
+/* The maps of a network: in_scale[2], in_shift[2], out_scale, out_shift and
+ * the quadratic baseline base[6] (coefficients of 1, x0, x1, x0^2, x0 x1,
+ * x1^2 in the inputs x = f * in_scale + in_shift); output =
+ * base(x) + out_shift + out_scale * z. */
+int kmpm_surrogate_normalization(kmpm_surrogate *s, int net, double map[12]);
+/* Feature ranges of the envelope: lo/hi of (net 0 f0, f1, net 1 f0, f1). */
+int kmpm_surrogate_envelope(kmpm_surrogate *s, double lo[4], double hi[4]);
+/* Use the surrogate in the step of a mixed-precision context whose
+ * materials are all this JC + shock material (NULL: back to the plain
+ * functions).  The context copies the networks. */
+int kmpm_ctx_set_surrogate(kmpm_ctx *ctx, const kmpm_surrogate *s);
+/* Particle updates since the last call that used a network, and that fell
+ * back to the plain functions (outside the envelope); resets both. */
+int kmpm_surrogate_stats(kmpm_ctx *ctx, int64_t *evaluations, int64_t *fallbacks);
+
 #ifdef __cplusplus
 }
 #endif

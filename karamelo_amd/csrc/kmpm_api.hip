@@ -1,6 +1,7 @@
 // kmpm_api.hip -- C ABI of the MI355X explicit MPM step: context, buffers,
 // phases and the stream-ordered step loop.
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -8,6 +9,7 @@
 #include <vector>
 
 #include "kmpm_ctx.h"
+#include "kmpm_surrogate.h"
 
 using namespace kmpm;
 
@@ -150,6 +152,37 @@ static void set_fixed_point_scales(kmpm_ctx *c, const std::vector<double> &hd, i
   }
 }
 
+// Device record of a material (Mat constructors, material.cpp:720-769,
+// with the plugin parameters in constructor order).
+DevMat kmpm::make_devmat(const kmpm_material &m) {
+  DevMat d;
+  memset(&d, 0, sizeof(d));
+  d.type = m.type;
+  d.eos = m.eos;
+  d.strength = m.strength;
+  d.damage = m.damage;
+  d.rho0 = m.rho0;
+  d.K = m.K;
+  d.G = m.G;
+  d.lambda = m.lambda;
+  d.signal_velocity = m.signal_velocity;
+  d.cfl_k43g = m.K + FOUR_THIRD * m.G;
+  {  // JC thermal factor at T = 0 (strength_jc.cpp:127-133)
+    const double mm = m.strength_p[6], Tr = m.strength_p[7], Tm = m.strength_p[8], T = 0;
+    double f = 1.0;
+    if (T < Tm) {
+      if (mm != 0 && T >= Tr) f = 1.0 - std::pow((T - Tr) / (Tm - Tr), mm);
+    } else {
+      f = 0.0;
+    }
+    d.jc_thermal = f;
+  }
+  memcpy(d.eos_p, m.eos_p, sizeof(d.eos_p));
+  memcpy(d.strength_p, m.strength_p, sizeof(d.strength_p));
+  memcpy(d.damage_p, m.damage_p, sizeof(d.damage_p));
+  return d;
+}
+
 extern "C" {
 
 int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
@@ -237,34 +270,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   }
   // material table
   std::vector<DevMat> dm(nmat);
-  for (int i = 0; i < nmat; i++) {
-    const kmpm_material &m = mats[i];
-    DevMat &d = dm[i];
-    memset(&d, 0, sizeof(d));
-    d.type = m.type;
-    d.eos = m.eos;
-    d.strength = m.strength;
-    d.damage = m.damage;
-    d.rho0 = m.rho0;
-    d.K = m.K;
-    d.G = m.G;
-    d.lambda = m.lambda;
-    d.signal_velocity = m.signal_velocity;
-    d.cfl_k43g = m.K + FOUR_THIRD * m.G;
-    {  // JC thermal factor at T = 0 (strength_jc.cpp:127-133)
-      const double mm = m.strength_p[6], Tr = m.strength_p[7], Tm = m.strength_p[8], T = 0;
-      double f = 1.0;
-      if (T < Tm) {
-        if (mm != 0 && T >= Tr) f = 1.0 - std::pow((T - Tr) / (Tm - Tr), mm);
-      } else {
-        f = 0.0;
-      }
-      d.jc_thermal = f;
-    }
-    memcpy(d.eos_p, m.eos_p, sizeof(d.eos_p));
-    memcpy(d.strength_p, m.strength_p, sizeof(d.strength_p));
-    memcpy(d.damage_p, m.damage_p, sizeof(d.damage_p));
-  }
+  for (int i = 0; i < nmat; i++) dm[i] = make_devmat(mats[i]);
   bool ok = dalloc(&c->d_mats, nmat) == hipSuccess &&
             hipMemcpy(c->d_mats, dm.data(), nmat * sizeof(DevMat), hipMemcpyHostToDevice) ==
                 hipSuccess;
@@ -318,7 +324,7 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   free_particle_buffers(c);
   void *ptrs[] = {c->d_mats, c->d_cell_count, c->d_cell_start, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
-                  c->d_sc, c->d_scan_tmp};
+                  c->d_sc, c->d_scan_tmp, c->d_surr};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->ev_pool) {
@@ -730,6 +736,43 @@ int kmpm_timing_read(kmpm_ctx *c, double ms[KMPM_NPHASE], int64_t launches[KMPM_
     ms[ph] += t;
     launches[ph] += 1;
   }
+  return KMPM_OK;
+}
+
+int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
+  if (!c) return KMPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!s) {
+    c->a.surr_on = 0;
+    return KMPM_OK;
+  }
+  // one network pair serves the whole wave, so every material must be the
+  // surrogate's JC + shock material
+  if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK) return KMPM_ERR_ARG;
+  for (int i = 0; i < c->nmat; i++)
+    if (!surrogate_matches(s, c->mats_host[i])) return KMPM_ERR_ARG;
+  SurrogateDev img;
+  int st = surrogate_device_image(s, &img);
+  if (st) return st;
+  if (!c->d_surr) HIPCHK(dalloc(&c->d_surr, 1));
+  HIPCHK(hipMemcpy(c->d_surr, &img, sizeof(img), hipMemcpyHostToDevice));
+  c->a.surr = c->d_surr;
+  c->a.surr_on = 1;
+  return KMPM_OK;
+}
+
+int kmpm_surrogate_stats(kmpm_ctx *c, int64_t *evals, int64_t *fallbacks) {
+  if (!c || !evals || !fallbacks) return KMPM_ERR_ARG;
+  DevScalars sc;
+  HIPCHK(hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *evals = (int64_t)sc.surr_evals;
+  *fallbacks = (int64_t)sc.surr_fallbacks;
+  sc.surr_evals = 0;
+  sc.surr_fallbacks = 0;
+  HIPCHK(hipMemcpyAsync((char *)c->d_sc + offsetof(DevScalars, surr_evals), &sc.surr_evals,
+                        2 * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return KMPM_OK;
 }
 

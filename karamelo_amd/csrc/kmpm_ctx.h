@@ -7,6 +7,8 @@
 
 namespace kmpm {
 
+struct SurrogateDev;
+
 // Tile geometry shared by the sort key, P2G, grid update and G2P kernels.
 struct TileGeom {
   int T[3];        // tile size in cells (T[2] == 1 in 2D)
@@ -62,6 +64,9 @@ struct KArgs {
   int nfv, nbf;
   DevFixVel fv[MAX_FIX];
   DevFixBF bf[MAX_FIX];
+  // learned constitutive surrogate (JC + shock class, mixed path)
+  const SurrogateDev *surr;
+  int surr_on;
   // scalars
   DevScalars *sc;
 };
@@ -94,6 +99,7 @@ struct kmpm_ctx {
   uint32_t *d_gmask;
   uint32_t *d_pmask;     // particle group masks (by pid)
   kmpm::DevScalars *d_sc;
+  kmpm::SurrogateDev *d_surr;
   void *d_scan_tmp;
   size_t scan_tmp_bytes;
   int64_t ncells;
@@ -120,5 +126,8 @@ hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);
 hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double *d_wd);
 hipError_t launch_cells(kmpm_ctx *c, int *d_ijk);
 size_t scan_temp_bytes(int64_t n);
+DevMat make_devmat(const kmpm_material &m);
+int surrogate_device_image(const kmpm_surrogate *s, SurrogateDev *out);
+bool surrogate_matches(const kmpm_surrogate *s, const kmpm_material &m);
 hipError_t copy_probe(int64_t bytes, int reps, double *gbps);
 }  // namespace kmpm

@@ -19,6 +19,7 @@
 
 #include "kmpm_constitutive.h"
 #include "kmpm_ctx.h"
+#include "kmpm_surrogate.h"
 
 namespace kmpm {
 
@@ -680,11 +681,13 @@ __device__ inline double update_F(double dt, const double *L, double *F) {
 // D = sym(L) (solid.cpp:1239), stress update (solid.cpp:1246-1374) and the
 // particle's CFL wave speed (:1376-1385); returns -1 for fully damaged
 // particles (skipped by the CFL maximum).
-template <int MC>
+// SURR: the JC + shock update with the learned material functions; every
+// lane of the wave calls it (live == false for lanes without a particle).
+template <int MC, bool SURR = false>
 __device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
                                           double *sig, double *eel, double &ep, double &epdot,
                                           double &dinit, double &dmg, int mat, const double *v,
-                                          int pid) {
+                                          int pid, bool live = true) {
   const DevMat &m = a.mats[mat];
   const double rho = m.rho0 / J;
   double D[6];
@@ -694,11 +697,15 @@ __device__ inline double stress_and_speed(const KArgs &a, double dt, const doubl
   D[3] = 0.5 * (L[1] + L[3]);
   D[4] = 0.5 * (L[2] + L[6]);
   D[5] = 0.5 * (L[5] + L[7]);
-  update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  if constexpr (SURR)
+    update_stress_jc_surrogate(m, *a.surr, live, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit,
+                               dmg, a.sc);
+  else
+    update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   if (dmg >= 1.0) return -1.0;
   const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
   const double c = sqrt(m.cfl_k43g / rho) + vm;
-  if (isnan(c)) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
+  if (isnan(c) && live) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
   return c;
 }
 
@@ -712,7 +719,7 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // ---------------------------------------------------------------------------
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
 __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
@@ -734,7 +741,10 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
-    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+    // live == false: a lane without a particle that still takes part in the
+    // wave-wide surrogate evaluation (it computes on particle p1 - 1 and
+    // stores nothing)
+    auto particle = [&](const int j, const bool live) {
       const int slot = a.order[j];
       double x[3], v[3];
       for (int d = 0; d < 3; d++) {
@@ -762,31 +772,36 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
       const int pid = IFLD(a.ia, PID, slot);
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
       for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
-      if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+      if (live && outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
       for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
       // State is written in sorted order as soon as each part is final, so
       // it does not stay live across the constitutive update.
       const int mat = IFLD(a.ia, MATI, slot);
-      for (int d = 0; d < 3; d++) {
-        STB(PX + d, j, x[d]);
-        STB(VX + d, j, v[d]);
+      if (live) {
+        for (int d = 0; d < 3; d++) {
+          STB(PX + d, j, x[d]);
+          STB(VX + d, j, v[d]);
+        }
+        a.key[j] = particle_key(a, x[0], x[1], x[2]);
+        IFLD(a.ib, PID, j) = pid;
+        IFLD(a.ib, MATI, j) = mat;
       }
-      a.key[j] = particle_key(a, x[0], x[1], x[2]);
-      IFLD(a.ib, PID, j) = pid;
-      IFLD(a.ib, MATI, j) = mat;
       const double vol0 = LDA(VOL0, slot);
-      STB(MASS, j, LDA(MASS, slot));
-      STB(VOL0, j, vol0);
+      if (live) {
+        STB(MASS, j, LDA(MASS, slot));
+        STB(VOL0, j, vol0);
+      }
       double J;
       {
         double F[9];
         for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot);
         J = update_F(dt, L, F);
-        for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
+        if (live)
+          for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
       }
-      STB(VOL, j, J * vol0);
+      if (live) STB(VOL, j, J * vol0);
       double dmg = LDA(DMG, slot);
-      if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+      if (live && J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
       double sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
         sig[e] = LDA(SXX + e, slot);
@@ -794,8 +809,9 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
       }
       double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
       double dinit = LDA(DINIT, slot);
-      const double c =
-          stress_and_speed<MC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
+      const double c = stress_and_speed<MC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
+                                                  mat, v, pid, live);
+      if (!live) return;
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
         STB(SXX + e, j, sig[e]);
@@ -805,6 +821,14 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
       STB(EPDOT, j, epdot);
       STB(DINIT, j, dinit);
       STB(DMG, j, dmg);
+    };
+    if constexpr (SURR) {
+      for (int jb = p0; jb < p1; jb += blockDim.x) {
+        const int j = jb + (int)threadIdx.x;
+        particle(j < p1 ? j : p1 - 1, j < p1);
+      }
+    } else {
+      for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) particle(j, true);
     }
   }
   block_max_to(cmax, a.sc);
@@ -1221,10 +1245,17 @@ hipError_t launch_grid_clean(kmpm_ctx *c) {
 hipError_t launch_g2p_usl(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
+  const bool surr = c->a.surr_on && c->a.mat_class == MC_JC_SHOCK &&
+                    c->a.prec == KMPM_PRECISION_MIXED;
   DISPATCH(c, MC_DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC>), dim3(g), dim3(256),
-                       0, c->stream, c->a);
+    constexpr bool CAN_SURR = MC == MC_JC_SHOCK && PREC == KMPM_PRECISION_MIXED;
+    if (CAN_SURR && surr)
+      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, CAN_SURR>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+    else
+      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, false>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
   })));
   return hipGetLastError();
 }

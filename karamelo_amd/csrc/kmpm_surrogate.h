@@ -1,0 +1,143 @@
+// kmpm_surrogate.h -- Solid::update_stress for a johnson_cook + shock
+// (+ damage_johnson_cook) material with learned material functions.
+//
+// The surrogate replaces the Johnson-Cook flow stress and failure strain,
+// the transcendental parts of StrengthJohnsonCook::update_deviatoric_stress
+// (strength_jc.cpp:115-133) and DamageJohnsonCook::compute_damage
+// (damage_jc.cpp:124-138), with two small MLPs (kmpm_mlp.h):
+//   net y : (eps_p^(1/4), ln epsdot*)              -> sigma_y / A
+//   net f : (triaxiality, ln(epsdot / epsdot0_d))  -> ln eps_f
+// The radial return, the damage accumulation and every branch of the
+// reference (damage >= 1, eps_p < 1e-10, triax <= -3, the epsdot filter,
+// the tension rule) stay exact.  Inputs outside the training envelope use
+// the plain FP64 functions (the fallback), counted in DevScalars.
+#pragma once
+
+#include "kmpm_constitutive.h"
+#include "kmpm_mlp.h"
+
+namespace kmpm {
+
+struct SurrogateDev {
+  MlpNet y, f;
+  int enabled;
+  int pad;
+  // training envelope
+  double ep_hi;          // eps_p in [1e-10, ep_hi]
+  double lnr_y_hi;       // ln epsdot* in [0, lnr_y_hi]
+  double triax_hi;       // triaxiality in (-3, triax_hi]
+  double lnr_f_hi;       // ln(epsdot / epsdot0_d) in [0, lnr_f_hi]
+  // yield stress of a particle that has not yielded (eps_p < 1e-10) at
+  // epsdot* = 1: A (1 + 1)^C times the thermal factor (strength_jc.cpp:115-133)
+  double sy_virgin;
+};
+
+// Counts one wave's network evaluations and fallbacks.
+__device__ inline void surrogate_count(DevScalars *sc, bool eval, bool fallback) {
+  const unsigned long long ne = __ballot(eval), nf = __ballot(fallback);
+  if ((threadIdx.x & 63) == 0) {
+    if (ne) atomicAdd(&sc->surr_evals, (unsigned long long)__popcll(ne));
+    if (nf) atomicAdd(&sc->surr_fallbacks, (unsigned long long)__popcll(nf));
+  }
+}
+
+// Solid::update_stress eos-strength branch (solid.cpp:1292-1374) for the
+// JC + shock class.  Every lane of the wave calls it; `live` is false for
+// lanes without a particle (their results are discarded).
+__device__ inline void update_stress_jc_surrogate(const DevMat &m, const SurrogateDev &sd,
+                                                  bool live, double dt, double h, double J,
+                                                  double rho, const double *D, double *sig,
+                                                  double *eel, double &ep, double &epdot,
+                                                  double &dinit, double &dmg, DevScalars *sc) {
+  const double pH = shock_pressure(m, J, rho, dmg, D, h);
+  // ---- strength (strength_jc.cpp:100-184) -------------------------------
+  const double G_ = m.strength_p[0], A = m.strength_p[1], epsdot0 = m.strength_p[4];
+  const bool dead = dmg >= 1.0;
+  double ratio = epdot / epsdot0;
+  ratio = ratio > 1.0 ? ratio : 1.0;
+  const float fy0 = sqrtf(sqrtf((float)ep));
+  const float fy1 = __builtin_amdgcn_logf((float)ratio) * 0.69314718055994531f;
+  const bool virgin = ep < 1.0e-10 && ratio == 1.0;
+  const bool use_y = live && !dead && !virgin && ep >= 1.0e-10 && ep <= sd.ep_hi &&
+                     (double)fy1 <= sd.lnr_y_hi;
+  const bool fb_y = live && !dead && !virgin && !use_y;
+  float ny = 1.0f;
+  if (__ballot(use_y)) ny = mlp_eval_wave(sd.y, fy0, fy1);
+  double sdev[6], dps = 0.0;
+  if (fb_y) {
+    jc_strength(m, sig, D, dt, ep, epdot, dmg, sdev, &dps);
+  } else if (dead) {
+    for (int e = 0; e < 6; e++) sdev[e] = 0;
+  } else {
+    double ys = virgin ? sd.sy_virgin : A * (double)ny;
+    double Gd = G_;
+    if (dmg > 0) {
+      Gd *= (1 - dmg);
+      ys *= (1 - dmg);
+    }
+    const double c = dt * 2.0 * Gd;
+    double t[6];
+    for (int e = 0; e < 6; e++) t[e] = sig[e] + c * D[e];
+    const double t3 = (t[0] + t[1] + t[2]) / 3.0;
+    t[0] -= t3;
+    t[1] -= t3;
+    t[2] -= t3;
+    const double J2 = SQRT_3_OVER_2 * norm_sym(t);
+    if (J2 < ys) {
+      for (int e = 0; e < 6; e++) sdev[e] = t[e];
+    } else {
+      dps = (J2 - ys) / (3.0 * Gd);
+      const double f = ys / J2;
+      for (int e = 0; e < 6; e++) sdev[e] = t[e] * f;
+    }
+  }
+  ep += dps;                                            // solid.cpp:1319
+  const double tav = 1000 * h / m.signal_velocity;      // solid.cpp:1324-1329
+  epdot -= epdot * dt / tav;
+  epdot += dps / tav;
+  epdot = epdot > 0.0 ? epdot : 0.0;
+  // ---- damage (damage_jc.cpp:91-143) -------------------------------------
+  bool use_f = false, fb_f = false;
+  if (m.damage == KMPM_DAMAGE_JOHNSON_COOK) {
+    const double d4 = m.damage_p[3], epsdot0_d = m.damage_p[5];
+    const bool skip = dps == 0 && dmg >= 1.0;
+    const double vm = SQRT_3_OVER_2 * norm_sym(sdev);
+    double triax = 0.0;
+    if (pH != 0.0 && vm != 0.0) triax = -pH / (vm + 0.001 * fabs(pH));
+    const bool reset = !skip && triax <= -3;
+    const double lnr = (d4 > 0.0 && epdot > epsdot0_d) ? log(epdot / epsdot0_d) : 0.0;
+    // eps_f only matters when plastic strain accumulates (dps / eps_f)
+    use_f = live && !skip && !reset && dps > 0 && triax <= sd.triax_hi && lnr <= sd.lnr_f_hi;
+    fb_f = live && !skip && !reset && dps > 0 && !use_f;
+    float nf = 0.0f;
+    if (__ballot(use_f)) nf = mlp_eval_wave(sd.f, (float)triax, (float)lnr);
+    if (fb_f) {
+      jc_damage(m, dinit, dmg, pH, sdev, epdot, dps);
+    } else if (reset) {
+      dinit = 0;
+    } else if (!skip) {
+      if (use_f) dinit += dps / (double)__builtin_amdgcn_exp2f(nf * 1.4426950408889634f);
+      if (dinit >= 1.0) {
+        const double dd = (dinit - 1.0) * 10;
+        dmg = dd < 1.0 ? dd : 1.0;
+      }
+    }
+  }
+  surrogate_count(sc, use_y || use_f, fb_y || fb_f);
+  // ---- assembly (solid.cpp:1353-1366) -----------------------------------
+  const double trD = D[0] + D[1] + D[2];
+  const double trE = eel[0] + eel[1] + eel[2];
+  if (dmg == 0 || pH >= 0) {
+    for (int e = 0; e < 3; e++) sig[e] = -pH * 1.0 + sdev[e];
+  } else {
+    const double pd = -pH * (1.0 - dmg);
+    for (int e = 0; e < 3; e++) sig[e] = pd * 1.0 + sdev[e];
+  }
+  for (int e = 3; e < 6; e++) sig[e] = sdev[e];
+  const double t3 = (dt * trD + trE) / 3.0;
+  const double Gd = dmg > 1e-10 ? m.G * (1 - dmg) : m.G;
+  for (int e = 0; e < 3; e++) eel[e] = t3 * 1.0 + sdev[e] / Gd;
+  for (int e = 3; e < 6; e++) eel[e] = sdev[e] / Gd;
+}
+
+}  // namespace kmpm

@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "libkmpm.so")
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NO_DEVICE, ERR_CAPACITY = 1, 2, 3, 4
 ERR_PARTICLE_LEFT_BOX, ERR_JACOBIAN, ERR_DTCFL, ERR_WAVE_SPEED, ERR_GRID = 10, 11, 12, 13, 14
+ERR_FIXED_POINT = 15
+SURROGATE_NPARAM = 1185
 METHOD_ULMPM, METHOD_TLMPM = 0, 1
 SCHEME_USL, SCHEME_MUSL = 0, 1
 SHAPE_LINEAR, SHAPE_CUBIC_SPLINE = 0, 1
@@ -63,6 +65,13 @@ class Region(C.Structure):
 
 
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+
+
+class SurrogateConfig(C.Structure):
+    _fields_ = [("eps_p_max", C.c_double), ("rate_max", C.c_double), ("triax_max", C.c_double),
+                ("samples", C.c_int64), ("iterations", C.c_int64), ("batch", C.c_int32),
+                ("reserved", C.c_int32), ("learning_rate", C.c_double), ("seed", C.c_uint64)]
 
 
 class Particles(C.Structure):
@@ -152,6 +161,17 @@ def _declare(lib):
         "kmpm_timing_enable": (C.c_int, [vp, C.c_int]),
         "kmpm_timing_read": (C.c_int, [vp, P(C.c_double), P(C.c_int64)]),
         "kmpm_copy_probe": (C.c_int, [C.c_int64, C.c_int, P(C.c_double)]),
+        "kmpm_surrogate_create": (C.c_int, [P(Material), P(SurrogateConfig), P(vp)]),
+        "kmpm_surrogate_destroy": (C.c_int, [vp]),
+        "kmpm_surrogate_train": (C.c_int, [vp, _DP]),
+        "kmpm_surrogate_tuples": (C.c_int, [vp, C.c_int, C.c_int64, C.c_uint64, _FP, _FP]),
+        "kmpm_surrogate_eval": (C.c_int, [vp, C.c_int, _FP, C.c_int64, _FP]),
+        "kmpm_surrogate_gradient": (C.c_int, [vp, C.c_int, _FP, _FP, C.c_int64, _FP, _DP]),
+        "kmpm_surrogate_params": (C.c_int, [vp, C.c_int, _FP, C.c_int, C.c_int]),
+        "kmpm_surrogate_normalization": (C.c_int, [vp, C.c_int, _DP]),
+        "kmpm_surrogate_envelope": (C.c_int, [vp, _DP, _DP]),
+        "kmpm_ctx_set_surrogate": (C.c_int, [vp, vp]),
+        "kmpm_surrogate_stats": (C.c_int, [vp, P(C.c_int64), P(C.c_int64)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -317,6 +337,15 @@ class Context:
         check(self.L.kmpm_get_tile_shape(self.h, t), "get_tile_shape")
         return tuple(t)
 
+    def set_surrogate(self, surrogate):
+        check(self.L.kmpm_ctx_set_surrogate(self.h, surrogate.h if surrogate else None),
+              "set_surrogate")
+
+    def surrogate_stats(self):
+        ev, fb = C.c_int64(), C.c_int64()
+        check(self.L.kmpm_surrogate_stats(self.h, C.byref(ev), C.byref(fb)), "surrogate_stats")
+        return ev.value, fb.value
+
     def debug_weights(self):
         d = self.cfg.dimension
         S1 = 2 if self.cfg.shape == SHAPE_LINEAR else 4
@@ -335,3 +364,87 @@ def copy_probe(nbytes, reps=20):
     g = C.c_double()
     check(lib().kmpm_copy_probe(int(nbytes), int(reps), C.byref(g)), "copy_probe")
     return g.value
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Surrogate:
+    """Learned JC flow-stress / failure-strain networks (kmpm_surrogate)."""
+
+    def __init__(self, mat, eps_p_max=2.0, rate_max=1e6, triax_max=2.0, samples=1 << 20,
+                 iterations=4000, batch=4096, learning_rate=3e-3, seed=20261202):
+        self.L = lib()
+        c = SurrogateConfig()
+        c.eps_p_max, c.rate_max, c.triax_max = eps_p_max, rate_max, triax_max
+        c.samples, c.iterations, c.batch = samples, iterations, batch
+        c.learning_rate, c.seed = learning_rate, seed
+        self.cfg = c
+        self._mat = mat
+        h = C.c_void_p()
+        check(self.L.kmpm_surrogate_create(C.byref(mat), C.byref(c), C.byref(h)),
+              "kmpm_surrogate_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.kmpm_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def train(self):
+        loss = (C.c_double * 2)()
+        check(self.L.kmpm_surrogate_train(self.h, loss), "surrogate_train")
+        return loss[0], loss[1]
+
+    def tuples(self, net, n, seed):
+        X = np.zeros((n, 2), np.float32)
+        Y = np.zeros(n, np.float32)
+        check(self.L.kmpm_surrogate_tuples(self.h, net, n, seed, X.ctypes.data_as(_FP),
+                                           Y.ctypes.data_as(_FP)), "surrogate_tuples")
+        return X, Y
+
+    def eval(self, net, X):
+        X = _f32(X)
+        out = np.zeros(len(X), np.float32)
+        check(self.L.kmpm_surrogate_eval(self.h, net, X.ctypes.data_as(_FP), len(X),
+                                         out.ctypes.data_as(_FP)), "surrogate_eval")
+        return out
+
+    def gradient(self, net, X, Y):
+        X, Y = _f32(X), _f32(Y)
+        g = np.zeros(SURROGATE_NPARAM, np.float32)
+        loss = C.c_double()
+        check(self.L.kmpm_surrogate_gradient(self.h, net, X.ctypes.data_as(_FP),
+                                             Y.ctypes.data_as(_FP), len(X),
+                                             g.ctypes.data_as(_FP), C.byref(loss)),
+              "surrogate_gradient")
+        return g, loss.value
+
+    def params(self, net):
+        p = np.zeros(SURROGATE_NPARAM, np.float32)
+        check(self.L.kmpm_surrogate_params(self.h, net, p.ctypes.data_as(_FP), len(p), 0),
+              "surrogate_params")
+        return p
+
+    def set_params(self, net, p):
+        p = _f32(p).copy()
+        check(self.L.kmpm_surrogate_params(self.h, net, p.ctypes.data_as(_FP), len(p), 1),
+              "surrogate_params")
+
+    def normalization(self, net):
+        m = (C.c_double * 12)()
+        check(self.L.kmpm_surrogate_normalization(self.h, net, m), "surrogate_normalization")
+        return {"in_scale": np.array(m[0:2]), "in_shift": np.array(m[2:4]),
+                "out_scale": m[4], "out_shift": m[5], "base": np.array(m[6:12])}
+
+    def envelope(self):
+        lo, hi = (C.c_double * 4)(), (C.c_double * 4)()
+        check(self.L.kmpm_surrogate_envelope(self.h, lo, hi), "surrogate_envelope")
+        return np.array(lo[:]), np.array(hi[:])

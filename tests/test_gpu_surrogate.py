@@ -1,0 +1,226 @@
+"""The learned JC surrogate: tuples, BF16 matrix-core kernels, training,
+its use in the step, and an ablation against the method without it.
+
+Bars:
+  * training tuples equal the oracle's plain JC flow stress / failure strain
+    at the same states (relative 1e-5; the features are FP32);
+  * device forward = CPU oracle forward with the same BF16 rounding points
+    (2e-4 of the output scale, plus FP32 rounding of the output);
+  * device gradient = CPU oracle gradient (relative L2 2e-3 per parameter
+    block), the oracle itself being pinned by finite differences
+    (tests/test_mlp_oracle.py);
+  * trained networks: flow stress within 1e-3, failure strain within 1e-2
+    (max relative error on fresh tuples);
+  * in the step: mixed + surrogate within 1e-3 relative L2 (displacement,
+    stress) of the FP64 oracle; the same step with untrained networks is
+    at least 10x worse (what training gains); inputs outside the envelope
+    fall back to the plain functions.
+"""
+import ctypes as C
+import dataclasses
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from karamelo_amd import native as N
+from karamelo_amd import problems as P
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SQRT_3_OVER_2 = 1.224744871
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _mixed(prob):
+    cfg = N.Config.from_buffer_copy(prob.cfg)
+    cfg.precision = N.PRECISION_MIXED
+    return dataclasses.replace(prob, cfg=cfg)
+
+
+_TRAINED = {}
+
+
+def trained(iterations=6000):
+    """One trained surrogate per session for the steel of the tensile bar."""
+    if iterations not in _TRAINED:
+        s = N.Surrogate(P.steel_jc(), samples=1 << 18, iterations=iterations, batch=4096)
+        s.loss = s.train()
+        say("  surrogate trained: loss %.3e / %.3e" % s.loss)
+        _TRAINED[iterations] = s
+    return _TRAINED[iterations]
+
+
+def test_tuples_match_oracle():
+    mat = P.steel_jc()
+    s = N.Surrogate(mat, samples=4096, iterations=0, batch=4096)
+    L = O.load()
+    # flow stress: sigma_y / A at eps_p = f0^4, epsdot = epsdot0 e^f1
+    X, Y = s.tuples(0, 300, seed=5)
+    A = mat.strength_p[1]
+    G = mat.strength_p[0]
+    for (f0, f1), y in zip(X.astype(np.float64), Y):
+        ep = f0 ** 4
+        epsdot = mat.strength_p[4] * np.exp(f1)
+        d = 10 * A / G
+        Dm = np.array([d, 0, 0, 0, -d, 0, 0, 0, 0.0])
+        out = np.zeros(9)
+        dps = C.c_double()
+        L.orc_jc_strength(C.byref(mat), O._dp(np.zeros(9)), O._dp(Dm), 1.0, ep, epsdot, 0.0,
+                          O._dp(out), C.byref(dps))
+        assert dps.value > 0
+        ys = SQRT_3_OVER_2 * np.sqrt(np.sum(out * out))
+        assert abs(y - ys / A) <= 1e-5 * abs(ys / A), (f0, f1, y, ys / A)
+    # failure strain: ln eps_f at the recorded triaxiality and rate
+    X, Y = s.tuples(1, 300, seed=6)
+    for (tr, lr), y in zip(X.astype(np.float64), Y):
+        a = 1 / np.sqrt(3.0)
+        sdev = np.array([a, 0, 0, 0, -a, 0, 0, 0, 0.0])
+        vm = SQRT_3_OVER_2 * np.sqrt(np.sum(sdev * sdev))
+        pH = -tr * vm / (1 + 0.001 * tr) if tr <= 0 else -tr * vm / (1 - 0.001 * tr)
+        epsdot = mat.damage_p[5] * np.exp(lr)
+        di, dm = C.c_double(0), C.c_double(0)
+        L.orc_jc_damage(C.byref(mat), C.byref(di), C.byref(dm), pH, O._dp(sdev), epsdot, 1e-3)
+        ef = 1e-3 / di.value
+        assert abs(y - np.log(ef)) <= 1e-5 * max(1.0, abs(np.log(ef))), (tr, lr, y, np.log(ef))
+    s.close()
+
+
+def _norm(s, net):
+    m = s.normalization(net)
+    return {"in_scale": m["in_scale"], "in_shift": m["in_shift"],
+            "out_scale": m["out_scale"], "out_shift": m["out_shift"], "base": m["base"]}
+
+
+@pytest.mark.parametrize("net", [0, 1])
+def test_forward_matches_oracle(net):
+    s = N.Surrogate(P.steel_jc(), samples=4096, iterations=0, batch=4096)
+    rng = np.random.default_rng(10 + net)
+    p = rng.normal(scale=0.5, size=M.NPARAM).astype(np.float32)
+    s.set_params(net, p)
+    X, _ = s.tuples(net, 1000, seed=3)
+    out = s.eval(net, X)
+    nm = _norm(s, net)
+    ref = M.predict(p.astype(np.float64), X.astype(np.float64), nm)
+    err = np.abs(out - ref).max() / nm["out_scale"]
+    say("  forward max |device - oracle| = %.2e (normalised)" % err)
+    # 2e-4 of the output scale plus FP32 rounding of the (baseline + residual) sum
+    assert np.all(np.abs(out - ref) <= 2e-4 * nm["out_scale"] + 2.4e-7 * np.abs(ref))
+    s.close()
+
+
+@pytest.mark.parametrize("net", [0, 1])
+def test_gradient_matches_oracle(net):
+    s = N.Surrogate(P.steel_jc(), samples=4096, iterations=0, batch=4096)
+    rng = np.random.default_rng(20 + net)
+    p = rng.normal(scale=0.5, size=M.NPARAM).astype(np.float32)
+    s.set_params(net, p)
+    X, Y = s.tuples(net, 1000, seed=4)      # 1000: a partial last block too
+    g, loss = s.gradient(net, X, Y)
+    nm = _norm(s, net)
+    lo, go = M.loss_and_grad(p.astype(np.float64), X.astype(np.float64), Y.astype(np.float64), nm)
+    assert abs(loss - lo) <= 1e-3 * lo
+    gd, gr = M.unpack(g.astype(np.float64)), M.unpack(go)
+    for name in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        e = np.linalg.norm(np.ravel(gd[name] - gr[name])) / np.linalg.norm(np.ravel(gr[name]))
+        say("  grad %s rel L2 %.2e" % (name, e))
+        assert e <= 2e-3, name
+    s.close()
+
+
+def test_training_accuracy():
+    s = trained()
+    for net, tol in ((0, 1e-3), (1, 1e-2)):
+        X, Y = s.tuples(net, 20000, seed=99)
+        out = s.eval(net, X)
+        if net == 0:
+            rel = np.abs(out - Y) / np.abs(Y)
+        else:       # ln eps_f -> eps_f
+            rel = np.abs(np.exp(out - Y) - 1)
+        say("  net %d: max rel %.2e, rms rel %.2e" % (net, rel.max(), np.sqrt(np.mean(rel ** 2))))
+        assert rel.max() <= tol
+
+
+def _run(prob, steps, surrogate=None):
+    ctx = P.context_for(_mixed(prob))
+    if surrogate is not None:
+        ctx.set_surrogate(surrogate)
+    ctx.step(steps)
+    st, tag = ctx.synchronize()
+    assert st == 0, (st, tag)
+    out = ctx.download()
+    stats = ctx.surrogate_stats()
+    ctx.close()
+    return out, stats
+
+
+def _errors(got, ref, x0, G, n):
+    def rel(a, b, floor=0.0):
+        return np.linalg.norm(a - b) / max(np.linalg.norm(b), floor)
+    return {"u": rel(got["x"] - x0, ref["x"] - x0),
+            "sigma": rel(got["sigma"], ref["sigma"], 1e-6 * G * np.sqrt(n)),
+            "ep": rel(got["eff_plastic_strain"], ref["eff_plastic_strain"], 1e-6 * np.sqrt(n))}
+
+
+def test_surrogate_step_ablation():
+    """Tensile bar (JC + shock + damage, USL cubic, plastic): plain mixed vs
+    mixed + trained surrogate vs mixed + untrained networks, all against the
+    FP64 oracle."""
+    prob = P.tensile_bar(h=6e-4, v_pull=40.0)
+    steps = 300
+    x0 = prob.particles["x"].copy()
+    o = O.oracle_for(prob)
+    assert o.step(steps) == 0
+    ref = o.download()
+    G = prob.mats[0].G
+    plain, st0 = _run(prob, steps)
+    s = trained()
+    surr, st1 = _run(prob, steps, s)
+    untrained = N.Surrogate(P.steel_jc(), samples=4096, iterations=0, batch=4096)
+    bad, st2 = _run(prob, steps, untrained)
+    e0, e1, e2 = (_errors(r, ref, x0, G, prob.np) for r in (plain, surr, bad))
+    say("  plain mixed  : " + ", ".join("%s=%.2e" % kv for kv in e0.items()))
+    say("  + surrogate  : " + ", ".join("%s=%.2e" % kv for kv in e1.items()) +
+        "  (evals %d, fallbacks %d)" % st1)
+    say("  + untrained  : " + ", ".join("%s=%.2e" % kv for kv in e2.items()))
+    assert st0 == (0, 0)
+    assert st1[0] > 0 and st1[1] <= 0.01 * st1[0]
+    assert e1["u"] <= 1e-3 and e1["sigma"] <= 1e-3
+    assert e2["sigma"] >= 10 * e1["sigma"]
+    untrained.close()
+
+
+def test_envelope_fallback_uses_plain_functions():
+    """A surrogate whose flow-stress envelope ends below the plastic strains
+    the run reaches: those updates fall back to the plain functions."""
+    prob = P.tensile_bar(h=6e-4, v_pull=40.0)
+    tiny = N.Surrogate(P.steel_jc(), eps_p_max=1e-6, samples=1 << 16, iterations=500, batch=4096)
+    tiny.train()
+    plain, _ = _run(prob, 300)
+    got, (ev, fb) = _run(prob, 300, tiny)
+    say("  evals %d, fallbacks %d" % (ev, fb))
+    assert fb > 0
+    e = np.linalg.norm(got["sigma"] - plain["sigma"]) / np.linalg.norm(plain["sigma"])
+    assert e <= 1e-3
+    tiny.close()
+
+
+def test_set_surrogate_rejects_other_contexts():
+    s = trained()
+    fp64 = P.context_for(P.tensile_bar(h=6e-4))
+    with pytest.raises(N.KmpmError):
+        fp64.set_surrogate(s)           # FP64 path: the plain functions only
+    fp64.close()
+    other = P.tensile_bar(h=6e-4, d_params=(0.1, 3.44, -2.12, 0.002, 0.61))
+    ctx = P.context_for(_mixed(other))
+    with pytest.raises(N.KmpmError):
+        ctx.set_surrogate(s)            # trained for another material
+    ctx.close()
