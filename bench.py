@@ -5,12 +5,17 @@ Workload: 3D ULMPM tensile bar with Johnson-Cook strength, shock EOS and JC
 damage (steel), cubic B-splines, USL, 2x2x2 particles per cell, cell 0.06 mm:
 4.27M particles.  A step is one complete USL step over all particles
 (sort, P2G, grid update with the end-velocity fixes, fused G2P + F + stress +
-CFL, dt update), synthetic lattice inputs, state resident in HBM.
+CFL, dt update), synthetic lattice inputs, state resident in HBM.  The
+default warm-up (1000 steps, ~9 us) strains the bar past yield, so the timed
+steps run the plastic branch of the constitutive update.
 
 Paths (--paths, the last one is the headline `value`):
-  plain-fp64  FP64 throughout, parity with the reference to 1e-10;
-  mixed       FP32 state and transfers, FP64 position and deformation
-              gradient, 64-bit fixed-point node sums (deterministic).
+  plain-fp64       FP64 throughout, parity with the reference to 1e-10;
+  mixed            FP32 state and transfers, FP64 position and deformation
+                   gradient, 64-bit fixed-point node sums (deterministic);
+  mixed+surrogate  mixed, with the JC flow stress and failure strain from
+                   the learned BF16 networks (trained on the device before
+                   the timed region, on tuples of the plain functions).
 Each path runs the same steps from the same inputs; the headline's
 displacement and stress are compared with the plain FP64 run's.
 
@@ -56,10 +61,16 @@ ALG_BYTES_PER_PARTICLE = {
     "plain-fp64": {"g2p": 422, "p2g": 119, "grid": 14, "sort": 12},
     "mixed": {"g2p": 307, "p2g": 75, "grid": 10.5, "sort": 12},
 }
+ALG_BYTES_PER_PARTICLE["mixed+surrogate"] = ALG_BYTES_PER_PARTICLE["mixed"]
+# Matrix-core work of one network evaluation: the 32x32 hidden layer as two
+# 32x32x16 BF16 products per 32 samples = 2 * 32 * 32 flops per sample.
+MFMA_FLOPS_PER_EVAL = 2 * 32 * 32
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 KERNEL_NAMES = {"g2p": "k_g2p_usl", "p2g": "k_p2g", "grid": "k_grid_update", "sort": "sort"}
 # Whole step, same counting (state once in, once out, grid once).
-STEP_ALG_BYTES_PER_PARTICLE = {"plain-fp64": 442, "mixed": 324}
-DTYPE = {"plain-fp64": "f64", "mixed": "f32+f64(x,F)+i64(grid sums)"}
+STEP_ALG_BYTES_PER_PARTICLE = {"plain-fp64": 442, "mixed": 324, "mixed+surrogate": 324}
+DTYPE = {"plain-fp64": "f64", "mixed": "f32+f64(x,F)+i64(grid sums)",
+         "mixed+surrogate": "f32+f64(x,F)+i64(grid sums)+bf16(mfma, f32 acc)"}
 
 
 def log(*a):
@@ -89,20 +100,49 @@ def read_traffic(kernel, workload):
         return None
 
 
-def run_path(prob, path, steps, warmup):
+def train_surrogate(mat):
+    """Trains the JC networks on the device; returns the surrogate and a
+    record of the training (not part of the timed step)."""
+    from karamelo_amd import native as N
+    t0 = time.perf_counter()
+    s = N.Surrogate(mat, samples=1 << 18, iterations=6000, batch=4096)
+    loss = s.train()
+    device_sync()
+    t1 = time.perf_counter()
+    # accuracy on fresh tuples of the plain functions
+    err = {}
+    import numpy as np
+    for net, name in ((0, "flow_stress"), (1, "failure_strain")):
+        X, Y = s.tuples(net, 20000, seed=99)
+        out = s.eval(net, X)
+        rel = np.abs(out - Y) / np.abs(Y) if net == 0 else np.abs(np.exp(out - Y) - 1)
+        err[name] = {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel ** 2)))}
+    iters, batch = 6000, 4096
+    # training MFMA work: forward (1 product) + backward (2 products) per sample
+    flops = 2 * iters * batch * 3 * MFMA_FLOPS_PER_EVAL
+    rec = {"seconds": t1 - t0, "final_loss": list(loss), "validation": err,
+           "iterations_per_net": iters, "batch": batch,
+           "mfma_tflops": flops / (t1 - t0) / 1e12}
+    return s, rec
+
+
+def run_path(prob, path, steps, warmup, surrogate=None):
     """Times `steps` steps of one path after `warmup` steps; returns the
     timing record and the downloaded particle state."""
     from karamelo_amd import native as N
     from karamelo_amd import problems as P
     cfg = N.Config.from_buffer_copy(prob.cfg)
-    cfg.precision = N.PRECISION_MIXED if path == "mixed" else N.PRECISION_FP64
+    cfg.precision = N.PRECISION_FP64 if path == "plain-fp64" else N.PRECISION_MIXED
     ctx = P.context_for(dataclasses.replace(prob, cfg=cfg))
+    if path == "mixed+surrogate":
+        ctx.set_surrogate(surrogate)
     log("[%s] warm-up %d steps ..." % (path, warmup))
     ctx.step(warmup)
     st, tag = ctx.synchronize()
     if st != 0:
         raise RuntimeError("device error %d (particle %d) during warm-up" % (st, tag))
     device_sync()
+    ctx.surrogate_stats()          # resets the counters: the timed steps only
     log("[%s] timed %d steps ..." % (path, steps))
     ctx.timing_enable(True)
     device_sync()
@@ -116,10 +156,12 @@ def run_path(prob, path, steps, warmup):
     st, tag = ctx.synchronize()
     if st != 0:
         raise RuntimeError("device error %d (particle %d) in the timed region" % (st, tag))
+    evals, fallbacks = ctx.surrogate_stats()
     state = ctx.download()
     ctx.close()
     secs = t1 - t0
-    return {"secs": secs, "phases": phases}, state
+    return {"secs": secs, "phases": phases, "surrogate_evals": evals,
+            "surrogate_fallbacks": fallbacks}, state
 
 
 def roofline_of(path, phases, npart, workload):
@@ -173,11 +215,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1000,
+                    help="untimed steps; 1000 strain the bar into the plastic regime")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--h", type=float, default=6e-5, help="cell size (m)")
-    ap.add_argument("--paths", default="plain-fp64,mixed",
+    ap.add_argument("--paths", default="plain-fp64,mixed,mixed+surrogate",
                     help="comma-separated paths to time; the last one is the headline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -196,8 +239,12 @@ def main():
     log("particles: %d" % npart)
     x0 = prob.particles["x"].copy()
     results, states = {}, {}
+    surrogate, training = None, None
+    if "mixed+surrogate" in paths:
+        log("training the surrogate ...")
+        surrogate, training = train_surrogate(prob.mats[0])
     for path in paths:
-        results[path], states[path] = run_path(prob, path, args.steps, args.warmup)
+        results[path], states[path] = run_path(prob, path, args.steps, args.warmup, surrogate)
     prob.particles = None
 
     head = paths[-1]
@@ -214,13 +261,31 @@ def main():
             "step_alg_gbs": STEP_ALG_BYTES_PER_PARTICLE[path] * npart / (rp["secs"] / args.steps) / 1e9,
             "roofline": roofline_of(path, rp["phases"], npart, workload),
         }
+        if path == "mixed+surrogate":
+            # surrogate evaluations happen inside the fused G2P launches
+            g2p_ms, g2p_n = rp["phases"]["g2p"]
+            ev = rp["surrogate_evals"]
+            mf = ev * MFMA_FLOPS_PER_EVAL / (g2p_ms * 1e-3) / 1e12
+            per_path[path]["surrogate"] = {
+                "evaluations": ev, "fallbacks": rp["surrogate_fallbacks"],
+                "fallback_fraction": rp["surrogate_fallbacks"] / max(1, args.steps * npart),
+                "mfma_roofline": {
+                    "bound": "mfma", "flops_per_eval": MFMA_FLOPS_PER_EVAL, "achieved": mf,
+                    "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": mf / MFMA_BF16_PEAK_TFLOPS,
+                    "note": "network flops over the time of the fused G2P launches"},
+                "training": training}
     accuracy = None
     if "plain-fp64" in states and head != "plain-fp64":
-        ref, got = states["plain-fp64"], states[head]
-        accuracy = {"vs": "plain FP64 GPU run, same inputs and steps",
-                    "displacement_rel_l2": rel_l2(got["x"] - x0, ref["x"] - x0),
-                    "stress_rel_l2": rel_l2(got["sigma"], ref["sigma"]),
-                    "velocity_rel_l2": rel_l2(got["v"], ref["v"])}
+        ref = states["plain-fp64"]
+        accuracy = {"vs": "plain FP64 GPU run, same inputs and steps"}
+        for path in paths:
+            if path == "plain-fp64":
+                continue
+            got = states[path]
+            accuracy[path] = {"displacement_rel_l2": rel_l2(got["x"] - x0, ref["x"] - x0),
+                              "stress_rel_l2": rel_l2(got["sigma"], ref["sigma"]),
+                              "velocity_rel_l2": rel_l2(got["v"], ref["v"])}
     states = None
 
     cpu = None

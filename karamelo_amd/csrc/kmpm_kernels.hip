@@ -687,7 +687,8 @@ template <int MC, bool SURR = false>
 __device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
                                           double *sig, double *eel, double &ep, double &epdot,
                                           double &dinit, double &dmg, int mat, const double *v,
-                                          int pid, bool live = true) {
+                                          int pid, bool live = true,
+                                          const SurrogateDev *sd = nullptr) {
   const DevMat &m = a.mats[mat];
   const double rho = m.rho0 / J;
   double D[6];
@@ -698,8 +699,8 @@ __device__ inline double stress_and_speed(const KArgs &a, double dt, const doubl
   D[4] = 0.5 * (L[2] + L[6]);
   D[5] = 0.5 * (L[5] + L[7]);
   if constexpr (SURR)
-    update_stress_jc_surrogate(m, *a.surr, live, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit,
-                               dmg, a.sc);
+    update_stress_jc_surrogate(m, *sd, live, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg,
+                               a.sc);
   else
     update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   if (dmg >= 1.0) return -1.0;
@@ -720,12 +721,21 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
-__global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
+__global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   __shared__ R nb[2 * DIM * BN];
+  // SURR: the networks and envelope, staged in LDS once per block (read at
+  // each use instead of being held in registers across the particle loop)
+  __shared__ __attribute__((aligned(16))) unsigned char sbuf[SURR ? sizeof(SurrogateDev) : 16];
+  const SurrogateDev *sd = reinterpret_cast<const SurrogateDev *>(sbuf);
+  if constexpr (SURR) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(a.surr);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(sbuf);
+    for (int q = threadIdx.x; q < (int)(sizeof(SurrogateDev) / 4); q += blockDim.x) dst[q] = src[q];
+  }
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -810,7 +820,7 @@ __global__ __launch_bounds__(256) void k_g2p_usl(KArgs a) {
       double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
       double dinit = LDA(DINIT, slot);
       const double c = stress_and_speed<MC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
-                                                  mat, v, pid, live);
+                                                  mat, v, pid, live, sd);
       if (!live) return;
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {

@@ -101,11 +101,11 @@ __device__ inline int mlp_drow(int i) {
 
 // Network output (in target units) for this lane's own features (f0, f1);
 // the network's affine maps are applied here.  Every lane of the wave must
-// call it (the matrix instructions use the whole wave).
+// call it (the matrix instructions use the whole wave).  Layer 1 is formed
+// one k-step (8 units) at a time, just before the matrix instruction that
+// consumes it, to keep few values live.
 __device__ inline float mlp_eval_wave(const MlpNet &n, float f0, float f1) {
   const int lane = threadIdx.x & 63, half = lane >> 5, col = lane & 31;
-  bf16x8_t a0, a1;
-  mlp_load_w2(n, a0, a1);
   const float x0 = fmaf(f0, n.in_scale[0], n.in_shift[0]);
   const float x1 = fmaf(f1, n.in_scale[1], n.in_shift[1]);
   float own = 0.0f;
@@ -113,15 +113,26 @@ __device__ inline float mlp_eval_wave(const MlpNet &n, float f0, float f1) {
   for (int T = 0; T < 2; T++) {
     const int src = 32 * T + col;
     const float g0 = __shfl(x0, src, 64), g1 = __shfl(x1, src, 64);
-    float h[16];
-    bf16x8_t b0, b1;
-    mlp_layer1(n.p, g0, g1, h, b0, b1);
     f32x16_t d = {};
-    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, d, 0, 0, 0);
+#pragma unroll
+    for (int st = 0; st < 2; st++) {
+      // Weights are re-read at each use: the opaque barriers keep the
+      // compiler from hoisting all of them into registers up front, which
+      // would cost ~90 registers in the kernels that call this.
+      asm volatile("" ::: "memory");
+      bf16x8_t a, b;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int k = 16 * st + 8 * half + i;
+        a[i] = bf16_from_bits(n.w2_bf16[col][k]);
+        b[i] = to_bf16(tanh_fast(fmaf(n.p.w1[k][1], g1, fmaf(n.p.w1[k][0], g0, n.p.b1[k]))));
+      }
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, d, 0, 0, 0);
+    }
     float part = 0.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
+      if ((i & 7) == 0) asm volatile("" ::: "memory");
       const int o = mlp_drow(i);
       part = fmaf(n.p.w3[o], tanh_fast(d[i] + n.p.b2[o]), part);
     }

@@ -32,6 +32,20 @@ struct SurrogateDev {
   double sy_virgin;
 };
 
+// The plain functions as out-of-line calls: the fallback is rare, and kept
+// out of line its registers do not add to the step kernel's footprint.
+__device__ __noinline__ void jc_strength_fallback(const DevMat &m, const double *sig,
+                                                  const double *D, double dt, double eps_p,
+                                                  double epsdot, double damage, double *out,
+                                                  double *dps) {
+  jc_strength(m, sig, D, dt, eps_p, epsdot, damage, out, dps);
+}
+__device__ __noinline__ void jc_damage_fallback(const DevMat &m, double &damage_init,
+                                                double &damage, double pH, const double *sdev,
+                                                double epsdot, double dps) {
+  jc_damage(m, damage_init, damage, pH, sdev, epsdot, dps);
+}
+
 // Counts one wave's network evaluations and fallbacks.
 __device__ inline void surrogate_count(DevScalars *sc, bool eval, bool fallback) {
   const unsigned long long ne = __ballot(eval), nf = __ballot(fallback);
@@ -65,7 +79,7 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
   if (__ballot(use_y)) ny = mlp_eval_wave(sd.y, fy0, fy1);
   double sdev[6], dps = 0.0;
   if (fb_y) {
-    jc_strength(m, sig, D, dt, ep, epdot, dmg, sdev, &dps);
+    jc_strength_fallback(m, sig, D, dt, ep, epdot, dmg, sdev, &dps);
   } else if (dead) {
     for (int e = 0; e < 6; e++) sdev[e] = 0;
   } else {
@@ -112,7 +126,7 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
     float nf = 0.0f;
     if (__ballot(use_f)) nf = mlp_eval_wave(sd.f, (float)triax, (float)lnr);
     if (fb_f) {
-      jc_damage(m, dinit, dmg, pH, sdev, epdot, dps);
+      jc_damage_fallback(m, dinit, dmg, pH, sdev, epdot, dps);
     } else if (reset) {
       dinit = 0;
     } else if (!skip) {
