@@ -37,10 +37,13 @@ enum {
   KMPM_ERR_DTCFL = 12,              /* dtCFL NaN or 0, ulmpm.cpp:535-544 */
   KMPM_ERR_WAVE_SPEED = 13,         /* NaN wave speed, solid.cpp:1387-1398 */
   KMPM_ERR_GRID = 14,               /* grid too small for the stencil */
-  KMPM_ERR_FIXED_POINT = 15         /* mixed path: a node sum left the range of
+  KMPM_ERR_FIXED_POINT = 15,        /* mixed path: a node sum left the range of
                                        its 64-bit fixed-point scale (no
                                        counterpart in the reference, which
                                        accumulates in FP64) */
+  KMPM_ERR_INTERNAL = 16            /* a particle was found outside the tile it
+                                       was sorted into (an internal
+                                       inconsistency; its access is clamped) */
 };
 
 /* ---- options ----------------------------------------------------------- */
@@ -67,6 +70,11 @@ typedef struct kmpm_config {
   int    dt_constant;     /* nonzero after set_dt (update.cpp:63-70) */
   int    reserved1;
   double dt_factor;       /* dt_factor (default 0.9, update.cpp:40) */
+  /* TLMPM: limits of the solid's region (Region::limits, see
+   * kmpm_region_limits), the extent of the solid's own grid
+   * (Grid::init(solidlo, solidhi), grid.cpp:91-97, solid.cpp:1833-1838,1927-1930).
+   * Unused by ULMPM, whose grid covers the box. */
+  double solidlo[3], solidhi[3];
 } kmpm_config;
 
 /* ---- materials: material(...) + eos/strength/damage styles ------------- */
@@ -134,9 +142,12 @@ typedef struct kmpm_particles {
 const char *kmpm_status_string(int status);
 /* Derive rho0/K/G/E/nu/lambda/signal_velocity (Mat ctors, material.cpp:720-769). */
 int kmpm_material_init(kmpm_material *mat);
-/* Grid node counts per dimension (Grid::init, grid.cpp:151-173, one rank). */
+/* Grid node counts per dimension (Grid::init, grid.cpp:151-173, one rank):
+ * over the box for ULMPM, over cfg->solidlo/solidhi for TLMPM. */
 int kmpm_grid_size(const kmpm_config *cfg, int64_t n_out[3]);
-/* Lattice population of a region (Solid::populate, solid.cpp:1810-2336, one rank).
+/* Lattice population of a region (Solid::populate, solid.cpp:1810-2336, one rank);
+ * the lattice starts at the box for ULMPM and at the region's limits for
+ * TLMPM (solid.cpp:1927-1936).
  * Writes up to `capacity` particles into `out` (arrays must be allocated);
  * *np_out receives the count (also when it exceeds capacity -> KMPM_ERR_CAPACITY).
  * ptag = ptag_offset + i + 1 (solid.cpp:2320). */
@@ -146,6 +157,9 @@ int kmpm_populate(const kmpm_config *cfg, const kmpm_region *region, int ppc1d,
                   int64_t *np_out);
 /* Region::match for one point (region.cpp:65-72 + the style's inside()). */
 int kmpm_region_match(const kmpm_region *region, int dimension, double x, double y, double z);
+/* Region::limits: bounding box lo/hi of a region (region_block.cpp:159-168,
+ * region_cylinder.cpp:113-137,173-182). */
+int kmpm_region_limits(const kmpm_region *region, double lo[3], double hi[3]);
 
 /* ---- device context ------------------------------------------------------ */
 typedef struct kmpm_ctx kmpm_ctx;

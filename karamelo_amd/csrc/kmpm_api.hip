@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kmpm_ctx.h"
+#include "kmpm_mat3.h"
 #include "kmpm_surrogate.h"
 
 using namespace kmpm;
@@ -189,7 +190,10 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
                     kmpm_ctx **out) {
   if (!cfg || !out || nmat < 1 || nmat > 16 || !mats) return KMPM_ERR_ARG;
   *out = nullptr;
-  if (cfg->method != KMPM_METHOD_ULMPM) return KMPM_ERR_ARG;
+  if (cfg->method != KMPM_METHOD_ULMPM && cfg->method != KMPM_METHOD_TLMPM) return KMPM_ERR_ARG;
+  // TLMPM runs the FP64 path
+  if (cfg->method == KMPM_METHOD_TLMPM && cfg->precision != KMPM_PRECISION_FP64)
+    return KMPM_ERR_ARG;
   if (cfg->scheme != KMPM_SCHEME_USL && cfg->scheme != KMPM_SCHEME_MUSL) return KMPM_ERR_ARG;
   if (cfg->shape != KMPM_SHAPE_LINEAR && cfg->shape != KMPM_SHAPE_CUBIC_SPLINE) return KMPM_ERR_ARG;
   if (cfg->precision != KMPM_PRECISION_FP64 && cfg->precision != KMPM_PRECISION_MIXED)
@@ -221,13 +225,16 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   }
   KArgs &a = c->a;
   a.dim = cfg->dimension;
+  a.tl = cfg->method == KMPM_METHOD_TLMPM;
   a.prec = cfg->precision;
   a.S1 = S1;
   a.scheme = cfg->scheme;
   for (int d = 0; d < 3; d++) {
     a.n[d] = (int)n3[d];
-    a.lo[d] = d < cfg->dimension ? cfg->boxlo[d] : 0.0;
-    a.hi[d] = d < cfg->dimension ? cfg->boxhi[d] : 0.0;
+    // the grid's bounds: the box (ULMPM) or the solid's limits (TLMPM)
+    const bool tl = cfg->method == KMPM_METHOD_TLMPM;
+    a.lo[d] = d < cfg->dimension ? (tl ? cfg->solidlo[d] : cfg->boxlo[d]) : 0.0;
+    a.hi[d] = d < cfg->dimension ? (tl ? cfg->solidhi[d] : cfg->boxhi[d]) : 0.0;
   }
   a.nn = n3[0] * n3[1] * n3[2];
   a.h = cfg->cellsize;
@@ -297,6 +304,8 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
     memset(&sc, 0, sizeof(sc));
     sc.dt = cfg->dt;
     sc.err_pid = -1;
+    const double one = 1.0;
+    memcpy(&sc.min_ratio_bits, &one, sizeof(one));
     ok = ok && hipMemcpy(c->d_sc, &sc, sizeof(sc), hipMemcpyHostToDevice) == hipSuccess;
   }
   a.mats = c->d_mats;
@@ -356,8 +365,12 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   a.np = (int)np;
   a.cap = cap;
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
-  HIPCHK(dalloc(&c->d_pa, (size_t)(mixed ? MX_ND : NDF) * cap));
-  HIPCHK(dalloc(&c->d_pb, (size_t)(mixed ? MX_ND : NDF) * cap));
+  const int ndf = mixed ? MX_ND : (a.tl ? NDF_TL : NDF);
+  if (a.tl)
+    for (int64_t i = 1; i < np; i++)
+      if (p->solid[i] != p->solid[0]) return KMPM_ERR_ARG;   // one solid: one TL grid
+  HIPCHK(dalloc(&c->d_pa, (size_t)ndf * cap));
+  HIPCHK(dalloc(&c->d_pb, (size_t)ndf * cap));
   if (mixed) {
     HIPCHK(dalloc(&c->d_fa, (size_t)MX_NF * cap));
     HIPCHK(dalloc(&c->d_fb, (size_t)MX_NF * cap));
@@ -382,7 +395,7 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   c->h_solid = (int32_t *)malloc(sizeof(int32_t) * cap);
   c->h_pmask = (uint32_t *)malloc(sizeof(uint32_t) * cap);
   c->h_x0 = (double *)malloc(sizeof(double) * 3 * cap);
-  std::vector<double> hd((size_t)NDF * cap, 0.0);
+  std::vector<double> hd((size_t)(a.tl ? NDF_TL : NDF) * cap, 0.0);
   std::vector<int> hi((size_t)NIF * cap, 0);
   auto D = [&](int f, int64_t i) -> double & { return hd[(size_t)f * cap + i]; };
   for (int64_t i = 0; i < np; i++) {
@@ -408,6 +421,17 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
     D(MASS, i) = p->mass[i];
     D(VOL0, i) = p->vol0[i];
     D(VOL, i) = p->vol ? p->vol[i] : p->vol0[i];
+    if (a.tl) {
+      // reference position and vol0 PK1 = vol0 J (R sigma R^T) F^-T of the
+      // uploaded F and sigma (zero for an unstressed solid)
+      double F[9], Finv[9], R[9], P[9];
+      for (int e = 0; e < 9; e++) F[e] = p->F[9 * i + e];
+      for (int d = 0; d < 3; d++) D(X0X + d, i) = p->x0 ? p->x0[3 * i + d] : p->x[3 * i + d];
+      m3_inverse(F, Finv);
+      if (!polar_rotation(F, R)) return KMPM_ERR_JACOBIAN;
+      pk1(p->vol0[i], m3_det(F), R, s, Finv, P);
+      for (int e = 0; e < 9; e++) D(P00 + e, i) = P[e];
+    }
     hi[(size_t)PID * cap + i] = (int)i;
     hi[(size_t)MATI * cap + i] = p->mat[i];
   }
@@ -425,6 +449,7 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   HIPCHK(hipMemcpy(c->d_pmask, c->h_pmask, np * sizeof(uint32_t), hipMemcpyHostToDevice));
   refresh_args(c);
   c->keys_valid = false;
+  c->tl_identity = false;
   c->sorted = false;
   return KMPM_OK;
 }
@@ -434,7 +459,7 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
   const int64_t np = c->a.np;
   if (p->np < np) return KMPM_ERR_CAPACITY;
   const int cap = c->a.cap;
-  std::vector<double> hd((size_t)NDF * cap);
+  std::vector<double> hd((size_t)(c->a.tl ? NDF_TL : NDF) * cap);
   std::vector<int> hi((size_t)NIF * cap);
   std::vector<double> dd;
   std::vector<float> ff;
@@ -530,6 +555,13 @@ int kmpm_fix_body_force(kmpm_ctx *c, uint32_t bit, const int set[3], const doubl
 // ---- phases ---------------------------------------------------------------
 int kmpm_compute_grid_weight_functions_and_gradients(kmpm_ctx *c) {
   if (!c) return KMPM_ERR_ARG;
+  if (c->tl_identity) {
+    // TLMPM weights are those of x0 (computed once, tlmpm.cpp:89,339): after
+    // the first step the state is stored in its sorted order and the cell
+    // lists stay valid
+    c->sorted = true;
+    return KMPM_OK;
+  }
   tbegin(c, KMPM_PHASE_SORT);
   if (!c->keys_valid) {
     HIPCHK(launch_keys(c));
@@ -557,15 +589,25 @@ int kmpm_update_grid_state(kmpm_ctx *c) {
   return KMPM_OK;
 }
 
+// After the G2P that wrote the state in sorted order: TLMPM keeps that
+// order for good (its keys come from x0), with the identity permutation.
+static int after_g2p_state_swap(kmpm_ctx *c) {
+  swap_state(c);
+  c->sorted = false;
+  c->keys_valid = true;
+  if (c->a.tl && !c->tl_identity) {
+    HIPCHK(launch_iota(c->d_order, c->a.np, c->stream));
+    c->tl_identity = true;
+  }
+  return KMPM_OK;
+}
+
 int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_G2P);
   HIPCHK(launch_g2p_usl(c));
   tend(c);
-  swap_state(c);
-  c->sorted = false;
-  c->keys_valid = true;
-  return KMPM_OK;
+  return after_g2p_state_swap(c);
 }
 
 int kmpm_grid_to_points(kmpm_ctx *c) {
@@ -592,10 +634,7 @@ int kmpm_update_stress(kmpm_ctx *c) {
   tbegin(c, KMPM_PHASE_STRESS_B);
   HIPCHK(launch_stress_musl_b(c));
   tend(c);
-  swap_state(c);
-  c->sorted = false;
-  c->keys_valid = true;
-  return KMPM_OK;
+  return after_g2p_state_swap(c);
 }
 
 int kmpm_adjust_dt(kmpm_ctx *c) {

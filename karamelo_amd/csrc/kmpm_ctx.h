@@ -34,6 +34,7 @@ struct KArgs {
   const DevMat *mats;
   TileGeom tg;
   // particles
+  int tl;           // TLMPM: weights from x0 on the solid's grid, F += dt Fdot
   int prec;         // KMPM_PRECISION_*
   int np, cap;
   double *pa;       // current state: FP64 NDF * cap, mixed MX_ND * cap
@@ -109,6 +110,8 @@ struct kmpm_ctx {
   std::vector<int> *ev_phase;         // phase of each pair
   bool keys_valid;       // d_key holds keys of the current positions
   bool sorted;           // order/tiles valid for the current step
+  bool tl_identity;      // TLMPM: the state is in its final sorted order (keys
+                         // come from x0 and never change)
 };
 
 namespace kmpm {
@@ -125,6 +128,7 @@ hipError_t launch_grid_clean(kmpm_ctx *c);
 hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);
 hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double *d_wd);
 hipError_t launch_cells(kmpm_ctx *c, int *d_ijk);
+hipError_t launch_iota(int *d, int n, hipStream_t stream);
 size_t scan_temp_bytes(int64_t n);
 DevMat make_devmat(const kmpm_material &m);
 int surrogate_device_image(const kmpm_surrogate *s, SurrogateDev *out);

@@ -24,6 +24,13 @@ enum DField {
   MASS, VOL0, VOL,           // mass, vol0, vol
   NDF                        // number of double fields
 };
+// TLMPM fields after the common ones: reference position x0 and vol0 PK1
+// (first Piola-Kirchhoff stress times vol0, row-major), solid.h:50-97.
+enum TLField {
+  X0X = NDF, X0Y, X0Z,
+  P00, P01, P02, P10, P11, P12, P20, P21, P22,
+  NDF_TL
+};
 // Int fields: PID = index into the static (upload-order) arrays, MAT = material.
 enum IField { PID = 0, MATI, NIF };
 
@@ -80,6 +87,8 @@ struct DevScalars {
   int64_t err_pid;                // pid of the first offending particle
   int tile_count;                 // active tiles this step
   int dil_count;                  // tiles whose owned nodes receive contributions
+  unsigned long long min_ratio_bits;  // TLMPM: smallest |Re eig(F)| of the step, at most 1
+                                      // (bits of a non-negative double)
   unsigned long long surr_evals;      // particle updates that used a surrogate network
   unsigned long long surr_fallbacks;  // particle updates outside the surrogate's envelope
 };

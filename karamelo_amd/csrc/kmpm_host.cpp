@@ -61,6 +61,7 @@ const char *kmpm_status_string(int s) {
     case KMPM_ERR_WAVE_SPEED: return "Error: max_p_wave_speed is nan";
     case KMPM_ERR_GRID: return "grid does not support the stencil on one device";
     case KMPM_ERR_FIXED_POINT: return "node sum exceeded the fixed-point range of the mixed path";
+    case KMPM_ERR_INTERNAL: return "internal error: particle outside its sorted tile";
     default: return "unknown status";
   }
 }
@@ -104,17 +105,47 @@ int kmpm_material_init(kmpm_material *m) {
 int kmpm_grid_size(const kmpm_config *c, int64_t out[3]) {
   if (!c || !(c->cellsize > 0) || c->dimension < 2 || c->dimension > 3) return KMPM_ERR_ARG;
   const double h = c->cellsize;
+  const bool tl = c->method == KMPM_METHOD_TLMPM;
   for (int d = 0; d < 3; d++) {
     if (d >= c->dimension) {
       out[d] = 1;
       continue;
     }
-    const double L = c->boxhi[d] - c->boxlo[d];
+    // the bounds of the grid: the box (ULMPM) or the solid (TLMPM), grid.cpp:91-97
+    const double lo = tl ? c->solidlo[d] : c->boxlo[d];
+    const double hi = tl ? c->solidhi[d] : c->boxhi[d];
+    if (tl && (lo < c->boxlo[d] || hi > c->boxhi[d] || !(hi > lo))) return KMPM_ERR_ARG;
+    const double L = hi - lo;
     const int g = global_nodes(L, h, d > 0);
-    const int l = local_nodes(L, c->boxlo[d], c->boxhi[d], h);
+    const int l = local_nodes(L, lo, hi, h);
     if (g != l) return KMPM_ERR_GRID;
     out[d] = g;
   }
+  return KMPM_OK;
+}
+
+int kmpm_region_limits(const kmpm_region *r, double lo[3], double hi[3]) {
+  if (!r || !lo || !hi) return KMPM_ERR_ARG;
+  const double *p = r->p;
+  if (r->kind == KMPM_REGION_BLOCK) {   // region_block.cpp:159-168
+    for (int d = 0; d < 3; d++) {
+      lo[d] = p[2 * d];
+      hi[d] = p[2 * d + 1];
+    }
+    return KMPM_OK;
+  }
+  if (r->kind != KMPM_REGION_CYLINDER) return KMPM_ERR_ARG;
+  // region_cylinder.cpp:113-137: the axis spans [lo, hi], the two others
+  // c1 -+ R and c2 -+ R
+  const double c1 = p[0], c2 = p[1], R = p[2], alo = p[3], ahi = p[4];
+  const int ax = r->axis;
+  const int o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;
+  lo[ax] = alo;
+  hi[ax] = ahi;
+  lo[o1] = c1 - R;
+  hi[o1] = c1 + R;
+  lo[o2] = c2 - R;
+  hi[o2] = c2 + R;
   return KMPM_OK;
 }
 
@@ -128,16 +159,28 @@ int kmpm_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
                   const kmpm_material *mat, int mat_index, int solid_index, int64_t ptag_offset,
                   kmpm_particles *out, int64_t capacity, int64_t *np_out) {
   if (!c || !region || !mat || !out || !np_out) return KMPM_ERR_ARG;
-  if (c->method != KMPM_METHOD_ULMPM) return KMPM_ERR_ARG;
+  if (c->method != KMPM_METHOD_ULMPM && c->method != KMPM_METHOD_TLMPM) return KMPM_ERR_ARG;
   const int dim = c->dimension;
   const double delta = c->cellsize;
-  // Lattice of cells over the box (Solid::populate, solid.cpp:1933-2002,
-  // one rank: sub-domain == box, no neighbour adjustments).
-  int ncell[3] = {1, 1, 1};
+  // Lattice of cells over the bounds (Solid::populate, solid.cpp:1927-2002,
+  // one rank: sub-domain == box, no neighbour adjustments).  The bounds are
+  // the box for ULMPM and the region's limits for TLMPM.
+  double blo[3], bhi[3];
+  if (c->method == KMPM_METHOD_TLMPM) {
+    kmpm_region_limits(region, blo, bhi);
+  } else {
+    for (int d = 0; d < 3; d++) {
+      blo[d] = c->boxlo[d];
+      bhi[d] = c->boxhi[d];
+    }
+  }
+  int ncell[3] = {1, 1, 1}, noff[3] = {0, 0, 0};
   for (int d = 0; d < dim; d++) {
-    const double L = c->boxhi[d] - c->boxlo[d];
-    int n = std::max(0, (int)std::ceil(std::max(0.0, L) / delta));
-    while (c->boxlo[d] + delta * (0 + n - 0.5) < c->boxhi[d]) n++;
+    const double Llo = std::max(0.0, c->boxlo[d] - blo[d]);
+    const double Lhi = std::max(0.0, std::min(c->boxhi[d], bhi[d]) - blo[d]);
+    noff[d] = (int)std::floor(Llo / delta);
+    int n = std::max(0, (int)std::ceil(Lhi / delta) - noff[d]);
+    while (blo[d] + delta * (noff[d] + n - 0.5) < std::min(c->boxhi[d], bhi[d])) n++;
     ncell[d] = n;
   }
   // Integration-point offsets inside a cell (solid.cpp:2026-2121).
@@ -175,9 +218,9 @@ int kmpm_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
     for (int j = 0; j < ncell[1]; j++)
       for (int k = 0; k < ncell[2]; k++)
         for (int q = 0; q < nip; q++) {
-          const double x = c->boxlo[0] + delta * (0 + i + 0.5 + ip[3 * q]);
-          const double y = c->boxlo[1] + delta * (0 + j + 0.5 + ip[3 * q + 1]);
-          const double z = dim == 3 ? c->boxlo[2] + delta * (0 + k + 0.5 + ip[3 * q + 2]) : 0;
+          const double x = blo[0] + delta * (noff[0] + i + 0.5 + ip[3 * q]);
+          const double y = blo[1] + delta * (noff[1] + j + 0.5 + ip[3 * q + 1]);
+          const double z = dim == 3 ? blo[2] + delta * (noff[2] + k + 0.5 + ip[3 * q + 2]) : 0;
           // Domain::inside_subdomain (domain.cpp:187-195) then Region::inside
           bool in_box = x >= c->boxlo[0] && x <= c->boxhi[0] && y >= c->boxlo[1] &&
                         y <= c->boxhi[1];

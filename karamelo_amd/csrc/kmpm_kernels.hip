@@ -19,6 +19,7 @@
 
 #include "kmpm_constitutive.h"
 #include "kmpm_ctx.h"
+#include "kmpm_mat3.h"
 #include "kmpm_surrogate.h"
 
 namespace kmpm {
@@ -74,6 +75,19 @@ __device__ inline int cell_of_clamped(double x, double lo, double inv_h, int n) 
   return (int)u;
 }
 
+// Cell of a particle relative to the origin of the tile it was sorted into.
+// The sort guarantees 0 <= result < T; a value outside (keys out of date)
+// would address outside the tile's LDS node block, so it is clamped and
+// reported instead.
+__device__ inline int tile_local(int c, int org, int T, DevScalars *sc) {
+  const int l = c - org;
+  if (l < 0 || l >= T) {
+    set_error(sc, KMPM_ERR_INTERNAL, -1);
+    return l < 0 ? 0 : T - 1;
+  }
+  return l;
+}
+
 __device__ inline int tile_key(const KArgs &a, int c0, int c1, int c2) {
   const TileGeom &g = a.tg;
   int t = ((c0 / g.T[0]) * g.nt[1] + c1 / g.T[1]) * g.nt[2] + c2 / g.T[2];
@@ -91,10 +105,14 @@ __device__ inline int particle_key(const KArgs &a, double x, double y, double z)
 // ---------------------------------------------------------------------------
 // sort
 // ---------------------------------------------------------------------------
+// Weight positions: current x (ULMPM) or reference x0 (TLMPM).
+__device__ inline int wpos(const KArgs &a) { return a.tl ? X0X : PX; }
+
 __global__ void k_keys(KArgs a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
-  a.key[i] = particle_key(a, PF(a.pa, PX, i), PF(a.pa, PY, i), PF(a.pa, PZ, i));
+  const int f = wpos(a);
+  a.key[i] = particle_key(a, PF(a.pa, f, i), PF(a.pa, f + 1, i), PF(a.pa, f + 2, i));
 }
 
 __global__ void k_sort_begin(KArgs a, int64_t ncells) {
@@ -192,21 +210,42 @@ __global__ void k_dilate(KArgs a) {
 constexpr int SCR_FIELDS = 7;   // scratch slot layout: mass, momentum[3], force[3]
 
 // Raw particle data a P2G lane loads for one particle (R: the transfer type).
-template <int DIM, bool REMAP, class R>
+// Stress terms a P2G particle carries: -vol sigma (symmetric: 6 in 3D,
+// xx,yy,xy in 2D) or, TLMPM, -vol0 PK1 (row-major: 9 in 3D, 4 in 2D).
+template <int DIM, bool TL> constexpr int p2g_nsig() {
+  return TL ? (DIM == 3 ? 9 : 4) : (DIM == 3 ? 6 : 3);
+}
+
+template <int DIM, bool REMAP, bool TL, class R>
 struct P2GIn {
   double x[DIM];
   R v[DIM], m;
-  R s[REMAP ? 1 : (DIM == 3 ? 6 : 3)];   // -vol * sigma (xx,yy,zz,xy,xz,yz | xx,yy,xy)
+  R s[REMAP ? 1 : p2g_nsig<DIM, TL>()];
 };
 
-template <int DIM, bool REMAP, int PREC, class R>
-__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, R> &in) {
+template <int DIM, bool REMAP, bool TL, int PREC, class R>
+__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R> &in) {
   if (REMAP) {
     for (int d = 0; d < DIM; d++) {
-      in.x[d] = a.xo[(size_t)d * a.cap + j];
+      in.x[d] = TL ? PF(a.pb, X0X + d, j) : a.xo[(size_t)d * a.cap + j];
       in.v[d] = (R)LDB(VX + d, j);
     }
     in.m = (R)LDB(MASS, j);
+  } else if (TL) {
+    const int slot = a.order[j];
+    for (int d = 0; d < DIM; d++) {
+      in.x[d] = PF(a.pa, X0X + d, slot);
+      in.v[d] = (R)LDA(VX + d, slot);
+    }
+    in.m = (R)LDA(MASS, slot);
+    if (DIM == 3) {
+      for (int e = 0; e < 9; e++) in.s[e] = (R)-PF(a.pa, P00 + e, slot);
+    } else {
+      in.s[0] = (R)-PF(a.pa, P00, slot);
+      in.s[1] = (R)-PF(a.pa, P01, slot);
+      in.s[2] = (R)-PF(a.pa, P10, slot);
+      in.s[3] = (R)-PF(a.pa, P11, slot);
+    }
   } else {
     const int slot = a.order[j];
     for (int d = 0; d < DIM; d++) {
@@ -225,7 +264,7 @@ __device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, R> &in)
   }
 }
 
-template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP, int PREC>
+template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP, int PREC, bool TL>
 __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   // FP64: double transfer arithmetic, double LDS node sums.  Mixed: float
   // transfer arithmetic, register sums flushed into 64-bit fixed-point LDS
@@ -240,7 +279,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
-  constexpr int NSIG = DIM == 3 ? 6 : 3;
+  constexpr int NSIG = p2g_nsig<DIM, TL>();
   constexpr int NW = DIM * S1;                              // (w, dw) pairs per particle
   constexpr int NF = REMAP ? (1 + DIM) : (1 + 2 * DIM);
   constexpr int NP2 = 2 + (REMAP ? 0 : (NSIG + 1) / 2);    // payload pairs: (m,v0),(v1,v2),sigma
@@ -284,10 +323,10 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
     const int npart = p1 - p0;
     // The particle data of the wave's next chunk is loaded into registers
     // while the current chunk is accumulated (hides HBM latency).
-    P2GIn<DIM, REMAP, R> in;
+    P2GIn<DIM, REMAP, TL, R> in;
     int c0 = wave * CH;
     if (c0 < npart && lane < min(CH, npart - c0))
-      p2g_load<DIM, REMAP, PREC>(a, p0 + c0 + lane, in);
+      p2g_load<DIM, REMAP, TL, PREC>(a, p0 + c0 + lane, in);
     for (; c0 < npart; c0 += 4 * CH) {
       const int cnt = min(CH, npart - c0);
       if (lane < cnt) {
@@ -295,7 +334,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
           const int c = cell_of_clamped(in.x[d], a.lo[d], a.inv_h, a.n[d]);
-          cl[d] = c - org[d];
+          cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
           double w[S1], dw[S1];
           weights_1d<S1>(in.x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
 #pragma unroll
@@ -313,7 +352,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
       wave_sync();
       const int cn = c0 + 4 * CH;
       if (cn < npart && lane < min(CH, npart - cn))
-        p2g_load<DIM, REMAP, PREC>(a, p0 + cn + lane, in);
+        p2g_load<DIM, REMAP, TL, PREC>(a, p0 + cn + lane, in);
 
       // Accumulate the group's run of particles; the LDS operands of the
       // next particle are loaded before the arithmetic of the current one
@@ -355,7 +394,22 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
           acc[1] = fma(wm, o.mv0.y, acc[1]);
           acc[2] = fma(wm, o.v12.x, acc[2]);
           if (DIM == 3) acc[3] = fma(wm, o.v12.y, acc[3]);
-          if (!REMAP) {
+          if (!REMAP && TL) {
+            // f_d += sum_e (-vol0 PK1)_de g_e, row-major pairs
+            constexpr int K = NS2 > 0 ? NS2 : 1;
+            if (DIM == 3) {
+              const R q00 = o.sg[0].x, q01 = o.sg[0].y, q02 = o.sg[1 % K].x, q10 = o.sg[1 % K].y,
+                      q11 = o.sg[2 % K].x, q12 = o.sg[2 % K].y, q20 = o.sg[3 % K].x,
+                      q21 = o.sg[3 % K].y, q22 = o.sg[4 % K].x;
+              acc[4] = fma(q02, g2, fma(q01, g1, fma(q00, g0, acc[4])));
+              acc[5] = fma(q12, g2, fma(q11, g1, fma(q10, g0, acc[5])));
+              acc[6] = fma(q22, g2, fma(q21, g1, fma(q20, g0, acc[6])));
+            } else {
+              const R q00 = o.sg[0].x, q01 = o.sg[0].y, q10 = o.sg[1 % K].x, q11 = o.sg[1 % K].y;
+              acc[3] = fma(q01, g1, fma(q00, g0, acc[3]));
+              acc[4] = fma(q11, g1, fma(q10, g0, acc[4]));
+            }
+          } else if (!REMAP) {
             constexpr int K = NS2 > 0 ? NS2 : 1;
             if (DIM == 3) {
               // f_d += sum_e (-vol sigma)_de g_e ; sym order xx,yy,zz,xy,xz,yz
@@ -766,7 +820,7 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
-        cl[d] = c - org[d];
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
         weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
       double vu[3], acc[3], L[9];
@@ -845,9 +899,147 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// TLMPM particle update after the gather (FP64): solid.cpp:1155-1244 (TL
+// branch) and 1246-1437.  F += dt Fdot, F^-1, J, vol, rho, R = polar(F),
+// D = sym(R^T (L + L^T) R) / 2 with L = Fdot F^-1, the stress update,
+// vol0 PK1, the wave speed and the smallest |Re eig(F)| (both -1 for fully
+// damaged particles, which the CFL skips).  Writes slot j of the next state.
+// ---------------------------------------------------------------------------
+__device__ inline void block_min_ratio(double val, DevScalars *sc) {
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    const double other = __shfl_xor(val, o, 64);
+    val = other < val ? other : val;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) red[wave] = val;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] < m ? red[w] : m;
+    if (m < 1.0) atomicMin(&sc->min_ratio_bits, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+template <int MC>
+__device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot, int j,
+                                 int pid, int mat, const double *v, double &cmax,
+                                 double &rmin) {
+  constexpr int PREC = KMPM_PRECISION_FP64;
+  double F[9], Finv[9], R[9];
+  for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot) + dt * Fdot[e];   // :1181-1182
+  m3_inverse(F, Finv);                                                   // :1186
+  const double J = m3_det(F);                                            // :1203
+  const double vol0 = LDA(VOL0, slot);
+  double dmg = LDA(DMG, slot);
+  if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);    // :1208-1215
+  if (!polar_rotation(F, R)) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);    // :1222-1232
+  double D[6];
+  tl_rate(Fdot, Finv, R, D);                                             // :1225-1226
+  const DevMat &m = a.mats[mat];
+  const double rho = m.rho0 / J;
+  double sig[6], eel[6];
+  for (int e = 0; e < 6; e++) {
+    sig[e] = LDA(SXX + e, slot);
+    eel[e] = LDA(EXX + e, slot);
+  }
+  double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot), dinit = LDA(DINIT, slot);
+  update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  double S[9] = {sig[0], sig[3], sig[4], sig[3], sig[1], sig[5], sig[4], sig[5], sig[2]};
+  double P[9];
+  pk1(vol0, J, R, S, Finv, P);
+  for (int e = 0; e < 9; e++) {
+    STB(F00 + e, j, F[e]);
+    STB(P00 + e, j, P[e]);
+  }
+  STB(VOL0, j, vol0);
+  STB(VOL, j, J * vol0);
+  for (int e = 0; e < 6; e++) {
+    STB(SXX + e, j, sig[e]);
+    STB(EXX + e, j, eel[e]);
+  }
+  STB(EP, j, ep);
+  STB(EPDOT, j, epdot);
+  STB(DINIT, j, dinit);
+  STB(DMG, j, dmg);
+  if (dmg >= 1.0) return;                                                // :1378-1379
+  const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
+  const double c = sqrt(m.cfl_k43g / rho) + vm;
+  if (isnan(c)) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
+  if (c > cmax) cmax = c;
+  double re[3];
+  eig_real_parts(F, re);                                                 // :1400-1408
+  for (int k = 0; k < 3; k++) rmin = fabs(re[k]) < rmin ? fabs(re[k]) : rmin;
+}
+
+// TLMPM fused USL G2P: weights of x0 on the solid's grid, Fdot(v_update),
+// G2P, advance (no box check), then tl_update.
+template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+__global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
+  constexpr int PREC = KMPM_PRECISION_FP64;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ double nb[2 * DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double dt = a.sc->dt;
+  const double inv_dt = 1.0 / dt;
+  const double alpha = a.flip;
+  double cmax = 0, rmin = 1.0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x0[3], x[3], v[3];
+      for (int d = 0; d < 3; d++) {
+        x0[d] = PF(a.pa, X0X + d, slot);
+        x[d] = PF(a.pa, PX + d, slot);
+        v[d] = PF(a.pa, VX + d, slot);
+      }
+      double w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double vu[3] = {0, 0, 0}, acc[3] = {0, 0, 0}, Fdot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vu, acc, Fdot);
+      const int pid = IFLD(a.ia, PID, slot);
+      const int mat = IFLD(a.ia, MATI, slot);
+      for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
+      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];                 // solid.cpp:616
+      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      for (int d = 0; d < 3; d++) {
+        PF(a.pb, PX + d, j) = x[d];
+        PF(a.pb, VX + d, j) = v[d];
+        PF(a.pb, X0X + d, j) = x0[d];
+      }
+      a.key[j] = particle_key(a, x0[0], x0[1], x0[2]);
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
+      tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+    }
+  }
+  block_max_to(cmax, a.sc);
+  block_min_ratio(rmin, a.sc);
+}
+
+// ---------------------------------------------------------------------------
 // MUSL part A: G2P + advance (musl.cpp:61-67); keeps start-of-step x
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ, int PREC>
+// TL: weights of x0 on the solid's grid, no box check, x0 carried along.
+template <int DIM, int S1, int TX, int TY, int TZ, int PREC, bool TL>
 __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
@@ -875,13 +1067,15 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
         x[d] = LDA(PX + d, slot);
         v[d] = LDA(VX + d, slot);
       }
+      double xw[3];
+      for (int d = 0; d < 3; d++) xw[d] = TL ? PF(a.pa, X0X + d, slot) : x[d];
       R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
-        const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
-        cl[d] = c - org[d];
-        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        const int c = cell_of_clamped(xw[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d_as<S1>(xw[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
       double vu[3], acc[3];
       {
@@ -894,17 +1088,23 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
         }
       }
       const int pid = IFLD(a.ia, PID, slot);
-      for (int d = 0; d < 3; d++) a.xo[(size_t)d * a.cap + j] = x[d];
+      if (TL) {
+        for (int d = 0; d < 3; d++) PF(a.pb, X0X + d, j) = xw[d];
+      } else {
+        for (int d = 0; d < 3; d++) a.xo[(size_t)d * a.cap + j] = x[d];
+      }
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
       for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
-      if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+      if (!TL && outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
       for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
       for (int d = 0; d < 3; d++) {
         STB(PX + d, j, x[d]);
         STB(VX + d, j, v[d]);
       }
       STB(MASS, j, LDA(MASS, slot));
-      a.key[j] = particle_key(a, x[0], x[1], x[2]);
+      // next step's cell: of the advanced x (ULMPM) or of x0 (TLMPM)
+      const double *xk = TL ? xw : x;
+      a.key[j] = particle_key(a, xk[0], xk[1], xk[2]);
     }
   }
 }
@@ -941,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
-        cl[d] = c - org[d];
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
         weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
       double L[9];
@@ -992,6 +1192,57 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
   block_max_to(cmax, a.sc);
 }
 
+// TLMPM MUSL part B: Fdot from the re-mapped v with the x0 weights
+// (compute_rate_deformation_gradient_TL(true), musl.cpp:75-77), then
+// tl_update.
+template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+__global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
+  constexpr int PREC = KMPM_PRECISION_FP64;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ double nb[DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double dt = a.sc->dt;
+  double cmax = 0, rmin = 1.0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x0[3];
+      for (int d = 0; d < 3; d++) x0[d] = PF(a.pa, X0X + d, slot);
+      double w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double Fdot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, (double *)nullptr,
+                                                             (double *)nullptr, Fdot);
+      const int pid = IFLD(a.ia, PID, slot);
+      const int mat = IFLD(a.ia, MATI, slot);
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      double v[3];
+      for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
+      tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+    }
+  }
+  block_max_to(cmax, a.sc);
+  block_min_ratio(rmin, a.sc);
+}
+
 // update_time + adjust_dt (update.cpp:201-206, ulmpm.cpp:525-551)
 __global__ void k_adjust_dt(KArgs a) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -999,7 +1250,9 @@ __global__ void k_adjust_dt(KArgs a) {
   sc->time += sc->dt;
   if (!a.dt_constant) {
     const double maxc = __longlong_as_double((long long)sc->max_c_bits);
-    double dtcfl = a.h * 1.0 / maxc;
+    // TLMPM scales by the smallest |Re eig(F)| (solid.cpp:1425); ULMPM by 1
+    const double ratio = a.tl ? __longlong_as_double((long long)sc->min_ratio_bits) : 1.0;
+    double dtcfl = a.h * ratio / maxc;
     dtcfl = 1.0e22 < dtcfl ? 1.0e22 : dtcfl;
     if ((dtcfl == 0 || isnan(dtcfl)) && sc->error == 0) {
       sc->error = KMPM_ERR_DTCFL;
@@ -1008,6 +1261,7 @@ __global__ void k_adjust_dt(KArgs a) {
     sc->dt = dtcfl * a.dt_factor;
   }
   sc->max_c_bits = 0;
+  sc->min_ratio_bits = (unsigned long long)__double_as_longlong(1.0);
   sc->step += 1;
 }
 
@@ -1081,7 +1335,7 @@ __global__ void k_debug_weights(KArgs a, int64_t *tag_out, double *w_out, double
   for (int d = 0; d < 3; d++)
     for (int q = 0; q < S1; q++) { w[d][q] = 1; dw[d][q] = 0; }
   for (int d = 0; d < a.dim; d++) {
-    const double x = PF(a.pa, PX + d, i);
+    const double x = PF(a.pa, wpos(a) + d, i);
     const int c = cell_of_clamped(x, a.lo[d], a.inv_h, a.n[d]);
     base[d] = c - off;
     weights_1d<S1>(x, a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
@@ -1123,7 +1377,7 @@ __global__ void k_cells(KArgs a, int *ijk) {
   for (int d = 0; d < 3; d++) {
     int v = 0;
     if (d < a.dim) {
-      const double x = PF(a.pa, PX + d, i);
+      const double x = PF(a.pa, wpos(a) + d, i);
       if (a.S1 == 2) v = (int)((x - a.lo[d]) * a.inv_h);
       else v = (int)((x - a.lo[d]) * a.inv_h - 1);
     }
@@ -1206,14 +1460,27 @@ hipError_t launch_sort(kmpm_ctx *c) {
 hipError_t launch_p2g(kmpm_ctx *c, int remap) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
+  if (c->a.tl) {   // TLMPM: FP64 only
+    DISPATCH(c, {
+      using TT = Tiles<DIM, S1>;
+      constexpr int PREC = KMPM_PRECISION_FP64;
+      if (remap)
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+      else
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+    });
+    return hipGetLastError();
+  }
   DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
     if (remap)
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC>), dim3(g),
-                         dim3(256), 0, c->stream, c->a);
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC, false>),
+                         dim3(g), dim3(256), 0, c->stream, c->a);
     else
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC>), dim3(g),
-                         dim3(256), 0, c->stream, c->a);
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC, false>),
+                         dim3(g), dim3(256), 0, c->stream, c->a);
   }));
   return hipGetLastError();
 }
@@ -1255,6 +1522,14 @@ hipError_t launch_grid_clean(kmpm_ctx *c) {
 hipError_t launch_g2p_usl(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
+  if (c->a.tl) {
+    DISPATCH(c, MC_DISPATCH(c, {
+      using TT = Tiles<DIM, S1>;
+      hipLaunchKernelGGL((k_g2p_usl_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256),
+                         0, c->stream, c->a);
+    }));
+    return hipGetLastError();
+  }
   const bool surr = c->a.surr_on && c->a.mat_class == MC_JC_SHOCK &&
                     c->a.prec == KMPM_PRECISION_MIXED;
   DISPATCH(c, MC_DISPATCH(c, PREC_DISPATCH(c, {
@@ -1273,10 +1548,18 @@ hipError_t launch_g2p_usl(kmpm_ctx *c) {
 hipError_t launch_g2p_musl_a(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
+  if (c->a.tl) {
+    DISPATCH(c, {
+      using TT = Tiles<DIM, S1>;
+      hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, KMPM_PRECISION_FP64, true>),
+                         dim3(g), dim3(256), 0, c->stream, c->a);
+    });
+    return hipGetLastError();
+  }
   DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
-    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, PREC>), dim3(g), dim3(256),
-                       0, c->stream, c->a);
+    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, PREC, false>), dim3(g),
+                       dim3(256), 0, c->stream, c->a);
   }));
   return hipGetLastError();
 }
@@ -1284,6 +1567,14 @@ hipError_t launch_g2p_musl_a(kmpm_ctx *c) {
 hipError_t launch_stress_musl_b(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
+  if (c->a.tl) {
+    DISPATCH(c, MC_DISPATCH(c, {
+      using TT = Tiles<DIM, S1>;
+      hipLaunchKernelGGL((k_stress_musl_b_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+    }));
+    return hipGetLastError();
+  }
   DISPATCH(c, MC_DISPATCH(c, PREC_DISPATCH(c, {
     using TT = Tiles<DIM, S1>;
     hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC>), dim3(g),
@@ -1319,6 +1610,17 @@ hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double
 hipError_t launch_cells(kmpm_ctx *c, int *d_ijk) {
   if (c->a.np == 0) return hipSuccess;
   hipLaunchKernelGGL(k_cells, dim3(nblk(c->a.np, 256)), dim3(256), 0, c->stream, c->a, d_ijk);
+  return hipGetLastError();
+}
+
+__global__ void k_iota(int *d, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = i;
+}
+
+hipError_t launch_iota(int *d, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, dim3(nblk(n, 256)), dim3(256), 0, stream, d, n);
   return hipGetLastError();
 }
 

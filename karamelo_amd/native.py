@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libkmpm.so")
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NO_DEVICE, ERR_CAPACITY = 1, 2, 3, 4
 ERR_PARTICLE_LEFT_BOX, ERR_JACOBIAN, ERR_DTCFL, ERR_WAVE_SPEED, ERR_GRID = 10, 11, 12, 13, 14
-ERR_FIXED_POINT = 15
+ERR_FIXED_POINT, ERR_INTERNAL = 15, 16
 SURROGATE_NPARAM = 1185
 METHOD_ULMPM, METHOD_TLMPM = 0, 1
 SCHEME_USL, SCHEME_MUSL = 0, 1
@@ -48,7 +48,8 @@ class Config(C.Structure):
                 ("shape", C.c_int), ("precision", C.c_int), ("reserved0", C.c_int),
                 ("boxlo", D3), ("boxhi", D3), ("cellsize", C.c_double),
                 ("pic_flip", C.c_double), ("dt", C.c_double), ("dt_constant", C.c_int),
-                ("reserved1", C.c_int), ("dt_factor", C.c_double)]
+                ("reserved1", C.c_int), ("dt_factor", C.c_double),
+                ("solidlo", D3), ("solidhi", D3)]
 
 
 class Material(C.Structure):
@@ -135,6 +136,7 @@ def _declare(lib):
         "kmpm_populate": (C.c_int, [P(Config), P(Region), C.c_int, P(Material), C.c_int, C.c_int,
                                     C.c_int64, P(Particles), C.c_int64, P(C.c_int64)]),
         "kmpm_region_match": (C.c_int, [P(Region), C.c_int, C.c_double, C.c_double, C.c_double]),
+        "kmpm_region_limits": (C.c_int, [P(Region), D3, D3]),
         "kmpm_ctx_create": (C.c_int, [P(Config), P(Material), C.c_int, P(vp)]),
         "kmpm_ctx_destroy": (C.c_int, [vp]),
         "kmpm_upload_particles": (C.c_int, [vp, P(Particles)]),
@@ -210,6 +212,13 @@ def check(code, what=""):
 def material_init(m):
     check(lib().kmpm_material_init(C.byref(m)), "kmpm_material_init")
     return m
+
+
+def region_limits(region):
+    """Region::limits: (lo[3], hi[3]) of a region's bounding box."""
+    lo, hi = D3(), D3()
+    check(lib().kmpm_region_limits(C.byref(region), lo, hi), "kmpm_region_limits")
+    return tuple(lo), tuple(hi)
 
 
 def grid_size(cfg):

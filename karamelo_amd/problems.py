@@ -50,7 +50,9 @@ class Problem:
 
 
 def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-16,
-                dt_constant=False, dt_factor=0.9, method="ulmpm"):
+                dt_constant=False, dt_factor=0.9, method="ulmpm", solid_region=None):
+    """kmpm_config for dimension/method/scheme/shape commands; for TLMPM the
+    solid's region gives the limits of its grid (Region::limits)."""
     c = N.Config()
     c.dimension = dim
     c.method = N.METHOD_KEYS[method]
@@ -65,6 +67,11 @@ def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-1
     c.dt = dt
     c.dt_constant = 1 if dt_constant else 0
     c.dt_factor = dt_factor
+    if solid_region is not None:
+        lo_, hi_ = N.region_limits(solid_region)
+        for d in range(3):
+            c.solidlo[d] = lo_[d] if d < dim else 0.0
+            c.solidhi[d] = hi_[d] if d < dim else 0.0
     return c
 
 
@@ -249,6 +256,49 @@ def dam_break(n_cells=32, ppc1d=2, steps=0, scheme="musl", shape="cubic-spline",
             sv[axis] = 1
             fixes.append(Fix("velocity_nodes", bit, tuple(sv), (0, 0, 0), (0, 0, 0)))
     return Problem("dam_break", cfg, [mat], p, node_mask=mask, fixes=fixes, steps=steps)
+
+
+def copper_jc():
+    """Copper with Johnson-Cook strength and shock EOS, SI units (the Taylor
+    bar of the reference's examples with the published JC constants):
+    E = 115 GPa, nu = 0.31 -> G; JC A = 90 MPa, B = 292 MPa, n = 0.31,
+    C = 0.025, epsdot0 = 1/s, m = 1.09, Tr = 0, Tm = 1356; shock rho0 = 8960,
+    K = rho0 c0^2, c0 = 3940 m/s, S = 1.49, Gamma = 2.0, cv = 0, Q1 = Q2 = 0."""
+    E, nu, rho, c0 = 115e9, lit(0.31), 8960.0, 3940.0
+    G = E / (2 * (1 + nu))
+    K = rho * c0 * c0
+    eos_p = (rho, K, c0, lit(1.49), 2.0, 0.0, 0.0, 0.0, 0.0)
+    jc_p = (G, 90e6, 292e6, lit(0.31), 1.0, lit(0.025), lit(1.09), 0.0, 1356.0)
+    return eos_strength_material("shock", eos_p, "johnson_cook", jc_p)
+
+
+def taylor_bar(h=1e-4, ppc1d=1, v0=190.0, shape="cubic-spline", scheme="usl", radius=3.2e-3,
+               length=32.4e-3, steps=0, populate_fn=None, dt_factor=0.5):
+    """3D TLMPM Taylor bar: copper cylinder along x (radius 3.2 mm, length
+    32.4 mm, centred at the origin) moving at +v0 towards a rigid wall at
+    its +x end, which is the velocity_nodes fix (zero x velocity) on the
+    grid nodes of the end face, as in
+    examples/Taylor-bar/cylindrical/TLMPM/cylindrical_taylor_bar.mpm:60-88.
+    h = 0.1 mm with 1 particle per cell gives ~1.04M particles."""
+    L2 = length / 2
+    cyl = cylinder(0.0, 0.0, radius, -L2, L2, axis=0)
+    margin = 2 * h
+    cfg = make_config(3, (-L2 - margin, -radius - margin, -radius - margin),
+                      (L2 + margin, radius + margin, radius + margin), h, scheme=scheme,
+                      shape=shape, flip=0.99, dt_factor=dt_factor, method="tlmpm",
+                      solid_region=cyl)
+    mat = copper_jc()
+    p = populate_solids(cfg, [(cyl, ppc1d, mat, 0)], fn=populate_fn)
+    p["v"][:, 0] = v0
+    # TL grid nodes (reference positions of the solid's grid)
+    n = N.grid_size(cfg)
+    X = np.repeat(cfg.solidlo[0] + h * np.arange(n[0]), n[1] * n[2])
+    mask = np.ones(int(np.prod(n)), dtype=np.uint32)
+    wall = X >= L2 - 0.5 * h          # region(x_wall - h/2, INF, ...)
+    mask[wall] |= 2
+    fixes = [Fix("velocity_nodes", 2, (1, 0, 0), (0, 0, 0), (0, 0, 0))]
+    return Problem("taylor_bar", cfg, [mat], p, node_mask=mask, fixes=fixes, steps=steps,
+                   notes="fix initial_velocity_particles applied at set-up (step-1 hook)")
 
 
 def context_for(problem):

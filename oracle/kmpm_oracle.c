@@ -10,7 +10,8 @@
  * reference's C++/Eigen expressions left to right; the library is compiled
  * with -ffp-contract=off so no fused multiply-add changes a rounding.
  *
- * Scope: ULMPM with the USL and MUSL schemes, linear and cubic-spline shape
+ * Scope: ULMPM and TLMPM (one solid, its own grid over the solid's limits)
+ * with the USL and MUSL schemes, linear and cubic-spline shape
  * functions, 2D and 3D, materials linear / eos-strength with eos
  * linear|shock|fluid, strength linear|johnson_cook|fluid, damage
  * damage_johnson_cook; fixes velocity_nodes (constant values) and
@@ -30,6 +31,110 @@ static double dmax(double a, double b) { return a > b ? a : b; } /* MAX macro */
 static double dmin(double a, double b) { return a < b ? a : b; }
 static int imin(int a, int b) { return a < b ? a : b; }
 static double square(double x) { return x * x; }                /* math_special.h */
+
+/* ------------------------------------------------------------------------ */
+/* 3x3 matrix helpers (row-major), in the order Eigen's fixed-size           */
+/* expressions evaluate: a product coefficient sums k = 0, 1, 2 in turn.     */
+/* ------------------------------------------------------------------------ */
+static void m3_mul(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+static void m3_transpose(const double *A, double *T) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) T[3 * j + i] = A[3 * i + j];
+}
+/* cofactor matrix C (C[i][j] = cofactor of A[i][j]); A^-1 = C^T / det A */
+static void m3_cofactors(const double *A, double *C) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      C[3 * i + j] = A[3 * i1 + j1] * A[3 * i2 + j2] - A[3 * i1 + j2] * A[3 * i2 + j1];
+    }
+}
+/* Matrix3d::inverse (Eigen's 3x3 path: cofactors of column 0 give the
+ * determinant, every entry is a cofactor times 1/det) */
+static void m3_inverse(const double *A, double *Ai) {
+  double C[9];
+  m3_cofactors(A, C);
+  double det = C[0] * A[0] + C[3] * A[3] + C[6] * A[6];
+  double invdet = 1.0 / det;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) Ai[3 * i + j] = C[3 * j + i] * invdet;
+}
+static double m3_det(const double *m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+/* Rotation R of the polar decomposition F = R U (PolDec, mpm_math.h:104-133,
+ * which takes R = U V^T from a JacobiSVD; R is unique for det F > 0).
+ * Newton's iteration X <- (X + X^-T) / 2 from X = F, until the largest
+ * change is at most 1e-9, then one more iteration (quadratic convergence
+ * takes it to rounding level).  Returns 0 when det X <= 0. */
+static int polar_rotation(const double *F, double *R) {
+  double X[9], C[9];
+  memcpy(X, F, 9 * sizeof(double));
+  int last = 0;
+  for (int it = 0; it < 100; it++) {
+    m3_cofactors(X, C);
+    double det = X[0] * C[0] + X[1] * C[1] + X[2] * C[2];
+    if (!(det > 0)) return 0;
+    double inv = 1.0 / det, diff = 0;
+    for (int e = 0; e < 9; e++) {
+      double xn = 0.5 * (X[e] + C[e] * inv);   /* X^-T = C / det */
+      diff = dmax(diff, fabs(xn - X[e]));
+      X[e] = xn;
+    }
+    if (last) break;
+    if (diff <= 1e-9) last = 1;
+  }
+  memcpy(R, X, 9 * sizeof(double));
+  return m3_det(R) > 0.0;
+}
+/* Real parts of the eigenvalues of F (EigenSolver<Matrix3d>, solid.cpp:1401-1408):
+ * the characteristic polynomial of G = F - s I, s = tr F / 3, is the depressed
+ * cubic t^3 + p t + q (p = sum of the principal 2x2 minors of G, q = -det G);
+ * three real roots in trigonometric form, otherwise one real root (Cardano)
+ * and a complex pair of real part -t/2. */
+static void eig_real_parts(const double *F, double *re) {
+  double s = (F[0] + F[4] + F[8]) / 3.0;
+  double G[9];
+  memcpy(G, F, 9 * sizeof(double));
+  G[0] -= s;
+  G[4] -= s;
+  G[8] -= s;
+  double p = (G[0] * G[4] - G[1] * G[3]) + (G[0] * G[8] - G[2] * G[6]) + (G[4] * G[8] - G[5] * G[7]);
+  double q = -m3_det(G);
+  if (p == 0 && q == 0) {
+    re[0] = re[1] = re[2] = s;
+    return;
+  }
+  double disc = q * q / 4.0 + p * p * p / 27.0;
+  if (disc > 0) {
+    double sq = sqrt(disc);
+    double t = cbrt(-q / 2.0 + sq) + cbrt(-q / 2.0 - sq);
+    re[0] = s + t;
+    re[1] = re[2] = s - t / 2.0;
+  } else {
+    double m = 2.0 * sqrt(-p / 3.0);
+    double arg = (3.0 * q / (2.0 * p)) * sqrt(-3.0 / p);
+    arg = dmin(1.0, dmax(-1.0, arg));
+    double th = acos(arg) / 3.0;
+    for (int k = 0; k < 3; k++) re[k] = s + m * cos(th - 2.0943951023931957 * k);
+  }
+}
+/* vol0PK1 = vol0 J (R sigma R^T) F^-T (solid.cpp:1277-1280,1368-1372) */
+static void pk1(double vol0, double J, const double *R, const double *sig, const double *Finv,
+                double *out) {
+  double Rt[9], RS[9], RSRt[9], FiT[9], a = vol0 * J;
+  m3_transpose(R, Rt);
+  m3_mul(R, sig, RS);
+  m3_mul(RS, Rt, RSRt);
+  for (int e = 0; e < 9; e++) RSRt[e] = a * RSRt[e];
+  m3_transpose(Finv, FiT);
+  m3_mul(RSRt, FiT, out);
+}
 
 /* ------------------------------------------------------------------------ */
 /* shape functions: basis_functions.h:21-104                                 */
@@ -123,18 +228,26 @@ int orc_material_init(kmpm_material *m) {
 /* ------------------------------------------------------------------------ */
 /* grid: Grid::init for one rank, grid.cpp:68-275                            */
 /* ------------------------------------------------------------------------ */
+/* bounds of the grid: the box (ULMPM) or the solid (TLMPM), grid.cpp:91-97 */
+static const double *grid_lo(const kmpm_config *c) {
+  return c->method == KMPM_METHOD_TLMPM ? c->solidlo : c->boxlo;
+}
+static const double *grid_hi(const kmpm_config *c) {
+  return c->method == KMPM_METHOD_TLMPM ? c->solidhi : c->boxhi;
+}
 static void grid_global_counts(const kmpm_config *c, int n[3]) {
   double h = c->cellsize;
-  double Lx = c->boxhi[0] - c->boxlo[0];                        /* :151-155 */
+  const double *blo = grid_lo(c), *bhi = grid_hi(c);
+  double Lx = bhi[0] - blo[0];                                  /* :151-155 */
   n[0] = ((int)(Lx / h)) + 1;
   while (n[0] * h <= Lx + 0.5 * h) n[0]++;
   if (c->dimension >= 2) {                                      /* :157-163 */
-    double Ly = c->boxhi[1] - c->boxlo[1];
+    double Ly = bhi[1] - blo[1];
     n[1] = (int)(((int)Ly) / h) + 1;
     while (n[1] * h <= Ly + 0.5 * h) n[1]++;
   } else n[1] = 1;
   if (c->dimension == 3) {                                      /* :165-171 */
-    double Lz = c->boxhi[2] - c->boxlo[2];
+    double Lz = bhi[2] - blo[2];
     n[2] = (int)(((int)Lz) / h) + 1;
     while (n[2] * h <= Lz + 0.5 * h) n[2]++;
   } else n[2] = 1;
@@ -142,17 +255,24 @@ static void grid_global_counts(const kmpm_config *c, int n[3]) {
 /* local node counts for a rank that owns the whole box (:177-206) */
 static void grid_local_counts(const kmpm_config *c, int n[3]) {
   double h = c->cellsize;
+  const double *blo = grid_lo(c), *bhi = grid_hi(c);
   for (int d = 0; d < 3; d++) {
     if (d >= c->dimension) { n[d] = 1; continue; }
-    double L = c->boxhi[d] - c->boxlo[d];
+    /* sub-domain == box: Loffsetlo = max(0, boxlo - lo) = 0 for a solid in the box */
+    double L = dmin(c->boxhi[d], bhi[d]) - blo[d];
     n[d] = (int)ceil(L / h);
     if (n[d] < 0) n[d] = 0;
-    while (c->boxlo[d] + h * (0 + n[d] - 0.5) < c->boxhi[d]) n[d]++;
+    while (blo[d] + h * (0 + n[d] - 0.5) < dmin(c->boxhi[d], bhi[d])) n[d]++;
   }
 }
 int orc_grid_size(const kmpm_config *c, int64_t out[3]) {
   int g[3], l[3];
   if (!c || c->cellsize <= 0) return KMPM_ERR_ARG;
+  if (c->method == KMPM_METHOD_TLMPM)
+    for (int d = 0; d < c->dimension; d++)
+      if (c->solidlo[d] < c->boxlo[d] || c->solidhi[d] > c->boxhi[d] ||
+          !(c->solidhi[d] > c->solidlo[d]))
+        return KMPM_ERR_ARG;
   grid_global_counts(c, g);
   grid_local_counts(c, l);
   for (int d = 0; d < 3; d++) {
@@ -187,6 +307,25 @@ static int region_inside(const kmpm_region *r, double x, double y, double z) {
   }
   return 0;
 }
+/* Region::limits: region_block.cpp:159-168; region_cylinder.cpp:113-137,173-182 */
+int orc_region_limits(const kmpm_region *r, double lo[3], double hi[3]) {
+  if (r->kind == KMPM_REGION_BLOCK) {
+    lo[0] = r->p[0]; hi[0] = r->p[1];
+    lo[1] = r->p[2]; hi[1] = r->p[3];
+    lo[2] = r->p[4]; hi[2] = r->p[5];
+    return KMPM_OK;
+  }
+  double c1 = r->p[0], c2 = r->p[1], R = r->p[2], alo = r->p[3], ahi = r->p[4];
+  if (r->axis == 0) {                 /* x: xlo=lo, ylo=c1-R, zlo=c2-R */
+    lo[0] = alo; hi[0] = ahi; lo[1] = c1 - R; hi[1] = c1 + R; lo[2] = c2 - R; hi[2] = c2 + R;
+  } else if (r->axis == 1) {          /* y: xlo=c1-R, ylo=lo, zlo=c2-R */
+    lo[0] = c1 - R; hi[0] = c1 + R; lo[1] = alo; hi[1] = ahi; lo[2] = c2 - R; hi[2] = c2 + R;
+  } else {                            /* z: xlo=c1-R, ylo=c2-R, zlo=lo */
+    lo[0] = c1 - R; hi[0] = c1 + R; lo[1] = c2 - R; hi[1] = c2 + R; lo[2] = alo; hi[2] = ahi;
+  }
+  return KMPM_OK;
+}
+
 int orc_region_match(const kmpm_region *r, int dimension, double x, double y, double z) {
   (void)dimension;
   int in = region_inside(r, x, y, z);
@@ -200,8 +339,15 @@ int orc_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
                  const kmpm_material *mat, int mat_index, int solid_index,
                  int64_t ptag_offset, kmpm_particles *out, int64_t capacity,
                  int64_t *np_out) {
-  if (c->method != KMPM_METHOD_ULMPM) return KMPM_ERR_ARG;
-  const double *boundlo = c->boxlo, *boundhi = c->boxhi;       /* :1933-1936 */
+  double rlo[3], rhi[3];
+  const double *boundlo = c->boxlo, *boundhi = c->boxhi;       /* :1927-1936 */
+  if (c->method == KMPM_METHOD_TLMPM) {
+    orc_region_limits(region, rlo, rhi);                        /* :1831-1838 */
+    boundlo = rlo;
+    boundhi = rhi;
+  } else if (c->method != KMPM_METHOD_ULMPM) {
+    return KMPM_ERR_ARG;
+  }
   const double *sublo = c->boxlo, *subhi = c->boxhi;
   double delta = c->cellsize;                                   /* :1863 */
   int dim = c->dimension;
@@ -318,6 +464,9 @@ struct orc_sim {
   double *x0, *x, *v, *vu, *a, *F, *sig, *eel, *L, *D;
   double *mass, *vol0, *vol, *rho0, *rho, *J;
   double *ep, *epdot, *dmg, *dinit;
+  /* TLMPM: Fdot, F^-1, R (polar), vol0 PK1 (solid.h:50-97) */
+  double *Fdot, *Finv, *R, *PK1;
+  int tl, wf_done, mass_done;         /* update_wf, update_mass_nodes (tlmpm.cpp:89,345) */
   /* neighbour lists: particle-major (neigh_pn) and node-major CSR (neigh_np) */
   int32_t *nn_pn; int64_t *neigh_pn; double *wf_pn, *wfd_pn;
   int64_t *np_start; int64_t *neigh_np; double *wf_np, *wfd_np; int64_t npairs_cap;
@@ -333,9 +482,13 @@ static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
   int64_t n3[3];
   if (orc_grid_size(c, n3) != KMPM_OK) return NULL;
-  if (c->method != KMPM_METHOD_ULMPM || (c->dimension != 2 && c->dimension != 3)) return NULL;
+  if ((c->method != KMPM_METHOD_ULMPM && c->method != KMPM_METHOD_TLMPM) ||
+      (c->dimension != 2 && c->dimension != 3))
+    return NULL;
   orc_sim *s = (orc_sim *)xcalloc(1, sizeof(orc_sim));
   s->c = *c;
+  s->tl = c->method == KMPM_METHOD_TLMPM;
+  const double *glo = grid_lo(c);
   s->dim = c->dimension;
   s->S1 = c->shape == KMPM_SHAPE_LINEAR ? 2 : 4;
   s->S = s->dim == 3 ? s->S1 * s->S1 * s->S1 : s->S1 * s->S1;
@@ -357,9 +510,9 @@ orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
   for (int i = 0; i < nx; i++)                                  /* grid.cpp:218-262 */
     for (int j = 0; j < ny; j++)
       for (int k = 0; k < nz; k++) {
-        s->gx0[3 * l] = c->boxlo[0] + (0 + i) * s->h;
-        s->gx0[3 * l + 1] = s->dim >= 2 ? c->boxlo[1] + (0 + j) * s->h : 0;
-        s->gx0[3 * l + 2] = s->dim == 3 ? c->boxlo[2] + (0 + k) * s->h : 0;
+        s->gx0[3 * l] = glo[0] + (0 + i) * s->h;
+        s->gx0[3 * l + 1] = s->dim >= 2 ? glo[1] + (0 + j) * s->h : 0;
+        s->gx0[3 * l + 2] = s->dim == 3 ? glo[2] + (0 + k) * s->h : 0;
         if (c->shape == KMPM_SHAPE_LINEAR) {
           s->gntype[3 * l] = s->gntype[3 * l + 1] = s->gntype[3 * l + 2] = 0;
         } else {
@@ -383,7 +536,8 @@ static void free_particles(orc_sim *s) {
   void *ptrs[] = {s->ptag, s->solid, s->mat, s->pmask, s->x0, s->x, s->v, s->vu, s->a,
                   s->F, s->sig, s->eel, s->L, s->D, s->mass, s->vol0, s->vol, s->rho0,
                   s->rho, s->J, s->ep, s->epdot, s->dmg, s->dinit, s->nn_pn, s->neigh_pn,
-                  s->wf_pn, s->wfd_pn, s->neigh_np, s->wf_np, s->wfd_np, s->cell_ijk};
+                  s->wf_pn, s->wfd_pn, s->neigh_np, s->wf_np, s->wfd_np, s->cell_ijk,
+                  s->Fdot, s->Finv, s->R, s->PK1};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
 }
 
@@ -393,6 +547,16 @@ void orc_destroy(orc_sim *s) {
   free(s->gx0); free(s->gntype); free(s->gmass); free(s->gv); free(s->gvu); free(s->gf);
   free(s->gmb); free(s->gmask); free(s->np_start); free(s->mats);
   free(s);
+}
+
+/* TLMPM state derived from F and sigma at upload: F^-1, R, J, vol0 PK1 */
+static void tl_particle_pk1(orc_sim *s, int64_t i) {
+  double *F = &s->F[9 * i];
+  m3_inverse(F, &s->Finv[9 * i]);
+  if (!polar_rotation(F, &s->R[9 * i]))
+    for (int e = 0; e < 9; e++) s->R[9 * i + e] = (e % 4 == 0) ? 1.0 : 0.0;
+  double J = m3_det(F);
+  pk1(s->vol0[i], J, &s->R[9 * i], &s->sig[9 * i], &s->Finv[9 * i], &s->PK1[9 * i]);
 }
 
 int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
@@ -408,10 +572,15 @@ int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
   A(mass, double, 1); A(vol0, double, 1); A(vol, double, 1); A(rho0, double, 1);
   A(rho, double, 1); A(J, double, 1); A(ep, double, 1); A(epdot, double, 1);
   A(dmg, double, 1); A(dinit, double, 1);
+  A(Fdot, double, 9); A(Finv, double, 9); A(R, double, 9); A(PK1, double, 9);
   A(nn_pn, int32_t, 1); A(neigh_pn, int64_t, S); A(wf_pn, double, S); A(wfd_pn, double, 3 * S);
   A(cell_ijk, int32_t, 3);
 #undef A
   s->npairs_cap = np * S;
+  s->wf_done = s->mass_done = 0;
+  if (s->tl)
+    for (int64_t i = 1; i < np; i++)
+      if (p->solid[i] != p->solid[0]) return KMPM_ERR_ARG;   /* one solid, one grid */
   s->neigh_np = (int64_t *)xcalloc(s->npairs_cap, sizeof(int64_t));
   s->wf_np = (double *)xcalloc(s->npairs_cap, sizeof(double));
   s->wfd_np = (double *)xcalloc(3 * s->npairs_cap, sizeof(double));
@@ -442,6 +611,7 @@ int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
     s->epdot[i] = p->eff_plastic_strain_rate[i];
     s->dmg[i] = p->damage[i];
     s->dinit[i] = p->damage_init[i];
+    if (s->tl) tl_particle_pk1(s, i);
   }
   return KMPM_OK;
 }
@@ -509,7 +679,8 @@ int orc_fix_body_force(orc_sim *s, uint32_t bit, const int set[3], const double 
 /* ulmpm.cpp:88-337 (one rank: map_ntag is the identity on [0,nnodes))       */
 /* ------------------------------------------------------------------------ */
 int orc_phase_weights(orc_sim *s) {
-  const double *lo = s->c.boxlo;
+  if (s->tl && s->wf_done) return KMPM_OK;                      /* tlmpm.cpp:89,339 */
+  const double *lo = grid_lo(&s->c);
   double inv_h = s->inv_h;
   int nx = s->n[0], ny = s->n[1], nz = s->n[2];
   int64_t nnodes = s->nn;
@@ -519,7 +690,8 @@ int orc_phase_weights(orc_sim *s) {
   (void)nx;
   for (int64_t in = 0; in <= s->nn; in++) s->np_start[in] = 0;
   for (int64_t ip = 0; ip < s->np; ip++) {
-    const double *xp = &s->x[3 * ip];
+    /* TLMPM: reference positions against the solid's grid (tlmpm.cpp:152-282) */
+    const double *xp = s->tl ? &s->x0[3 * ip] : &s->x[3 * ip];
     int i0, j0, k0;
     if (lin) {                                                  /* :161-165 */
       i0 = (int)((xp[0] - lo[0]) * inv_h);
@@ -598,6 +770,7 @@ int orc_phase_weights(orc_sim *s) {
       for (int d = 0; d < 3; d++) s->wfd_np[3 * o + d] = s->wfd_pn[3 * q + d];
     }
   free(fill);
+  s->wf_done = 1;
   return KMPM_OK;
 }
 
@@ -612,15 +785,17 @@ int orc_phase_p2g(orc_sim *s) {
   int nsolid_max = 0;
   for (int64_t ip = 0; ip < s->np; ip++)
     if (s->solid[ip] + 1 > nsolid_max) nsolid_max = s->solid[ip] + 1;
-  for (int64_t in = 0; in < s->nn; in++) {
+  /* mass: solid 0 resets, every solid accumulates in ascending ip; TLMPM
+   * computes it on the first step only (tlmpm.cpp:344-351) */
+  for (int64_t in = 0; in < s->nn && !(s->tl && s->mass_done); in++) {
     int64_t b = s->np_start[in], e = s->np_start[in + 1];
-    /* mass: solid 0 resets, every solid accumulates in ascending ip */
     s->gmass[in] = 0;
     for (int64_t o = b; o < e; o++) {
       int64_t ip = s->neigh_np[o];
       s->gmass[in] += s->wf_np[o] * s->mass[ip];
     }
   }
+  s->mass_done = 1;
   for (int64_t in = 0; in < s->nn; in++) {
     int64_t b = s->np_start[in], e = s->np_start[in + 1];
     double *gv = &s->gv[3 * in], *gf = &s->gf[3 * in], *gmb = &s->gmb[3 * in];
@@ -644,15 +819,24 @@ int orc_phase_p2g(orc_sim *s) {
       }
       o = o2;
     }
-    /* internal force (:510-515): f -= vol * (sigma * wfd) */
+    /* internal force: UL f -= vol * (sigma * wfd) (:510-515); TL
+     * f -= vol0PK1 * wfd0 (compute_internal_forces_nodes_TL :452-480) */
     for (o = b; o < e; o++) {
       int64_t ip = s->neigh_np[o];
-      const double *sg = &s->sig[9 * ip], *wd = &s->wfd_np[3 * o];
-      for (int d = 0; d < 3; d++) {
-        double t = sg[3 * d] * wd[0] + sg[3 * d + 1] * wd[1] + sg[3 * d + 2] * wd[2];
-        gf[d] -= s->vol[ip] * t;
+      const double *wd = &s->wfd_np[3 * o];
+      if (s->tl) {
+        const double *P = &s->PK1[9 * ip];
+        for (int d = 0; d < 3; d++)
+          gf[d] -= P[3 * d] * wd[0] + P[3 * d + 1] * wd[1] + P[3 * d + 2] * wd[2];
+      } else {
+        const double *sg = &s->sig[9 * ip];
+        for (int d = 0; d < 3; d++) {
+          double t = sg[3 * d] * wd[0] + sg[3 * d + 1] * wd[1] + sg[3 * d + 2] * wd[2];
+          gf[d] -= s->vol[ip] * t;
+        }
       }
-      /* mb += wf * mbp with mbp == 0 (ULMPM::reset, ulmpm.cpp:553-563) */
+      /* mb += wf * mbp with mbp == 0 (ULMPM::reset ulmpm.cpp:553-563,
+       * TLMPM::reset tlmpm.cpp:516-525) */
     }
   }
   /* body force fix (fix_body_force.cpp:117-146) */
@@ -732,10 +916,12 @@ static void phase_velocities_to_grid(orc_sim *s) {
 }
 
 /* Solid::compute_rate_deformation_gradient_UL solid.cpp:860-936 */
+/* TLMPM: Solid::compute_rate_deformation_gradient_TL solid.cpp:798-858, the
+ * same sums over the reference gradients into Fdot */
 static void phase_rate_of_deformation(orc_sim *s, int doublemapping) {
   const double *vn = doublemapping ? s->gv : s->gvu;
   for (int64_t ip = 0; ip < s->np; ip++) {
-    double *L = &s->L[9 * ip];
+    double *L = s->tl ? &s->Fdot[9 * ip] : &s->L[9 * ip];
     for (int e = 0; e < 9; e++) L[e] = 0;
     for (int c = 0; c < s->nn_pn[ip]; c++) {
       int64_t q = ip * s->S + c, in = s->neigh_pn[q];
@@ -771,7 +957,8 @@ static int phase_g2p(orc_sim *s) {
     for (int d = 0; d < 3; d++) a[d] *= inv_dt;
     for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * vu[d];
     const double *x = &s->x[3 * ip];
-    if (!(x[0] >= lo[0] && x[0] <= hi[0] && x[1] >= lo[1] && x[1] <= hi[1] &&
+    if (!s->tl &&        /* the box check is UL only (solid.cpp:617-627) */
+        !(x[0] >= lo[0] && x[0] <= hi[0] && x[1] >= lo[1] && x[1] <= hi[1] &&
           x[2] >= (s->dim == 3 ? lo[2] : 0) && x[2] <= (s->dim == 3 ? hi[2] : 0))) {
       if (s->status == KMPM_OK) { s->status = KMPM_ERR_PARTICLE_LEFT_BOX; s->err_ptag = s->ptag[ip]; }
     }
@@ -897,20 +1084,41 @@ static int phase_F_and_stress(orc_sim *s) {
   double max_c = 0;
   for (int64_t ip = 0; ip < s->np; ip++) {
     double *F = &s->F[9 * ip], *L = &s->L[9 * ip], *D = &s->D[9 * ip];
-    double M[9], Fn[9];
-    for (int e = 0; e < 9; e++) M[e] = ((e % 4 == 0) ? 1.0 : 0.0) + dt * L[e];
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++)
-        Fn[3 * i + j] = M[3 * i] * F[j] + M[3 * i + 1] * F[3 + j] + M[3 * i + 2] * F[6 + j];
-    memcpy(F, Fn, sizeof(Fn));
+    if (s->tl) {                                                /* :1181-1182 */
+      for (int e = 0; e < 9; e++) F[e] += dt * s->Fdot[9 * ip + e];
+    } else {                                                    /* :1183-1184 */
+      double M[9], Fn[9];
+      for (int e = 0; e < 9; e++) M[e] = ((e % 4 == 0) ? 1.0 : 0.0) + dt * L[e];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          Fn[3 * i + j] = M[3 * i] * F[j] + M[3 * i + 1] * F[3 + j] + M[3 * i + 2] * F[6 + j];
+      memcpy(F, Fn, sizeof(Fn));
+    }
+    if (s->tl) m3_inverse(F, &s->Finv[9 * ip]);                 /* :1186 */
     s->J[ip] = det3(F);
     s->vol[ip] = s->J[ip] * s->vol0[ip];
     if (s->J[ip] <= 0.0 && s->dmg[ip] < 1.0) {
       if (s->status == KMPM_OK) { s->status = KMPM_ERR_JACOBIAN; s->err_ptag = s->ptag[ip]; }
     }
     s->rho[ip] = s->rho0[ip] / s->J[ip];
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) D[3 * i + j] = 0.5 * (L[3 * i + j] + L[3 * j + i]);
+    if (s->tl) {                                                /* :1218-1234 */
+      double *R = &s->R[9 * ip];
+      if (!polar_rotation(F, R) && s->status == KMPM_OK) {
+        s->status = KMPM_ERR_JACOBIAN;                          /* PolDec failure aborts */
+        s->err_ptag = s->ptag[ip];
+      }
+      double Lt[9], Sym[9], Rt[9], T1[9], T2[9];
+      m3_mul(&s->Fdot[9 * ip], &s->Finv[9 * ip], L);            /* L = Fdot F^-1 */
+      m3_transpose(L, Lt);
+      for (int e = 0; e < 9; e++) Sym[e] = L[e] + Lt[e];
+      m3_transpose(R, Rt);
+      m3_mul(Rt, Sym, T1);
+      m3_mul(T1, R, T2);
+      for (int e = 0; e < 9; e++) D[e] = 0.5 * T2[e];
+    } else {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) D[3 * i + j] = 0.5 * (L[3 * i + j] + L[3 * j + i]);
+    }
   }
   double tav_h = 1000 * s->h;
   for (int64_t ip = 0; ip < s->np; ip++) {
@@ -967,7 +1175,12 @@ static int phase_F_and_stress(orc_sim *s) {
       for (int e = 0; e < 9; e++) eel[e] = t3 * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e] / Gd;
     }
   }
-  /* CFL (:1376-1437), UL: min_h_ratio = 1 */
+  if (s->tl)                                                    /* :1277-1280,1368-1372 */
+    for (int64_t ip = 0; ip < s->np; ip++)
+      pk1(s->vol0[ip], s->J[ip], &s->R[9 * ip], &s->sig[9 * ip], &s->Finv[9 * ip],
+          &s->PK1[9 * ip]);
+  /* CFL (:1376-1437); UL: min_h_ratio = 1, TL: the smallest |Re eig(F)| */
+  double min_h_ratio = 1.0;
   for (int64_t ip = 0; ip < s->np; ip++) {
     if (s->dmg[ip] >= 1.0) continue;
     const kmpm_material *mt = &s->mats[s->mat[ip]];
@@ -977,8 +1190,13 @@ static int phase_F_and_stress(orc_sim *s) {
     if (isnan(max_c)) {
       if (s->status == KMPM_OK) { s->status = KMPM_ERR_WAVE_SPEED; s->err_ptag = s->ptag[ip]; }
     }
+    if (s->tl) {
+      double re[3];
+      eig_real_parts(&s->F[9 * ip], re);
+      for (int k = 0; k < 3; k++) min_h_ratio = dmin(min_h_ratio, fabs(re[k]));
+    }
   }
-  s->dtCFL = dmin(s->dtCFL, s->h * 1.0 / max_c);
+  s->dtCFL = dmin(s->dtCFL, s->h * min_h_ratio / max_c);
   if (isnan(s->dtCFL) && s->status == KMPM_OK) s->status = KMPM_ERR_DTCFL;
   return s->status;
 }
@@ -1075,13 +1293,14 @@ int orc_cell_index(orc_sim *s, int32_t *ijk, int64_t np) {
   return KMPM_OK;
 }
 
-/* Cell of a particle for the sort: trunc((x - boxlo) * inv_h) per
- * dimension (the linear-shape base index of ulmpm.cpp:163-165), clamped to
- * the node range; key is tile-major. */
+/* Cell of a particle for the sort: trunc((x - lo) * inv_h) per dimension
+ * (the linear-shape base index of ulmpm.cpp:163-165; TLMPM: x0 against the
+ * solid's grid), clamped to the node range; key is tile-major. */
 static int64_t cell_key(orc_sim *s, int64_t ip, const int T[3]) {
   int c[3] = {0, 0, 0}, nt[3];
+  const double *xp = s->tl ? s->x0 : s->x, *glo = grid_lo(&s->c);
   for (int d = 0; d < s->dim; d++) {
-    double u = (s->x[3 * ip + d] - s->c.boxlo[d]) * s->inv_h;
+    double u = (xp[3 * ip + d] - glo[d]) * s->inv_h;
     int ci = (u >= 0 && u < 2147483647.0) ? (int)u : (u >= 0 ? s->n[d] - 1 : 0);
     if (ci > s->n[d] - 1) ci = s->n[d] - 1;
     c[d] = ci;
@@ -1112,3 +1331,6 @@ int orc_sort_permutation(orc_sim *s, const int tile[3], int64_t *sorted, int64_t
 }
 
 int64_t orc_error_particle(orc_sim *s) { return s->err_ptag; }
+
+int orc_polar_rotation(const double F[9], double R[9]) { return polar_rotation(F, R); }
+void orc_eig_real_parts(const double F[9], double re[3]) { eig_real_parts(F, re); }

@@ -27,6 +27,7 @@ typedef struct orc_sim orc_sim;
 int orc_material_init(kmpm_material *mat);
 int orc_grid_size(const kmpm_config *cfg, int64_t n_out[3]);
 int orc_region_match(const kmpm_region *region, int dimension, double x, double y, double z);
+int orc_region_limits(const kmpm_region *region, double lo[3], double hi[3]);
 int orc_populate(const kmpm_config *cfg, const kmpm_region *region, int ppc1d,
                  const kmpm_material *mat, int mat_index, int solid_index,
                  int64_t ptag_offset, kmpm_particles *out, int64_t capacity,
@@ -73,6 +74,10 @@ double orc_shock_pressure(const kmpm_material *m, double J, double rho, double d
 void orc_jc_damage(const kmpm_material *m, double *damage_init, double *damage,
                    double pH, const double sdev[9], double epsdot, double dps);
 double orc_basis(int shape, int derivative, double r, int ntype, double inv_h);
+/* TLMPM matrix functions: rotation of the polar decomposition of F (row-major
+ * 3x3; returns 0 when det F <= 0) and the real parts of F's eigenvalues */
+int orc_polar_rotation(const double F[9], double R[9]);
+void orc_eig_real_parts(const double F[9], double re[3]);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,9 @@ def load(variant="liboracle.so"):
             "orc_material_init": (C.c_int, [P(N.Material)]),
             "orc_grid_size": (C.c_int, [P(N.Config), N.I64_3]),
             "orc_region_match": (C.c_int, [P(N.Region), C.c_int, C.c_double, C.c_double, C.c_double]),
+            "orc_region_limits": (C.c_int, [P(N.Region), N.D3, N.D3]),
+            "orc_polar_rotation": (C.c_int, [D, D]),
+            "orc_eig_real_parts": (None, [D, D]),
             "orc_populate": (C.c_int, [P(N.Config), P(N.Region), C.c_int, P(N.Material), C.c_int,
                                        C.c_int, C.c_int64, P(N.Particles), C.c_int64, P(C.c_int64)]),
             "orc_create": (vp, [P(N.Config), P(N.Material), C.c_int]),
