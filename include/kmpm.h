@@ -41,9 +41,12 @@ enum {
                                        its 64-bit fixed-point scale (no
                                        counterpart in the reference, which
                                        accumulates in FP64) */
-  KMPM_ERR_INTERNAL = 16            /* a particle was found outside the tile it
+  KMPM_ERR_INTERNAL = 16,           /* a particle was found outside the tile it
                                        was sorted into (an internal
                                        inconsistency; its access is clamped) */
+  KMPM_ERR_INSTANCE = 17            /* batched context: a particle's stencil
+                                       reaches outside its instance's tiles
+                                       (no counterpart in the reference) */
 };
 
 /* ---- options ----------------------------------------------------------- */
@@ -164,6 +167,8 @@ int kmpm_region_limits(const kmpm_region *region, double lo[3], double hi[3]);
 /* ---- device context ------------------------------------------------------ */
 typedef struct kmpm_ctx kmpm_ctx;
 
+/* Materials per context (each instance of a batched context may have its own). */
+enum { KMPM_MAX_MATERIALS = 1024 };
 /* Creates the context on the current HIP device: grid (Grid::init), material
  * table, streams.  Replaces Update::create_method/create_scheme + Domain::set_dimension. */
 int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
@@ -213,6 +218,28 @@ int kmpm_step(kmpm_ctx *ctx, int64_t nsteps);
 /* Waits for the stream and returns the sticky device error (first offending
  * particle tag in *ptag_out, may be NULL). */
 int kmpm_synchronize(kmpm_ctx *ctx, int64_t *ptag_out);
+
+/* ---- batched instances and captured steps ----------------------------------
+ * No counterpart in the reference, which runs one simulation per process.
+ * A batched context packs many independent simulations ("instances") side
+ * by side on one grid: instance i owns the cells [cell_lo, cell_hi) per axis
+ * (3 ints each, the z range [0, 1) in 2D), bounds that are multiples of the
+ * tile shape (kmpm_get_tile_shape); ranges must not overlap, and every
+ * particle's stencil must stay in the cells of its instance
+ * (KMPM_ERR_INSTANCE otherwise).  Each instance
+ * has its own dt (from its own particles' CFL condition, the rule of
+ * ulmpm.cpp:521-544) and time; results equal those of the instance run alone
+ * on the same grid.  ULMPM only.  ninst = 0 returns to one simulation.
+ * Resets every instance's time to 0 and dt to config.dt. */
+int kmpm_set_instances(kmpm_ctx *ctx, int ninst, const int32_t *cell_lo,
+                       const int32_t *cell_hi);
+/* dt and time per instance (arrays of ninst; NULL skipped); waits. */
+int kmpm_instance_time(kmpm_ctx *ctx, int ninst, double *dt, double *time);
+/* nsteps steps as replays of one HIP graph that holds `graph_steps`
+ * captured steps (captured on the first call and whenever graph_steps or
+ * the buffers change), then the remainder eagerly.  Same results as
+ * kmpm_step; phase timing must be off. */
+int kmpm_step_graph(kmpm_ctx *ctx, int64_t nsteps, int64_t graph_steps);
 
 /* ---- phase timing ---------------------------------------------------------- */
 /* Phases whose launches can be timed with HIP events recorded on the

@@ -188,7 +188,7 @@ extern "C" {
 
 int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
                     kmpm_ctx **out) {
-  if (!cfg || !out || nmat < 1 || nmat > 16 || !mats) return KMPM_ERR_ARG;
+  if (!cfg || !out || nmat < 1 || nmat > KMPM_MAX_MATERIALS || !mats) return KMPM_ERR_ARG;
   *out = nullptr;
   if (cfg->method != KMPM_METHOD_ULMPM && cfg->method != KMPM_METHOD_TLMPM) return KMPM_ERR_ARG;
   // TLMPM runs the FP64 path
@@ -218,6 +218,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   c->device = dev;
   c->cu_count = prop.multiProcessorCount;
   c->nmat = nmat;
+  c->mats_host = (kmpm_material *)malloc(nmat * sizeof(kmpm_material));
   memcpy(c->mats_host, mats, nmat * sizeof(kmpm_material));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     free(c);
@@ -331,11 +332,14 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   if (!c) return KMPM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particle_buffers(c);
+  for (auto &g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   void *ptrs[] = {c->d_mats, c->d_cell_count, c->d_cell_start, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
-                  c->d_sc, c->d_scan_tmp, c->d_surr};
+                  c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  free(c->mats_host);
   if (c->ev_pool) {
     for (hipEvent_t e : *c->ev_pool) (void)hipEventDestroy(e);
     delete c->ev_pool;
@@ -667,6 +671,171 @@ int kmpm_step(kmpm_ctx *c, int64_t nsteps) {
 int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
   if (!c) return KMPM_ERR_ARG;
   return status_from_device(c, ptag_out);
+}
+
+// ---- batched instances ------------------------------------------------------
+// Tile ranges of instances given as cell ranges (multiples of the tile
+// shape); false when a range is not tile-aligned or leaves the grid.
+static bool instance_tiles(const TileGeom &g, int dim, const int32_t *clo, const int32_t *chi,
+                           int tlo[3], int thi[3]) {
+  for (int d = 0; d < 3; d++) {
+    int lo = clo[d], hi = chi[d];
+    if (d >= dim) {
+      if (lo != 0 || hi != 1) return false;
+      tlo[d] = 0;
+      thi[d] = 1;
+      continue;
+    }
+    if (lo < 0 || hi <= lo || lo % g.T[d] || hi % g.T[d]) return false;
+    tlo[d] = lo / g.T[d];
+    thi[d] = hi / g.T[d];
+    if (thi[d] > g.nt[d]) return false;
+  }
+  return true;
+}
+
+int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
+  if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
+  if (ninst > 0 && c->a.tl) return KMPM_ERR_ARG;
+  const TileGeom &g = c->a.tg;
+  std::vector<int> map(g.ntiles, -1);
+  std::vector<InstScalars> is(ninst);
+  for (int i = 0; i < ninst; i++) {
+    InstScalars &q = is[i];
+    memset(&q, 0, sizeof(q));
+    if (!instance_tiles(g, c->a.dim, clo + 3 * i, chi + 3 * i, q.tlo, q.thi)) return KMPM_ERR_ARG;
+    for (int x = q.tlo[0]; x < q.thi[0]; x++)
+      for (int y = q.tlo[1]; y < q.thi[1]; y++)
+        for (int z = q.tlo[2]; z < q.thi[2]; z++) {
+          int &m = map[((int64_t)x * g.nt[1] + y) * g.nt[2] + z];
+          if (m >= 0) return KMPM_ERR_ARG;   // overlapping ranges
+          m = i;
+        }
+    q.dt = c->cfg.dt;
+    const double one = 1.0;
+    memcpy(&q.min_ratio_bits, &one, sizeof(one));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->d_inst) (void)hipFree(c->d_inst);
+  if (c->d_tinst) (void)hipFree(c->d_tinst);
+  c->d_inst = nullptr;
+  c->d_tinst = nullptr;
+  c->a.ninst = 0;
+  c->a.inst = nullptr;
+  c->a.tinst = nullptr;
+  if (ninst == 0) return KMPM_OK;
+  HIPCHK(dalloc(&c->d_inst, ninst));
+  HIPCHK(dalloc(&c->d_tinst, g.ntiles));
+  HIPCHK(hipMemcpy(c->d_inst, is.data(), ninst * sizeof(InstScalars), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_tinst, map.data(), g.ntiles * sizeof(int), hipMemcpyHostToDevice));
+  c->a.ninst = ninst;
+  c->a.inst = c->d_inst;
+  c->a.tinst = c->d_tinst;
+  return KMPM_OK;
+}
+
+int kmpm_instance_time(kmpm_ctx *c, int ninst, double *dt, double *time) {
+  if (!c || ninst != c->a.ninst) return KMPM_ERR_ARG;
+  if (ninst == 0) return KMPM_OK;
+  std::vector<InstScalars> is(ninst);
+  HIPCHK(hipMemcpyAsync(is.data(), c->d_inst, ninst * sizeof(InstScalars), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < ninst; i++) {
+    if (dt) dt[i] = is[i].dt;
+    if (time) time[i] = is[i].time;
+  }
+  return KMPM_OK;
+}
+
+// ---- captured steps -----------------------------------------------------------
+static HostStepState save_state(const kmpm_ctx *c) {
+  HostStepState s;
+  memcpy(&s.a, &c->a, sizeof(KArgs));
+  s.keys_valid = c->keys_valid;
+  s.sorted = c->sorted;
+  s.tl_identity = c->tl_identity;
+  return s;
+}
+
+static void load_state(kmpm_ctx *c, const HostStepState &s) {
+  memcpy(&c->a, &s.a, sizeof(KArgs));
+  c->d_pa = s.a.pa;
+  c->d_pb = s.a.pb;
+  c->d_fa = s.a.fa;
+  c->d_fb = s.a.fb;
+  c->d_ia = s.a.ia;
+  c->d_ib = s.a.ib;
+  c->keys_valid = s.keys_valid;
+  c->sorted = s.sorted;
+  c->tl_identity = s.tl_identity;
+}
+
+static bool same_state(const HostStepState &x, const HostStepState &y) {
+  return memcmp(&x.a, &y.a, sizeof(KArgs)) == 0 && x.keys_valid == y.keys_valid &&
+         x.sorted == y.sorted && x.tl_identity == y.tl_identity;
+}
+
+// Captures `steps` steps of kmpm_step from the current host state into a
+// graph (nothing runs); leaves the host state at the end of those steps.
+static int capture_steps(kmpm_ctx *c, int64_t steps, kmpm_ctx::StepGraph &out) {
+  out.start = save_state(c);
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  int st = kmpm_step(c, steps);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (st != KMPM_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    load_state(c, out.start);
+    return st != KMPM_OK ? st : KMPM_ERR_HIP;
+  }
+  out.end = save_state(c);
+  const hipError_t ei = hipGraphInstantiate(&out.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    out.exec = nullptr;
+    load_state(c, out.start);
+    return KMPM_ERR_HIP;
+  }
+  out.steps = steps;
+  load_state(c, out.start);
+  return KMPM_OK;
+}
+
+int kmpm_step_graph(kmpm_ctx *c, int64_t nsteps, int64_t graph_steps) {
+  if (!c || nsteps < 0 || graph_steps < 1 || c->timing) return KMPM_ERR_ARG;
+  int st;
+  // steps whose launches differ from the steady state (first cell keys,
+  // TLMPM's first sort) run eagerly
+  while (nsteps > 0 && (!c->keys_valid || (c->a.tl && !c->tl_identity))) {
+    if ((st = kmpm_step(c, 1))) return st;
+    nsteps--;
+  }
+  while (nsteps >= graph_steps) {
+    const HostStepState now = save_state(c);
+    kmpm_ctx::StepGraph *g = nullptr;
+    for (auto &q : c->graphs)
+      if (q.exec && q.steps == graph_steps && same_state(q.start, now)) g = &q;
+    if (!g) {
+      // replace the entry that does not end where this state starts (an
+      // odd step count alternates between two start states)
+      g = &c->graphs[0];
+      for (auto &q : c->graphs)
+        if (!q.exec) g = &q;
+      if (g->exec && c->graphs[1].exec) {
+        g = same_state(c->graphs[0].end, now) ? &c->graphs[1] : &c->graphs[0];
+      }
+      if (g->exec) {
+        (void)hipGraphExecDestroy(g->exec);
+        g->exec = nullptr;
+      }
+      if ((st = capture_steps(c, graph_steps, *g))) return st;
+    }
+    HIPCHK(hipGraphLaunch(g->exec, c->stream));
+    load_state(c, g->end);
+    nsteps -= graph_steps;
+  }
+  return kmpm_step(c, nsteps);
 }
 
 int kmpm_get_time(kmpm_ctx *c, double *dt, double *time, int64_t *step) {

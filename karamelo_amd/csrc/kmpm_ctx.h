@@ -65,11 +65,22 @@ struct KArgs {
   int nfv, nbf;
   DevFixVel fv[MAX_FIX];
   DevFixBF bf[MAX_FIX];
+  // batched instances (0: one instance, whose dt/time are in sc)
+  int ninst;
+  InstScalars *inst;     // [ninst]
+  const int *tinst;      // tile -> instance, -1 for tiles no instance reaches
   // learned constitutive surrogate (JC + shock class, mixed path)
   const SurrogateDev *surr;
   int surr_on;
   // scalars
   DevScalars *sc;
+};
+
+// Host-side state that selects what a step launches: the kernel arguments
+// (with the current/next buffers) and the phase flags.
+struct HostStepState {
+  KArgs a;
+  bool keys_valid, sorted, tl_identity;
 };
 
 }  // namespace kmpm
@@ -81,7 +92,7 @@ struct kmpm_ctx {
   int cu_count;
   hipStream_t stream;
   int nmat;
-  kmpm_material mats_host[16];
+  kmpm_material *mats_host;   // [nmat]
   // static per-particle data in upload order (index = pid)
   int64_t *h_ptag;
   int32_t *h_solid;
@@ -101,6 +112,15 @@ struct kmpm_ctx {
   uint32_t *d_pmask;     // particle group masks (by pid)
   kmpm::DevScalars *d_sc;
   kmpm::SurrogateDev *d_surr;
+  kmpm::InstScalars *d_inst;
+  int *d_tinst;
+  // captured step graphs: each replays `steps` steps from the host state
+  // it was captured in (buffer ping-pong and flags) to its end state
+  struct StepGraph {
+    hipGraphExec_t exec;
+    int64_t steps;
+    kmpm::HostStepState start, end;
+  } graphs[2];
   void *d_scan_tmp;
   size_t scan_tmp_bytes;
   int64_t ncells;

@@ -93,6 +93,15 @@ struct DevScalars {
   unsigned long long surr_fallbacks;  // particle updates outside the surrogate's envelope
 };
 
+// Per-instance scalars of a batched context (independent instances packed
+// side by side on one grid, each with its own dt, time and CFL).
+struct InstScalars {
+  double dt, time;
+  unsigned long long max_c_bits;
+  unsigned long long min_ratio_bits;
+  int tlo[3], thi[3];   // the instance's tiles: [tlo, thi) per axis
+};
+
 constexpr double SQRT_3_OVER_2 = 1.224744871;                            // strength_jc.cpp:29
 constexpr double FOUR_THIRD = 1.333333333333333333333333333333333333333; // solid.cpp:39
 

@@ -62,6 +62,7 @@ const char *kmpm_status_string(int s) {
     case KMPM_ERR_GRID: return "grid does not support the stencil on one device";
     case KMPM_ERR_FIXED_POINT: return "node sum exceeded the fixed-point range of the mixed path";
     case KMPM_ERR_INTERNAL: return "internal error: particle outside its sorted tile";
+    case KMPM_ERR_INSTANCE: return "a particle reaches outside the tiles of its instance";
     default: return "unknown status";
   }
 }

@@ -88,6 +88,13 @@ __device__ inline int tile_local(int c, int org, int T, DevScalars *sc) {
   return l;
 }
 
+// dt of the instance a tile belongs to (batched contexts) or of the context.
+__device__ inline double tile_dt(const KArgs &a, int t) {
+  if (!a.ninst) return a.sc->dt;
+  const int i = a.tinst[t];
+  return i >= 0 ? a.inst[i].dt : a.sc->dt;
+}
+
 __device__ inline int tile_key(const KArgs &a, int c0, int c1, int c2) {
   const TileGeom &g = a.tg;
   int t = ((c0 / g.T[0]) * g.nt[1] + c1 / g.T[1]) * g.nt[2] + c2 / g.T[2];
@@ -95,11 +102,37 @@ __device__ inline int tile_key(const KArgs &a, int c0, int c1, int c2) {
   return t * g.cpt + l;
 }
 
-__device__ inline int particle_key(const KArgs &a, double x, double y, double z) {
-  int c0 = cell_of_clamped(x, a.lo[0], a.inv_h, a.n[0]);
-  int c1 = cell_of_clamped(y, a.lo[1], a.inv_h, a.n[1]);
-  int c2 = a.dim == 3 ? cell_of_clamped(z, a.lo[2], a.inv_h, a.n[2]) : 0;
-  return tile_key(a, c0, c1, c2);
+// Batched contexts: a particle's stencil nodes must lie inside the tile
+// range of the instance that owns its tile, so every node a particle reaches
+// is advanced with that instance's dt and no instance sees another's mass.
+__device__ inline void check_instance(const KArgs &a, const int c[3], int t, int i) {
+  const int ii = a.tinst[t];
+  bool ok = ii >= 0;
+  if (ok) {
+    const InstScalars &s = a.inst[ii];
+    const int lo_off = a.S1 == 4 ? 1 : 0, hi_off = a.S1 == 4 ? 2 : 1;
+    for (int d = 0; d < a.dim; d++) {
+      const int nlo = s.tlo[d] * a.tg.T[d];
+      int nhi = s.thi[d] * a.tg.T[d];        // exclusive
+      nhi = nhi < a.n[d] ? nhi : a.n[d];
+      // nodes below 0 or at n and beyond carry no weight
+      const int b0 = c[d] - lo_off < 0 ? 0 : c[d] - lo_off;
+      const int b1 = c[d] + hi_off > a.n[d] - 1 ? a.n[d] - 1 : c[d] + hi_off;
+      ok = ok && b0 >= nlo && b1 < nhi;
+    }
+  }
+  if (!ok) set_error(a.sc, KMPM_ERR_INSTANCE, i);
+}
+
+// Sort key of a particle at (x, y, z); `pid` names it in a batched-context error.
+__device__ inline int particle_key(const KArgs &a, double x, double y, double z, int pid) {
+  int c[3];
+  c[0] = cell_of_clamped(x, a.lo[0], a.inv_h, a.n[0]);
+  c[1] = cell_of_clamped(y, a.lo[1], a.inv_h, a.n[1]);
+  c[2] = a.dim == 3 ? cell_of_clamped(z, a.lo[2], a.inv_h, a.n[2]) : 0;
+  const int key = tile_key(a, c[0], c[1], c[2]);
+  if (a.ninst) check_instance(a, c, key / a.tg.cpt, pid);
+  return key;
 }
 
 // ---------------------------------------------------------------------------
@@ -112,7 +145,8 @@ __global__ void k_keys(KArgs a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
   const int f = wpos(a);
-  a.key[i] = particle_key(a, PF(a.pa, f, i), PF(a.pa, f + 1, i), PF(a.pa, f + 2, i));
+  a.key[i] = particle_key(a, PF(a.pa, f, i), PF(a.pa, f + 1, i), PF(a.pa, f + 2, i),
+                          IFLD(a.ia, PID, i));
 }
 
 __global__ void k_sort_begin(KArgs a, int64_t ncells) {
@@ -496,9 +530,9 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
   const int up = S1 == 4 ? 1 : 0;
   const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int total = a.sc->dil_count * NO;
-  const double dt = a.sc->dt;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
     const int t = a.dil_list[q / NO], o = q % NO;
+    const double dt = tile_dt(a, t);
     const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
     const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
@@ -608,7 +642,7 @@ __global__ void k_grid_clean(KArgs a) {
 // ---------------------------------------------------------------------------
 // G2P helpers
 // ---------------------------------------------------------------------------
-__device__ inline void block_max_to(double val, DevScalars *sc) {
+__device__ inline void block_max_to(double val, unsigned long long *dst) {
   // val >= 0 (or NaN handled by caller); wave max then LDS then atomic max
   __shared__ double red[4];
   for (int o = 32; o > 0; o >>= 1) {
@@ -621,9 +655,47 @@ __device__ inline void block_max_to(double val, DevScalars *sc) {
   if (threadIdx.x == 0) {
     double m = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] > m ? red[w] : m;
-    if (m > 0) atomicMax(&sc->max_c_bits, (unsigned long long)__double_as_longlong(m));
+    if (m > 0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
   }
 }
+
+__device__ inline void block_min_ratio(double val, unsigned long long *dst) {
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    const double other = __shfl_xor(val, o, 64);
+    val = other < val ? other : val;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) red[wave] = val;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] < m ? red[w] : m;
+    if (m < 1.0) atomicMin(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// CFL of a batched context: the block's maximum wave speed (and smallest
+// |Re eig(F)|) of tile t go to the tile's instance; the accumulators restart.
+// Block-uniform (it synchronises the block).
+__device__ inline void flush_tile_cfl(const KArgs &a, int t, double &cmax, double &rmin) {
+  __shared__ unsigned long long sink[2];   // for tiles no instance owns
+  const int i = a.tinst[t];
+  block_max_to(cmax, i >= 0 ? &a.inst[i].max_c_bits : &sink[0]);
+  if (a.tl) block_min_ratio(rmin, i >= 0 ? &a.inst[i].min_ratio_bits : &sink[1]);
+  cmax = 0;
+  rmin = 1.0;
+  __syncthreads();   // thread 0 has read the reduction slots before they are reused
+}
+
+// End of a CFL kernel: the block's reductions into the context's scalars
+// (batched contexts have flushed per tile).
+__device__ inline void finish_cfl(const KArgs &a, double cmax, double rmin) {
+  if (a.ninst) return;
+  block_max_to(cmax, &a.sc->max_c_bits);
+  if (a.tl) block_min_ratio(rmin, &a.sc->min_ratio_bits);
+}
+
 
 // Load the tile's node block (velocities) into LDS: comps [nc][BN].
 template <int DIM, int S1, int TX, int TY, int TZ, int MODE, int PREC>
@@ -793,12 +865,12 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
-  const double dt = a.sc->dt;
-  const double inv_dt = 1.0 / dt;
   const double alpha = a.flip;
-  double cmax = 0;
+  double cmax = 0, rmin = 1.0;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
@@ -846,7 +918,7 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
           STB(PX + d, j, x[d]);
           STB(VX + d, j, v[d]);
         }
-        a.key[j] = particle_key(a, x[0], x[1], x[2]);
+        a.key[j] = particle_key(a, x[0], x[1], x[2], pid);
         IFLD(a.ib, PID, j) = pid;
         IFLD(a.ib, MATI, j) = mat;
       }
@@ -894,8 +966,9 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
     } else {
       for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) particle(j, true);
     }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
   }
-  block_max_to(cmax, a.sc);
+  finish_cfl(a, cmax, rmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -905,21 +978,6 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
 // vol0 PK1, the wave speed and the smallest |Re eig(F)| (both -1 for fully
 // damaged particles, which the CFL skips).  Writes slot j of the next state.
 // ---------------------------------------------------------------------------
-__device__ inline void block_min_ratio(double val, DevScalars *sc) {
-  __shared__ double red[4];
-  for (int o = 32; o > 0; o >>= 1) {
-    const double other = __shfl_xor(val, o, 64);
-    val = other < val ? other : val;
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) red[wave] = val;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] < m ? red[w] : m;
-    if (m < 1.0) atomicMin(&sc->min_ratio_bits, (unsigned long long)__double_as_longlong(m));
-  }
-}
 
 template <int MC>
 __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot, int j,
@@ -984,12 +1042,12 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
-  const double dt = a.sc->dt;
-  const double inv_dt = 1.0 / dt;
   const double alpha = a.flip;
   double cmax = 0, rmin = 1.0;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
@@ -1024,15 +1082,15 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
         PF(a.pb, VX + d, j) = v[d];
         PF(a.pb, X0X + d, j) = x0[d];
       }
-      a.key[j] = particle_key(a, x0[0], x0[1], x0[2]);
+      a.key[j] = particle_key(a, x0[0], x0[1], x0[2], pid);
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
       PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
       tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
     }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
   }
-  block_max_to(cmax, a.sc);
-  block_min_ratio(rmin, a.sc);
+  finish_cfl(a, cmax, rmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -1049,11 +1107,11 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
-  const double dt = a.sc->dt;
-  const double inv_dt = 1.0 / dt;
   const double alpha = a.flip;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
@@ -1104,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
       STB(MASS, j, LDA(MASS, slot));
       // next step's cell: of the advanced x (ULMPM) or of x0 (TLMPM)
       const double *xk = TL ? xw : x;
-      a.key[j] = particle_key(a, xk[0], xk[1], xk[2]);
+      a.key[j] = particle_key(a, xk[0], xk[1], xk[2], pid);
     }
   }
 }
@@ -1122,10 +1180,10 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
-  const double dt = a.sc->dt;
-  double cmax = 0;
+  double cmax = 0, rmin = 1.0;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
@@ -1188,8 +1246,9 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
       STB(DINIT, j, dinit);
       STB(DMG, j, dmg);
     }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
   }
-  block_max_to(cmax, a.sc);
+  finish_cfl(a, cmax, rmin);
 }
 
 // TLMPM MUSL part B: Fdot from the re-mapped v with the x0 weights
@@ -1205,10 +1264,10 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
-  const double dt = a.sc->dt;
   double cmax = 0, rmin = 1.0;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
@@ -1238,14 +1297,38 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
       for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
       tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
     }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
   }
-  block_max_to(cmax, a.sc);
-  block_min_ratio(rmin, a.sc);
+  finish_cfl(a, cmax, rmin);
 }
 
 // update_time + adjust_dt (update.cpp:201-206, ulmpm.cpp:525-551)
+// Batched contexts: every instance advances its own time and takes the dt
+// of its own CFL condition (one thread per instance).
+__device__ inline void adjust_dt_instances(const KArgs &a) {
+  for (int i = threadIdx.x; i < a.ninst; i += blockDim.x) {
+    InstScalars &s = a.inst[i];
+    s.time += s.dt;
+    if (!a.dt_constant) {
+      const double maxc = __longlong_as_double((long long)s.max_c_bits);
+      double dtcfl = a.h / maxc;
+      dtcfl = 1.0e22 < dtcfl ? 1.0e22 : dtcfl;
+      if (dtcfl == 0 || isnan(dtcfl)) set_error(a.sc, KMPM_ERR_DTCFL, -1);
+      s.dt = dtcfl * a.dt_factor;
+    }
+    s.max_c_bits = 0;
+    s.min_ratio_bits = (unsigned long long)__double_as_longlong(1.0);
+  }
+  if (threadIdx.x == 0) a.sc->step += 1;
+}
+
 __global__ void k_adjust_dt(KArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (blockIdx.x != 0) return;
+  if (a.ninst) {
+    adjust_dt_instances(a);
+    return;
+  }
+  if (threadIdx.x != 0) return;
   DevScalars *sc = a.sc;
   sc->time += sc->dt;
   if (!a.dt_constant) {
@@ -1584,7 +1667,7 @@ hipError_t launch_stress_musl_b(kmpm_ctx *c) {
 }
 
 hipError_t launch_adjust_dt(kmpm_ctx *c) {
-  hipLaunchKernelGGL(k_adjust_dt, dim3(1), dim3(64), 0, c->stream, c->a);
+  hipLaunchKernelGGL(k_adjust_dt, dim3(1), dim3(256), 0, c->stream, c->a);
   return hipGetLastError();
 }
 

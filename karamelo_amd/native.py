@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libkmpm.so")
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NO_DEVICE, ERR_CAPACITY = 1, 2, 3, 4
 ERR_PARTICLE_LEFT_BOX, ERR_JACOBIAN, ERR_DTCFL, ERR_WAVE_SPEED, ERR_GRID = 10, 11, 12, 13, 14
-ERR_FIXED_POINT, ERR_INTERNAL = 15, 16
+ERR_FIXED_POINT, ERR_INTERNAL, ERR_INSTANCE = 15, 16, 17
 SURROGATE_NPARAM = 1185
 METHOD_ULMPM, METHOD_TLMPM = 0, 1
 SCHEME_USL, SCHEME_MUSL = 0, 1
@@ -154,6 +154,9 @@ def _declare(lib):
         "kmpm_adjust_dt": (C.c_int, [vp]),
         "kmpm_step": (C.c_int, [vp, C.c_int64]),
         "kmpm_synchronize": (C.c_int, [vp, P(C.c_int64)]),
+        "kmpm_set_instances": (C.c_int, [vp, C.c_int, P(C.c_int32), P(C.c_int32)]),
+        "kmpm_instance_time": (C.c_int, [vp, C.c_int, _DP, _DP]),
+        "kmpm_step_graph": (C.c_int, [vp, C.c_int64, C.c_int64]),
         "kmpm_get_time": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_int64)]),
         "kmpm_reduce_energy": (C.c_int, [vp, C.c_uint32, P(C.c_double), P(C.c_double)]),
         "kmpm_download_grid": (C.c_int, [vp, _DP, _DP, _DP, C.c_int64]),
@@ -299,6 +302,28 @@ class Context:
 
     def step(self, n):
         check(self.L.kmpm_step(self.h, int(n)), "step")
+
+    def step_graph(self, n, graph_steps):
+        """n steps as replays of a captured graph of `graph_steps` steps."""
+        check(self.L.kmpm_step_graph(self.h, int(n), int(graph_steps)), "step_graph")
+
+    def set_instances(self, cell_lo, cell_hi):
+        """Batched instances: instance i owns cells [cell_lo[i], cell_hi[i])
+        (ninst x 3, tile-aligned)."""
+        lo = np.ascontiguousarray(cell_lo, dtype=np.int32).reshape(-1, 3)
+        hi = np.ascontiguousarray(cell_hi, dtype=np.int32).reshape(-1, 3)
+        assert lo.shape == hi.shape
+        ip = C.POINTER(C.c_int32)
+        check(self.L.kmpm_set_instances(self.h, len(lo), lo.ctypes.data_as(ip),
+                                        hi.ctypes.data_as(ip)), "set_instances")
+        self.ninst = len(lo)
+
+    def instance_time(self):
+        n = getattr(self, "ninst", 0)
+        dt, t = np.zeros(n), np.zeros(n)
+        check(self.L.kmpm_instance_time(self.h, n, dt.ctypes.data_as(_DP), t.ctypes.data_as(_DP)),
+              "instance_time")
+        return dt, t
 
     def synchronize(self):
         t = C.c_int64(-1)

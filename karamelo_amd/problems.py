@@ -313,3 +313,131 @@ def context_for(problem):
         else:
             ctx.fix_body_force(f.bit, f.set, f.value)
     return ctx
+
+
+# ---------------------------------------------------------------------------
+# batched instances
+# ---------------------------------------------------------------------------
+INSTANCE_ALIGN = 16   # cells: instance ranges are multiples of every 2D tile extent
+
+
+@dataclasses.dataclass
+class Batch:
+    """Independent instances packed side by side on one grid."""
+    problem: Problem          # every instance's particles, materials, walls
+    cell_lo: np.ndarray       # [ninst, 3] cell ranges of the instances
+    cell_hi: np.ndarray
+    params: list              # per instance: particles, v0, material, bar cells
+
+    @property
+    def ninst(self):
+        return len(self.params)
+
+    def instance(self, i):
+        """Instance i alone on the same grid (same materials and walls)."""
+        p = self.problem
+        keep = p.particles["solid"] == i
+        parts = {k: v[keep].copy() for k, v in p.particles.items()}
+        return Problem("%s[%d]" % (p.name, i), p.cfg, p.mats, parts, node_mask=p.node_mask,
+                       fixes=p.fixes, steps=p.steps)
+
+
+def _jc_instance(base, rng, spread):
+    """Copper or steel JC + shock material with A, B, n, C scaled by
+    independent U(1 - spread, 1 + spread) factors; steel keeps JC damage."""
+    m = copper_jc() if base == "copper" else steel_jc(damage=True)
+    f = rng.uniform(1 - spread, 1 + spread, size=4)
+    sp = list(m.strength_p)
+    for k, idx in enumerate((1, 2, 3, 5)):        # A, B, n, C
+        sp[idx] *= f[k]
+    m.strength_p[:] = sp
+    return N.material_init(m), f
+
+
+def taylor_plates(n_inst=256, np_lo=10_000, np_hi=200_000, h=1e-4, ppc1d=2, seed=20261202,
+                  aspect=5.0, v_lo=50.0, v_hi=300.0, spread=0.2, scheme="usl",
+                  shape="cubic-spline", precision="fp64", steps=0, populate_fn=None,
+                  dt_factor=0.5, align=INSTANCE_ALIGN):
+    """Batched 2D plane-strain Taylor impacts (ULMPM): n_inst bars of
+    log-uniform particle counts in [np_lo, np_hi] (length = aspect x width),
+    each of copper or steel (chosen at random) with its Johnson-Cook A, B, n,
+    C scaled by U(0.8, 1.2), moving at U(v_lo, v_hi) along +x into a rigid
+    wall (velocity_nodes vx = 0 on the nodes at and beyond the bar's front
+    face).  Each instance owns a tile-aligned cell range with room for the
+    bar's flattening; the ranges are shelf-packed on one grid."""
+    rng = np.random.default_rng(seed)
+    spec = []
+    for i in range(n_inst):
+        n_target = float(np.exp(rng.uniform(np.log(np_lo), np.log(np_hi))))
+        cells = n_target / (ppc1d * ppc1d)
+        W = max(4, int(round(np.sqrt(cells / aspect))))
+        L = max(4, int(round(cells / W)))
+        base = "copper" if rng.uniform() < 0.5 else "steel"
+        v0 = float(rng.uniform(v_lo, v_hi))
+        mat, f = _jc_instance(base, rng, spread)
+        side = int(np.ceil(0.75 * W)) + 6
+        wx = -(-(6 + L + 8) // align) * align
+        wy = -(-(W + 2 * side) // align) * align
+        spec.append(dict(W=W, L=L, base=base, v0=v0, mat=mat, scale=f, wx=wx, wy=wy))
+    # shelf packing, tallest first, rows of about the square root of the area
+    order = sorted(range(n_inst), key=lambda i: -spec[i]["wy"])
+    area = sum(s["wx"] * s["wy"] for s in spec)
+    row_w = max(max(s["wx"] for s in spec), int(np.ceil(np.sqrt(area) * 1.05 / align)) * align)
+    x = y = row_h = 0
+    NX = 0
+    for i in order:
+        s = spec[i]
+        if x + s["wx"] > row_w:
+            y += row_h
+            x = row_h = 0
+        s["ox"], s["oy"] = x, y
+        x += s["wx"]
+        row_h = max(row_h, s["wy"])
+        NX = max(NX, x)
+    NY = y + row_h
+    cfg = make_config(2, (0.0, 0.0), (NX * h, NY * h), h, scheme=scheme, shape=shape,
+                      flip=0.99, dt_factor=dt_factor)
+    if precision == "mixed":
+        cfg.precision = N.PRECISION_MIXED
+    mats = [s["mat"] for s in spec]
+    nx, ny, _ = N.grid_size(cfg)
+    mask = np.ones(nx * ny, dtype=np.uint32)
+    parts, off = [], 0
+    lo = np.zeros((n_inst, 3), dtype=np.int32)
+    hi = np.zeros((n_inst, 3), dtype=np.int32)
+    hi[:, 2] = 1
+    params = []
+    for i, s in enumerate(spec):
+        ox, oy = s["ox"], s["oy"]
+        lo[i, :2] = (ox, oy)
+        hi[i, :2] = (ox + s["wx"], oy + s["wy"])
+        bx0 = ox + 6
+        by0 = oy + (s["wy"] - s["W"]) // 2
+        bar = block(bx0 * h, (bx0 + s["L"]) * h, by0 * h, (by0 + s["W"]) * h)
+        # populate on the instance's own cells (same lattice as the grid's)
+        sub = make_config(2, (ox * h, oy * h), ((ox + s["wx"]) * h, (oy + s["wy"]) * h), h,
+                          scheme=scheme, shape=shape)
+        arrs = N.populate(sub, bar, ppc1d, s["mat"], i, i, off, fn=populate_fn)
+        arrs["v"][:, 0] = s["v0"]
+        off += len(arrs["ptag"])
+        parts.append(arrs)
+        # wall: nodes of the instance at and beyond the front face
+        front = bx0 + s["L"]
+        ii = np.arange(front, ox + s["wx"])
+        jj = np.arange(oy, oy + s["wy"])
+        ii = ii[ii < nx]
+        jj = jj[jj < ny]
+        mask[(ii[:, None] * ny + jj[None, :]).ravel()] |= 2
+        params.append(dict(np=len(arrs["ptag"]), v0=s["v0"], base=s["base"],
+                           scale=s["scale"].tolist(), bar_cells=(s["L"], s["W"])))
+    p = N.concat_particles(parts)
+    fixes = [Fix("velocity_nodes", 2, (1, 0, 0), (0, 0, 0), (0, 0, 0))]
+    prob = Problem("taylor_plates", cfg, mats, p, node_mask=mask, fixes=fixes, steps=steps,
+                   notes="fix initial_velocity_particles applied at set-up (step-1 hook)")
+    return Batch(prob, lo, hi, params)
+
+
+def context_for_batch(batch):
+    ctx = context_for(batch.problem)
+    ctx.set_instances(batch.cell_lo, batch.cell_hi)
+    return ctx
