@@ -282,6 +282,22 @@ int kmpm_download_cells(kmpm_ctx *ctx, int32_t *cell_ijk, int64_t *sorted_ptags,
 /* Tile shape (cells per tile per axis) of the cell sort key: particles are
  * ordered by tile, then by cell inside the tile (x-major), then by ptag. */
 int kmpm_get_tile_shape(kmpm_ctx *ctx, int tile[3]);
+
+/* ---- tile shapes and autotuning --------------------------------------------
+ * No counterpart in the reference (its loops have no tiles).  The tile
+ * kernels are compiled for KMPM_NTILE_VARIANTS tile shapes per dimension and
+ * shape function (variant 0, the default: 4x4x4 cells in 3D, 8x8 in 2D;
+ * the others lengthen the z (3D) or y (2D) extent, the contiguous axis of
+ * the node arrays).  Changing the variant keeps the particle state; the next
+ * step redoes its cell keys and sort.  Batched instances must stay aligned
+ * to the new tiles (KMPM_ERR_ARG otherwise). */
+enum { KMPM_NTILE_VARIANTS = 3 };
+int kmpm_set_tile_variant(kmpm_ctx *ctx, int variant);
+/* Times `trial_steps` steps (after one warm-up step) of every variant from
+ * the current state, restores that state, and keeps the fastest variant.
+ * ms_per_step[v] is -1 for a variant the instances do not align with. */
+int kmpm_autotune(kmpm_ctx *ctx, int trial_steps, int *variant,
+                  double ms_per_step[KMPM_NTILE_VARIANTS]);
 /* Dense stencil weights of the last weight pass for parity checks:
  * per particle (ptag order) S = 2^d (linear) or 4^d (cubic) entries of
  * node tag (-1 when the pair is dropped), w and grad w [3].  Arrays of

@@ -27,6 +27,10 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 HIP_SOURCES = ["kmpm_api.hip", "kmpm_kernels.hip", "kmpm_surrogate.hip"]
 HOST_SOURCES = ["kmpm_host.cpp"]
+# kmpm_tiles.hip is compiled once per (dimension, shape function, tile
+# variant); the variants build in parallel
+TILE_SOURCE = "kmpm_tiles.hip"
+TILE_VARIANTS = [(d, s, v) for d in (3, 2) for s in (4, 2) for v in range(3)]
 
 
 def _run(cmd):
@@ -42,12 +46,19 @@ def build(verbose=False):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
     os.makedirs(BUILD_DIR, exist_ok=True)
     jobs = []
+    tile_jobs = []
     for s in HIP_SOURCES:
         obj = os.path.join(BUILD_DIR, s + ".o")
         jobs.append(([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", obj], obj))
+    for d, s1, v in TILE_VARIANTS:
+        obj = os.path.join(BUILD_DIR, "kmpm_tiles_%d%d%d.o" % (d, s1, v))
+        defs = ["-DKMPM_TILE_DIM=%d" % d, "-DKMPM_TILE_S1=%d" % s1, "-DKMPM_TILE_V=%d" % v]
+        tile_jobs.append(([HIPCC] + HIP_FLAGS + defs + ["-c", os.path.join(CSRC, TILE_SOURCE), "-o",
+                                                        obj], obj))
     for s in HOST_SOURCES:
         obj = os.path.join(BUILD_DIR, s + ".o")
         jobs.append((["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", obj], obj))
+    jobs = tile_jobs + jobs          # the longest compiles first
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         outs = list(ex.map(lambda j: _run(j[0]), jobs))
     if verbose:

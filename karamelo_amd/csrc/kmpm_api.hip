@@ -15,7 +15,7 @@
 using namespace kmpm;
 
 namespace kmpm {
-void tile_shape(int dim, int S1, int T[3]);
+void tile_shape(int dim, int S1, int variant, int T[3]);
 }
 
 #define HIPCHK(x)                          \
@@ -184,6 +184,110 @@ DevMat kmpm::make_devmat(const kmpm_material &m) {
   return d;
 }
 
+// Tile ranges of instances given as cell ranges (multiples of the tile
+// shape); false when a range is not tile-aligned or leaves the grid.
+static bool instance_tiles(const TileGeom &g, int dim, const int32_t *clo, const int32_t *chi,
+                           int tlo[3], int thi[3]) {
+  for (int d = 0; d < 3; d++) {
+    int lo = clo[d], hi = chi[d];
+    if (d >= dim) {
+      if (lo != 0 || hi != 1) return false;
+      tlo[d] = 0;
+      thi[d] = 1;
+      continue;
+    }
+    if (lo < 0 || hi <= lo || lo % g.T[d] || hi % g.T[d]) return false;
+    tlo[d] = lo / g.T[d];
+    thi[d] = hi / g.T[d];
+    if (thi[d] > g.nt[d]) return false;
+  }
+  return true;
+}
+
+// Tile geometry of a tile variant and every buffer sized by it (cell lists,
+// tile lists, scan workspace, P2G partial blocks, the instance tile map).
+// The state keeps its values; its cell keys and sort are redone on the next
+// step.
+static int setup_tiles(kmpm_ctx *c, int variant) {
+  KArgs &a = c->a;
+  TileGeom g;
+  tile_shape(a.dim, a.S1, variant, g.T);
+  if (variant < 0 || variant >= KMPM_NTILE_VARIANTS || g.T[0] == 0) return KMPM_ERR_ARG;
+  g.ntiles = 1;
+  g.cpt = 1;
+  g.bn = 1;
+  g.off = a.S1 == 4 ? 1 : 0;
+  for (int d = 0; d < 3; d++) {
+    g.nt[d] = (a.n[d] + g.T[d] - 1) / g.T[d];
+    g.E[d] = d < a.dim ? g.T[d] + a.S1 - 1 : 1;
+    g.ntiles *= g.nt[d];
+    g.cpt *= g.T[d];
+    g.bn *= g.E[d];
+  }
+  const int64_t ncells = (int64_t)g.ntiles * g.cpt;
+  if (ncells >= 2147483647LL) return KMPM_ERR_ARG;
+  // instances keep their cell ranges, which must be multiples of the tiles
+  const int ninst = a.ninst;
+  std::vector<int> map;
+  std::vector<InstScalars> is(ninst);
+  if (ninst) {
+    map.assign(g.ntiles, -1);
+    HIPCHK(hipMemcpyAsync(is.data(), c->d_inst, ninst * sizeof(InstScalars),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int32_t *cells = c->inst_cells->data();
+    for (int i = 0; i < ninst; i++) {
+      InstScalars &q = is[i];
+      if (!instance_tiles(g, a.dim, cells + 6 * i, cells + 6 * i + 3, q.tlo, q.thi))
+        return KMPM_ERR_ARG;
+      for (int x = q.tlo[0]; x < q.thi[0]; x++)
+        for (int y = q.tlo[1]; y < q.thi[1]; y++)
+          for (int z = q.tlo[2]; z < q.thi[2]; z++)
+            map[((int64_t)x * g.nt[1] + y) * g.nt[2] + z] = i;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  void *old[] = {c->d_cell_count, c->d_cell_start, c->d_tile_list, c->d_tile_slot,
+                 c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst};
+  for (void *p : old)
+    if (p) (void)hipFree(p);
+  c->d_cell_count = c->d_cell_start = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
+  c->d_scan_tmp = nullptr;
+  c->d_scratch = nullptr;
+  c->d_tinst = nullptr;
+  HIPCHK(dalloc(&c->d_cell_count, ncells + 1));
+  HIPCHK(dalloc(&c->d_cell_start, ncells + 1));
+  HIPCHK(dalloc(&c->d_tile_list, g.ntiles));
+  HIPCHK(dalloc(&c->d_tile_slot, g.ntiles));
+  HIPCHK(dalloc(&c->d_dil_list, g.ntiles));
+  HIPCHK(hipMemset(c->d_tile_slot, 0xff, g.ntiles * sizeof(int)));
+  c->scan_tmp_bytes = scan_temp_bytes(ncells + 1);
+  HIPCHK(hipMalloc(&c->d_scan_tmp, c->scan_tmp_bytes ? c->scan_tmp_bytes : 16));
+  // P2G partial blocks: one per active tile, at most min(tiles, particles)
+  const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(g.ntiles, a.np));
+  HIPCHK(dalloc(&c->d_scratch, (size_t)slots * 7 * g.bn));
+  if (ninst) {
+    HIPCHK(dalloc(&c->d_tinst, g.ntiles));
+    HIPCHK(hipMemcpy(c->d_tinst, map.data(), g.ntiles * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_inst, is.data(), ninst * sizeof(InstScalars), hipMemcpyHostToDevice));
+  }
+  a.tg = g;
+  c->ncells = ncells;
+  c->tile_variant = variant;
+  a.cell_count = c->d_cell_count;
+  a.cell_start = c->d_cell_start;
+  a.tile_list = c->d_tile_list;
+  a.tile_slot = c->d_tile_slot;
+  a.dil_list = c->d_dil_list;
+  a.scratch = c->d_scratch;
+  a.scratch_slots = (int)slots;
+  a.tinst = c->d_tinst;
+  c->keys_valid = false;
+  c->sorted = false;
+  c->tl_identity = false;
+  return KMPM_OK;
+}
+
 extern "C" {
 
 int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
@@ -244,25 +348,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   a.dt_factor = cfg->dt_factor;
   a.dt_constant = cfg->dt_constant;
   a.nmat = nmat;
-  TileGeom &g = a.tg;
-  tile_shape(a.dim, S1, g.T);
-  g.ntiles = 1;
-  g.cpt = 1;
-  g.bn = 1;
-  g.off = S1 == 4 ? 1 : 0;
-  for (int d = 0; d < 3; d++) {
-    g.nt[d] = (a.n[d] + g.T[d] - 1) / g.T[d];
-    g.E[d] = d < a.dim ? g.T[d] + S1 - 1 : 1;
-    g.ntiles *= g.nt[d];
-    g.cpt *= g.T[d];
-    g.bn *= g.E[d];
-  }
-  c->ncells = (int64_t)g.ntiles * g.cpt;
-  if (c->ncells >= 2147483647LL) {
-    (void)hipStreamDestroy(c->stream);
-    free(c);
-    return KMPM_ERR_ARG;
-  }
+  c->inst_cells = new std::vector<int32_t>();
   // material class of the stress kernels
   {
     bool all_lin = true, all_jc = true, all_fluid = true;
@@ -284,23 +370,14 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
                 hipSuccess;
   ok = ok && dalloc(&c->d_gmass, a.nn) == hipSuccess && dalloc(&c->d_gv, 3 * a.nn) == hipSuccess &&
        dalloc(&c->d_gvu, 3 * a.nn) == hipSuccess && dalloc(&c->d_gmask, a.nn) == hipSuccess;
-  ok = ok && dalloc(&c->d_cell_count, c->ncells + 1) == hipSuccess &&
-       dalloc(&c->d_cell_start, c->ncells + 1) == hipSuccess;
-  ok = ok && dalloc(&c->d_tile_list, g.ntiles) == hipSuccess &&
-       dalloc(&c->d_tile_slot, g.ntiles) == hipSuccess &&
-       dalloc(&c->d_dil_list, g.ntiles) == hipSuccess && dalloc(&c->d_sc, 1) == hipSuccess;
-  if (ok) {
-    c->scan_tmp_bytes = scan_temp_bytes(c->ncells + 1);
-    ok = hipMalloc(&c->d_scan_tmp, c->scan_tmp_bytes ? c->scan_tmp_bytes : 16) == hipSuccess;
-  }
+  ok = ok && dalloc(&c->d_sc, 1) == hipSuccess;
   if (ok) {
     std::vector<uint32_t> ones(a.nn, 1u);  // Grid::grow sets mask = 1 (grid.cpp:429)
     ok = hipMemcpy(c->d_gmask, ones.data(), a.nn * sizeof(uint32_t), hipMemcpyHostToDevice) ==
              hipSuccess &&
          hipMemset(c->d_gmass, 0, a.nn * sizeof(double)) == hipSuccess &&
          hipMemset(c->d_gv, 0, 3 * a.nn * sizeof(double)) == hipSuccess &&
-         hipMemset(c->d_gvu, 0, 3 * a.nn * sizeof(double)) == hipSuccess &&
-         hipMemset(c->d_tile_slot, 0xff, g.ntiles * sizeof(int)) == hipSuccess;
+         hipMemset(c->d_gvu, 0, 3 * a.nn * sizeof(double)) == hipSuccess;
     DevScalars sc;
     memset(&sc, 0, sizeof(sc));
     sc.dt = cfg->dt;
@@ -314,15 +391,14 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   a.gv = c->d_gv;
   a.gvu = c->d_gvu;
   a.gmask = c->d_gmask;
-  a.cell_count = c->d_cell_count;
-  a.cell_start = c->d_cell_start;
-  a.tile_list = c->d_tile_list;
-  a.tile_slot = c->d_tile_slot;
-  a.dil_list = c->d_dil_list;
   a.sc = c->d_sc;
   if (!ok) {
     kmpm_ctx_destroy(c);
     return KMPM_ERR_HIP;
+  }
+  if ((st = setup_tiles(c, 0)) != KMPM_OK) {
+    kmpm_ctx_destroy(c);
+    return st;
   }
   *out = c;
   return KMPM_OK;
@@ -340,6 +416,7 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   free(c->mats_host);
+  delete c->inst_cells;
   if (c->ev_pool) {
     for (hipEvent_t e : *c->ev_pool) (void)hipEventDestroy(e);
     delete c->ev_pool;
@@ -387,13 +464,10 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   HIPCHK(dalloc(&c->d_order_tmp, cap));
   HIPCHK(dalloc(&c->d_order, cap));
   HIPCHK(dalloc(&c->d_pmask, cap));
-  // scratch: one partial node block per active tile (at most min(tiles, np))
-  const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(a.tg.ntiles, np));
-  if (c->d_scratch) (void)hipFree(c->d_scratch);
-  c->d_scratch = nullptr;
-  HIPCHK(dalloc(&c->d_scratch, (size_t)slots * 7 * a.tg.bn));
-  a.scratch = c->d_scratch;
-  a.scratch_slots = (int)slots;
+  {
+    const int st = setup_tiles(c, c->tile_variant);   // P2G partial blocks for np particles
+    if (st != KMPM_OK) return st;
+  }
 
   c->h_ptag = (int64_t *)malloc(sizeof(int64_t) * cap);
   c->h_solid = (int32_t *)malloc(sizeof(int32_t) * cap);
@@ -674,26 +748,6 @@ int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
 }
 
 // ---- batched instances ------------------------------------------------------
-// Tile ranges of instances given as cell ranges (multiples of the tile
-// shape); false when a range is not tile-aligned or leaves the grid.
-static bool instance_tiles(const TileGeom &g, int dim, const int32_t *clo, const int32_t *chi,
-                           int tlo[3], int thi[3]) {
-  for (int d = 0; d < 3; d++) {
-    int lo = clo[d], hi = chi[d];
-    if (d >= dim) {
-      if (lo != 0 || hi != 1) return false;
-      tlo[d] = 0;
-      thi[d] = 1;
-      continue;
-    }
-    if (lo < 0 || hi <= lo || lo % g.T[d] || hi % g.T[d]) return false;
-    tlo[d] = lo / g.T[d];
-    thi[d] = hi / g.T[d];
-    if (thi[d] > g.nt[d]) return false;
-  }
-  return true;
-}
-
 int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
   if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
   if (ninst > 0 && c->a.tl) return KMPM_ERR_ARG;
@@ -731,6 +785,12 @@ int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t
   c->a.ninst = ninst;
   c->a.inst = c->d_inst;
   c->a.tinst = c->d_tinst;
+  c->inst_cells->resize(6 * (size_t)ninst);
+  for (int i = 0; i < ninst; i++)
+    for (int d = 0; d < 3; d++) {
+      (*c->inst_cells)[6 * i + d] = clo[3 * i + d];
+      (*c->inst_cells)[6 * i + 3 + d] = chi[3 * i + d];
+    }
   return KMPM_OK;
 }
 
@@ -746,6 +806,119 @@ int kmpm_instance_time(kmpm_ctx *c, int ninst, double *dt, double *time) {
     if (time) time[i] = is[i].time;
   }
   return KMPM_OK;
+}
+
+// ---- tile variants and autotuning ---------------------------------------------
+int kmpm_set_tile_variant(kmpm_ctx *c, int variant) {
+  if (!c) return KMPM_ERR_ARG;
+  if (variant == c->tile_variant) return KMPM_OK;
+  return setup_tiles(c, variant);
+}
+
+// Device copy of the particle state and scalars that a step changes.
+struct SavedState {
+  double *pa = nullptr;
+  float *fa = nullptr;
+  int *ia = nullptr;
+  InstScalars *inst = nullptr;
+  DevScalars sc;
+  size_t nd = 0, nf = 0, ni = 0;
+};
+
+static int save_particles(kmpm_ctx *c, SavedState &s) {
+  const KArgs &a = c->a;
+  const bool mixed = a.prec == KMPM_PRECISION_MIXED;
+  s.nd = (size_t)(mixed ? MX_ND : (a.tl ? NDF_TL : NDF)) * a.cap;
+  s.nf = mixed ? (size_t)MX_NF * a.cap : 0;
+  s.ni = (size_t)NIF * a.cap;
+  HIPCHK(dalloc(&s.pa, s.nd));
+  if (s.nf) HIPCHK(dalloc(&s.fa, s.nf));
+  HIPCHK(dalloc(&s.ia, s.ni));
+  if (a.ninst) HIPCHK(dalloc(&s.inst, a.ninst));
+  HIPCHK(hipMemcpyAsync(s.pa, c->d_pa, s.nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (s.nf)
+    HIPCHK(hipMemcpyAsync(s.fa, c->d_fa, s.nf * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(s.ia, c->d_ia, s.ni * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+  if (a.ninst)
+    HIPCHK(hipMemcpyAsync(s.inst, c->d_inst, a.ninst * sizeof(InstScalars),
+                          hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(&s.sc, c->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return KMPM_OK;
+}
+
+// Back to the saved state (in the buffers that are current now); its cell
+// keys and sort are redone on the next step.
+static int restore_particles(kmpm_ctx *c, const SavedState &s) {
+  HIPCHK(hipMemcpyAsync(c->d_pa, s.pa, s.nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (s.nf)
+    HIPCHK(hipMemcpyAsync(c->d_fa, s.fa, s.nf * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_ia, s.ia, s.ni * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+  if (c->a.ninst)
+    HIPCHK(hipMemcpyAsync(c->d_inst, s.inst, c->a.ninst * sizeof(InstScalars),
+                          hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_sc, &s.sc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->keys_valid = false;
+  c->sorted = false;
+  c->tl_identity = false;
+  return KMPM_OK;
+}
+
+static void free_saved(SavedState &s) {
+  void *p[] = {s.pa, s.fa, s.ia, s.inst};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+}
+
+int kmpm_autotune(kmpm_ctx *c, int trial_steps, int *variant, double ms[KMPM_NTILE_VARIANTS]) {
+  if (!c || trial_steps < 1 || c->timing || c->a.np == 0) return KMPM_ERR_ARG;
+  int st = status_from_device(c, nullptr);
+  if (st != KMPM_OK) return st;
+  SavedState saved;
+  if ((st = save_particles(c, saved)) != KMPM_OK) {
+    free_saved(saved);
+    return st;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int best = c->tile_variant;
+  double best_ms = 1e300;
+  const int start_variant = c->tile_variant;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = KMPM_ERR_HIP;
+  for (int v = 0; v < KMPM_NTILE_VARIANTS && st == KMPM_OK; v++) {
+    double t = -1;
+    // the saved instance scalars come back first: setup_tiles recomputes
+    // their tile ranges for this variant
+    if ((st = restore_particles(c, saved)) != KMPM_OK) break;
+    const int sv = setup_tiles(c, v);
+    if (sv == KMPM_ERR_ARG) {         // instances not aligned with these tiles
+      if (ms) ms[v] = -1;
+      continue;
+    }
+    if ((st = sv) != KMPM_OK) break;
+    if ((st = kmpm_step(c, 1)) != KMPM_OK) break;   // first step: cell keys of this variant
+    if (hipEventRecord(e0, c->stream) != hipSuccess) st = KMPM_ERR_HIP;
+    if (st == KMPM_OK) st = kmpm_step(c, trial_steps);
+    if (st == KMPM_OK && hipEventRecord(e1, c->stream) != hipSuccess) st = KMPM_ERR_HIP;
+    if (st == KMPM_OK) st = status_from_device(c, nullptr);
+    float f = 0;
+    if (st == KMPM_OK && hipEventElapsedTime(&f, e0, e1) != hipSuccess) st = KMPM_ERR_HIP;
+    if (st != KMPM_OK) break;
+    t = (double)f / trial_steps;
+    if (ms) ms[v] = t;
+    if (t < best_ms) {
+      best_ms = t;
+      best = v;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  const int keep = st == KMPM_OK ? best : start_variant;
+  int st2 = restore_particles(c, saved);
+  if (st2 == KMPM_OK) st2 = setup_tiles(c, keep);
+  free_saved(saved);
+  if (variant) *variant = c->tile_variant;
+  return st != KMPM_OK ? st : st2;
 }
 
 // ---- captured steps -----------------------------------------------------------

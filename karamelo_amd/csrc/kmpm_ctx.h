@@ -114,6 +114,8 @@ struct kmpm_ctx {
   kmpm::SurrogateDev *d_surr;
   kmpm::InstScalars *d_inst;
   int *d_tinst;
+  std::vector<int32_t> *inst_cells;   // per instance: cell_lo[3], cell_hi[3]
+  int tile_variant;                   // Tiles<dim, S1, variant> of the tile kernels
   // captured step graphs: each replays `steps` steps from the host state
   // it was captured in (buffer ping-pong and flags) to its end state
   struct StepGraph {

@@ -1,0 +1,1267 @@
+// kmpm_kernels.h -- device functions and tile-templated kernels of the
+// explicit MPM step (the P2G, grid update and G2P kernels are compiled
+// once per tile shape; see kmpm_tile_launch.h).
+//
+// Step structure (USL):
+//   sort   : cell key per particle -> counting sort by tile-major cell key,
+//            ties broken by particle id (deterministic permutation)
+//   P2G    : one workgroup per active tile; lanes = stencil nodes; each lane
+//            accumulates the particles of a cell in registers, flushes into an
+//            LDS node block of the tile; the block is written to a per-tile
+//            partial buffer (no global atomics)
+//   grid   : each owned node of a touched tile sums the partial blocks that
+//            cover it, then v = p/m, v_update = v + dt (f + mb)/m, fixes
+//   G2P    : one workgroup per active tile; LDS block of node velocities;
+//            lanes = particles: gather, L, x/v update, F, stress, CFL, next
+//            cell key; state is written in sorted order (the permutation is
+//            applied for free)
+#pragma once
+
+#include <type_traits>
+
+#include "kmpm_constitutive.h"
+#include "kmpm_ctx.h"
+#include "kmpm_mat3.h"
+#include "kmpm_surrogate.h"
+
+namespace kmpm {
+
+#define PF(base, f, i) ((base)[(size_t)(f) * (size_t)a.cap + (size_t)(i)])
+#define IFLD(base, f, i) ((base)[(size_t)(f) * (size_t)a.cap + (size_t)(i)])
+
+// Particle field access for both state layouts.  Values travel as double;
+// the mixed layout widens its FP32 fields on load and rounds them on store.
+template <int PREC>
+__device__ inline double ldp(const KArgs &a, const double *d, const float *f, int fld, int i) {
+  if (PREC == KMPM_PRECISION_FP64) return d[(size_t)fld * a.cap + i];
+  if (mx_is_double(fld)) return d[(size_t)mx_dslot(fld) * a.cap + i];
+  return (double)f[(size_t)mx_fslot(fld) * a.cap + i];
+}
+template <int PREC>
+__device__ inline void stp(const KArgs &a, double *d, float *f, int fld, int i, double v) {
+  if (PREC == KMPM_PRECISION_FP64) d[(size_t)fld * a.cap + i] = v;
+  else if (mx_is_double(fld)) d[(size_t)mx_dslot(fld) * a.cap + i] = v;
+  else f[(size_t)mx_fslot(fld) * a.cap + i] = (float)v;
+}
+#define LDA(f, i) ldp<PREC>(a, a.pa, a.fa, (f), (i))
+#define LDB(f, i) ldp<PREC>(a, a.pb, a.fb, (f), (i))
+#define STB(f, i, v) stp<PREC>(a, a.pb, a.fb, (f), (i), (v))
+
+// Grid arrays hold R = Real<PREC>::T values in the double allocations.
+template <int PREC> __device__ inline typename Real<PREC>::T *gptr(double *p) {
+  return reinterpret_cast<typename Real<PREC>::T *>(p);
+}
+
+__device__ inline void set_error(DevScalars *sc, int code, int64_t pid) {
+  if (atomicCAS(&sc->error, 0, code) == 0) sc->err_pid = pid;
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Scaled flush of a P2G register sum into a 64-bit fixed-point LDS node sum.
+__device__ inline void fx_add(unsigned long long *dst, double val, double scale, DevScalars *sc) {
+  const double q = val * scale;
+  if (!(fabs(q) < FX_FLUSH_LIMIT)) set_error(sc, KMPM_ERR_FIXED_POINT, -1);
+  const long long iv = fabs(q) < FX_FLUSH_LIMIT ? __double2ll_rn(q) : 0;
+  atomicAdd(dst, (unsigned long long)iv);
+}
+
+__device__ inline int cell_of_clamped(double x, double lo, double inv_h, int n) {
+  double u = (x - lo) * inv_h;
+  if (!(u >= 0.0)) return 0;   // negative or NaN
+  if (u >= (double)(n - 1)) return n - 1;
+  return (int)u;
+}
+
+// Cell of a particle relative to the origin of the tile it was sorted into.
+// The sort guarantees 0 <= result < T; a value outside (keys out of date)
+// would address outside the tile's LDS node block, so it is clamped and
+// reported instead.
+__device__ inline int tile_local(int c, int org, int T, DevScalars *sc) {
+  const int l = c - org;
+  if (l < 0 || l >= T) {
+    set_error(sc, KMPM_ERR_INTERNAL, -1);
+    return l < 0 ? 0 : T - 1;
+  }
+  return l;
+}
+
+// dt of the instance a tile belongs to (batched contexts) or of the context.
+__device__ inline double tile_dt(const KArgs &a, int t) {
+  if (!a.ninst) return a.sc->dt;
+  const int i = a.tinst[t];
+  return i >= 0 ? a.inst[i].dt : a.sc->dt;
+}
+
+__device__ inline int tile_key(const KArgs &a, int c0, int c1, int c2) {
+  const TileGeom &g = a.tg;
+  int t = ((c0 / g.T[0]) * g.nt[1] + c1 / g.T[1]) * g.nt[2] + c2 / g.T[2];
+  int l = ((c0 % g.T[0]) * g.T[1] + c1 % g.T[1]) * g.T[2] + c2 % g.T[2];
+  return t * g.cpt + l;
+}
+
+// Batched contexts: a particle's stencil nodes must lie inside the tile
+// range of the instance that owns its tile, so every node a particle reaches
+// is advanced with that instance's dt and no instance sees another's mass.
+__device__ inline void check_instance(const KArgs &a, const int c[3], int t, int i) {
+  const int ii = a.tinst[t];
+  bool ok = ii >= 0;
+  if (ok) {
+    const InstScalars &s = a.inst[ii];
+    const int lo_off = a.S1 == 4 ? 1 : 0, hi_off = a.S1 == 4 ? 2 : 1;
+    for (int d = 0; d < a.dim; d++) {
+      const int nlo = s.tlo[d] * a.tg.T[d];
+      int nhi = s.thi[d] * a.tg.T[d];        // exclusive
+      nhi = nhi < a.n[d] ? nhi : a.n[d];
+      // nodes below 0 or at n and beyond carry no weight
+      const int b0 = c[d] - lo_off < 0 ? 0 : c[d] - lo_off;
+      const int b1 = c[d] + hi_off > a.n[d] - 1 ? a.n[d] - 1 : c[d] + hi_off;
+      ok = ok && b0 >= nlo && b1 < nhi;
+    }
+  }
+  if (!ok) set_error(a.sc, KMPM_ERR_INSTANCE, i);
+}
+
+// Sort key of a particle at (x, y, z); `pid` names it in a batched-context error.
+__device__ inline int particle_key(const KArgs &a, double x, double y, double z, int pid) {
+  int c[3];
+  c[0] = cell_of_clamped(x, a.lo[0], a.inv_h, a.n[0]);
+  c[1] = cell_of_clamped(y, a.lo[1], a.inv_h, a.n[1]);
+  c[2] = a.dim == 3 ? cell_of_clamped(z, a.lo[2], a.inv_h, a.n[2]) : 0;
+  const int key = tile_key(a, c[0], c[1], c[2]);
+  if (a.ninst) check_instance(a, c, key / a.tg.cpt, pid);
+  return key;
+}
+
+// Weight positions: current x (ULMPM) or reference x0 (TLMPM).
+__device__ inline int wpos(const KArgs &a) { return a.tl ? X0X : PX; }
+
+// ---------------------------------------------------------------------------
+// P2G: lanes = stencil nodes, register accumulation per cell, LDS tile block
+// ---------------------------------------------------------------------------
+constexpr int SCR_FIELDS = 7;   // scratch slot layout: mass, momentum[3], force[3]
+
+// Raw particle data a P2G lane loads for one particle (R: the transfer type).
+// Stress terms a P2G particle carries: -vol sigma (symmetric: 6 in 3D,
+// xx,yy,xy in 2D) or, TLMPM, -vol0 PK1 (row-major: 9 in 3D, 4 in 2D).
+template <int DIM, bool TL> constexpr int p2g_nsig() {
+  return TL ? (DIM == 3 ? 9 : 4) : (DIM == 3 ? 6 : 3);
+}
+
+template <int DIM, bool REMAP, bool TL, class R>
+struct P2GIn {
+  double x[DIM];
+  R v[DIM], m;
+  R s[REMAP ? 1 : p2g_nsig<DIM, TL>()];
+};
+
+template <int DIM, bool REMAP, bool TL, int PREC, class R>
+__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R> &in) {
+  if (REMAP) {
+    for (int d = 0; d < DIM; d++) {
+      in.x[d] = TL ? PF(a.pb, X0X + d, j) : a.xo[(size_t)d * a.cap + j];
+      in.v[d] = (R)LDB(VX + d, j);
+    }
+    in.m = (R)LDB(MASS, j);
+  } else if (TL) {
+    const int slot = a.order[j];
+    for (int d = 0; d < DIM; d++) {
+      in.x[d] = PF(a.pa, X0X + d, slot);
+      in.v[d] = (R)LDA(VX + d, slot);
+    }
+    in.m = (R)LDA(MASS, slot);
+    if (DIM == 3) {
+      for (int e = 0; e < 9; e++) in.s[e] = (R)-PF(a.pa, P00 + e, slot);
+    } else {
+      in.s[0] = (R)-PF(a.pa, P00, slot);
+      in.s[1] = (R)-PF(a.pa, P01, slot);
+      in.s[2] = (R)-PF(a.pa, P10, slot);
+      in.s[3] = (R)-PF(a.pa, P11, slot);
+    }
+  } else {
+    const int slot = a.order[j];
+    for (int d = 0; d < DIM; d++) {
+      in.x[d] = LDA(PX + d, slot);
+      in.v[d] = (R)LDA(VX + d, slot);
+    }
+    in.m = (R)LDA(MASS, slot);
+    const R nvol = (R)-LDA(VOL, slot);
+    if (DIM == 3) {
+      for (int e = 0; e < 6; e++) in.s[e] = nvol * (R)LDA(SXX + e, slot);
+    } else {
+      in.s[0] = nvol * (R)LDA(SXX, slot);
+      in.s[1] = nvol * (R)LDA(SYY, slot);
+      in.s[2] = nvol * (R)LDA(SXY, slot);
+    }
+  }
+}
+
+template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP, int PREC, bool TL>
+__global__ __launch_bounds__(256) void k_p2g(KArgs a) {
+  // FP64: double transfer arithmetic, double LDS node sums.  Mixed: float
+  // transfer arithmetic, register sums flushed into 64-bit fixed-point LDS
+  // node sums (order-independent, so the scatter is deterministic).
+  using R = typename Real<PREC>::T;
+  using R2 = typename Real<PREC>::T2;
+  constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
+  using BT = typename std::conditional<FX, unsigned long long, double>::type;
+  constexpr int S = DIM == 3 ? S1 * S1 * S1 : S1 * S1;
+  constexpr int G = 64 / S;
+  constexpr int RUN = CH / G;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  constexpr int NSIG = p2g_nsig<DIM, TL>();
+  constexpr int NW = DIM * S1;                              // (w, dw) pairs per particle
+  constexpr int NF = REMAP ? (1 + DIM) : (1 + 2 * DIM);
+  constexpr int NP2 = 2 + (REMAP ? 0 : (NSIG + 1) / 2);    // payload pairs: (m,v0),(v1,v2),sigma
+  static_assert(64 % S == 0 && CH % G == 0 && CH <= 64, "stencil/chunk shape");
+  // LDS node block strides.  For the 3D cubic 4x4x4 tile the rows are padded
+  // (z stride 8, x stride 58 doubles) so the 64 lanes' atomic adds of one
+  // cell hit every bank at most twice, the minimum for 64 doubles.
+  constexpr bool PAD = DIM == 3 && S1 == 4 && TX == 4 && TY == 4 && TZ == 4;
+  constexpr int PSB = PAD ? 8 : EZ;
+  constexpr int PSA = PAD ? 58 : EY * EZ;
+  constexpr int PBN = EX * PSA;
+  static_assert(PSA >= EY * PSB, "block stride");
+
+  __shared__ BT blk[NF * PBN];
+  __shared__ R2 stw[4][NW][CH + 1];   // +1: the S1 rows a lane group reads fall in distinct banks
+  __shared__ R2 spay[4][NP2][CH];
+  __shared__ int stg_cell[4][CH];
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+
+  // lane -> stencil node offsets
+  const int g = lane / S, s = lane % S;
+  const int sa = DIM == 3 ? s / (S1 * S1) : s / S1;
+  const int sb = DIM == 3 ? (s / S1) % S1 : s % S1;
+  const int sc = DIM == 3 ? s % S1 : 0;
+  const int sofs = sa * PSA + sb * PSB + sc;
+
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const int tz = DIM == 3 ? t % nt2 : 0;
+    const int ty = (t / nt2) % nt1;
+    const int tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    for (int q = tid; q < NF * PBN; q += 256) blk[q] = 0;
+    __syncthreads();
+
+    const int npart = p1 - p0;
+    // The particle data of the wave's next chunk is loaded into registers
+    // while the current chunk is accumulated (hides HBM latency).
+    P2GIn<DIM, REMAP, TL, R> in;
+    int c0 = wave * CH;
+    if (c0 < npart && lane < min(CH, npart - c0))
+      p2g_load<DIM, REMAP, TL, PREC>(a, p0 + c0 + lane, in);
+    for (; c0 < npart; c0 += 4 * CH) {
+      const int cnt = min(CH, npart - c0);
+      if (lane < cnt) {
+        int cl[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+          const int c = cell_of_clamped(in.x[d], a.lo[d], a.inv_h, a.n[d]);
+          cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+          double w[S1], dw[S1];
+          weights_1d<S1>(in.x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
+#pragma unroll
+          for (int q = 0; q < S1; q++) stw[wave][d * S1 + q][lane] = R2{(R)w[q], (R)dw[q]};
+        }
+        stg_cell[wave][lane] = cl[0] * PSA + cl[1] * PSB + cl[2];
+        spay[wave][0][lane] = R2{in.m, in.v[0]};
+        spay[wave][1][lane] = R2{in.v[1], DIM == 3 ? in.v[DIM - 1] : (R)0};
+        if (!REMAP) {
+#pragma unroll
+          for (int e = 0; e < NSIG; e += 2)
+            spay[wave][2 + e / 2][lane] = R2{in.s[e], e + 1 < NSIG ? in.s[e + 1] : (R)0};
+        }
+      }
+      wave_sync();
+      const int cn = c0 + 4 * CH;
+      if (cn < npart && lane < min(CH, npart - cn))
+        p2g_load<DIM, REMAP, TL, PREC>(a, p0 + cn + lane, in);
+
+      // Accumulate the group's run of particles; the LDS operands of the
+      // next particle are loaded before the arithmetic of the current one
+      // (two operand sets used alternately, no register copies).
+      const int jb = g * RUN, je = min(jb + RUN, cnt);
+      if (jb < je) {
+        constexpr int NS2 = REMAP ? 0 : (NSIG + 1) / 2;
+        struct Ops {
+          int cell;
+          R2 px, py, pz, mv0, v12;
+          R2 sg[NS2 > 0 ? NS2 : 1];
+        };
+        auto load = [&](int j, Ops &o) {
+          o.cell = stg_cell[wave][j];
+          o.px = stw[wave][sa][j];
+          o.py = stw[wave][S1 + sb][j];
+          o.pz = DIM == 3 ? stw[wave][2 * S1 + sc][j] : R2{(R)1, (R)0};
+          o.mv0 = spay[wave][0][j];
+          o.v12 = spay[wave][1][j];
+          for (int e = 0; e < NS2; e++) o.sg[e] = spay[wave][2 + e][j];
+        };
+        R acc[NF];
+        for (int f = 0; f < NF; f++) acc[f] = 0;
+        auto compute = [&](const Ops &o) {
+          R w, g0, g1, g2 = 0;
+          if (DIM == 3) {
+            const R wxy = o.px.x * o.py.x;
+            w = wxy * o.pz.x;
+            g0 = o.px.y * o.py.x * o.pz.x;
+            g1 = o.px.x * o.py.y * o.pz.x;
+            g2 = wxy * o.pz.y;
+          } else {
+            w = o.px.x * o.py.x;
+            g0 = o.px.y * o.py.x;
+            g1 = o.px.x * o.py.y;
+          }
+          const R wm = w * o.mv0.x;
+          acc[0] += wm;
+          acc[1] = fma(wm, o.mv0.y, acc[1]);
+          acc[2] = fma(wm, o.v12.x, acc[2]);
+          if (DIM == 3) acc[3] = fma(wm, o.v12.y, acc[3]);
+          if (!REMAP && TL) {
+            // f_d += sum_e (-vol0 PK1)_de g_e, row-major pairs
+            constexpr int K = NS2 > 0 ? NS2 : 1;
+            if (DIM == 3) {
+              const R q00 = o.sg[0].x, q01 = o.sg[0].y, q02 = o.sg[1 % K].x, q10 = o.sg[1 % K].y,
+                      q11 = o.sg[2 % K].x, q12 = o.sg[2 % K].y, q20 = o.sg[3 % K].x,
+                      q21 = o.sg[3 % K].y, q22 = o.sg[4 % K].x;
+              acc[4] = fma(q02, g2, fma(q01, g1, fma(q00, g0, acc[4])));
+              acc[5] = fma(q12, g2, fma(q11, g1, fma(q10, g0, acc[5])));
+              acc[6] = fma(q22, g2, fma(q21, g1, fma(q20, g0, acc[6])));
+            } else {
+              const R q00 = o.sg[0].x, q01 = o.sg[0].y, q10 = o.sg[1 % K].x, q11 = o.sg[1 % K].y;
+              acc[3] = fma(q01, g1, fma(q00, g0, acc[3]));
+              acc[4] = fma(q11, g1, fma(q10, g0, acc[4]));
+            }
+          } else if (!REMAP) {
+            constexpr int K = NS2 > 0 ? NS2 : 1;
+            if (DIM == 3) {
+              // f_d += sum_e (-vol sigma)_de g_e ; sym order xx,yy,zz,xy,xz,yz
+              const R sxx = o.sg[0].x, syy = o.sg[0].y, szz = o.sg[1 % K].x,
+                      sxy = o.sg[1 % K].y, sxz = o.sg[2 % K].x, syz = o.sg[2 % K].y;
+              acc[4] = fma(sxz, g2, fma(sxy, g1, fma(sxx, g0, acc[4])));
+              acc[5] = fma(syz, g2, fma(syy, g1, fma(sxy, g0, acc[5])));
+              acc[6] = fma(szz, g2, fma(syz, g1, fma(sxz, g0, acc[6])));
+            } else {
+              const R sxx = o.sg[0].x, syy = o.sg[0].y, sxy = o.sg[1 % K].x;
+              acc[3] = fma(sxy, g1, fma(sxx, g0, acc[3]));
+              acc[4] = fma(syy, g1, fma(sxy, g0, acc[4]));
+            }
+          }
+          (void)g2;
+        };
+        auto flush = [&](int cell) {
+          for (int f = 0; f < NF; f++) {
+            if constexpr (FX) {
+              const int k = f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE;
+              fx_add(&blk[f * PBN + cell + sofs], (double)acc[f], a.fxs[k], a.sc);
+            } else {
+              atomicAdd(&blk[f * PBN + cell + sofs], acc[f]);
+            }
+            acc[f] = 0;
+          }
+        };
+        Ops A, B;
+        load(jb, A);
+        int cur = A.cell;
+        for (int j = jb;; j += 2) {
+          load(j + 1 < je ? j + 1 : j, B);
+          compute(A);
+          if (j + 1 >= je) break;
+          if (B.cell != cur) {
+            flush(cur);
+            cur = B.cell;
+          }
+          load(j + 2 < je ? j + 2 : j + 1, A);
+          compute(B);
+          if (j + 2 >= je) break;
+          if (A.cell != cur) {
+            flush(cur);
+            cur = A.cell;
+          }
+        }
+        flush(cur);
+      }
+      wave_sync();
+    }
+    __syncthreads();
+    BT *dst = reinterpret_cast<BT *>(a.scratch) + (size_t)li * SCR_FIELDS * BN;
+    // blk fields: mass, mom[DIM], force[DIM] -> scratch fields mass, mom[3],
+    // force[3]; components a 2D run does not have are written as zeros.
+    const int f_lo = REMAP ? 1 : 0, f_hi = REMAP ? 4 : SCR_FIELDS;
+    for (int q = tid + f_lo * BN; q < f_hi * BN; q += 256) {
+      const int sf = q / BN, o = q % BN;
+      int bf;
+      if (sf == 0) bf = 0;
+      else if (sf <= 3) bf = sf <= DIM ? sf : -1;
+      else bf = (sf - 3) <= DIM ? DIM + (sf - 3) : -1;
+      const int bz = o % EZ, by = (o / EZ) % EY, bx = o / (EZ * EY);
+      dst[q] = bf >= 0 ? blk[bf * PBN + bx * PSA + by * PSB + bz] : (BT)0;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// grid: sum partial blocks, v = p/m, v_update, fixes (grid.cpp:448-466,
+// fix_body_force.cpp:117-146, fix_velocity_nodes.cpp:130-271)
+// ---------------------------------------------------------------------------
+template <int DIM, int S1, int TX, int TY, int TZ, int PREC>
+__global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
+  using R = typename Real<PREC>::T;
+  constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
+  using BT = typename std::conditional<FX, long long, double>::type;
+  R *gmass = gptr<PREC>(a.gmass), *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int TZZ = DIM == 3 ? TZ : 1;
+  constexpr int NO = TX * TY * TZZ;
+  const int off = S1 == 4 ? 1 : 0;
+  const int up = S1 == 4 ? 1 : 0;
+  const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int total = a.sc->dil_count * NO;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const int t = a.dil_list[q / NO], o = q % NO;
+    const double dt = tile_dt(a, t);
+    const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
+    const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
+    if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
+    const int64_t tag = ((int64_t)i * a.n[1] + j) * a.n[2] + k;
+    BT isum[SCR_FIELDS];
+    for (int f = 0; f < SCR_FIELDS; f++) isum[f] = 0;
+    for (int dx = -1; dx <= up; dx++)
+      for (int dy = -1; dy <= up; dy++)
+        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? up : 0); dz++) {
+          const int ux = tx + dx, uy = ty + dy, uz = tz + dz;
+          if (ux < 0 || uy < 0 || uz < 0 || ux >= nt0 || uy >= nt1 || uz >= nt2) continue;
+          const int slot = a.tile_slot[(ux * nt1 + uy) * nt2 + uz];
+          if (slot < 0) continue;
+          const int bx = i - (ux * TX - off), by = j - (uy * TY - off);
+          const int bz = DIM == 3 ? k - (uz * TZ - off) : 0;
+          if (bx < 0 || by < 0 || bz < 0 || bx >= EX || by >= EY || bz >= EZ) continue;
+          const int idx = (bx * EY + by) * EZ + bz;
+          const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + idx;
+          if (remap) {
+            for (int f = 1; f <= 3; f++) isum[f] += src[f * BN];
+          } else {
+            for (int f = 0; f < SCR_FIELDS; f++) isum[f] += src[f * BN];
+          }
+        }
+    double sum[SCR_FIELDS];
+    for (int f = 0; f < SCR_FIELDS; f++)
+      sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
+                  : (double)isum[f];
+    const uint32_t mask = a.gmask[tag];
+    if (remap) {
+      // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
+      // node mass, then FixVelocityNodes::post_velocities_to_grid.
+      const double m = gmass[tag];
+      double v[3];
+      for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+      for (int fi = 0; fi < a.nfv; fi++)
+        if (mask & a.fv[fi].bit)
+          for (int d = 0; d < 3; d++)
+            if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
+      for (int d = 0; d < 3; d++) gv[d * a.nn + tag] = (R)v[d];
+      continue;
+    }
+    const double m = sum[0];
+    double v[3], vu[3], mb[3] = {0, 0, 0};
+    for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+    for (int fi = 0; fi < a.nbf; fi++)
+      if (m > 0 && (mask & a.bf[fi].bit))
+        for (int d = 0; d < 3; d++)
+          if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
+    for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
+    for (int fi = 0; fi < a.nfv; fi++)
+      if (mask & a.fv[fi].bit)
+        for (int d = 0; d < 3; d++)
+          if (a.fv[fi].set[d]) {
+            vu[d] = a.fv[fi].val[d];
+            v[d] = a.fv[fi].prev[d];
+          }
+    gmass[tag] = (R)m;
+    for (int d = 0; d < 3; d++) {
+      gv[d * a.nn + tag] = (R)v[d];
+      gvu[d * a.nn + tag] = (R)vu[d];
+    }
+  }
+}
+
+// Grid state for download: nodes outside every touched tile hold zeros
+// (and their velocity-fix values), as the reference's reset leaves them.
+template <int DIM, int TX, int TY, int TZ, int PREC>
+__global__ void k_grid_clean(KArgs a) {
+  using R = typename Real<PREC>::T;
+  R *gmass = gptr<PREC>(a.gmass), *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int S1 = a.S1;
+  const int up = S1 == 4 ? 1 : 0;
+  for (int64_t tag = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tag < a.nn; tag += stride) {
+    const int k = (int)(tag % a.n[2]), j = (int)((tag / a.n[2]) % a.n[1]),
+              i = (int)(tag / ((int64_t)a.n[2] * a.n[1]));
+    const int tx = i / TX, ty = j / TY, tz = DIM == 3 ? k / TZ : 0;
+    bool touched = false;
+    for (int dx = -1; dx <= up && !touched; dx++)
+      for (int dy = -1; dy <= up && !touched; dy++)
+        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? up : 0) && !touched; dz++) {
+          const int ux = tx + dx, uy = ty + dy, uz = tz + dz;
+          if (ux < 0 || uy < 0 || uz < 0 || ux >= a.tg.nt[0] || uy >= nt1 || uz >= nt2) continue;
+          if (a.tile_slot[(ux * nt1 + uy) * nt2 + uz] >= 0) touched = true;
+        }
+    if (touched) continue;
+    double v[3] = {0, 0, 0}, vu[3] = {0, 0, 0};
+    const uint32_t mask = a.gmask[tag];
+    for (int fi = 0; fi < a.nfv; fi++)
+      if (mask & a.fv[fi].bit)
+        for (int d = 0; d < 3; d++)
+          if (a.fv[fi].set[d]) {
+            vu[d] = a.fv[fi].val[d];
+            v[d] = a.fv[fi].prev[d];
+          }
+    gmass[tag] = 0;
+    for (int d = 0; d < 3; d++) {
+      gv[d * a.nn + tag] = (R)v[d];
+      gvu[d * a.nn + tag] = (R)vu[d];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// G2P helpers
+// ---------------------------------------------------------------------------
+__device__ inline void block_max_to(double val, unsigned long long *dst) {
+  // val >= 0 (or NaN handled by caller); wave max then LDS then atomic max
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    double other = __shfl_xor(val, o, 64);
+    val = other > val ? other : val;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) red[wave] = val;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] > m ? red[w] : m;
+    if (m > 0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__device__ inline void block_min_ratio(double val, unsigned long long *dst) {
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    const double other = __shfl_xor(val, o, 64);
+    val = other < val ? other : val;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) red[wave] = val;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] < m ? red[w] : m;
+    if (m < 1.0) atomicMin(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// CFL of a batched context: the block's maximum wave speed (and smallest
+// |Re eig(F)|) of tile t go to the tile's instance; the accumulators restart.
+// Block-uniform (it synchronises the block).
+__device__ inline void flush_tile_cfl(const KArgs &a, int t, double &cmax, double &rmin) {
+  __shared__ unsigned long long sink[2];   // for tiles no instance owns
+  const int i = a.tinst[t];
+  block_max_to(cmax, i >= 0 ? &a.inst[i].max_c_bits : &sink[0]);
+  if (a.tl) block_min_ratio(rmin, i >= 0 ? &a.inst[i].min_ratio_bits : &sink[1]);
+  cmax = 0;
+  rmin = 1.0;
+  __syncthreads();   // thread 0 has read the reduction slots before they are reused
+}
+
+// End of a CFL kernel: the block's reductions into the context's scalars
+// (batched contexts have flushed per tile).
+__device__ inline void finish_cfl(const KArgs &a, double cmax, double rmin) {
+  if (a.ninst) return;
+  block_max_to(cmax, &a.sc->max_c_bits);
+  if (a.tl) block_min_ratio(rmin, &a.sc->min_ratio_bits);
+}
+
+
+// Load the tile's node block (velocities) into LDS: comps [nc][BN].
+template <int DIM, int S1, int TX, int TY, int TZ, int MODE, int PREC>
+__device__ inline void load_node_block(const KArgs &a, typename Real<PREC>::T *nb, int ox, int oy,
+                                       int oz) {
+  using R = typename Real<PREC>::T;
+  const R *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
+  // MODE 0: v_update and v_update - v (USL / MUSL part A); MODE 1: v (MUSL part B)
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  const int off = S1 == 4 ? 1 : 0;
+  for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+    const int bz = q % EZ, by = (q / EZ) % EY, bx = q / (EZ * EY);
+    const int i = ox - off + bx, j = oy - off + by, k = DIM == 3 ? oz - off + bz : 0;
+    const bool ok = i >= 0 && j >= 0 && k >= 0 && i < a.n[0] && j < a.n[1] && k < a.n[2];
+    const int64_t tag = ok ? ((int64_t)i * a.n[1] + j) * a.n[2] + k : 0;
+    for (int d = 0; d < DIM; d++) {
+      if (MODE == 0) {
+        const R vu = ok ? gvu[d * a.nn + tag] : (R)0;
+        const R v = ok ? gv[d * a.nn + tag] : (R)0;
+        nb[d * BN + q] = vu;
+        nb[(DIM + d) * BN + q] = vu - v;
+      } else {
+        nb[d * BN + q] = ok ? gv[d * a.nn + tag] : (R)0;
+      }
+    }
+  }
+}
+
+// Gather over the stencil: vu, acc (MODE 0 / A), L (USL / B).
+template <int DIM, int S1, int EY, int EZ, int BN, bool WANT_VA, bool WANT_L, bool L_FROM_V,
+          class R>
+__device__ inline void gather_stencil(const R *nb, const int *cl, const R (*w)[S1],
+                                      const R (*dw)[S1], R *vu, R *acc, R *L) {
+#pragma unroll
+  for (int ia = 0; ia < S1; ia++) {
+#pragma unroll
+    for (int ib = 0; ib < S1; ib++) {
+      R wxy = w[0][ia] * w[1][ib];
+      R dxy = dw[0][ia] * w[1][ib];
+      R xdy = w[0][ia] * dw[1][ib];
+      // Opaque barrier: the previous row's accumulations complete here, and
+      // neither this row's LDS loads nor its weight products move above it.
+      // Without it the scheduler defers whole FMA chains and spills.
+      if (WANT_VA)
+        for (int d = 0; d < DIM; d++) asm volatile("" : "+v"(vu[d]), "+v"(acc[d]));
+      if (WANT_L)
+        for (int d = 0; d < 9; d++) asm volatile("" : "+v"(L[d]));
+      asm volatile("" : "+v"(wxy), "+v"(dxy), "+v"(xdy)::"memory");
+#pragma unroll
+      for (int ic = 0; ic < (DIM == 3 ? S1 : 1); ic++) {
+        R wt, g0, g1, g2 = 0;
+        int idx;
+        if (DIM == 3) {
+          wt = wxy * w[2][ic];
+          g0 = dxy * w[2][ic];
+          g1 = xdy * w[2][ic];
+          g2 = wxy * dw[2][ic];
+          idx = ((cl[0] + ia) * EY + (cl[1] + ib)) * EZ + (cl[2] + ic);
+        } else {
+          wt = wxy;
+          g0 = dxy;
+          g1 = xdy;
+          idx = (cl[0] + ia) * EY + (cl[1] + ib);
+        }
+        R vn[DIM];
+        for (int d = 0; d < DIM; d++) vn[d] = nb[d * BN + idx];
+        if (WANT_VA) {
+          for (int d = 0; d < DIM; d++) {
+            vu[d] = fma(wt, vn[d], vu[d]);
+            acc[d] = fma(wt, nb[(DIM + d) * BN + idx], acc[d]);
+          }
+        }
+        if (WANT_L) {
+          const R gg[3] = {g0, g1, g2};
+          for (int d = 0; d < DIM; d++)
+            for (int e = 0; e < DIM; e++) L[3 * d + e] = fma(vn[d], gg[e], L[3 * d + e]);
+        }
+        (void)L_FROM_V;
+      }
+    }
+  }
+}
+
+// 1D weights of one axis in FP64 (reference rounding), stored as R.
+template <int S1, class R>
+__device__ inline void weights_1d_as(double x, double lo, double h, double inv_h, int n, int base,
+                                     R *w, R *dw) {
+  double wd[S1], dwd[S1];
+  weights_1d<S1>(x, lo, h, inv_h, n, base, wd, dwd);
+#pragma unroll
+  for (int q = 0; q < S1; q++) {
+    w[q] = (R)wd[q];
+    dw[q] = (R)dwd[q];
+  }
+}
+
+// F update (solid.cpp:1181-1216, UL): F = (I + dt L) F, J = det F.
+__device__ inline double update_F(double dt, const double *L, double *F) {
+  double M[9], Fn[9];
+  for (int e = 0; e < 9; e++) M[e] = ((e % 4 == 0) ? 1.0 : 0.0) + dt * L[e];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++)
+      Fn[3 * r + c] = M[3 * r] * F[c] + M[3 * r + 1] * F[3 + c] + M[3 * r + 2] * F[6 + c];
+  for (int e = 0; e < 9; e++) F[e] = Fn[e];
+  return det3(F);
+}
+
+// D = sym(L) (solid.cpp:1239), stress update (solid.cpp:1246-1374) and the
+// particle's CFL wave speed (:1376-1385); returns -1 for fully damaged
+// particles (skipped by the CFL maximum).
+// SURR: the JC + shock update with the learned material functions; every
+// lane of the wave calls it (live == false for lanes without a particle).
+template <int MC, bool SURR = false>
+__device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
+                                          double *sig, double *eel, double &ep, double &epdot,
+                                          double &dinit, double &dmg, int mat, const double *v,
+                                          int pid, bool live = true,
+                                          const SurrogateDev *sd = nullptr) {
+  const DevMat &m = a.mats[mat];
+  const double rho = m.rho0 / J;
+  double D[6];
+  D[0] = 0.5 * (L[0] + L[0]);
+  D[1] = 0.5 * (L[4] + L[4]);
+  D[2] = 0.5 * (L[8] + L[8]);
+  D[3] = 0.5 * (L[1] + L[3]);
+  D[4] = 0.5 * (L[2] + L[6]);
+  D[5] = 0.5 * (L[5] + L[7]);
+  if constexpr (SURR)
+    update_stress_jc_surrogate(m, *sd, live, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg,
+                               a.sc);
+  else
+    update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  if (dmg >= 1.0) return -1.0;
+  const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
+  const double c = sqrt(m.cfl_k43g / rho) + vm;
+  if (isnan(c) && live) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
+  return c;
+}
+
+__device__ inline bool outside_box(const KArgs &a, const double *x) {
+  // Domain::inside (domain.cpp:174-182); in 2D z == boxlo[2] == boxhi[2] == 0
+  for (int d = 0; d < a.dim; d++)
+    if (!(x[d] >= a.lo[d] && x[d] <= a.hi[d])) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
+// ---------------------------------------------------------------------------
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
+__global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
+  using R = typename Real<PREC>::T;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ R nb[2 * DIM * BN];
+  // SURR: the networks and envelope, staged in LDS once per block (read at
+  // each use instead of being held in registers across the particle loop)
+  __shared__ __attribute__((aligned(16))) unsigned char sbuf[SURR ? sizeof(SurrogateDev) : 16];
+  const SurrogateDev *sd = reinterpret_cast<const SurrogateDev *>(sbuf);
+  if constexpr (SURR) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(a.surr);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(sbuf);
+    for (int q = threadIdx.x; q < (int)(sizeof(SurrogateDev) / 4); q += blockDim.x) dst[q] = src[q];
+  }
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double alpha = a.flip;
+  double cmax = 0, rmin = 1.0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const double inv_dt = 1.0 / dt;
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    // live == false: a lane without a particle that still takes part in the
+    // wave-wide surrogate evaluation (it computes on particle p1 - 1 and
+    // stores nothing)
+    auto particle = [&](const int j, const bool live) {
+      const int slot = a.order[j];
+      double x[3], v[3];
+      for (int d = 0; d < 3; d++) {
+        x[d] = LDA(PX + d, slot);
+        v[d] = LDA(VX + d, slot);
+      }
+      R w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double vu[3], acc[3], L[9];
+      {
+        R vur[3] = {0, 0, 0}, accr[3] = {0, 0, 0}, Lr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vur, accr, Lr);
+        for (int d = 0; d < 3; d++) {
+          vu[d] = vur[d];
+          acc[d] = accr[d];
+        }
+        for (int e = 0; e < 9; e++) L[e] = Lr[e];
+      }
+      const int pid = IFLD(a.ia, PID, slot);
+      for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
+      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
+      if (live && outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      // State is written in sorted order as soon as each part is final, so
+      // it does not stay live across the constitutive update.
+      const int mat = IFLD(a.ia, MATI, slot);
+      if (live) {
+        for (int d = 0; d < 3; d++) {
+          STB(PX + d, j, x[d]);
+          STB(VX + d, j, v[d]);
+        }
+        a.key[j] = particle_key(a, x[0], x[1], x[2], pid);
+        IFLD(a.ib, PID, j) = pid;
+        IFLD(a.ib, MATI, j) = mat;
+      }
+      const double vol0 = LDA(VOL0, slot);
+      if (live) {
+        STB(MASS, j, LDA(MASS, slot));
+        STB(VOL0, j, vol0);
+      }
+      double J;
+      {
+        double F[9];
+        for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot);
+        J = update_F(dt, L, F);
+        if (live)
+          for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
+      }
+      if (live) STB(VOL, j, J * vol0);
+      double dmg = LDA(DMG, slot);
+      if (live && J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+      double sig[6], eel[6];
+      for (int e = 0; e < 6; e++) {
+        sig[e] = LDA(SXX + e, slot);
+        eel[e] = LDA(EXX + e, slot);
+      }
+      double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
+      double dinit = LDA(DINIT, slot);
+      const double c = stress_and_speed<MC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
+                                                  mat, v, pid, live, sd);
+      if (!live) return;
+      if (c > cmax) cmax = c;
+      for (int e = 0; e < 6; e++) {
+        STB(SXX + e, j, sig[e]);
+        STB(EXX + e, j, eel[e]);
+      }
+      STB(EP, j, ep);
+      STB(EPDOT, j, epdot);
+      STB(DINIT, j, dinit);
+      STB(DMG, j, dmg);
+    };
+    if constexpr (SURR) {
+      for (int jb = p0; jb < p1; jb += blockDim.x) {
+        const int j = jb + (int)threadIdx.x;
+        particle(j < p1 ? j : p1 - 1, j < p1);
+      }
+    } else {
+      for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) particle(j, true);
+    }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+  }
+  finish_cfl(a, cmax, rmin);
+}
+
+// ---------------------------------------------------------------------------
+// TLMPM particle update after the gather (FP64): solid.cpp:1155-1244 (TL
+// branch) and 1246-1437.  F += dt Fdot, F^-1, J, vol, rho, R = polar(F),
+// D = sym(R^T (L + L^T) R) / 2 with L = Fdot F^-1, the stress update,
+// vol0 PK1, the wave speed and the smallest |Re eig(F)| (both -1 for fully
+// damaged particles, which the CFL skips).  Writes slot j of the next state.
+// ---------------------------------------------------------------------------
+
+template <int MC>
+__device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot, int j,
+                                 int pid, int mat, const double *v, double &cmax,
+                                 double &rmin) {
+  constexpr int PREC = KMPM_PRECISION_FP64;
+  double F[9], Finv[9], R[9];
+  for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot) + dt * Fdot[e];   // :1181-1182
+  m3_inverse(F, Finv);                                                   // :1186
+  const double J = m3_det(F);                                            // :1203
+  const double vol0 = LDA(VOL0, slot);
+  double dmg = LDA(DMG, slot);
+  if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);    // :1208-1215
+  if (!polar_rotation(F, R)) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);    // :1222-1232
+  double D[6];
+  tl_rate(Fdot, Finv, R, D);                                             // :1225-1226
+  const DevMat &m = a.mats[mat];
+  const double rho = m.rho0 / J;
+  double sig[6], eel[6];
+  for (int e = 0; e < 6; e++) {
+    sig[e] = LDA(SXX + e, slot);
+    eel[e] = LDA(EXX + e, slot);
+  }
+  double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot), dinit = LDA(DINIT, slot);
+  update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  double S[9] = {sig[0], sig[3], sig[4], sig[3], sig[1], sig[5], sig[4], sig[5], sig[2]};
+  double P[9];
+  pk1(vol0, J, R, S, Finv, P);
+  for (int e = 0; e < 9; e++) {
+    STB(F00 + e, j, F[e]);
+    STB(P00 + e, j, P[e]);
+  }
+  STB(VOL0, j, vol0);
+  STB(VOL, j, J * vol0);
+  for (int e = 0; e < 6; e++) {
+    STB(SXX + e, j, sig[e]);
+    STB(EXX + e, j, eel[e]);
+  }
+  STB(EP, j, ep);
+  STB(EPDOT, j, epdot);
+  STB(DINIT, j, dinit);
+  STB(DMG, j, dmg);
+  if (dmg >= 1.0) return;                                                // :1378-1379
+  const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
+  const double c = sqrt(m.cfl_k43g / rho) + vm;
+  if (isnan(c)) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
+  if (c > cmax) cmax = c;
+  double re[3];
+  eig_real_parts(F, re);                                                 // :1400-1408
+  for (int k = 0; k < 3; k++) rmin = fabs(re[k]) < rmin ? fabs(re[k]) : rmin;
+}
+
+// TLMPM fused USL G2P: weights of x0 on the solid's grid, Fdot(v_update),
+// G2P, advance (no box check), then tl_update.
+template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+__global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
+  constexpr int PREC = KMPM_PRECISION_FP64;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ double nb[2 * DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double alpha = a.flip;
+  double cmax = 0, rmin = 1.0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const double inv_dt = 1.0 / dt;
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x0[3], x[3], v[3];
+      for (int d = 0; d < 3; d++) {
+        x0[d] = PF(a.pa, X0X + d, slot);
+        x[d] = PF(a.pa, PX + d, slot);
+        v[d] = PF(a.pa, VX + d, slot);
+      }
+      double w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double vu[3] = {0, 0, 0}, acc[3] = {0, 0, 0}, Fdot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vu, acc, Fdot);
+      const int pid = IFLD(a.ia, PID, slot);
+      const int mat = IFLD(a.ia, MATI, slot);
+      for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
+      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];                 // solid.cpp:616
+      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      for (int d = 0; d < 3; d++) {
+        PF(a.pb, PX + d, j) = x[d];
+        PF(a.pb, VX + d, j) = v[d];
+        PF(a.pb, X0X + d, j) = x0[d];
+      }
+      a.key[j] = particle_key(a, x0[0], x0[1], x0[2], pid);
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
+      tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+    }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+  }
+  finish_cfl(a, cmax, rmin);
+}
+
+// ---------------------------------------------------------------------------
+// MUSL part A: G2P + advance (musl.cpp:61-67); keeps start-of-step x
+// ---------------------------------------------------------------------------
+// TL: weights of x0 on the solid's grid, no box check, x0 carried along.
+template <int DIM, int S1, int TX, int TY, int TZ, int PREC, bool TL>
+__global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
+  using R = typename Real<PREC>::T;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ R nb[2 * DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  const double alpha = a.flip;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const double inv_dt = 1.0 / dt;
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x[3], v[3];
+      for (int d = 0; d < 3; d++) {
+        x[d] = LDA(PX + d, slot);
+        v[d] = LDA(VX + d, slot);
+      }
+      double xw[3];
+      for (int d = 0; d < 3; d++) xw[d] = TL ? PF(a.pa, X0X + d, slot) : x[d];
+      R w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(xw[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d_as<S1>(xw[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double vu[3], acc[3];
+      {
+        R vur[3] = {0, 0, 0}, accr[3] = {0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, true, false, false>(nb, cl, w, dw, vur, accr,
+                                                                (R *)nullptr);
+        for (int d = 0; d < 3; d++) {
+          vu[d] = vur[d];
+          acc[d] = accr[d];
+        }
+      }
+      const int pid = IFLD(a.ia, PID, slot);
+      if (TL) {
+        for (int d = 0; d < 3; d++) PF(a.pb, X0X + d, j) = xw[d];
+      } else {
+        for (int d = 0; d < 3; d++) a.xo[(size_t)d * a.cap + j] = x[d];
+      }
+      for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
+      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
+      if (!TL && outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      for (int d = 0; d < 3; d++) {
+        STB(PX + d, j, x[d]);
+        STB(VX + d, j, v[d]);
+      }
+      STB(MASS, j, LDA(MASS, slot));
+      // next step's cell: of the advanced x (ULMPM) or of x0 (TLMPM)
+      const double *xk = TL ? xw : x;
+      a.key[j] = particle_key(a, xk[0], xk[1], xk[2], pid);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MUSL part B: L from re-mapped v (old weights), F, stress, CFL (musl.cpp:75-77)
+// ---------------------------------------------------------------------------
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
+__global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
+  using R = typename Real<PREC>::T;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ R nb[DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  double cmax = 0, rmin = 1.0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x[3];
+      for (int d = 0; d < 3; d++) x[d] = a.xo[(size_t)d * a.cap + j];
+      R w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double L[9];
+      {
+        R Lr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, (R *)nullptr,
+                                                               (R *)nullptr, Lr);
+        for (int e = 0; e < 9; e++) L[e] = Lr[e];
+      }
+      const int pid = IFLD(a.ia, PID, slot);
+      double v[3];
+      for (int d = 0; d < 3; d++) v[d] = LDB(VX + d, j);
+      const int mat = IFLD(a.ia, MATI, slot);
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      const double vol0 = LDA(VOL0, slot);
+      STB(VOL0, j, vol0);
+      double J;
+      {
+        double F[9];
+        for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot);
+        J = update_F(dt, L, F);
+        for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
+      }
+      STB(VOL, j, J * vol0);
+      double dmg = LDA(DMG, slot);
+      if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+      double sig[6], eel[6];
+      for (int e = 0; e < 6; e++) {
+        sig[e] = LDA(SXX + e, slot);
+        eel[e] = LDA(EXX + e, slot);
+      }
+      double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
+      double dinit = LDA(DINIT, slot);
+      const double c =
+          stress_and_speed<MC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
+      if (c > cmax) cmax = c;
+      for (int e = 0; e < 6; e++) {
+        STB(SXX + e, j, sig[e]);
+        STB(EXX + e, j, eel[e]);
+      }
+      STB(EP, j, ep);
+      STB(EPDOT, j, epdot);
+      STB(DINIT, j, dinit);
+      STB(DMG, j, dmg);
+    }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+  }
+  finish_cfl(a, cmax, rmin);
+}
+
+// TLMPM MUSL part B: Fdot from the re-mapped v with the x0 weights
+// (compute_rate_deformation_gradient_TL(true), musl.cpp:75-77), then
+// tl_update.
+template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+__global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
+  constexpr int PREC = KMPM_PRECISION_FP64;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
+  __shared__ double nb[DIM * BN];
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int off = S1 == 4 ? 1 : 0;
+  const int ntile = a.sc->tile_count;
+  double cmax = 0, rmin = 1.0;
+  for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
+    const int t = a.tile_list[li];
+    const double dt = tile_dt(a, t);
+    const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    __syncthreads();
+    load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
+    __syncthreads();
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
+      const int slot = a.order[j];
+      double x0[3];
+      for (int d = 0; d < 3; d++) x0[d] = PF(a.pa, X0X + d, slot);
+      double w[3][S1], dw[3][S1];
+      int cl[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
+        cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
+        weights_1d<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+      }
+      double Fdot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, (double *)nullptr,
+                                                             (double *)nullptr, Fdot);
+      const int pid = IFLD(a.ia, PID, slot);
+      const int mat = IFLD(a.ia, MATI, slot);
+      IFLD(a.ib, PID, j) = pid;
+      IFLD(a.ib, MATI, j) = mat;
+      double v[3];
+      for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
+      tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+    }
+    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+  }
+  finish_cfl(a, cmax, rmin);
+}
+
+// ---------------------------------------------------------------------------
+// tile shapes and launch helpers
+// ---------------------------------------------------------------------------
+// Tile shapes (cells per axis) the tile kernels are compiled for, CH the
+// particles a P2G wave stages at once.  The node arrays are z-contiguous in
+// 3D (tag = (i ny + j) nz + k) and y-contiguous in 2D, so the longer extents
+// go there: they lengthen the contiguous runs of the grid update and of the
+// node block loads.  kmpm_autotune picks the variant per context.
+constexpr int NTILE_VARIANTS = 3;
+template <int DIM, int S1, int V> struct Tiles;
+template <> struct Tiles<3, 4, 0> { static constexpr int TX = 4, TY = 4, TZ = 4, CH = 24; };
+template <> struct Tiles<3, 4, 1> { static constexpr int TX = 4, TY = 4, TZ = 8, CH = 24; };
+template <> struct Tiles<3, 4, 2> { static constexpr int TX = 4, TY = 8, TZ = 8, CH = 24; };
+template <> struct Tiles<3, 2, 0> { static constexpr int TX = 4, TY = 4, TZ = 4, CH = 32; };
+template <> struct Tiles<3, 2, 1> { static constexpr int TX = 4, TY = 4, TZ = 8, CH = 32; };
+template <> struct Tiles<3, 2, 2> { static constexpr int TX = 8, TY = 8, TZ = 8, CH = 32; };
+template <> struct Tiles<2, 4, 0> { static constexpr int TX = 8, TY = 8, TZ = 1, CH = 64; };
+template <> struct Tiles<2, 4, 1> { static constexpr int TX = 8, TY = 16, TZ = 1, CH = 64; };
+template <> struct Tiles<2, 4, 2> { static constexpr int TX = 16, TY = 16, TZ = 1, CH = 64; };
+template <> struct Tiles<2, 2, 0> { static constexpr int TX = 8, TY = 8, TZ = 1, CH = 64; };
+template <> struct Tiles<2, 2, 1> { static constexpr int TX = 8, TY = 16, TZ = 1, CH = 64; };
+template <> struct Tiles<2, 2, 2> { static constexpr int TX = 16, TY = 16, TZ = 1, CH = 64; };
+
+static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+static inline int persistent_grid(kmpm_ctx *c, int per_cu) {
+  int64_t g = (int64_t)c->cu_count * per_cu;
+  int64_t maxt = c->a.tg.ntiles < c->a.np ? c->a.tg.ntiles : c->a.np;
+  if (maxt < 1) maxt = 1;
+  return (int)(g < maxt ? g : maxt);
+}
+
+#define PREC_DISPATCH(c, ...)                                                             \
+  do {                                                                                    \
+    if ((c)->a.prec == KMPM_PRECISION_MIXED) { constexpr int PREC = KMPM_PRECISION_MIXED; __VA_ARGS__; } \
+    else { constexpr int PREC = KMPM_PRECISION_FP64; __VA_ARGS__; }                        \
+  } while (0)
+
+#define MC_DISPATCH(c, ...)                                                         \
+  do {                                                                              \
+    switch ((c)->a.mat_class) {                                                     \
+      case MC_LINEAR: { constexpr int MC = MC_LINEAR; __VA_ARGS__; } break;         \
+      case MC_JC_SHOCK: { constexpr int MC = MC_JC_SHOCK; __VA_ARGS__; } break;     \
+      case MC_FLUID: { constexpr int MC = MC_FLUID; __VA_ARGS__; } break;           \
+      default: { constexpr int MC = MC_GENERIC; __VA_ARGS__; } break;               \
+    }                                                                               \
+  } while (0)
+
+// Launchers of the tile kernels for one (dimension, shape, tile variant);
+// defined in kmpm_tile_launch.h and instantiated once per variant
+// (kmpm_tiles.hip), so the variants compile in parallel.
+template <int DIM, int S1, int V> struct TileLaunch {
+  static hipError_t p2g(kmpm_ctx *c, int remap);
+  static hipError_t grid_update(kmpm_ctx *c, int remap);
+  static hipError_t grid_clean(kmpm_ctx *c);
+  static hipError_t g2p_usl(kmpm_ctx *c);
+  static hipError_t g2p_musl_a(kmpm_ctx *c);
+  static hipError_t stress_musl_b(kmpm_ctx *c);
+};
+
+}  // namespace kmpm

@@ -1,0 +1,124 @@
+// kmpm_tile_launch.h -- launchers of the tile kernels for one (dimension,
+// shape function, tile variant).  Included only by kmpm_tiles.hip, which
+// instantiates TileLaunch for one variant per object file.
+#pragma once
+
+#include "kmpm_kernels.h"
+
+namespace kmpm {
+
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::p2g(kmpm_ctx *c, int remap) {
+  using TT = Tiles<DIM, S1, V>;
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 8);
+  if (c->a.tl) {   // TLMPM: FP64 only
+    constexpr int PREC = KMPM_PRECISION_FP64;
+    if (remap)
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC, true>),
+                         dim3(g), dim3(256), 0, c->stream, c->a);
+    else
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC, true>),
+                         dim3(g), dim3(256), 0, c->stream, c->a);
+    return hipGetLastError();
+  }
+  PREC_DISPATCH(c, {
+    if (remap)
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC, false>),
+                         dim3(g), dim3(256), 0, c->stream, c->a);
+    else
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC, false>),
+                         dim3(g), dim3(256), 0, c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::grid_update(kmpm_ctx *c, int remap) {
+  using TT = Tiles<DIM, S1, V>;
+  int64_t total = (int64_t)c->a.tg.ntiles * c->a.tg.T[0] * c->a.tg.T[1] * c->a.tg.T[2];
+  int g = nblk(total, 256);
+  if (g > c->cu_count * 8) g = c->cu_count * 8;
+  if (g < 1) g = 1;
+  PREC_DISPATCH(c, {
+    hipLaunchKernelGGL((k_grid_update<DIM, S1, TT::TX, TT::TY, TT::TZ, PREC>), dim3(g), dim3(256),
+                       0, c->stream, c->a, remap);
+  });
+  return hipGetLastError();
+}
+
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::grid_clean(kmpm_ctx *c) {
+  using TT = Tiles<DIM, S1, V>;
+  int g = nblk(c->a.nn, 256);
+  if (g > c->cu_count * 16) g = c->cu_count * 16;
+  PREC_DISPATCH(c, {
+    hipLaunchKernelGGL((k_grid_clean<DIM, TT::TX, TT::TY, TT::TZ, PREC>), dim3(g), dim3(256), 0,
+                       c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::g2p_usl(kmpm_ctx *c) {
+  using TT = Tiles<DIM, S1, V>;
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 8);
+  if (c->a.tl) {
+    MC_DISPATCH(c, {
+      hipLaunchKernelGGL((k_g2p_usl_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256),
+                         0, c->stream, c->a);
+    });
+    return hipGetLastError();
+  }
+  const bool surr = c->a.surr_on && c->a.mat_class == MC_JC_SHOCK &&
+                    c->a.prec == KMPM_PRECISION_MIXED;
+  MC_DISPATCH(c, PREC_DISPATCH(c, {
+    constexpr bool CAN_SURR = MC == MC_JC_SHOCK && PREC == KMPM_PRECISION_MIXED;
+    if (CAN_SURR && surr)
+      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, CAN_SURR>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+    else
+      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, false>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+  }));
+  return hipGetLastError();
+}
+
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::g2p_musl_a(kmpm_ctx *c) {
+  using TT = Tiles<DIM, S1, V>;
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 8);
+  if (c->a.tl) {
+    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, KMPM_PRECISION_FP64, true>),
+                       dim3(g), dim3(256), 0, c->stream, c->a);
+    return hipGetLastError();
+  }
+  PREC_DISPATCH(c, {
+    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, PREC, false>), dim3(g),
+                       dim3(256), 0, c->stream, c->a);
+  });
+  return hipGetLastError();
+}
+
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::stress_musl_b(kmpm_ctx *c) {
+  using TT = Tiles<DIM, S1, V>;
+  if (c->a.np == 0) return hipSuccess;
+  const int g = persistent_grid(c, 8);
+  if (c->a.tl) {
+    MC_DISPATCH(c, {
+      hipLaunchKernelGGL((k_stress_musl_b_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+    });
+    return hipGetLastError();
+  }
+  MC_DISPATCH(c, PREC_DISPATCH(c, {
+    hipLaunchKernelGGL((k_stress_musl_b<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC>), dim3(g),
+                       dim3(256), 0, c->stream, c->a);
+  }));
+  return hipGetLastError();
+}
+
+}  // namespace kmpm

@@ -16,6 +16,7 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_NO_DEVICE, ERR_CAPACITY = 1, 2, 3, 4
 ERR_PARTICLE_LEFT_BOX, ERR_JACOBIAN, ERR_DTCFL, ERR_WAVE_SPEED, ERR_GRID = 10, 11, 12, 13, 14
 ERR_FIXED_POINT, ERR_INTERNAL, ERR_INSTANCE = 15, 16, 17
+NTILE_VARIANTS = 3
 SURROGATE_NPARAM = 1185
 METHOD_ULMPM, METHOD_TLMPM = 0, 1
 SCHEME_USL, SCHEME_MUSL = 0, 1
@@ -157,6 +158,8 @@ def _declare(lib):
         "kmpm_set_instances": (C.c_int, [vp, C.c_int, P(C.c_int32), P(C.c_int32)]),
         "kmpm_instance_time": (C.c_int, [vp, C.c_int, _DP, _DP]),
         "kmpm_step_graph": (C.c_int, [vp, C.c_int64, C.c_int64]),
+        "kmpm_set_tile_variant": (C.c_int, [vp, C.c_int]),
+        "kmpm_autotune": (C.c_int, [vp, C.c_int, P(C.c_int), _DP]),
         "kmpm_get_time": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_int64)]),
         "kmpm_reduce_energy": (C.c_int, [vp, C.c_uint32, P(C.c_double), P(C.c_double)]),
         "kmpm_download_grid": (C.c_int, [vp, _DP, _DP, _DP, C.c_int64]),
@@ -370,6 +373,18 @@ class Context:
         t = I3()
         check(self.L.kmpm_get_tile_shape(self.h, t), "get_tile_shape")
         return tuple(t)
+
+    def set_tile_variant(self, variant):
+        check(self.L.kmpm_set_tile_variant(self.h, int(variant)), "set_tile_variant")
+
+    def autotune(self, trial_steps=10):
+        """Times every tile variant from the current state (which is kept)
+        and keeps the fastest; returns (variant, ms per step per variant)."""
+        v = C.c_int(-1)
+        ms = np.zeros(NTILE_VARIANTS)
+        check(self.L.kmpm_autotune(self.h, int(trial_steps), C.byref(v), ms.ctypes.data_as(_DP)),
+              "autotune")
+        return v.value, ms
 
     def set_surrogate(self, surrogate):
         check(self.L.kmpm_ctx_set_surrogate(self.h, surrogate.h if surrogate else None),

@@ -32,12 +32,13 @@ def _instance_fields(p, i):
     return {k: v[keep] for k, v in p.items()}
 
 
-def compare_instances(batch, steps, chunk, tol=TOL_STEP, conditioned=False):
+def compare_instances(batch, steps, chunk, tol=TOL_STEP, conditioned=False, tile_variant=0):
     """Each instance against its oracle run alone.  With conditioned=True the
     bar per field is max(tol, 10 x e_self), e_self being the largest
     relative difference between the instance's oracle and its oracle runs
     with every position (+-1 ulp) or velocity (+1 ulp) perturbed."""
     ctx = P.context_for_batch(batch)
+    ctx.set_tile_variant(tile_variant)
     oracles = [O.oracle_for(batch.instance(i)) for i in range(batch.ninst)]
     others = [[O.oracle_for(_perturbed(batch.instance(i), k)) for k in ("x+", "x-", "v+")]
               if conditioned else [] for i in range(batch.ninst)]

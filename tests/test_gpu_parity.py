@@ -144,7 +144,8 @@ def _perturbed(prob, kind):
     return q
 
 
-def run_and_compare(prob, steps, chunk, tol=TOL_STEP, check_energy=True, conditioned=False):
+def run_and_compare(prob, steps, chunk, tol=TOL_STEP, check_energy=True, conditioned=False,
+                    tile_variant=0):
     """GPU vs oracle every `chunk` steps.  With conditioned=True the bar per
     field is max(tol, 10 x e_self), e_self being the largest relative
     difference between the oracle and three oracle runs whose input
@@ -152,6 +153,7 @@ def run_and_compare(prob, steps, chunk, tol=TOL_STEP, check_energy=True, conditi
     as close to the oracle as the oracle is to itself."""
     floors = field_floors(prob)
     ctx = P.context_for(prob)
+    ctx.set_tile_variant(tile_variant)
     o = O.oracle_for(prob)
     others = [O.oracle_for(_perturbed(prob, k)) for k in ("x+", "x-", "v+")] if conditioned else []
     worst = {}
