@@ -126,7 +126,7 @@ def train_surrogate(mat):
     return s, rec
 
 
-def run_path(prob, path, steps, warmup, surrogate=None):
+def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5):
     """Times `steps` steps of one path after `warmup` steps; returns the
     timing record and the downloaded particle state."""
     from karamelo_amd import native as N
@@ -141,6 +141,13 @@ def run_path(prob, path, steps, warmup, surrogate=None):
     st, tag = ctx.synchronize()
     if st != 0:
         raise RuntimeError("device error %d (particle %d) during warm-up" % (st, tag))
+    # the autotuner times every tile variant from the warmed-up state, which
+    # it restores, and keeps the fastest for the timed steps
+    if autotune_steps > 0:
+        variant, tune_ms = ctx.autotune(autotune_steps)
+    else:
+        variant, tune_ms = 0, [0.0] * 3
+    log("[%s] autotune: tile %s, ms/step per variant %s" % (path, ctx.tile_shape(), tune_ms))
     device_sync()
     ctx.surrogate_stats()          # resets the counters: the timed steps only
     log("[%s] timed %d steps ..." % (path, steps))
@@ -158,10 +165,65 @@ def run_path(prob, path, steps, warmup, surrogate=None):
         raise RuntimeError("device error %d (particle %d) in the timed region" % (st, tag))
     evals, fallbacks = ctx.surrogate_stats()
     state = ctx.download()
+    tile = ctx.tile_shape()
     ctx.close()
     secs = t1 - t0
     return {"secs": secs, "phases": phases, "surrogate_evals": evals,
-            "surrogate_fallbacks": fallbacks}, state
+            "surrogate_fallbacks": fallbacks,
+            "autotune": {"tile": list(tile), "variant": variant,
+                         "ms_per_step_per_variant": [round(float(v), 4) for v in tune_ms]}}, state
+
+
+def run_batched(steps, warmup, graph_steps=10, n_inst=256):
+    """The batched workload: n_inst independent 2D plane-strain Taylor
+    impacts (particle counts log-uniform in 10k-200k, JC A, B, n, C within
+    +-20% of copper or steel, v0 uniform in 50-300 m/s), mixed precision, on
+    one grid, each with its own dt; the timed steps are replays of a
+    captured graph of `graph_steps` steps."""
+    from karamelo_amd import problems as P
+    log("[batched] building %d instances ..." % n_inst)
+    batch = P.taylor_plates(n_inst=n_inst, precision="mixed")
+    npart = batch.problem.np
+    ctx = P.context_for_batch(batch)
+    batch.problem.particles = None
+    log("[batched] %d particles; warm-up %d steps ..." % (npart, warmup))
+    ctx.step(warmup)
+    st, tag = ctx.synchronize()
+    if st != 0:
+        raise RuntimeError("batched: device error %d (particle %d) during warm-up" % (st, tag))
+    variant, tune_ms = ctx.autotune(5)
+    log("[batched] autotune: tile %s, ms/step per variant %s" % (ctx.tile_shape(), tune_ms))
+    ctx.timing_enable(True)
+    ctx.step(graph_steps)
+    phases = ctx.timing_read()
+    ctx.timing_enable(False)
+    ctx.step_graph(graph_steps, graph_steps)       # capture and a first replay
+    device_sync()
+    ctx.synchronize()
+    log("[batched] timed %d steps (graph replays of %d) ..." % (steps, graph_steps))
+    t0 = time.perf_counter()
+    ctx.step_graph(steps, graph_steps)
+    ctx.synchronize()
+    device_sync()
+    t1 = time.perf_counter()
+    st, tag = ctx.synchronize()
+    if st != 0:
+        raise RuntimeError("batched: device error %d (particle %d)" % (st, tag))
+    dt, tm = ctx.instance_time()
+    tile = ctx.tile_shape()
+    ctx.close()
+    secs = t1 - t0
+    return {"workload": "taylor_plates_2d_%dx" % n_inst, "instances": n_inst, "particles": npart,
+            "particles_per_instance": {"min": min(q["np"] for q in batch.params),
+                                       "max": max(q["np"] for q in batch.params)},
+            "precision": "mixed", "steps": steps, "warmup": warmup,
+            "graph_steps_per_launch": graph_steps,
+            "value": npart * steps / secs, "unit": "particle-steps/s",
+            "ms_per_step": secs * 1e3 / steps,
+            "phases_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in phases.items() if v[1]},
+            "instance_dt_range_s": [float(dt.min()), float(dt.max())],
+            "autotune": {"tile": list(tile), "variant": variant,
+                         "ms_per_step_per_variant": [round(float(v), 4) for v in tune_ms]}}
 
 
 def roofline_of(path, phases, npart, workload):
@@ -222,6 +284,12 @@ def main():
     ap.add_argument("--h", type=float, default=6e-5, help="cell size (m)")
     ap.add_argument("--paths", default="plain-fp64,mixed,mixed+surrogate",
                     help="comma-separated paths to time; the last one is the headline")
+    ap.add_argument("--autotune-steps", type=int, default=5,
+                    help="trial steps per tile variant (0: keep the default tiles)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-instances workload")
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--batched-warmup", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -244,7 +312,8 @@ def main():
         log("training the surrogate ...")
         surrogate, training = train_surrogate(prob.mats[0])
     for path in paths:
-        results[path], states[path] = run_path(prob, path, args.steps, args.warmup, surrogate)
+        results[path], states[path] = run_path(prob, path, args.steps, args.warmup, surrogate,
+                                               autotune_steps=args.autotune_steps)
     prob.particles = None
 
     head = paths[-1]
@@ -260,6 +329,7 @@ def main():
             "phases_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in rp["phases"].items() if v[1]},
             "step_alg_gbs": STEP_ALG_BYTES_PER_PARTICLE[path] * npart / (rp["secs"] / args.steps) / 1e9,
             "roofline": roofline_of(path, rp["phases"], npart, workload),
+            "autotune": rp["autotune"],
         }
         if path == "mixed+surrogate":
             # surrogate evaluations happen inside the fused G2P launches
@@ -288,6 +358,10 @@ def main():
                               "velocity_rel_l2": rel_l2(got["v"], ref["v"])}
     states = None
 
+    batched = None
+    if not args.no_batched:
+        batched = run_batched(args.batched_steps, args.batched_warmup)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -309,6 +383,7 @@ def main():
                                   if "plain-fp64" in per_path else None),
         "accuracy_vs_fp64": accuracy,
         "paths": per_path,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -90,11 +90,12 @@ __device__ inline int tile_local(int c, int org, int T, DevScalars *sc) {
   return l;
 }
 
-// dt of the instance a tile belongs to (batched contexts) or of the context.
-__device__ inline double tile_dt(const KArgs &a, int t) {
-  if (!a.ninst) return a.sc->dt;
+// dt of the instance a tile belongs to (batched contexts) or of the context
+// (dt_ctx, read once per kernel).
+__device__ inline double tile_dt(const KArgs &a, int t, double dt_ctx) {
+  if (!a.ninst) return dt_ctx;
   const int i = a.tinst[t];
-  return i >= 0 ? a.inst[i].dt : a.sc->dt;
+  return i >= 0 ? a.inst[i].dt : dt_ctx;
 }
 
 __device__ inline int tile_key(const KArgs &a, int c0, int c1, int c2) {
@@ -126,15 +127,23 @@ __device__ inline void check_instance(const KArgs &a, const int c[3], int t, int
   if (!ok) set_error(a.sc, KMPM_ERR_INSTANCE, i);
 }
 
-// Sort key of a particle at (x, y, z); `pid` names it in a batched-context error.
-__device__ inline int particle_key(const KArgs &a, double x, double y, double z, int pid) {
-  int c[3];
-  c[0] = cell_of_clamped(x, a.lo[0], a.inv_h, a.n[0]);
-  c[1] = cell_of_clamped(y, a.lo[1], a.inv_h, a.n[1]);
-  c[2] = a.dim == 3 ? cell_of_clamped(z, a.lo[2], a.inv_h, a.n[2]) : 0;
-  const int key = tile_key(a, c[0], c[1], c[2]);
-  if (a.ninst) check_instance(a, c, key / a.tg.cpt, pid);
-  return key;
+// Sort key of a particle at (x, y, z).
+__device__ inline int particle_key(const KArgs &a, double x, double y, double z) {
+  const int c0 = cell_of_clamped(x, a.lo[0], a.inv_h, a.n[0]);
+  const int c1 = cell_of_clamped(y, a.lo[1], a.inv_h, a.n[1]);
+  const int c2 = a.dim == 3 ? cell_of_clamped(z, a.lo[2], a.inv_h, a.n[2]) : 0;
+  return tile_key(a, c0, c1, c2);
+}
+
+// Cell (i, j, k) of a sort key (the inverse of tile_key).
+__device__ inline void key_cell(const KArgs &a, int key, int c[3]) {
+  const TileGeom &g = a.tg;
+  const int t = key / g.cpt, l = key % g.cpt;
+  const int tz = t % g.nt[2], ty = (t / g.nt[2]) % g.nt[1], tx = t / (g.nt[2] * g.nt[1]);
+  const int lz = l % g.T[2], ly = (l / g.T[2]) % g.T[1], lx = l / (g.T[2] * g.T[1]);
+  c[0] = tx * g.T[0] + lx;
+  c[1] = ty * g.T[1] + ly;
+  c[2] = tz * g.T[2] + lz;
 }
 
 // Weight positions: current x (ULMPM) or reference x0 (TLMPM).
@@ -432,9 +441,10 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
   const int up = S1 == 4 ? 1 : 0;
   const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int total = a.sc->dil_count * NO;
+  const double dt_ctx = a.sc->dt;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
     const int t = a.dil_list[q / NO], o = q % NO;
-    const double dt = tile_dt(a, t);
+    const double dt = tile_dt(a, t, dt_ctx);
     const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
     const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
@@ -577,14 +587,15 @@ __device__ inline void block_min_ratio(double val, unsigned long long *dst) {
   }
 }
 
-// CFL of a batched context: the block's maximum wave speed (and smallest
-// |Re eig(F)|) of tile t go to the tile's instance; the accumulators restart.
-// Block-uniform (it synchronises the block).
+// CFL of a batched context: the block's maximum wave speed (and, TLMPM,
+// smallest |Re eig(F)|) of tile t go to the tile's instance; the
+// accumulators restart.  Block-uniform (it synchronises the block).
+template <bool TL>
 __device__ inline void flush_tile_cfl(const KArgs &a, int t, double &cmax, double &rmin) {
   __shared__ unsigned long long sink[2];   // for tiles no instance owns
   const int i = a.tinst[t];
   block_max_to(cmax, i >= 0 ? &a.inst[i].max_c_bits : &sink[0]);
-  if (a.tl) block_min_ratio(rmin, i >= 0 ? &a.inst[i].min_ratio_bits : &sink[1]);
+  if (TL) block_min_ratio(rmin, i >= 0 ? &a.inst[i].min_ratio_bits : &sink[1]);
   cmax = 0;
   rmin = 1.0;
   __syncthreads();   // thread 0 has read the reduction slots before they are reused
@@ -592,12 +603,12 @@ __device__ inline void flush_tile_cfl(const KArgs &a, int t, double &cmax, doubl
 
 // End of a CFL kernel: the block's reductions into the context's scalars
 // (batched contexts have flushed per tile).
+template <bool TL>
 __device__ inline void finish_cfl(const KArgs &a, double cmax, double rmin) {
   if (a.ninst) return;
   block_max_to(cmax, &a.sc->max_c_bits);
-  if (a.tl) block_min_ratio(rmin, &a.sc->min_ratio_bits);
+  if (TL) block_min_ratio(rmin, &a.sc->min_ratio_bits);
 }
-
 
 // Load the tile's node block (velocities) into LDS: comps [nc][BN].
 template <int DIM, int S1, int TX, int TY, int TZ, int MODE, int PREC>
@@ -716,7 +727,8 @@ __device__ inline double stress_and_speed(const KArgs &a, double dt, const doubl
                                           double *sig, double *eel, double &ep, double &epdot,
                                           double &dinit, double &dmg, int mat, const double *v,
                                           int pid, bool live = true,
-                                          const SurrogateDev *sd = nullptr) {
+                                          const SurrogateDev *sd = nullptr,
+                                          SurrCount *cnt = nullptr) {
   const DevMat &m = a.mats[mat];
   const double rho = m.rho0 / J;
   double D[6];
@@ -728,7 +740,7 @@ __device__ inline double stress_and_speed(const KArgs &a, double dt, const doubl
   D[5] = 0.5 * (L[5] + L[7]);
   if constexpr (SURR)
     update_stress_jc_surrogate(m, *sd, live, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg,
-                               a.sc);
+                               *cnt);
   else
     update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   if (dmg >= 1.0) return -1.0;
@@ -767,11 +779,13 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
+  const double dt_ctx = a.sc->dt;
   const double alpha = a.flip;
   double cmax = 0, rmin = 1.0;
+  SurrCount scnt = {0, 0};
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
-    const double dt = tile_dt(a, t);
+    const double dt = tile_dt(a, t, dt_ctx);
     const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
@@ -820,7 +834,7 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
           STB(PX + d, j, x[d]);
           STB(VX + d, j, v[d]);
         }
-        a.key[j] = particle_key(a, x[0], x[1], x[2], pid);
+        a.key[j] = particle_key(a, x[0], x[1], x[2]);
         IFLD(a.ib, PID, j) = pid;
         IFLD(a.ib, MATI, j) = mat;
       }
@@ -848,7 +862,7 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
       double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
       double dinit = LDA(DINIT, slot);
       const double c = stress_and_speed<MC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
-                                                  mat, v, pid, live, sd);
+                                                  mat, v, pid, live, sd, &scnt);
       if (!live) return;
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
@@ -868,9 +882,10 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
     } else {
       for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) particle(j, true);
     }
-    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+    if (a.ninst) flush_tile_cfl<false>(a, t, cmax, rmin);
   }
-  finish_cfl(a, cmax, rmin);
+  finish_cfl<false>(a, cmax, rmin);
+  if constexpr (SURR) surrogate_count_flush(scnt, a.sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -944,11 +959,12 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
+  const double dt_ctx = a.sc->dt;
   const double alpha = a.flip;
   double cmax = 0, rmin = 1.0;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
-    const double dt = tile_dt(a, t);
+    const double dt = tile_dt(a, t, dt_ctx);
     const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
@@ -984,15 +1000,15 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
         PF(a.pb, VX + d, j) = v[d];
         PF(a.pb, X0X + d, j) = x0[d];
       }
-      a.key[j] = particle_key(a, x0[0], x0[1], x0[2], pid);
+      a.key[j] = particle_key(a, x0[0], x0[1], x0[2]);
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
       PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
       tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
     }
-    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+    if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }
-  finish_cfl(a, cmax, rmin);
+  finish_cfl<true>(a, cmax, rmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -1009,10 +1025,11 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
+  const double dt_ctx = a.sc->dt;
   const double alpha = a.flip;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
-    const double dt = tile_dt(a, t);
+    const double dt = tile_dt(a, t, dt_ctx);
     const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
@@ -1064,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
       STB(MASS, j, LDA(MASS, slot));
       // next step's cell: of the advanced x (ULMPM) or of x0 (TLMPM)
       const double *xk = TL ? xw : x;
-      a.key[j] = particle_key(a, xk[0], xk[1], xk[2], pid);
+      a.key[j] = particle_key(a, xk[0], xk[1], xk[2]);
     }
   }
 }
@@ -1082,10 +1099,11 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
+  const double dt_ctx = a.sc->dt;
   double cmax = 0, rmin = 1.0;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
-    const double dt = tile_dt(a, t);
+    const double dt = tile_dt(a, t, dt_ctx);
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
@@ -1148,9 +1166,9 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
       STB(DINIT, j, dinit);
       STB(DMG, j, dmg);
     }
-    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+    if (a.ninst) flush_tile_cfl<false>(a, t, cmax, rmin);
   }
-  finish_cfl(a, cmax, rmin);
+  finish_cfl<false>(a, cmax, rmin);
 }
 
 // TLMPM MUSL part B: Fdot from the re-mapped v with the x0 weights
@@ -1166,10 +1184,11 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
+  const double dt_ctx = a.sc->dt;
   double cmax = 0, rmin = 1.0;
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
-    const double dt = tile_dt(a, t);
+    const double dt = tile_dt(a, t, dt_ctx);
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
@@ -1199,9 +1218,9 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
       for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
       tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
     }
-    if (a.ninst) flush_tile_cfl(a, t, cmax, rmin);
+    if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }
-  finish_cfl(a, cmax, rmin);
+  finish_cfl<true>(a, cmax, rmin);
 }
 
 // ---------------------------------------------------------------------------

@@ -28,8 +28,7 @@ __global__ void k_keys(KArgs a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
   const int f = wpos(a);
-  a.key[i] = particle_key(a, PF(a.pa, f, i), PF(a.pa, f + 1, i), PF(a.pa, f + 2, i),
-                          IFLD(a.ia, PID, i));
+  a.key[i] = particle_key(a, PF(a.pa, f, i), PF(a.pa, f + 1, i), PF(a.pa, f + 2, i));
 }
 
 __global__ void k_sort_begin(KArgs a, int64_t ncells) {
@@ -64,6 +63,12 @@ __global__ void k_count(KArgs a) {
   if (head) base = atomicAdd(&a.cell_count[key], end - lane);
   base = __shfl(base, start, 64);
   if (valid) a.rank[i] = base + (lane - start);
+  // batched contexts: the particle's stencil must stay in its instance
+  if (valid && a.ninst) {
+    int c[3];
+    key_cell(a, key, c);
+    check_instance(a, c, key / a.tg.cpt, IFLD(a.ia, PID, i));
+  }
 }
 
 __global__ void k_scatter(KArgs a) {

@@ -46,12 +46,20 @@ __device__ __noinline__ void jc_damage_fallback(const DevMat &m, double &damage_
   jc_damage(m, damage_init, damage, pH, sdev, epsdot, dps);
 }
 
-// Counts one wave's network evaluations and fallbacks.
-__device__ inline void surrogate_count(DevScalars *sc, bool eval, bool fallback) {
-  const unsigned long long ne = __ballot(eval), nf = __ballot(fallback);
+// A wave's count of network evaluations and fallbacks, kept in registers
+// (every lane holds the wave's totals) and added to the context's counters
+// once per wave at the end of the kernel.
+struct SurrCount {
+  unsigned long long ev, fb;
+};
+__device__ inline void surrogate_count(SurrCount &c, bool eval, bool fallback) {
+  c.ev += (unsigned long long)__popcll(__ballot(eval));
+  c.fb += (unsigned long long)__popcll(__ballot(fallback));
+}
+__device__ inline void surrogate_count_flush(const SurrCount &c, DevScalars *sc) {
   if ((threadIdx.x & 63) == 0) {
-    if (ne) atomicAdd(&sc->surr_evals, (unsigned long long)__popcll(ne));
-    if (nf) atomicAdd(&sc->surr_fallbacks, (unsigned long long)__popcll(nf));
+    if (c.ev) atomicAdd(&sc->surr_evals, c.ev);
+    if (c.fb) atomicAdd(&sc->surr_fallbacks, c.fb);
   }
 }
 
@@ -62,7 +70,7 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
                                                   bool live, double dt, double h, double J,
                                                   double rho, const double *D, double *sig,
                                                   double *eel, double &ep, double &epdot,
-                                                  double &dinit, double &dmg, DevScalars *sc) {
+                                                  double &dinit, double &dmg, SurrCount &cnt) {
   const double pH = shock_pressure(m, J, rho, dmg, D, h);
   // ---- strength (strength_jc.cpp:100-184) -------------------------------
   const double G_ = m.strength_p[0], A = m.strength_p[1], epsdot0 = m.strength_p[4];
@@ -137,7 +145,7 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
       }
     }
   }
-  surrogate_count(sc, use_y || use_f, fb_y || fb_f);
+  surrogate_count(cnt, use_y || use_f, fb_y || fb_f);
   // ---- assembly (solid.cpp:1353-1366) -----------------------------------
   const double trD = D[0] + D[1] + D[2];
   const double trE = eel[0] + eel[1] + eel[2];
