@@ -127,7 +127,10 @@ static void join_mixed(const std::vector<double> &dd, const std::vector<float> &
 //   force     vol_ref * sigma_ref / h,   vol_ref = 4 vol_max,
 //             sigma_ref = 4 max(|sigma|_max, max (K + 4/3 G))
 // (|grad w| <= 1/h for both shape families).
-static void set_fixed_point_scales(kmpm_ctx *c, const std::vector<double> &hd, int cap,
+// Power-of-two scales of the fixed-point node sums: one particle's
+// contribution stays below 2^FX_CONTRIB_BITS.  The P2G kernels scale in FP32,
+// so a scale must lie inside the float range (false otherwise).
+static bool set_fixed_point_scales(kmpm_ctx *c, const std::vector<double> &hd, int cap,
                                    int64_t np) {
   double mmax = 0, vmax = 0, smax = 0, volmax = 0;
   for (int64_t i = 0; i < np; i++) {
@@ -148,9 +151,11 @@ static void set_fixed_point_scales(kmpm_ctx *c, const std::vector<double> &hd, i
   for (int k = 0; k < FX_N; k++) {
     int e = 0;
     std::frexp(bound[k] > 0 && std::isfinite(bound[k]) ? bound[k] : 1.0, &e);   // bound < 2^e
+    if (FX_CONTRIB_BITS - e > 120 || FX_CONTRIB_BITS - e < -120) return false;
     c->a.fxs[k] = std::ldexp(1.0, FX_CONTRIB_BITS - e);
     c->a.fxi[k] = std::ldexp(1.0, e - FX_CONTRIB_BITS);
   }
+  return true;
 }
 
 // Device record of a material (Mat constructors, material.cpp:720-769,
@@ -519,7 +524,7 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
     split_mixed(hd, cap, dd, ff);
     HIPCHK(hipMemcpy(c->d_pa, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_fa, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice));
-    set_fixed_point_scales(c, hd, cap, np);
+    if (!set_fixed_point_scales(c, hd, cap, np)) return KMPM_ERR_FIXED_POINT;
   } else {
     HIPCHK(hipMemcpy(c->d_pa, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice));
   }

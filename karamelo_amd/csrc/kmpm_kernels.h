@@ -62,14 +62,6 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Scaled flush of a P2G register sum into a 64-bit fixed-point LDS node sum.
-__device__ inline void fx_add(unsigned long long *dst, double val, double scale, DevScalars *sc) {
-  const double q = val * scale;
-  if (!(fabs(q) < FX_FLUSH_LIMIT)) set_error(sc, KMPM_ERR_FIXED_POINT, -1);
-  const long long iv = fabs(q) < FX_FLUSH_LIMIT ? __double2ll_rn(q) : 0;
-  atomicAdd(dst, (unsigned long long)iv);
-}
-
 __device__ inline int cell_of_clamped(double x, double lo, double inv_h, int n) {
   double u = (x - lo) * inv_h;
   if (!(u >= 0.0)) return 0;   // negative or NaN
@@ -150,7 +142,7 @@ __device__ inline void key_cell(const KArgs &a, int key, int c[3]) {
 __device__ inline int wpos(const KArgs &a) { return a.tl ? X0X : PX; }
 
 // ---------------------------------------------------------------------------
-// P2G: lanes = stencil nodes, register accumulation per cell, LDS tile block
+// P2G: staged particles, lane = stencil column, register sums per cell
 // ---------------------------------------------------------------------------
 constexpr int SCR_FIELDS = 7;   // scratch slot layout: mass, momentum[3], force[3]
 
@@ -209,51 +201,92 @@ __device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R> 
   }
 }
 
-template <int DIM, int S1, int TX, int TY, int TZ, int CH, bool REMAP, int PREC, bool TL>
+// Stress term S_de of a staged P2G payload: -vol sigma (symmetric, stored
+// xx,yy,zz,xy,xz,yz in 3D and xx,yy,xy in 2D) or, TLMPM, -vol0 PK1 (row-major).
+template <int DIM, bool TL, class R>
+__device__ inline R p2g_stress(const R *s, int d, int e) {
+  if (TL) return s[d * DIM + e];
+  if (DIM == 3) {
+    if (d == e) return s[d];
+    const int o = d + e;   // (0,1) -> xy, (0,2) -> xz, (1,2) -> yz
+    return s[o == 1 ? 3 : o == 2 ? 4 : 5];
+  }
+  return d == e ? s[d] : s[2];
+}
+
+// Scaled flush of a P2G register sum into a 64-bit fixed-point LDS node sum,
+// in FP32: q = val * scale is exact (a power-of-two scale within the float
+// range), rounded to the nearest integer (ties to even, as in FP64) and
+// split into 32-bit words.
+__device__ inline void fx_add_f(unsigned long long *dst, float val, float scale,
+                                DevScalars *sc) {
+  const float q = rintf(val * scale);
+  const float aq = fabsf(q);
+  const bool ok = aq < (float)FX_FLUSH_LIMIT;
+  if (!ok) set_error(sc, KMPM_ERR_FIXED_POINT, -1);
+  // |q| = hi 2^32 + lo exactly: lo is a multiple of ulp(q) below 2^32
+  const float aqc = ok ? aq : 0.0f;
+  const float hi = floorf(aqc * 2.3283064365386963e-10f);   // |q| / 2^32, below 2^22
+  const float lo = fmaf(-hi, 4294967296.0f, aqc);            // in [0, 2^32)
+  const unsigned long long mag =
+      ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
+  if (ok) atomicAdd(dst, q < 0 ? 0ull - mag : mag);
+}
+
+// P2G over the active tiles, one workgroup per tile at a time.
+//
+// The tile's particles are staged in chunks of CHK (sorted by cell): one
+// thread per particle evaluates its shape functions (FP64, the reference's
+// expressions) and stores the per-axis (w, dw) pairs and its payload in LDS.
+// Each run of particles of one cell (a segment) goes to a lane group: a
+// lane owns one stencil column (the nodes that share all but the last axis
+// offset) and accumulates its S1 nodes of that column in registers over the
+// segment, so a node sum costs one multiply-add per field and particle; the
+// group adds its sums to the tile's LDS node block once per segment.  The
+// block is then written to the tile's partial buffer (no global atomics).
+//
+// FP64: double arithmetic and LDS sums.  Mixed: float arithmetic, register
+// sums flushed into 64-bit fixed-point LDS sums (order-independent, so the
+// scatter is deterministic).
+template <int DIM, int S1, int TX, int TY, int TZ, bool REMAP, int PREC, bool TL>
 __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
-  // FP64: double transfer arithmetic, double LDS node sums.  Mixed: float
-  // transfer arithmetic, register sums flushed into 64-bit fixed-point LDS
-  // node sums (order-independent, so the scatter is deterministic).
   using R = typename Real<PREC>::T;
   using R2 = typename Real<PREC>::T2;
   constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
   using BT = typename std::conditional<FX, unsigned long long, double>::type;
-  constexpr int S = DIM == 3 ? S1 * S1 * S1 : S1 * S1;
-  constexpr int G = 64 / S;
-  constexpr int RUN = CH / G;
+  constexpr int GL = DIM == 3 ? S1 * S1 : S1;   // lanes (stencil columns) per group
+  constexpr int NGW = 64 / GL;                   // groups per wave
+  constexpr int NG = 4 * NGW;                    // groups per block
+  constexpr int CHK = 128;                       // particles staged per chunk
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
-  constexpr int BN = EX * EY * EZ;
-  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   constexpr int NSIG = p2g_nsig<DIM, TL>();
-  constexpr int NW = DIM * S1;                              // (w, dw) pairs per particle
   constexpr int NF = REMAP ? (1 + DIM) : (1 + 2 * DIM);
-  constexpr int NP2 = 2 + (REMAP ? 0 : (NSIG + 1) / 2);    // payload pairs: (m,v0),(v1,v2),sigma
-  static_assert(64 % S == 0 && CH % G == 0 && CH <= 64, "stencil/chunk shape");
-  // LDS node block strides.  For the 3D cubic 4x4x4 tile the rows are padded
-  // (z stride 8, x stride 58 doubles) so the 64 lanes' atomic adds of one
-  // cell hit every bank at most twice, the minimum for 64 doubles.
-  constexpr bool PAD = DIM == 3 && S1 == 4 && TX == 4 && TY == 4 && TZ == 4;
-  constexpr int PSB = PAD ? 8 : EZ;
-  constexpr int PSA = PAD ? 58 : EY * EZ;
+  constexpr int NPAY = 1 + DIM + (REMAP ? 0 : NSIG);      // m, m v, stress terms
+  constexpr int PSB = DIM == 3 ? EZ : 1;   // node block strides (last axis contiguous)
+  constexpr int PSA = DIM == 3 ? EY * EZ : EY;
   constexpr int PBN = EX * PSA;
-  static_assert(PSA >= EY * PSB, "block stride");
+  static_assert(64 % GL == 0, "stencil shape");
 
   __shared__ BT blk[NF * PBN];
-  __shared__ R2 stw[4][NW][CH + 1];   // +1: the S1 rows a lane group reads fall in distinct banks
-  __shared__ R2 spay[4][NP2][CH];
-  __shared__ int stg_cell[4][CH];
+  __shared__ R2 stw[CHK][DIM * S1];     // (w, dw) per axis and stencil offset
+  __shared__ R spay[CHK][NPAY + 1];     // +1: rows of distinct particles in distinct banks
+  __shared__ int scl[CHK];              // block offset of the particle's cell
+  __shared__ int seg[CHK + 1];          // segment starts; seg[nseg] = chunk size
+  __shared__ int wheads[4];
+  __shared__ int nseg_s;
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
 
-  // lane -> stencil node offsets
-  const int g = lane / S, s = lane % S;
-  const int sa = DIM == 3 ? s / (S1 * S1) : s / S1;
-  const int sb = DIM == 3 ? (s / S1) % S1 : s % S1;
-  const int sc = DIM == 3 ? s % S1 : 0;
-  const int sofs = sa * PSA + sb * PSB + sc;
+  // lane -> group and stencil column
+  const int gl = lane % GL, q = wave * NGW + lane / GL;
+  const int sa = DIM == 3 ? gl / S1 : gl;
+  const int sb = DIM == 3 ? gl % S1 : 0;
+  const int colofs = sa * PSA + sb * PSB;
+  R fs[FX_N];
+  for (int k = 0; k < FX_N; k++) fs[k] = (R)a.fxs[k];
 
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
@@ -261,20 +294,17 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
     const int ty = (t / nt2) % nt1;
     const int tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
+    constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
     const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
-    for (int q = tid; q < NF * PBN; q += 256) blk[q] = 0;
-    __syncthreads();
+    for (int i = tid; i < NF * PBN; i += 256) blk[i] = 0;
 
-    const int npart = p1 - p0;
-    // The particle data of the wave's next chunk is loaded into registers
-    // while the current chunk is accumulated (hides HBM latency).
-    P2GIn<DIM, REMAP, TL, R> in;
-    int c0 = wave * CH;
-    if (c0 < npart && lane < min(CH, npart - c0))
-      p2g_load<DIM, REMAP, TL, PREC>(a, p0 + c0 + lane, in);
-    for (; c0 < npart; c0 += 4 * CH) {
-      const int cnt = min(CH, npart - c0);
-      if (lane < cnt) {
+    for (int c0 = p0; c0 < p1; c0 += CHK) {
+      const int n = min(CHK, p1 - c0);
+      __syncthreads();   // the previous chunk's segments are done
+      // ---- stage: one thread per particle ----
+      if (tid < n) {
+        P2GIn<DIM, REMAP, TL, R> in;
+        p2g_load<DIM, REMAP, TL, PREC>(a, c0 + tid, in);
         int cl[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
@@ -283,141 +313,110 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
           double w[S1], dw[S1];
           weights_1d<S1>(in.x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
 #pragma unroll
-          for (int q = 0; q < S1; q++) stw[wave][d * S1 + q][lane] = R2{(R)w[q], (R)dw[q]};
+          for (int k = 0; k < S1; k++) stw[tid][d * S1 + k] = R2{(R)w[k], (R)dw[k]};
         }
-        stg_cell[wave][lane] = cl[0] * PSA + cl[1] * PSB + cl[2];
-        spay[wave][0][lane] = R2{in.m, in.v[0]};
-        spay[wave][1][lane] = R2{in.v[1], DIM == 3 ? in.v[DIM - 1] : (R)0};
-        if (!REMAP) {
+        const int cell = DIM == 3 ? cl[0] * PSA + cl[1] * PSB + cl[2] : cl[0] * PSA + cl[1];
+        scl[tid] = cell;
+        spay[tid][0] = in.m;
+        for (int d = 0; d < DIM; d++) spay[tid][1 + d] = in.m * in.v[d];
+        if (!REMAP)
+          for (int e = 0; e < NSIG; e++) spay[tid][1 + DIM + e] = in.s[e];
+      }
+      __syncthreads();
+      // cells are sorted inside the tile: a segment starts where the cell changes
+      const bool head = tid < n && (tid == 0 || scl[tid] != scl[tid - 1]);
+      // ---- segment starts: ballot per wave, prefix over the waves ----
+      const unsigned long long hb = __ballot(head);
+      if (lane == 0) wheads[wave] = __popcll(hb);
+      __syncthreads();
+      int before = 0;
+      for (int w = 0; w < wave; w++) before += wheads[w];
+      if (head) seg[before + __popcll(hb & ((1ull << lane) - 1ull))] = tid;
+      if (tid == 0) {
+        const int tot = wheads[0] + wheads[1] + wheads[2] + wheads[3];
+        nseg_s = tot;
+        seg[tot] = n;
+      }
+      __syncthreads();
+      const int nseg = nseg_s;
+      // ---- accumulate: group q takes segments q, q + NG, ... ----
+      for (int s = q; s < nseg; s += NG) {
+        const int i0 = seg[s], i1 = seg[s + 1];
+        R acc[S1][NF];
 #pragma unroll
-          for (int e = 0; e < NSIG; e += 2)
-            spay[wave][2 + e / 2][lane] = R2{in.s[e], e + 1 < NSIG ? in.s[e + 1] : (R)0};
-        }
-      }
-      wave_sync();
-      const int cn = c0 + 4 * CH;
-      if (cn < npart && lane < min(CH, npart - cn))
-        p2g_load<DIM, REMAP, TL, PREC>(a, p0 + cn + lane, in);
-
-      // Accumulate the group's run of particles; the LDS operands of the
-      // next particle are loaded before the arithmetic of the current one
-      // (two operand sets used alternately, no register copies).
-      const int jb = g * RUN, je = min(jb + RUN, cnt);
-      if (jb < je) {
-        constexpr int NS2 = REMAP ? 0 : (NSIG + 1) / 2;
-        struct Ops {
-          int cell;
-          R2 px, py, pz, mv0, v12;
-          R2 sg[NS2 > 0 ? NS2 : 1];
-        };
-        auto load = [&](int j, Ops &o) {
-          o.cell = stg_cell[wave][j];
-          o.px = stw[wave][sa][j];
-          o.py = stw[wave][S1 + sb][j];
-          o.pz = DIM == 3 ? stw[wave][2 * S1 + sc][j] : R2{(R)1, (R)0};
-          o.mv0 = spay[wave][0][j];
-          o.v12 = spay[wave][1][j];
-          for (int e = 0; e < NS2; e++) o.sg[e] = spay[wave][2 + e][j];
-        };
-        R acc[NF];
-        for (int f = 0; f < NF; f++) acc[f] = 0;
-        auto compute = [&](const Ops &o) {
-          R w, g0, g1, g2 = 0;
-          if (DIM == 3) {
-            const R wxy = o.px.x * o.py.x;
-            w = wxy * o.pz.x;
-            g0 = o.px.y * o.py.x * o.pz.x;
-            g1 = o.px.x * o.py.y * o.pz.x;
-            g2 = wxy * o.pz.y;
+        for (int k = 0; k < S1; k++)
+#pragma unroll
+          for (int f = 0; f < NF; f++) acc[k][f] = 0;
+        for (int i = i0; i < i1; i++) {
+          // column factors: c = product of the non-last-axis weights, ce[e]
+          // the same with axis e differentiated
+          R c, ce[DIM - 1];
+          if constexpr (DIM == 3) {
+            const R2 px = stw[i][sa], py = stw[i][S1 + sb];
+            c = px.x * py.x;
+            ce[0] = px.y * py.x;
+            ce[DIM - 2] = px.x * py.y;
           } else {
-            w = o.px.x * o.py.x;
-            g0 = o.px.y * o.py.x;
-            g1 = o.px.x * o.py.y;
+            const R2 px = stw[i][sa];
+            c = px.x;
+            ce[0] = px.y;
           }
-          const R wm = w * o.mv0.x;
-          acc[0] += wm;
-          acc[1] = fma(wm, o.mv0.y, acc[1]);
-          acc[2] = fma(wm, o.v12.x, acc[2]);
-          if (DIM == 3) acc[3] = fma(wm, o.v12.y, acc[3]);
-          if (!REMAP && TL) {
-            // f_d += sum_e (-vol0 PK1)_de g_e, row-major pairs
-            constexpr int K = NS2 > 0 ? NS2 : 1;
-            if (DIM == 3) {
-              const R q00 = o.sg[0].x, q01 = o.sg[0].y, q02 = o.sg[1 % K].x, q10 = o.sg[1 % K].y,
-                      q11 = o.sg[2 % K].x, q12 = o.sg[2 % K].y, q20 = o.sg[3 % K].x,
-                      q21 = o.sg[3 % K].y, q22 = o.sg[4 % K].x;
-              acc[4] = fma(q02, g2, fma(q01, g1, fma(q00, g0, acc[4])));
-              acc[5] = fma(q12, g2, fma(q11, g1, fma(q10, g0, acc[5])));
-              acc[6] = fma(q22, g2, fma(q21, g1, fma(q20, g0, acc[6])));
-            } else {
-              const R q00 = o.sg[0].x, q01 = o.sg[0].y, q10 = o.sg[1 % K].x, q11 = o.sg[1 % K].y;
-              acc[3] = fma(q01, g1, fma(q00, g0, acc[3]));
-              acc[4] = fma(q11, g1, fma(q10, g0, acc[4]));
-            }
-          } else if (!REMAP) {
-            constexpr int K = NS2 > 0 ? NS2 : 1;
-            if (DIM == 3) {
-              // f_d += sum_e (-vol sigma)_de g_e ; sym order xx,yy,zz,xy,xz,yz
-              const R sxx = o.sg[0].x, syy = o.sg[0].y, szz = o.sg[1 % K].x,
-                      sxy = o.sg[1 % K].y, sxz = o.sg[2 % K].x, syz = o.sg[2 % K].y;
-              acc[4] = fma(sxz, g2, fma(sxy, g1, fma(sxx, g0, acc[4])));
-              acc[5] = fma(syz, g2, fma(syy, g1, fma(sxy, g0, acc[5])));
-              acc[6] = fma(szz, g2, fma(syz, g1, fma(sxz, g0, acc[6])));
-            } else {
-              const R sxx = o.sg[0].x, syy = o.sg[0].y, sxy = o.sg[1 % K].x;
-              acc[3] = fma(sxy, g1, fma(sxx, g0, acc[3]));
-              acc[4] = fma(syy, g1, fma(sxy, g0, acc[4]));
+          const R *p = spay[i];
+          const R mc = p[0] * c;
+          R mvc[DIM], fa[DIM], fb[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; d++) mvc[d] = p[1 + d] * c;
+          if (!REMAP) {
+            const R *sg = p + 1 + DIM;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+              R u = p2g_stress<DIM, TL>(sg, d, 0) * ce[0];
+              if constexpr (DIM == 3) u = fma(p2g_stress<DIM, TL>(sg, d, 1), ce[DIM - 2], u);
+              fa[d] = u;
+              fb[d] = p2g_stress<DIM, TL>(sg, d, DIM - 1) * c;
             }
           }
-          (void)g2;
-        };
-        auto flush = [&](int cell) {
-          for (int f = 0; f < NF; f++) {
-            if constexpr (FX) {
-              const int k = f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE;
-              fx_add(&blk[f * PBN + cell + sofs], (double)acc[f], a.fxs[k], a.sc);
-            } else {
-              atomicAdd(&blk[f * PBN + cell + sofs], acc[f]);
-            }
-            acc[f] = 0;
-          }
-        };
-        Ops A, B;
-        load(jb, A);
-        int cur = A.cell;
-        for (int j = jb;; j += 2) {
-          load(j + 1 < je ? j + 1 : j, B);
-          compute(A);
-          if (j + 1 >= je) break;
-          if (B.cell != cur) {
-            flush(cur);
-            cur = B.cell;
-          }
-          load(j + 2 < je ? j + 2 : j + 1, A);
-          compute(B);
-          if (j + 2 >= je) break;
-          if (A.cell != cur) {
-            flush(cur);
-            cur = A.cell;
+#pragma unroll
+          for (int k = 0; k < S1; k++) {
+            const R2 wz = stw[i][(DIM - 1) * S1 + k];
+            acc[k][0] = fma(mc, wz.x, acc[k][0]);
+#pragma unroll
+            for (int d = 0; d < DIM; d++) acc[k][1 + d] = fma(mvc[d], wz.x, acc[k][1 + d]);
+            if (!REMAP)
+#pragma unroll
+              for (int d = 0; d < DIM; d++)
+                acc[k][1 + DIM + d] = fma(fb[d], wz.y, fma(fa[d], wz.x, acc[k][1 + DIM + d]));
           }
         }
-        flush(cur);
+        const int base = scl[i0] + colofs;
+#pragma unroll
+        for (int k = 0; k < S1; k++)
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            BT *dst = &blk[f * PBN + base + k];
+            if constexpr (FX) {
+              const int kk = f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE;
+              fx_add_f(dst, acc[k][f], fs[kk], a.sc);
+            } else {
+              atomicAdd(dst, acc[k][f]);
+            }
+          }
       }
-      wave_sync();
     }
     __syncthreads();
+    constexpr int BN = EX * EY * EZ;
     BT *dst = reinterpret_cast<BT *>(a.scratch) + (size_t)li * SCR_FIELDS * BN;
     // blk fields: mass, mom[DIM], force[DIM] -> scratch fields mass, mom[3],
     // force[3]; components a 2D run does not have are written as zeros.
     const int f_lo = REMAP ? 1 : 0, f_hi = REMAP ? 4 : SCR_FIELDS;
-    for (int q = tid + f_lo * BN; q < f_hi * BN; q += 256) {
-      const int sf = q / BN, o = q % BN;
+    for (int qq = tid + f_lo * BN; qq < f_hi * BN; qq += 256) {
+      const int sf = qq / BN, o = qq % BN;
       int bf;
       if (sf == 0) bf = 0;
       else if (sf <= 3) bf = sf <= DIM ? sf : -1;
       else bf = (sf - 3) <= DIM ? DIM + (sf - 3) : -1;
       const int bz = o % EZ, by = (o / EZ) % EY, bx = o / (EZ * EY);
-      dst[q] = bf >= 0 ? blk[bf * PBN + bx * PSA + by * PSB + bz] : (BT)0;
+      dst[qq] = bf >= 0 ? blk[bf * PBN + bx * PSA + by * PSB + bz] : (BT)0;
     }
     __syncthreads();
   }

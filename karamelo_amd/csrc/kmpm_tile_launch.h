@@ -15,19 +15,19 @@ hipError_t TileLaunch<DIM, S1, V>::p2g(kmpm_ctx *c, int remap) {
   if (c->a.tl) {   // TLMPM: FP64 only
     constexpr int PREC = KMPM_PRECISION_FP64;
     if (remap)
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC, true>),
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, true, PREC, true>),
                          dim3(g), dim3(256), 0, c->stream, c->a);
     else
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC, true>),
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, false, PREC, true>),
                          dim3(g), dim3(256), 0, c->stream, c->a);
     return hipGetLastError();
   }
   PREC_DISPATCH(c, {
     if (remap)
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, true, PREC, false>),
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, true, PREC, false>),
                          dim3(g), dim3(256), 0, c->stream, c->a);
     else
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, TT::CH, false, PREC, false>),
+      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, false, PREC, false>),
                          dim3(g), dim3(256), 0, c->stream, c->a);
   });
   return hipGetLastError();
