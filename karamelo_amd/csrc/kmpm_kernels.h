@@ -638,55 +638,78 @@ __device__ inline void load_node_block(const KArgs &a, typename Real<PREC>::T *n
 }
 
 // Gather over the stencil: vu, acc (MODE 0 / A), L (USL / B).
+// The weights are products of per-axis factors, so the sums are contracted
+// one axis at a time (last axis first): per node only the multiply-adds of
+// the innermost contraction remain, and the weight products are never
+// formed per node.
+//   vu_d  = sum_abc wx_a wy_b wz_c V_d          acc_d = sum wx wy wz A_d
+//   L_d0  = sum dwx wy wz V_d   L_d1 = sum wx dwy wz V_d   L_d2 = sum wx wy dwz V_d
 template <int DIM, int S1, int EY, int EZ, int BN, bool WANT_VA, bool WANT_L, bool L_FROM_V,
           class R>
 __device__ inline void gather_stencil(const R *nb, const int *cl, const R (*w)[S1],
                                       const R (*dw)[S1], R *vu, R *acc, R *L) {
+  (void)L_FROM_V;
 #pragma unroll
   for (int ia = 0; ia < S1; ia++) {
+    if (WANT_VA)
+      for (int d = 0; d < DIM; d++) asm volatile("" : "+v"(vu[d]), "+v"(acc[d]));
+    if (WANT_L)
+      for (int d = 0; d < 9; d++) asm volatile("" : "+v"(L[d]));
+    // contractions over the axes after the first, for this x offset
+    R uw[DIM], ua[DIM], ud1[DIM], ud2[DIM];   // sum wy wz V, wy wz A, dwy wz V, wy dwz V
+#pragma unroll
+    for (int d = 0; d < DIM; d++) uw[d] = ua[d] = ud1[d] = ud2[d] = 0;
 #pragma unroll
     for (int ib = 0; ib < S1; ib++) {
-      R wxy = w[0][ia] * w[1][ib];
-      R dxy = dw[0][ia] * w[1][ib];
-      R xdy = w[0][ia] * dw[1][ib];
-      // Opaque barrier: the previous row's accumulations complete here, and
-      // neither this row's LDS loads nor its weight products move above it.
-      // Without it the scheduler defers whole FMA chains and spills.
-      if (WANT_VA)
-        for (int d = 0; d < DIM; d++) asm volatile("" : "+v"(vu[d]), "+v"(acc[d]));
-      if (WANT_L)
-        for (int d = 0; d < 9; d++) asm volatile("" : "+v"(L[d]));
-      asm volatile("" : "+v"(wxy), "+v"(dxy), "+v"(xdy)::"memory");
+      // Opaque barrier: the previous row's accumulations complete here and
+      // this row's LDS loads do not move above it (keeps registers bounded).
 #pragma unroll
-      for (int ic = 0; ic < (DIM == 3 ? S1 : 1); ic++) {
-        R wt, g0, g1, g2 = 0;
-        int idx;
-        if (DIM == 3) {
-          wt = wxy * w[2][ic];
-          g0 = dxy * w[2][ic];
-          g1 = xdy * w[2][ic];
-          g2 = wxy * dw[2][ic];
-          idx = ((cl[0] + ia) * EY + (cl[1] + ib)) * EZ + (cl[2] + ic);
-        } else {
-          wt = wxy;
-          g0 = dxy;
-          g1 = xdy;
-          idx = (cl[0] + ia) * EY + (cl[1] + ib);
-        }
-        R vn[DIM];
-        for (int d = 0; d < DIM; d++) vn[d] = nb[d * BN + idx];
-        if (WANT_VA) {
+      for (int d = 0; d < DIM; d++) asm volatile("" : "+v"(uw[d]), "+v"(ua[d]), "+v"(ud1[d]), "+v"(ud2[d])::"memory");
+      if (DIM == 3) {
+        const int row = ((cl[0] + ia) * EY + (cl[1] + ib)) * EZ + cl[2];
+        R tw[DIM], ta[DIM], tz[DIM];   // sum_c wz V, wz A, dwz V
+#pragma unroll
+        for (int d = 0; d < DIM; d++) tw[d] = ta[d] = tz[d] = 0;
+#pragma unroll
+        for (int ic = 0; ic < S1; ic++) {
+#pragma unroll
           for (int d = 0; d < DIM; d++) {
-            vu[d] = fma(wt, vn[d], vu[d]);
-            acc[d] = fma(wt, nb[(DIM + d) * BN + idx], acc[d]);
+            const R vn = nb[d * BN + row + ic];
+            tw[d] = fma(w[2][ic], vn, tw[d]);
+            if (WANT_L) tz[d] = fma(dw[2][ic], vn, tz[d]);
+            if (WANT_VA) ta[d] = fma(w[2][ic], nb[(DIM + d) * BN + row + ic], ta[d]);
           }
         }
-        if (WANT_L) {
-          const R gg[3] = {g0, g1, g2};
-          for (int d = 0; d < DIM; d++)
-            for (int e = 0; e < DIM; e++) L[3 * d + e] = fma(vn[d], gg[e], L[3 * d + e]);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+          uw[d] = fma(w[1][ib], tw[d], uw[d]);
+          if (WANT_VA) ua[d] = fma(w[1][ib], ta[d], ua[d]);
+          if (WANT_L) {
+            ud1[d] = fma(dw[1][ib], tw[d], ud1[d]);
+            ud2[d] = fma(w[1][ib], tz[d], ud2[d]);
+          }
         }
-        (void)L_FROM_V;
+      } else {
+        const int idx = (cl[0] + ia) * EY + (cl[1] + ib);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+          const R vn = nb[d * BN + idx];
+          uw[d] = fma(w[1][ib], vn, uw[d]);
+          if (WANT_L) ud1[d] = fma(dw[1][ib], vn, ud1[d]);
+          if (WANT_VA) ua[d] = fma(w[1][ib], nb[(DIM + d) * BN + idx], ua[d]);
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      if (WANT_VA) {
+        vu[d] = fma(w[0][ia], uw[d], vu[d]);
+        acc[d] = fma(w[0][ia], ua[d], acc[d]);
+      }
+      if (WANT_L) {
+        L[3 * d + 0] = fma(dw[0][ia], uw[d], L[3 * d + 0]);
+        L[3 * d + 1] = fma(w[0][ia], ud1[d], L[3 * d + 1]);
+        if (DIM == 3) L[3 * d + 2] = fma(w[0][ia], ud2[d], L[3 * d + 2]);
       }
     }
   }
