@@ -1,5 +1,5 @@
 // kmpm_mlp.h -- the learned constitutive surrogate: a 2 -> 32 -> 32 -> 1
-// tanh MLP whose hidden 32x32 layer runs on the gfx950 BF16 matrix cores
+// ISRU-activated MLP whose hidden 32x32 layer runs on the gfx950 BF16 matrix cores
 // (v_mfma_f32_32x32x16_bf16, FP32 accumulation).  Layers 1 and 3 are FP32
 // vector arithmetic.
 //
@@ -62,11 +62,16 @@ __device__ inline __bf16 bf16_from_bits(uint16_t b) {
   return __builtin_bit_cast(__bf16, b);
 }
 
-// tanh from the hardware 2^x and reciprocal: 1 - 2 / (1 + e^{2x}), a few
-// ulp, saturating to +-1.
-__device__ inline float tanh_fast(float x) {
-  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);   // 2 log2(e)
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+// The activation: the inverse square root unit x / sqrt(1 + x^2) (Carlile
+// et al. 2017, "ISRU"), odd, bounded by +-1 like tanh, one hardware
+// reciprocal square root where tanh needs an exponential and a reciprocal.
+__device__ inline float mlp_act(float x) {
+  return x * __builtin_amdgcn_rsqf(fmaf(x, x, 1.0f));
+}
+// Its derivative from the activation h: (1 + x^2)^(-3/2) = (1 - h^2)^(3/2).
+__device__ inline float mlp_act_grad(float h) {
+  const float t = fmaf(-h, h, 1.0f);
+  return t * __builtin_amdgcn_sqrtf(t);
 }
 
 // The A operand (W2) of this lane for k-steps 0 and 1.
@@ -87,7 +92,7 @@ __device__ inline void mlp_layer1(const MlpParams &p, float x0, float x1, float 
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const int k = (i < 8 ? 0 : 16) + 8 * half + (i & 7);
-    h[i] = tanh_fast(fmaf(p.w1[k][1], x1, fmaf(p.w1[k][0], x0, p.b1[k])));
+    h[i] = mlp_act(fmaf(p.w1[k][1], x1, fmaf(p.w1[k][0], x0, p.b1[k])));
     if (i < 8) b0[i] = to_bf16(h[i]);
     else b1[i - 8] = to_bf16(h[i]);
   }
@@ -125,7 +130,7 @@ __device__ inline float mlp_eval_wave(const MlpNet &n, float f0, float f1) {
       for (int i = 0; i < 8; i++) {
         const int k = 16 * st + 8 * half + i;
         a[i] = bf16_from_bits(n.w2_bf16[col][k]);
-        b[i] = to_bf16(tanh_fast(fmaf(n.p.w1[k][1], g1, fmaf(n.p.w1[k][0], g0, n.p.b1[k]))));
+        b[i] = to_bf16(mlp_act(fmaf(n.p.w1[k][1], g1, fmaf(n.p.w1[k][0], g0, n.p.b1[k]))));
       }
       d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, d, 0, 0, 0);
     }
@@ -134,7 +139,7 @@ __device__ inline float mlp_eval_wave(const MlpNet &n, float f0, float f1) {
     for (int i = 0; i < 16; i++) {
       if ((i & 7) == 0) asm volatile("" ::: "memory");
       const int o = mlp_drow(i);
-      part = fmaf(n.p.w3[o], tanh_fast(d[i] + n.p.b2[o]), part);
+      part = fmaf(n.p.w3[o], mlp_act(d[i] + n.p.b2[o]), part);
     }
     part += __shfl_xor(part, 32, 64);
     if (T == half) own = part + n.p.b3;

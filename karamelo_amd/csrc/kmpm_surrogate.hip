@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const MlpNet *netp, const floa
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int o = mlp_drow(i);
-      h2[i] = tanh_fast(d[i] + P.b2[o]);
+      h2[i] = mlp_act(d[i] + P.b2[o]);
       part = fmaf(P.w3[o], h2[i], part);
     }
     part += __shfl_xor(part, 32, 64);
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const MlpNet *netp, const floa
     for (int i = 0; i < 16; i++) {
       const int o = mlp_drow(i);
       atomicAdd(&gacc[OW3 + o], g * h2[i]);
-      const float g2 = g * P.w3[o] * (1.0f - h2[i] * h2[i]);
+      const float g2 = g * P.w3[o] * mlp_act_grad(h2[i]);
       atomicAdd(&gacc[OB2 + o], g2);
       sg2[wave][o][col] = __builtin_bit_cast(uint16_t, to_bf16(g2));
     }
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const MlpNet *netp, const floa
       }
       dw2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ag, bh, dw2, 0, 0, 0);
     }
-    // G1[k][p] = (sum_o W2[o][k] G2[o][p]) (1 - H1[k][p]^2)
+    // G1[k][p] = (sum_o W2[o][k] G2[o][p]) act'(H1[k][p])
     f32x16_t g1a = {};
 #pragma unroll
     for (int st = 0; st < 2; st++) {
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const MlpNet *netp, const floa
     for (int i = 0; i < 16; i++) {
       const int k = mlp_drow(i);
       const float hv = sh1f[wave][k][col];
-      const float gk = g1a[i] * (1.0f - hv * hv);
+      const float gk = g1a[i] * mlp_act_grad(hv);
       atomicAdd(&gacc[OW1 + k * MLP_IN + 0], gk * g0);
       atomicAdd(&gacc[OW1 + k * MLP_IN + 1], gk * g1);
       atomicAdd(&gacc[OB1 + k], gk);

@@ -4,11 +4,12 @@ TEST INFRASTRUCTURE ONLY: the checker the device's BF16 matrix-core
 training and inference kernels (karamelo_amd/csrc/kmpm_mlp.h,
 kmpm_surrogate.hip) are compared against.  Nothing in the product imports it.
 
-The network is 2 -> 32 -> 32 -> 1 with tanh activations over a quadratic
+The network is 2 -> 32 -> 32 -> 1 with ISRU activations,
+act(u) = u / sqrt(1 + u^2) (act' = (1 - act^2)^(3/2)), over a quadratic
 baseline:
     x  = f * in_scale + in_shift                 (features -> inputs, FP32)
-    h1 = tanh(W1 x + b1)
-    h2 = tanh(W2 h1 + b2)
+    h1 = act(W1 x + b1)
+    h2 = act(W2 h1 + b2)
     z  = w3 . h2 + b3
     output = base(x) + out_shift + out_scale * z
     base(x) = c0 + x0 (c1 + c3 x0 + c4 x1) + x1 (c2 + c5 x1)   (FP32, this order)
@@ -28,6 +29,17 @@ import numpy as np
 
 H, NI = 32, 2
 NPARAM = H * NI + H + H * H + H + H + 1
+
+
+def act(u):
+    """The inverse square root unit u / sqrt(1 + u^2)."""
+    return u / np.sqrt(1.0 + u * u)
+
+
+def act_grad(h):
+    """d act / du from the activation h: (1 - h^2)^(3/2)."""
+    t = 1.0 - h * h
+    return t * np.sqrt(t)
 
 
 def unpack(p):
@@ -80,9 +92,9 @@ def forward(p, F, norm, bf16_round=True):
     """Raw outputs z for features F [n][2]; returns (z, cache)."""
     q = unpack(p)
     X = inputs(F, norm).astype(np.float64)
-    h1 = np.tanh(X @ q["w1"].T + q["b1"])
+    h1 = act(X @ q["w1"].T + q["b1"])
     z2 = _r(h1, bf16_round) @ _r(q["w2"], bf16_round).T + q["b2"]
-    h2 = np.tanh(z2)
+    h2 = act(z2)
     z = h2 @ q["w3"] + q["b3"]
     return z, (q, X, h1, h2)
 
@@ -103,10 +115,10 @@ def loss_and_grad(p, F, T, norm, bf16_round=True):
     loss = float(np.mean(r * r))
     g = 2.0 * r / n                                   # dL/dz
     grad = {"b3": g.sum(), "w3": h2.T @ g}
-    g2 = g[:, None] * q["w3"][None, :] * (1.0 - h2 * h2)
+    g2 = g[:, None] * q["w3"][None, :] * act_grad(h2)
     grad["b2"] = g2.sum(axis=0)
     grad["w2"] = _r(g2, bf16_round).T @ _r(h1, bf16_round)          # [out][in]
-    g1 = (_r(g2, bf16_round) @ _r(q["w2"], bf16_round)) * (1.0 - h1 * h1)
+    g1 = (_r(g2, bf16_round) @ _r(q["w2"], bf16_round)) * act_grad(h1)
     grad["b1"] = g1.sum(axis=0)
     grad["w1"] = g1.T @ X
     return loss, pack(grad)
