@@ -87,7 +87,16 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
   if (__ballot(use_y)) ny = mlp_eval_wave(sd.y, fy0, fy1);
   double sdev[6], dps = 0.0;
   if (fb_y) {
-    jc_strength_fallback(m, sig, D, dt, ep, epdot, dmg, sdev, &dps);
+    // the call takes addresses: copies keep sig, D and sdev in registers on
+    // the other paths (only these copies live in the stack frame)
+    double sig_c[6], D_c[6], out_c[6], dps_c = 0.0;
+    for (int e = 0; e < 6; e++) {
+      sig_c[e] = sig[e];
+      D_c[e] = D[e];
+    }
+    jc_strength_fallback(m, sig_c, D_c, dt, ep, epdot, dmg, out_c, &dps_c);
+    for (int e = 0; e < 6; e++) sdev[e] = out_c[e];
+    dps = dps_c;
   } else if (dead) {
     for (int e = 0; e < 6; e++) sdev[e] = 0;
   } else {
@@ -134,7 +143,11 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
     float nf = 0.0f;
     if (__ballot(use_f)) nf = mlp_eval_wave(sd.f, (float)triax, (float)lnr);
     if (fb_f) {
-      jc_damage_fallback(m, dinit, dmg, pH, sdev, epdot, dps);
+      double di_c = dinit, dm_c = dmg, sdev_c[6];
+      for (int e = 0; e < 6; e++) sdev_c[e] = sdev[e];
+      jc_damage_fallback(m, di_c, dm_c, pH, sdev_c, epdot, dps);
+      dinit = di_c;
+      dmg = dm_c;
     } else if (reset) {
       dinit = 0;
     } else if (!skip) {
