@@ -209,6 +209,25 @@ static bool instance_tiles(const TileGeom &g, int dim, const int32_t *clo, const
   return true;
 }
 
+// Tile-major storage index of node `tag` (the device's node_sidx).
+static int64_t host_sidx(const KArgs &a, int64_t tag) {
+  const TileGeom &g = a.tg;
+  const int k = (int)(tag % a.n[2]), j = (int)((tag / a.n[2]) % a.n[1]);
+  const int i = (int)(tag / ((int64_t)a.n[2] * a.n[1]));
+  const int t = ((i / g.T[0]) * g.nt[1] + j / g.T[1]) * g.nt[2] + k / g.T[2];
+  return (int64_t)t * g.cpt + ((i % g.T[0]) * g.T[1] + j % g.T[1]) * g.T[2] + k % g.T[2];
+}
+
+// The node group masks (kept on the host in tag order) in storage order.
+static int upload_gmask(kmpm_ctx *c) {
+  const KArgs &a = c->a;
+  std::vector<uint32_t> m((size_t)a.ns, 0u);
+  const std::vector<uint32_t> &h = *c->h_gmask;
+  for (int64_t tag = 0; tag < a.nn; tag++) m[host_sidx(a, tag)] = h[tag];
+  HIPCHK(hipMemcpy(c->d_gmask, m.data(), a.ns * sizeof(uint32_t), hipMemcpyHostToDevice));
+  return KMPM_OK;
+}
+
 // Tile geometry of a tile variant and every buffer sized by it (cell lists,
 // tile lists, scan workspace, P2G partial blocks, the instance tile map).
 // The state keeps its values; its cell keys and sort are redone on the next
@@ -253,13 +272,25 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   void *old[] = {c->d_cell_count, c->d_cell_start, c->d_tile_list, c->d_tile_slot,
-                 c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst};
+                 c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst,
+                 c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask};
   for (void *p : old)
     if (p) (void)hipFree(p);
   c->d_cell_count = c->d_cell_start = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
   c->d_scan_tmp = nullptr;
   c->d_scratch = nullptr;
   c->d_tinst = nullptr;
+  c->d_gmass = c->d_gv = c->d_gvu = nullptr;
+  c->d_gmask = nullptr;
+  // the grid, tile-major over whole tiles
+  const int64_t ns = ncells;
+  HIPCHK(dalloc(&c->d_gmass, ns));
+  HIPCHK(dalloc(&c->d_gv, 3 * ns));
+  HIPCHK(dalloc(&c->d_gvu, 3 * ns));
+  HIPCHK(dalloc(&c->d_gmask, ns));
+  HIPCHK(hipMemset(c->d_gmass, 0, ns * sizeof(double)));
+  HIPCHK(hipMemset(c->d_gv, 0, 3 * ns * sizeof(double)));
+  HIPCHK(hipMemset(c->d_gvu, 0, 3 * ns * sizeof(double)));
   HIPCHK(dalloc(&c->d_cell_count, ncells + 1));
   HIPCHK(dalloc(&c->d_cell_start, ncells + 1));
   HIPCHK(dalloc(&c->d_tile_list, g.ntiles));
@@ -277,8 +308,17 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
     HIPCHK(hipMemcpy(c->d_inst, is.data(), ninst * sizeof(InstScalars), hipMemcpyHostToDevice));
   }
   a.tg = g;
+  a.ns = ns;
+  a.gmass = c->d_gmass;
+  a.gv = c->d_gv;
+  a.gvu = c->d_gvu;
+  a.gmask = c->d_gmask;
   c->ncells = ncells;
   c->tile_variant = variant;
+  {
+    const int st = upload_gmask(c);
+    if (st != KMPM_OK) return st;
+  }
   a.cell_count = c->d_cell_count;
   a.cell_start = c->d_cell_start;
   a.tile_list = c->d_tile_list;
@@ -373,16 +413,9 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   bool ok = dalloc(&c->d_mats, nmat) == hipSuccess &&
             hipMemcpy(c->d_mats, dm.data(), nmat * sizeof(DevMat), hipMemcpyHostToDevice) ==
                 hipSuccess;
-  ok = ok && dalloc(&c->d_gmass, a.nn) == hipSuccess && dalloc(&c->d_gv, 3 * a.nn) == hipSuccess &&
-       dalloc(&c->d_gvu, 3 * a.nn) == hipSuccess && dalloc(&c->d_gmask, a.nn) == hipSuccess;
   ok = ok && dalloc(&c->d_sc, 1) == hipSuccess;
   if (ok) {
-    std::vector<uint32_t> ones(a.nn, 1u);  // Grid::grow sets mask = 1 (grid.cpp:429)
-    ok = hipMemcpy(c->d_gmask, ones.data(), a.nn * sizeof(uint32_t), hipMemcpyHostToDevice) ==
-             hipSuccess &&
-         hipMemset(c->d_gmass, 0, a.nn * sizeof(double)) == hipSuccess &&
-         hipMemset(c->d_gv, 0, 3 * a.nn * sizeof(double)) == hipSuccess &&
-         hipMemset(c->d_gvu, 0, 3 * a.nn * sizeof(double)) == hipSuccess;
+    c->h_gmask = new std::vector<uint32_t>(a.nn, 1u);   // Grid::grow sets mask = 1 (grid.cpp:429)
     DevScalars sc;
     memset(&sc, 0, sizeof(sc));
     sc.dt = cfg->dt;
@@ -392,10 +425,6 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
     ok = ok && hipMemcpy(c->d_sc, &sc, sizeof(sc), hipMemcpyHostToDevice) == hipSuccess;
   }
   a.mats = c->d_mats;
-  a.gmass = c->d_gmass;
-  a.gv = c->d_gv;
-  a.gvu = c->d_gvu;
-  a.gmask = c->d_gmask;
   a.sc = c->d_sc;
   if (!ok) {
     kmpm_ctx_destroy(c);
@@ -422,6 +451,7 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
     if (p) (void)hipFree(p);
   free(c->mats_host);
   delete c->inst_cells;
+  delete c->h_gmask;
   if (c->ev_pool) {
     for (hipEvent_t e : *c->ev_pool) (void)hipEventDestroy(e);
     delete c->ev_pool;
@@ -606,9 +636,9 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
 
 int kmpm_set_node_mask(kmpm_ctx *c, const uint32_t *mask, int64_t nn) {
   if (!c || !mask || nn != c->a.nn) return KMPM_ERR_ARG;
-  HIPCHK(hipMemcpyAsync(c->d_gmask, mask, nn * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return KMPM_OK;
+  c->h_gmask->assign(mask, mask + nn);
+  return upload_gmask(c);
 }
 
 int kmpm_fix_velocity_nodes(kmpm_ctx *c, uint32_t bit, const int set[3], const double val[3],
@@ -1051,22 +1081,25 @@ int kmpm_download_grid(kmpm_ctx *c, double *mass, double *v, double *vu, int64_t
   // layout [node][comp]
   const bool mixed = c->a.prec == KMPM_PRECISION_MIXED;
   const size_t esz = mixed ? sizeof(float) : sizeof(double);
-  std::vector<char> raw(3 * nn * esz);
+  const int64_t ns = c->a.ns;
+  std::vector<char> raw(3 * ns * esz);
   auto val = [&](size_t k) {
     return mixed ? (double)reinterpret_cast<const float *>(raw.data())[k]
                  : reinterpret_cast<const double *>(raw.data())[k];
   };
+  std::vector<int64_t> sidx(nn);   // tile-major storage -> tag order
+  for (int64_t i = 0; i < nn; i++) sidx[i] = host_sidx(c->a, i);
   if (mass) {
-    HIPCHK(hipMemcpy(raw.data(), c->d_gmass, nn * esz, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < nn; i++) mass[i] = val(i);
+    HIPCHK(hipMemcpy(raw.data(), c->d_gmass, ns * esz, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nn; i++) mass[i] = val(sidx[i]);
   }
   double *dsts[2] = {v, vu};
   double *srcs[2] = {c->d_gv, c->d_gvu};
   for (int k = 0; k < 2; k++) {
     if (!dsts[k]) continue;
-    HIPCHK(hipMemcpy(raw.data(), srcs[k], 3 * nn * esz, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(raw.data(), srcs[k], 3 * ns * esz, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < nn; i++)
-      for (int d = 0; d < 3; d++) dsts[k][3 * i + d] = val((size_t)d * nn + i);
+      for (int d = 0; d < 3; d++) dsts[k][3 * i + d] = val((size_t)d * ns + sidx[i]);
   }
   return KMPM_OK;
 }

@@ -57,9 +57,11 @@ struct KArgs {
   int *dil_list;    // tiles whose owned nodes get contributions
   double *scratch;  // partial node blocks [slot][field][bn] (int64 in the mixed path)
   int scratch_slots;
-  // grid (tag order, SoA per component); the mixed path stores FP32 values
-  // in the same allocations
-  double *gmass, *gv, *gvu;   // gv/gvu: 3 * nn
+  // grid, tile-major (node_sidx), SoA per component, ns entries per
+  // component (whole tiles: edge tiles are padded); the mixed path stores
+  // FP32 values in the same allocations
+  int64_t ns;
+  double *gmass, *gv, *gvu;   // gv/gvu: 3 * ns
   uint32_t *gmask;
   // fixes
   int nfv, nbf;
@@ -115,6 +117,7 @@ struct kmpm_ctx {
   kmpm::InstScalars *d_inst;
   int *d_tinst;
   std::vector<int32_t> *inst_cells;   // per instance: cell_lo[3], cell_hi[3]
+  std::vector<uint32_t> *h_gmask;     // node group masks in tag order
   int tile_variant;                   // Tiles<dim, S1, variant> of the tile kernels
   // captured step graphs: each replays `steps` steps from the host state
   // it was captured in (buffer ping-pong and flags) to its end state

@@ -146,6 +146,55 @@ __device__ inline int wpos(const KArgs &a) { return a.tl ? X0X : PX; }
 // ---------------------------------------------------------------------------
 constexpr int SCR_FIELDS = 7;   // scratch slot layout: mass, momentum[3], force[3]
 
+// Grid storage is tile-major: node (i, j, k) lives at tile * cells-per-tile
+// + its index inside the tile (x-major, z fastest), the same order as the
+// cell sort key, so a tile's nodes are one contiguous run of every grid array.
+template <int TX, int TY, int TZ>
+__device__ inline int64_t node_sidx(const KArgs &a, int i, int j, int k) {
+  const TileGeom &g = a.tg;
+  const int t = ((i / TX) * g.nt[1] + j / TY) * g.nt[2] + k / TZ;
+  return (int64_t)t * (TX * TY * TZ) + ((i % TX) * TY + j % TY) * TZ + k % TZ;
+}
+
+// A tile's partial node block is stored as sub-blocks, one per tile its
+// nodes belong to.  Per axis the block's nodes split into classes: cubic,
+// the node before the tile, the tile's own T nodes and the two after it;
+// linear, the own nodes and the one after.  Sub-blocks are ordered by class
+// (x, then y, then z) and each is stored (x, y, z)-major, so the grid update
+// of a tile reads every source sub-block as one contiguous run.
+template <int S1, int T, bool ONE = false> struct AxisCls {
+  static constexpr int n = ONE ? 1 : (S1 == 4 ? 3 : 2);
+  // first block offset of class k (the classes tile the block in order)
+  __host__ __device__ static constexpr int lo_b(int k) {
+    return ONE ? 0 : S1 == 4 ? (k == 0 ? 0 : k == 1 ? 1 : T + 1) : (k == 0 ? 0 : T);
+  }
+  __host__ __device__ static constexpr int size(int k) {
+    return ONE ? 1 : S1 == 4 ? (k == 0 ? 1 : k == 1 ? T : 2) : (k == 0 ? T : 1);
+  }
+  // tile offset of the class's nodes from the block's tile, and the first
+  // local index they have in that tile
+  __host__ __device__ static constexpr int toff(int k) { return ONE ? 0 : S1 == 4 ? k - 1 : k; }
+  __host__ __device__ static constexpr int lo_l(int k) {
+    return ONE ? 0 : (S1 == 4 && k == 0) ? T - 1 : 0;
+  }
+  __host__ __device__ static constexpr int cls(int b) {
+    return ONE ? 0 : S1 == 4 ? (b < 1 ? 0 : b < T + 1 ? 1 : 2) : (b < T ? 0 : 1);
+  }
+};
+
+// Position of block node (bx, by, bz) in the sub-block layout.
+template <int DIM, int S1, int TX, int TY, int TZ>
+__device__ inline int subblock_pos(int bx, int by, int bz) {
+  using CX = AxisCls<S1, TX>;
+  using CY = AxisCls<S1, TY>;
+  using CZ = AxisCls<S1, TZ, DIM == 2>;
+  constexpr int EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  const int kx = CX::cls(bx), ky = CY::cls(by), kz = CZ::cls(bz);
+  const int sx = CX::size(kx), sy = CY::size(ky), sz = CZ::size(kz);
+  const int lx = CX::lo_b(kx), ly = CY::lo_b(ky), lz = CZ::lo_b(kz);
+  return lx * EY * EZ + sx * ly * EZ + sx * sy * lz + ((bx - lx) * sy + (by - ly)) * sz + (bz - lz);
+}
+
 // Raw particle data a P2G lane loads for one particle (R: the transfer type).
 // Stress terms a P2G particle carries: -vol sigma (symmetric: 6 in 3D,
 // xx,yy,xy in 2D) or, TLMPM, -vol0 PK1 (row-major: 9 in 3D, 4 in 2D).
@@ -274,6 +323,14 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   __shared__ int seg[CHK + 1];          // segment starts; seg[nseg] = chunk size
   __shared__ int wheads[4];
   __shared__ int nseg_s;
+  // sub-block layout position -> LDS node block offset, for the partial
+  // block write (contiguous stores, gathered from LDS)
+  constexpr int BNP = EX * EY * EZ;
+  __shared__ short pinv[BNP];
+  for (int o = threadIdx.x; o < BNP; o += 256) {
+    const int bz = o % EZ, by = (o / EZ) % EY, bx = o / (EZ * EY);
+    pinv[subblock_pos<DIM, S1, TX, TY, TZ>(bx, by, bz)] = (short)(bx * PSA + by * PSB + bz);
+  }
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
@@ -410,13 +467,12 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
     // force[3]; components a 2D run does not have are written as zeros.
     const int f_lo = REMAP ? 1 : 0, f_hi = REMAP ? 4 : SCR_FIELDS;
     for (int qq = tid + f_lo * BN; qq < f_hi * BN; qq += 256) {
-      const int sf = qq / BN, o = qq % BN;
+      const int sf = qq / BN, p = qq % BN;   // p: position in the sub-block layout
       int bf;
       if (sf == 0) bf = 0;
       else if (sf <= 3) bf = sf <= DIM ? sf : -1;
       else bf = (sf - 3) <= DIM ? DIM + (sf - 3) : -1;
-      const int bz = o % EZ, by = (o / EZ) % EY, bx = o / (EZ * EY);
-      dst[qq] = bf >= 0 ? blk[bf * PBN + bx * PSA + by * PSB + bz] : (BT)0;
+      dst[qq] = bf >= 0 ? blk[bf * PBN + pinv[p]] : (BT)0;
     }
     __syncthreads();
   }
@@ -431,13 +487,14 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
   using R = typename Real<PREC>::T;
   constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
   using BT = typename std::conditional<FX, long long, double>::type;
+  using CX = AxisCls<S1, TX>;
+  using CY = AxisCls<S1, TY>;
+  using CZ = AxisCls<S1, TZ, DIM == 2>;
   R *gmass = gptr<PREC>(a.gmass), *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   constexpr int TZZ = DIM == 3 ? TZ : 1;
   constexpr int NO = TX * TY * TZZ;
-  const int off = S1 == 4 ? 1 : 0;
-  const int up = S1 == 4 ? 1 : 0;
   const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int total = a.sc->dil_count * NO;
   const double dt_ctx = a.sc->dt;
@@ -448,43 +505,52 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
     const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
     const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
     if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
-    const int64_t tag = ((int64_t)i * a.n[1] + j) * a.n[2] + k;
+    const int64_t sn = (int64_t)t * NO + o;   // tile-major storage index
     BT isum[SCR_FIELDS];
     for (int f = 0; f < SCR_FIELDS; f++) isum[f] = 0;
-    for (int dx = -1; dx <= up; dx++)
-      for (int dy = -1; dy <= up; dy++)
-        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? up : 0); dz++) {
-          const int ux = tx + dx, uy = ty + dy, uz = tz + dz;
-          if (ux < 0 || uy < 0 || uz < 0 || ux >= nt0 || uy >= nt1 || uz >= nt2) continue;
+    // the sub-blocks of the source tiles whose blocks cover this node
+#pragma unroll
+    for (int kx = 0; kx < CX::n; kx++) {
+      const int ux = tx - CX::toff(kx), rx = lx - CX::lo_l(kx);
+      if (rx < 0 || rx >= CX::size(kx) || ux < 0 || ux >= nt0) continue;
+#pragma unroll
+      for (int ky = 0; ky < CY::n; ky++) {
+        const int uy = ty - CY::toff(ky), ry = ly - CY::lo_l(ky);
+        if (ry < 0 || ry >= CY::size(ky) || uy < 0 || uy >= nt1) continue;
+#pragma unroll
+        for (int kz = 0; kz < CZ::n; kz++) {
+          const int uz = tz - CZ::toff(kz), rz = lz - CZ::lo_l(kz);
+          if (rz < 0 || rz >= CZ::size(kz) || uz < 0 || uz >= nt2) continue;
           const int slot = a.tile_slot[(ux * nt1 + uy) * nt2 + uz];
           if (slot < 0) continue;
-          const int bx = i - (ux * TX - off), by = j - (uy * TY - off);
-          const int bz = DIM == 3 ? k - (uz * TZ - off) : 0;
-          if (bx < 0 || by < 0 || bz < 0 || bx >= EX || by >= EY || bz >= EZ) continue;
-          const int idx = (bx * EY + by) * EZ + bz;
-          const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + idx;
+          const int sx = CX::size(kx), sy = CY::size(ky), sz = CZ::size(kz);
+          const int pos = CX::lo_b(kx) * EY * EZ + sx * CY::lo_b(ky) * EZ +
+                          sx * sy * CZ::lo_b(kz) + (rx * sy + ry) * sz + rz;
+          const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + pos;
           if (remap) {
             for (int f = 1; f <= 3; f++) isum[f] += src[f * BN];
           } else {
             for (int f = 0; f < SCR_FIELDS; f++) isum[f] += src[f * BN];
           }
         }
+      }
+    }
     double sum[SCR_FIELDS];
     for (int f = 0; f < SCR_FIELDS; f++)
       sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
                   : (double)isum[f];
-    const uint32_t mask = a.gmask[tag];
+    const uint32_t mask = a.gmask[sn];
     if (remap) {
       // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
       // node mass, then FixVelocityNodes::post_velocities_to_grid.
-      const double m = gmass[tag];
+      const double m = gmass[sn];
       double v[3];
       for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
       for (int fi = 0; fi < a.nfv; fi++)
         if (mask & a.fv[fi].bit)
           for (int d = 0; d < 3; d++)
             if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
-      for (int d = 0; d < 3; d++) gv[d * a.nn + tag] = (R)v[d];
+      for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
       continue;
     }
     const double m = sum[0];
@@ -502,10 +568,10 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
             vu[d] = a.fv[fi].val[d];
             v[d] = a.fv[fi].prev[d];
           }
-    gmass[tag] = (R)m;
+    gmass[sn] = (R)m;
     for (int d = 0; d < 3; d++) {
-      gv[d * a.nn + tag] = (R)v[d];
-      gvu[d * a.nn + tag] = (R)vu[d];
+      gv[d * a.ns + sn] = (R)v[d];
+      gvu[d * a.ns + sn] = (R)vu[d];
     }
   }
 }
@@ -516,14 +582,18 @@ template <int DIM, int TX, int TY, int TZ, int PREC>
 __global__ void k_grid_clean(KArgs a) {
   using R = typename Real<PREC>::T;
   R *gmass = gptr<PREC>(a.gmass), *gv = gptr<PREC>(a.gv), *gvu = gptr<PREC>(a.gvu);
+  constexpr int TZZ = DIM == 3 ? TZ : 1;
+  constexpr int NO = TX * TY * TZZ;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int S1 = a.S1;
   const int up = S1 == 4 ? 1 : 0;
-  for (int64_t tag = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tag < a.nn; tag += stride) {
-    const int k = (int)(tag % a.n[2]), j = (int)((tag / a.n[2]) % a.n[1]),
-              i = (int)(tag / ((int64_t)a.n[2] * a.n[1]));
-    const int tx = i / TX, ty = j / TY, tz = DIM == 3 ? k / TZ : 0;
+  for (int64_t sn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; sn < a.ns; sn += stride) {
+    const int t = (int)(sn / NO), o = (int)(sn % NO);
+    const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
+    const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
+    if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;   // padding of edge tiles
     bool touched = false;
     for (int dx = -1; dx <= up && !touched; dx++)
       for (int dy = -1; dy <= up && !touched; dy++)
@@ -534,7 +604,7 @@ __global__ void k_grid_clean(KArgs a) {
         }
     if (touched) continue;
     double v[3] = {0, 0, 0}, vu[3] = {0, 0, 0};
-    const uint32_t mask = a.gmask[tag];
+    const uint32_t mask = a.gmask[sn];
     for (int fi = 0; fi < a.nfv; fi++)
       if (mask & a.fv[fi].bit)
         for (int d = 0; d < 3; d++)
@@ -542,10 +612,10 @@ __global__ void k_grid_clean(KArgs a) {
             vu[d] = a.fv[fi].val[d];
             v[d] = a.fv[fi].prev[d];
           }
-    gmass[tag] = 0;
+    gmass[sn] = 0;
     for (int d = 0; d < 3; d++) {
-      gv[d * a.nn + tag] = (R)v[d];
-      gvu[d * a.nn + tag] = (R)vu[d];
+      gv[d * a.ns + sn] = (R)v[d];
+      gvu[d * a.ns + sn] = (R)vu[d];
     }
   }
 }
@@ -623,15 +693,15 @@ __device__ inline void load_node_block(const KArgs &a, typename Real<PREC>::T *n
     const int bz = q % EZ, by = (q / EZ) % EY, bx = q / (EZ * EY);
     const int i = ox - off + bx, j = oy - off + by, k = DIM == 3 ? oz - off + bz : 0;
     const bool ok = i >= 0 && j >= 0 && k >= 0 && i < a.n[0] && j < a.n[1] && k < a.n[2];
-    const int64_t tag = ok ? ((int64_t)i * a.n[1] + j) * a.n[2] + k : 0;
+    const int64_t sn = ok ? node_sidx<TX, TY, DIM == 3 ? TZ : 1>(a, i, j, k) : 0;
     for (int d = 0; d < DIM; d++) {
       if (MODE == 0) {
-        const R vu = ok ? gvu[d * a.nn + tag] : (R)0;
-        const R v = ok ? gv[d * a.nn + tag] : (R)0;
+        const R vu = ok ? gvu[d * a.ns + sn] : (R)0;
+        const R v = ok ? gv[d * a.ns + sn] : (R)0;
         nb[d * BN + q] = vu;
         nb[(DIM + d) * BN + q] = vu - v;
       } else {
-        nb[d * BN + q] = ok ? gv[d * a.nn + tag] : (R)0;
+        nb[d * BN + q] = ok ? gv[d * a.ns + sn] : (R)0;
       }
     }
   }
