@@ -176,6 +176,31 @@ __device__ inline int cell_of(double x, double lo, double inv_h, int n, bool &ba
 template <int S1>
 __device__ inline void weights_1d(double x, double lo, double h, double inv_h, int n, int base,
                                   double *w, double *dw) {
+  if (S1 == 4 && base >= 2 && base + 3 <= n - 3) {
+    // Interior stencil (every node type 0): stencil slot a lies on the
+    // spline piece r in [1 - a, 2 - a), so each slot evaluates its piece's
+    // expression of bf_cubic / bf_dcubic directly (the same operations, so
+    // the same roundings).  A position whose r falls outside the expected
+    // piece by rounding takes the general branches below.
+    double r[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) r[a] = (x - (lo + (0 + base + a) * h)) * inv_h;
+    if (r[0] >= 1 && r[0] < 2 && r[1] >= 0 && r[1] < 1 && r[2] >= -1 && r[2] < 0 &&
+        r[3] >= -2 && r[3] < -1) {
+      w[0] = ((-1.0 / 6.0 * r[0] + 1) * r[0] - 2) * r[0] + 4.0 / 3.0;
+      dw[0] = inv_h * ((-0.5 * r[0] + 2) * r[0] - 2);
+      w[1] = (0.5 * r[1] - 1) * r[1] * r[1] + 2.0 / 3.0;
+      dw[1] = inv_h * (3.0 / 2.0 * r[1] - 2) * r[1];
+      w[2] = (-0.5 * r[2] - 1) * r[2] * r[2] + 2.0 / 3.0;
+      dw[2] = inv_h * (-3.0 / 2.0 * r[2] - 2) * r[2];
+      w[3] = ((1.0 / 6.0 * r[3] + 1) * r[3] + 2) * r[3] + 4.0 / 3.0;
+      dw[3] = inv_h * ((0.5 * r[3] + 2) * r[3] + 2);
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        if (w[a] == 0) dw[a] = 0;
+      return;
+    }
+  }
 #pragma unroll
   for (int a = 0; a < S1; a++) {
     int i = base + a;

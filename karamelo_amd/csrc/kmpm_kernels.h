@@ -263,23 +263,22 @@ __device__ inline R p2g_stress(const R *s, int d, int e) {
   return d == e ? s[d] : s[2];
 }
 
-// Scaled flush of a P2G register sum into a 64-bit fixed-point LDS node sum,
-// in FP32: q = val * scale is exact (a power-of-two scale within the float
-// range), rounded to the nearest integer (ties to even, as in FP64) and
-// split into 32-bit words.
+// Scaled flush of a P2G register sum into a 64-bit fixed-point LDS node sum:
+// q = val * scale is exact in FP32 (a power-of-two scale within the float
+// range) and rounded to the nearest integer (ties to even, as in FP64).  Its
+// two's-complement words are hi = floor(q / 2^32) and lo = q - hi 2^32, in
+// [0, 2^32), formed exactly in FP64.
 __device__ inline void fx_add_f(unsigned long long *dst, float val, float scale,
                                 DevScalars *sc) {
   const float q = rintf(val * scale);
-  const float aq = fabsf(q);
-  const bool ok = aq < (float)FX_FLUSH_LIMIT;
+  const bool ok = fabsf(q) < (float)FX_FLUSH_LIMIT;
   if (!ok) set_error(sc, KMPM_ERR_FIXED_POINT, -1);
-  // |q| = hi 2^32 + lo exactly: lo is a multiple of ulp(q) below 2^32
-  const float aqc = ok ? aq : 0.0f;
-  const float hi = floorf(aqc * 2.3283064365386963e-10f);   // |q| / 2^32, below 2^22
-  const float lo = fmaf(-hi, 4294967296.0f, aqc);            // in [0, 2^32)
-  const unsigned long long mag =
-      ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
-  if (ok) atomicAdd(dst, q < 0 ? 0ull - mag : mag);
+  const float qc = ok ? q : 0.0f;
+  const float hi = floorf(qc * 2.3283064365386963e-10f);   // |hi| < 2^22
+  const double lo = fma(-(double)hi, 4294967296.0, (double)qc);
+  const unsigned long long iv =
+      ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)(unsigned)lo;
+  if (ok) atomicAdd(dst, iv);
 }
 
 // P2G over the active tiles, one workgroup per tile at a time.
