@@ -100,12 +100,12 @@ def read_traffic(kernel, workload):
         return None
 
 
-def train_surrogate(mat):
+def train_surrogate(mat, iters=6000):
     """Trains the JC networks on the device; returns the surrogate and a
     record of the training (not part of the timed step)."""
     from karamelo_amd import native as N
     t0 = time.perf_counter()
-    s = N.Surrogate(mat, samples=1 << 18, iterations=6000, batch=4096)
+    s = N.Surrogate(mat, samples=1 << 18, iterations=iters, batch=4096)
     loss = s.train()
     device_sync()
     t1 = time.perf_counter()
@@ -117,7 +117,7 @@ def train_surrogate(mat):
         out = s.eval(net, X)
         rel = np.abs(out - Y) / np.abs(Y) if net == 0 else np.abs(np.exp(out - Y) - 1)
         err[name] = {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel ** 2)))}
-    iters, batch = 6000, 4096
+    batch = 4096
     # training MFMA work: forward (1 product) + backward (2 products) per sample
     flops = 2 * iters * batch * 3 * MFMA_FLOPS_PER_EVAL
     rec = {"seconds": t1 - t0, "final_loss": list(loss), "validation": err,
@@ -233,7 +233,7 @@ def roofline_of(path, phases, npart, workload):
     alg = ALG_BYTES_PER_PARTICLE[path].get(dom, 0) * npart or None
     kname = KERNEL_NAMES.get(dom, dom)
     achieved = alg / avg_s / 1e9 if alg else None
-    wl = workload if path == "plain-fp64" else workload + "_mixed"
+    wl = workload + {"plain-fp64": "", "mixed": "_mixed", "mixed+surrogate": "_mixed_surrogate"}[path]
     return {"bound": "hbm", "kernel": kname,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
@@ -286,6 +286,10 @@ def main():
                     help="comma-separated paths to time; the last one is the headline")
     ap.add_argument("--autotune-steps", type=int, default=5,
                     help="trial steps per tile variant (0: keep the default tiles)")
+    ap.add_argument("--prestrain", type=float, default=0.0,
+                    help="initial effective plastic strain of every particle (counter runs: "
+                         "the surrogate is in use from the first step without a warm-up)")
+    ap.add_argument("--train-iterations", type=int, default=6000)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-instances workload")
     ap.add_argument("--batched-steps", type=int, default=100)
@@ -302,6 +306,8 @@ def main():
     paths = args.paths.split(",")
     log("building workload ...")
     prob = P.tensile_bar(h=args.h)
+    if args.prestrain:
+        prob.particles["eff_plastic_strain"][:] = args.prestrain
     npart = prob.np
     workload = "tensile_bar_jc_4.27M" if abs(args.h - 6e-5) < 1e-12 else "tensile_bar_jc_h%g" % args.h
     log("particles: %d" % npart)
@@ -310,7 +316,7 @@ def main():
     surrogate, training = None, None
     if "mixed+surrogate" in paths:
         log("training the surrogate ...")
-        surrogate, training = train_surrogate(prob.mats[0])
+        surrogate, training = train_surrogate(prob.mats[0], args.train_iterations)
     for path in paths:
         results[path], states[path] = run_path(prob, path, args.steps, args.warmup, surrogate,
                                                autotune_steps=args.autotune_steps)

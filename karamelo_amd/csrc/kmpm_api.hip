@@ -47,7 +47,8 @@ static void free_particle_buffers(kmpm_ctx *c) {
     if (p) (void)hipFree(p);
   c->d_pa = c->d_pb = c->d_xo = nullptr;
   c->d_fa = c->d_fb = nullptr;
-  c->d_ia = c->d_ib = c->d_key = c->d_rank = c->d_order_tmp = c->d_order = nullptr;
+  c->d_ia = c->d_ib = c->d_key = c->d_rank = c->d_order = nullptr;
+  c->d_order_tmp = nullptr;
   c->d_pmask = nullptr;
   free(c->h_ptag);
   free(c->h_solid);

@@ -47,7 +47,7 @@ struct KArgs {
   double *xo;       // MUSL: positions at the start of the step, sorted slots (3*cap)
   int *key;         // cell key per current slot
   int *rank;        // arrival rank within the cell
-  int *order_tmp;   // counting-sort scatter
+  int4 *order_tmp;  // counting-sort scatter: (slot, pid, key, -) at each arrival position
   int *order;       // sorted position -> current slot
   // cells and tiles
   int *cell_count;  // [ncells + 1]
@@ -105,7 +105,8 @@ struct kmpm_ctx {
   double *d_pa, *d_pb, *d_xo;
   float *d_fa, *d_fb;
   int *d_ia, *d_ib;
-  int *d_key, *d_rank, *d_order_tmp, *d_order;
+  int *d_key, *d_rank, *d_order;
+  int4 *d_order_tmp;
   int *d_cell_count, *d_cell_start;
   int *d_tile_list, *d_tile_slot, *d_dil_list;
   double *d_scratch;

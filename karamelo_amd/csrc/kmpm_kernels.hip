@@ -71,10 +71,13 @@ __global__ void k_count(KArgs a) {
   }
 }
 
+// Scatter to the arrival position, with the pid and key beside the slot in
+// one 16-byte store, so the rank fix reads its cell's entries contiguously.
 __global__ void k_scatter(KArgs a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
-  a.order_tmp[a.cell_start[a.key[i]] + a.rank[i]] = i;
+  const int key = a.key[i];
+  a.order_tmp[a.cell_start[key] + a.rank[i]] = int4{i, IFLD(a.ia, PID, i), key, 0};
 }
 
 // Arrival order inside a cell is not deterministic; the final position is
@@ -82,13 +85,11 @@ __global__ void k_scatter(KArgs a) {
 __global__ void k_rankfix(KArgs a) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.np) return;
-  int i = a.order_tmp[j];
-  int k = a.key[i];
-  int s = a.cell_start[k], e = a.cell_start[k + 1];
-  int pid = IFLD(a.ia, PID, i);
+  const int4 e4 = a.order_tmp[j];
+  const int s = a.cell_start[e4.z], e = a.cell_start[e4.z + 1];
   int r = 0;
-  for (int q = s; q < e; q++) r += IFLD(a.ia, PID, a.order_tmp[q]) < pid ? 1 : 0;
-  a.order[s + r] = i;
+  for (int q = s; q < e; q++) r += a.order_tmp[q].y < e4.y ? 1 : 0;
+  a.order[s + r] = e4.x;
 }
 
 __global__ void k_tiles(KArgs a) {
