@@ -42,13 +42,14 @@ static hipError_t dalloc(T **p, size_t n) {
 
 static void free_particle_buffers(kmpm_ctx *c) {
   void *ptrs[] = {c->d_pa, c->d_pb, c->d_fa, c->d_fb, c->d_xo, c->d_ia, c->d_ib, c->d_key,
-                  c->d_rank, c->d_order_tmp, c->d_order, c->d_pmask};
+                  c->d_rank, c->d_order_tmp, c->d_sort_tmp, c->d_order, c->d_pmask};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_pa = c->d_pb = c->d_xo = nullptr;
   c->d_fa = c->d_fb = nullptr;
   c->d_ia = c->d_ib = c->d_key = c->d_rank = c->d_order = nullptr;
   c->d_order_tmp = nullptr;
+  c->d_sort_tmp = nullptr;
   c->d_pmask = nullptr;
   free(c->h_ptag);
   free(c->h_solid);
@@ -72,6 +73,7 @@ static void refresh_args(kmpm_ctx *c) {
   a.key = c->d_key;
   a.rank = c->d_rank;
   a.order_tmp = c->d_order_tmp;
+  a.sort_tmp = c->d_sort_tmp;
   a.order = c->d_order;
 }
 
@@ -251,6 +253,8 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   }
   const int64_t ncells = (int64_t)g.ntiles * g.cpt;
   if (ncells >= 2147483647LL) return KMPM_ERR_ARG;
+  // the per-tile cell sort's LDS table; cell keys split into tile and cell by shift and mask
+  if (g.cpt > TILE_SORT_MAX_CPT || (g.cpt & (g.cpt - 1)) != 0) return KMPM_ERR_ARG;
   // instances keep their cell ranges, which must be multiples of the tiles
   const int ninst = a.ninst;
   std::vector<int> map;
@@ -272,12 +276,12 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
     }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  void *old[] = {c->d_cell_count, c->d_cell_start, c->d_tile_list, c->d_tile_slot,
+  void *old[] = {c->d_tile_np, c->d_tile_p0, c->d_tile_list, c->d_tile_slot,
                  c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst,
                  c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask};
   for (void *p : old)
     if (p) (void)hipFree(p);
-  c->d_cell_count = c->d_cell_start = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
+  c->d_tile_np = c->d_tile_p0 = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
   c->d_scan_tmp = nullptr;
   c->d_scratch = nullptr;
   c->d_tinst = nullptr;
@@ -292,13 +296,14 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   HIPCHK(hipMemset(c->d_gmass, 0, ns * sizeof(double)));
   HIPCHK(hipMemset(c->d_gv, 0, 3 * ns * sizeof(double)));
   HIPCHK(hipMemset(c->d_gvu, 0, 3 * ns * sizeof(double)));
-  HIPCHK(dalloc(&c->d_cell_count, ncells + 1));
-  HIPCHK(dalloc(&c->d_cell_start, ncells + 1));
+  HIPCHK(dalloc(&c->d_tile_np, g.ntiles + 1));
+  HIPCHK(dalloc(&c->d_tile_p0, g.ntiles + 1));
+  HIPCHK(hipMemset(c->d_tile_np, 0, (g.ntiles + 1) * sizeof(int)));   // zero between sorts
   HIPCHK(dalloc(&c->d_tile_list, g.ntiles));
   HIPCHK(dalloc(&c->d_tile_slot, g.ntiles));
   HIPCHK(dalloc(&c->d_dil_list, g.ntiles));
   HIPCHK(hipMemset(c->d_tile_slot, 0xff, g.ntiles * sizeof(int)));
-  c->scan_tmp_bytes = scan_temp_bytes(ncells + 1);
+  c->scan_tmp_bytes = scan_temp_bytes(g.ntiles + 1);
   HIPCHK(hipMalloc(&c->d_scan_tmp, c->scan_tmp_bytes ? c->scan_tmp_bytes : 16));
   // P2G partial blocks: one per active tile, at most min(tiles, particles)
   const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(g.ntiles, a.np));
@@ -320,8 +325,8 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
     const int st = upload_gmask(c);
     if (st != KMPM_OK) return st;
   }
-  a.cell_count = c->d_cell_count;
-  a.cell_start = c->d_cell_start;
+  a.tile_np = c->d_tile_np;
+  a.tile_p0 = c->d_tile_p0;
   a.tile_list = c->d_tile_list;
   a.tile_slot = c->d_tile_slot;
   a.dil_list = c->d_dil_list;
@@ -445,7 +450,7 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   free_particle_buffers(c);
   for (auto &g : c->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  void *ptrs[] = {c->d_mats, c->d_cell_count, c->d_cell_start, c->d_tile_list, c->d_tile_slot,
+  void *ptrs[] = {c->d_mats, c->d_tile_np, c->d_tile_p0, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
                   c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst};
   for (void *p : ptrs)
@@ -498,6 +503,7 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   HIPCHK(dalloc(&c->d_key, cap));
   HIPCHK(dalloc(&c->d_rank, cap));
   HIPCHK(dalloc(&c->d_order_tmp, cap));
+  HIPCHK(dalloc(&c->d_sort_tmp, cap));
   HIPCHK(dalloc(&c->d_order, cap));
   HIPCHK(dalloc(&c->d_pmask, cap));
   {

@@ -75,6 +75,12 @@ enum MatClass { MC_GENERIC = 0, MC_LINEAR = 1, MC_JC_SHOCK = 2, MC_FLUID = 3 };
 struct DevFixVel { uint32_t bit; int set[3]; double val[3], prev[3]; };
 struct DevFixBF { uint32_t bit; int set[3]; double val[3]; };
 constexpr int MAX_FIX = 8;
+// Per-tile cell sort: cells per tile covered by its LDS offset table, and the
+// particles per tile it holds in registers (TILE_SORT_KPL per lane of one
+// wavefront) and orders in LDS; larger tiles order through memory.
+constexpr int TILE_SORT_MAX_CPT = 1024;
+constexpr int TILE_SORT_KPL = 16;
+constexpr int TILE_SORT_CAP = 64 * TILE_SORT_KPL;
 
 // ---- device scalars (time, dt, CFL accumulator, sticky error) -------------
 struct DevScalars {

@@ -350,8 +350,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
     const int ty = (t / nt2) % nt1;
     const int tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
-    constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
-    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     for (int i = tid; i < NF * PBN; i += 256) blk[i] = 0;
 
     for (int c0 = p0; c0 < p1; c0 += CHK) {
@@ -856,7 +855,6 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
-  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   __shared__ R nb[2 * DIM * BN];
   // SURR: the networks and envelope, staged in LDS once per block (read at
   // each use instead of being held in registers across the particle loop)
@@ -880,7 +878,7 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
     const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
-    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
@@ -1045,7 +1043,6 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
   constexpr int PREC = KMPM_PRECISION_FP64;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
-  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   __shared__ double nb[2 * DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
@@ -1059,7 +1056,7 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
     const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
-    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
@@ -1111,7 +1108,6 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
-  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   __shared__ R nb[2 * DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
@@ -1124,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
     const double inv_dt = 1.0 / dt;
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
-    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
@@ -1185,7 +1181,6 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
-  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   __shared__ R nb[DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
@@ -1197,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
     const double dt = tile_dt(a, t, dt_ctx);
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
-    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
@@ -1270,7 +1265,6 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
   constexpr int PREC = KMPM_PRECISION_FP64;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
-  constexpr int CPT = TX * TY * (DIM == 3 ? TZ : 1);
   __shared__ double nb[DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
@@ -1282,7 +1276,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
     const double dt = tile_dt(a, t, dt_ctx);
     const int tz = DIM == 3 ? t % nt2 : 0, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
     const int org[3] = {tx * TX, ty * TY, DIM == 3 ? tz * TZ : 0};
-    const int p0 = a.cell_start[(int64_t)t * CPT], p1 = a.cell_start[(int64_t)(t + 1) * CPT];
+    const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();

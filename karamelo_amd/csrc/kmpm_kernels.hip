@@ -31,100 +31,286 @@ __global__ void k_keys(KArgs a) {
   a.key[i] = particle_key(a, PF(a.pa, f, i), PF(a.pa, f + 1, i), PF(a.pa, f + 2, i));
 }
 
-__global__ void k_sort_begin(KArgs a, int64_t ncells) {
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= ncells; q += stride)
-    a.cell_count[q] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+// Resets the step's tile-list counters when no particle runs k_count (which
+// resets them otherwise).  Per-tile counts are zero between sorts: k_tiles
+// clears them once the scan has consumed them.
+__global__ void k_sort_begin(KArgs a) {
+  if (threadIdx.x == 0) {
     a.sc->tile_count = 0;
     a.sc->dil_count = 0;
   }
 }
 
-// Counts particles per cell and gives each its arrival rank.  Particles are
-// stored in the previous step's sorted order, so equal keys come in runs:
-// one atomic per run of equal keys inside a wavefront.
-__global__ void k_count(KArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const bool valid = i < a.np;
-  const int key = valid ? a.key[i] : -1;
-  const int prev = __shfl_up(key, 1, 64);
-  const bool head = valid && (lane == 0 || prev != key);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long live = __ballot(valid);
-  // run of this lane: [start, end)
-  const unsigned long long below = heads & ((2ull << lane) - 1ull);   // heads at lanes <= lane
-  const int start = below ? 63 - __clzll(below) : 0;
-  const unsigned long long above = lane < 63 ? (heads >> (lane + 1)) : 0ull;
-  const int nlive = __popcll(live);
-  const int end = above ? lane + 1 + __ffsll(above) - 1 : nlive;
-  int base = 0;
-  if (head) base = atomicAdd(&a.cell_count[key], end - lane);
-  base = __shfl(base, start, 64);
-  if (valid) a.rank[i] = base + (lane - start);
-  // batched contexts: the particle's stencil must stay in its instance
-  if (valid && a.ninst) {
-    int c[3];
-    key_cell(a, key, c);
-    check_instance(a, c, key / a.tg.cpt, IFLD(a.ia, PID, i));
+// Particles per lane in k_count and k_scatter: a lane handles SORT_PPT
+// particles a wavefront apart, so one wavefront has SORT_PPT loads or atomics
+// in flight before it waits.  Wavefronts whose rows lie inside [0, np) take
+// a path with no per-lane conditions around loads and stores, so that the
+// compiler's wait counts stay exact and loads are not serialised behind the
+// atomics and stores of earlier rows.  Cells per tile are a power of two.
+constexpr int SORT_PPT = 8;
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_PER_WAVE = 64 * SORT_PPT;
+constexpr int SORT_PER_BLOCK = SORT_BLOCK * SORT_PPT;
+
+template <bool FULL>
+__device__ inline void count_rows(const KArgs &a, int i0, int lane) {
+  const int cshift = __ffs(a.tg.cpt) - 1;
+  int key[SORT_PPT], tile[SORT_PPT], base[SORT_PPT], start[SORT_PPT];
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) {
+    const int i = i0 + k * 64;
+    key[k] = a.key[FULL ? i : min(i, a.np - 1)];
   }
-}
-
-// Scatter to the arrival position, with the pid and key beside the slot in
-// one 16-byte store, so the rank fix reads its cell's entries contiguously.
-__global__ void k_scatter(KArgs a) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  const int key = a.key[i];
-  a.order_tmp[a.cell_start[key] + a.rank[i]] = int4{i, IFLD(a.ia, PID, i), key, 0};
-}
-
-// Arrival order inside a cell is not deterministic; the final position is
-// the rank of the particle id inside its cell segment.
-__global__ void k_rankfix(KArgs a) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.np) return;
-  const int4 e4 = a.order_tmp[j];
-  const int s = a.cell_start[e4.z], e = a.cell_start[e4.z + 1];
-  int r = 0;
-  for (int q = s; q < e; q++) r += a.order_tmp[q].y < e4.y ? 1 : 0;
-  a.order[s + r] = e4.x;
-}
-
-__global__ void k_tiles(KArgs a) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.tg.ntiles) return;
-  int p0 = a.cell_start[(int64_t)t * a.tg.cpt], p1 = a.cell_start[(int64_t)(t + 1) * a.tg.cpt];
-  if (p1 > p0) {
-    int slot = atomicAdd(&a.sc->tile_count, 1);
-    a.tile_list[slot] = t;
-    a.tile_slot[t] = slot;
-  } else {
-    a.tile_slot[t] = -1;
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++)
+    tile[k] = FULL || i0 + k * 64 < a.np ? key[k] >> cshift : -1;
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) {
+    const bool valid = FULL || i0 + k * 64 < a.np;
+    const int prev = __shfl_up(tile[k], 1, 64);
+    const bool head = valid && (lane == 0 || prev != tile[k]);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long live = __ballot(valid);
+    // run of this lane: [start, end)
+    const unsigned long long below = heads & ((2ull << lane) - 1ull);   // heads at lanes <= lane
+    start[k] = below ? 63 - __clzll(below) : 0;
+    const unsigned long long above = lane < 63 ? (heads >> (lane + 1)) : 0ull;
+    const int end = above ? lane + 1 + __ffsll(above) - 1 : __popcll(live);
+    base[k] = 0;
+    if (head) base[k] = atomicAdd(&a.tile_np[tile[k]], end - lane);
   }
-}
-
-// Tiles whose owned nodes lie in the node block of an active tile: the
-// active tile itself and its neighbours at -1 (and +1 for cubic) per axis.
-__global__ void k_dilate(KArgs a) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.tg.ntiles) return;
-  const TileGeom &g = a.tg;
-  int tz = t % g.nt[2], ty = (t / g.nt[2]) % g.nt[1], tx = t / (g.nt[2] * g.nt[1]);
-  int up = a.S1 == 4 ? 1 : 0;
-  bool hit = false;
-  for (int dx = -1; dx <= up && !hit; dx++)
-    for (int dy = -1; dy <= up && !hit; dy++)
-      for (int dz = (a.dim == 3 ? -1 : 0); dz <= (a.dim == 3 ? up : 0) && !hit; dz++) {
-        int ux = tx + dx, uy = ty + dy, uz = tz + dz;
-        if (ux < 0 || uy < 0 || uz < 0 || ux >= g.nt[0] || uy >= g.nt[1] || uz >= g.nt[2]) continue;
-        if (a.tile_slot[(ux * g.nt[1] + uy) * g.nt[2] + uz] >= 0) hit = true;
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) {
+    const int i = i0 + k * 64;
+    const int b = __shfl(base[k], start[k], 64);
+    if (FULL || i < a.np) a.rank[i] = b + (lane - start[k]);
+  }
+  if (a.ninst) {   // batched contexts: the particle's stencil must stay in its instance
+#pragma unroll
+    for (int k = 0; k < SORT_PPT; k++) {
+      const int i = i0 + k * 64;
+      if (FULL || i < a.np) {
+        int c[3];
+        key_cell(a, key[k], c);
+        check_instance(a, c, tile[k], IFLD(a.ia, PID, i));
       }
-  if (hit) {
-    int slot = atomicAdd(&a.sc->dil_count, 1);
-    a.dil_list[slot] = t;
+    }
   }
+}
+
+// Counts particles per tile and gives each its arrival rank in the tile.
+// Particles are stored in the previous step's sorted order, so equal tiles
+// come in runs: one atomic per run of a tile inside a wavefront's row.
+__global__ __launch_bounds__(SORT_BLOCK) void k_count(KArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc->tile_count = 0;
+    a.sc->dil_count = 0;
+  }
+  const int lane = threadIdx.x & 63;
+  const int w0 = blockIdx.x * SORT_PER_BLOCK + (threadIdx.x >> 6) * SORT_PER_WAVE;
+  if (w0 >= a.np) return;
+  if (w0 + SORT_PER_WAVE <= a.np) count_rows<true>(a, w0 + lane, lane);
+  else count_rows<false>(a, w0 + lane, lane);
+}
+
+template <bool FULL>
+__device__ inline void scatter_rows(const KArgs &a, int i0) {
+  const int cpt = a.tg.cpt, cshift = __ffs(cpt) - 1;
+  int key[SORT_PPT], rank[SORT_PPT], pid[SORT_PPT], p0[SORT_PPT];
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) {
+    const int i = FULL ? i0 + k * 64 : min(i0 + k * 64, a.np - 1);
+    key[k] = a.key[i];
+    rank[k] = a.rank[i];
+    pid[k] = IFLD(a.ia, PID, i);
+  }
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) p0[k] = a.tile_p0[key[k] >> cshift];
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) {
+    const int i = i0 + k * 64;
+    if (FULL || i < a.np)
+      a.order_tmp[p0[k] + rank[k]] = int4{i, pid[k], key[k] & (cpt - 1), 0};
+  }
+}
+
+// Scatter to the arrival position inside the tile's range, with the pid and
+// the cell inside the tile beside the slot in one 16-byte store.
+__global__ __launch_bounds__(SORT_BLOCK) void k_scatter(KArgs a) {
+  const int w0 = blockIdx.x * SORT_PER_BLOCK + (threadIdx.x >> 6) * SORT_PER_WAVE;
+  if (w0 >= a.np) return;
+  if (w0 + SORT_PER_WAVE <= a.np) scatter_rows<true>(a, w0 + (threadIdx.x & 63));
+  else scatter_rows<false>(a, w0 + (threadIdx.x & 63));
+}
+
+// Appends the flagged lanes' values to a list with one atomic per wavefront.
+__device__ inline void wave_append(bool flag, int value, int *count, int *list, int *slot_out) {
+  const unsigned long long m = __ballot(flag);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+  if (flag) {
+    list[slot] = value;
+    if (slot_out) *slot_out = slot;
+  }
+}
+
+// Active tiles (particles in the tile) and the tiles whose owned nodes lie in
+// the node block of an active tile: the active tile itself and its
+// neighbours at -1 (and +1 for cubic) per axis.  Clears the per-tile counts
+// for the next sort (the scan has consumed them).
+__global__ void k_tiles(KArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const TileGeom &g = a.tg;
+  const bool in = t < g.ntiles;
+  if (t <= g.ntiles) a.tile_np[t] = 0;
+  bool active = false, hit = false;
+  if (in) {
+    active = a.tile_p0[t + 1] > a.tile_p0[t];
+    const int tz = t % g.nt[2], ty = (t / g.nt[2]) % g.nt[1], tx = t / (g.nt[2] * g.nt[1]);
+    // all neighbours' ranges in flight at once (no early exit)
+    const int up = a.S1 == 4 ? 1 : 0;
+    const int zlo = a.dim == 3 ? -1 : 0, zhi = a.dim == 3 ? up : 0;
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dz = -1; dz <= 1; dz++) {
+          const int ux = tx + dx, uy = ty + dy, uz = tz + dz;
+          const bool use = dx <= up && dy <= up && dz >= zlo && dz <= zhi && ux >= 0 &&
+                           uy >= 0 && uz >= 0 && ux < g.nt[0] && uy < g.nt[1] && uz < g.nt[2];
+          const int u = use ? (ux * g.nt[1] + uy) * g.nt[2] + uz : t;
+          hit |= a.tile_p0[u + 1] > a.tile_p0[u];
+        }
+  }
+  int slot = -1;
+  wave_append(active, t, &a.sc->tile_count, a.tile_list, &slot);
+  if (in) a.tile_slot[t] = slot;
+  wave_append(hit, t, &a.sc->dil_count, a.dil_list, nullptr);
+}
+
+// Orders the particles of each active tile by cell, then by particle id (the
+// arrival order of the scatter is not deterministic): a counting sort over
+// the tile's cells in LDS, then each particle's rank among the particles of
+// its cell with smaller ids.  One wavefront per tile, so that many tiles are
+// in flight per CU.  A tile of up to TILE_SORT_CAP particles is read once into
+// registers (TILE_SORT_KPL per lane), its ids placed by cell in LDS, and each
+// lane ranks its own particles; a larger one places (slot, id) pairs in its
+// own range of sort_tmp and ranks from there.  The next tile's range is
+// fetched while the current one is ordered.  Dynamic LDS:
+// tile_sort_lds_bytes(cpt).
+__global__ __launch_bounds__(64) void k_tile_sort(KArgs a) {
+  extern __shared__ int lds[];
+  const int cpt = a.tg.cpt;
+  int *off = lds;                       // [cpt + 1] first position of each cell
+  int *cur = off + cpt + 1;             // [cpt] counts, then placement cursors
+  int *s_pid = cur + cpt;               // [TILE_SORT_CAP] ids placed by cell
+  const int lane = threadIdx.x;
+  const int per = (cpt + 63) >> 6;      // cells per lane in the scan
+  const int ntile = a.sc->tile_count;
+  int li = blockIdx.x;
+  int p0 = 0, p1 = 0;
+  if (li < ntile) {
+    const int t = a.tile_list[li];
+    p0 = a.tile_p0[t];
+    p1 = a.tile_p0[t + 1];
+  }
+  for (; li < ntile; li += gridDim.x) {
+    const int n = p1 - p0;
+    const int4 *in = a.order_tmp + p0;
+    const bool inreg = n <= TILE_SORT_CAP;
+    int4 e[TILE_SORT_KPL];
+#pragma unroll
+    for (int k = 0; k < TILE_SORT_KPL; k++) {
+      const int j = k * 64 + lane;
+      e[k] = inreg && j < n ? in[j] : int4{0, 0, -1, 0};
+    }
+    // the next tile's range, in flight during this tile's ordering
+    const int nli = li + gridDim.x;
+    const int nt = nli < ntile ? a.tile_list[nli] : 0;
+    __syncthreads();   // the previous tile is done with LDS
+    for (int c = lane; c < cpt; c += 64) cur[c] = 0;
+    __syncthreads();
+    if (inreg) {
+#pragma unroll
+      for (int k = 0; k < TILE_SORT_KPL; k++)
+        if (e[k].z >= 0) atomicAdd(&cur[e[k].z], 1);
+    } else {
+      for (int j = lane; j < n; j += 64) atomicAdd(&cur[in[j].z], 1);
+    }
+    __syncthreads();
+    {   // exclusive scan of the counts, `per` consecutive cells per lane
+      const int b = lane * per;
+      int sum = 0;
+      for (int k = 0; k < per; k++) sum += b + k < cpt ? cur[b + k] : 0;
+      int x = sum;
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+      }
+      int ex = x - sum;
+      for (int k = 0; k < per; k++)
+        if (b + k < cpt) {
+          const int v = cur[b + k];
+          off[b + k] = ex;
+          cur[b + k] = ex;
+          ex += v;
+        }
+      if (lane == 0) off[cpt] = n;
+    }
+    __syncthreads();
+    int np0 = 0, np1 = 0;
+    if (nli < ntile) {
+      np0 = a.tile_p0[nt];
+      np1 = a.tile_p0[nt + 1];
+    }
+    if (inreg) {
+#pragma unroll
+      for (int k = 0; k < TILE_SORT_KPL; k++)
+        if (e[k].z >= 0) s_pid[atomicAdd(&cur[e[k].z], 1)] = e[k].y;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < TILE_SORT_KPL; k++)
+        if (e[k].z >= 0) {
+          const int s = off[e[k].z], q1 = off[e[k].z + 1];
+          int r = 0;
+          for (int q = s; q < q1; q++) r += s_pid[q] < e[k].y ? 1 : 0;
+          a.order[p0 + s + r] = e[k].x;
+        }
+    } else {
+      int2 *g = a.sort_tmp + p0;
+      for (int j = lane; j < n; j += 64) {
+        const int4 v = in[j];
+        g[atomicAdd(&cur[v.z], 1)] = int2{v.x, v.y};
+      }
+      __threadfence_block();
+      __syncthreads();
+      for (int j = lane; j < n; j += 64) {
+        int lo = 0, hi = cpt;   // the last cell with off <= j: off[lo] <= j < off[lo + 1]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (off[mid] <= j) lo = mid;
+          else hi = mid;
+        }
+        const int2 me = g[j];
+        const int s = off[lo], q1 = off[lo + 1];
+        int r = 0;
+        for (int q = s; q < q1; q++) r += g[q].y < me.y ? 1 : 0;
+        a.order[p0 + s + r] = me.x;
+      }
+    }
+    p0 = np0;
+    p1 = np1;
+  }
+}
+
+size_t tile_sort_lds_bytes(int cpt) {
+  return (size_t)(2 * cpt + 1 + TILE_SORT_CAP) * sizeof(int);
 }
 
 // update_time + adjust_dt (update.cpp:201-206, ulmpm.cpp:525-551)
@@ -357,22 +543,22 @@ size_t scan_temp_bytes(int64_t n) {
 
 hipError_t launch_sort(kmpm_ctx *c) {
   KArgs &a = c->a;
-  const int64_t nc = c->ncells;
-  int g = (int)(nc / 256 + 1);
-  if (g > c->cu_count * 16) g = c->cu_count * 16;
-  hipLaunchKernelGGL(k_sort_begin, dim3(g), dim3(256), 0, c->stream, a, nc);
-  if (a.np > 0) {
-    hipLaunchKernelGGL(k_count, dim3(nblk(a.np, 256)), dim3(256), 0, c->stream, a);
-  }
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, c->scan_tmp_bytes, a.cell_count,
-                                                  a.cell_start, (int)(nc + 1), c->stream);
+  const int64_t nt = a.tg.ntiles;
+  if (a.np > 0)
+    hipLaunchKernelGGL(k_count, dim3(nblk(a.np, SORT_PER_BLOCK)), dim3(SORT_BLOCK), 0, c->stream,
+                       a);
+  else
+    hipLaunchKernelGGL(k_sort_begin, dim3(1), dim3(64), 0, c->stream, a);
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, c->scan_tmp_bytes, a.tile_np,
+                                                  a.tile_p0, (int)(nt + 1), c->stream);
   if (e != hipSuccess) return e;
-  if (a.np > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(a.np, 256)), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_rankfix, dim3(nblk(a.np, 256)), dim3(256), 0, c->stream, a);
-  }
-  hipLaunchKernelGGL(k_tiles, dim3(nblk(a.tg.ntiles, 256)), dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL(k_dilate, dim3(nblk(a.tg.ntiles, 256)), dim3(256), 0, c->stream, a);
+  if (a.np > 0)
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(a.np, SORT_PER_BLOCK)), dim3(SORT_BLOCK), 0,
+                       c->stream, a);
+  hipLaunchKernelGGL(k_tiles, dim3(nblk(nt + 1, 256)), dim3(256), 0, c->stream, a);
+  if (a.np > 0)
+    hipLaunchKernelGGL(k_tile_sort, dim3(persistent_grid(c, 16)), dim3(64),
+                       tile_sort_lds_bytes(a.tg.cpt), c->stream, a);
   return hipGetLastError();
 }
 

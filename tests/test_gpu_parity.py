@@ -198,6 +198,34 @@ def run_and_compare(prob, steps, chunk, tol=TOL_STEP, check_energy=True, conditi
     return worst, po
 
 
+def test_dense_tiles_sort_and_steps():
+    """125 particles per cell: the bar's full tiles hold more particles than
+    the per-tile sort orders in local memory (1024), its edge tiles fewer, so
+    both orderings run.  Sort permutation bit-exact, then 10 steps vs the
+    oracle."""
+    say("dense tiles: sort and 10 steps")
+    prob = P.tensile_bar(h=6e-4, ppc1d=5)
+    ctx = P.context_for(prob)
+    o = O.oracle_for(prob)
+    ctx.phase("compute_grid_weight_functions_and_gradients")
+    o.phase("weights")
+    ijk_g, sorted_g = ctx.cells()
+    assert np.array_equal(ijk_g, o.cells())
+    T = ctx.tile_shape()
+    assert np.array_equal(sorted_g, o.sort_permutation(T))
+    n = N.grid_size(prob.cfg)
+    lo = np.array(prob.cfg.boxlo)
+    c = np.minimum(((prob.particles["x"] - lo) / prob.cfg.cellsize).astype(np.int64),
+                   np.array(n) - 1)
+    nt = [(n[d] + T[d] - 1) // T[d] for d in range(3)]
+    tile = ((c[:, 0] // T[0]) * nt[1] + c[:, 1] // T[1]) * nt[2] + c[:, 2] // T[2]
+    per_tile = np.bincount(tile)
+    per_tile = per_tile[per_tile > 0]
+    assert per_tile.max() > 1024 and per_tile.min() <= 1024, (per_tile.min(), per_tile.max())
+    ctx.close()
+    run_and_compare(prob, 10, 5)
+
+
 def test_two_disks_3000_steps_conditioned():
     """The full collision.  Linear shapes with lattice-aligned uniform motion
     amplify 1-ulp differences (derivative_linear(0) == 0 at node crossings),
