@@ -42,12 +42,12 @@ static hipError_t dalloc(T **p, size_t n) {
 
 static void free_particle_buffers(kmpm_ctx *c) {
   void *ptrs[] = {c->d_pa, c->d_pb, c->d_fa, c->d_fb, c->d_xo, c->d_ia, c->d_ib, c->d_key,
-                  c->d_rank, c->d_order_tmp, c->d_sort_tmp, c->d_order, c->d_pmask};
+                  c->d_order_tmp, c->d_sort_tmp, c->d_order, c->d_pmask};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_pa = c->d_pb = c->d_xo = nullptr;
   c->d_fa = c->d_fb = nullptr;
-  c->d_ia = c->d_ib = c->d_key = c->d_rank = c->d_order = nullptr;
+  c->d_ia = c->d_ib = c->d_key = c->d_order = nullptr;
   c->d_order_tmp = nullptr;
   c->d_sort_tmp = nullptr;
   c->d_pmask = nullptr;
@@ -71,7 +71,6 @@ static void refresh_args(kmpm_ctx *c) {
   a.ib = c->d_ib;
   a.xo = c->d_xo;
   a.key = c->d_key;
-  a.rank = c->d_rank;
   a.order_tmp = c->d_order_tmp;
   a.sort_tmp = c->d_sort_tmp;
   a.order = c->d_order;
@@ -276,12 +275,12 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
     }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  void *old[] = {c->d_tile_np, c->d_tile_p0, c->d_tile_list, c->d_tile_slot,
+  void *old[] = {c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                  c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst,
                  c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask};
   for (void *p : old)
     if (p) (void)hipFree(p);
-  c->d_tile_np = c->d_tile_p0 = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
+  c->d_tile_np = c->d_tile_p0 = c->d_tile_cur = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
   c->d_scan_tmp = nullptr;
   c->d_scratch = nullptr;
   c->d_tinst = nullptr;
@@ -298,6 +297,7 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   HIPCHK(hipMemset(c->d_gvu, 0, 3 * ns * sizeof(double)));
   HIPCHK(dalloc(&c->d_tile_np, g.ntiles + 1));
   HIPCHK(dalloc(&c->d_tile_p0, g.ntiles + 1));
+  HIPCHK(dalloc(&c->d_tile_cur, g.ntiles));
   HIPCHK(hipMemset(c->d_tile_np, 0, (g.ntiles + 1) * sizeof(int)));   // zero between sorts
   HIPCHK(dalloc(&c->d_tile_list, g.ntiles));
   HIPCHK(dalloc(&c->d_tile_slot, g.ntiles));
@@ -327,6 +327,7 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   }
   a.tile_np = c->d_tile_np;
   a.tile_p0 = c->d_tile_p0;
+  a.tile_cur = c->d_tile_cur;
   a.tile_list = c->d_tile_list;
   a.tile_slot = c->d_tile_slot;
   a.dil_list = c->d_dil_list;
@@ -450,7 +451,7 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   free_particle_buffers(c);
   for (auto &g : c->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  void *ptrs[] = {c->d_mats, c->d_tile_np, c->d_tile_p0, c->d_tile_list, c->d_tile_slot,
+  void *ptrs[] = {c->d_mats, c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
                   c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst};
   for (void *p : ptrs)
@@ -501,7 +502,6 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   HIPCHK(dalloc(&c->d_ia, (size_t)NIF * cap));
   HIPCHK(dalloc(&c->d_ib, (size_t)NIF * cap));
   HIPCHK(dalloc(&c->d_key, cap));
-  HIPCHK(dalloc(&c->d_rank, cap));
   HIPCHK(dalloc(&c->d_order_tmp, cap));
   HIPCHK(dalloc(&c->d_sort_tmp, cap));
   HIPCHK(dalloc(&c->d_order, cap));

@@ -46,13 +46,13 @@ struct KArgs {
   double fxs[FX_N], fxi[FX_N];  // fixed-point scales (powers of two) and inverses
   double *xo;       // MUSL: positions at the start of the step, sorted slots (3*cap)
   int *key;         // cell key per current slot
-  int *rank;        // arrival rank within the tile
-  int4 *order_tmp;  // counting-sort scatter: (slot, pid, local cell, -) at each arrival position
-  int2 *sort_tmp;   // cell-ordered (slot, pid) of tiles too large for LDS
+  int *order_tmp;   // counting-sort scatter: slots grouped by tile, in arrival order
+  int *sort_tmp;    // cell-ordered slots of tiles too large for LDS
   int *order;       // sorted position -> current slot
   // tiles
   int *tile_np;     // [ntiles + 1] particles per tile
   int *tile_p0;     // [ntiles + 1] first sorted position of each tile
+  int *tile_cur;    // [ntiles] scatter cursor of each tile
   int *tile_list;   // active tiles (by list slot)
   int *tile_slot;   // tile -> list slot or -1
   int *dil_list;    // tiles whose owned nodes get contributions
@@ -106,10 +106,10 @@ struct kmpm_ctx {
   double *d_pa, *d_pb, *d_xo;
   float *d_fa, *d_fb;
   int *d_ia, *d_ib;
-  int *d_key, *d_rank, *d_order;
-  int4 *d_order_tmp;
-  int2 *d_sort_tmp;
-  int *d_tile_np, *d_tile_p0;
+  int *d_key, *d_order;
+  int *d_order_tmp;
+  int *d_sort_tmp;
+  int *d_tile_np, *d_tile_p0, *d_tile_cur;
   int *d_tile_list, *d_tile_slot, *d_dil_list;
   double *d_scratch;
   double *d_gmass, *d_gv, *d_gvu;

@@ -4,6 +4,8 @@
 // order; accumulation loops use explicit fma().
 #pragma once
 
+#include <cstddef>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -98,6 +100,10 @@ struct DevScalars {
   unsigned long long surr_evals;      // particle updates that used a surrogate network
   unsigned long long surr_fallbacks;  // particle updates outside the surrogate's envelope
 };
+// k_tiles appends to both tile lists with one 64-bit atomic on the pair
+static_assert(offsetof(DevScalars, tile_count) % 8 == 0 &&
+                  offsetof(DevScalars, dil_count) == offsetof(DevScalars, tile_count) + 4,
+              "tile_count and dil_count form one aligned 64-bit word");
 
 // Per-instance scalars of a batched context (independent instances packed
 // side by side on one grid, each with its own dt, time and CFL).

@@ -3,8 +3,9 @@
 // observables), and the launchers that select the tile variant.
 //
 // Step structure (USL):
-//   sort   : cell key per particle -> counting sort by tile-major cell key,
-//            ties broken by particle id (deterministic permutation)
+//   sort   : cell key per particle -> stable counting sort by tile-major
+//            cell key (ties keep the previous step's sorted order, which is
+//            ascending tag after upload: a deterministic permutation)
 //   P2G    : one workgroup per active tile; lanes = stencil nodes; each lane
 //            accumulates the particles of a cell in registers, flushes into an
 //            LDS node block of the tile; the block is written to a per-tile
@@ -52,10 +53,30 @@ constexpr int SORT_BLOCK = 256;
 constexpr int SORT_PER_WAVE = 64 * SORT_PPT;
 constexpr int SORT_PER_BLOCK = SORT_BLOCK * SORT_PPT;
 
+// Runs of equal tiles in one row of a wavefront: the run of this lane starts
+// at lane `start` and ends before lane `end`; `head` marks the first lane.
+struct TileRun {
+  int start, end;
+  bool head;
+};
+
+__device__ inline TileRun tile_run(int tile, bool valid, int lane) {
+  const int prev = __shfl_up(tile, 1, 64);
+  TileRun r;
+  r.head = valid && (lane == 0 || prev != tile);
+  const unsigned long long heads = __ballot(r.head);
+  const unsigned long long live = __ballot(valid);
+  const unsigned long long below = heads & ((2ull << lane) - 1ull);   // heads at lanes <= lane
+  r.start = below ? 63 - __clzll(below) : 0;
+  const unsigned long long above = lane < 63 ? (heads >> (lane + 1)) : 0ull;
+  r.end = above ? lane + 1 + __ffsll(above) - 1 : __popcll(live);
+  return r;
+}
+
 template <bool FULL>
 __device__ inline void count_rows(const KArgs &a, int i0, int lane) {
   const int cshift = __ffs(a.tg.cpt) - 1;
-  int key[SORT_PPT], tile[SORT_PPT], base[SORT_PPT], start[SORT_PPT];
+  int key[SORT_PPT], tile[SORT_PPT];
 #pragma unroll
   for (int k = 0; k < SORT_PPT; k++) {
     const int i = i0 + k * 64;
@@ -66,24 +87,8 @@ __device__ inline void count_rows(const KArgs &a, int i0, int lane) {
     tile[k] = FULL || i0 + k * 64 < a.np ? key[k] >> cshift : -1;
 #pragma unroll
   for (int k = 0; k < SORT_PPT; k++) {
-    const bool valid = FULL || i0 + k * 64 < a.np;
-    const int prev = __shfl_up(tile[k], 1, 64);
-    const bool head = valid && (lane == 0 || prev != tile[k]);
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long live = __ballot(valid);
-    // run of this lane: [start, end)
-    const unsigned long long below = heads & ((2ull << lane) - 1ull);   // heads at lanes <= lane
-    start[k] = below ? 63 - __clzll(below) : 0;
-    const unsigned long long above = lane < 63 ? (heads >> (lane + 1)) : 0ull;
-    const int end = above ? lane + 1 + __ffsll(above) - 1 : __popcll(live);
-    base[k] = 0;
-    if (head) base[k] = atomicAdd(&a.tile_np[tile[k]], end - lane);
-  }
-#pragma unroll
-  for (int k = 0; k < SORT_PPT; k++) {
-    const int i = i0 + k * 64;
-    const int b = __shfl(base[k], start[k], 64);
-    if (FULL || i < a.np) a.rank[i] = b + (lane - start[k]);
+    const TileRun r = tile_run(tile[k], FULL || i0 + k * 64 < a.np, lane);
+    if (r.head) atomicAdd(&a.tile_np[tile[k]], r.end - lane);
   }
   if (a.ninst) {   // batched contexts: the particle's stencil must stay in its instance
 #pragma unroll
@@ -98,9 +103,9 @@ __device__ inline void count_rows(const KArgs &a, int i0, int lane) {
   }
 }
 
-// Counts particles per tile and gives each its arrival rank in the tile.
-// Particles are stored in the previous step's sorted order, so equal tiles
-// come in runs: one atomic per run of a tile inside a wavefront's row.
+// Counts particles per tile.  Particles are stored in the previous step's
+// sorted order, so equal tiles come in runs: one atomic per run of a tile
+// inside a wavefront's row.
 __global__ __launch_bounds__(SORT_BLOCK) void k_count(KArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.sc->tile_count = 0;
@@ -114,55 +119,45 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_count(KArgs a) {
 }
 
 template <bool FULL>
-__device__ inline void scatter_rows(const KArgs &a, int i0) {
-  const int cpt = a.tg.cpt, cshift = __ffs(cpt) - 1;
-  int key[SORT_PPT], rank[SORT_PPT], pid[SORT_PPT], p0[SORT_PPT];
-#pragma unroll
-  for (int k = 0; k < SORT_PPT; k++) {
-    const int i = FULL ? i0 + k * 64 : min(i0 + k * 64, a.np - 1);
-    key[k] = a.key[i];
-    rank[k] = a.rank[i];
-    pid[k] = IFLD(a.ia, PID, i);
-  }
-#pragma unroll
-  for (int k = 0; k < SORT_PPT; k++) p0[k] = a.tile_p0[key[k] >> cshift];
+__device__ inline void scatter_rows(const KArgs &a, int i0, int lane) {
+  const int cshift = __ffs(a.tg.cpt) - 1;
+  int tile[SORT_PPT], base[SORT_PPT], start[SORT_PPT];
 #pragma unroll
   for (int k = 0; k < SORT_PPT; k++) {
     const int i = i0 + k * 64;
-    if (FULL || i < a.np)
-      a.order_tmp[p0[k] + rank[k]] = int4{i, pid[k], key[k] & (cpt - 1), 0};
+    tile[k] = a.key[FULL ? i : min(i, a.np - 1)] >> cshift;
+  }
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) {
+    const TileRun r = tile_run(tile[k], FULL || i0 + k * 64 < a.np, lane);
+    start[k] = r.start;
+    base[k] = 0;
+    if (r.head) base[k] = atomicAdd(&a.tile_cur[tile[k]], r.end - lane);
+  }
+#pragma unroll
+  for (int k = 0; k < SORT_PPT; k++) {
+    const int i = i0 + k * 64;
+    const int pos = __shfl(base[k], start[k], 64) + (lane - start[k]);
+    if (FULL || i < a.np) a.order_tmp[pos] = i;
   }
 }
 
-// Scatter to the arrival position inside the tile's range, with the pid and
-// the cell inside the tile beside the slot in one 16-byte store.
+// Groups the slots by tile: each run of a tile in a wavefront's row takes a
+// range of the tile's positions from its cursor.  The order inside a tile
+// is settled by k_tile_sort.
 __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(KArgs a) {
+  const int lane = threadIdx.x & 63;
   const int w0 = blockIdx.x * SORT_PER_BLOCK + (threadIdx.x >> 6) * SORT_PER_WAVE;
   if (w0 >= a.np) return;
-  if (w0 + SORT_PER_WAVE <= a.np) scatter_rows<true>(a, w0 + (threadIdx.x & 63));
-  else scatter_rows<false>(a, w0 + (threadIdx.x & 63));
-}
-
-// Appends the flagged lanes' values to a list with one atomic per wavefront.
-__device__ inline void wave_append(bool flag, int value, int *count, int *list, int *slot_out) {
-  const unsigned long long m = __ballot(flag);
-  if (!m) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll(m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(m));
-  base = __shfl(base, leader, 64);
-  const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
-  if (flag) {
-    list[slot] = value;
-    if (slot_out) *slot_out = slot;
-  }
+  if (w0 + SORT_PER_WAVE <= a.np) scatter_rows<true>(a, w0 + lane, lane);
+  else scatter_rows<false>(a, w0 + lane, lane);
 }
 
 // Active tiles (particles in the tile) and the tiles whose owned nodes lie in
 // the node block of an active tile: the active tile itself and its
-// neighbours at -1 (and +1 for cubic) per axis.  Clears the per-tile counts
-// for the next sort (the scan has consumed them).
+// neighbours at -1 (and +1 for cubic) per axis.  Sets the scatter cursors
+// and clears the per-tile counts for the next sort (the scan has consumed
+// them).
 __global__ void k_tiles(KArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const TileGeom &g = a.tg;
@@ -170,7 +165,9 @@ __global__ void k_tiles(KArgs a) {
   if (t <= g.ntiles) a.tile_np[t] = 0;
   bool active = false, hit = false;
   if (in) {
-    active = a.tile_p0[t + 1] > a.tile_p0[t];
+    const int p0 = a.tile_p0[t];
+    a.tile_cur[t] = p0;
+    active = a.tile_p0[t + 1] > p0;
     const int tz = t % g.nt[2], ty = (t / g.nt[2]) % g.nt[1], tx = t / (g.nt[2] * g.nt[1]);
     // all neighbours' ranges in flight at once (no early exit)
     const int up = a.S1 == 4 ? 1 : 0;
@@ -188,28 +185,40 @@ __global__ void k_tiles(KArgs a) {
           hit |= a.tile_p0[u + 1] > a.tile_p0[u];
         }
   }
-  int slot = -1;
-  wave_append(active, t, &a.sc->tile_count, a.tile_list, &slot);
-  if (in) a.tile_slot[t] = slot;
-  wave_append(hit, t, &a.sc->dil_count, a.dil_list, nullptr);
+  // one atomic per wavefront appends to both lists (the counts are the low
+  // and high halves of one 64-bit word, neither exceeds ntiles)
+  const unsigned long long ma = __ballot(active), mh = __ballot(hit);
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(ma | mh | 1ull) - 1;
+  unsigned long long base = 0;
+  if (lane == leader && (ma | mh))
+    base = atomicAdd((unsigned long long *)&a.sc->tile_count,
+                     (unsigned long long)__popcll(ma) | ((unsigned long long)__popcll(mh) << 32));
+  base = __shfl(base, leader, 64);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int aslot = (int)(base & 0xffffffffull) + __popcll(ma & below);
+  const int hslot = (int)(base >> 32) + __popcll(mh & below);
+  if (active) a.tile_list[aslot] = t;
+  if (in) a.tile_slot[t] = active ? aslot : -1;
+  if (hit) a.dil_list[hslot] = t;
 }
 
-// Orders the particles of each active tile by cell, then by particle id (the
-// arrival order of the scatter is not deterministic): a counting sort over
-// the tile's cells in LDS, then each particle's rank among the particles of
-// its cell with smaller ids.  One wavefront per tile, so that many tiles are
-// in flight per CU.  A tile of up to TILE_SORT_CAP particles is read once into
-// registers (TILE_SORT_KPL per lane), its ids placed by cell in LDS, and each
-// lane ranks its own particles; a larger one places (slot, id) pairs in its
-// own range of sort_tmp and ranks from there.  The next tile's range is
-// fetched while the current one is ordered.  Dynamic LDS:
-// tile_sort_lds_bytes(cpt).
+// Orders the slots of each active tile by cell, ties by slot: a stable
+// counting sort, since the slots are the previous step's sorted order (the
+// upload order, ascending tag, before the first step).  A counting sort over
+// the tile's cells in LDS, then each slot's rank among the smaller slots of
+// its cell.  One wavefront per tile, so that many tiles are in flight per CU.
+// A tile of up to TILE_SORT_CAP particles is read once into registers
+// (TILE_SORT_KPL per lane) and its slots placed by cell in LDS; a larger one
+// places them in its own range of sort_tmp and finds a position's cell by
+// binary search of the offsets.  The next tile's range is fetched while the
+// current one is ordered.  Dynamic LDS: tile_sort_lds_bytes(cpt).
 __global__ __launch_bounds__(64) void k_tile_sort(KArgs a) {
   extern __shared__ int lds[];
-  const int cpt = a.tg.cpt;
+  const int cpt = a.tg.cpt, cmask = cpt - 1;
   int *off = lds;                       // [cpt + 1] first position of each cell
   int *cur = off + cpt + 1;             // [cpt] counts, then placement cursors
-  int *s_pid = cur + cpt;               // [TILE_SORT_CAP] ids placed by cell
+  int *s_slot = cur + cpt;              // [TILE_SORT_CAP] slots placed by cell
   const int lane = threadIdx.x;
   const int per = (cpt + 63) >> 6;      // cells per lane in the scan
   const int ntile = a.sc->tile_count;
@@ -222,14 +231,16 @@ __global__ __launch_bounds__(64) void k_tile_sort(KArgs a) {
   }
   for (; li < ntile; li += gridDim.x) {
     const int n = p1 - p0;
-    const int4 *in = a.order_tmp + p0;
+    const int *in = a.order_tmp + p0;
     const bool inreg = n <= TILE_SORT_CAP;
-    int4 e[TILE_SORT_KPL];
+    int sl[TILE_SORT_KPL], ce[TILE_SORT_KPL];
 #pragma unroll
     for (int k = 0; k < TILE_SORT_KPL; k++) {
       const int j = k * 64 + lane;
-      e[k] = inreg && j < n ? in[j] : int4{0, 0, -1, 0};
+      sl[k] = inreg && j < n ? in[j] : -1;
     }
+#pragma unroll
+    for (int k = 0; k < TILE_SORT_KPL; k++) ce[k] = sl[k] >= 0 ? a.key[sl[k]] & cmask : -1;
     // the next tile's range, in flight during this tile's ordering
     const int nli = li + gridDim.x;
     const int nt = nli < ntile ? a.tile_list[nli] : 0;
@@ -239,9 +250,9 @@ __global__ __launch_bounds__(64) void k_tile_sort(KArgs a) {
     if (inreg) {
 #pragma unroll
       for (int k = 0; k < TILE_SORT_KPL; k++)
-        if (e[k].z >= 0) atomicAdd(&cur[e[k].z], 1);
+        if (ce[k] >= 0) atomicAdd(&cur[ce[k]], 1);
     } else {
-      for (int j = lane; j < n; j += 64) atomicAdd(&cur[in[j].z], 1);
+      for (int j = lane; j < n; j += 64) atomicAdd(&cur[a.key[in[j]] & cmask], 1);
     }
     __syncthreads();
     {   // exclusive scan of the counts, `per` consecutive cells per lane
@@ -272,21 +283,21 @@ __global__ __launch_bounds__(64) void k_tile_sort(KArgs a) {
     if (inreg) {
 #pragma unroll
       for (int k = 0; k < TILE_SORT_KPL; k++)
-        if (e[k].z >= 0) s_pid[atomicAdd(&cur[e[k].z], 1)] = e[k].y;
+        if (ce[k] >= 0) s_slot[atomicAdd(&cur[ce[k]], 1)] = sl[k];
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < TILE_SORT_KPL; k++)
-        if (e[k].z >= 0) {
-          const int s = off[e[k].z], q1 = off[e[k].z + 1];
+        if (ce[k] >= 0) {
+          const int s = off[ce[k]], q1 = off[ce[k] + 1];
           int r = 0;
-          for (int q = s; q < q1; q++) r += s_pid[q] < e[k].y ? 1 : 0;
-          a.order[p0 + s + r] = e[k].x;
+          for (int q = s; q < q1; q++) r += s_slot[q] < sl[k] ? 1 : 0;
+          a.order[p0 + s + r] = sl[k];
         }
     } else {
-      int2 *g = a.sort_tmp + p0;
+      int *g = a.sort_tmp + p0;
       for (int j = lane; j < n; j += 64) {
-        const int4 v = in[j];
-        g[atomicAdd(&cur[v.z], 1)] = int2{v.x, v.y};
+        const int v = in[j];
+        g[atomicAdd(&cur[a.key[v] & cmask], 1)] = v;
       }
       __threadfence_block();
       __syncthreads();
@@ -297,11 +308,11 @@ __global__ __launch_bounds__(64) void k_tile_sort(KArgs a) {
           if (off[mid] <= j) lo = mid;
           else hi = mid;
         }
-        const int2 me = g[j];
+        const int me = g[j];
         const int s = off[lo], q1 = off[lo + 1];
         int r = 0;
-        for (int q = s; q < q1; q++) r += g[q].y < me.y ? 1 : 0;
-        a.order[p0 + s + r] = me.x;
+        for (int q = s; q < q1; q++) r += g[q] < me ? 1 : 0;
+        a.order[p0 + s + r] = me;
       }
     }
     p0 = np0;
@@ -552,13 +563,13 @@ hipError_t launch_sort(kmpm_ctx *c) {
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, c->scan_tmp_bytes, a.tile_np,
                                                   a.tile_p0, (int)(nt + 1), c->stream);
   if (e != hipSuccess) return e;
-  if (a.np > 0)
+  hipLaunchKernelGGL(k_tiles, dim3(nblk(nt + 1, 256)), dim3(256), 0, c->stream, a);
+  if (a.np > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblk(a.np, SORT_PER_BLOCK)), dim3(SORT_BLOCK), 0,
                        c->stream, a);
-  hipLaunchKernelGGL(k_tiles, dim3(nblk(nt + 1, 256)), dim3(256), 0, c->stream, a);
-  if (a.np > 0)
-    hipLaunchKernelGGL(k_tile_sort, dim3(persistent_grid(c, 16)), dim3(64),
+    hipLaunchKernelGGL(k_tile_sort, dim3(persistent_grid(c, 32)), dim3(64),
                        tile_sort_lds_bytes(a.tg.cpt), c->stream, a);
+  }
   return hipGetLastError();
 }
 

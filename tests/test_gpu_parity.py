@@ -226,6 +226,47 @@ def test_dense_tiles_sort_and_steps():
     run_and_compare(prob, 10, 5)
 
 
+def _sort_keys(prob, T, x):
+    """Tile-major cell keys of positions x (cell index clamped to the grid)."""
+    n = N.grid_size(prob.cfg)
+    lo = np.array(prob.cfg.boxlo)
+    u = (x - lo) * (1.0 / prob.cfg.cellsize)
+    c = np.where(u >= 0.0, np.minimum(u, np.array(n) - 1.0), 0.0).astype(np.int64)
+    c[:, 2] = np.where(prob.cfg.dimension == 3, c[:, 2], 0)
+    nt = [(n[d] + T[d] - 1) // T[d] for d in range(3)]
+    tile = ((c[:, 0] // T[0]) * nt[1] + c[:, 1] // T[1]) * nt[2] + c[:, 2] // T[2]
+    loc = ((c[:, 0] % T[0]) * T[1] + c[:, 1] % T[1]) * T[2] + c[:, 2] % T[2]
+    return tile * (T[0] * T[1] * T[2]) + loc
+
+
+def test_sort_is_stable_across_steps():
+    """Ties in a cell keep the previous storage order, which is the previous
+    sort's order: each step's permutation is the stable sort, by cell key, of
+    the previous one (the first is by key, then ascending tag)."""
+    say("stable sort across 60 steps")
+    prob = P.two_disks()
+    ctx = P.context_for(prob)
+    T = ctx.tile_shape()
+    prev = prob.particles["ptag"].copy()
+    moved = 0
+    for step in range(60):
+        ctx.phase("compute_grid_weight_functions_and_gradients")
+        _, got = ctx.cells()
+        p = ctx.download()
+        keys = _sort_keys(prob, T, p["x"])[np.searchsorted(p["ptag"], prev)]
+        want = prev[np.argsort(keys, kind="stable")]
+        assert np.array_equal(got, want), step
+        moved += int((got != prev).sum())
+        prev = got
+        for ph in ("particles_to_grid", "update_grid_state", "grid_to_points_and_stress",
+                   "adjust_dt"):
+            ctx.phase(ph)
+        assert ctx.synchronize()[0] == 0
+    say("  positions changed over the steps: %d" % moved)
+    assert moved > 0
+    ctx.close()
+
+
 def test_two_disks_3000_steps_conditioned():
     """The full collision.  Linear shapes with lattice-aligned uniform motion
     amplify 1-ulp differences (derivative_linear(0) == 0 at node crossings),
