@@ -267,18 +267,18 @@ __device__ inline R p2g_stress(const R *s, int d, int e) {
 // q = val * scale is exact in FP32 (a power-of-two scale within the float
 // range) and rounded to the nearest integer (ties to even, as in FP64).  Its
 // two's-complement words are hi = floor(q / 2^32) and lo = q - hi 2^32, in
-// [0, 2^32), formed exactly in FP64.
-__device__ inline void fx_add_f(unsigned long long *dst, float val, float scale,
-                                DevScalars *sc) {
+// [0, 2^32), formed exactly in FP64.  Returns false, and adds zero, when |q|
+// reaches the flush limit (or val is NaN); the caller reports that once.
+__device__ inline bool fx_add_f(unsigned long long *dst, float val, float scale) {
   const float q = rintf(val * scale);
   const bool ok = fabsf(q) < (float)FX_FLUSH_LIMIT;
-  if (!ok) set_error(sc, KMPM_ERR_FIXED_POINT, -1);
   const float qc = ok ? q : 0.0f;
   const float hi = floorf(qc * 2.3283064365386963e-10f);   // |hi| < 2^22
   const double lo = fma(-(double)hi, 4294967296.0, (double)qc);
   const unsigned long long iv =
       ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)(unsigned)lo;
-  if (ok) atomicAdd(dst, iv);
+  atomicAdd(dst, iv);
+  return ok;
 }
 
 // P2G over the active tiles, one workgroup per tile at a time.
@@ -444,6 +444,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
           }
         }
         const int base = scl[i0] + colofs;
+        bool ok = true;
 #pragma unroll
         for (int k = 0; k < S1; k++)
 #pragma unroll
@@ -451,11 +452,12 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
             BT *dst = &blk[f * PBN + base + k];
             if constexpr (FX) {
               const int kk = f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE;
-              fx_add_f(dst, acc[k][f], fs[kk], a.sc);
+              ok &= fx_add_f(dst, acc[k][f], fs[kk]);
             } else {
               atomicAdd(dst, acc[k][f]);
             }
           }
+        if (!ok) set_error(a.sc, KMPM_ERR_FIXED_POINT, -1);
       }
     }
     __syncthreads();
