@@ -104,9 +104,9 @@ static void tend(kmpm_ctx *c) {
 }
 
 // Mixed layout <-> the FP64 field layout used on the host.
-static void split_mixed(const std::vector<double> &hd, int cap, std::vector<double> &dd,
-                        std::vector<float> &ff) {
-  for (int f = 0; f < NDF; f++)
+static void split_mixed(const std::vector<double> &hd, int cap, bool tl,
+                        std::vector<double> &dd, std::vector<float> &ff) {
+  for (int f = 0; f < (tl ? NDF_TL : NDF); f++)
     for (int i = 0; i < cap; i++) {
       const double v = hd[(size_t)f * cap + i];
       if (mx_is_double(f)) dd[(size_t)mx_dslot(f) * cap + i] = v;
@@ -114,8 +114,8 @@ static void split_mixed(const std::vector<double> &hd, int cap, std::vector<doub
     }
 }
 static void join_mixed(const std::vector<double> &dd, const std::vector<float> &ff, int cap,
-                       std::vector<double> &hd) {
-  for (int f = 0; f < NDF; f++)
+                       bool tl, std::vector<double> &hd) {
+  for (int f = 0; f < (tl ? NDF_TL : NDF); f++)
     for (int i = 0; i < cap; i++)
       hd[(size_t)f * cap + i] = mx_is_double(f) ? dd[(size_t)mx_dslot(f) * cap + i]
                                                 : (double)ff[(size_t)mx_fslot(f) * cap + i];
@@ -347,9 +347,6 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   if (!cfg || !out || nmat < 1 || nmat > KMPM_MAX_MATERIALS || !mats) return KMPM_ERR_ARG;
   *out = nullptr;
   if (cfg->method != KMPM_METHOD_ULMPM && cfg->method != KMPM_METHOD_TLMPM) return KMPM_ERR_ARG;
-  // TLMPM runs the FP64 path
-  if (cfg->method == KMPM_METHOD_TLMPM && cfg->precision != KMPM_PRECISION_FP64)
-    return KMPM_ERR_ARG;
   if (cfg->scheme != KMPM_SCHEME_USL && cfg->scheme != KMPM_SCHEME_MUSL) return KMPM_ERR_ARG;
   if (cfg->shape != KMPM_SHAPE_LINEAR && cfg->shape != KMPM_SHAPE_CUBIC_SPLINE) return KMPM_ERR_ARG;
   if (cfg->precision != KMPM_PRECISION_FP64 && cfg->precision != KMPM_PRECISION_MIXED)
@@ -488,15 +485,15 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   a.np = (int)np;
   a.cap = cap;
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
-  const int ndf = mixed ? MX_ND : (a.tl ? NDF_TL : NDF);
+  const int ndf = state_nd(mixed, a.tl);
   if (a.tl)
     for (int64_t i = 1; i < np; i++)
       if (p->solid[i] != p->solid[0]) return KMPM_ERR_ARG;   // one solid: one TL grid
   HIPCHK(dalloc(&c->d_pa, (size_t)ndf * cap));
   HIPCHK(dalloc(&c->d_pb, (size_t)ndf * cap));
   if (mixed) {
-    HIPCHK(dalloc(&c->d_fa, (size_t)MX_NF * cap));
-    HIPCHK(dalloc(&c->d_fb, (size_t)MX_NF * cap));
+    HIPCHK(dalloc(&c->d_fa, (size_t)state_nf(true, a.tl) * cap));
+    HIPCHK(dalloc(&c->d_fb, (size_t)state_nf(true, a.tl) * cap));
   }
   HIPCHK(dalloc(&c->d_xo, (size_t)3 * cap));
   HIPCHK(dalloc(&c->d_ia, (size_t)NIF * cap));
@@ -556,9 +553,9 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
     hi[(size_t)MATI * cap + i] = p->mat[i];
   }
   if (mixed) {
-    std::vector<double> dd((size_t)MX_ND * cap);
-    std::vector<float> ff((size_t)MX_NF * cap);
-    split_mixed(hd, cap, dd, ff);
+    std::vector<double> dd((size_t)state_nd(true, a.tl) * cap);
+    std::vector<float> ff((size_t)state_nf(true, a.tl) * cap);
+    split_mixed(hd, cap, a.tl, dd, ff);
     HIPCHK(hipMemcpy(c->d_pa, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_fa, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice));
     if (!set_fixed_point_scales(c, hd, cap, np)) return KMPM_ERR_FIXED_POINT;
@@ -585,8 +582,8 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
   std::vector<float> ff;
   const bool mixed = c->a.prec == KMPM_PRECISION_MIXED;
   if (mixed) {
-    dd.resize((size_t)MX_ND * cap);
-    ff.resize((size_t)MX_NF * cap);
+    dd.resize((size_t)state_nd(true, c->a.tl) * cap);
+    ff.resize((size_t)state_nf(true, c->a.tl) * cap);
     HIPCHK(hipMemcpyAsync(dd.data(), c->d_pa, dd.size() * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     HIPCHK(hipMemcpyAsync(ff.data(), c->d_fa, ff.size() * sizeof(float), hipMemcpyDeviceToHost,
@@ -598,7 +595,7 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
   HIPCHK(hipMemcpyAsync(hi.data(), c->d_ia, hi.size() * sizeof(int), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (mixed) join_mixed(dd, ff, cap, hd);
+  if (mixed) join_mixed(dd, ff, cap, c->a.tl, hd);
   auto D = [&](int f, int64_t i) { return hd[(size_t)f * cap + i]; };
   for (int64_t i = 0; i < np; i++) {
     const int64_t q = hi[(size_t)PID * cap + i];  // output index = upload index
@@ -870,8 +867,8 @@ struct SavedState {
 static int save_particles(kmpm_ctx *c, SavedState &s) {
   const KArgs &a = c->a;
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
-  s.nd = (size_t)(mixed ? MX_ND : (a.tl ? NDF_TL : NDF)) * a.cap;
-  s.nf = mixed ? (size_t)MX_NF * a.cap : 0;
+  s.nd = (size_t)state_nd(mixed, a.tl) * a.cap;
+  s.nf = (size_t)state_nf(mixed, a.tl) * a.cap;
   s.ni = (size_t)NIF * a.cap;
   HIPCHK(dalloc(&s.pa, s.nd));
   if (s.nf) HIPCHK(dalloc(&s.fa, s.nf));

@@ -37,8 +37,8 @@ struct KArgs {
   int tl;           // TLMPM: weights from x0 on the solid's grid, F += dt Fdot
   int prec;         // KMPM_PRECISION_*
   int np, cap;
-  double *pa;       // current state: FP64 NDF * cap, mixed MX_ND * cap
-  float *fa;        // current FP32 state (mixed: MX_NF * cap)
+  double *pa;       // current state: state_nd(mixed, tl) * cap doubles
+  float *fa;        // current FP32 state (mixed: state_nf(true, tl) * cap)
   int *ia;          // current int state (NIF * cap)
   double *pb;       // next state
   float *fb;

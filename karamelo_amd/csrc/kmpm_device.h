@@ -39,11 +39,30 @@ enum IField { PID = 0, MATI, NIF };
 // Mixed-precision layout (KMPM_PRECISION_MIXED): position and deformation
 // gradient stay FP64 (a double block of 12 fields: x, y, z, F00..F22), every
 // other field is FP32 (a float block of NDF - 12 fields in DField order).
+// TLMPM adds the reference position to the double block (x0, slots 12-14;
+// the weights and cell keys are computed from it) and vol0 PK1 to the float
+// block (after the common fields).
 constexpr int MX_ND = 12;
 constexpr int MX_NF = NDF - MX_ND;
-__host__ __device__ constexpr bool mx_is_double(int f) { return f <= PZ || (f >= F00 && f <= F22); }
-__host__ __device__ constexpr int mx_dslot(int f) { return f <= PZ ? f : f - F00 + 3; }
-__host__ __device__ constexpr int mx_fslot(int f) { return f < F00 ? f - VX : f - SXX + 3; }
+constexpr int MX_ND_TL = MX_ND + 3;
+constexpr int MX_NF_TL = MX_NF + 9;
+__host__ __device__ constexpr bool mx_is_double(int f) {
+  return f <= PZ || (f >= F00 && f <= F22) || (f >= X0X && f <= X0Z);
+}
+__host__ __device__ constexpr int mx_dslot(int f) {
+  return f <= PZ ? f : f <= F22 ? f - F00 + 3 : f - X0X + MX_ND;
+}
+__host__ __device__ constexpr int mx_fslot(int f) {
+  return f < F00 ? f - VX : f < NDF ? f - SXX + 3 : f - P00 + MX_NF;
+}
+static_assert(mx_fslot(VOL) == MX_NF - 1 && mx_dslot(F22) == MX_ND - 1, "mixed layout");
+// Field counts of the double and float blocks per method and precision.
+__host__ __device__ constexpr int state_nd(bool mixed, bool tl) {
+  return mixed ? (tl ? MX_ND_TL : MX_ND) : (tl ? NDF_TL : NDF);
+}
+__host__ __device__ constexpr int state_nf(bool mixed, bool tl) {
+  return mixed ? (tl ? MX_NF_TL : MX_NF) : 0;
+}
 
 // Real type of the bulk state and of the transfer arithmetic per precision.
 template <int PREC> struct Real { using T = double; using T2 = double2; };

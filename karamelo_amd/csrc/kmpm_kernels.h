@@ -139,7 +139,10 @@ __device__ inline void key_cell(const KArgs &a, int key, int c[3]) {
 }
 
 // Weight positions: current x (ULMPM) or reference x0 (TLMPM).
-__device__ inline int wpos(const KArgs &a) { return a.tl ? X0X : PX; }
+// Index in the double block (FP64 fields, or the mixed layout's double slots).
+__device__ inline int wpos(const KArgs &a) {
+  return a.tl ? (a.prec == KMPM_PRECISION_MIXED ? mx_dslot(X0X) : X0X) : PX;
+}
 
 // ---------------------------------------------------------------------------
 // P2G: staged particles, lane = stencil column, register sums per cell
@@ -213,24 +216,24 @@ template <int DIM, bool REMAP, bool TL, int PREC, class R>
 __device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R> &in) {
   if (REMAP) {
     for (int d = 0; d < DIM; d++) {
-      in.x[d] = TL ? PF(a.pb, X0X + d, j) : a.xo[(size_t)d * a.cap + j];
+      in.x[d] = TL ? LDB(X0X + d, j) : a.xo[(size_t)d * a.cap + j];
       in.v[d] = (R)LDB(VX + d, j);
     }
     in.m = (R)LDB(MASS, j);
   } else if (TL) {
     const int slot = a.order[j];
     for (int d = 0; d < DIM; d++) {
-      in.x[d] = PF(a.pa, X0X + d, slot);
+      in.x[d] = LDA(X0X + d, slot);
       in.v[d] = (R)LDA(VX + d, slot);
     }
     in.m = (R)LDA(MASS, slot);
     if (DIM == 3) {
-      for (int e = 0; e < 9; e++) in.s[e] = (R)-PF(a.pa, P00 + e, slot);
+      for (int e = 0; e < 9; e++) in.s[e] = (R)-LDA(P00 + e, slot);
     } else {
-      in.s[0] = (R)-PF(a.pa, P00, slot);
-      in.s[1] = (R)-PF(a.pa, P01, slot);
-      in.s[2] = (R)-PF(a.pa, P10, slot);
-      in.s[3] = (R)-PF(a.pa, P11, slot);
+      in.s[0] = (R)-LDA(P00, slot);
+      in.s[1] = (R)-LDA(P01, slot);
+      in.s[2] = (R)-LDA(P10, slot);
+      in.s[3] = (R)-LDA(P11, slot);
     }
   } else {
     const int slot = a.order[j];
@@ -987,11 +990,10 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
 // damaged particles, which the CFL skips).  Writes slot j of the next state.
 // ---------------------------------------------------------------------------
 
-template <int MC>
+template <int MC, int PREC>
 __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot, int j,
                                  int pid, int mat, const double *v, double &cmax,
                                  double &rmin) {
-  constexpr int PREC = KMPM_PRECISION_FP64;
   double F[9], Finv[9], R[9];
   for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot) + dt * Fdot[e];   // :1181-1182
   m3_inverse(F, Finv);                                                   // :1186
@@ -1040,12 +1042,12 @@ __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, 
 
 // TLMPM fused USL G2P: weights of x0 on the solid's grid, Fdot(v_update),
 // G2P, advance (no box check), then tl_update.
-template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
-  constexpr int PREC = KMPM_PRECISION_FP64;
+  using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
-  __shared__ double nb[2 * DIM * BN];
+  __shared__ R nb[2 * DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -1066,35 +1068,43 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
       const int slot = a.order[j];
       double x0[3], x[3], v[3];
       for (int d = 0; d < 3; d++) {
-        x0[d] = PF(a.pa, X0X + d, slot);
-        x[d] = PF(a.pa, PX + d, slot);
-        v[d] = PF(a.pa, VX + d, slot);
+        x0[d] = LDA(X0X + d, slot);
+        x[d] = LDA(PX + d, slot);
+        v[d] = LDA(VX + d, slot);
       }
-      double w[3][S1], dw[3][S1];
+      R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
-        weights_1d<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        weights_1d_as<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
-      double vu[3] = {0, 0, 0}, acc[3] = {0, 0, 0}, Fdot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vu, acc, Fdot);
+      double vu[3], acc[3], Fdot[9];
+      {
+        R vur[3] = {0, 0, 0}, accr[3] = {0, 0, 0}, Fr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, true, true, false>(nb, cl, w, dw, vur, accr, Fr);
+        for (int d = 0; d < 3; d++) {
+          vu[d] = vur[d];
+          acc[d] = accr[d];
+        }
+        for (int e = 0; e < 9; e++) Fdot[e] = Fr[e];
+      }
       const int pid = IFLD(a.ia, PID, slot);
       const int mat = IFLD(a.ia, MATI, slot);
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
       for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];                 // solid.cpp:616
       for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
       for (int d = 0; d < 3; d++) {
-        PF(a.pb, PX + d, j) = x[d];
-        PF(a.pb, VX + d, j) = v[d];
-        PF(a.pb, X0X + d, j) = x0[d];
+        STB(PX + d, j, x[d]);
+        STB(VX + d, j, v[d]);
+        STB(X0X + d, j, x0[d]);
       }
       a.key[j] = particle_key(a, x0[0], x0[1], x0[2]);
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
-      PF(a.pb, MASS, j) = PF(a.pa, MASS, slot);
-      tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+      STB(MASS, j, LDA(MASS, slot));
+      tl_update<MC, PREC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
     }
     if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }
@@ -1134,7 +1144,7 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
         v[d] = LDA(VX + d, slot);
       }
       double xw[3];
-      for (int d = 0; d < 3; d++) xw[d] = TL ? PF(a.pa, X0X + d, slot) : x[d];
+      for (int d = 0; d < 3; d++) xw[d] = TL ? LDA(X0X + d, slot) : x[d];
       R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
@@ -1155,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
       }
       const int pid = IFLD(a.ia, PID, slot);
       if (TL) {
-        for (int d = 0; d < 3; d++) PF(a.pb, X0X + d, j) = xw[d];
+        for (int d = 0; d < 3; d++) STB(X0X + d, j, xw[d]);
       } else {
         for (int d = 0; d < 3; d++) a.xo[(size_t)d * a.cap + j] = x[d];
       }
@@ -1262,12 +1272,12 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
 // TLMPM MUSL part B: Fdot from the re-mapped v with the x0 weights
 // (compute_rate_deformation_gradient_TL(true), musl.cpp:75-77), then
 // tl_update.
-template <int DIM, int S1, int TX, int TY, int TZ, int MC>
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
-  constexpr int PREC = KMPM_PRECISION_FP64;
+  using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
-  __shared__ double nb[DIM * BN];
+  __shared__ R nb[DIM * BN];
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -1285,25 +1295,29 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
       double x0[3];
-      for (int d = 0; d < 3; d++) x0[d] = PF(a.pa, X0X + d, slot);
-      double w[3][S1], dw[3][S1];
+      for (int d = 0; d < 3; d++) x0[d] = LDA(X0X + d, slot);
+      R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
-        weights_1d<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        weights_1d_as<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
-      double Fdot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, (double *)nullptr,
-                                                             (double *)nullptr, Fdot);
+      double Fdot[9];
+      {
+        R Fr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        gather_stencil<DIM, S1, EY, EZ, BN, false, true, true>(nb, cl, w, dw, (R *)nullptr,
+                                                               (R *)nullptr, Fr);
+        for (int e = 0; e < 9; e++) Fdot[e] = Fr[e];
+      }
       const int pid = IFLD(a.ia, PID, slot);
       const int mat = IFLD(a.ia, MATI, slot);
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
       double v[3];
-      for (int d = 0; d < 3; d++) v[d] = PF(a.pb, VX + d, j);
-      tl_update<MC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+      for (int d = 0; d < 3; d++) v[d] = LDB(VX + d, j);
+      tl_update<MC, PREC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
     }
     if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }

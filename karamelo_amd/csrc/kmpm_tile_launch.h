@@ -12,14 +12,15 @@ hipError_t TileLaunch<DIM, S1, V>::p2g(kmpm_ctx *c, int remap) {
   using TT = Tiles<DIM, S1, V>;
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
-  if (c->a.tl) {   // TLMPM: FP64 only
-    constexpr int PREC = KMPM_PRECISION_FP64;
-    if (remap)
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, true, PREC, true>),
-                         dim3(g), dim3(256), 0, c->stream, c->a);
-    else
-      hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, false, PREC, true>),
-                         dim3(g), dim3(256), 0, c->stream, c->a);
+  if (c->a.tl) {
+    PREC_DISPATCH(c, {
+      if (remap)
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, true, PREC, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+      else
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, false, PREC, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+    });
     return hipGetLastError();
   }
   PREC_DISPATCH(c, {
@@ -65,10 +66,10 @@ hipError_t TileLaunch<DIM, S1, V>::g2p_usl(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
   if (c->a.tl) {
-    MC_DISPATCH(c, {
-      hipLaunchKernelGGL((k_g2p_usl_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g), dim3(256),
-                         0, c->stream, c->a);
-    });
+    MC_DISPATCH(c, PREC_DISPATCH(c, {
+      hipLaunchKernelGGL((k_g2p_usl_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+    }));
     return hipGetLastError();
   }
   const bool surr = c->a.surr_on && c->a.mat_class == MC_JC_SHOCK &&
@@ -91,8 +92,10 @@ hipError_t TileLaunch<DIM, S1, V>::g2p_musl_a(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
   if (c->a.tl) {
-    hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, KMPM_PRECISION_FP64, true>),
-                       dim3(g), dim3(256), 0, c->stream, c->a);
+    PREC_DISPATCH(c, {
+      hipLaunchKernelGGL((k_g2p_musl_a<DIM, S1, TT::TX, TT::TY, TT::TZ, PREC, true>), dim3(g),
+                         dim3(256), 0, c->stream, c->a);
+    });
     return hipGetLastError();
   }
   PREC_DISPATCH(c, {
@@ -108,10 +111,10 @@ hipError_t TileLaunch<DIM, S1, V>::stress_musl_b(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   const int g = persistent_grid(c, 8);
   if (c->a.tl) {
-    MC_DISPATCH(c, {
-      hipLaunchKernelGGL((k_stress_musl_b_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC>), dim3(g),
+    MC_DISPATCH(c, PREC_DISPATCH(c, {
+      hipLaunchKernelGGL((k_stress_musl_b_tl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC>), dim3(g),
                          dim3(256), 0, c->stream, c->a);
-    });
+    }));
     return hipGetLastError();
   }
   MC_DISPATCH(c, PREC_DISPATCH(c, {

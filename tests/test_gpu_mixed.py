@@ -174,6 +174,48 @@ def test_mixed_two_disks_energy(shape):
     ctx.close()
 
 
+def test_mixed_tl_p2g_and_grid_update():
+    say("mixed TL p2g/grid")
+    prob = _randomize(P.taylor_bar(h=8e-4), sscale=1e8)
+    o = O.oracle_for(prob)
+    ctx = P.context_for(mixed(prob))
+    for ph in ("compute_grid_weight_functions_and_gradients", "particles_to_grid",
+               "update_grid_state"):
+        ctx.phase(ph)
+    for ph in ("weights", "p2g", "grid_update"):
+        o.phase(ph)
+    mg, vg, vug = ctx.grid()
+    mo, vo, vuo, fo = o.grid()
+    assert ctx.synchronize()[0] == 0
+    assert np.abs(mg - mo).max() <= TOL_GRID_MIXED * np.abs(mo).max()
+    assert np.abs(vg - vo).max() <= TOL_GRID_MIXED * np.abs(vo).max()
+    assert np.abs(vug - vuo).max() <= TOL_GRID_MIXED * np.abs(vuo).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("scheme,shape", [("usl", "cubic-spline"), ("musl", "linear")])
+def test_mixed_tl_taylor_bar(scheme, shape):
+    """TLMPM on the mixed path (FP32 state and transfers; F, its inverse,
+    polar decomposition and J in FP64) against the FP64 oracle."""
+    say("mixed TL Taylor bar: %s %s" % (scheme, shape))
+    prob = P.taylor_bar(h=8e-4, scheme=scheme, shape=shape)
+    w = run_mixed_vs_oracle(prob, 200, 100)
+    assert w["u"] <= TOL_RUN_MIXED and w["sigma"] <= TOL_RUN_MIXED
+
+
+def test_mixed_tl_run_is_bitwise_deterministic():
+    prob = P.taylor_bar(h=8e-4)
+    outs = []
+    for _ in range(2):
+        ctx = P.context_for(mixed(prob))
+        ctx.step(60)
+        assert ctx.synchronize()[0] == 0
+        outs.append(ctx.download())
+        ctx.close()
+    for f in ("x", "v", "sigma", "F", "eff_plastic_strain"):
+        assert np.array_equal(outs[0][f], outs[1][f]), f
+
+
 @pytest.mark.parametrize("scheme", ["usl", "musl"])
 def test_mixed_run_is_bitwise_deterministic(scheme):
     say("determinism: %s" % scheme)

@@ -99,12 +99,8 @@ def test_tl_sort_is_kept_after_the_first_step():
     ctx.close()
 
 
-def test_tl_rejects_mixed_precision_and_several_solids():
+def test_tl_rejects_several_solids():
     prob = P.taylor_bar(h=8e-4)
-    cfg = N.Config.from_buffer_copy(prob.cfg)
-    cfg.precision = N.PRECISION_MIXED
-    with pytest.raises(N.KmpmError):
-        N.Context(cfg, prob.mats)
     ctx = N.Context(prob.cfg, prob.mats)
     two = {k: v.copy() for k, v in prob.particles.items()}
     two["solid"][len(two["solid"]) // 2:] = 1
