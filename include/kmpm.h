@@ -345,6 +345,21 @@ int kmpm_surrogate_train(kmpm_surrogate *s, double final_loss[2]);
  * features [n][2], targets [n]. */
 int kmpm_surrogate_tuples(kmpm_surrogate *s, int net, int64_t n, uint64_t seed,
                           float *features, float *targets);
+/* Tuples at the states a context's simulation visits (the outputs of a
+ * batched run streamed as training data): slot k * stride of the context's
+ * current state, k < n = ceil(np / stride), gives the features of network
+ * `net` (net 0: its eps_p and plastic strain rate; net 1: the triaxiality
+ * of its stress and its rate), labelled by the plain functions of this
+ * surrogate's material.  Rows without such a state are NaN (net 0: eps_p
+ * below 1e-10; net 1: zero pressure or deviator, triaxiality <= -3).
+ * features [n][2], targets [n].  No counterpart in the reference. */
+int kmpm_surrogate_tuples_from_ctx(kmpm_surrogate *s, kmpm_ctx *ctx, int net, int64_t stride,
+                                   float *features, float *targets, int64_t n);
+/* Replaces rows [offset, offset + n) of network net's training set (all
+ * finite values); a following kmpm_surrogate_train continues from the
+ * current parameters on the updated set. */
+int kmpm_surrogate_set_tuples(kmpm_surrogate *s, int net, int64_t offset, const float *features,
+                              const float *targets, int64_t n);
 /* Network outputs (target units) for features [n][2]. */
 int kmpm_surrogate_eval(kmpm_surrogate *s, int net, const float *features, int64_t n,
                         float *out);

@@ -32,40 +32,28 @@ __device__ inline double u01(uint64_t seed, uint64_t i, uint64_t k) {
 
 constexpr double EPS_P_MIN = 1.0e-10;   // strength_jc.cpp:118: below it sigma_y = A
 
-// Flow-stress tuples: a deviatoric rate large enough that the trial state is
-// plastic; the returned stress then sits on the yield surface.
-__global__ void k_tuples_y(DevMat m, double q_lo, double q_hi, double lnr_hi, double d_rate,
-                           uint64_t seed, int64_t n, float *X, float *Y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double q = q_lo + (q_hi - q_lo) * u01(seed, i, 0);
-  const double ep = fmax((q * q) * (q * q), EPS_P_MIN);
-  const double lnr = lnr_hi * u01(seed, i, 1);
-  const double epsdot0 = m.strength_p[4];
-  const double epsdot = epsdot0 * exp(lnr);
+// A flow-stress tuple at (eps_p, epsdot): a deviatoric rate large enough
+// that the trial state is plastic; the returned stress then sits on the
+// yield surface.  Target NaN if it is not plastic (never for a valid d_rate).
+__device__ inline void tuple_y(const DevMat &m, double ep, double epsdot, double d_rate, float *x,
+                               float *y) {
   const double sig[6] = {0, 0, 0, 0, 0, 0};
   const double D[6] = {d_rate, -d_rate, 0, 0, 0, 0};
   double out[6], dps = 0;
   jc_strength(m, sig, D, 1.0, ep, epsdot, 0.0, out, &dps);
   const double ys = SQRT_3_OVER_2 * norm_sym(out);
-  double ratio = epsdot / epsdot0;
+  double ratio = epsdot / m.strength_p[4];
   ratio = ratio > 1.0 ? ratio : 1.0;
-  X[2 * i] = sqrtf(sqrtf((float)ep));
-  X[2 * i + 1] = __builtin_amdgcn_logf((float)ratio) * 0.69314718055994531f;
-  Y[i] = dps > 0 ? (float)(ys / m.strength_p[1]) : NAN;   // NaN: not plastic (never for valid d_rate)
+  x[0] = sqrtf(sqrtf((float)ep));
+  x[1] = __builtin_amdgcn_logf((float)ratio) * 0.69314718055994531f;
+  *y = dps > 0 ? (float)(ys / m.strength_p[1]) : NAN;
 }
 
-// Failure-strain tuples: a unit von Mises deviator and the pressure that
-// gives the sampled triaxiality; eps_f = dps / damage_init increment.
-__global__ void k_tuples_f(DevMat m, double triax_hi, double lnr_hi, uint64_t seed, int64_t n,
-                           float *X, float *Y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  // (-3, triax_hi]; -3 itself is the reset branch (damage_jc.cpp:119-122)
-  const double tr = -3.0 + 1e-6 + (triax_hi + 3.0 - 1e-6) * (1.0 - u01(seed, i, 0));
-  const double lnr = lnr_hi * u01(seed, i, 1);
+// A failure-strain tuple at triaxiality tr in (-3, ...) and rate epsdot: a
+// unit von Mises deviator and the pressure that gives tr; eps_f = dps /
+// damage_init increment.
+__device__ inline void tuple_f(const DevMat &m, double tr, double epsdot, float *x, float *y) {
   const double d4 = m.damage_p[3], epsdot0 = m.damage_p[5];
-  const double epsdot = epsdot0 * exp(lnr);
   const double a = 0.57735026918962576;   // 1/sqrt(3): sqrt(3/2) |s| ~ 1
   const double sdev[6] = {a, -a, 0, 0, 0, 0};
   const double vm = SQRT_3_OVER_2 * norm_sym(sdev);
@@ -75,9 +63,80 @@ __global__ void k_tuples_f(DevMat m, double triax_hi, double lnr_hi, uint64_t se
   const double dps = 1.0e-3;
   jc_damage(m, dinit, dmg, pH, sdev, epsdot, dps);
   const double feat_r = (d4 > 0.0 && epsdot > epsdot0) ? log(epsdot / epsdot0) : 0.0;
-  X[2 * i] = (float)tr_used;
-  X[2 * i + 1] = (float)feat_r;
-  Y[i] = dinit > 0 ? (float)log(dps / dinit) : NAN;
+  x[0] = (float)tr_used;
+  x[1] = (float)feat_r;
+  *y = dinit > 0 ? (float)log(dps / dinit) : NAN;
+}
+
+// Flow-stress tuples at sampled states: eps_p^(1/4) uniform in [q_lo, q_hi],
+// ln epsdot* uniform in [0, lnr_hi].
+__global__ void k_tuples_y(DevMat m, double q_lo, double q_hi, double lnr_hi, double d_rate,
+                           uint64_t seed, int64_t n, float *X, float *Y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q = q_lo + (q_hi - q_lo) * u01(seed, i, 0);
+  const double ep = fmax((q * q) * (q * q), EPS_P_MIN);
+  const double lnr = lnr_hi * u01(seed, i, 1);
+  tuple_y(m, ep, m.strength_p[4] * exp(lnr), d_rate, X + 2 * i, Y + i);
+}
+
+// Failure-strain tuples at sampled states: triaxiality in (-3, triax_hi]
+// (-3 itself is the reset branch, damage_jc.cpp:119-122), ln(epsdot /
+// epsdot0_d) uniform in [0, lnr_hi].
+__global__ void k_tuples_f(DevMat m, double triax_hi, double lnr_hi, uint64_t seed, int64_t n,
+                           float *X, float *Y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tr = -3.0 + 1e-6 + (triax_hi + 3.0 - 1e-6) * (1.0 - u01(seed, i, 0));
+  const double lnr = lnr_hi * u01(seed, i, 1);
+  tuple_f(m, tr, m.damage_p[5] * exp(lnr), X + 2 * i, Y + i);
+}
+
+// A field of particle slot i of a context's current state (FP64 layout or
+// the mixed layout's double and float blocks).
+__device__ inline double state_field(const KArgs &a, int f, int i) {
+  if (a.prec == KMPM_PRECISION_MIXED)
+    return mx_is_double(f) ? a.pa[(size_t)mx_dslot(f) * a.cap + i]
+                           : (double)a.fa[(size_t)mx_fslot(f) * a.cap + i];
+  return a.pa[(size_t)f * a.cap + i];
+}
+
+// Tuples at the states a simulation visits: slot k * stride of the context's
+// state gives the features of network `net` (net 0: its eps_p and rate;
+// net 1: the triaxiality of its stress, -p / (vm + 0.001 |p|) with p =
+// -tr(sigma) / 3 and vm the von Mises stress of dev(sigma), and its rate),
+// labelled by the plain functions of material m.  Features and target are
+// NaN where the particle has no such state (eps_p below 1e-10; a zero
+// pressure or deviator, or a triaxiality at or below -3).
+__global__ void k_ctx_tuples(KArgs a, DevMat m, int net, int64_t stride, int64_t n,
+                             double d_rate, float *X, float *Y) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = (int)(k * stride);
+  const double epdot = state_field(a, EPDOT, i);
+  if (net == 0) {
+    const double ep = state_field(a, EP, i);
+    if (ep >= EPS_P_MIN) {
+      tuple_y(m, ep, epdot, d_rate, X + 2 * k, Y + k);
+      return;
+    }
+  } else {
+    double s[6];
+    for (int e = 0; e < 6; e++) s[e] = state_field(a, SXX + e, i);
+    const double p = -(s[0] + s[1] + s[2]) / 3.0;
+    const double d[6] = {s[0] + p, s[1] + p, s[2] + p, s[3], s[4], s[5]};
+    const double vm = SQRT_3_OVER_2 * norm_sym(d);
+    if (p != 0.0 && vm != 0.0) {
+      const double tr = -p / (vm + 0.001 * fabs(p));
+      if (tr > -3.0) {
+        tuple_f(m, tr, epdot, X + 2 * k, Y + k);
+        return;
+      }
+    }
+  }
+  X[2 * k] = NAN;
+  X[2 * k + 1] = NAN;
+  Y[k] = NAN;
 }
 
 // Network outputs for n feature pairs (inference path of the step kernels).
@@ -544,6 +603,51 @@ int kmpm_surrogate_tuples(kmpm_surrogate *s, int net, int64_t n, uint64_t seed, 
   (void)hipFree(dX);
   (void)hipFree(dY);
   return st;
+}
+
+int kmpm_surrogate_tuples_from_ctx(kmpm_surrogate *s, kmpm_ctx *ctx, int net, int64_t stride,
+                                   float *features, float *targets, int64_t n) {
+  if (!s || !ctx || net < 0 || net > 1 || stride < 1 || n < 0) return KMPM_ERR_ARG;
+  const int64_t np = ctx->a.np;
+  if (n != (np + stride - 1) / stride || (n > 0 && (!features || !targets))) return KMPM_ERR_ARG;
+  if (n == 0) return KMPM_OK;
+  SCHK(hipStreamSynchronize(ctx->stream));   // the state the context's steps left
+  float *dX = nullptr, *dY = nullptr;
+  SCHK(hipMalloc(&dX, 2 * n * sizeof(float)));
+  if (hipMalloc(&dY, n * sizeof(float)) != hipSuccess) {
+    (void)hipFree(dX);
+    return KMPM_ERR_HIP;
+  }
+  hipLaunchKernelGGL(k_ctx_tuples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream,
+                     ctx->a, s->dmat, net, stride, n, s->d_rate, dX, dY);
+  int st = hipGetLastError() == hipSuccess ? KMPM_OK : KMPM_ERR_HIP;
+  if (!st && (hipMemcpyAsync(features, dX, 2 * n * sizeof(float), hipMemcpyDeviceToHost, s->stream) !=
+                  hipSuccess ||
+              hipMemcpyAsync(targets, dY, n * sizeof(float), hipMemcpyDeviceToHost, s->stream) !=
+                  hipSuccess ||
+              hipStreamSynchronize(s->stream) != hipSuccess))
+    st = KMPM_ERR_HIP;
+  (void)hipFree(dX);
+  (void)hipFree(dY);
+  return st;
+}
+
+int kmpm_surrogate_set_tuples(kmpm_surrogate *s, int net, int64_t offset, const float *features,
+                              const float *targets, int64_t n) {
+  if (!s || net < 0 || net > 1 || offset < 0 || n < 0 || offset + n > s->cfg.samples)
+    return KMPM_ERR_ARG;
+  if (n == 0) return KMPM_OK;
+  if (!features || !targets) return KMPM_ERR_ARG;
+  for (int64_t i = 0; i < n; i++)
+    if (!std::isfinite(features[2 * i]) || !std::isfinite(features[2 * i + 1]) ||
+        !std::isfinite(targets[i]))
+      return KMPM_ERR_ARG;
+  SCHK(hipMemcpyAsync(s->d_X[net] + 2 * offset, features, 2 * n * sizeof(float),
+                      hipMemcpyHostToDevice, s->stream));
+  SCHK(hipMemcpyAsync(s->d_Y[net] + offset, targets, n * sizeof(float), hipMemcpyHostToDevice,
+                      s->stream));
+  SCHK(hipStreamSynchronize(s->stream));
+  return KMPM_OK;
 }
 
 int kmpm_surrogate_eval(kmpm_surrogate *s, int net, const float *features, int64_t n, float *out) {

@@ -173,6 +173,9 @@ def _declare(lib):
         "kmpm_surrogate_destroy": (C.c_int, [vp]),
         "kmpm_surrogate_train": (C.c_int, [vp, _DP]),
         "kmpm_surrogate_tuples": (C.c_int, [vp, C.c_int, C.c_int64, C.c_uint64, _FP, _FP]),
+        "kmpm_surrogate_tuples_from_ctx": (C.c_int, [vp, vp, C.c_int, C.c_int64, _FP, _FP,
+                                                      C.c_int64]),
+        "kmpm_surrogate_set_tuples": (C.c_int, [vp, C.c_int, C.c_int64, _FP, _FP, C.c_int64]),
         "kmpm_surrogate_eval": (C.c_int, [vp, C.c_int, _FP, C.c_int64, _FP]),
         "kmpm_surrogate_gradient": (C.c_int, [vp, C.c_int, _FP, _FP, C.c_int64, _FP, _DP]),
         "kmpm_surrogate_params": (C.c_int, [vp, C.c_int, _FP, C.c_int, C.c_int]),
@@ -458,6 +461,26 @@ class Surrogate:
         check(self.L.kmpm_surrogate_tuples(self.h, net, n, seed, X.ctypes.data_as(_FP),
                                            Y.ctypes.data_as(_FP)), "surrogate_tuples")
         return X, Y
+
+    def tuples_from_context(self, ctx, net, stride=1):
+        """Tuples at the states of a context's particles (slots 0, stride,
+        ...), labelled by this surrogate's plain functions; NaN rows where
+        a particle has no such state."""
+        n = (ctx.np + stride - 1) // stride
+        X = np.zeros((n, 2), np.float32)
+        Y = np.zeros(n, np.float32)
+        check(self.L.kmpm_surrogate_tuples_from_ctx(self.h, ctx.h, net, stride,
+                                                    X.ctypes.data_as(_FP), Y.ctypes.data_as(_FP),
+                                                    n), "surrogate_tuples_from_ctx")
+        return X, Y
+
+    def set_tuples(self, net, offset, X, Y):
+        """Replace rows [offset, offset + len(Y)) of a network's training set."""
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        check(self.L.kmpm_surrogate_set_tuples(self.h, net, offset, X.ctypes.data_as(_FP),
+                                               Y.ctypes.data_as(_FP), len(Y)),
+              "surrogate_set_tuples")
 
     def eval(self, net, X):
         X = _f32(X)

@@ -143,6 +143,89 @@ def test_gradient_matches_oracle(net):
     s.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tuples_from_context_states(precision):
+    """Tuples streamed from a context's particle states, as a batched run's
+    outputs feed training: the features are the particles' eps_p and rate
+    (net 0) and their stress triaxiality and rate (net 1), the targets the
+    oracle's plain functions there, and the rows are NaN exactly where a
+    particle has no such state."""
+    mat = P.steel_jc()
+    prob = P.tensile_bar(h=6e-4)
+    if precision == "mixed":
+        prob = _mixed(prob)
+    p = prob.particles
+    n = prob.np
+    rng = np.random.default_rng(7)
+    ep = rng.uniform(0.0, 0.5, n)
+    ep[::7] = 0.0
+    epdot = 10.0 ** rng.uniform(-4, 4, n)
+    sig = rng.normal(scale=3e8, size=(n, 3, 3))
+    sig = 0.5 * (sig + sig.transpose(0, 2, 1))
+    sig[::5] = -1e9 * np.eye(3)              # pure pressure: no deviator
+    sig[1::5] -= 5e10 * np.eye(3)            # triaxiality below -3
+    p["eff_plastic_strain"][:] = ep
+    p["eff_plastic_strain_rate"][:] = epdot
+    p["sigma"][:] = sig.reshape(n, 9)
+    ctx = P.context_for(prob)
+    st = ctx.download()                      # the stored values (FP32 on the mixed path)
+    ep, epdot = st["eff_plastic_strain"], st["eff_plastic_strain_rate"]
+    sig = st["sigma"].reshape(n, 3, 3)
+    s = N.Surrogate(mat, samples=4096, iterations=0, batch=4096)
+    L = O.load()
+    A = mat.strength_p[1]
+    # net 0
+    X, Y = s.tuples_from_context(ctx, 0)
+    valid = ep >= 1e-10
+    assert np.array_equal(np.isnan(Y), ~valid)
+    assert np.allclose(X[valid, 0], ep[valid] ** 0.25, rtol=1e-6, atol=0)
+    lr = np.log(np.maximum(epdot / mat.strength_p[4], 1.0))
+    assert np.allclose(X[valid, 1], lr[valid], rtol=1e-5, atol=1e-6)
+    G = mat.strength_p[0]
+    for i in np.flatnonzero(valid)[:400]:
+        d = 10 * A / G
+        Dm = np.array([d, 0, 0, 0, -d, 0, 0, 0, 0.0])
+        out = np.zeros(9)
+        dps = C.c_double()
+        L.orc_jc_strength(C.byref(mat), O._dp(np.zeros(9)), O._dp(Dm), 1.0, ep[i], epdot[i], 0.0,
+                          O._dp(out), C.byref(dps))
+        ys = SQRT_3_OVER_2 * np.sqrt(np.sum(out * out))
+        assert abs(Y[i] - ys / A) <= 1e-5 * abs(ys / A), (i, Y[i], ys / A)
+    # net 1
+    X, Y = s.tuples_from_context(ctx, 1)
+    pr = -np.trace(sig, axis1=1, axis2=2) / 3
+    dev = sig + pr[:, None, None] * np.eye(3)
+    vm = SQRT_3_OVER_2 * np.sqrt((dev * dev).sum(axis=(1, 2)))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tr = -pr / (vm + 0.001 * np.abs(pr))
+    valid = (pr != 0) & (vm > 1e-6 * np.abs(pr)) & (tr > -3)
+    assert np.array_equal(np.isnan(Y), ~valid)
+    assert valid.sum() > n // 2 and (~valid).sum() > n // 4
+    assert np.allclose(X[valid, 0], tr[valid], rtol=1e-5, atol=1e-6)
+    d4, e0d = mat.damage_p[3], mat.damage_p[5]
+    lr = np.where((d4 > 0) & (epdot > e0d), np.log(np.maximum(epdot, 1e-300) / e0d), 0.0)
+    assert np.allclose(X[valid, 1], lr[valid], rtol=1e-5, atol=1e-6)
+    for i in np.flatnonzero(valid)[:400]:
+        t = float(X[i, 0])
+        a = 1 / np.sqrt(3.0)
+        sdev = np.array([a, 0, 0, 0, -a, 0, 0, 0, 0.0])
+        v = SQRT_3_OVER_2 * np.sqrt(np.sum(sdev * sdev))
+        pH = -t * v / (1 + 0.001 * t) if t <= 0 else -t * v / (1 - 0.001 * t)
+        di, dm = C.c_double(0), C.c_double(0)
+        L.orc_jc_damage(C.byref(mat), C.byref(di), C.byref(dm), pH, O._dp(sdev), epdot[i], 1e-3)
+        ef = 1e-3 / di.value
+        assert abs(Y[i] - np.log(ef)) <= 1e-4 * max(1.0, abs(np.log(ef))), (i, Y[i], np.log(ef))
+    # the streamed rows replace part of the training set; training continues
+    keep = ~np.isnan(Y)
+    s.set_tuples(1, 0, X[keep][:1000], Y[keep][:1000])
+    with pytest.raises(N.KmpmError):
+        s.set_tuples(1, 0, X[~keep][:10], Y[~keep][:10])
+    with pytest.raises(N.KmpmError):
+        s.set_tuples(1, 4090, X[keep][:10], Y[keep][:10])
+    ctx.close()
+    s.close()
+
+
 def test_training_accuracy():
     s = trained()
     for net, tol in ((0, 1e-3), (1, 1e-2)):
