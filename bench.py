@@ -211,6 +211,7 @@ def run_batched(steps, warmup, graph_steps=10, n_inst=256):
         raise RuntimeError("batched: device error %d (particle %d)" % (st, tag))
     dt, tm = ctx.instance_time()
     tile = ctx.tile_shape()
+    streamed = stream_tuples(ctx)
     ctx.close()
     secs = t1 - t0
     return {"workload": "taylor_plates_2d_%dx" % n_inst, "instances": n_inst, "particles": npart,
@@ -223,7 +224,40 @@ def run_batched(steps, warmup, graph_steps=10, n_inst=256):
             "phases_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in phases.items() if v[1]},
             "instance_dt_range_s": [float(dt.min()), float(dt.max())],
             "autotune": {"tile": list(tile), "variant": variant,
-                         "ms_per_step_per_variant": [round(float(v), 4) for v in tune_ms]}}
+                         "ms_per_step_per_variant": [round(float(v), 4) for v in tune_ms]},
+            "streamed_tuples": streamed}
+
+
+def stream_tuples(ctx, stride=16, rows=1 << 15, iterations=6000):
+    """The batched run's states as surrogate training data (outside the
+    timed region): every `stride`-th particle's state gives a tuple of each
+    network of a steel JC surrogate, labelled by the plain functions; the
+    tuples inside the networks' envelope replace rows of its training set,
+    and training continues on them."""
+    import numpy as np
+    from karamelo_amd import native as N
+    from karamelo_amd import problems as P
+    t0 = time.perf_counter()
+    s = N.Surrogate(P.steel_jc(), samples=1 << 16, iterations=iterations, batch=4096)
+    lo, hi = s.envelope()
+    out = {"stride": stride, "rows": {}, "in_envelope": {}}
+    for net in (0, 1):
+        X, Y = s.tuples_from_context(ctx, net, stride)
+        ok = np.isfinite(Y)
+        inside = ok & np.all((X >= lo[2 * net:2 * net + 2]) & (X <= hi[2 * net:2 * net + 2]), axis=1)
+        k = int(min(inside.sum(), rows))
+        if k:
+            s.set_tuples(net, 0, X[inside][:k], Y[inside][:k])
+        out["rows"]["net%d" % net] = int(ok.sum())
+        out["in_envelope"]["net%d" % net] = int(inside.sum())
+    t1 = time.perf_counter()
+    loss = s.train()
+    t2 = time.perf_counter()
+    s.close()
+    out.update({"stream_seconds": t1 - t0, "retrain_iterations": iterations,
+                "retrain_seconds": t2 - t1, "final_loss": [float(v) for v in loss]})
+    log("[batched] streamed tuples: %s" % out)
+    return out
 
 
 def roofline_of(path, phases, npart, workload):
