@@ -215,6 +215,16 @@ int kmpm_adjust_dt(kmpm_ctx *ctx);
 /* nsteps complete steps of the configured scheme (USL::run / MUSL::run),
  * stream-ordered, no host synchronisation inside. */
 int kmpm_step(kmpm_ctx *ctx, int64_t nsteps);
+/* RunTime::command (run_time.cpp:31-49) with the scheme loop's end test
+ * (usl.cpp:43-93, musl.cpp:43-94): time_end = time + duration (duration as
+ * the input script's parser gives it); steps run while the time is below
+ * the loop condition "time<" + to_string(time_end), whose literal the
+ * script parser reads with stof, and the loop breaks after a step whose
+ * time exceeds time_end.  At most max_steps steps; *steps_done (may be
+ * NULL) receives their number.  The time is read after every step (one
+ * host synchronisation per step); a device error ends the loop and is
+ * reported by kmpm_synchronize.  Not for batched contexts. */
+int kmpm_run_time(kmpm_ctx *ctx, double duration, int64_t max_steps, int64_t *steps_done);
 /* Waits for the stream and returns the sticky device error (first offending
  * particle tag in *ptag_out, may be NULL). */
 int kmpm_synchronize(kmpm_ctx *ctx, int64_t *ptag_out);

@@ -2,6 +2,7 @@
 // phases and the stream-ordered step loop.
 #include <algorithm>
 #include <cstddef>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1048,6 +1049,32 @@ int kmpm_step_graph(kmpm_ctx *c, int64_t nsteps, int64_t graph_steps) {
     nsteps -= graph_steps;
   }
   return kmpm_step(c, nsteps);
+}
+
+int kmpm_run_time(kmpm_ctx *c, double duration, int64_t max_steps, int64_t *steps_done) {
+  if (steps_done) *steps_done = 0;
+  if (!c || max_steps < 0 || c->a.ninst > 0) return KMPM_ERR_ARG;
+  DevScalars sc;
+  auto read = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return KMPM_OK;
+  };
+  int st = read();
+  if (st) return st;
+  const double time_end = duration + sc.time;          // run_time.cpp:43
+  char lit[512];                                       // std::to_string(double): "%f"
+  std::snprintf(lit, sizeof(lit), "%f", time_end);
+  const double cond = (double)std::strtof(lit, nullptr);   // input.cpp:407-411 (stof)
+  int64_t n = 0;
+  while (n < max_steps && sc.time < cond && sc.error == 0) {
+    if ((st = kmpm_step(c, 1))) break;
+    n++;
+    if ((st = read())) break;
+    if (sc.time > time_end) break;                     // usl.cpp:84-88
+  }
+  if (steps_done) *steps_done = n;
+  return st;
 }
 
 int kmpm_get_time(kmpm_ctx *c, double *dt, double *time, int64_t *step) {

@@ -154,6 +154,7 @@ def _declare(lib):
         "kmpm_update_stress": (C.c_int, [vp]),
         "kmpm_adjust_dt": (C.c_int, [vp]),
         "kmpm_step": (C.c_int, [vp, C.c_int64]),
+        "kmpm_run_time": (C.c_int, [vp, C.c_double, C.c_int64, P(C.c_int64)]),
         "kmpm_synchronize": (C.c_int, [vp, P(C.c_int64)]),
         "kmpm_set_instances": (C.c_int, [vp, C.c_int, P(C.c_int32), P(C.c_int32)]),
         "kmpm_instance_time": (C.c_int, [vp, C.c_int, _DP, _DP]),
@@ -308,6 +309,12 @@ class Context:
 
     def step(self, n):
         check(self.L.kmpm_step(self.h, int(n)), "step")
+
+    def run_time(self, duration, max_steps=1 << 40):
+        """run_time(duration) of an input script; returns the steps run."""
+        n = C.c_int64()
+        check(self.L.kmpm_run_time(self.h, float(duration), int(max_steps), C.byref(n)), "run_time")
+        return n.value
 
     def step_graph(self, n, graph_steps):
         """n steps as replays of a captured graph of `graph_steps` steps."""

@@ -267,6 +267,74 @@ def test_sort_is_stable_across_steps():
     ctx.close()
 
 
+def _run_time_rule(t0, duration, times):
+    """Steps the reference's run_time(duration) takes, given the time after
+    each step: the loop runs while time < stof(to_string(t0 + duration))
+    and breaks after a step whose time exceeds t0 + duration."""
+    t_end = duration + t0
+    cond = float(np.float32("%f" % t_end))
+    t, n = t0, 0
+    while t < cond:
+        t = times(n)
+        n += 1
+        if t > t_end:
+            break
+    return n
+
+
+def test_run_time_follows_the_reference_loop():
+    """run_time(duration) against the reference's loop rule: with set_dt
+    (two disks) the step count follows from the time sums alone; with the
+    CFL dt (tensile bar) from the same rule applied step by step on a
+    second context.  The states then match the oracle at that count."""
+    say("run_time")
+    prob = P.two_disks()
+    dt = prob.cfg.dt
+    acc = [0.0]
+
+    def sums(k):
+        acc[0] += dt
+        return acc[0]
+    duration = float(np.float32(0.05))            # the script literal 0.05
+    want = _run_time_rule(0.0, duration, sums)
+    ctx = P.context_for(prob)
+    got = ctx.run_time(duration)
+    assert ctx.synchronize()[0] == 0
+    assert got == want, (got, want)
+    _, t, steps = ctx.time()
+    assert steps == got and abs(t - acc[0]) <= 1e-15
+    o = O.oracle_for(prob)
+    assert o.step(got) == 0
+    pg, po = ctx.download(), o.download()
+    floors = field_floors(prob)
+    for f in FIELDS:
+        assert rel_l2(pg[f], po[f], floors[f]) <= TOL_STEP, f
+    ctx.close()
+    # CFL dt, after a few steps (run_time starts from the current time)
+    prob = P.tensile_bar(h=6e-4)
+    a, b = P.context_for(prob), P.context_for(prob)
+    a.step(5)
+    b.step(5)
+    _, t0, _ = b.time()
+    duration = 40 * b.time()[0]
+
+    def by_hand(k):
+        b.step(1)
+        return b.time()[1]
+    want = _run_time_rule(t0, duration, by_hand)
+    got = a.run_time(duration)
+    assert got == want and got > 30, (got, want)
+    (_, ta, sa), (_, tb, sb) = a.time(), b.time()
+    assert sa == sb and abs(ta - tb) <= 1e-10 * tb
+    pa, pb = a.download(), b.download()
+    floors = field_floors(prob)
+    for f in FIELDS:
+        assert rel_l2(pa[f], pb[f], floors[f]) <= TOL_STEP, f
+    assert a.run_time(duration, max_steps=3) == 3
+    a.close()
+    b.close()
+
+
 def test_two_disks_3000_steps_conditioned():
     """The full collision.  Linear shapes with lattice-aligned uniform motion
     amplify 1-ulp differences (derivative_linear(0) == 0 at node crossings),
