@@ -335,6 +335,49 @@ def test_run_time_follows_the_reference_loop():
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_empty_and_single_particle_contexts(precision):
+    """No particles: steps run, the time advances and downloads are empty.
+    One particle: 20 steps against the oracle (FP64: every field; mixed:
+    position and velocity within 1e-3).  An isolated particle is the mixed
+    path's worst case for stress: every node of its stencil has a small
+    mass, and the fixed-point momentum quantum (set by the material's wave
+    speed) is then a large relative error of the node velocity, which the
+    velocity gradient turns into stress noise."""
+    say("empty and single-particle contexts: %s" % precision)
+    base = P.tensile_bar(h=6e-4)
+    if precision == "mixed":
+        cfg = N.Config.from_buffer_copy(base.cfg)
+        cfg.precision = N.PRECISION_MIXED
+        base = dataclasses.replace(base, cfg=cfg)
+    empty = dataclasses.replace(base, particles={k: v[:0].copy() for k, v in base.particles.items()})
+    ctx = P.context_for(empty)
+    ctx.step(5)
+    assert ctx.synchronize()[0] == 0
+    dt, t, steps = ctx.time()
+    assert steps == 5 and t > 0
+    assert len(ctx.download()["x"]) == 0
+    assert ctx.energy() == (0.0, 0.0)
+    ctx.close()
+    mid = len(base.particles["x"]) // 2
+    one = dataclasses.replace(base, particles={k: v[mid:mid + 1].copy()
+                                               for k, v in base.particles.items()})
+    one.particles["v"][:] = (3.0, -2.0, 1.0)
+    ctx = P.context_for(one)
+    ref = dataclasses.replace(one, cfg=N.Config.from_buffer_copy(one.cfg))
+    ref.cfg.precision = N.PRECISION_FP64
+    o = O.oracle_for(ref)
+    ctx.step(20)
+    assert o.step(20) == 0
+    assert ctx.synchronize()[0] == 0
+    pg, po = ctx.download(), o.download()
+    tol = TOL_STEP if precision == "fp64" else 1e-3
+    floors = field_floors(one)
+    for f in ("x", "v", "sigma", "F") if precision == "fp64" else ("x", "v"):
+        assert rel_l2(pg[f], po[f], floors[f]) <= tol, f
+    ctx.close()
+
+
 def test_two_disks_3000_steps_conditioned():
     """The full collision.  Linear shapes with lattice-aligned uniform motion
     amplify 1-ulp differences (derivative_linear(0) == 0 at node crossings),
