@@ -812,12 +812,41 @@ __device__ inline double update_F(double dt, const double *L, double *F) {
   return det3(F);
 }
 
+// The constitutive update in the arithmetic of the path: FP64, or FP32 on
+// the mixed path (whose state is stored in FP32), with J and rho in FP64.
+template <int MC, int PREC>
+__device__ inline void stress_update(const DevMat &m, double dt, double h, double J, double rho,
+                                     const double *D, double *sig, double *eel, double &ep,
+                                     double &epdot, double &dinit, double &dmg) {
+  if constexpr (PREC == KMPM_PRECISION_MIXED) {
+    float Df[6], sf[6], ef[6];
+    for (int e = 0; e < 6; e++) {
+      Df[e] = (float)D[e];
+      sf[e] = (float)sig[e];
+      ef[e] = (float)eel[e];
+    }
+    float epf = (float)ep, epdf = (float)epdot, dif = (float)dinit, dmf = (float)dmg;
+    update_stress_point<MC, float>(m, (float)dt, (float)h, J, rho, Df, sf, ef, epf, epdf, dif,
+                                   dmf);
+    for (int e = 0; e < 6; e++) {
+      sig[e] = sf[e];
+      eel[e] = ef[e];
+    }
+    ep = epf;
+    epdot = epdf;
+    dinit = dif;
+    dmg = dmf;
+  } else {
+    update_stress_point<MC>(m, dt, h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  }
+}
+
 // D = sym(L) (solid.cpp:1239), stress update (solid.cpp:1246-1374) and the
 // particle's CFL wave speed (:1376-1385); returns -1 for fully damaged
 // particles (skipped by the CFL maximum).
 // SURR: the JC + shock update with the learned material functions; every
 // lane of the wave calls it (live == false for lanes without a particle).
-template <int MC, bool SURR = false>
+template <int MC, int PREC, bool SURR = false>
 __device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
                                           double *sig, double *eel, double &ep, double &epdot,
                                           double &dinit, double &dmg, int mat, const double *v,
@@ -837,7 +866,7 @@ __device__ inline double stress_and_speed(const KArgs &a, double dt, const doubl
     update_stress_jc_surrogate(m, *sd, live, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg,
                                *cnt);
   else
-    update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+    stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   if (dmg >= 1.0) return -1.0;
   const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
   const double c = sqrt(m.cfl_k43g / rho) + vm;
@@ -955,7 +984,7 @@ __global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
       }
       double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
       double dinit = LDA(DINIT, slot);
-      const double c = stress_and_speed<MC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
+      const double c = stress_and_speed<MC, PREC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
                                                   mat, v, pid, live, sd, &scnt);
       if (!live) return;
       if (c > cmax) cmax = c;
@@ -1012,7 +1041,7 @@ __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, 
     eel[e] = LDA(EXX + e, slot);
   }
   double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot), dinit = LDA(DINIT, slot);
-  update_stress_point<MC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   double S[9] = {sig[0], sig[3], sig[4], sig[3], sig[1], sig[5], sig[4], sig[5], sig[2]};
   double P[9];
   pk1(vol0, J, R, S, Finv, P);
@@ -1253,7 +1282,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
       double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
       double dinit = LDA(DINIT, slot);
       const double c =
-          stress_and_speed<MC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
+          stress_and_speed<MC, PREC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
         STB(SXX + e, j, sig[e]);

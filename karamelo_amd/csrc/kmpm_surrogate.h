@@ -64,115 +64,129 @@ __device__ inline void surrogate_count_flush(const SurrCount &c, DevScalars *sc)
 }
 
 // Solid::update_stress eos-strength branch (solid.cpp:1292-1374) for the
-// JC + shock class.  Every lane of the wave calls it; `live` is false for
-// lanes without a particle (their results are discarded).
+// JC + shock class on the mixed path.  Every lane of the wave calls it;
+// `live` is false for lanes without a particle (their results are
+// discarded).  The pressure is formed in FP64 from J (a small compression
+// cancels in rho / rho0 - 1); the rest in FP32, the arithmetic of the mixed
+// state, except the plain-function fallbacks, which stay FP64.
 __device__ inline void update_stress_jc_surrogate(const DevMat &m, const SurrogateDev &sd,
                                                   bool live, double dt, double h, double J,
                                                   double rho, const double *D, double *sig,
                                                   double *eel, double &ep, double &epdot,
                                                   double &dinit, double &dmg, SurrCount &cnt) {
-  const double pH = shock_pressure(m, J, rho, dmg, D, h);
+  const float pH = (float)shock_pressure<double>(m, J, rho, dmg, D, h);
+  const float dtf = (float)dt;
+  float Df[6];
+  for (int e = 0; e < 6; e++) Df[e] = (float)D[e];
+  float epf = (float)ep, epdf = (float)epdot, dif = (float)dinit, dmf = (float)dmg;
   // ---- strength (strength_jc.cpp:100-184) -------------------------------
-  const double G_ = m.strength_p[0], A = m.strength_p[1], epsdot0 = m.strength_p[4];
-  const bool dead = dmg >= 1.0;
-  double ratio = epdot / epsdot0;
-  ratio = ratio > 1.0 ? ratio : 1.0;
-  const float fy0 = sqrtf(sqrtf((float)ep));
-  const float fy1 = __builtin_amdgcn_logf((float)ratio) * 0.69314718055994531f;
-  const bool virgin = ep < 1.0e-10 && ratio == 1.0;
-  const bool use_y = live && !dead && !virgin && ep >= 1.0e-10 && ep <= sd.ep_hi &&
-                     (double)fy1 <= sd.lnr_y_hi;
+  const float G_ = (float)m.strength_p[0], A = (float)m.strength_p[1],
+              epsdot0 = (float)m.strength_p[4];
+  const bool dead = dmf >= 1.0f;
+  float ratio = __fdividef(epdf, epsdot0);
+  ratio = ratio > 1.0f ? ratio : 1.0f;
+  const float fy0 = sqrtf(sqrtf(epf));
+  const float fy1 = __builtin_amdgcn_logf(ratio) * 0.69314718055994531f;
+  const bool virgin = epf < 1.0e-10f && ratio == 1.0f;
+  const bool use_y = live && !dead && !virgin && epf >= 1.0e-10f && epf <= (float)sd.ep_hi &&
+                     fy1 <= (float)sd.lnr_y_hi;
   const bool fb_y = live && !dead && !virgin && !use_y;
   float ny = 1.0f;
   if (__ballot(use_y)) ny = mlp_eval_wave(sd.y, fy0, fy1);
-  double sdev[6], dps = 0.0;
+  float sdev[6], dps = 0.0f;
   if (fb_y) {
-    // the call takes addresses: copies keep sig, D and sdev in registers on
-    // the other paths (only these copies live in the stack frame)
+    // the call takes addresses: copies keep the stress, D and sdev in
+    // registers on the other paths (only these copies live in the stack
+    // frame)
     double sig_c[6], D_c[6], out_c[6], dps_c = 0.0;
     for (int e = 0; e < 6; e++) {
       sig_c[e] = sig[e];
       D_c[e] = D[e];
     }
     jc_strength_fallback(m, sig_c, D_c, dt, ep, epdot, dmg, out_c, &dps_c);
-    for (int e = 0; e < 6; e++) sdev[e] = out_c[e];
-    dps = dps_c;
+    for (int e = 0; e < 6; e++) sdev[e] = (float)out_c[e];
+    dps = (float)dps_c;
   } else if (dead) {
     for (int e = 0; e < 6; e++) sdev[e] = 0;
   } else {
-    double ys = virgin ? sd.sy_virgin : A * (double)ny;
-    double Gd = G_;
-    if (dmg > 0) {
-      Gd *= (1 - dmg);
-      ys *= (1 - dmg);
+    float ys = virgin ? (float)sd.sy_virgin : A * ny;
+    float Gd = G_;
+    if (dmf > 0) {
+      Gd *= (1 - dmf);
+      ys *= (1 - dmf);
     }
-    const double c = dt * 2.0 * Gd;
-    double t[6];
-    for (int e = 0; e < 6; e++) t[e] = sig[e] + c * D[e];
-    const double t3 = (t[0] + t[1] + t[2]) / 3.0;
+    const float c = dtf * 2.0f * Gd;
+    float t[6];
+    for (int e = 0; e < 6; e++) t[e] = (float)sig[e] + c * Df[e];
+    const float t3 = (t[0] + t[1] + t[2]) / 3.0f;
     t[0] -= t3;
     t[1] -= t3;
     t[2] -= t3;
-    const double J2 = SQRT_3_OVER_2 * norm_sym(t);
+    const float J2 = (float)SQRT_3_OVER_2 * norm_sym_t(t);
     if (J2 < ys) {
       for (int e = 0; e < 6; e++) sdev[e] = t[e];
     } else {
-      dps = (J2 - ys) / (3.0 * Gd);
-      const double f = ys / J2;
+      dps = __fdividef(J2 - ys, 3.0f * Gd);
+      const float f = __fdividef(ys, J2);
       for (int e = 0; e < 6; e++) sdev[e] = t[e] * f;
     }
   }
-  ep += dps;                                            // solid.cpp:1319
-  const double tav = 1000 * h / m.signal_velocity;      // solid.cpp:1324-1329
-  epdot -= epdot * dt / tav;
-  epdot += dps / tav;
-  epdot = epdot > 0.0 ? epdot : 0.0;
+  epf += dps;                                                       // solid.cpp:1319
+  const float tav = (float)(1000 * h / m.signal_velocity);          // solid.cpp:1324-1329
+  epdf -= __fdividef(epdf * dtf, tav);
+  epdf += __fdividef(dps, tav);
+  epdf = epdf > 0.0f ? epdf : 0.0f;
   // ---- damage (damage_jc.cpp:91-143) -------------------------------------
   bool use_f = false, fb_f = false;
   if (m.damage == KMPM_DAMAGE_JOHNSON_COOK) {
-    const double d4 = m.damage_p[3], epsdot0_d = m.damage_p[5];
-    const bool skip = dps == 0 && dmg >= 1.0;
-    const double vm = SQRT_3_OVER_2 * norm_sym(sdev);
-    double triax = 0.0;
-    if (pH != 0.0 && vm != 0.0) triax = -pH / (vm + 0.001 * fabs(pH));
+    const float d4 = (float)m.damage_p[3], epsdot0_d = (float)m.damage_p[5];
+    const bool skip = dps == 0 && dmf >= 1.0f;
+    const float vm = (float)SQRT_3_OVER_2 * norm_sym_t(sdev);
+    float triax = 0.0f;
+    if (pH != 0.0f && vm != 0.0f) triax = __fdividef(-pH, vm + 0.001f * fabsf(pH));
     const bool reset = !skip && triax <= -3;
-    const double lnr = (d4 > 0.0 && epdot > epsdot0_d) ? log(epdot / epsdot0_d) : 0.0;
+    const float lnr = (d4 > 0.0f && epdf > epsdot0_d) ? __logf(__fdividef(epdf, epsdot0_d)) : 0.0f;
     // eps_f only matters when plastic strain accumulates (dps / eps_f)
-    use_f = live && !skip && !reset && dps > 0 && triax <= sd.triax_hi && lnr <= sd.lnr_f_hi;
+    use_f = live && !skip && !reset && dps > 0 && triax <= (float)sd.triax_hi &&
+            lnr <= (float)sd.lnr_f_hi;
     fb_f = live && !skip && !reset && dps > 0 && !use_f;
     float nf = 0.0f;
-    if (__ballot(use_f)) nf = mlp_eval_wave(sd.f, (float)triax, (float)lnr);
+    if (__ballot(use_f)) nf = mlp_eval_wave(sd.f, triax, lnr);
     if (fb_f) {
-      double di_c = dinit, dm_c = dmg, sdev_c[6];
+      double di_c = dif, dm_c = dmf, sdev_c[6];
       for (int e = 0; e < 6; e++) sdev_c[e] = sdev[e];
-      jc_damage_fallback(m, di_c, dm_c, pH, sdev_c, epdot, dps);
-      dinit = di_c;
-      dmg = dm_c;
+      jc_damage_fallback(m, di_c, dm_c, pH, sdev_c, epdf, dps);
+      dif = (float)di_c;
+      dmf = (float)dm_c;
     } else if (reset) {
-      dinit = 0;
+      dif = 0;
     } else if (!skip) {
-      if (use_f) dinit += dps / (double)__builtin_amdgcn_exp2f(nf * 1.4426950408889634f);
-      if (dinit >= 1.0) {
-        const double dd = (dinit - 1.0) * 10;
-        dmg = dd < 1.0 ? dd : 1.0;
+      if (use_f) dif += __fdividef(dps, __builtin_amdgcn_exp2f(nf * 1.4426950408889634f));
+      if (dif >= 1.0f) {
+        const float dd = (dif - 1.0f) * 10;
+        dmf = dd < 1.0f ? dd : 1.0f;
       }
     }
   }
   surrogate_count(cnt, use_y || use_f, fb_y || fb_f);
   // ---- assembly (solid.cpp:1353-1366) -----------------------------------
-  const double trD = D[0] + D[1] + D[2];
-  const double trE = eel[0] + eel[1] + eel[2];
-  if (dmg == 0 || pH >= 0) {
-    for (int e = 0; e < 3; e++) sig[e] = -pH * 1.0 + sdev[e];
+  const float trD = Df[0] + Df[1] + Df[2];
+  const float trE = (float)eel[0] + (float)eel[1] + (float)eel[2];
+  if (dmf == 0 || pH >= 0) {
+    for (int e = 0; e < 3; e++) sig[e] = -pH * 1.0f + sdev[e];
   } else {
-    const double pd = -pH * (1.0 - dmg);
-    for (int e = 0; e < 3; e++) sig[e] = pd * 1.0 + sdev[e];
+    const float pd = -pH * (1.0f - dmf);
+    for (int e = 0; e < 3; e++) sig[e] = pd * 1.0f + sdev[e];
   }
   for (int e = 3; e < 6; e++) sig[e] = sdev[e];
-  const double t3 = (dt * trD + trE) / 3.0;
-  const double Gd = dmg > 1e-10 ? m.G * (1 - dmg) : m.G;
-  for (int e = 0; e < 3; e++) eel[e] = t3 * 1.0 + sdev[e] / Gd;
-  for (int e = 3; e < 6; e++) eel[e] = sdev[e] / Gd;
+  const float t3 = (dtf * trD + trE) / 3.0f;
+  const float Gd = dmf > 1e-10f ? (float)m.G * (1 - dmf) : (float)m.G;
+  for (int e = 0; e < 3; e++) eel[e] = t3 * 1.0f + __fdividef(sdev[e], Gd);
+  for (int e = 3; e < 6; e++) eel[e] = __fdividef(sdev[e], Gd);
+  ep = epf;
+  epdot = epdf;
+  dinit = dif;
+  dmg = dmf;
 }
 
 }  // namespace kmpm
