@@ -885,7 +885,7 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
-__global__ __launch_bounds__(256, SURR ? 3 : 1) void k_g2p_usl(KArgs a) {
+__global__ __launch_bounds__(256, SURR ? 3 : (PREC == KMPM_PRECISION_MIXED ? 4 : 1)) void k_g2p_usl(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
