@@ -885,7 +885,13 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
-__global__ __launch_bounds__(256, SURR ? 3 : (PREC == KMPM_PRECISION_MIXED ? 4 : 1)) void k_g2p_usl(KArgs a) {
+// Mixed linear and JC classes fit four waves per SIMD without spilling (the
+// fluid and generic classes would spill at that bound).
+__global__ __launch_bounds__(256, SURR ? 3
+                                       : (PREC == KMPM_PRECISION_MIXED &&
+                                                  (MC == MC_LINEAR || MC == MC_JC_SHOCK)
+                                              ? 4
+                                              : 1)) void k_g2p_usl(KArgs a) {
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
