@@ -56,7 +56,10 @@ enum { KMPM_METHOD_ULMPM = 0, KMPM_METHOD_TLMPM = 1 };
 enum { KMPM_SCHEME_USL = 0, KMPM_SCHEME_MUSL = 1 };
 /* shape functions -> basis_functions.h:21-104 */
 enum { KMPM_SHAPE_LINEAR = 0, KMPM_SHAPE_CUBIC_SPLINE = 1 };
-/* arithmetic of the device step */
+/* arithmetic of the device step: FP64 throughout, or mixed (FP32 state,
+ * transfers and constitutive update; FP64 position, deformation gradient,
+ * J and pressure; 64-bit fixed-point node mass, momentum and force sums),
+ * for both methods */
 enum { KMPM_PRECISION_FP64 = 0, KMPM_PRECISION_MIXED = 1 };
 
 typedef struct kmpm_config {
