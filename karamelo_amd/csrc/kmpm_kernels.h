@@ -814,28 +814,17 @@ __device__ inline double update_F(double dt, const double *L, double *F) {
 
 // The constitutive update in the arithmetic of the path: FP64, or FP32 on
 // the mixed path (whose state is stored in FP32), with J and rho in FP64.
-template <int MC, int PREC>
+// The stress state (sig, eel, ep, epdot, dinit, dmg) is held in the type of
+// the path's storage, R.
+template <int MC, int PREC, class R = typename Real<PREC>::T>
 __device__ inline void stress_update(const DevMat &m, double dt, double h, double J, double rho,
-                                     const double *D, double *sig, double *eel, double &ep,
-                                     double &epdot, double &dinit, double &dmg) {
+                                     const double *D, R *sig, R *eel, R &ep, R &epdot, R &dinit,
+                                     R &dmg) {
   if constexpr (PREC == KMPM_PRECISION_MIXED) {
-    float Df[6], sf[6], ef[6];
-    for (int e = 0; e < 6; e++) {
-      Df[e] = (float)D[e];
-      sf[e] = (float)sig[e];
-      ef[e] = (float)eel[e];
-    }
-    float epf = (float)ep, epdf = (float)epdot, dif = (float)dinit, dmf = (float)dmg;
-    update_stress_point<MC, float>(m, (float)dt, (float)h, J, rho, Df, sf, ef, epf, epdf, dif,
-                                   dmf);
-    for (int e = 0; e < 6; e++) {
-      sig[e] = sf[e];
-      eel[e] = ef[e];
-    }
-    ep = epf;
-    epdot = epdf;
-    dinit = dif;
-    dmg = dmf;
+    float Df[6];
+    for (int e = 0; e < 6; e++) Df[e] = (float)D[e];
+    update_stress_point<MC, float>(m, (float)dt, (float)h, J, rho, Df, sig, eel, ep, epdot, dinit,
+                                   dmg);
   } else {
     update_stress_point<MC>(m, dt, h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   }
@@ -846,11 +835,10 @@ __device__ inline void stress_update(const DevMat &m, double dt, double h, doubl
 // particles (skipped by the CFL maximum).
 // SURR: the JC + shock update with the learned material functions; every
 // lane of the wave calls it (live == false for lanes without a particle).
-template <int MC, int PREC, bool SURR = false>
+template <int MC, int PREC, bool SURR = false, class R = typename Real<PREC>::T>
 __device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
-                                          double *sig, double *eel, double &ep, double &epdot,
-                                          double &dinit, double &dmg, int mat, const double *v,
-                                          int pid, bool live = true,
+                                          R *sig, R *eel, R &ep, R &epdot, R &dinit, R &dmg,
+                                          int mat, const double *v, int pid, bool live = true,
                                           const SurrogateDev *sd = nullptr,
                                           SurrCount *cnt = nullptr) {
   const DevMat &m = a.mats[mat];
@@ -886,8 +874,10 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // ---------------------------------------------------------------------------
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
 // Mixed linear and JC classes fit four waves per SIMD without spilling (the
-// fluid and generic classes would spill at that bound).
-__global__ __launch_bounds__(256, SURR ? 3
+// fluid and generic classes would spill at that bound).  The surrogate
+// kernel spills some 40 registers at four waves and is still faster there
+// than at three (0.62 against 0.64 ms on the 4.27M bar).
+__global__ __launch_bounds__(256, SURR ? 4
                                        : (PREC == KMPM_PRECISION_MIXED &&
                                                   (MC == MC_LINEAR || MC == MC_JC_SHOCK)
                                               ? 4
@@ -981,15 +971,15 @@ __global__ __launch_bounds__(256, SURR ? 3
           for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
       }
       if (live) STB(VOL, j, J * vol0);
-      double dmg = LDA(DMG, slot);
+      R dmg = (R)LDA(DMG, slot);
       if (live && J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
-      double sig[6], eel[6];
+      R sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
-        sig[e] = LDA(SXX + e, slot);
-        eel[e] = LDA(EXX + e, slot);
+        sig[e] = (R)LDA(SXX + e, slot);
+        eel[e] = (R)LDA(EXX + e, slot);
       }
-      double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
-      double dinit = LDA(DINIT, slot);
+      R ep = (R)LDA(EP, slot), epdot = (R)LDA(EPDOT, slot);
+      R dinit = (R)LDA(DINIT, slot);
       const double c = stress_and_speed<MC, PREC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
                                                   mat, v, pid, live, sd, &scnt);
       if (!live) return;
@@ -1034,19 +1024,20 @@ __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, 
   m3_inverse(F, Finv);                                                   // :1186
   const double J = m3_det(F);                                            // :1203
   const double vol0 = LDA(VOL0, slot);
-  double dmg = LDA(DMG, slot);
+  using RS = typename Real<PREC>::T;   // the stored state's type
+  RS dmg = (RS)LDA(DMG, slot);
   if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);    // :1208-1215
   if (!polar_rotation(F, R)) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);    // :1222-1232
   double D[6];
   tl_rate(Fdot, Finv, R, D);                                             // :1225-1226
   const DevMat &m = a.mats[mat];
   const double rho = m.rho0 / J;
-  double sig[6], eel[6];
+  RS sig[6], eel[6];
   for (int e = 0; e < 6; e++) {
-    sig[e] = LDA(SXX + e, slot);
-    eel[e] = LDA(EXX + e, slot);
+    sig[e] = (RS)LDA(SXX + e, slot);
+    eel[e] = (RS)LDA(EXX + e, slot);
   }
-  double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot), dinit = LDA(DINIT, slot);
+  RS ep = (RS)LDA(EP, slot), epdot = (RS)LDA(EPDOT, slot), dinit = (RS)LDA(DINIT, slot);
   stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   double S[9] = {sig[0], sig[3], sig[4], sig[3], sig[1], sig[5], sig[4], sig[5], sig[2]};
   double P[9];
@@ -1278,15 +1269,15 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
         for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
       }
       STB(VOL, j, J * vol0);
-      double dmg = LDA(DMG, slot);
+      R dmg = (R)LDA(DMG, slot);
       if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
-      double sig[6], eel[6];
+      R sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
-        sig[e] = LDA(SXX + e, slot);
-        eel[e] = LDA(EXX + e, slot);
+        sig[e] = (R)LDA(SXX + e, slot);
+        eel[e] = (R)LDA(EXX + e, slot);
       }
-      double ep = LDA(EP, slot), epdot = LDA(EPDOT, slot);
-      double dinit = LDA(DINIT, slot);
+      R ep = (R)LDA(EP, slot), epdot = (R)LDA(EPDOT, slot);
+      R dinit = (R)LDA(DINIT, slot);
       const double c =
           stress_and_speed<MC, PREC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
       if (c > cmax) cmax = c;

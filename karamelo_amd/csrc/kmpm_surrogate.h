@@ -71,14 +71,14 @@ __device__ inline void surrogate_count_flush(const SurrCount &c, DevScalars *sc)
 // state, except the plain-function fallbacks, which stay FP64.
 __device__ inline void update_stress_jc_surrogate(const DevMat &m, const SurrogateDev &sd,
                                                   bool live, double dt, double h, double J,
-                                                  double rho, const double *D, double *sig,
-                                                  double *eel, double &ep, double &epdot,
-                                                  double &dinit, double &dmg, SurrCount &cnt) {
+                                                  double rho, const double *D, float *sig,
+                                                  float *eel, float &ep, float &epdot,
+                                                  float &dinit, float &dmg, SurrCount &cnt) {
   const float pH = (float)shock_pressure<double>(m, J, rho, dmg, D, h);
   const float dtf = (float)dt;
   float Df[6];
   for (int e = 0; e < 6; e++) Df[e] = (float)D[e];
-  float epf = (float)ep, epdf = (float)epdot, dif = (float)dinit, dmf = (float)dmg;
+  float epf = ep, epdf = epdot, dif = dinit, dmf = dmg;
   // ---- strength (strength_jc.cpp:100-184) -------------------------------
   const float G_ = (float)m.strength_p[0], A = (float)m.strength_p[1],
               epsdot0 = (float)m.strength_p[4];
@@ -103,7 +103,7 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
       sig_c[e] = sig[e];
       D_c[e] = D[e];
     }
-    jc_strength_fallback(m, sig_c, D_c, dt, ep, epdot, dmg, out_c, &dps_c);
+    jc_strength_fallback(m, sig_c, D_c, dt, epf, epdf, dmf, out_c, &dps_c);
     for (int e = 0; e < 6; e++) sdev[e] = (float)out_c[e];
     dps = (float)dps_c;
   } else if (dead) {
@@ -117,7 +117,7 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
     }
     const float c = dtf * 2.0f * Gd;
     float t[6];
-    for (int e = 0; e < 6; e++) t[e] = (float)sig[e] + c * Df[e];
+    for (int e = 0; e < 6; e++) t[e] = sig[e] + c * Df[e];
     const float t3 = (t[0] + t[1] + t[2]) / 3.0f;
     t[0] -= t3;
     t[1] -= t3;
@@ -171,7 +171,7 @@ __device__ inline void update_stress_jc_surrogate(const DevMat &m, const Surroga
   surrogate_count(cnt, use_y || use_f, fb_y || fb_f);
   // ---- assembly (solid.cpp:1353-1366) -----------------------------------
   const float trD = Df[0] + Df[1] + Df[2];
-  const float trE = (float)eel[0] + (float)eel[1] + (float)eel[2];
+  const float trE = eel[0] + eel[1] + eel[2];
   if (dmf == 0 || pH >= 0) {
     for (int e = 0; e < 3; e++) sig[e] = -pH * 1.0f + sdev[e];
   } else {
