@@ -284,6 +284,18 @@ __device__ inline bool fx_add_f(unsigned long long *dst, float val, float scale)
   return ok;
 }
 
+// fx_add_f's word for |val * scale| < 2^51, in three instructions: t =
+// val * scale + 1.5 * 2^52 is rounded once, to the nearest integer (ties to
+// even, as rintf; val * scale is exact in FP64), and the bits of t are then
+// those of 1.5 * 2^52 plus q = rint(val * scale) as a 64-bit integer.
+constexpr double FX_MAGIC = 6755399441055744.0;              // 1.5 * 2^52
+constexpr unsigned long long FX_MAGIC_BITS = 0x4338000000000000ull;
+constexpr double FX_FAST_LIMIT = 2251799813685248.0;         // 2^51
+__device__ inline unsigned long long fx_word_fast(float val, double scale) {
+  const double t = fma((double)val, scale, FX_MAGIC);
+  return (unsigned long long)__double_as_longlong(t) - FX_MAGIC_BITS;
+}
+
 // P2G over the active tiles, one workgroup per tile at a time.
 //
 // The tile's particles are staged in chunks of CHK (sorted by cell): one
@@ -346,6 +358,15 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   const int colofs = sa * PSA + sb * PSB;
   R fs[FX_N];
   for (int k = 0; k < FX_N; k++) fs[k] = (R)a.fxs[k];
+  // |sum| < flim[k] implies |sum * fxs[k]| < 2^51 (a power of two, clamped
+  // into the float range, which only sends more sums to the exact split)
+  double fsd[FX_N];
+  float flim[FX_N];
+  for (int k = 0; k < FX_N; k++) {
+    fsd[k] = a.fxs[k];
+    const double l = FX_FAST_LIMIT / a.fxs[k];
+    flim[k] = l > 0x1p126 ? 0x1p126f : (float)l;
+  }
 
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
@@ -448,18 +469,37 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
         }
         const int base = scl[i0] + colofs;
         bool ok = true;
+        if constexpr (FX) {
+          // the three-instruction words unless a sum of the wave is beyond
+          // their range (then the exact split, which also reports overflow)
+          bool fast = true;
 #pragma unroll
-        for (int k = 0; k < S1; k++)
+          for (int k = 0; k < S1; k++)
 #pragma unroll
-          for (int f = 0; f < NF; f++) {
-            BT *dst = &blk[f * PBN + base + k];
-            if constexpr (FX) {
-              const int kk = f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE;
-              ok &= fx_add_f(dst, acc[k][f], fs[kk]);
-            } else {
-              atomicAdd(dst, acc[k][f]);
-            }
+            for (int f = 0; f < NF; f++)
+              fast &= fabsf(acc[k][f]) < flim[f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE];
+          if (__ballot(!fast) == 0) {
+#pragma unroll
+            for (int k = 0; k < S1; k++)
+#pragma unroll
+              for (int f = 0; f < NF; f++)
+                atomicAdd(&blk[f * PBN + base + k],
+                          fx_word_fast(acc[k][f],
+                                       fsd[f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE]));
+          } else {
+#pragma unroll
+            for (int k = 0; k < S1; k++)
+#pragma unroll
+              for (int f = 0; f < NF; f++)
+                ok &= fx_add_f(&blk[f * PBN + base + k], acc[k][f],
+                               fs[f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE]);
           }
+        } else {
+#pragma unroll
+          for (int k = 0; k < S1; k++)
+#pragma unroll
+            for (int f = 0; f < NF; f++) atomicAdd(&blk[f * PBN + base + k], acc[k][f]);
+        }
         if (!ok) set_error(a.sc, KMPM_ERR_FIXED_POINT, -1);
       }
     }

@@ -95,6 +95,41 @@ def test_mixed_p2g_and_grid_update(name):
     ctx.close()
 
 
+def test_mixed_p2g_crowded_cell_takes_the_exact_split():
+    """300 particles next to one node: a cell run (segment) of up to 128
+    particles sums to about 2^52 mass quanta, past the 2^51 range of the
+    three-instruction fixed-point word, so those waves take the exact
+    two-word split; the node sums still match the FP64 oracle."""
+    say("mixed p2g: crowded cell")
+    prob = _randomize(P.two_disks(ppc1d=4))
+    p = prob.particles
+    h = prob.cfg.cellsize
+    idx = np.flatnonzero(p["solid"] == 0)[:300]
+    assert len(idx) == 300
+    k = np.arange(300)
+    # node (5, 5) at (0.25, 0.25); offsets of 1e-3 h to 1e-2 h keep every
+    # particle inside cell (5, 5) with a linear weight of about 0.99 to it
+    p["x"][idx, 0] = 0.25 + h * 1e-3 * (1 + k % 10)
+    p["x"][idx, 1] = 0.25 + h * 1e-3 * (1 + (k // 10) % 10)
+    o = O.oracle_for(prob)
+    ctx = P.context_for(mixed(prob))
+    for ph in ("compute_grid_weight_functions_and_gradients", "particles_to_grid",
+               "update_grid_state"):
+        ctx.phase(ph)
+    for ph in ("weights", "p2g", "grid_update"):
+        o.phase(ph)
+    mg, vg, _ = ctx.grid()
+    mo, vo, _, _ = o.grid()
+    st, _ = ctx.synchronize()
+    ctx.close()
+    assert st == 0
+    node = int(np.argmax(mo))
+    assert mo[node] > 250 * p["mass"][idx[0]]
+    assert np.abs(mg - mo).max() <= TOL_GRID_MIXED * np.abs(mo).max()
+    assert np.abs(vg - vo).max() <= TOL_GRID_MIXED * np.abs(vo).max()
+    assert abs(mg.sum() - p["mass"].sum()) <= 1e-6 * p["mass"].sum()
+
+
 def run_mixed_vs_oracle(prob, steps, chunk):
     """Mixed GPU run against the FP64 oracle every `chunk` steps; returns the
     worst relative L2 error of displacement and stress."""
