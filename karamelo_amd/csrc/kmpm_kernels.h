@@ -538,83 +538,192 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
   constexpr int BN = EX * EY * EZ;
   constexpr int TZZ = DIM == 3 ? TZ : 1;
   constexpr int NO = TX * TY * TZZ;
-  const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
-  const int total = a.sc->dil_count * NO;
-  const double dt_ctx = a.sc->dt;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
-    const int t = a.dil_list[q / NO], o = q % NO;
-    const double dt = tile_dt(a, t, dt_ctx);
-    const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
-    const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
-    const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
-    if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
-    const int64_t sn = (int64_t)t * NO + o;   // tile-major storage index
-    BT isum[SCR_FIELDS];
-    for (int f = 0; f < SCR_FIELDS; f++) isum[f] = 0;
-    // the sub-blocks of the source tiles whose blocks cover this node
+  if constexpr (FX) {
+    // Mixed: one group of 256 / TPB threads per dilated tile.  The tile's
+    // nodes lie in the partial blocks of the tiles around it, one contiguous
+    // sub-block per source tile at the same block position in every source;
+    // the group streams each source's run (coalesced) and adds every word
+    // into the tile's node sums in LDS with 64-bit integer atomics (exact, so
+    // the order of the adds does not matter).
+    constexpr int TPB = NO >= 256 ? 1 : 256 / NO;   // tiles per workgroup
+    constexpr int GS = 256 / TPB;                     // threads per tile
+    constexpr int NC = CX::n * CY::n * CZ::n;
+    __shared__ unsigned long long acc[TPB][SCR_FIELDS][NO];
+    __shared__ unsigned short otab[BN];   // block position -> node of the gathering tile
+    __shared__ unsigned char ctab[BN];    // block position -> source class combo
+    __shared__ int sslot[TPB][NC];
+    const int tid = threadIdx.x, grp = tid / GS, gl = tid % GS;
+    for (int o = tid; o < BN; o += 256) {
+      const int bz = o % EZ, by = (o / EZ) % EY, bx = o / (EZ * EY);
+      const int kx = CX::cls(bx), ky = CY::cls(by), kz = CZ::cls(bz);
+      const int lx = bx - CX::lo_b(kx) + CX::lo_l(kx), ly = by - CY::lo_b(ky) + CY::lo_l(ky),
+                lz = bz - CZ::lo_b(kz) + CZ::lo_l(kz);
+      const int p = subblock_pos<DIM, S1, TX, TY, TZ>(bx, by, bz);
+      ctab[p] = (unsigned char)((kx * CY::n + ky) * CZ::n + kz);
+      otab[p] = (unsigned short)((lx * TY + ly) * TZZ + lz);
+    }
+    const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+    const int ndil = a.sc->dil_count;
+    const double dt_ctx = a.sc->dt;
+    const unsigned long long *scr = reinterpret_cast<const unsigned long long *>(a.scratch);
+    const int f0 = remap ? 1 : 0, f1 = remap ? 4 : SCR_FIELDS;
+    for (int l0 = blockIdx.x * TPB; l0 < ndil; l0 += gridDim.x * TPB) {
+      const int li = l0 + grp;
+      const bool have = li < ndil;
+      const int t = have ? a.dil_list[li] : 0;
+      const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+      __syncthreads();   // tables written; the previous tiles' nodes are done
+      for (int q = gl; q < SCR_FIELDS * NO; q += GS) acc[grp][q / NO][q % NO] = 0;
+      if (gl < NC) {
+        const int kx = gl / (CY::n * CZ::n), ky = (gl / CZ::n) % CY::n, kz = gl % CZ::n;
+        const int ux = tx - CX::toff(kx), uy = ty - CY::toff(ky), uz = tz - CZ::toff(kz);
+        const bool in = have && ux >= 0 && ux < nt0 && uy >= 0 && uy < nt1 && uz >= 0 &&
+                        uz < nt2;
+        sslot[grp][gl] = in ? a.tile_slot[(ux * nt1 + uy) * nt2 + uz] : -1;
+      }
+      __syncthreads();
+      for (int p = gl; p < BN; p += GS) {
+        const int sl = sslot[grp][ctab[p]];
+        if (sl < 0) continue;
+        const unsigned long long *src = scr + (size_t)sl * SCR_FIELDS * BN + p;
+        unsigned long long v[SCR_FIELDS];
 #pragma unroll
-    for (int kx = 0; kx < CX::n; kx++) {
-      const int ux = tx - CX::toff(kx), rx = lx - CX::lo_l(kx);
-      if (rx < 0 || rx >= CX::size(kx) || ux < 0 || ux >= nt0) continue;
+        for (int f = 0; f < SCR_FIELDS; f++)
+          if (f >= f0 && f < f1) v[f] = src[f * BN];
+        const int o = otab[p];
 #pragma unroll
-      for (int ky = 0; ky < CY::n; ky++) {
-        const int uy = ty - CY::toff(ky), ry = ly - CY::lo_l(ky);
-        if (ry < 0 || ry >= CY::size(ky) || uy < 0 || uy >= nt1) continue;
-#pragma unroll
-        for (int kz = 0; kz < CZ::n; kz++) {
-          const int uz = tz - CZ::toff(kz), rz = lz - CZ::lo_l(kz);
-          if (rz < 0 || rz >= CZ::size(kz) || uz < 0 || uz >= nt2) continue;
-          const int slot = a.tile_slot[(ux * nt1 + uy) * nt2 + uz];
-          if (slot < 0) continue;
-          const int sx = CX::size(kx), sy = CY::size(ky), sz = CZ::size(kz);
-          const int pos = CX::lo_b(kx) * EY * EZ + sx * CY::lo_b(ky) * EZ +
-                          sx * sy * CZ::lo_b(kz) + (rx * sy + ry) * sz + rz;
-          const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + pos;
-          if (remap) {
-            for (int f = 1; f <= 3; f++) isum[f] += src[f * BN];
-          } else {
-            for (int f = 0; f < SCR_FIELDS; f++) isum[f] += src[f * BN];
-          }
+        for (int f = 0; f < SCR_FIELDS; f++)
+          if (f >= f0 && f < f1) atomicAdd(&acc[grp][f][o], v[f]);
+      }
+      __syncthreads();
+      if (!have) continue;
+      const double dt = tile_dt(a, t, dt_ctx);
+      for (int o = gl; o < NO; o += GS) {
+        const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
+        const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
+        if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
+        const int64_t sn = (int64_t)t * NO + o;   // tile-major storage index
+        BT isum[SCR_FIELDS];
+        for (int f = 0; f < SCR_FIELDS; f++) isum[f] = (BT)acc[grp][f][o];
+        double sum[SCR_FIELDS];
+        for (int f = 0; f < SCR_FIELDS; f++)
+          sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
+                      : (double)isum[f];
+        const uint32_t mask = a.gmask[sn];
+        if (remap) {
+          // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
+          // node mass, then FixVelocityNodes::post_velocities_to_grid.
+          const double m = gmass[sn];
+          double v[3];
+          for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+          for (int fi = 0; fi < a.nfv; fi++)
+            if (mask & a.fv[fi].bit)
+              for (int d = 0; d < 3; d++)
+                if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
+          for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
+          continue;
+        }
+        const double m = sum[0];
+        double v[3], vu[3], mb[3] = {0, 0, 0};
+        for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+        for (int fi = 0; fi < a.nbf; fi++)
+          if (m > 0 && (mask & a.bf[fi].bit))
+            for (int d = 0; d < 3; d++)
+              if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
+        for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
+        for (int fi = 0; fi < a.nfv; fi++)
+          if (mask & a.fv[fi].bit)
+            for (int d = 0; d < 3; d++)
+              if (a.fv[fi].set[d]) {
+                vu[d] = a.fv[fi].val[d];
+                v[d] = a.fv[fi].prev[d];
+              }
+        gmass[sn] = (R)m;
+        for (int d = 0; d < 3; d++) {
+          gv[d * a.ns + sn] = (R)v[d];
+          gvu[d * a.ns + sn] = (R)vu[d];
         }
       }
     }
-    double sum[SCR_FIELDS];
-    for (int f = 0; f < SCR_FIELDS; f++)
-      sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
-                  : (double)isum[f];
-    const uint32_t mask = a.gmask[sn];
-    if (remap) {
-      // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
-      // node mass, then FixVelocityNodes::post_velocities_to_grid.
-      const double m = gmass[sn];
-      double v[3];
+  } else {
+    const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+    const int total = a.sc->dil_count * NO;
+    const double dt_ctx = a.sc->dt;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+      const int t = a.dil_list[q / NO], o = q % NO;
+      const double dt = tile_dt(a, t, dt_ctx);
+      const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+      const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
+      const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
+      if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
+      const int64_t sn = (int64_t)t * NO + o;   // tile-major storage index
+      BT isum[SCR_FIELDS];
+      for (int f = 0; f < SCR_FIELDS; f++) isum[f] = 0;
+      // the sub-blocks of the source tiles whose blocks cover this node
+  #pragma unroll
+      for (int kx = 0; kx < CX::n; kx++) {
+        const int ux = tx - CX::toff(kx), rx = lx - CX::lo_l(kx);
+        if (rx < 0 || rx >= CX::size(kx) || ux < 0 || ux >= nt0) continue;
+  #pragma unroll
+        for (int ky = 0; ky < CY::n; ky++) {
+          const int uy = ty - CY::toff(ky), ry = ly - CY::lo_l(ky);
+          if (ry < 0 || ry >= CY::size(ky) || uy < 0 || uy >= nt1) continue;
+  #pragma unroll
+          for (int kz = 0; kz < CZ::n; kz++) {
+            const int uz = tz - CZ::toff(kz), rz = lz - CZ::lo_l(kz);
+            if (rz < 0 || rz >= CZ::size(kz) || uz < 0 || uz >= nt2) continue;
+            const int slot = a.tile_slot[(ux * nt1 + uy) * nt2 + uz];
+            if (slot < 0) continue;
+            const int sx = CX::size(kx), sy = CY::size(ky), sz = CZ::size(kz);
+            const int pos = CX::lo_b(kx) * EY * EZ + sx * CY::lo_b(ky) * EZ +
+                            sx * sy * CZ::lo_b(kz) + (rx * sy + ry) * sz + rz;
+            const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + pos;
+            if (remap) {
+              for (int f = 1; f <= 3; f++) isum[f] += src[f * BN];
+            } else {
+              for (int f = 0; f < SCR_FIELDS; f++) isum[f] += src[f * BN];
+            }
+          }
+        }
+      }
+      double sum[SCR_FIELDS];
+      for (int f = 0; f < SCR_FIELDS; f++)
+        sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
+                    : (double)isum[f];
+      const uint32_t mask = a.gmask[sn];
+      if (remap) {
+        // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
+        // node mass, then FixVelocityNodes::post_velocities_to_grid.
+        const double m = gmass[sn];
+        double v[3];
+        for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+        for (int fi = 0; fi < a.nfv; fi++)
+          if (mask & a.fv[fi].bit)
+            for (int d = 0; d < 3; d++)
+              if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
+        for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
+        continue;
+      }
+      const double m = sum[0];
+      double v[3], vu[3], mb[3] = {0, 0, 0};
       for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+      for (int fi = 0; fi < a.nbf; fi++)
+        if (m > 0 && (mask & a.bf[fi].bit))
+          for (int d = 0; d < 3; d++)
+            if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
+      for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
       for (int fi = 0; fi < a.nfv; fi++)
         if (mask & a.fv[fi].bit)
           for (int d = 0; d < 3; d++)
-            if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
-      for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
-      continue;
-    }
-    const double m = sum[0];
-    double v[3], vu[3], mb[3] = {0, 0, 0};
-    for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
-    for (int fi = 0; fi < a.nbf; fi++)
-      if (m > 0 && (mask & a.bf[fi].bit))
-        for (int d = 0; d < 3; d++)
-          if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
-    for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
-    for (int fi = 0; fi < a.nfv; fi++)
-      if (mask & a.fv[fi].bit)
-        for (int d = 0; d < 3; d++)
-          if (a.fv[fi].set[d]) {
-            vu[d] = a.fv[fi].val[d];
-            v[d] = a.fv[fi].prev[d];
-          }
-    gmass[sn] = (R)m;
-    for (int d = 0; d < 3; d++) {
-      gv[d * a.ns + sn] = (R)v[d];
-      gvu[d * a.ns + sn] = (R)vu[d];
+            if (a.fv[fi].set[d]) {
+              vu[d] = a.fv[fi].val[d];
+              v[d] = a.fv[fi].prev[d];
+            }
+      gmass[sn] = (R)m;
+      for (int d = 0; d < 3; d++) {
+        gv[d * a.ns + sn] = (R)v[d];
+        gvu[d * a.ns + sn] = (R)vu[d];
+      }
     }
   }
 }
