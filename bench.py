@@ -17,7 +17,10 @@ Paths (--paths, the last one is the headline `value`):
                    the learned BF16 networks (trained on the device before
                    the timed region, on tuples of the plain functions).
 Each path runs the same steps from the same inputs; the headline's
-displacement and stress are compared with the plain FP64 run's.
+displacement and stress are compared with the plain FP64 run's.  The timed
+steps are replays of a captured HIP graph of --graph-steps steps; the
+per-phase launch times (phases, roofline) come from as many eager steps
+again, with HIP events on the library's stream around each phase.
 
 Prints ONE JSON line.  Progress goes to stderr.
 """
@@ -126,7 +129,7 @@ def train_surrogate(mat, iters=6000):
     return s, rec
 
 
-def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5):
+def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_steps=10):
     """Times `steps` steps of one path after `warmup` steps; returns the
     timing record and the downloaded particle state."""
     from karamelo_amd import native as N
@@ -148,22 +151,32 @@ def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5):
     else:
         variant, tune_ms = 0, [0.0] * 3
     log("[%s] autotune: tile %s, ms/step per variant %s" % (path, ctx.tile_shape(), tune_ms))
+    # the steps are replays of a graph of `graph_steps` captured steps; the
+    # first call captures it (and replays it once), outside the timed region
+    ctx.step_graph(graph_steps, graph_steps)
     device_sync()
     ctx.surrogate_stats()          # resets the counters: the timed steps only
-    log("[%s] timed %d steps ..." % (path, steps))
-    ctx.timing_enable(True)
-    device_sync()
+    log("[%s] timed %d steps (graph replays of %d) ..." % (path, steps, graph_steps))
     t0 = time.perf_counter()
-    ctx.step(steps)
+    ctx.step_graph(steps, graph_steps)
     ctx.synchronize()
     device_sync()
     t1 = time.perf_counter()
-    phases = ctx.timing_read()
-    ctx.timing_enable(False)
     st, tag = ctx.synchronize()
     if st != 0:
         raise RuntimeError("device error %d (particle %d) in the timed region" % (st, tag))
     evals, fallbacks = ctx.surrogate_stats()
+    # per-phase launch times: as many eager steps again, with HIP events
+    # recorded on the library's stream around each phase
+    log("[%s] %d eager steps with phase events ..." % (path, steps))
+    ctx.timing_enable(True)
+    ctx.step(steps)
+    ctx.synchronize()
+    phases = ctx.timing_read()
+    ctx.timing_enable(False)
+    st, tag = ctx.synchronize()
+    if st != 0:
+        raise RuntimeError("device error %d (particle %d) in the phase-timed steps" % (st, tag))
     state = ctx.download()
     tile = ctx.tile_shape()
     ctx.close()
@@ -324,6 +337,8 @@ def main():
                     help="initial effective plastic strain of every particle (counter runs: "
                          "the surrogate is in use from the first step without a warm-up)")
     ap.add_argument("--train-iterations", type=int, default=6000)
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="steps per captured graph in the timed region")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-instances workload")
     ap.add_argument("--batched-steps", type=int, default=100)
@@ -353,7 +368,8 @@ def main():
         surrogate, training = train_surrogate(prob.mats[0], args.train_iterations)
     for path in paths:
         results[path], states[path] = run_path(prob, path, args.steps, args.warmup, surrogate,
-                                               autotune_steps=args.autotune_steps)
+                                               autotune_steps=args.autotune_steps,
+                                               graph_steps=args.graph_steps)
     prob.particles = None
 
     head = paths[-1]
