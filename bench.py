@@ -345,9 +345,8 @@ def main():
     ap.add_argument("--batched-warmup", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the step has no multi-GPU path (a run "
+              "would shard the tiles across ranks); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from karamelo_amd import problems as P
