@@ -95,6 +95,44 @@ def test_mixed_p2g_and_grid_update(name):
     ctx.close()
 
 
+@pytest.mark.parametrize("name,variant", [(n, v) for n in ("two_disks_cubic", "tensile_cubic",
+                                                             "tensile_linear")
+                                          for v in (0, 1, 2)])
+def test_mixed_p2g_and_grid_update_tile_variants(name, variant):
+    """Every tile shape of the mixed grid update (64 to 512 nodes per tile,
+    2D and 3D, both shape functions): one P2G + grid update against the
+    FP64 oracle, to the bars of the default shape."""
+    say("mixed p2g/grid: %s, tile variant %d" % (name, variant))
+    prob = _randomize(small_problems()[name]())
+    o = O.oracle_for(prob)
+    ctx = P.context_for(mixed(prob))
+    ctx.set_tile_variant(variant)
+    say("  tile %s" % (ctx.tile_shape(),))
+    for ph in ("compute_grid_weight_functions_and_gradients", "particles_to_grid",
+               "update_grid_state"):
+        ctx.phase(ph)
+    for ph in ("weights", "p2g", "grid_update"):
+        o.phase(ph)
+    mg, vg, vug = ctx.grid()
+    mo, vo, vuo, _ = o.grid()
+    st, _ = ctx.synchronize()
+    ctx.close()
+    assert st == 0
+    assert np.abs(mg - mo).max() <= TOL_GRID_MIXED * np.abs(mo).max()
+    assert np.abs(vg - vo).max() <= TOL_GRID_MIXED * np.abs(vo).max()
+    assert np.abs(vug - vuo).max() <= TOL_GRID_MIXED * np.abs(vuo).max()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_mixed_musl_dam_break_tile_variants(variant):
+    """MUSL (its second P2G re-maps momentum through the grid update's
+    momentum-only form) on every tile shape: 60 mixed steps of the dam break
+    within the mixed bar of the FP64 oracle."""
+    say("mixed MUSL dam break, tile variant %d" % variant)
+    w = run_mixed_vs_oracle(P.dam_break(n_cells=16), 60, 60, variant=variant)
+    assert w["u"] <= TOL_RUN_MIXED and w["sigma"] <= TOL_RUN_MIXED
+
+
 def test_mixed_p2g_crowded_cell_takes_the_exact_split():
     """300 particles next to one node: a cell run (segment) of up to 128
     particles sums to about 2^52 mass quanta, past the 2^51 range of the
@@ -130,12 +168,15 @@ def test_mixed_p2g_crowded_cell_takes_the_exact_split():
     assert abs(mg.sum() - p["mass"].sum()) <= 1e-6 * p["mass"].sum()
 
 
-def run_mixed_vs_oracle(prob, steps, chunk):
-    """Mixed GPU run against the FP64 oracle every `chunk` steps; returns the
-    worst relative L2 error of displacement and stress."""
+def run_mixed_vs_oracle(prob, steps, chunk, variant=None):
+    """Mixed GPU run (tile variant `variant`, or the default) against the
+    FP64 oracle every `chunk` steps; returns the worst relative L2 error of
+    displacement and stress."""
     x0 = prob.particles["x"].copy()
     o = O.oracle_for(prob)
     ctx = P.context_for(mixed(prob))
+    if variant is not None:
+        ctx.set_tile_variant(variant)
     G = max(m.G for m in prob.mats)
     worst = {"u": 0.0, "sigma": 0.0, "v": 0.0}
     done = 0
