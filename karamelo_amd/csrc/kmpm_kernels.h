@@ -1025,7 +1025,8 @@ template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
 // Mixed linear and JC classes fit four waves per SIMD without spilling (the
 // fluid and generic classes would spill at that bound).  The surrogate
 // kernel spills some 40 registers at four waves and is still faster there
-// than at three (0.62 against 0.64 ms on the 4.27M bar).
+// than at three or five (0.61 against 0.62 and 0.84 ms on the 4.27M bar);
+// the plain mixed kernel takes 0.42 ms at four waves and 0.57 at five.
 __global__ __launch_bounds__(256, SURR ? 4
                                        : (PREC == KMPM_PRECISION_MIXED &&
                                                   (MC == MC_LINEAR || MC == MC_JC_SHOCK)
