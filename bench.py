@@ -5,9 +5,12 @@ Workload: 3D ULMPM tensile bar with Johnson-Cook strength, shock EOS and JC
 damage (steel), cubic B-splines, USL, 2x2x2 particles per cell, cell 0.06 mm:
 4.27M particles.  A step is one complete USL step over all particles
 (sort, P2G, grid update with the end-velocity fixes, fused G2P + F + stress +
-CFL, dt update), synthetic lattice inputs, state resident in HBM.  The
-default warm-up (1000 steps, ~9 us) strains the bar past yield, so the timed
-steps run the plastic branch of the constitutive update.
+CFL, dt update), synthetic lattice inputs, state resident in HBM.  Every
+path first runs --preload-steps untimed steps from rest (1000 steps, ~9 us of
+pulling), which strain the whole bar past yield, then --warmup untimed steps:
+the timed steps run the plastic branch of the constitutive update whatever
+--warmup is, and the headline fails unless the networks evaluated about
+every particle in every timed step.
 
 Paths (--paths, the last one is the headline `value`):
   plain-fp64       FP64 throughout, parity with the reference to 1e-10;
@@ -129,9 +132,10 @@ def train_surrogate(mat, iters=6000):
     return s, rec
 
 
-def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_steps=10):
-    """Times `steps` steps of one path after `warmup` steps; returns the
-    timing record and the downloaded particle state."""
+def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_steps=10,
+             preload=1000):
+    """Times `steps` steps of one path after `preload` + `warmup` steps;
+    returns the timing record and the downloaded particle state."""
     from karamelo_amd import native as N
     from karamelo_amd import problems as P
     cfg = N.Config.from_buffer_copy(prob.cfg)
@@ -139,6 +143,8 @@ def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_
     ctx = P.context_for(dataclasses.replace(prob, cfg=cfg))
     if path == "mixed+surrogate":
         ctx.set_surrogate(surrogate)
+    log("[%s] pre-load %d steps (pulling the bar past yield) ..." % (path, preload))
+    ctx.step(preload)
     log("[%s] warm-up %d steps ..." % (path, warmup))
     ctx.step(warmup)
     st, tag = ctx.synchronize()
@@ -295,17 +301,54 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (nb if nb > 0 else 1.0))
 
 
-def cpu_baseline(steps=2):
-    """The oracle (a single-threaded FP64 port of the reference's loops,
-    -O3 build) on a bounded sample of the same workload: the same bar
-    cross-section, cell size, particles per cell, materials and fixes, but a
-    2 mm long section (about 10% of the particles), one warm-up step then
-    `steps` timed steps."""
+def host_cpu():
+    """CPU model, physical cores of the host, and the CPUs this process may
+    run on (the box's share)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count()
+    return {"model": model, "physical_cores": len(cores) or None, "cpus_available": share}
+
+
+def cpu_baseline(steps=20):
+    """The oracle (an FP64 restatement of the reference's loops: per-step
+    neighbour lists, node-major gather P2G) built -O3 with OpenMP over its
+    particle and node loops, on every CPU this process may use, on a bounded
+    sample of the same workload: the same bar cross-section, cell size,
+    particles per cell, materials and fixes, but a 2 mm long section (about
+    10% of the particles), one warm-up step then `steps` timed steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from karamelo_amd import problems as P
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cpu["cpus_available"]
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     prob = P.tensile_bar(length=0.002)
     o = O.oracle_for(prob, variant="liboracle_perf.so")
+    try:
+        threads = C.CDLL("libgomp.so.1").omp_get_max_threads()
+    except OSError:
+        pass
     o.step(1)
     t0 = time.perf_counter()
     st = o.step(steps)
@@ -314,20 +357,27 @@ def cpu_baseline(steps=2):
         raise RuntimeError("oracle step failed: %d" % st)
     value = prob.np * steps / (t1 - t0)
     o.close()
-    return {"value": value, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "oracle -O3, 1 thread: tensile bar section 2 mm long (%d particles, "
-                      "h=0.06 mm, 2x2x2 ppc, JC+shock+damage, cubic, USL), 1 warm-up + %d "
-                      "timed steps (%.1f s)" % (prob.np, steps, t1 - t0)}
+    return {"value": value, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "cpu": cpu,
+            "sample": "oracle -O3 + OpenMP, %d threads on %s (%s physical cores on the host, "
+                      "%s CPUs available to this process): tensile bar section 2 mm long (%d "
+                      "particles, h=0.06 mm, 2x2x2 ppc, JC+shock+damage, cubic, USL), 1 warm-up "
+                      "+ %d timed steps (%.1f s)" % (threads, cpu["model"], cpu["physical_cores"],
+                                                    cpu["cpus_available"], prob.np, steps,
+                                                    t1 - t0)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=1000,
-                    help="untimed steps; 1000 strain the bar into the plastic regime")
+    ap.add_argument("--warmup", type=int, default=10,
+                    help="untimed steps after the pre-load")
+    ap.add_argument("--preload-steps", type=int, default=1000,
+                    help="untimed steps from rest before the warm-up; 1000 strain the whole "
+                         "bar past yield, so the timed steps are plastic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--h", type=float, default=6e-5, help="cell size (m)")
     ap.add_argument("--paths", default="plain-fp64,mixed,mixed+surrogate",
                     help="comma-separated paths to time; the last one is the headline")
@@ -368,7 +418,8 @@ def main():
     for path in paths:
         results[path], states[path] = run_path(prob, path, args.steps, args.warmup, surrogate,
                                                autotune_steps=args.autotune_steps,
-                                               graph_steps=args.graph_steps)
+                                               graph_steps=args.graph_steps,
+                                               preload=args.preload_steps)
     prob.particles = None
 
     head = paths[-1]
@@ -390,6 +441,11 @@ def main():
             # surrogate evaluations happen inside the fused G2P launches
             g2p_ms, g2p_n = rp["phases"]["g2p"]
             ev = rp["surrogate_evals"]
+            # a surrogate number is only a surrogate number if the networks ran
+            if ev < 0.5 * args.steps * npart:
+                raise RuntimeError("mixed+surrogate: %d network evaluations in %d timed steps of "
+                                   "%d particles; the timed steps are not plastic (raise "
+                                   "--preload-steps)" % (ev, args.steps, npart))
             mf = ev * MFMA_FLOPS_PER_EVAL / (g2p_ms * 1e-3) / 1e12
             per_path[path]["surrogate"] = {
                 "evaluations": ev, "fallbacks": rp["surrogate_fallbacks"],

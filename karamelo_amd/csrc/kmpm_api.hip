@@ -84,6 +84,16 @@ static void swap_state(kmpm_ctx *c) {
   refresh_args(c);
 }
 
+// Captured step graphs hold the buffers and library workspaces of the
+// moment they were captured; anything that reallocates them drops the graphs.
+static void drop_graphs(kmpm_ctx *c) {
+  for (auto &g : c->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.steps = 0;
+  }
+}
+
 // Phase timing: a start/stop HIP event pair recorded on the context stream
 // around each launch of a phase while timing is enabled.
 static void tbegin(kmpm_ctx *c, int phase) {
@@ -133,8 +143,8 @@ static void join_mixed(const std::vector<double> &dd, const std::vector<float> &
 // Power-of-two scales of the fixed-point node sums: one particle's
 // contribution stays below 2^FX_CONTRIB_BITS.  The P2G kernels scale in FP32,
 // so a scale must lie inside the float range (false otherwise).
-static bool set_fixed_point_scales(kmpm_ctx *c, const std::vector<double> &hd, int cap,
-                                   int64_t np) {
+static bool fixed_point_scales(const kmpm_ctx *c, const std::vector<double> &hd, int cap,
+                               int64_t np, double fxs[FX_N], double fxi[FX_N]) {
   double mmax = 0, vmax = 0, smax = 0, volmax = 0;
   for (int64_t i = 0; i < np; i++) {
     mmax = std::max(mmax, std::fabs(hd[(size_t)MASS * cap + i]));
@@ -155,8 +165,8 @@ static bool set_fixed_point_scales(kmpm_ctx *c, const std::vector<double> &hd, i
     int e = 0;
     std::frexp(bound[k] > 0 && std::isfinite(bound[k]) ? bound[k] : 1.0, &e);   // bound < 2^e
     if (FX_CONTRIB_BITS - e > 120 || FX_CONTRIB_BITS - e < -120) return false;
-    c->a.fxs[k] = std::ldexp(1.0, FX_CONTRIB_BITS - e);
-    c->a.fxi[k] = std::ldexp(1.0, e - FX_CONTRIB_BITS);
+    fxs[k] = std::ldexp(1.0, FX_CONTRIB_BITS - e);
+    fxi[k] = std::ldexp(1.0, e - FX_CONTRIB_BITS);
   }
   return true;
 }
@@ -276,6 +286,7 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
     }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   void *old[] = {c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                  c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst,
                  c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask};
@@ -419,6 +430,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
             hipMemcpy(c->d_mats, dm.data(), nmat * sizeof(DevMat), hipMemcpyHostToDevice) ==
                 hipSuccess;
   ok = ok && dalloc(&c->d_sc, 1) == hipSuccess;
+  ok = ok && dalloc(&c->d_red, 2 + 2 * ENERGY_BLOCKS) == hipSuccess;
   if (ok) {
     c->h_gmask = new std::vector<uint32_t>(a.nn, 1u);   // Grid::grow sets mask = 1 (grid.cpp:429)
     DevScalars sc;
@@ -447,11 +459,10 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   if (!c) return KMPM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particle_buffers(c);
-  for (auto &g : c->graphs)
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  drop_graphs(c);
   void *ptrs[] = {c->d_mats, c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
-                  c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst};
+                  c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst, c->d_red};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   free(c->mats_host);
@@ -479,49 +490,21 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
     if (i > 0 && (p->ptag[i] <= p->ptag[i - 1] || p->solid[i] < p->solid[i - 1]))
       return KMPM_ERR_ARG;  // ascending tags, ordered by solid
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
-  free_particle_buffers(c);
-  const int cap = (int)std::max<int64_t>(np, 1);
   KArgs &a = c->a;
-  a.np = (int)np;
-  a.cap = cap;
-  const bool mixed = a.prec == KMPM_PRECISION_MIXED;
-  const int ndf = state_nd(mixed, a.tl);
   if (a.tl)
     for (int64_t i = 1; i < np; i++)
       if (p->solid[i] != p->solid[0]) return KMPM_ERR_ARG;   // one solid: one TL grid
-  HIPCHK(dalloc(&c->d_pa, (size_t)ndf * cap));
-  HIPCHK(dalloc(&c->d_pb, (size_t)ndf * cap));
-  if (mixed) {
-    HIPCHK(dalloc(&c->d_fa, (size_t)state_nf(true, a.tl) * cap));
-    HIPCHK(dalloc(&c->d_fb, (size_t)state_nf(true, a.tl) * cap));
-  }
-  HIPCHK(dalloc(&c->d_xo, (size_t)3 * cap));
-  HIPCHK(dalloc(&c->d_ia, (size_t)NIF * cap));
-  HIPCHK(dalloc(&c->d_ib, (size_t)NIF * cap));
-  HIPCHK(dalloc(&c->d_key, cap));
-  HIPCHK(dalloc(&c->d_order_tmp, cap));
-  HIPCHK(dalloc(&c->d_sort_tmp, cap));
-  HIPCHK(dalloc(&c->d_order, cap));
-  HIPCHK(dalloc(&c->d_pmask, cap));
-  {
-    const int st = setup_tiles(c, c->tile_variant);   // P2G partial blocks for np particles
-    if (st != KMPM_OK) return st;
-  }
+  const int cap = (int)std::max<int64_t>(np, 1);
+  const bool mixed = a.prec == KMPM_PRECISION_MIXED;
+  const int ndf = state_nd(mixed, a.tl);
 
-  c->h_ptag = (int64_t *)malloc(sizeof(int64_t) * cap);
-  c->h_solid = (int32_t *)malloc(sizeof(int32_t) * cap);
-  c->h_pmask = (uint32_t *)malloc(sizeof(uint32_t) * cap);
-  c->h_x0 = (double *)malloc(sizeof(double) * 3 * cap);
+  // Host images of the new state.  Everything that can reject the upload is
+  // checked here, while the context still holds its previous particles.
   std::vector<double> hd((size_t)(a.tl ? NDF_TL : NDF) * cap, 0.0);
   std::vector<int> hi((size_t)NIF * cap, 0);
   auto D = [&](int f, int64_t i) -> double & { return hd[(size_t)f * cap + i]; };
   for (int64_t i = 0; i < np; i++) {
-    c->h_ptag[i] = p->ptag[i];
-    c->h_solid[i] = p->solid[i];
-    c->h_pmask[i] = p->mask ? p->mask[i] : 1u;
     for (int d = 0; d < 3; d++) {
-      c->h_x0[3 * i + d] = p->x0 ? p->x0[3 * i + d] : p->x[3 * i + d];
       D(PX + d, i) = p->x[3 * i + d];
       D(VX + d, i) = p->v[3 * i + d];
     }
@@ -553,22 +536,87 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
     hi[(size_t)PID * cap + i] = (int)i;
     hi[(size_t)MATI * cap + i] = p->mat[i];
   }
+  double fxs[FX_N], fxi[FX_N];
+  if (mixed && !fixed_point_scales(c, hd, cap, np, fxs, fxi)) return KMPM_ERR_FIXED_POINT;
+
+  // The previous particles go.  From here a failure leaves the context
+  // without particles and rejects every step until an upload succeeds.
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  free_particle_buffers(c);
+  c->upload_failed = true;
+  a.np = 0;
+  a.cap = 0;
+  refresh_args(c);
+  HIPCHK(dalloc(&c->d_pa, (size_t)ndf * cap));
+  HIPCHK(dalloc(&c->d_pb, (size_t)ndf * cap));
+  if (mixed) {
+    HIPCHK(dalloc(&c->d_fa, (size_t)state_nf(true, a.tl) * cap));
+    HIPCHK(dalloc(&c->d_fb, (size_t)state_nf(true, a.tl) * cap));
+  }
+  HIPCHK(dalloc(&c->d_xo, (size_t)3 * cap));
+  HIPCHK(dalloc(&c->d_ia, (size_t)NIF * cap));
+  HIPCHK(dalloc(&c->d_ib, (size_t)NIF * cap));
+  HIPCHK(dalloc(&c->d_key, cap));
+  HIPCHK(dalloc(&c->d_order_tmp, cap));
+  HIPCHK(dalloc(&c->d_sort_tmp, cap));
+  HIPCHK(dalloc(&c->d_order, cap));
+  HIPCHK(dalloc(&c->d_pmask, cap));
+  a.np = (int)np;
+  a.cap = cap;
+  {
+    const int st = setup_tiles(c, c->tile_variant);   // P2G partial blocks for np particles
+    if (st != KMPM_OK) {
+      a.np = 0;
+      return st;
+    }
+  }
+  c->h_ptag = (int64_t *)malloc(sizeof(int64_t) * cap);
+  c->h_solid = (int32_t *)malloc(sizeof(int32_t) * cap);
+  c->h_pmask = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+  c->h_x0 = (double *)malloc(sizeof(double) * 3 * cap);
+  if (!c->h_ptag || !c->h_solid || !c->h_pmask || !c->h_x0) {
+    a.np = 0;
+    return KMPM_ERR_ARG;
+  }
+  for (int64_t i = 0; i < np; i++) {
+    c->h_ptag[i] = p->ptag[i];
+    c->h_solid[i] = p->solid[i];
+    c->h_pmask[i] = p->mask ? p->mask[i] : 1u;
+    for (int d = 0; d < 3; d++) c->h_x0[3 * i + d] = p->x0 ? p->x0[3 * i + d] : p->x[3 * i + d];
+  }
+  refresh_args(c);
+  auto fail = [&](int st) {
+    a.np = 0;
+    return st;
+  };
   if (mixed) {
     std::vector<double> dd((size_t)state_nd(true, a.tl) * cap);
     std::vector<float> ff((size_t)state_nf(true, a.tl) * cap);
     split_mixed(hd, cap, a.tl, dd, ff);
-    HIPCHK(hipMemcpy(c->d_pa, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_fa, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice));
-    if (!set_fixed_point_scales(c, hd, cap, np)) return KMPM_ERR_FIXED_POINT;
-  } else {
-    HIPCHK(hipMemcpy(c->d_pa, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (hipMemcpy(c->d_pa, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->d_fa, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return fail(KMPM_ERR_HIP);
+    for (int k = 0; k < FX_N; k++) {
+      a.fxs[k] = fxs[k];
+      a.fxi[k] = fxi[k];
+    }
+  } else if (hipMemcpy(c->d_pa, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice) !=
+             hipSuccess) {
+    return fail(KMPM_ERR_HIP);
   }
-  HIPCHK(hipMemcpy(c->d_ia, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_pmask, c->h_pmask, np * sizeof(uint32_t), hipMemcpyHostToDevice));
-  refresh_args(c);
+  if (hipMemcpy(c->d_ia, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(c->d_pmask, c->h_pmask, np * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      launch_iota(c->d_order, (int)np, c->stream) != hipSuccess)
+    return fail(KMPM_ERR_HIP);
   c->keys_valid = false;
   c->tl_identity = false;
   c->sorted = false;
+  c->upload_failed = false;
   return KMPM_OK;
 }
 
@@ -673,6 +721,7 @@ int kmpm_fix_body_force(kmpm_ctx *c, uint32_t bit, const int set[3], const doubl
 // ---- phases ---------------------------------------------------------------
 int kmpm_compute_grid_weight_functions_and_gradients(kmpm_ctx *c) {
   if (!c) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   if (c->tl_identity) {
     // TLMPM weights are those of x0 (computed once, tlmpm.cpp:89,339): after
     // the first step the state is stored in its sorted order and the cell
@@ -693,6 +742,7 @@ int kmpm_compute_grid_weight_functions_and_gradients(kmpm_ctx *c) {
 
 int kmpm_particles_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G);
   HIPCHK(launch_p2g(c, 0));
   tend(c);
@@ -701,6 +751,7 @@ int kmpm_particles_to_grid(kmpm_ctx *c) {
 
 int kmpm_update_grid_state(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_GRID);
   HIPCHK(launch_grid_update(c, 0));
   tend(c);
@@ -722,6 +773,7 @@ static int after_g2p_state_swap(kmpm_ctx *c) {
 
 int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_G2P);
   HIPCHK(launch_g2p_usl(c));
   tend(c);
@@ -730,6 +782,7 @@ int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
 
 int kmpm_grid_to_points(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_G2P_A);
   HIPCHK(launch_g2p_musl_a(c));
   tend(c);
@@ -738,6 +791,7 @@ int kmpm_grid_to_points(kmpm_ctx *c) {
 
 int kmpm_velocities_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G_B);
   HIPCHK(launch_p2g(c, 1));
   tend(c);
@@ -749,6 +803,7 @@ int kmpm_velocities_to_grid(kmpm_ctx *c) {
 
 int kmpm_update_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_STRESS_B);
   HIPCHK(launch_stress_musl_b(c));
   tend(c);
@@ -757,6 +812,7 @@ int kmpm_update_stress(kmpm_ctx *c) {
 
 int kmpm_adjust_dt(kmpm_ctx *c) {
   if (!c) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_DT);
   HIPCHK(launch_adjust_dt(c));
   tend(c);
@@ -1090,14 +1146,11 @@ int kmpm_get_time(kmpm_ctx *c, double *dt, double *time, int64_t *step) {
 
 int kmpm_reduce_energy(kmpm_ctx *c, uint32_t bit, double *ek, double *es) {
   if (!c || !ek || !es) return KMPM_ERR_ARG;
-  double *d_out = nullptr;
-  HIPCHK(hipMalloc(&d_out, (2 + 2 * 64) * sizeof(double)));
-  hipError_t e = launch_energy(c, bit, d_out);
+  hipError_t e = launch_energy(c, bit, c->d_red);
   double h[2] = {0, 0};
   if (e == hipSuccess)
-    e = hipMemcpyAsync(h, d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    e = hipMemcpyAsync(h, c->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(d_out);
   if (e != hipSuccess) return KMPM_ERR_HIP;
   *ek = h[0];
   *es = h[1];
@@ -1192,6 +1245,7 @@ int kmpm_timing_read(kmpm_ctx *c, double ms[KMPM_NPHASE], int64_t launches[KMPM_
 int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
   if (!c) return KMPM_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   if (!s) {
     c->a.surr_on = 0;
     return KMPM_OK;

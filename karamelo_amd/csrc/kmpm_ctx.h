@@ -136,6 +136,9 @@ struct kmpm_ctx {
   bool timing;
   std::vector<hipEvent_t> *ev_pool;   // recorded pairs: start, stop
   std::vector<int> *ev_phase;         // phase of each pair
+  bool upload_failed;    // an upload failed after freeing the previous particles:
+                         // steps are rejected until an upload succeeds
+  double *d_red;         // energy reduction: 2 results + 2 per reduction block
   bool keys_valid;       // d_key holds keys of the current positions
   bool sorted;           // order/tiles valid for the current step
   bool tl_identity;      // TLMPM: the state is in its final sorted order (keys

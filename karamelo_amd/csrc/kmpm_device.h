@@ -96,6 +96,8 @@ enum MatClass { MC_GENERIC = 0, MC_LINEAR = 1, MC_JC_SHOCK = 2, MC_FLUID = 3 };
 struct DevFixVel { uint32_t bit; int set[3]; double val[3], prev[3]; };
 struct DevFixBF { uint32_t bit; int set[3]; double val[3]; };
 constexpr int MAX_FIX = 8;
+// Blocks of the particle energy reduction (k_energy), summed in a fixed order.
+constexpr int ENERGY_BLOCKS = 64;
 // Per-tile cell sort: cells per tile covered by its LDS offset table, and the
 // particles per tile it holds in registers (TILE_SORT_KPL per lane of one
 // wavefront) and orders in LDS; larger tiles order through memory.
@@ -110,7 +112,8 @@ struct DevScalars {
                                   // non-negative double; ordered like the value)
   int64_t step;
   int error;                      // first error code (sticky)
-  int pad0;
+  int frozen;                     // set by the step that raised `error`: later steps
+                                  // launch but change nothing
   int64_t err_pid;                // pid of the first offending particle
   int tile_count;                 // active tiles this step
   int dil_count;                  // tiles whose owned nodes receive contributions

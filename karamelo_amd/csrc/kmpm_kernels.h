@@ -56,6 +56,27 @@ __device__ inline void set_error(DevScalars *sc, int code, int64_t pid) {
   if (atomicCAS(&sc->error, 0, code) == 0) sc->err_pid = pid;
 }
 
+// A step that raises a sticky error ends normally and k_adjust_dt then sets
+// `frozen` (the reference aborts at that point, solid.cpp:617-627,
+// :1208-1215).  Every later step still launches, as a captured graph must,
+// but its kernels return at entry; `frozen` only changes between steps, so
+// the whole grid of a kernel sees one value.  The kernels that write the
+// next state copy the current one into it slot by slot instead, so the
+// host's buffer swap keeps the state of the failing step.
+__device__ inline bool frozen(const KArgs &a) { return a.sc->frozen != 0; }
+
+__device__ inline void frozen_copy(const KArgs &a) {
+  const bool mixed = a.prec == KMPM_PRECISION_MIXED;
+  const int64_t nd = (int64_t)state_nd(mixed, a.tl) * a.cap;
+  const int64_t nf = (int64_t)state_nf(mixed, a.tl) * a.cap;
+  const int64_t ni = (int64_t)NIF * a.cap;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = i0; i < nd; i += stride) a.pb[i] = a.pa[i];
+  for (int64_t i = i0; i < nf; i += stride) a.fb[i] = a.fa[i];
+  for (int64_t i = i0; i < ni; i += stride) a.ib[i] = a.ia[i];
+}
+
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -313,6 +334,7 @@ __device__ inline unsigned long long fx_word_fast(float val, double scale) {
 // scatter is deterministic).
 template <int DIM, int S1, int TX, int TY, int TZ, bool REMAP, int PREC, bool TL>
 __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
+  if (frozen(a)) return;
   using R = typename Real<PREC>::T;
   using R2 = typename Real<PREC>::T2;
   constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
@@ -527,6 +549,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
 // ---------------------------------------------------------------------------
 template <int DIM, int S1, int TX, int TY, int TZ, int PREC>
 __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
+  if (frozen(a)) return;
   using R = typename Real<PREC>::T;
   constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
   using BT = typename std::conditional<FX, long long, double>::type;
@@ -1032,6 +1055,10 @@ __global__ __launch_bounds__(256, SURR ? 4
                                                   (MC == MC_LINEAR || MC == MC_JC_SHOCK)
                                               ? 4
                                               : 1)) void k_g2p_usl(KArgs a) {
+  if (frozen(a)) {
+    frozen_copy(a);
+    return;
+  }
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
@@ -1220,6 +1247,10 @@ __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, 
 // G2P, advance (no box check), then tl_update.
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
+  if (frozen(a)) {
+    frozen_copy(a);
+    return;
+  }
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
@@ -1293,6 +1324,7 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
 // TL: weights of x0 on the solid's grid, no box check, x0 carried along.
 template <int DIM, int S1, int TX, int TY, int TZ, int PREC, bool TL>
 __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
+  if (frozen(a)) return;
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
@@ -1366,6 +1398,10 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
 // ---------------------------------------------------------------------------
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
+  if (frozen(a)) {
+    frozen_copy(a);
+    return;
+  }
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
@@ -1450,6 +1486,10 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
 // tl_update.
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
+  if (frozen(a)) {
+    frozen_copy(a);
+    return;
+  }
   using R = typename Real<PREC>::T;
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;

@@ -26,6 +26,7 @@ namespace kmpm {
 // sort
 // ---------------------------------------------------------------------------
 __global__ void k_keys(KArgs a) {
+  if (frozen(a)) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
   const int f = wpos(a);
@@ -36,6 +37,7 @@ __global__ void k_keys(KArgs a) {
 // resets them otherwise).  Per-tile counts are zero between sorts: k_tiles
 // clears them once the scan has consumed them.
 __global__ void k_sort_begin(KArgs a) {
+  if (frozen(a)) return;
   if (threadIdx.x == 0) {
     a.sc->tile_count = 0;
     a.sc->dil_count = 0;
@@ -107,6 +109,7 @@ __device__ inline void count_rows(const KArgs &a, int i0, int lane) {
 // sorted order, so equal tiles come in runs: one atomic per run of a tile
 // inside a wavefront's row.
 __global__ __launch_bounds__(SORT_BLOCK) void k_count(KArgs a) {
+  if (frozen(a)) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.sc->tile_count = 0;
     a.sc->dil_count = 0;
@@ -146,6 +149,7 @@ __device__ inline void scatter_rows(const KArgs &a, int i0, int lane) {
 // range of the tile's positions from its cursor.  The order inside a tile
 // is settled by k_tile_sort.
 __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(KArgs a) {
+  if (frozen(a)) return;
   const int lane = threadIdx.x & 63;
   const int w0 = blockIdx.x * SORT_PER_BLOCK + (threadIdx.x >> 6) * SORT_PER_WAVE;
   if (w0 >= a.np) return;
@@ -159,6 +163,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(KArgs a) {
 // and clears the per-tile counts for the next sort (the scan has consumed
 // them).
 __global__ void k_tiles(KArgs a) {
+  if (frozen(a)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const TileGeom &g = a.tg;
   const bool in = t < g.ntiles;
@@ -214,6 +219,7 @@ __global__ void k_tiles(KArgs a) {
 // binary search of the offsets.  The next tile's range is fetched while the
 // current one is ordered.  Dynamic LDS: tile_sort_lds_bytes(cpt).
 __global__ __launch_bounds__(64) void k_tile_sort(KArgs a) {
+  if (frozen(a)) return;
   extern __shared__ int lds[];
   const int cpt = a.tg.cpt, cmask = cpt - 1;
   int *off = lds;                       // [cpt + 1] first position of each cell
@@ -344,10 +350,15 @@ __device__ inline void adjust_dt_instances(const KArgs &a) {
   if (threadIdx.x == 0) a.sc->step += 1;
 }
 
+// The end of a step: time and dt (update.cpp:201-206, ulmpm.cpp:525-551).  A
+// step that raised a sticky error freezes the context (see frozen()); a
+// frozen context's time, dt and step count stay as they are.
 __global__ void k_adjust_dt(KArgs a) {
-  if (blockIdx.x != 0) return;
+  if (blockIdx.x != 0 || frozen(a)) return;
   if (a.ninst) {
     adjust_dt_instances(a);
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(&a.sc->error, 0) != 0) a.sc->frozen = 1;
     return;
   }
   if (threadIdx.x != 0) return;
@@ -368,6 +379,7 @@ __global__ void k_adjust_dt(KArgs a) {
   sc->max_c_bits = 0;
   sc->min_ratio_bits = (unsigned long long)__double_as_longlong(1.0);
   sc->step += 1;
+  if (sc->error != 0) sc->frozen = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -579,7 +591,7 @@ hipError_t launch_adjust_dt(kmpm_ctx *c) {
 }
 
 hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2) {
-  const int nb = 64;
+  const int nb = ENERGY_BLOCKS;
   PREC_DISPATCH(c, hipLaunchKernelGGL(k_energy<PREC>, dim3(nb), dim3(256), 0, c->stream, c->a,
                                       c->d_pmask, bit, d_out2 + 2));
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, c->stream, d_out2 + 2, nb, d_out2);

@@ -23,6 +23,20 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* The checker (liboracle.so) is serial.  The CPU-baseline build
+ * (liboracle_perf.so, -fopenmp) runs the particle and node loops on every
+ * thread; there the node-major neighbour lists fill in thread order, so its
+ * sums may differ from the checker's in rounding. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#define OMP_ATOMIC _Pragma("omp atomic")
+#define OMP_ATOMIC_CAPTURE _Pragma("omp atomic capture")
+#else
+#define OMP_FOR
+#define OMP_ATOMIC
+#define OMP_ATOMIC_CAPTURE
+#endif
+
 #define SQRT_3_OVER_2 1.224744871                              /* strength_jc.cpp:29 */
 #define FOUR_THIRD 1.333333333333333333333333333333333333333   /* solid.cpp:39 */
 #define MAXFIX 16
@@ -479,6 +493,19 @@ struct orc_sim {
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
+/* first error of the run (the reference aborts on it) */
+static void set_status(orc_sim *s, int code, int64_t tag) {
+#ifdef _OPENMP
+#pragma omp critical(orc_status)
+#endif
+  {
+    if (s->status == KMPM_OK) {
+      s->status = code;
+      s->err_ptag = tag;
+    }
+  }
+}
+
 orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
   int64_t n3[3];
   if (orc_grid_size(c, n3) != KMPM_OK) return NULL;
@@ -686,10 +713,12 @@ int orc_phase_weights(orc_sim *s) {
   int64_t nnodes = s->nn;
   int lin = s->c.shape == KMPM_SHAPE_LINEAR;
   int S1 = s->S1;
-  int64_t cand[64];
   (void)nx;
+  OMP_FOR
   for (int64_t in = 0; in <= s->nn; in++) s->np_start[in] = 0;
+  OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
+    int64_t cand[64];
     /* TLMPM: reference positions against the solid's grid (tlmpm.cpp:152-282) */
     const double *xp = s->tl ? &s->x0[3 * ip] : &s->x[3 * ip];
     int i0, j0, k0;
@@ -752,6 +781,7 @@ int orc_phase_weights(orc_sim *s) {
         s->wfd_pn[3 * q + 1] = sv[0] * sd[1];
         s->wfd_pn[3 * q + 2] = 0;
       }
+      OMP_ATOMIC
       s->np_start[in + 1]++;
       cnt++;
     }
@@ -761,10 +791,13 @@ int orc_phase_weights(orc_sim *s) {
    * push_back at :269-275 builds them */
   for (int64_t in = 0; in < s->nn; in++) s->np_start[in + 1] += s->np_start[in];
   int64_t *fill = (int64_t *)xcalloc(s->nn, sizeof(int64_t));
+  OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++)
     for (int c = 0; c < s->nn_pn[ip]; c++) {
-      int64_t q = ip * s->S + c, in = s->neigh_pn[q];
-      int64_t o = s->np_start[in] + fill[in]++;
+      int64_t q = ip * s->S + c, in = s->neigh_pn[q], k;
+      OMP_ATOMIC_CAPTURE
+      k = fill[in]++;
+      int64_t o = s->np_start[in] + k;
       s->neigh_np[o] = ip;
       s->wf_np[o] = s->wf_pn[q];
       for (int d = 0; d < 3; d++) s->wfd_np[3 * o + d] = s->wfd_pn[3 * q + d];
@@ -787,15 +820,19 @@ int orc_phase_p2g(orc_sim *s) {
     if (s->solid[ip] + 1 > nsolid_max) nsolid_max = s->solid[ip] + 1;
   /* mass: solid 0 resets, every solid accumulates in ascending ip; TLMPM
    * computes it on the first step only (tlmpm.cpp:344-351) */
-  for (int64_t in = 0; in < s->nn && !(s->tl && s->mass_done); in++) {
-    int64_t b = s->np_start[in], e = s->np_start[in + 1];
-    s->gmass[in] = 0;
-    for (int64_t o = b; o < e; o++) {
-      int64_t ip = s->neigh_np[o];
-      s->gmass[in] += s->wf_np[o] * s->mass[ip];
+  if (!(s->tl && s->mass_done)) {
+    OMP_FOR
+    for (int64_t in = 0; in < s->nn; in++) {
+      int64_t b = s->np_start[in], e = s->np_start[in + 1];
+      s->gmass[in] = 0;
+      for (int64_t o = b; o < e; o++) {
+        int64_t ip = s->neigh_np[o];
+        s->gmass[in] += s->wf_np[o] * s->mass[ip];
+      }
     }
   }
   s->mass_done = 1;
+  OMP_FOR
   for (int64_t in = 0; in < s->nn; in++) {
     int64_t b = s->np_start[in], e = s->np_start[in + 1];
     double *gv = &s->gv[3 * in], *gf = &s->gf[3 * in], *gmb = &s->gmb[3 * in];
@@ -842,6 +879,7 @@ int orc_phase_p2g(orc_sim *s) {
   /* body force fix (fix_body_force.cpp:117-146) */
   for (int k = 0; k < s->nbf; k++) {
     fixbf_t *f = &s->bf[k];
+    OMP_FOR
     for (int64_t in = 0; in < s->nn; in++) {
       if (s->gmass[in] > 0 && (s->gmask[in] & f->bit)) {
         double fv[3] = {0, 0, 0};
@@ -861,6 +899,7 @@ int orc_phase_p2g(orc_sim *s) {
 /* ------------------------------------------------------------------------ */
 int orc_phase_grid_update(orc_sim *s) {
   double dt = s->dt;
+  OMP_FOR
   for (int64_t in = 0; in < s->nn; in++) {
     double m = s->gmass[in];
     for (int d = 0; d < 3; d++) {
@@ -872,6 +911,7 @@ int orc_phase_grid_update(orc_sim *s) {
   }
   for (int k = 0; k < s->nfv; k++) {
     fixvel_t *f = &s->fv[k];
+    OMP_FOR
     for (int64_t in = 0; in < s->nn; in++) {
       if (s->gmask[in] & f->bit) {
         for (int d = 0; d < 3; d++)
@@ -886,6 +926,7 @@ int orc_phase_grid_update(orc_sim *s) {
  * compute_velocity_nodes (solid.cpp:337-390), then
  * FixVelocityNodes::post_velocities_to_grid (fix_velocity_nodes.cpp:227-271) */
 static void phase_velocities_to_grid(orc_sim *s) {
+  OMP_FOR
   for (int64_t in = 0; in < s->nn; in++) {
     int64_t b = s->np_start[in], e = s->np_start[in + 1];
     double *gv = &s->gv[3 * in];
@@ -909,6 +950,7 @@ static void phase_velocities_to_grid(orc_sim *s) {
   }
   for (int k = 0; k < s->nfv; k++) {
     fixvel_t *f = &s->fv[k];
+    OMP_FOR
     for (int64_t in = 0; in < s->nn; in++)
       if (s->gmask[in] & f->bit)
         for (int d = 0; d < 3; d++) if (f->set[d]) s->gv[3 * in + d] = f->val[d];
@@ -920,6 +962,7 @@ static void phase_velocities_to_grid(orc_sim *s) {
  * same sums over the reference gradients into Fdot */
 static void phase_rate_of_deformation(orc_sim *s, int doublemapping) {
   const double *vn = doublemapping ? s->gv : s->gvu;
+  OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
     double *L = s->tl ? &s->Fdot[9 * ip] : &s->L[9 * ip];
     for (int e = 0; e < 9; e++) L[e] = 0;
@@ -945,6 +988,7 @@ static int phase_g2p(orc_sim *s) {
   double inv_dt = 1.0 / s->dt;
   double alpha = s->c.pic_flip;
   const double *lo = s->c.boxlo, *hi = s->c.boxhi;
+  OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
     double *vu = &s->vu[3 * ip], *a = &s->a[3 * ip];
     for (int d = 0; d < 3; d++) vu[d] = a[d] = 0;
@@ -960,9 +1004,10 @@ static int phase_g2p(orc_sim *s) {
     if (!s->tl &&        /* the box check is UL only (solid.cpp:617-627) */
         !(x[0] >= lo[0] && x[0] <= hi[0] && x[1] >= lo[1] && x[1] <= hi[1] &&
           x[2] >= (s->dim == 3 ? lo[2] : 0) && x[2] <= (s->dim == 3 ? hi[2] : 0))) {
-      if (s->status == KMPM_OK) { s->status = KMPM_ERR_PARTICLE_LEFT_BOX; s->err_ptag = s->ptag[ip]; }
+      set_status(s, KMPM_ERR_PARTICLE_LEFT_BOX, s->ptag[ip]);
     }
   }
+  OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++)
     for (int d = 0; d < 3; d++)
       s->v[3 * ip + d] = (1 - alpha) * s->vu[3 * ip + d] +
@@ -1082,6 +1127,7 @@ void orc_jc_damage(const kmpm_material *mt, double *damage_init, double *damage,
 static int phase_F_and_stress(orc_sim *s) {
   double dt = s->dt;
   double max_c = 0;
+  OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
     double *F = &s->F[9 * ip], *L = &s->L[9 * ip], *D = &s->D[9 * ip];
     if (s->tl) {                                                /* :1181-1182 */
@@ -1098,15 +1144,13 @@ static int phase_F_and_stress(orc_sim *s) {
     s->J[ip] = det3(F);
     s->vol[ip] = s->J[ip] * s->vol0[ip];
     if (s->J[ip] <= 0.0 && s->dmg[ip] < 1.0) {
-      if (s->status == KMPM_OK) { s->status = KMPM_ERR_JACOBIAN; s->err_ptag = s->ptag[ip]; }
+      set_status(s, KMPM_ERR_JACOBIAN, s->ptag[ip]);
     }
     s->rho[ip] = s->rho0[ip] / s->J[ip];
     if (s->tl) {                                                /* :1218-1234 */
       double *R = &s->R[9 * ip];
-      if (!polar_rotation(F, R) && s->status == KMPM_OK) {
-        s->status = KMPM_ERR_JACOBIAN;                          /* PolDec failure aborts */
-        s->err_ptag = s->ptag[ip];
-      }
+      if (!polar_rotation(F, R))                                /* PolDec failure aborts */
+        set_status(s, KMPM_ERR_JACOBIAN, s->ptag[ip]);
       double Lt[9], Sym[9], Rt[9], T1[9], T2[9];
       m3_mul(&s->Fdot[9 * ip], &s->Finv[9 * ip], L);            /* L = Fdot F^-1 */
       m3_transpose(L, Lt);
@@ -1121,6 +1165,7 @@ static int phase_F_and_stress(orc_sim *s) {
     }
   }
   double tav_h = 1000 * s->h;
+  OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
     const kmpm_material *mt = &s->mats[s->mat[ip]];
     double *sg = &s->sig[9 * ip], *D = &s->D[9 * ip], *eel = &s->eel[9 * ip];
@@ -1175,12 +1220,17 @@ static int phase_F_and_stress(orc_sim *s) {
       for (int e = 0; e < 9; e++) eel[e] = t3 * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e] / Gd;
     }
   }
-  if (s->tl)                                                    /* :1277-1280,1368-1372 */
+  if (s->tl) {                                                  /* :1277-1280,1368-1372 */
+    OMP_FOR
     for (int64_t ip = 0; ip < s->np; ip++)
       pk1(s->vol0[ip], s->J[ip], &s->R[9 * ip], &s->sig[9 * ip], &s->Finv[9 * ip],
           &s->PK1[9 * ip]);
+  }
   /* CFL (:1376-1437); UL: min_h_ratio = 1, TL: the smallest |Re eig(F)| */
   double min_h_ratio = 1.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(max : max_c) reduction(min : min_h_ratio)
+#endif
   for (int64_t ip = 0; ip < s->np; ip++) {
     if (s->dmg[ip] >= 1.0) continue;
     const kmpm_material *mt = &s->mats[s->mat[ip]];
@@ -1188,7 +1238,7 @@ static int phase_F_and_stress(orc_sim *s) {
     max_c = dmax(max_c, sqrt((mt->K + FOUR_THIRD * mt->G) / s->rho[ip]) +
                             dmax(dmax(fabs(v[0]), fabs(v[1])), fabs(v[2])));
     if (isnan(max_c)) {
-      if (s->status == KMPM_OK) { s->status = KMPM_ERR_WAVE_SPEED; s->err_ptag = s->ptag[ip]; }
+      set_status(s, KMPM_ERR_WAVE_SPEED, s->ptag[ip]);
     }
     if (s->tl) {
       double re[3];

@@ -446,6 +446,57 @@ def test_negative_jacobian_is_reported():
     ctx.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_steps_after_an_error_change_nothing(graph):
+    """The reference aborts on a particle leaving the box (solid.cpp:617-627):
+    after the step that raised the error, further steps (eager or graph
+    replays) leave the state, time and step count as they are."""
+    say("error freezes the context (graph=%s)" % graph)
+    prob = P.two_disks()
+    prob.particles["v"][5, 0] = -100.0
+    ctx = P.context_for(prob)
+    ctx.step(20)
+    st, tag = ctx.synchronize()
+    assert st == N.ERR_PARTICLE_LEFT_BOX
+    before, t_before = ctx.download(), ctx.time()
+    if graph:
+        ctx.step_graph(12, 4)
+    else:
+        ctx.step(7)
+    st2, tag2 = ctx.synchronize()
+    assert (st2, tag2) == (st, tag)
+    after, t_after = ctx.download(), ctx.time()
+    assert t_after == t_before
+    for f in ("x", "v", "sigma", "F", "strain_el", "ptag"):
+        assert np.array_equal(after[f], before[f]), f
+    ctx.close()
+
+
+def test_rejected_upload_keeps_the_previous_particles():
+    """An upload rejected by its checks (here a particle whose F has no
+    polar decomposition on a TLMPM context) changes nothing: the context
+    keeps its particles and steps on as if the call had not been made."""
+    say("rejected upload keeps the previous particles")
+    prob = P.taylor_bar(h=8e-4)
+    a = P.context_for(prob)
+    b = P.context_for(prob)
+    a.step(3)
+    b.step(3)
+    bad = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in prob.particles.items()}
+    bad["F"][4] = [1, 0, 0, 0, 1, 0, 0, 0, -1]
+    with pytest.raises(N.KmpmError) as e:
+        a.upload(bad)
+    assert e.value.code == N.ERR_JACOBIAN
+    a.step(5)
+    b.step(5)
+    assert a.synchronize()[0] == 0 and b.synchronize()[0] == 0
+    pa, pb = a.download(), b.download()
+    for f in ("x", "v", "sigma", "F"):
+        assert np.array_equal(pa[f], pb[f]), f
+    a.close()
+    b.close()
+
+
 def test_full_size_tensile_bar_properties():
     """4.27M particles: cell indices of the reference rule, a complete sort
     permutation ordered by (tile, cell, ptag), and mass and momentum
