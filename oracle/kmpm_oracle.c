@@ -1122,6 +1122,64 @@ void orc_jc_damage(const kmpm_material *mt, double *damage_init, double *damage,
   if (*damage_init >= 1.0) *damage = dmin((*damage_init - 1.0) * 10, 1.0);
 }
 
+/* Solid::update_stress solid.cpp:1246-1374 for one particle: the LINEAR
+ * material (:1268-1280), or EOS + strength (+ damage) with the eff. plastic
+ * strain, its rate filter (:1319-1329), the tension rule (:1353-1356) and
+ * the elastic strain (:1358-1366).  h is the cell size (tav = 1000 h / c). */
+void orc_update_stress_point(const kmpm_material *mt, double h, double dt, double J, double rho,
+                             const double D[9], double sg[9], double eel[9], double *ep,
+                             double *epdot, double *dinit, double *dmg) {
+  if (mt->type == KMPM_MAT_LINEAR) {                           /* :1268-1280 */
+    double si[9];
+    for (int e = 0; e < 9; e++) si[e] = dt * D[e];
+    for (int e = 0; e < 9; e++) eel[e] += si[e];
+    double tr = trace3(si);
+    for (int e = 0; e < 9; e++)
+      sg[e] += 2 * mt->G * si[e] + mt->lambda * tr * ((e % 4 == 0) ? 1.0 : 0.0);
+    return;
+  }
+  double pH = 0, dps = 0, sdev[9];                              /* :1292-1374 */
+  /* EOS */
+  if (mt->eos == KMPM_EOS_SHOCK) {
+    pH = orc_shock_pressure(mt, J, rho, *dmg, D, h);
+  } else if (mt->eos == KMPM_EOS_FLUID) {                       /* eos_fluid.cpp:68-73 */
+    double mu = rho / mt->eos_p[0];
+    pH = mt->eos_p[1] * (pow(mu, mt->eos_p[2]) - 1.0);
+  } else {                                                      /* eos_linear.cpp:71-74 */
+    pH = mt->eos_p[1] * (1 - J) * (1 - *dmg);
+  }
+  /* strength */
+  if (mt->strength == KMPM_STRENGTH_JOHNSON_COOK) {
+    orc_jc_strength(mt, sg, D, dt, *ep, *epdot, *dmg, sdev, &dps);
+  } else if (mt->strength == KMPM_STRENGTH_FLUID) {             /* strength_fluid.cpp:46-58 */
+    double dv[9];
+    deviator(D, dv);
+    for (int e = 0; e < 9; e++) sdev[e] = 2.0 * mt->strength_p[0] * dv[e];
+  } else {                                                      /* strength_linear.cpp:59-71 */
+    double dv[9], ds[9];
+    deviator(D, dv);
+    deviator(sg, ds);
+    double c2 = 2.0 * mt->strength_p[0] * (1 - *dmg);
+    for (int e = 0; e < 9; e++) sdev[e] = ds[e] + dt * (c2 * dv[e]);
+  }
+  *ep += dps;                                                   /* :1319 */
+  double tav = 1000 * h / mt->signal_velocity;                  /* :1324-1329 */
+  *epdot -= *epdot * dt / tav;
+  *epdot += dps / tav;
+  *epdot = dmax(0.0, *epdot);
+  if (mt->damage == KMPM_DAMAGE_JOHNSON_COOK)                   /* :1331-1341 */
+    orc_jc_damage(mt, dinit, dmg, pH, sdev, *epdot, dps);
+  double dm = *dmg;
+  if (dm == 0 || pH >= 0) {                                     /* :1353-1356 */
+    for (int e = 0; e < 9; e++) sg[e] = -pH * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e];
+  } else {
+    for (int e = 0; e < 9; e++) sg[e] = -pH * (1.0 - dm) * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e];
+  }
+  double t3 = (dt * trace3(D) + trace3(eel)) / 3.0;             /* :1358-1366 */
+  double Gd = dm > 1e-10 ? mt->G * (1 - dm) : mt->G;
+  for (int e = 0; e < 9; e++) eel[e] = t3 * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e] / Gd;
+}
+
 /* Solid::update_deformation_gradient solid.cpp:1155-1244 (UL),
  * Solid::update_stress solid.cpp:1246-1438 */
 static int phase_F_and_stress(orc_sim *s) {
@@ -1164,62 +1222,11 @@ static int phase_F_and_stress(orc_sim *s) {
         for (int j = 0; j < 3; j++) D[3 * i + j] = 0.5 * (L[3 * i + j] + L[3 * j + i]);
     }
   }
-  double tav_h = 1000 * s->h;
   OMP_FOR
-  for (int64_t ip = 0; ip < s->np; ip++) {
-    const kmpm_material *mt = &s->mats[s->mat[ip]];
-    double *sg = &s->sig[9 * ip], *D = &s->D[9 * ip], *eel = &s->eel[9 * ip];
-    if (mt->type == KMPM_MAT_LINEAR) {                         /* :1268-1280 */
-      double si[9];
-      for (int e = 0; e < 9; e++) si[e] = dt * D[e];
-      for (int e = 0; e < 9; e++) eel[e] += si[e];
-      double tr = trace3(si);
-      for (int e = 0; e < 9; e++)
-        sg[e] += 2 * mt->G * si[e] + mt->lambda * tr * ((e % 4 == 0) ? 1.0 : 0.0);
-    } else {                                                    /* :1292-1374 */
-      double pH = 0, dps = 0, sdev[9];
-      /* EOS */
-      if (mt->eos == KMPM_EOS_SHOCK) {
-        pH = orc_shock_pressure(mt, s->J[ip], s->rho[ip], s->dmg[ip], D, s->h);
-      } else if (mt->eos == KMPM_EOS_FLUID) {                   /* eos_fluid.cpp:68-73 */
-        double mu = s->rho[ip] / mt->eos_p[0];
-        pH = mt->eos_p[1] * (pow(mu, mt->eos_p[2]) - 1.0);
-      } else {                                                  /* eos_linear.cpp:71-74 */
-        pH = mt->eos_p[1] * (1 - s->J[ip]) * (1 - s->dmg[ip]);
-      }
-      /* strength */
-      if (mt->strength == KMPM_STRENGTH_JOHNSON_COOK) {
-        orc_jc_strength(mt, sg, D, dt, s->ep[ip], s->epdot[ip], s->dmg[ip], sdev, &dps);
-      } else if (mt->strength == KMPM_STRENGTH_FLUID) {         /* strength_fluid.cpp:46-58 */
-        double dv[9];
-        deviator(D, dv);
-        for (int e = 0; e < 9; e++) sdev[e] = 2.0 * mt->strength_p[0] * dv[e];
-      } else {                                                  /* strength_linear.cpp:59-71 */
-        double dv[9], ds[9];
-        deviator(D, dv);
-        deviator(sg, ds);
-        double c2 = 2.0 * mt->strength_p[0] * (1 - s->dmg[ip]);
-        for (int e = 0; e < 9; e++) sdev[e] = ds[e] + dt * (c2 * dv[e]);
-      }
-      s->ep[ip] += dps;                                         /* :1319 */
-      double tav = tav_h / mt->signal_velocity;                 /* :1324-1329 */
-      s->epdot[ip] -= s->epdot[ip] * dt / tav;
-      s->epdot[ip] += dps / tav;
-      s->epdot[ip] = dmax(0.0, s->epdot[ip]);
-      if (mt->damage == KMPM_DAMAGE_JOHNSON_COOK)               /* :1331-1341 */
-        orc_jc_damage(mt, &s->dinit[ip], &s->dmg[ip], pH, sdev, s->epdot[ip], dps);
-      double dm = s->dmg[ip];
-      if (dm == 0 || pH >= 0) {                                 /* :1353-1356 */
-        for (int e = 0; e < 9; e++) sg[e] = -pH * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e];
-      } else {
-        for (int e = 0; e < 9; e++)
-          sg[e] = -pH * (1.0 - dm) * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e];
-      }
-      double t3 = (dt * trace3(D) + trace3(eel)) / 3.0;         /* :1358-1366 */
-      double Gd = dm > 1e-10 ? mt->G * (1 - dm) : mt->G;
-      for (int e = 0; e < 9; e++) eel[e] = t3 * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e] / Gd;
-    }
-  }
+  for (int64_t ip = 0; ip < s->np; ip++)
+    orc_update_stress_point(&s->mats[s->mat[ip]], s->h, dt, s->J[ip], s->rho[ip], &s->D[9 * ip],
+                            &s->sig[9 * ip], &s->eel[9 * ip], &s->ep[ip], &s->epdot[ip],
+                            &s->dinit[ip], &s->dmg[ip]);
   if (s->tl) {                                                  /* :1277-1280,1368-1372 */
     OMP_FOR
     for (int64_t ip = 0; ip < s->np; ip++)

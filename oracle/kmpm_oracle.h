@@ -74,6 +74,12 @@ double orc_shock_pressure(const kmpm_material *m, double J, double rho, double d
 void orc_jc_damage(const kmpm_material *m, double *damage_init, double *damage,
                    double pH, const double sdev[9], double epsdot, double dps);
 double orc_basis(int shape, int derivative, double r, int ntype, double inv_h);
+/* Solid::update_stress for one particle (solid.cpp:1246-1374): sigma,
+ * strain_el (row-major 3x3), eff. plastic strain, its rate, damage_init and
+ * damage are updated in place from D (3x3), J, rho, dt and the cell size */
+void orc_update_stress_point(const kmpm_material *m, double cellsize, double dt, double J,
+                             double rho, const double D[9], double sigma[9], double strain_el[9],
+                             double *eps_p, double *epsdot, double *damage_init, double *damage);
 /* TLMPM matrix functions: rotation of the polar decomposition of F (row-major
  * 3x3; returns 0 when det F <= 0) and the real parts of F's eigenvalues */
 int orc_polar_rotation(const double F[9], double R[9]);

@@ -310,3 +310,239 @@ def test_sort_permutation_is_sorted_and_complete():
     o = O.oracle_for(prob)
     perm = o.sort_permutation((4, 4, 4))
     assert sorted(perm.tolist()) == sorted(prob.particles["ptag"].tolist())
+
+
+# ---------------------------------------------------------------------------
+# Solid::update_stress for one particle (solid.cpp:1246-1374) and the step's
+# composition quirks, each against a closed form or a textbook relation.
+# ---------------------------------------------------------------------------
+SQ32 = 1.224744871          # the reference's truncated sqrt(3/2) (strength_jc.cpp:29)
+
+
+class _State:
+    """One particle's stress state, updated in place by the oracle."""
+
+    def __init__(self, sigma=None, strain_el=None, ep=0.0, epdot=0.0, dinit=0.0, dmg=0.0):
+        self.sig = np.zeros(9) if sigma is None else np.array(sigma, float).ravel()
+        self.eel = np.zeros(9) if strain_el is None else np.array(strain_el, float).ravel()
+        self.ep, self.epdot = C.c_double(ep), C.c_double(epdot)
+        self.dinit, self.dmg = C.c_double(dinit), C.c_double(dmg)
+
+    def update(self, m, h, dt, J, rho, D):
+        d = np.ascontiguousarray(np.asarray(D, float).ravel())
+        O.load().orc_update_stress_point(C.byref(m), h, dt, J, rho, O._dp(d), O._dp(self.sig),
+                                         O._dp(self.eel), C.byref(self.ep), C.byref(self.epdot),
+                                         C.byref(self.dinit), C.byref(self.dmg))
+        return self.sig.reshape(3, 3)
+
+
+def _load_point():
+    L = O.load()
+    L.orc_update_stress_point.restype = None
+    D = C.POINTER(C.c_double)
+    L.orc_update_stress_point.argtypes = [C.POINTER(N.Material), C.c_double, C.c_double,
+                                          C.c_double, C.c_double, D, D, D, D, D, D, D]
+    return L
+
+
+def _perfect_plastic_steel():
+    """JC with B = 0 and C = 0: a constant yield stress A, no rate term."""
+    return P.eos_strength_material("shock", (7850.0, 160e9, 4570.0, 1.49, 2.17, 477.0, 0.0, 0, 0),
+                                   "johnson_cook",
+                                   (80e9, 792e6, 0.0, 0.26, 1.0, 0.0, 1.0, 0.0, 1793.0))
+
+
+def test_plastic_strain_rate_filter_closed_form():
+    """Steady plastic flow in pure shear at a constant yield stress: every
+    step returns the same increment dps = sqrt(3/2) * 2 dt |D| / 3 (the
+    trial stress stays parallel to the stress on the yield surface), and
+    the filter epsdot <- epsdot (1 - dt/tav) + dps/tav with
+    tav = 1000 h / c_signal, c_signal = sqrt((K + 4G/3)/rho0)
+    (solid.cpp:1324-1329, material.cpp:731-734) has the geometric closed
+    form epsdot_k = (dps/dt)(1 - (1 - dt/tav)^k) -> dps/dt."""
+    _load_point()
+    m = _perfect_plastic_steel()
+    rho0, K, G, A = 7850.0, 160e9, 80e9, 792e6
+    h, dt = 1e-3, 1e-7
+    D = np.array([[0, 1, 0], [1, 0, 0], [0, 0, 0]]) * 5e3         # |D| dt = 7e-4
+    nD = np.linalg.norm(D)
+    s0 = A / (SQ32 * nD) * D                                       # on the yield surface
+    st = _State(sigma=s0, ep=0.01)
+    c = math.sqrt((K + 4.0 / 3.0 * G) / rho0)
+    tav = 1000 * h / c
+    dps = SQ32 * 2 * dt * nD / 3
+    q = 1 - dt / tav
+    ep0 = 0.01
+    for k in range(1, 2001):
+        st.update(m, h, dt, 1.0, rho0, D)
+        if k in (1, 2, 10, 500, 2000):
+            want = dps / dt * (1 - q ** k)
+            assert abs(st.epdot.value - want) <= 1e-9 * want, (k, st.epdot.value, want)
+            assert abs(st.ep.value - (ep0 + k * dps)) <= 1e-9 * k * dps
+    # the stress stays on the yield surface, parallel to D
+    s = st.sig.reshape(3, 3)
+    assert abs(SQ32 * np.linalg.norm(s) - A) <= 1e-9 * A
+    assert np.allclose(s / np.linalg.norm(s), D / nD, atol=1e-12)
+    # elastic steps (D = 0): no increment, the rate decays as (1 - dt/tav)^k
+    e0 = st.epdot.value
+    for k in range(1, 301):
+        st.update(m, h, dt, 1.0, rho0, np.zeros((3, 3)))
+    assert abs(st.epdot.value - e0 * q ** 300) <= 1e-9 * e0
+
+
+@pytest.mark.parametrize("eos", ["shock", "linear"])
+def test_tension_rule_scales_pressure_twice(eos):
+    """A damaged particle in tension: the EOS already scales a negative
+    pressure by (1 - D) (eos_shock.cpp:109-121, eos_linear.cpp:71-74) and
+    the stress assembly scales it again (solid.cpp:1353-1356), so
+    sigma = -p_undamaged (1 - D)^2 I; in compression sigma = -p I."""
+    _load_point()
+    rho0, K, c0, S = 8960.0, 1.4e11, 3940.0, 1.49
+    if eos == "shock":
+        m = P.eos_strength_material("shock", (rho0, K, c0, S, 2.0, 0.0, 0.0, 0, 0),
+                                    "linear", (45e9,))
+    else:
+        m = P.eos_strength_material("linear", (rho0, K), "linear", (45e9,))
+
+    def p_undamaged(J):
+        if eos == "linear":
+            return K * (1 - J)
+        mu = (rho0 / J) / rho0 - 1
+        return rho0 * c0 ** 2 * mu * (1 + mu) / (1 - (S - 1) * mu) ** 2
+
+    Dmg = 0.5
+    for J in (1.02, 0.97):
+        st = _State(dmg=Dmg)
+        sig = st.update(m, 1e-3, 1e-8, J, rho0 / J, np.zeros((3, 3)))
+        p = p_undamaged(J)
+        if J > 1:
+            assert p < 0
+            want = -p * (1 - Dmg) ** 2
+        else:
+            assert p > 0
+            want = -p if eos == "shock" else -p * (1 - Dmg)    # linear EOS: (1-D) always
+        assert np.allclose(np.diag(sig), want, rtol=1e-12, atol=0), (J, np.diag(sig), want)
+        assert np.all(sig[~np.eye(3, dtype=bool)] == 0)
+
+
+def test_fluid_eos_and_viscous_strength():
+    """eos fluid p = K((rho/rho0)^Gamma - 1): p(rho0) = 0 and p(2 rho0) =
+    K (2^7 - 1) for Gamma = 7 (eos_fluid.cpp:68-73); strength fluid is
+    viscous, s = 2 mu dev(D), not incremental (strength_fluid.cpp:46-58)."""
+    _load_point()
+    rho0, K, gam, mu = 1000.0, 2e6, 7.0, 1e-3
+    m = P.eos_strength_material("fluid", (rho0, K, gam), "fluid", (mu,))
+    D = np.array([[1.0, 2.0, 0.0], [2.0, -3.0, 0.5], [0.0, 0.5, 4.0]])
+    devD = D - np.trace(D) / 3 * np.eye(3)
+    st = _State(sigma=np.diag([5.0, 6.0, 7.0]))                  # previous stress is forgotten
+    sig = st.update(m, 1e-2, 1e-4, 0.5, 2 * rho0, D)
+    want = -127.0 * K * np.eye(3) + 2 * mu * devD
+    assert np.allclose(sig, want, rtol=1e-13, atol=1e-12 * K)
+    st = _State()
+    sig = st.update(m, 1e-2, 1e-4, 1.0, rho0, D)
+    assert np.allclose(sig, 2 * mu * devD, rtol=1e-13, atol=0)
+    assert st.ep.value == 0 and st.epdot.value == 0
+
+
+def test_linear_strength_and_elastic_strain():
+    """strength linear: s = dev(sigma) + dt 2G (1 - D) dev(D)
+    (strength_linear.cpp:59-71); eos linear p = K (1 - J)(1 - D); the
+    elastic strain becomes (dt tr D + tr eps_el)/3 I + s / (G (1 - D))
+    (solid.cpp:1358-1366)."""
+    _load_point()
+    rho0, K, G = 2700.0, 7e10, 2.6e10
+    m = P.eos_strength_material("linear", (rho0, K), "linear", (G,))
+    Dmg, J, dt = 0.2, 0.9, 1e-6
+    sig0 = np.array([[3e6, 1e6, 0], [1e6, -2e6, 5e5], [0, 5e5, 1e6]])
+    eel0 = np.diag([1e-4, 2e-4, -1e-4])
+    D = np.array([[10.0, 3.0, 0.0], [3.0, -4.0, 1.0], [0.0, 1.0, 2.0]])
+    dev = lambda a: a - np.trace(a) / 3 * np.eye(3)
+    st = _State(sigma=sig0, strain_el=eel0, dmg=Dmg)
+    sig = st.update(m, 1e-3, dt, J, rho0 / J, D)
+    s = dev(sig0) + dt * 2 * G * (1 - Dmg) * dev(D)
+    p = K * (1 - J) * (1 - Dmg)
+    assert p > 0
+    assert np.allclose(sig, -p * np.eye(3) + s, rtol=1e-13, atol=1e-9)
+    eel = (dt * np.trace(D) + np.trace(eel0)) / 3 * np.eye(3) + s / (G * (1 - Dmg))
+    assert np.allclose(st.eel.reshape(3, 3), eel, rtol=1e-12, atol=1e-18)
+
+
+def test_linear_material_is_hypoelastic_lame():
+    """material linear: sigma += 2G dt D + lambda tr(dt D) I with the Lame
+    constants of E, nu (solid.cpp:1268-1280): uniaxial strain gives
+    sigma_xx = (lambda + 2G) e, sigma_yy = sigma_zz = lambda e."""
+    _load_point()
+    E, nu = 200e9, 0.3
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    G = E / (2 * (1 + nu))
+    m = P.linear_material(7850.0, E, nu)
+    st = _State()
+    e = 1e-4
+    sig = st.update(m, 1e-3, 1e-6, 1.0, 7850.0, np.diag([e / 1e-6, 0, 0]))
+    assert np.allclose(np.diag(sig), [(lam + 2 * G) * e, lam * e, lam * e], rtol=1e-12)
+    assert np.allclose(st.eel.reshape(3, 3), np.diag([e, 0, 0]), rtol=1e-12)
+
+
+def _one_particle(v, dims=3, solid=0, x=None, mass=1.0, cfg=None, mat=None):
+    arrs = N.empty_particles(1)
+    arrs["ptag"][0] = 1
+    arrs["solid"][0] = solid
+    arrs["x"][0] = x if x is not None else [0.523, 0.411, 0.377][:3]
+    arrs["x0"][0] = arrs["x"][0]
+    arrs["v"][0] = v
+    arrs["F"][0] = np.eye(3).ravel()
+    arrs["mass"][0] = mass
+    arrs["vol0"][0] = arrs["vol"][0] = 1e-3
+    arrs["rho0"][0] = arrs["rho"][0] = 1000.0
+    arrs["J"][0] = 1.0
+    return arrs
+
+
+def test_first_dt_and_cfl_one_particle():
+    """The first step runs with dt = 1e-16 (update.cpp:38); the next dt is
+    dt_factor * h / (sqrt((K + 4G/3)/rho) + max_i |v_i|) (solid.cpp:1376-1399,
+    ulmpm.cpp:525-551) with K, G the textbook moduli of E, nu."""
+    E, nu, rho0, h, fac = 1e7, 0.25, 1000.0, 0.05, 0.9
+    K = E / (3 * (1 - 2 * nu))
+    G = E / (2 * (1 + nu))
+    cfg = P.make_config(3, (0, 0, 0), (1, 1, 1), h, dt=1e-16, dt_factor=fac)
+    cfg.dt_constant = 0
+    m = P.linear_material(rho0, E, nu)
+    v = [3.0, -4.0, 1.0]
+    arrs = _one_particle(v)
+    o = O.Oracle(cfg, [m])
+    o.upload(arrs)
+    assert o.step(1) == 0
+    dt1, t1, n1 = o.time()
+    assert t1 == 1e-16 and n1 == 1
+    J = O.Oracle.download(o)["J"][0]
+    c = math.sqrt((K + 4.0 / 3.0 * G) / (rho0 / J)) + 4.0
+    assert abs(dt1 - fac * h / c) <= 1e-14 * dt1
+    assert o.step(1) == 0
+    dt2, t2, _ = o.time()
+    assert t2 == 1e-16 + dt1
+    o.close()
+
+
+def test_velocity_normalised_per_solid_sums_to_mass_average():
+    """Two particles at the same point in two solids: compute_velocity_nodes
+    adds each solid's sum(w m v) divided by the node's total mass
+    (solid.cpp:357-378), so every node holds the mass-weighted average
+    (m1 v1 + m2 v2)/(m1 + m2) and the grid momentum equals the particles'."""
+    cfg = P.make_config(3, (0, 0, 0), (1, 1, 1), 0.1, shape="cubic-spline")
+    m = P.linear_material(1000.0, 1e6, 0.3)
+    a = _one_particle([1.0, 0.5, 0.0], solid=0, mass=1.0)
+    b = _one_particle([-1.0, 0.0, 2.0], solid=1, mass=3.0)
+    b["ptag"][0] = 2
+    arrs = N.concat_particles([a, b])
+    o = O.Oracle(cfg, [m])
+    o.upload(arrs)
+    o.phase("weights")
+    o.phase("p2g")
+    gm, gv, _, _ = o.grid()
+    hit = gm > 0
+    want = (1.0 * np.array([1.0, 0.5, 0.0]) + 3.0 * np.array([-1.0, 0.0, 2.0])) / 4.0
+    assert hit.sum() == 64
+    assert np.allclose(gv[hit], want, rtol=1e-14, atol=1e-15)
+    assert np.allclose((gm[:, None] * gv).sum(0), 4.0 * want, rtol=1e-13)
+    o.close()
