@@ -61,6 +61,14 @@ enum { KMPM_SHAPE_LINEAR = 0, KMPM_SHAPE_CUBIC_SPLINE = 1 };
  * J and pressure; 64-bit fixed-point node mass, momentum and force sums),
  * for both methods */
 enum { KMPM_PRECISION_FP64 = 0, KMPM_PRECISION_MIXED = 1 };
+/* summation order of the FP64 step: the fast order of the tiled kernels, or
+ * the reference's order (node-major P2G over each node's particles in
+ * ascending index with the per-solid velocity sums, solid.cpp:317-522;
+ * per-particle gathers in the candidate order of ulmpm.cpp:167-265), which
+ * reproduces the reference's roundings.  The reference order costs
+ * O(nodes x particles) per P2G: it is for parity runs of small problems
+ * (ULMPM, FP64, one instance). */
+enum { KMPM_ORDER_FAST = 0, KMPM_ORDER_REFERENCE = 1 };
 
 typedef struct kmpm_config {
   int dimension;          /* 2 or 3 (domain.cpp:496-551) */
@@ -68,7 +76,7 @@ typedef struct kmpm_config {
   int scheme;             /* KMPM_SCHEME_* (reference default: MUSL, update.cpp:42-45) */
   int shape;              /* KMPM_SHAPE_* */
   int precision;          /* KMPM_PRECISION_* */
-  int reserved0;
+  int order;              /* KMPM_ORDER_* (FP64 only) */
   double boxlo[3], boxhi[3];
   double cellsize;        /* grid cell size h */
   double pic_flip;        /* FLIP blending alpha (update.cpp:163-166) */

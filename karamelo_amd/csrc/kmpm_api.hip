@@ -43,7 +43,7 @@ static hipError_t dalloc(T **p, size_t n) {
 
 static void free_particle_buffers(kmpm_ctx *c) {
   void *ptrs[] = {c->d_pa, c->d_pb, c->d_fa, c->d_fb, c->d_xo, c->d_ia, c->d_ib, c->d_key,
-                  c->d_order_tmp, c->d_sort_tmp, c->d_order, c->d_pmask};
+                  c->d_order_tmp, c->d_sort_tmp, c->d_order, c->d_pmask, c->d_solid};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_pa = c->d_pb = c->d_xo = nullptr;
@@ -52,6 +52,7 @@ static void free_particle_buffers(kmpm_ctx *c) {
   c->d_order_tmp = nullptr;
   c->d_sort_tmp = nullptr;
   c->d_pmask = nullptr;
+  c->d_solid = nullptr;
   free(c->h_ptag);
   free(c->h_solid);
   free(c->h_pmask);
@@ -75,6 +76,7 @@ static void refresh_args(kmpm_ctx *c) {
   a.order_tmp = c->d_order_tmp;
   a.sort_tmp = c->d_sort_tmp;
   a.order = c->d_order;
+  a.solid_of = c->d_solid;
 }
 
 static void swap_state(kmpm_ctx *c) {
@@ -364,6 +366,10 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   if (cfg->precision != KMPM_PRECISION_FP64 && cfg->precision != KMPM_PRECISION_MIXED)
     return KMPM_ERR_ARG;
   if (cfg->dimension != 2 && cfg->dimension != 3) return KMPM_ERR_ARG;
+  if (cfg->order != KMPM_ORDER_FAST && cfg->order != KMPM_ORDER_REFERENCE) return KMPM_ERR_ARG;
+  if (cfg->order == KMPM_ORDER_REFERENCE &&
+      (cfg->method != KMPM_METHOD_ULMPM || cfg->precision != KMPM_PRECISION_FP64))
+    return KMPM_ERR_ARG;
   int64_t n3[3];
   int st = kmpm_grid_size(cfg, n3);
   if (st != KMPM_OK) return st;
@@ -392,6 +398,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   KArgs &a = c->a;
   a.dim = cfg->dimension;
   a.tl = cfg->method == KMPM_METHOD_TLMPM;
+  a.ref_order = cfg->order == KMPM_ORDER_REFERENCE;
   a.prec = cfg->precision;
   a.S1 = S1;
   a.scheme = cfg->scheme;
@@ -562,6 +569,7 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   HIPCHK(dalloc(&c->d_sort_tmp, cap));
   HIPCHK(dalloc(&c->d_order, cap));
   HIPCHK(dalloc(&c->d_pmask, cap));
+  HIPCHK(dalloc(&c->d_solid, cap));
   a.np = (int)np;
   a.cap = cap;
   {
@@ -610,6 +618,8 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   if (hipMemcpy(c->d_ia, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(c->d_pmask, c->h_pmask, np * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(c->d_solid, c->h_solid, np * sizeof(int32_t), hipMemcpyHostToDevice) !=
           hipSuccess ||
       launch_iota(c->d_order, (int)np, c->stream) != hipSuccess)
     return fail(KMPM_ERR_HIP);
@@ -722,6 +732,12 @@ int kmpm_fix_body_force(kmpm_ctx *c, uint32_t bit, const int set[3], const doubl
 int kmpm_compute_grid_weight_functions_and_gradients(kmpm_ctx *c) {
   if (!c) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
+  if (c->a.ref_order) {
+    // the reference-order kernels evaluate each pair's weights where they
+    // use them, over the state in upload order
+    c->sorted = true;
+    return KMPM_OK;
+  }
   if (c->tl_identity) {
     // TLMPM weights are those of x0 (computed once, tlmpm.cpp:89,339): after
     // the first step the state is stored in its sorted order and the cell
@@ -744,7 +760,7 @@ int kmpm_particles_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G);
-  HIPCHK(launch_p2g(c, 0));
+  HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 0) : launch_p2g(c, 0));
   tend(c);
   return KMPM_OK;
 }
@@ -753,7 +769,7 @@ int kmpm_update_grid_state(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_GRID);
-  HIPCHK(launch_grid_update(c, 0));
+  HIPCHK(c->a.ref_order ? launch_ref_grid(c, 0) : launch_grid_update(c, 0));
   tend(c);
   return KMPM_OK;
 }
@@ -775,7 +791,7 @@ int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_G2P);
-  HIPCHK(launch_g2p_usl(c));
+  HIPCHK(c->a.ref_order ? launch_ref_g2p_usl(c) : launch_g2p_usl(c));
   tend(c);
   return after_g2p_state_swap(c);
 }
@@ -784,7 +800,7 @@ int kmpm_grid_to_points(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_G2P_A);
-  HIPCHK(launch_g2p_musl_a(c));
+  HIPCHK(c->a.ref_order ? launch_ref_g2p_musl_a(c) : launch_g2p_musl_a(c));
   tend(c);
   return KMPM_OK;
 }
@@ -793,10 +809,10 @@ int kmpm_velocities_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G_B);
-  HIPCHK(launch_p2g(c, 1));
+  HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 1) : launch_p2g(c, 1));
   tend(c);
   tbegin(c, KMPM_PHASE_GRID_B);
-  HIPCHK(launch_grid_update(c, 1));
+  HIPCHK(c->a.ref_order ? launch_ref_grid(c, 1) : launch_grid_update(c, 1));
   tend(c);
   return KMPM_OK;
 }
@@ -805,7 +821,7 @@ int kmpm_update_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_STRESS_B);
-  HIPCHK(launch_stress_musl_b(c));
+  HIPCHK(c->a.ref_order ? launch_ref_stress_musl_b(c) : launch_stress_musl_b(c));
   tend(c);
   return after_g2p_state_swap(c);
 }
@@ -846,7 +862,7 @@ int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
 // ---- batched instances ------------------------------------------------------
 int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
   if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
-  if (ninst > 0 && c->a.tl) return KMPM_ERR_ARG;
+  if (ninst > 0 && (c->a.tl || c->a.ref_order)) return KMPM_ERR_ARG;
   const TileGeom &g = c->a.tg;
   std::vector<int> map(g.ntiles, -1);
   std::vector<InstScalars> is(ninst);
@@ -1159,7 +1175,7 @@ int kmpm_reduce_energy(kmpm_ctx *c, uint32_t bit, double *ek, double *es) {
 
 int kmpm_download_grid(kmpm_ctx *c, double *mass, double *v, double *vu, int64_t nn) {
   if (!c || nn != c->a.nn) return KMPM_ERR_ARG;
-  HIPCHK(launch_grid_clean(c));
+  if (!c->a.ref_order) HIPCHK(launch_grid_clean(c));   // the reference order writes every node
   HIPCHK(hipStreamSynchronize(c->stream));
   // device layout [comp][node] (FP32 values in the mixed path) -> reference
   // layout [node][comp]

@@ -35,6 +35,8 @@ struct KArgs {
   TileGeom tg;
   // particles
   int tl;           // TLMPM: weights from x0 on the solid's grid, F += dt Fdot
+  int ref_order;    // KMPM_ORDER_REFERENCE: the reference-order kernels
+  const int *solid_of;   // solid index by pid
   int prec;         // KMPM_PRECISION_*
   int np, cap;
   double *pa;       // current state: state_nd(mixed, tl) * cap doubles
@@ -115,6 +117,7 @@ struct kmpm_ctx {
   double *d_gmass, *d_gv, *d_gvu;
   uint32_t *d_gmask;
   uint32_t *d_pmask;     // particle group masks (by pid)
+  int *d_solid;          // solid index (by pid)
   kmpm::DevScalars *d_sc;
   kmpm::SurrogateDev *d_surr;
   kmpm::InstScalars *d_inst;
@@ -160,6 +163,12 @@ hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);
 hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double *d_wd);
 hipError_t launch_cells(kmpm_ctx *c, int *d_ijk);
 hipError_t launch_iota(int *d, int n, hipStream_t stream);
+// reference-order FP64 kernels (kmpm_reforder.hip)
+hipError_t launch_ref_p2g(kmpm_ctx *c, int remap);
+hipError_t launch_ref_grid(kmpm_ctx *c, int remap);
+hipError_t launch_ref_g2p_usl(kmpm_ctx *c);
+hipError_t launch_ref_g2p_musl_a(kmpm_ctx *c);
+hipError_t launch_ref_stress_musl_b(kmpm_ctx *c);
 size_t scan_temp_bytes(int64_t n);
 DevMat make_devmat(const kmpm_material &m);
 int surrogate_device_image(const kmpm_surrogate *s, SurrogateDev *out);

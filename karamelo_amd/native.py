@@ -22,6 +22,7 @@ METHOD_ULMPM, METHOD_TLMPM = 0, 1
 SCHEME_USL, SCHEME_MUSL = 0, 1
 SHAPE_LINEAR, SHAPE_CUBIC_SPLINE = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+ORDER_FAST, ORDER_REFERENCE = 0, 1
 MAT_LINEAR, MAT_EOS_STRENGTH = 0, 1
 EOS_NONE, EOS_LINEAR, EOS_SHOCK, EOS_FLUID = 0, 1, 2, 3
 STRENGTH_NONE, STRENGTH_LINEAR, STRENGTH_JOHNSON_COOK, STRENGTH_FLUID = 0, 1, 2, 3
@@ -46,7 +47,7 @@ I64_3 = C.c_int64 * 3
 
 class Config(C.Structure):
     _fields_ = [("dimension", C.c_int), ("method", C.c_int), ("scheme", C.c_int),
-                ("shape", C.c_int), ("precision", C.c_int), ("reserved0", C.c_int),
+                ("shape", C.c_int), ("precision", C.c_int), ("order", C.c_int),
                 ("boxlo", D3), ("boxhi", D3), ("cellsize", C.c_double),
                 ("pic_flip", C.c_double), ("dt", C.c_double), ("dt_constant", C.c_int),
                 ("reserved1", C.c_int), ("dt_factor", C.c_double),

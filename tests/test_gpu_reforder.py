@@ -1,0 +1,130 @@
+"""The reference-order FP64 mode (KMPM_ORDER_REFERENCE) against the oracle,
+through the C ABI.
+
+In this mode the GPU sums P2G node-major over each node's particles in
+ascending index with the per-solid velocity sums (solid.cpp:317-522) and
+gathers each particle's stencil in the reference's candidate order
+(ulmpm.cpp:167-265), with no fused multiply-adds, so the whole step rounds as
+the reference's loops do.  Bar: for steps made of + - * / and sqrt only (the
+linear material, set_dt), every particle field equal to the oracle's bit
+for bit, after every chunk of the 3000 steps of the two-disk problem; with
+the library transcendentals of the other materials (pow, exp, log, whose
+last bits differ between the device's and the host's math libraries), 1e-12
+relative L2.  Energies are parallel reductions: 1e-12.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from karamelo_amd import native as N
+from karamelo_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "sigma", "F", "strain_el", "eff_plastic_strain", "eff_plastic_strain_rate",
+          "damage_init", "damage")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _ref_order(prob):
+    cfg = N.Config.from_buffer_copy(prob.cfg)
+    cfg.order = N.ORDER_REFERENCE
+    return dataclasses.replace(prob, cfg=cfg)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    ok = ~np.isnan(b)
+    nb = np.linalg.norm(b[ok])
+    return np.linalg.norm(a[ok] - b[ok]) / (nb if nb > 0 else 1.0)
+
+
+def run(prob, steps, chunk, exact):
+    ctx = P.context_for(_ref_order(prob))
+    o = O.oracle_for(prob)
+    done, worst = 0, {}
+    while done < steps:
+        n = min(chunk, steps - done)
+        ctx.step(n)
+        assert o.step(n) == 0
+        done += n
+        st, tag = ctx.synchronize()
+        assert st == 0, (st, tag)
+        dtg, tg, sg = ctx.time()
+        dto, to, so = o.time()
+        assert sg == so == done
+        pg, po = ctx.download(), o.download()
+        assert np.array_equal(pg["ptag"], po["ptag"])
+        for f in FIELDS:
+            e = _rel(pg[f], po[f])
+            worst[f] = max(worst.get(f, 0.0), e)
+            if exact:
+                assert np.array_equal(pg[f], po[f], equal_nan=True), (f, done, e)
+            else:
+                assert e <= 1e-12, (f, done, e)
+        if exact:
+            assert (dtg, tg) == (dto, to)
+        else:
+            assert abs(dtg - dto) <= 1e-12 * dto and abs(tg - to) <= 1e-12 * to
+        eg, eo = ctx.energy(), o.energy()
+        for a, b in zip(eg, eo):
+            assert abs(a - b) <= 1e-12 * (abs(eo[0]) + abs(eo[1])), (done, eg, eo)
+    say("  worst rel L2: " + ", ".join("%s=%.1e" % kv for kv in worst.items()))
+    ctx.close()
+    o.close()
+
+
+def test_two_disks_3000_steps_bit_exact():
+    """C1 (2D ULMPM USL, linear shapes, linear elastic disks in contact,
+    set_dt): every field bit-identical to the oracle at every 100 steps."""
+    say("reference order: two disks, 3000 steps")
+    run(P.two_disks(), 3000, 100, exact=True)
+
+
+def test_two_disks_cubic_bit_exact():
+    say("reference order: two disks, cubic shapes, 1000 steps")
+    prob = P.two_disks()
+    prob.cfg.shape = N.SHAPE_CUBIC_SPLINE
+    run(prob, 1000, 100, exact=True)
+
+
+def test_two_disks_musl_cfl_dt_bit_exact():
+    """MUSL (the re-map with the step's start weights) and the CFL dt rule
+    (no set_dt: first dt 1e-16, then dt_factor h / max c)."""
+    say("reference order: two disks, MUSL, CFL dt, 500 steps")
+    prob = P.two_disks()
+    prob.cfg.scheme = N.SCHEME_MUSL
+    prob.cfg.dt_constant = 0
+    prob.cfg.dt = 1e-16
+    run(prob, 500, 100, exact=True)
+
+
+def test_tensile_bar_jc_reference_order():
+    """3D cubic USL, JC + shock + JC damage, the end fixes: transcendental
+    material functions, 1e-12."""
+    say("reference order: tensile bar, JC, 200 steps")
+    run(P.tensile_bar(h=6e-4, v_pull=40.0), 200, 50, exact=False)
+
+
+def test_dam_break_musl_reference_order():
+    say("reference order: dam break MUSL cubic, 150 steps")
+    run(P.dam_break(n_cells=16), 150, 50, exact=False)
+
+
+def test_reference_order_rejects_other_paths():
+    for change in ("mixed", "tlmpm"):
+        prob = P.two_disks() if change == "mixed" else P.taylor_bar(h=8e-4)
+        cfg = N.Config.from_buffer_copy(prob.cfg)
+        cfg.order = N.ORDER_REFERENCE
+        if change == "mixed":
+            cfg.precision = N.PRECISION_MIXED
+        with pytest.raises(N.KmpmError):
+            N.Context(cfg, prob.mats)
