@@ -200,6 +200,54 @@ int kmpm_fix_velocity_nodes(kmpm_ctx *ctx, uint32_t groupbit, const int set[3],
 int kmpm_fix_body_force(kmpm_ctx *ctx, uint32_t groupbit, const int set[3],
                         const double value[3]);
 
+/* ---- fix values that depend on time --------------------------------------
+ * The input script gives each component of fix velocity_nodes as an
+ * expression (fix_velocity_nodes.cpp:64-116) that the reference parses
+ * again at every evaluation (Var::result, var.cpp:52-59, Input::parsev,
+ * input.cpp:374-735).  kmpm_expr_compile turns such a text into a program
+ * of `time` (the time at the start of the step) and `dt` (the input
+ * script's dt variable: 0 in the first step unless set_dt, then the step's
+ * dt, input.cpp:55, ulmpm.cpp:550) that the device evaluates every step,
+ * following the parser's rules: literals read by stof, its operator
+ * precedence and unary-minus handling, constant folding, and constants of
+ * a time-dependent expression re-read from their "%.15f" text.  previous
+ * != 0 first replaces the text "time" by "time - dt" (fix_velocity_nodes.cpp:
+ * 69-77), which gives the value the fix imposes on v.  Variables: time, dt,
+ * PI and the nvar constants names[]/values[] (user variables of the script;
+ * the last definition of a name wins).  Operators + - * / ^ ** e E > >= < <=
+ * == != !; functions exp sqrt cos sin tan log atan2. */
+enum { KMPM_EXPR_MAX = 32, KMPM_EXPR_STACK = 8 };
+enum {
+  KMPM_OP_CONST = 0, KMPM_OP_TIME, KMPM_OP_DT,
+  KMPM_OP_ADD, KMPM_OP_SUB, KMPM_OP_MUL, KMPM_OP_DIV, KMPM_OP_POW,
+  KMPM_OP_GT, KMPM_OP_GE, KMPM_OP_LT, KMPM_OP_LE, KMPM_OP_EQ, KMPM_OP_NE,
+  KMPM_OP_ATAN2,
+  KMPM_OP_NEG, KMPM_OP_NOT, KMPM_OP_EXP, KMPM_OP_SQRT, KMPM_OP_COS, KMPM_OP_SIN,
+  KMPM_OP_TAN, KMPM_OP_LOG
+};
+typedef struct kmpm_expr {
+  int32_t n;                      /* operations in postfix order; 0: not set */
+  int32_t op[KMPM_EXPR_MAX];      /* KMPM_OP_* */
+  double c[KMPM_EXPR_MAX];        /* value of each KMPM_OP_CONST */
+} kmpm_expr;
+/* KMPM_ERR_ARG for text the reference's parser rejects or that uses
+ * anything beyond the list above; KMPM_ERR_CAPACITY beyond KMPM_EXPR_MAX
+ * operations or KMPM_EXPR_STACK stack levels. */
+int kmpm_expr_compile(const char *text, int previous, int nvar, const char *const *names,
+                      const double *values, kmpm_expr *out);
+/* fix velocity_nodes with time-dependent components: value[d] gives
+ * v_update(t) and previous[d] the v it sets in the same step (compile both
+ * from the same text, previous with previous = 1); n == 0 leaves the
+ * component free (NULL in the script).  Not for batched contexts. */
+int kmpm_fix_velocity_nodes_expr(kmpm_ctx *ctx, uint32_t groupbit, const kmpm_expr value[3],
+                                 const kmpm_expr previous[3]);
+/* The reaction force of velocity fix `fix` (0-based, in the order the
+ * velocity_nodes fixes were added) in the last completed step:
+ * sum over its nodes of m (v_imposed - v_update) / dt, the Vars <id>_x,
+ * <id>_y, <id>_z the reference publishes (fix_velocity_nodes.cpp:161-224).
+ * Waits for the stream. */
+int kmpm_fix_velocity_nodes_force(kmpm_ctx *ctx, int fix, double force[3]);
+
 /* ---- step phases (Method virtuals, method.h:33-48) ------------------------ */
 /* compute_grid_weight_functions_and_gradients (ulmpm.cpp:88-337): particle
  * cell indices + deterministic cell sort; weights are evaluated in the

@@ -438,6 +438,8 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
                 hipSuccess;
   ok = ok && dalloc(&c->d_sc, 1) == hipSuccess;
   ok = ok && dalloc(&c->d_red, 2 + 2 * ENERGY_BLOCKS) == hipSuccess;
+  ok = ok && dalloc(&c->d_fexpr, 6 * MAX_FIX) == hipSuccess &&
+       hipMemset(c->d_fexpr, 0, 6 * MAX_FIX * sizeof(kmpm_expr)) == hipSuccess;
   if (ok) {
     c->h_gmask = new std::vector<uint32_t>(a.nn, 1u);   // Grid::grow sets mask = 1 (grid.cpp:429)
     DevScalars sc;
@@ -450,6 +452,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   }
   a.mats = c->d_mats;
   a.sc = c->d_sc;
+  a.fexpr = c->d_fexpr;
   if (!ok) {
     kmpm_ctx_destroy(c);
     return KMPM_ERR_HIP;
@@ -469,7 +472,8 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   drop_graphs(c);
   void *ptrs[] = {c->d_mats, c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
-                  c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst, c->d_red};
+                  c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst, c->d_red,
+                  c->d_fexpr};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   free(c->mats_host);
@@ -709,11 +713,48 @@ int kmpm_fix_velocity_nodes(kmpm_ctx *c, uint32_t bit, const int set[3], const d
   if (!c || !set || !val || !prev || c->a.nfv >= MAX_FIX) return KMPM_ERR_ARG;
   DevFixVel &f = c->a.fv[c->a.nfv++];
   f.bit = bit;
+  f.expr = 0;
   for (int d = 0; d < 3; d++) {
     f.set[d] = set[d] && d < c->a.dim;  // fix_velocity_nodes.cpp:81-116
     f.val[d] = val[d];
     f.prev[d] = prev[d];
   }
+  return KMPM_OK;
+}
+
+int kmpm_fix_velocity_nodes_expr(kmpm_ctx *c, uint32_t bit, const kmpm_expr value[3],
+                                 const kmpm_expr previous[3]) {
+  if (!c || !value || !previous || c->a.nfv >= MAX_FIX || c->a.ninst > 0) return KMPM_ERR_ARG;
+  kmpm_expr prog[6];
+  memset(prog, 0, sizeof(prog));
+  int set[3] = {0, 0, 0};
+  for (int d = 0; d < 3; d++) {
+    const kmpm_expr &v = value[d], &p = previous[d];
+    if (v.n < 0 || v.n > KMPM_EXPR_MAX || p.n < 0 || p.n > KMPM_EXPR_MAX) return KMPM_ERR_ARG;
+    if ((v.n == 0) != (p.n == 0)) return KMPM_ERR_ARG;
+    set[d] = v.n > 0 && d < c->a.dim;     // NULL, or a component the dimension lacks
+    if (set[d]) {
+      prog[d] = v;
+      prog[3 + d] = p;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int idx = c->a.nfv;
+  HIPCHK(hipMemcpy(c->d_fexpr + 6 * idx, prog, sizeof(prog), hipMemcpyHostToDevice));
+  DevFixVel &f = c->a.fv[c->a.nfv++];
+  memset(&f, 0, sizeof(f));
+  f.bit = bit;
+  f.expr = 1;
+  for (int d = 0; d < 3; d++) f.set[d] = set[d];
+  return KMPM_OK;
+}
+
+int kmpm_fix_velocity_nodes_force(kmpm_ctx *c, int fix, double force[3]) {
+  if (!c || !force || fix < 0 || fix >= c->a.nfv) return KMPM_ERR_ARG;
+  DevScalars sc;
+  HIPCHK(hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int d = 0; d < 3; d++) force[d] = sc.step > 0 ? sc.fv_force[(sc.step - 1) & 1][fix][d] : 0.0;
   return KMPM_OK;
 }
 
@@ -863,6 +904,8 @@ int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
 int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
   if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
   if (ninst > 0 && (c->a.tl || c->a.ref_order)) return KMPM_ERR_ARG;
+  for (int q = 0; q < c->a.nfv; q++)
+    if (ninst > 0 && c->a.fv[q].expr) return KMPM_ERR_ARG;   // values of the context's time
   const TileGeom &g = c->a.tg;
   std::vector<int> map(g.ntiles, -1);
   std::vector<InstScalars> is(ninst);

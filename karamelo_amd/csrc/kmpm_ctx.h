@@ -70,6 +70,7 @@ struct KArgs {
   int nfv, nbf;
   DevFixVel fv[MAX_FIX];
   DevFixBF bf[MAX_FIX];
+  const kmpm_expr *fexpr;   // [MAX_FIX][6] programs of expression-valued velocity fixes
   // batched instances (0: one instance, whose dt/time are in sc)
   int ninst;
   InstScalars *inst;     // [ninst]
@@ -117,6 +118,7 @@ struct kmpm_ctx {
   double *d_gmass, *d_gv, *d_gvu;
   uint32_t *d_gmask;
   uint32_t *d_pmask;     // particle group masks (by pid)
+  kmpm_expr *d_fexpr;    // [MAX_FIX][6]
   int *d_solid;          // solid index (by pid)
   kmpm::DevScalars *d_sc;
   kmpm::SurrogateDev *d_surr;

@@ -93,7 +93,9 @@ struct DevMat {
 // into the kernel, which keeps its register footprint small.
 enum MatClass { MC_GENERIC = 0, MC_LINEAR = 1, MC_JC_SHOCK = 2, MC_FLUID = 3 };
 
-struct DevFixVel { uint32_t bit; int set[3]; double val[3], prev[3]; };
+// A velocity fix: constant values (val, prev) or, with expr != 0, the
+// programs KArgs::fexpr[6 * index + 0..2] (value) and [+ 3..5] (previous).
+struct DevFixVel { uint32_t bit; int set[3]; int expr; double val[3], prev[3]; };
 struct DevFixBF { uint32_t bit; int set[3]; double val[3]; };
 constexpr int MAX_FIX = 8;
 // Blocks of the particle energy reduction (k_energy), summed in a fixed order.
@@ -121,6 +123,11 @@ struct DevScalars {
                                       // (bits of a non-negative double)
   unsigned long long surr_evals;      // particle updates that used a surrogate network
   unsigned long long surr_fallbacks;  // particle updates outside the surrogate's envelope
+  // reaction force of each velocity fix, double-buffered by step parity: the
+  // step that starts with `step` completed steps sums into [step & 1], and
+  // k_adjust_dt clears the other buffer for the next step
+  double fv_force[2][MAX_FIX][3];
+  double fv_last[MAX_FIX][6];            // the last grid update's fix values (val, prev)
 };
 // k_tiles appends to both tile lists with one 64-bit atomic on the pair
 static_assert(offsetof(DevScalars, tile_count) % 8 == 0 &&

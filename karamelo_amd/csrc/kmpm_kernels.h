@@ -65,6 +65,108 @@ __device__ inline void set_error(DevScalars *sc, int code, int64_t pid) {
 // host's buffer swap keeps the state of the failing step.
 __device__ inline bool frozen(const KArgs &a) { return a.sc->frozen != 0; }
 
+// ---- velocity fixes: values, reaction force ---------------------------------
+// A fix value program (kmpm_expr_compile) at time t with the input script's
+// dt variable.
+__device__ inline double expr_eval(const kmpm_expr &e, double t, double dt) {
+  double st[KMPM_EXPR_STACK];
+  int sp = 0;
+  for (int q = 0; q < e.n && q < KMPM_EXPR_MAX; q++) {
+    const int o = e.op[q];
+    if (o == KMPM_OP_CONST || o == KMPM_OP_TIME || o == KMPM_OP_DT) {
+      if (sp >= KMPM_EXPR_STACK) return NAN;
+      st[sp++] = o == KMPM_OP_CONST ? e.c[q] : o == KMPM_OP_TIME ? t : dt;
+    } else if (o <= KMPM_OP_ATAN2) {
+      if (sp < 2) return NAN;
+      const double y = st[--sp], x = st[sp - 1];
+      double r;
+      switch (o) {
+        case KMPM_OP_ADD: r = x + y; break;
+        case KMPM_OP_SUB: r = x - y; break;
+        case KMPM_OP_MUL: r = x * y; break;
+        case KMPM_OP_DIV: r = x / y; break;
+        case KMPM_OP_POW: r = pow(x, y); break;
+        case KMPM_OP_GT: r = x > y; break;
+        case KMPM_OP_GE: r = x >= y; break;
+        case KMPM_OP_LT: r = x < y; break;
+        case KMPM_OP_LE: r = x <= y; break;
+        case KMPM_OP_EQ: r = x == y; break;
+        case KMPM_OP_NE: r = x != y; break;
+        default: r = atan2(x, y); break;
+      }
+      st[sp - 1] = r;
+    } else {
+      if (sp < 1) return NAN;
+      const double x = st[sp - 1];
+      double r;
+      switch (o) {
+        case KMPM_OP_NEG: r = -x; break;
+        case KMPM_OP_NOT: r = !x; break;
+        case KMPM_OP_EXP: r = exp(x); break;
+        case KMPM_OP_SQRT: r = sqrt(x); break;
+        case KMPM_OP_COS: r = cos(x); break;
+        case KMPM_OP_SIN: r = sin(x); break;
+        case KMPM_OP_TAN: r = tan(x); break;
+        default: r = log(x); break;
+      }
+      st[sp - 1] = r;
+    }
+  }
+  return sp == 1 ? st[0] : NAN;
+}
+
+// The values every velocity fix imposes in this step, into LDS (val[3],
+// prev[3] per fix), and the fix force accumulators cleared; every thread of
+// the block calls it (it synchronises).  Expressions see the time at the
+// start of the step and the script's dt: 0 in the first step unless set_dt
+// (input.cpp:55, update.cpp:69-76), the step's dt after that (ulmpm.cpp:550).
+__device__ inline void fix_values_to_lds(const KArgs &a, double (*fxv)[6], double (*fsum)[3]) {
+  const int tid = threadIdx.x;
+  if (tid < a.nfv * 6) {
+    const int fi = tid / 6, w = tid % 6;
+    const DevFixVel &f = a.fv[fi];
+    double v;
+    if (f.expr) {
+      const double dtv = (a.sc->step == 0 && !a.dt_constant) ? 0.0 : a.sc->dt;
+      v = expr_eval(a.fexpr[6 * fi + w], a.sc->time, dtv);
+    } else {
+      v = w < 3 ? f.val[w] : f.prev[w - 3];
+    }
+    fxv[fi][w] = v;
+    if (blockIdx.x == 0) a.sc->fv_last[fi][w] = v;
+  }
+  if (tid < a.nfv * 3) fsum[tid / 3][tid % 3] = 0.0;
+  __syncthreads();
+}
+
+// post_update_grid_state of the velocity fixes on one node, in fix order
+// (fix_velocity_nodes.cpp:130-225): v_update <- value, v <- previous value,
+// and the reaction sum (1/dt) m (value - v_update) into the block's LDS sums.
+__device__ inline void apply_velocity_fixes(const KArgs &a, uint32_t mask, double m, double inv_dt,
+                                            const double (*fxv)[6], double (*fsum)[3],
+                                            double *v, double *vu) {
+  for (int fi = 0; fi < a.nfv; fi++) {
+    const DevFixVel &f = a.fv[fi];
+    if (!(mask & f.bit)) continue;
+    for (int d = 0; d < 3; d++) {
+      if (!f.set[d]) continue;
+      const double dv = fxv[fi][d] - vu[d];
+      vu[d] = fxv[fi][d];
+      v[d] = fxv[fi][3 + d];
+      const double r = (inv_dt * m) * dv;
+      if (r != 0.0) atomicAdd(&fsum[fi][d], r);
+    }
+  }
+}
+
+// End of a grid update: the block's reaction sums into the step's buffer.
+__device__ inline void flush_fix_forces(const KArgs &a, double (*fsum)[3]) {
+  __syncthreads();
+  const int tid = threadIdx.x;
+  if (tid < a.nfv * 3 && fsum[tid / 3][tid % 3] != 0.0)
+    atomicAdd(&a.sc->fv_force[a.sc->step & 1][tid / 3][tid % 3], fsum[tid / 3][tid % 3]);
+}
+
 __device__ inline void frozen_copy(const KArgs &a) {
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
   const int64_t nd = (int64_t)state_nd(mixed, a.tl) * a.cap;
@@ -561,6 +663,9 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
   constexpr int BN = EX * EY * EZ;
   constexpr int TZZ = DIM == 3 ? TZ : 1;
   constexpr int NO = TX * TY * TZZ;
+  __shared__ double fxv[MAX_FIX][6];
+  __shared__ double fsum[MAX_FIX][3];
+  fix_values_to_lds(a, fxv, fsum);
   if constexpr (FX) {
     // Mixed: one group of 256 / TPB threads per dilated tile.  The tile's
     // nodes lie in the partial blocks of the tiles around it, one contiguous
@@ -642,7 +747,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
           for (int fi = 0; fi < a.nfv; fi++)
             if (mask & a.fv[fi].bit)
               for (int d = 0; d < 3; d++)
-                if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
+                if (a.fv[fi].set[d]) v[d] = fxv[fi][d];
           for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
           continue;
         }
@@ -654,13 +759,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
             for (int d = 0; d < 3; d++)
               if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
         for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
-        for (int fi = 0; fi < a.nfv; fi++)
-          if (mask & a.fv[fi].bit)
-            for (int d = 0; d < 3; d++)
-              if (a.fv[fi].set[d]) {
-                vu[d] = a.fv[fi].val[d];
-                v[d] = a.fv[fi].prev[d];
-              }
+        apply_velocity_fixes(a, mask, m, 1.0 / dt, fxv, fsum, v, vu);
         gmass[sn] = (R)m;
         for (int d = 0; d < 3; d++) {
           gv[d * a.ns + sn] = (R)v[d];
@@ -723,7 +822,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
         for (int fi = 0; fi < a.nfv; fi++)
           if (mask & a.fv[fi].bit)
             for (int d = 0; d < 3; d++)
-              if (a.fv[fi].set[d]) v[d] = a.fv[fi].val[d];
+              if (a.fv[fi].set[d]) v[d] = fxv[fi][d];
         for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
         continue;
       }
@@ -735,13 +834,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
           for (int d = 0; d < 3; d++)
             if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
       for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
-      for (int fi = 0; fi < a.nfv; fi++)
-        if (mask & a.fv[fi].bit)
-          for (int d = 0; d < 3; d++)
-            if (a.fv[fi].set[d]) {
-              vu[d] = a.fv[fi].val[d];
-              v[d] = a.fv[fi].prev[d];
-            }
+      apply_velocity_fixes(a, mask, m, 1.0 / dt, fxv, fsum, v, vu);
       gmass[sn] = (R)m;
       for (int d = 0; d < 3; d++) {
         gv[d * a.ns + sn] = (R)v[d];
@@ -749,6 +842,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
       }
     }
   }
+  flush_fix_forces(a, fsum);
 }
 
 // Grid state for download: nodes outside every touched tile hold zeros
@@ -784,8 +878,8 @@ __global__ void k_grid_clean(KArgs a) {
       if (mask & a.fv[fi].bit)
         for (int d = 0; d < 3; d++)
           if (a.fv[fi].set[d]) {
-            vu[d] = a.fv[fi].val[d];
-            v[d] = a.fv[fi].prev[d];
+            vu[d] = a.sc->fv_last[fi][d];      // the values of the last grid update
+            v[d] = a.sc->fv_last[fi][3 + d];
           }
     gmass[sn] = 0;
     for (int d = 0; d < 3; d++) {

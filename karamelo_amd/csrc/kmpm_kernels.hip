@@ -347,7 +347,10 @@ __device__ inline void adjust_dt_instances(const KArgs &a) {
     s.max_c_bits = 0;
     s.min_ratio_bits = (unsigned long long)__double_as_longlong(1.0);
   }
-  if (threadIdx.x == 0) a.sc->step += 1;
+  if (threadIdx.x == 0) {
+    a.sc->step += 1;
+    for (int q = 0; q < MAX_FIX * 3; q++) a.sc->fv_force[a.sc->step & 1][q / 3][q % 3] = 0.0;
+  }
 }
 
 // The end of a step: time and dt (update.cpp:201-206, ulmpm.cpp:525-551).  A
@@ -379,6 +382,7 @@ __global__ void k_adjust_dt(KArgs a) {
   sc->max_c_bits = 0;
   sc->min_ratio_bits = (unsigned long long)__double_as_longlong(1.0);
   sc->step += 1;
+  for (int q = 0; q < MAX_FIX * 3; q++) sc->fv_force[sc->step & 1][q / 3][q % 3] = 0.0;
   if (sc->error != 0) sc->frozen = 1;
 }
 

@@ -177,9 +177,9 @@ __global__ __launch_bounds__(256) void k_ref_p2g(KArgs a) {
 }
 
 // Grid update (grid.cpp:448-466) with the body force mb = sum_fixes g m on
-// nodes with mass (fix_body_force.cpp:117-146), then the velocity fixes
-// (fix_velocity_nodes.cpp:130-271); the re-map applies only the fixes.
-__global__ __launch_bounds__(256) void k_ref_grid(KArgs a, int remap) {
+// nodes with mass (fix_body_force.cpp:117-146); the velocity fixes follow in
+// k_ref_fixes.
+__global__ __launch_bounds__(256) void k_ref_grid(KArgs a) {
   if (frozen(a)) return;
   const int64_t tag = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tag >= a.nn) return;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_ref_grid(KArgs a, int remap) {
   const int64_t ns = a.ns;
   const uint32_t mask = a.gmask[s];
   const double dt = a.sc->dt;
-  if (!remap) {
+  {
     const double m = a.gmass[s];
     double mb[3] = {0, 0, 0};
     for (int q = 0; q < a.nbf; q++) {
@@ -207,18 +207,47 @@ __global__ __launch_bounds__(256) void k_ref_grid(KArgs a, int remap) {
       a.gvu[d * ns + s] = m != 0 ? v + dt * (f + mb[d]) / m : v;
     }
   }
+}
+
+// The velocity fixes in the reference's order: fix after fix, each over the
+// nodes in ascending tag, with the reaction force summed in that order
+// (FixVelocityNodes::post_update_grid_state, fix_velocity_nodes.cpp:130-225;
+// after the re-map, post_velocities_to_grid, :227-271).  One thread does the
+// sums; the mode is for small problems.
+__global__ __launch_bounds__(256) void k_ref_fixes(KArgs a, int remap) {
+  if (frozen(a)) return;
+  __shared__ double fxv[MAX_FIX][6];
+  __shared__ double fsum[MAX_FIX][3];
+  fix_values_to_lds(a, fxv, fsum);
+  if (threadIdx.x != 0 || a.nfv == 0) return;
+  const int64_t ns = a.ns;
+  const int n1 = a.n[1], n2 = a.n[2];
+  const double inv_dt = 1.0 / a.sc->dt;
+  double *ftot = a.sc->fv_force[a.sc->step & 1][0];
   for (int q = 0; q < a.nfv; q++) {
     const DevFixVel &f = a.fv[q];
-    if (!(mask & f.bit)) continue;
-    for (int d = 0; d < 3; d++) {
-      if (!f.set[d]) continue;
-      if (remap) {
-        a.gv[d * ns + s] = f.val[d];
-      } else {
-        a.gvu[d * ns + s] = f.val[d];
-        a.gv[d * ns + s] = f.prev[d];
+    double fx[3] = {0, 0, 0};
+    for (int64_t tag = 0; tag < a.nn; tag++) {
+      const int k = (int)(tag % n2), j = (int)((tag / n2) % n1),
+                i = (int)(tag / ((int64_t)n2 * n1));
+      const int64_t s = ref_sidx(a, i, j, k);
+      if (!(a.gmask[s] & f.bit)) continue;
+      double dv[3] = {0, 0, 0};
+      for (int d = 0; d < 3; d++) {
+        if (!f.set[d]) continue;
+        if (remap) {
+          a.gv[d * ns + s] = fxv[q][d];
+          continue;
+        }
+        dv[d] = fxv[q][d] - a.gvu[d * ns + s];
+        a.gvu[d * ns + s] = fxv[q][d];
+        a.gv[d * ns + s] = fxv[q][3 + d];
       }
+      const double c = inv_dt * a.gmass[s];
+      for (int d = 0; d < 3; d++) fx[d] += c * dv[d];
     }
+    if (!remap)
+      for (int d = 0; d < 3; d++) ftot[3 * q + d] = fx[d];
   }
 }
 
@@ -427,7 +456,9 @@ hipError_t launch_ref_p2g(kmpm_ctx *c, int remap) {
 }
 
 hipError_t launch_ref_grid(kmpm_ctx *c, int remap) {
-  hipLaunchKernelGGL(k_ref_grid, dim3(ref_blocks(c->a.nn)), dim3(256), 0, c->stream, c->a, remap);
+  if (!remap)
+    hipLaunchKernelGGL(k_ref_grid, dim3(ref_blocks(c->a.nn)), dim3(256), 0, c->stream, c->a);
+  hipLaunchKernelGGL(k_ref_fixes, dim3(1), dim3(256), 0, c->stream, c->a, remap);
   return hipGetLastError();
 }
 

@@ -128,6 +128,17 @@ def concat_particles(parts):
     return {k: np.concatenate([p[k] for p in parts]) for k in PARTICLE_FIELDS}
 
 
+EXPR_MAX = 32
+
+
+class Expr(C.Structure):
+    """kmpm_expr: a fix value program of time and dt."""
+    _fields_ = [("n", C.c_int32), ("op", C.c_int32 * EXPR_MAX), ("c", C.c_double * EXPR_MAX)]
+
+
+Expr3 = Expr * 3
+
+
 def _declare(lib):
     vp = C.c_void_p
     P = C.POINTER
@@ -146,6 +157,10 @@ def _declare(lib):
         "kmpm_set_node_mask": (C.c_int, [vp, P(C.c_uint32), C.c_int64]),
         "kmpm_fix_velocity_nodes": (C.c_int, [vp, C.c_uint32, I3, D3, D3]),
         "kmpm_fix_body_force": (C.c_int, [vp, C.c_uint32, I3, D3]),
+        "kmpm_expr_compile": (C.c_int, [C.c_char_p, C.c_int, C.c_int, P(C.c_char_p), _DP,
+                                        P(Expr)]),
+        "kmpm_fix_velocity_nodes_expr": (C.c_int, [vp, C.c_uint32, Expr3, Expr3]),
+        "kmpm_fix_velocity_nodes_force": (C.c_int, [vp, C.c_int, D3]),
         "kmpm_compute_grid_weight_functions_and_gradients": (C.c_int, [vp]),
         "kmpm_particles_to_grid": (C.c_int, [vp]),
         "kmpm_update_grid_state": (C.c_int, [vp]),
@@ -305,6 +320,20 @@ class Context:
     def fix_body_force(self, bit, setv, val):
         check(self.L.kmpm_fix_body_force(self.h, bit, I3(*setv), D3(*val)), "fix_body_force")
 
+    def fix_velocity_nodes_expr(self, bit, texts, variables=None):
+        """fix(id, velocity_nodes, group, vx, vy, vz) with the script's
+        expression texts (None or "NULL": free component) and its constant
+        user variables {name: value}."""
+        val, prev = compile_fix_values(texts, variables)
+        check(self.L.kmpm_fix_velocity_nodes_expr(self.h, bit, val, prev),
+              "fix_velocity_nodes_expr")
+
+    def fix_force(self, fix):
+        """Reaction force of velocity fix number `fix` in the last step."""
+        f = D3()
+        check(self.L.kmpm_fix_velocity_nodes_force(self.h, int(fix), f), "fix_velocity_nodes_force")
+        return np.array(f[:])
+
     def phase(self, name):
         check(getattr(self.L, "kmpm_" + name)(self.h), name)
 
@@ -417,6 +446,29 @@ class Context:
                                         w.ctypes.data_as(_DP), wd.ctypes.data_as(_DP), self.np),
               "debug_weights")
         return tag, w, wd
+
+
+def expr_compile(text, previous=False, variables=None):
+    """kmpm_expr_compile: the program of an input-script expression."""
+    variables = variables or {}
+    names = (C.c_char_p * max(1, len(variables)))(*[k.encode() for k in variables])
+    vals = (C.c_double * max(1, len(variables)))(*[float(v) for v in variables.values()])
+    e = Expr()
+    check(lib().kmpm_expr_compile(text.encode(), 1 if previous else 0, len(variables), names,
+                                  vals, C.byref(e)), "expr_compile(%r)" % text)
+    return e
+
+
+def compile_fix_values(texts, variables=None):
+    """Value and previous-value programs of the three components of a
+    velocity_nodes fix (None or "NULL": the component is free)."""
+    val, prev = Expr3(), Expr3()
+    for d, t in enumerate(texts):
+        if t is None or t == "NULL":
+            continue
+        val[d] = expr_compile(t, False, variables)
+        prev[d] = expr_compile(t, True, variables)
+    return val, prev
 
 
 def copy_probe(nbytes, reps=20):

@@ -26,11 +26,12 @@ def lit_e(mant, exp):
 
 @dataclasses.dataclass
 class Fix:
-    kind: str                 # "velocity_nodes" | "body_force"
+    kind: str                 # "velocity_nodes" | "velocity_nodes_expr" | "body_force"
     bit: int
     set: tuple
-    value: tuple
+    value: tuple              # velocity_nodes_expr: the script's texts (None: free)
     prev: Optional[tuple] = None
+    variables: Optional[dict] = None    # velocity_nodes_expr: the script's constants
 
 
 @dataclasses.dataclass
@@ -194,16 +195,19 @@ def steel_jc(damage=True, d_params=(0.05, 3.44, -2.12, 0.002, 0.61)):
 
 def tensile_bar(h=6e-5, ppc1d=2, v_pull=10.0, shape="cubic-spline", scheme="usl",
                 damage=True, d_params=(0.05, 3.44, -2.12, 0.002, 0.61), steps=0,
-                populate_fn=None, length=0.02, width=0.0024):
+                populate_fn=None, length=0.02, width=0.0024, pull=None, material=None):
     """3D ULMPM tensile bar with JC strength, shock EOS and JC damage.
     Bar length x width x width centred at the origin in a box twice its
     size; nodes within 0.5 mm of each end are driven at -+v_pull along x
     (fix velocity_nodes).  h = 0.06 mm, 2x2x2 particles per cell gives
-    4.27M particles."""
+    4.27M particles.  pull: the right end's x velocity as an input-script
+    expression of time with the variable v = v_pull (the left end takes
+    "-(" pull ")"), instead of the constant v_pull.  material: another
+    material than the steel."""
     L2, W2 = length / 2, width / 2
     cfg = make_config(3, (-2 * L2, -2 * W2, -2 * W2), (2 * L2, 2 * W2, 2 * W2), h,
                       scheme=scheme, shape=shape, flip=0.99)
-    mat = steel_jc(damage, d_params)
+    mat = material if material is not None else steel_jc(damage, d_params)
     p = populate_solids(cfg, [(block(-L2, L2, -W2, W2, -W2, W2), ppc1d, mat, 0)],
                         fn=populate_fn)
     X, Y, Z = node_coords(cfg)
@@ -213,8 +217,14 @@ def tensile_bar(h=6e-5, ppc1d=2, v_pull=10.0, shape="cubic-spline", scheme="usl"
     mask = np.ones(len(X), dtype=np.uint32)
     mask[left] |= 2
     mask[right] |= 4
-    fixes = [Fix("velocity_nodes", 2, (1, 0, 0), (-v_pull, 0, 0), (-v_pull, 0, 0)),
-             Fix("velocity_nodes", 4, (1, 0, 0), (v_pull, 0, 0), (v_pull, 0, 0))]
+    if pull is None:
+        fixes = [Fix("velocity_nodes", 2, (1, 0, 0), (-v_pull, 0, 0), (-v_pull, 0, 0)),
+                 Fix("velocity_nodes", 4, (1, 0, 0), (v_pull, 0, 0), (v_pull, 0, 0))]
+    else:
+        var = {"v": lit(v_pull)}
+        fixes = [Fix("velocity_nodes_expr", 2, (1, 0, 0), ("-(%s)" % pull, None, None),
+                     variables=var),
+                 Fix("velocity_nodes_expr", 4, (1, 0, 0), (pull, None, None), variables=var)]
     return Problem("tensile_bar", cfg, [mat], p, node_mask=mask, fixes=fixes, steps=steps)
 
 
@@ -310,6 +320,8 @@ def context_for(problem):
     for f in problem.fixes:
         if f.kind == "velocity_nodes":
             ctx.fix_velocity_nodes(f.bit, f.set, f.value, f.prev)
+        elif f.kind == "velocity_nodes_expr":
+            ctx.fix_velocity_nodes_expr(f.bit, f.value, f.variables)
         else:
             ctx.fix_body_force(f.bit, f.set, f.value)
     return ctx

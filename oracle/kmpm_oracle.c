@@ -18,6 +18,7 @@
  * body_force (constant values); computes kinetic/strain energy.  One MPI rank.
  */
 #include "kmpm_oracle.h"
+#include "oracle_var.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -459,7 +460,13 @@ int orc_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
 /* ------------------------------------------------------------------------ */
 /* simulation state                                                         */
 /* ------------------------------------------------------------------------ */
-typedef struct { uint32_t bit; int set[3]; double val[3], prev[3]; } fixvel_t;
+/* velocity_nodes: constant values, or (expr) the Vars of the script's text
+ * and of its "time - dt" form; cur/force: this step's values and reaction */
+typedef struct {
+  uint32_t bit; int set[3]; double val[3], prev[3];
+  int expr; ovar val_v[3], prev_v[3];
+  double cur[3], cur_prev[3], force[3];
+} fixvel_t;
 typedef struct { uint32_t bit; int set[3]; double val[3]; } fixbf_t;
 
 struct orc_sim {
@@ -684,12 +691,65 @@ int orc_fix_velocity_nodes(orc_sim *s, uint32_t bit, const int set[3], const dou
                            const double prev[3]) {
   if (s->nfv >= MAXFIX) return KMPM_ERR_ARG;
   fixvel_t *f = &s->fv[s->nfv++];
+  memset(f, 0, sizeof(*f));
   f->bit = bit;
   for (int d = 0; d < 3; d++) {
     f->set[d] = set[d] && d < s->dim;   /* fix_velocity_nodes.cpp:81,98 */
     f->val[d] = val[d];
     f->prev[d] = prev[d];
   }
+  return KMPM_OK;
+}
+
+/* The script's dt variable in the current step: 0 in the first step unless
+ * set_dt (input.cpp:55, update.cpp:69-76), the step's dt after that
+ * (ulmpm.cpp:550). */
+static double dt_variable(const orc_sim *s) {
+  return (s->step <= 1 && !s->c.dt_constant) ? 0.0 : s->dt;
+}
+
+/* fix(id, velocity_nodes, group, vx, vy, vz) with expression texts (NULL or
+ * "NULL": free component), parsed at creation as the constructor does
+ * (fix_velocity_nodes.cpp:64-116) */
+int orc_fix_velocity_nodes_expr(orc_sim *s, uint32_t bit, const char *const texts[3], int nvar,
+                                const char *const *names, const double *values) {
+  if (s->nfv >= MAXFIX) return KMPM_ERR_ARG;
+  fixvel_t f;
+  memset(&f, 0, sizeof(f));
+  f.bit = bit;
+  f.expr = 1;
+  for (int d = 0; d < 3; d++) {
+    if (!texts[d] || !strcmp(texts[d], "NULL") || d >= s->dim) continue;
+    char prev[OV_EQ];
+    ovenv env = {s->time, dt_variable(s), nvar, names, values, 0};
+    f.val_v[d] = ov_parse(texts[d], &env);
+    ov_previous_text(texts[d], prev, sizeof(prev));
+    f.prev_v[d] = ov_parse(prev, &env);
+    if (env.error) return KMPM_ERR_ARG;
+    f.set[d] = 1;
+  }
+  s->fv[s->nfv++] = f;
+  return KMPM_OK;
+}
+
+/* One Var of the script at (time, dt): the text (or its "time - dt" form)
+ * parsed once, then its result (var.cpp:52-59). */
+int orc_var_eval(const char *text, int previous, double time, double dt, int nvar,
+                 const char *const *names, const double *values, double *out) {
+  char prev[OV_EQ];
+  ovenv env = {0.0, 0.0, nvar, names, values, 0};
+  if (previous) ov_previous_text(text, prev, sizeof(prev));
+  ovar v = ov_parse(previous ? prev : text, &env);
+  if (env.error) return KMPM_ERR_ARG;
+  int err = 0;
+  *out = ov_result(&v, time, dt, &err);
+  return err ? KMPM_ERR_ARG : KMPM_OK;
+}
+
+/* the Vars <id>_x, _y, _z of the last step (fix_velocity_nodes.cpp:218-224) */
+int orc_fix_velocity_force(orc_sim *s, int fix, double out[3]) {
+  if (fix < 0 || fix >= s->nfv) return KMPM_ERR_ARG;
+  for (int d = 0; d < 3; d++) out[d] = s->fv[fix].force[d];
   return KMPM_OK;
 }
 
@@ -909,15 +969,34 @@ int orc_phase_grid_update(orc_sim *s) {
         s->gvu[3 * in + d] = s->gv[3 * in + d];
     }
   }
+  /* FixVelocityNodes::post_update_grid_state, fix after fix, nodes in order,
+   * with the reaction force sum (1/dt) m (value - v_update) */
+  double inv_dt = 1.0 / s->dt;
   for (int k = 0; k < s->nfv; k++) {
     fixvel_t *f = &s->fv[k];
-    OMP_FOR
+    int err = 0;
+    for (int d = 0; d < 3; d++) {
+      if (!f->set[d]) continue;
+      f->cur[d] = f->expr ? ov_result(&f->val_v[d], s->time, dt_variable(s), &err) : f->val[d];
+      f->cur_prev[d] = f->expr ? ov_result(&f->prev_v[d], s->time, dt_variable(s), &err)
+                               : f->prev[d];
+    }
+    if (err) set_status(s, KMPM_ERR_ARG, -1);
+    double ftot[3] = {0, 0, 0};
     for (int64_t in = 0; in < s->nn; in++) {
       if (s->gmask[in] & f->bit) {
+        double Dv[3] = {0, 0, 0};
         for (int d = 0; d < 3; d++)
-          if (f->set[d]) { s->gvu[3 * in + d] = f->val[d]; s->gv[3 * in + d] = f->prev[d]; }
+          if (f->set[d]) {
+            Dv[d] = f->cur[d] - s->gvu[3 * in + d];
+            s->gvu[3 * in + d] = f->cur[d];
+            s->gv[3 * in + d] = f->cur_prev[d];
+          }
+        double c = inv_dt * s->gmass[in];
+        for (int d = 0; d < 3; d++) ftot[d] += c * Dv[d];
       }
     }
+    for (int d = 0; d < 3; d++) f->force[d] = ftot[d];
   }
   return KMPM_OK;
 }
@@ -953,7 +1032,7 @@ static void phase_velocities_to_grid(orc_sim *s) {
     OMP_FOR
     for (int64_t in = 0; in < s->nn; in++)
       if (s->gmask[in] & f->bit)
-        for (int d = 0; d < 3; d++) if (f->set[d]) s->gv[3 * in + d] = f->val[d];
+        for (int d = 0; d < 3; d++) if (f->set[d]) s->gv[3 * in + d] = f->cur[d];
   }
 }
 

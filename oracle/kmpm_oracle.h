@@ -43,6 +43,17 @@ int orc_fix_velocity_nodes(orc_sim *s, uint32_t groupbit, const int set[3],
                            const double value[3], const double value_prev[3]);
 int orc_fix_body_force(orc_sim *s, uint32_t groupbit, const int set[3],
                        const double value[3]);
+/* velocity_nodes with the script's expression texts (NULL / "NULL": free),
+ * evaluated every step as the reference's Vars are; user variables are
+ * constants names[]/values[] */
+int orc_fix_velocity_nodes_expr(orc_sim *s, uint32_t groupbit, const char *const texts[3],
+                                int nvar, const char *const *names, const double *values);
+/* a script expression (or its "time - dt" form) evaluated as a Var at
+ * (time, dt) */
+int orc_var_eval(const char *text, int previous, double time, double dt, int nvar,
+                 const char *const *names, const double *values, double *out);
+/* reaction force of velocity fix `fix` in the last step */
+int orc_fix_velocity_force(orc_sim *s, int fix, double out[3]);
 int orc_step(orc_sim *s, int64_t nsteps);
 int orc_get_time(orc_sim *s, double *dt, double *time, int64_t *step);
 int orc_energy(orc_sim *s, uint32_t groupbit, double *ek, double *es);

@@ -64,6 +64,11 @@ def load(variant="liboracle.so"):
                                                 D, C.c_double]),
             "orc_jc_damage": (None, [P(N.Material), D, D, C.c_double, D, C.c_double, C.c_double]),
             "orc_basis": (C.c_double, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double]),
+            "orc_fix_velocity_nodes_expr": (C.c_int, [vp, C.c_uint32, P(C.c_char_p) , C.c_int,
+                                                      P(C.c_char_p), D]),
+            "orc_fix_velocity_force": (C.c_int, [vp, C.c_int, D]),
+            "orc_var_eval": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int,
+                                       P(C.c_char_p), D, D]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -140,6 +145,17 @@ class Oracle:
     def fix_body_force(self, bit, setv, val):
         assert self.L.orc_fix_body_force(self.h, bit, N.I3(*setv), N.D3(*val)) == 0
 
+    def fix_velocity_nodes_expr(self, bit, texts, variables=None):
+        t = (C.c_char_p * 3)(*[None if x is None else x.encode() for x in texts])
+        names, vals = _vars(variables)
+        assert self.L.orc_fix_velocity_nodes_expr(self.h, bit, t, len(variables or {}), names,
+                                                  _dp(vals)) == 0
+
+    def fix_force(self, fix):
+        f = np.zeros(3)
+        assert self.L.orc_fix_velocity_force(self.h, fix, _dp(f)) == 0
+        return f
+
     def step(self, n):
         return self.L.orc_step(self.h, int(n))
 
@@ -188,6 +204,22 @@ class Oracle:
         return out
 
 
+def _vars(variables):
+    variables = variables or {}
+    names = (C.c_char_p * max(1, len(variables)))(*[k.encode() for k in variables])
+    vals = np.array(list(variables.values()) or [0.0], dtype=float)
+    return names, vals
+
+
+def var_eval(text, previous, time, dt, variables=None):
+    """The oracle's Var of a script expression at (time, dt)."""
+    names, vals = _vars(variables)
+    out = C.c_double()
+    st = load().orc_var_eval(text.encode(), 1 if previous else 0, time, dt,
+                             len(variables or {}), names, _dp(vals), C.byref(out))
+    return out.value if st == 0 else None
+
+
 def oracle_for(problem, variant="liboracle.so"):
     o = Oracle(problem.cfg, problem.mats, variant)
     o.upload(problem.particles)
@@ -196,6 +228,8 @@ def oracle_for(problem, variant="liboracle.so"):
     for f in problem.fixes:
         if f.kind == "velocity_nodes":
             o.fix_velocity_nodes(f.bit, f.set, f.value, f.prev)
+        elif f.kind == "velocity_nodes_expr":
+            o.fix_velocity_nodes_expr(f.bit, f.value, f.variables)
         else:
             o.fix_body_force(f.bit, f.set, f.value)
     return o
