@@ -268,42 +268,49 @@ def dam_break(n_cells=32, ppc1d=2, steps=0, scheme="musl", shape="cubic-spline",
     return Problem("dam_break", cfg, [mat], p, node_mask=mask, fixes=fixes, steps=steps)
 
 
-def copper_jc():
+def copper_jc(q=(0.0, 0.0)):
     """Copper with Johnson-Cook strength and shock EOS, SI units (the Taylor
     bar of the reference's examples with the published JC constants):
     E = 115 GPa, nu = 0.31 -> G; JC A = 90 MPa, B = 292 MPa, n = 0.31,
     C = 0.025, epsdot0 = 1/s, m = 1.09, Tr = 0, Tm = 1356; shock rho0 = 8960,
-    K = rho0 c0^2, c0 = 3940 m/s, S = 1.49, Gamma = 2.0, cv = 0, Q1 = Q2 = 0."""
+    K = rho0 c0^2, c0 = 3940 m/s, S = 1.49, Gamma = 2.0, cv = 0, (Q1, Q2) = q
+    (the artificial viscosity, eos_shock.cpp:53-61, off by default)."""
     E, nu, rho, c0 = 115e9, lit(0.31), 8960.0, 3940.0
     G = E / (2 * (1 + nu))
     K = rho * c0 * c0
-    eos_p = (rho, K, c0, lit(1.49), 2.0, 0.0, 0.0, 0.0, 0.0)
+    eos_p = (rho, K, c0, lit(1.49), 2.0, 0.0, 0.0, lit(q[0]), lit(q[1]))
     jc_p = (G, 90e6, 292e6, lit(0.31), 1.0, lit(0.025), lit(1.09), 0.0, 1356.0)
     return eos_strength_material("shock", eos_p, "johnson_cook", jc_p)
 
 
 def taylor_bar(h=1e-4, ppc1d=1, v0=190.0, shape="cubic-spline", scheme="usl", radius=3.2e-3,
-               length=32.4e-3, steps=0, populate_fn=None, dt_factor=0.5):
+               length=32.4e-3, steps=0, populate_fn=None, dt_factor=0.5, material=None,
+               method="tlmpm"):
     """3D TLMPM Taylor bar: copper cylinder along x (radius 3.2 mm, length
     32.4 mm, centred at the origin) moving at +v0 towards a rigid wall at
     its +x end, which is the velocity_nodes fix (zero x velocity) on the
     grid nodes of the end face, as in
     examples/Taylor-bar/cylindrical/TLMPM/cylindrical_taylor_bar.mpm:60-88.
-    h = 0.1 mm with 1 particle per cell gives ~1.04M particles."""
+    h = 0.1 mm with 1 particle per cell gives ~1.04M particles.  material:
+    another material than the copper; method "ulmpm": the same impact on
+    the box grid."""
     L2 = length / 2
     cyl = cylinder(0.0, 0.0, radius, -L2, L2, axis=0)
     margin = 2 * h
     cfg = make_config(3, (-L2 - margin, -radius - margin, -radius - margin),
                       (L2 + margin, radius + margin, radius + margin), h, scheme=scheme,
-                      shape=shape, flip=0.99, dt_factor=dt_factor, method="tlmpm",
+                      shape=shape, flip=0.99, dt_factor=dt_factor, method=method,
                       solid_region=cyl)
-    mat = copper_jc()
+    mat = material if material is not None else copper_jc()
     p = populate_solids(cfg, [(cyl, ppc1d, mat, 0)], fn=populate_fn)
     p["v"][:, 0] = v0
-    # TL grid nodes (reference positions of the solid's grid)
-    n = N.grid_size(cfg)
-    X = np.repeat(cfg.solidlo[0] + h * np.arange(n[0]), n[1] * n[2])
-    mask = np.ones(int(np.prod(n)), dtype=np.uint32)
+    if method == "tlmpm":
+        # TL grid nodes (reference positions of the solid's grid)
+        n = N.grid_size(cfg)
+        X = np.repeat(cfg.solidlo[0] + h * np.arange(n[0]), n[1] * n[2])
+    else:
+        X = node_coords(cfg)[0]
+    mask = np.ones(len(X), dtype=np.uint32)
     wall = X >= L2 - 0.5 * h          # region(x_wall - h/2, INF, ...)
     mask[wall] |= 2
     fixes = [Fix("velocity_nodes", 2, (1, 0, 0), (0, 0, 0), (0, 0, 0))]

@@ -491,8 +491,13 @@ def test_rejected_upload_keeps_the_previous_particles():
     b.step(5)
     assert a.synchronize()[0] == 0 and b.synchronize()[0] == 0
     pa, pb = a.download(), b.download()
+    # the FP64 P2G adds node sums through LDS atomics, whose order follows
+    # wave timing, so two identical runs agree to round-off, not bit for
+    # bit; a partly applied upload (F = diag(1, 1, -1) at one particle)
+    # would differ at O(1)
     for f in ("x", "v", "sigma", "F"):
-        assert np.array_equal(pa[f], pb[f]), f
+        assert rel_l2(pa[f], pb[f]) <= 1e-12, f
+    assert np.array_equal(pa["ptag"], pb["ptag"])
     a.close()
     b.close()
 
