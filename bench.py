@@ -16,6 +16,11 @@ Paths (--paths, the last one is the headline `value`):
   plain-fp64       FP64 throughout, parity with the reference to 1e-10;
   mixed            FP32 state and transfers, FP64 position and deformation
                    gradient, 64-bit fixed-point node sums (deterministic);
+  mixed+cutnet     mixed, with both constitutive calls (JC strength and JC
+                   damage) replaced by one 18 -> 64 x5 -> 9 GELU network on
+                   the BF16 matrix cores (trained on the device before the
+                   timed region on 50M tuples of the plain functions; the
+                   plain functions judge a sample of its updates);
   mixed+surrogate  mixed, with the JC flow stress and failure strain from
                    the learned BF16 networks (trained on the device before
                    the timed region, on tuples of the plain functions).
@@ -68,15 +73,31 @@ ALG_BYTES_PER_PARTICLE = {
     "mixed": {"g2p": 307, "p2g": 75, "grid": 10.5, "sort": 12},
 }
 ALG_BYTES_PER_PARTICLE["mixed+surrogate"] = ALG_BYTES_PER_PARTICLE["mixed"]
+# mixed+cutnet: the G2P kernel leaves the constitutive update to the cut
+# network's kernel and stages its inputs (D, pressure, wave speed: 8 floats
+# = 32 B per particle); the cut kernel reads sigma, strain, eps_p, epsdot_p,
+# damage_init, damage (64 B) and the staged 32 B, writes the same 64 B.
+ALG_BYTES_PER_PARTICLE["mixed+cutnet"] = {"g2p": 307 - 128 + 32, "p2g": 75, "grid": 10.5,
+                                          "sort": 12, "cut": 160}
+# Matrix-core work of one cut network evaluation, algorithmic
+# (2 (18 x 64 + 4 x 64 x 64 + 64 x 9) flops), and per training sample
+# (forward, the input-gradient products of layers 2..6, the weight
+# gradients of all six layers).
+CUTNET_FLOPS_PER_EVAL = 2 * (18 * 64 + 4 * 64 * 64 + 64 * 9)
+CUTNET_FLOPS_PER_TRAIN_SAMPLE = CUTNET_FLOPS_PER_EVAL + 2 * (4 * 64 * 64 + 64 * 9) + \
+    CUTNET_FLOPS_PER_EVAL
 # Matrix-core work of one network evaluation: the 32x32 hidden layer as two
 # 32x32x16 BF16 products per 32 samples = 2 * 32 * 32 flops per sample.
 MFMA_FLOPS_PER_EVAL = 2 * 32 * 32
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-KERNEL_NAMES = {"g2p": "k_g2p_usl", "p2g": "k_p2g", "grid": "k_grid_update", "sort": "sort"}
+KERNEL_NAMES = {"g2p": "k_g2p_usl", "p2g": "k_p2g", "grid": "k_grid_update", "sort": "sort",
+                "cut": "k_stress_cutnet"}
 # Whole step, same counting (state once in, once out, grid once).
-STEP_ALG_BYTES_PER_PARTICLE = {"plain-fp64": 442, "mixed": 324, "mixed+surrogate": 324}
+STEP_ALG_BYTES_PER_PARTICLE = {"plain-fp64": 442, "mixed": 324, "mixed+surrogate": 324,
+                               "mixed+cutnet": 324}
 DTYPE = {"plain-fp64": "f64", "mixed": "f32+f64(x,F)+i64(grid sums)",
-         "mixed+surrogate": "f32+f64(x,F)+i64(grid sums)+bf16(mfma, f32 acc)"}
+         "mixed+surrogate": "f32+f64(x,F)+i64(grid sums)+bf16(mfma, f32 acc)",
+         "mixed+cutnet": "f32+f64(x,F)+i64(grid sums)+bf16(mfma, f32 acc)"}
 
 
 def log(*a):
@@ -132,8 +153,33 @@ def train_surrogate(mat, iters=6000):
     return s, rec
 
 
+def train_cutnet(mat, dt, h, iterations, samples):
+    """Trains the cut network on the device; returns it and a record of the
+    training (not part of the timed step)."""
+    from karamelo_amd import native as N
+    t0 = time.perf_counter()
+    net = N.CutNet(mat, dt=dt, cellsize=h, samples=samples, iterations=iterations,
+                   increment_min=1e-5, increment_max=0.05, batch=16384)
+    device_sync()
+    t1 = time.perf_counter()
+    loss, secs = net.train()
+    import numpy as np
+    X, Y = net.tuples(20000, seed=99)
+    out = net.eval(X)
+    plastic = Y[:, 6] > 0
+    s_err = float(np.linalg.norm(out[plastic, :6] - Y[plastic, :6]) /
+                  np.linalg.norm(Y[plastic, :6]))
+    rec = {"tuples": samples, "tuple_and_fit_seconds": t1 - t0, "iterations": iterations,
+           "batch": 16384, "train_seconds": secs, "final_loss": loss,
+           "validation": {"deviatoric_stress_rel_l2_plastic": s_err,
+                          "branch_agreement": float(np.mean((out[:, 6] > 0) == plastic))},
+           "mfma_tflops": iterations * 16384 * CUTNET_FLOPS_PER_TRAIN_SAMPLE / secs / 1e12}
+    log("[cutnet] trained: %s" % rec)
+    return net, rec
+
+
 def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_steps=10,
-             preload=1000):
+             preload=1000, cutnet=None):
     """Times `steps` steps of one path after `preload` + `warmup` steps;
     returns the timing record and the downloaded particle state."""
     from karamelo_amd import native as N
@@ -143,6 +189,8 @@ def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_
     ctx = P.context_for(dataclasses.replace(prob, cfg=cfg))
     if path == "mixed+surrogate":
         ctx.set_surrogate(surrogate)
+    if path == "mixed+cutnet":
+        ctx.set_cutnet(cutnet)
     log("[%s] pre-load %d steps (pulling the bar past yield) ..." % (path, preload))
     ctx.step(preload)
     log("[%s] warm-up %d steps ..." % (path, warmup))
@@ -161,7 +209,7 @@ def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_
     # first call captures it (and replays it once), outside the timed region
     ctx.step_graph(graph_steps, graph_steps)
     device_sync()
-    ctx.surrogate_stats()          # resets the counters: the timed steps only
+    ctx.cutnet_stats()             # resets the counters: the timed steps only
     log("[%s] timed %d steps (graph replays of %d) ..." % (path, steps, graph_steps))
     t0 = time.perf_counter()
     ctx.step_graph(steps, graph_steps)
@@ -171,7 +219,7 @@ def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_
     st, tag = ctx.synchronize()
     if st != 0:
         raise RuntimeError("device error %d (particle %d) in the timed region" % (st, tag))
-    evals, fallbacks = ctx.surrogate_stats()
+    evals, fallbacks, judged, judged_err = ctx.cutnet_stats()
     # per-phase launch times: as many eager steps again, with HIP events
     # recorded on the library's stream around each phase
     log("[%s] %d eager steps with phase events ..." % (path, steps))
@@ -188,7 +236,7 @@ def run_path(prob, path, steps, warmup, surrogate=None, autotune_steps=5, graph_
     ctx.close()
     secs = t1 - t0
     return {"secs": secs, "phases": phases, "surrogate_evals": evals,
-            "surrogate_fallbacks": fallbacks,
+            "surrogate_fallbacks": fallbacks, "judged": judged, "judged_stress_rel_l2": judged_err,
             "autotune": {"tile": list(tile), "variant": variant,
                          "ms_per_step_per_variant": [round(float(v), 4) for v in tune_ms]}}, state
 
@@ -286,7 +334,8 @@ def roofline_of(path, phases, npart, workload):
     alg = ALG_BYTES_PER_PARTICLE[path].get(dom, 0) * npart or None
     kname = KERNEL_NAMES.get(dom, dom)
     achieved = alg / avg_s / 1e9 if alg else None
-    wl = workload + {"plain-fp64": "", "mixed": "_mixed", "mixed+surrogate": "_mixed_surrogate"}[path]
+    wl = workload + {"plain-fp64": "", "mixed": "_mixed", "mixed+surrogate": "_mixed_surrogate",
+                     "mixed+cutnet": "_mixed_cutnet"}[path]
     return {"bound": "hbm", "kernel": kname,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
@@ -379,7 +428,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--h", type=float, default=6e-5, help="cell size (m)")
-    ap.add_argument("--paths", default="plain-fp64,mixed,mixed+surrogate",
+    ap.add_argument("--paths", default="plain-fp64,mixed,mixed+cutnet,mixed+surrogate",
                     help="comma-separated paths to time; the last one is the headline")
     ap.add_argument("--autotune-steps", type=int, default=5,
                     help="trial steps per tile variant (0: keep the default tiles)")
@@ -387,6 +436,9 @@ def main():
                     help="initial effective plastic strain of every particle (counter runs: "
                          "the surrogate is in use from the first step without a warm-up)")
     ap.add_argument("--train-iterations", type=int, default=6000)
+    ap.add_argument("--cutnet-iterations", type=int, default=20000)
+    ap.add_argument("--cutnet-tuples", type=int, default=50_000_000,
+                    help="training tuples of the cut network (device-generated)")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="steps per captured graph in the timed region")
     ap.add_argument("--no-batched", action="store_true",
@@ -399,6 +451,7 @@ def main():
               "would shard the tiles across ranks); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     from karamelo_amd import problems as P
 
     paths = args.paths.split(",")
@@ -415,11 +468,21 @@ def main():
     if "mixed+surrogate" in paths:
         log("training the surrogate ...")
         surrogate, training = train_surrogate(prob.mats[0], args.train_iterations)
+    cutnet, cut_training = None, None
+    if "mixed+cutnet" in paths:
+        m = prob.mats[0]
+        # the CFL time step of the bar at rest (ulmpm.cpp:525-551, solid.cpp:1376-1385)
+        dt = prob.cfg.dt_factor * args.h / float(np.sqrt((m.K + 4.0 / 3.0 * m.G) / m.rho0))
+        log("training the cut network (dt %.3g s) ..." % dt)
+        cutnet, cut_training = train_cutnet(m, dt, args.h, args.cutnet_iterations,
+                                            args.cutnet_tuples)
     for path in paths:
         results[path], states[path] = run_path(prob, path, args.steps, args.warmup, surrogate,
                                                autotune_steps=args.autotune_steps,
                                                graph_steps=args.graph_steps,
-                                               preload=args.preload_steps)
+                                               preload=args.preload_steps, cutnet=cutnet)
+    if cutnet is not None:
+        cutnet.close()
     prob.particles = None
 
     head = paths[-1]
@@ -437,6 +500,26 @@ def main():
             "roofline": roofline_of(path, rp["phases"], npart, workload),
             "autotune": rp["autotune"],
         }
+        if path == "mixed+cutnet":
+            cut_ms, cut_n = rp["phases"]["cut"]
+            ev = rp["surrogate_evals"]
+            if ev < 0.5 * args.steps * npart:
+                raise RuntimeError("mixed+cutnet: %d network evaluations in %d timed steps of "
+                                   "%d particles" % (ev, args.steps, npart))
+            mf = ev * CUTNET_FLOPS_PER_EVAL / (cut_ms * 1e-3) / 1e12
+            per_path[path]["cutnet"] = {
+                "evaluations": ev, "fallbacks": rp["surrogate_fallbacks"],
+                "fallback_fraction": rp["surrogate_fallbacks"] / max(1, args.steps * npart),
+                "judged_updates": rp["judged"],
+                "judged_deviatoric_stress_rel_l2": rp["judged_stress_rel_l2"],
+                "mfma_roofline": {
+                    "bound": "mfma", "kernel": "k_stress_cutnet",
+                    "flops_per_eval": CUTNET_FLOPS_PER_EVAL, "achieved": mf,
+                    "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": mf / MFMA_BF16_PEAK_TFLOPS,
+                    "avg_launch_ms": cut_ms / max(cut_n, 1),
+                    "note": "algorithmic network flops over the cut kernel's launch time"},
+                "training": cut_training}
         if path == "mixed+surrogate":
             # surrogate evaluations happen inside the fused G2P launches
             g2p_ms, g2p_n = rp["phases"]["g2p"]

@@ -323,7 +323,8 @@ enum {
   KMPM_PHASE_GRID_B = 6,    /* MUSL velocities_to_grid (node velocities) */
   KMPM_PHASE_STRESS_B = 7,  /* MUSL rate of deformation + F + stress */
   KMPM_PHASE_DT = 8,        /* update_time + adjust_dt */
-  KMPM_NPHASE = 9
+  KMPM_PHASE_CUT = 9,       /* USL with the cut network: its constitutive update */
+  KMPM_NPHASE = 10
 };
 /* enable != 0 starts recording (and clears totals); 0 stops. */
 int kmpm_timing_enable(kmpm_ctx *ctx, int enable);
@@ -453,6 +454,94 @@ int kmpm_ctx_set_surrogate(kmpm_ctx *ctx, const kmpm_surrogate *s);
 /* Particle updates since the last call that used a network, and that fell
  * back to the plain functions (outside the envelope); resets both. */
 int kmpm_surrogate_stats(kmpm_ctx *ctx, int64_t *evaluations, int64_t *fallbacks);
+
+/* ---- the cut network ----------------------------------------------------------
+ * One network in place of both constitutive calls of Solid::update_stress
+ * for a johnson_cook + shock (+ damage_johnson_cook) material: the strength
+ * call (solid.cpp:1314-1316, StrengthJohnsonCook::update_deviatoric_stress,
+ * strength_jc.cpp:100-184) and the damage call (solid.cpp:1337-1339,
+ * DamageJohnsonCook::compute_damage, damage_jc.cpp:91-143).  The EOS call
+ * before, the eps_p accumulation and rate filter between them and the
+ * stress / strain assembly after stay the plain code.
+ *   18 features -> 64 -> 64 -> 64 -> 64 -> 64 -> 9, GELU (tanh form), every
+ *   product on the BF16 matrix cores with FP32 accumulation, over an FP32
+ *   linear skip of the inputs.
+ * Features (A = JC A, G = shear modulus, Gd = G (1 - damage), epsdot0 =
+ * JC reference rate): t / A [6] with t = dev(sigma_n + 2 Gd dt D) the
+ * trial deviator, 2 G dt dev(D) / A [6], eps_p^(1/4),
+ * ln max(epsdot_p / epsdot0, 1), damage, damage_init, pH / A, q / A with
+ * q the trial von Mises stress.
+ * Outputs: dev(sigma_{n+1}) / A [6], the signed plastic increment
+ *   G (q - sigma_y) / (Gd A) (= 3 G dps / A where positive),
+ *   3 G (damage_init_{n+1} - damage_init_n) / A, damage_{n+1}.
+ * In the step the plastic branch is the network's where its dps output is
+ * positive; the elastic branch keeps the trial deviator exactly, the
+ * damage_init reset at triaxiality <= -3 and the damage closed form are
+ * applied exactly.  States outside the training envelope use the plain FP64
+ * functions, which also judge a sample of the network's updates.  Training
+ * runs on the device on tuples the plain functions generate.  No
+ * counterpart in the reference. */
+typedef struct kmpm_cutnet_config {
+  double eps_p_max;        /* envelope and sampling: eps_p in [0, eps_p_max] */
+  double rate_min;         /* sampled epsdot_p / epsdot0, log-uniform in [rate_min, rate_max] */
+  double rate_max;         /* (the envelope's bound) */
+  double damage_max;       /* damage in [0, damage_max), damage_max < 1 */
+  double damage_init_max;  /* damage_init in [0, damage_init_max] */
+  double pressure_max;     /* |pH| / A <= pressure_max */
+  double increment_min;    /* 2 G dt vm(dev D) / A, log-uniform in [min, max] */
+  double increment_max;
+  double dt;               /* time step of the sampled states (the rate filter) */
+  double cellsize;         /* h of the simulations (the filter's 1000 h / c_signal) */
+  int64_t samples;         /* training tuples generated on the device */
+  int64_t iterations;      /* default Adam iterations (kmpm_cutnet_train takes its own) */
+  int32_t batch;           /* tuples per iteration, a multiple of 128 */
+  int32_t judge_stride;    /* in the step, one wave in judge_stride also runs the plain
+                              strength and sums the squared error (0: no judge) */
+  double learning_rate;    /* initial Adam step; cosine decay to 1% of it */
+  uint64_t seed;
+} kmpm_cutnet_config;
+
+typedef struct kmpm_cutnet kmpm_cutnet;
+/* FP32 parameters (flat order: w1[64][18], b1[64], then w_l[64][64], b_l[64]
+ * for l = 2..5, w6[9][64], b6[9]; weights [out][in]) */
+enum { KMPM_CUTNET_NPARAM = 18441, KMPM_CUTNET_NI = 18, KMPM_CUTNET_NO = 9 };
+
+/* Generates the training tuples, fits the input maps and the linear skip,
+ * and initialises the network (JC strength + shock EOS material). */
+int kmpm_cutnet_create(const kmpm_material *mat, const kmpm_cutnet_config *cfg,
+                       kmpm_cutnet **out);
+int kmpm_cutnet_destroy(kmpm_cutnet *net);
+/* `iterations` Adam iterations (forward + backward on the matrix cores);
+ * final_loss: mean loss of the last (up to 16) iterations; seconds: wall time. */
+int kmpm_cutnet_train(kmpm_cutnet *net, int64_t iterations, double *final_loss, double *seconds);
+/* n fresh tuples from the plain functions: features [n][18], targets [n][9]. */
+int kmpm_cutnet_tuples(kmpm_cutnet *net, int64_t n, uint64_t seed, float *features,
+                       float *targets);
+/* Network outputs (target units) for features [n][18] -> outputs [n][9]. */
+int kmpm_cutnet_eval(kmpm_cutnet *net, const float *features, int64_t n, float *outputs);
+/* Gradient of L = (1/n) sum_p sum_k (z_k - (y_k - skip_k(x)) / out_scale_k)^2
+ * with respect to the FP32 parameters (flat order), z the raw network output. */
+int kmpm_cutnet_gradient(kmpm_cutnet *net, const float *features, const float *targets,
+                         int64_t n, float *grad, double *loss);
+/* Read (set = 0) or replace (set = 1) the FP32 parameters. */
+int kmpm_cutnet_params(kmpm_cutnet *net, float *params, int nparam, int set);
+/* The maps: in_scale[18], in_shift[18] (x = f in_scale + in_shift),
+ * out_scale[9], skip[9][19] (output = skip[k][0..17] . x + skip[k][18] +
+ * out_scale[k] z_k); set = 1 replaces them. */
+int kmpm_cutnet_maps(kmpm_cutnet *net, float *in_scale, float *in_shift, float *out_scale,
+                     float *skip, int set);
+/* The envelope: eps_p, epsdot/epsdot0, damage, damage_init, |pH|/A, trial
+ * von Mises / A, relative increment (upper bounds). */
+int kmpm_cutnet_envelope(kmpm_cutnet *net, double env[7]);
+/* Use the network in the USL step of a mixed-precision ULMPM context with
+ * one material, this network's (NULL: back to the plain functions); the
+ * context copies it. */
+int kmpm_ctx_set_cutnet(kmpm_ctx *ctx, const kmpm_cutnet *net);
+/* Since the last call: particle updates by the network, by the plain
+ * fallback, updates judged, and the judged sums of the squared Frobenius
+ * error of the deviatoric stress and of the plain deviatoric stress. */
+int kmpm_cutnet_stats(kmpm_ctx *ctx, int64_t *evaluations, int64_t *fallbacks, int64_t *judged,
+                      double *judge_err2, double *judge_ref2);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@
 
 #include "kmpm_ctx.h"
 #include "kmpm_mat3.h"
+#include "kmpm_cutnet.h"
 #include "kmpm_surrogate.h"
 
 using namespace kmpm;
@@ -473,7 +474,7 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   void *ptrs[] = {c->d_mats, c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
                   c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst, c->d_red,
-                  c->d_fexpr};
+                  c->d_fexpr, c->d_cut, c->d_cut_stage};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   free(c->mats_host);
@@ -576,6 +577,18 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   HIPCHK(dalloc(&c->d_solid, cap));
   a.np = (int)np;
   a.cap = cap;
+  if (c->d_cut_stage && c->cut_stage_cap < cap) {   // the cut network's staging, [8][cap]
+    const int was_on = a.cut_on;
+    (void)hipFree(c->d_cut_stage);
+    c->d_cut_stage = nullptr;
+    c->cut_stage_cap = 0;
+    a.cut_stage = nullptr;
+    a.cut_on = 0;
+    HIPCHK(dalloc(&c->d_cut_stage, (size_t)8 * cap));
+    c->cut_stage_cap = cap;
+    a.cut_stage = c->d_cut_stage;
+    a.cut_on = was_on;
+  }
   {
     const int st = setup_tiles(c, c->tile_variant);   // P2G partial blocks for np particles
     if (st != KMPM_OK) {
@@ -834,6 +847,11 @@ int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
   tbegin(c, KMPM_PHASE_G2P);
   HIPCHK(c->a.ref_order ? launch_ref_g2p_usl(c) : launch_g2p_usl(c));
   tend(c);
+  if (c->a.cut_on) {
+    tbegin(c, KMPM_PHASE_CUT);
+    HIPCHK(launch_cutnet_stress(c));
+    tend(c);
+  }
   return after_g2p_state_swap(c);
 }
 
@@ -1321,6 +1339,63 @@ int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
   HIPCHK(hipMemcpy(c->d_surr, &img, sizeof(img), hipMemcpyHostToDevice));
   c->a.surr = c->d_surr;
   c->a.surr_on = 1;
+  return KMPM_OK;
+}
+
+int kmpm_ctx_set_cutnet(kmpm_ctx *c, const kmpm_cutnet *s) {
+  if (!c) return KMPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  if (!s) {
+    c->a.cut_on = 0;
+    return KMPM_OK;
+  }
+  // one network serves every particle of the wave: one material, the cut
+  // network's; USL on one instance of the mixed ULMPM path
+  if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
+      c->a.ref_order || c->a.ninst || c->cfg.scheme != KMPM_SCHEME_USL || c->nmat != 1 ||
+      !cutnet_matches(s, c->mats_host[0]))
+    return KMPM_ERR_ARG;
+  if (!c->d_cut) HIPCHK(dalloc(&c->d_cut, 1));
+  if (c->cut_stage_cap < c->a.cap || !c->d_cut_stage) {
+    if (c->d_cut_stage) (void)hipFree(c->d_cut_stage);
+    c->d_cut_stage = nullptr;
+    c->cut_stage_cap = 0;
+    HIPCHK(dalloc(&c->d_cut_stage, (size_t)8 * (c->a.cap > 0 ? c->a.cap : 1)));
+    c->cut_stage_cap = c->a.cap > 0 ? c->a.cap : 1;
+  }
+  const int st = cutnet_device_image(s, c->d_cut, c->stream);
+  if (st) return st;
+  c->a.cut = c->d_cut;
+  c->a.cut_stage = c->d_cut_stage;
+  c->a.cut_on = 1;
+  return KMPM_OK;
+}
+
+int kmpm_cutnet_stats(kmpm_ctx *c, int64_t *evals, int64_t *fallbacks, int64_t *judged,
+                      double *judge_err2, double *judge_ref2) {
+  if (!c || !evals || !fallbacks || !judged || !judge_err2 || !judge_ref2) return KMPM_ERR_ARG;
+  DevScalars sc;
+  HIPCHK(hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *evals = (int64_t)sc.surr_evals;
+  *fallbacks = (int64_t)sc.surr_fallbacks;
+  *judged = (int64_t)sc.judge_n;
+  *judge_err2 = sc.judge_err2;
+  *judge_ref2 = sc.judge_ref2;
+  const unsigned long long zc[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync((char *)c->d_sc + offsetof(DevScalars, surr_evals), zc, sizeof(zc),
+                        hipMemcpyHostToDevice, c->stream));
+  struct {
+    unsigned long long n;
+    double e, r;
+  } zj = {0, 0.0, 0.0};
+  static_assert(offsetof(DevScalars, judge_err2) == offsetof(DevScalars, judge_n) + 8 &&
+                    offsetof(DevScalars, judge_ref2) == offsetof(DevScalars, judge_n) + 16,
+                "judge counters are contiguous");
+  HIPCHK(hipMemcpyAsync((char *)c->d_sc + offsetof(DevScalars, judge_n), &zj, 24,
+                        hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return KMPM_OK;
 }
 

@@ -8,6 +8,7 @@
 namespace kmpm {
 
 struct SurrogateDev;
+struct CutNetDev;
 
 // Tile geometry shared by the sort key, P2G, grid update and G2P kernels.
 struct TileGeom {
@@ -78,6 +79,12 @@ struct KArgs {
   // learned constitutive surrogate (JC + shock class, mixed path)
   const SurrogateDev *surr;
   int surr_on;
+  // cut network (JC + shock + JC damage class, mixed path, USL): the fused
+  // G2P stages the cut's inputs (rate of deformation, pressure, wave speed)
+  // in cut_stage [8][cap] and a separate kernel runs the network
+  const CutNetDev *cut;
+  float *cut_stage;
+  int cut_on;
   // scalars
   DevScalars *sc;
 };
@@ -122,6 +129,9 @@ struct kmpm_ctx {
   int *d_solid;          // solid index (by pid)
   kmpm::DevScalars *d_sc;
   kmpm::SurrogateDev *d_surr;
+  kmpm::CutNetDev *d_cut;
+  float *d_cut_stage;     // [8][cut_stage_cap]
+  int cut_stage_cap;
   kmpm::InstScalars *d_inst;
   int *d_tinst;
   std::vector<int32_t> *inst_cells;   // per instance: cell_lo[3], cell_hi[3]
@@ -159,6 +169,7 @@ hipError_t launch_grid_update(kmpm_ctx *c, int remap);
 hipError_t launch_g2p_usl(kmpm_ctx *c);
 hipError_t launch_g2p_musl_a(kmpm_ctx *c);
 hipError_t launch_stress_musl_b(kmpm_ctx *c);
+hipError_t launch_cutnet_stress(kmpm_ctx *c);   // kmpm_cutnet.hip
 hipError_t launch_adjust_dt(kmpm_ctx *c);
 hipError_t launch_grid_clean(kmpm_ctx *c);
 hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);
@@ -175,5 +186,7 @@ size_t scan_temp_bytes(int64_t n);
 DevMat make_devmat(const kmpm_material &m);
 int surrogate_device_image(const kmpm_surrogate *s, SurrogateDev *out);
 bool surrogate_matches(const kmpm_surrogate *s, const kmpm_material &m);
+bool cutnet_matches(const kmpm_cutnet *s, const kmpm_material &m);
+int cutnet_device_image(const kmpm_cutnet *s, CutNetDev *dst_dev, hipStream_t stream);
 hipError_t copy_probe(int64_t bytes, int reps, double *gbps);
 }  // namespace kmpm

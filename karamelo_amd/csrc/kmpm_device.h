@@ -128,6 +128,10 @@ struct DevScalars {
   // k_adjust_dt clears the other buffer for the next step
   double fv_force[2][MAX_FIX][3];
   double fv_last[MAX_FIX][6];            // the last grid update's fix values (val, prev)
+  // cut network: network updates the plain functions also judged, and the
+  // judged sums of squared deviatoric-stress error and of the plain stress
+  unsigned long long judge_n;
+  double judge_err2, judge_ref2;
 };
 // k_tiles appends to both tile lists with one 64-bit atomic on the pair
 static_assert(offsetof(DevScalars, tile_count) % 8 == 0 &&

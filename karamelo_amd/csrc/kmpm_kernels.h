@@ -1138,7 +1138,11 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // ---------------------------------------------------------------------------
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
-template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool SURR>
+// SURR: 0 the plain constitutive update, 1 the learned JC material
+// functions (kmpm_surrogate.h), 2 the cut network: the constitutive update
+// is left to k_stress_cutnet, and this kernel stages its inputs (the rate of
+// deformation, the EOS pressure and the wave speed) in a.cut_stage.
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, int SURR>
 // Mixed linear and JC classes fit four waves per SIMD without spilling (the
 // fluid and generic classes would spill at that bound).  The surrogate
 // kernel spills some 40 registers at four waves and is still faster there
@@ -1159,9 +1163,9 @@ __global__ __launch_bounds__(256, SURR ? 4
   __shared__ R nb[2 * DIM * BN];
   // SURR: the networks and envelope, staged in LDS once per block (read at
   // each use instead of being held in registers across the particle loop)
-  __shared__ __attribute__((aligned(16))) unsigned char sbuf[SURR ? sizeof(SurrogateDev) : 16];
+  __shared__ __attribute__((aligned(16))) unsigned char sbuf[SURR == 1 ? sizeof(SurrogateDev) : 16];
   const SurrogateDev *sd = reinterpret_cast<const SurrogateDev *>(sbuf);
-  if constexpr (SURR) {
+  if constexpr (SURR == 1) {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(a.surr);
     uint32_t *dst = reinterpret_cast<uint32_t *>(sbuf);
     for (int q = threadIdx.x; q < (int)(sizeof(SurrogateDev) / 4); q += blockDim.x) dst[q] = src[q];
@@ -1244,6 +1248,29 @@ __global__ __launch_bounds__(256, SURR ? 4
       if (live) STB(VOL, j, J * vol0);
       R dmg = (R)LDA(DMG, slot);
       if (live && J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+      if constexpr (SURR == 2) {
+        // the cut's inputs: D = sym(L) (solid.cpp:1239), the EOS pressure
+        // (eos_shock.cpp:99-133, FP64 from J as on the plain mixed path) and
+        // the wave speed before the damage test (solid.cpp:1376-1385)
+        const DevMat &m = a.mats[mat];
+        const double rho = m.rho0 / J;
+        double D[6];
+        D[0] = L[0];
+        D[1] = L[4];
+        D[2] = L[8];
+        D[3] = 0.5 * (L[1] + L[3]);
+        D[4] = 0.5 * (L[2] + L[6]);
+        D[5] = 0.5 * (L[5] + L[7]);
+        const double pH = shock_pressure<double>(m, J, rho, (double)dmg, D, a.h);
+        const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
+        const double c = sqrt(m.cfl_k43g / rho) + vm;
+        float *st = a.cut_stage;
+        const size_t cs = (size_t)a.cap;
+        for (int e = 0; e < 6; e++) st[e * cs + j] = (float)D[e];
+        st[6 * cs + j] = (float)pH;
+        st[7 * cs + j] = (float)c;
+        return;
+      }
       R sig[6], eel[6];
       for (int e = 0; e < 6; e++) {
         sig[e] = (R)LDA(SXX + e, slot);
@@ -1251,8 +1278,9 @@ __global__ __launch_bounds__(256, SURR ? 4
       }
       R ep = (R)LDA(EP, slot), epdot = (R)LDA(EPDOT, slot);
       R dinit = (R)LDA(DINIT, slot);
-      const double c = stress_and_speed<MC, PREC, SURR>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
-                                                  mat, v, pid, live, sd, &scnt);
+      const double c = stress_and_speed<MC, PREC, SURR == 1>(a, dt, L, J, sig, eel, ep, epdot,
+                                                             dinit, dmg, mat, v, pid, live, sd,
+                                                             &scnt);
       if (!live) return;
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
@@ -1264,7 +1292,7 @@ __global__ __launch_bounds__(256, SURR ? 4
       STB(DINIT, j, dinit);
       STB(DMG, j, dmg);
     };
-    if constexpr (SURR) {
+    if constexpr (SURR == 1) {
       for (int jb = p0; jb < p1; jb += blockDim.x) {
         const int j = jb + (int)threadIdx.x;
         particle(j < p1 ? j : p1 - 1, j < p1);
@@ -1275,7 +1303,7 @@ __global__ __launch_bounds__(256, SURR ? 4
     if (a.ninst) flush_tile_cfl<false>(a, t, cmax, rmin);
   }
   finish_cfl<false>(a, cmax, rmin);
-  if constexpr (SURR) surrogate_count_flush(scnt, a.sc);
+  if constexpr (SURR == 1) surrogate_count_flush(scnt, a.sc);
 }
 
 // ---------------------------------------------------------------------------

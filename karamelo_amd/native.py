@@ -28,7 +28,7 @@ EOS_NONE, EOS_LINEAR, EOS_SHOCK, EOS_FLUID = 0, 1, 2, 3
 STRENGTH_NONE, STRENGTH_LINEAR, STRENGTH_JOHNSON_COOK, STRENGTH_FLUID = 0, 1, 2, 3
 DAMAGE_NONE, DAMAGE_JOHNSON_COOK = 0, 1
 REGION_BLOCK, REGION_CYLINDER = 0, 1
-PHASES = ["sort", "p2g", "grid", "g2p", "g2p_a", "p2g_b", "grid_b", "stress_b", "dt"]
+PHASES = ["sort", "p2g", "grid", "g2p", "g2p_a", "p2g_b", "grid_b", "stress_b", "dt", "cut"]
 NPHASE = len(PHASES)
 
 # style keys of the reference's plugin registries -> enum values
@@ -75,6 +75,18 @@ class SurrogateConfig(C.Structure):
     _fields_ = [("eps_p_max", C.c_double), ("rate_max", C.c_double), ("triax_max", C.c_double),
                 ("samples", C.c_int64), ("iterations", C.c_int64), ("batch", C.c_int32),
                 ("reserved", C.c_int32), ("learning_rate", C.c_double), ("seed", C.c_uint64)]
+
+
+class CutNetConfig(C.Structure):
+    _fields_ = [("eps_p_max", C.c_double), ("rate_min", C.c_double), ("rate_max", C.c_double),
+                ("damage_max", C.c_double), ("damage_init_max", C.c_double),
+                ("pressure_max", C.c_double), ("increment_min", C.c_double),
+                ("increment_max", C.c_double), ("dt", C.c_double), ("cellsize", C.c_double),
+                ("samples", C.c_int64), ("iterations", C.c_int64), ("batch", C.c_int32),
+                ("judge_stride", C.c_int32), ("learning_rate", C.c_double), ("seed", C.c_uint64)]
+
+
+CUTNET_NPARAM, CUTNET_NI, CUTNET_NO = 18441, 18, 9
 
 
 class Particles(C.Structure):
@@ -200,6 +212,17 @@ def _declare(lib):
         "kmpm_surrogate_envelope": (C.c_int, [vp, _DP, _DP]),
         "kmpm_ctx_set_surrogate": (C.c_int, [vp, vp]),
         "kmpm_surrogate_stats": (C.c_int, [vp, P(C.c_int64), P(C.c_int64)]),
+        "kmpm_cutnet_create": (C.c_int, [P(Material), P(CutNetConfig), P(vp)]),
+        "kmpm_cutnet_destroy": (C.c_int, [vp]),
+        "kmpm_cutnet_train": (C.c_int, [vp, C.c_int64, _DP, _DP]),
+        "kmpm_cutnet_tuples": (C.c_int, [vp, C.c_int64, C.c_uint64, _FP, _FP]),
+        "kmpm_cutnet_eval": (C.c_int, [vp, _FP, C.c_int64, _FP]),
+        "kmpm_cutnet_gradient": (C.c_int, [vp, _FP, _FP, C.c_int64, _FP, _DP]),
+        "kmpm_cutnet_params": (C.c_int, [vp, _FP, C.c_int, C.c_int]),
+        "kmpm_cutnet_maps": (C.c_int, [vp, _FP, _FP, _FP, _FP, C.c_int]),
+        "kmpm_cutnet_envelope": (C.c_int, [vp, _DP]),
+        "kmpm_ctx_set_cutnet": (C.c_int, [vp, vp]),
+        "kmpm_cutnet_stats": (C.c_int, [vp, P(C.c_int64), P(C.c_int64), P(C.c_int64), _DP, _DP]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -430,6 +453,19 @@ class Context:
         check(self.L.kmpm_ctx_set_surrogate(self.h, surrogate.h if surrogate else None),
               "set_surrogate")
 
+    def set_cutnet(self, net):
+        check(self.L.kmpm_ctx_set_cutnet(self.h, net.h if net else None), "set_cutnet")
+
+    def cutnet_stats(self):
+        """(network updates, fallbacks, judged updates, judged relative L2
+        error of the deviatoric stress) since the last call."""
+        ev, fb, nj = C.c_int64(), C.c_int64(), C.c_int64()
+        e2, r2 = C.c_double(), C.c_double()
+        check(self.L.kmpm_cutnet_stats(self.h, C.byref(ev), C.byref(fb), C.byref(nj),
+                                       C.byref(e2), C.byref(r2)), "cutnet_stats")
+        rel = (e2.value / r2.value) ** 0.5 if r2.value > 0 else 0.0
+        return ev.value, fb.value, nj.value, rel
+
     def surrogate_stats(self):
         ev, fb = C.c_int64(), C.c_int64()
         check(self.L.kmpm_surrogate_stats(self.h, C.byref(ev), C.byref(fb)), "surrogate_stats")
@@ -580,3 +616,94 @@ class Surrogate:
         lo, hi = (C.c_double * 4)(), (C.c_double * 4)()
         check(self.L.kmpm_surrogate_envelope(self.h, lo, hi), "surrogate_envelope")
         return np.array(lo[:]), np.array(hi[:])
+
+
+class CutNet:
+    """The cut network: one 18 -> 64 x5 -> 9 GELU MLP on the BF16 matrix
+    cores in place of both constitutive calls (kmpm_cutnet)."""
+
+    def __init__(self, mat, dt, cellsize, eps_p_max=0.5, rate_min=1e-3, rate_max=1e4,
+                 damage_max=0.5, damage_init_max=0.9, pressure_max=3.0, increment_min=1e-6,
+                 increment_max=1e-2, samples=1 << 22, iterations=3000, batch=16384,
+                 judge_stride=64, learning_rate=2e-3, seed=20261203):
+        self.L = lib()
+        c = CutNetConfig()
+        c.eps_p_max, c.rate_min, c.rate_max = eps_p_max, rate_min, rate_max
+        c.damage_max, c.damage_init_max, c.pressure_max = damage_max, damage_init_max, pressure_max
+        c.increment_min, c.increment_max = increment_min, increment_max
+        c.dt, c.cellsize = dt, cellsize
+        c.samples, c.iterations, c.batch = samples, iterations, batch
+        c.judge_stride, c.learning_rate, c.seed = judge_stride, learning_rate, seed
+        self.cfg = c
+        self._mat = mat
+        h = C.c_void_p()
+        check(self.L.kmpm_cutnet_create(C.byref(mat), C.byref(c), C.byref(h)),
+              "kmpm_cutnet_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.kmpm_cutnet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def train(self, iterations=None):
+        """Returns (mean loss of the last iterations, seconds)."""
+        it = self.cfg.iterations if iterations is None else iterations
+        loss, secs = C.c_double(), C.c_double()
+        check(self.L.kmpm_cutnet_train(self.h, it, C.byref(loss), C.byref(secs)), "cutnet_train")
+        return loss.value, secs.value
+
+    def tuples(self, n, seed):
+        X = np.zeros((n, CUTNET_NI), np.float32)
+        Y = np.zeros((n, CUTNET_NO), np.float32)
+        check(self.L.kmpm_cutnet_tuples(self.h, n, seed, X.ctypes.data_as(_FP),
+                                        Y.ctypes.data_as(_FP)), "cutnet_tuples")
+        return X, Y
+
+    def eval(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros((len(X), CUTNET_NO), np.float32)
+        check(self.L.kmpm_cutnet_eval(self.h, X.ctypes.data_as(_FP), len(X),
+                                      out.ctypes.data_as(_FP)), "cutnet_eval")
+        return out
+
+    def gradient(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(CUTNET_NPARAM, np.float32)
+        loss = C.c_double()
+        check(self.L.kmpm_cutnet_gradient(self.h, X.ctypes.data_as(_FP), Y.ctypes.data_as(_FP),
+                                          len(X), g.ctypes.data_as(_FP), C.byref(loss)),
+              "cutnet_gradient")
+        return g, loss.value
+
+    def params(self):
+        p = np.zeros(CUTNET_NPARAM, np.float32)
+        check(self.L.kmpm_cutnet_params(self.h, p.ctypes.data_as(_FP), len(p), 0), "cutnet_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32).copy()
+        check(self.L.kmpm_cutnet_params(self.h, p.ctypes.data_as(_FP), len(p), 1), "cutnet_params")
+
+    def maps(self):
+        isc = np.zeros(CUTNET_NI, np.float32)
+        ish = np.zeros(CUTNET_NI, np.float32)
+        osc = np.zeros(CUTNET_NO, np.float32)
+        sk = np.zeros((CUTNET_NO, CUTNET_NI + 1), np.float32)
+        check(self.L.kmpm_cutnet_maps(self.h, isc.ctypes.data_as(_FP), ish.ctypes.data_as(_FP),
+                                      osc.ctypes.data_as(_FP), sk.ctypes.data_as(_FP), 0),
+              "cutnet_maps")
+        return {"in_scale": isc, "in_shift": ish, "out_scale": osc, "skip": sk}
+
+    def envelope(self):
+        e = (C.c_double * 7)()
+        check(self.L.kmpm_cutnet_envelope(self.h, e), "cutnet_envelope")
+        return dict(zip(("eps_p", "rate", "damage", "damage_init", "pressure", "q_trial",
+                         "increment"), e[:]))
