@@ -523,6 +523,20 @@ int kmpm_cutnet_eval(kmpm_cutnet *net, const float *features, int64_t n, float *
  * with respect to the FP32 parameters (flat order), z the raw network output. */
 int kmpm_cutnet_gradient(kmpm_cutnet *net, const float *features, const float *targets,
                          int64_t n, float *grad, double *loss);
+/* kmpm_cutnet_gradient without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight training loops, harnesses that forbid hipMalloc).
+ * features [n][18], targets [n][9], grad [KMPM_CUTNET_NPARAM] and loss [1]
+ * are DEVICE pointers on the network's device, which must be current.  The
+ * block partial sums live in a fixed file-scope device array (64 partial
+ * gradients, 4.7 MB per GPU, held until the process ends), so n may not
+ * exceed KMPM_CUTNET_NOALLOC_MAX_TUPLES (each block's FP32 sums then cover
+ * at most 2^20 tuples): larger n returns KMPM_ERR_CAPACITY.  Stream-ordered
+ * on the network's stream and synchronised before returning; calls that
+ * overlap on one GPU share the array and race. */
+enum { KMPM_CUTNET_NOALLOC_MAX_TUPLES = 1 << 26 };
+int kmpm_cutnet_gradient_noalloc(kmpm_cutnet *net, const float *features, const float *targets,
+                                 int64_t n, float *grad, float *loss);
 /* Read (set = 0) or replace (set = 1) the FP32 parameters. */
 int kmpm_cutnet_params(kmpm_cutnet *net, float *params, int nparam, int set);
 /* The maps: in_scale[18], in_shift[18] (x = f in_scale + in_shift),

@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -506,15 +507,20 @@ __global__ __launch_bounds__(256, 1) void k_cn_grad(const CutNetDev *net, const 
 
 // Sums the block partials in a fixed order into out[0..NPARAM] (gradient,
 // then the loss); with adam != 0 takes one Adam step on the parameters.
-__global__ void k_cn_reduce(float *p, const float *partial, int nblocks, float *out, float *mom,
-                            float *vel, int adam, float lr, float b1, float b2, float eps, float c1,
-                            float c2) {
+// The loss goes to out_loss[0].
+__global__ void k_cn_reduce(float *p, const float *partial, int nblocks, float *out,
+                            float *out_loss, float *mom, float *vel, int adam, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q > CN_NPARAM) return;
   float g = 0.0f;
   for (int b = 0; b < nblocks; b++) g += partial[(size_t)b * (CN_NPARAM + 1) + q];
+  if (q == CN_NPARAM) {
+    out_loss[0] = g;
+    return;
+  }
   out[q] = g;
-  if (!adam || q == CN_NPARAM) return;
+  if (!adam) return;
   const float m = b1 * mom[q] + (1.0f - b1) * g;
   const float v = b2 * vel[q] + (1.0f - b2) * g * g;
   mom[q] = m;
@@ -701,6 +707,7 @@ struct kmpm_cutnet {
   float *d_partial = nullptr, *d_out = nullptr;
   int partial_blocks = 0;
   int64_t iter_done = 0;
+  int device = 0;
   hipStream_t stream = nullptr;
 };
 
@@ -761,8 +768,8 @@ int cn_grad_pass(kmpm_cutnet *s, const float *F, const float *Y, int64_t n, int 
   const float c1 = adam ? 1.0f / (1.0f - powf(b1, (float)t)) : 1.0f;
   const float c2 = adam ? 1.0f / (1.0f - powf(b2, (float)t)) : 1.0f;
   hipLaunchKernelGGL(k_cn_reduce, dim3((CN_NPARAM + 1 + 255) / 256), dim3(256), 0, s->stream,
-                     s->d_p, s->d_partial, nb, s->d_out, s->d_mom, s->d_vel, adam, lr, b1, b2,
-                     1e-8f, c1, c2);
+                     s->d_p, s->d_partial, nb, s->d_out, s->d_out + CN_NPARAM, s->d_mom, s->d_vel,
+                     adam, lr, b1, b2, 1e-8f, c1, c2);
   if (hipGetLastError() != hipSuccess) return KMPM_ERR_HIP;
   return adam ? cn_pack(s) : KMPM_OK;
 }
@@ -892,6 +899,7 @@ int kmpm_cutnet_create(const kmpm_material *mat, const kmpm_cutnet_config *cfg,
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return KMPM_ERR_NO_DEVICE;
   kmpm_cutnet *s = new kmpm_cutnet();
+  s->device = dev;
   s->cfg = c;
   s->mat = *mat;
   s->dmat = make_devmat(*mat);
@@ -1128,6 +1136,57 @@ int kmpm_cutnet_maps(kmpm_cutnet *s, float *in_scale, float *in_shift, float *ou
   std::memcpy(in_shift, H.in_shift, sizeof(H.in_shift));
   std::memcpy(out_scale, H.out_scale, sizeof(H.out_scale));
   std::memcpy(skip, H.skip, sizeof(H.skip));
+  return KMPM_OK;
+}
+
+// ---- the gradient without allocation ------------------------------------------
+//
+// kmpm_cutnet_gradient_noalloc keeps the block partial sums of the gradient
+// in `scratch`, a file-scope device array of CN_NOALLOC_BLOCKS partial
+// vectors (4.7 MB) that every GPU the library runs on holds until the
+// process ends.  Its address on each device is resolved once, with that
+// device current, on the entry point's first call there.  Calls that
+// overlap on one GPU share the array and race.
+constexpr int CN_NOALLOC_BLOCKS = 64;
+static __device__ __align__(256) float scratch[CN_NOALLOC_BLOCKS * (CN_NPARAM + 1)];
+
+namespace {
+constexpr int CN_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[CN_MAX_DEVICES];   // by device; nullptr until resolved
+
+int cn_scratch_ptr(int dev, float **out) {
+  if (dev < 0 || dev >= CN_MAX_DEVICES) return KMPM_ERR_NO_DEVICE;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return KMPM_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<float *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return KMPM_OK;
+}
+}  // namespace
+
+int kmpm_cutnet_gradient_noalloc(kmpm_cutnet *s, const float *features, const float *targets,
+                                 int64_t n, float *grad, float *loss) {
+  if (!s || !features || !targets || !grad || !loss) return KMPM_ERR_ARG;
+  if (n <= 0 || n > KMPM_CUTNET_NOALLOC_MAX_TUPLES) return KMPM_ERR_CAPACITY;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return KMPM_ERR_NO_DEVICE;
+  if (dev != s->device) return KMPM_ERR_ARG;
+  float *part = nullptr;
+  const int st = cn_scratch_ptr(dev, &part);
+  if (st) return st;
+  int64_t nb = (n + (int64_t)CN_GRAD_WAVES * 32 - 1) / ((int64_t)CN_GRAD_WAVES * 32);
+  if (nb > CN_NOALLOC_BLOCKS) nb = CN_NOALLOC_BLOCKS;
+  hipLaunchKernelGGL(k_cn_grad, dim3((unsigned)nb), dim3(256), 0, s->stream, s->d_net, s->d_nt,
+                     features, targets, n, 1.0f / (float)n, part);
+  hipLaunchKernelGGL(k_cn_reduce, dim3((CN_NPARAM + 1 + 255) / 256), dim3(256), 0, s->stream,
+                     s->d_p, part, (int)nb, grad, loss, s->d_mom, s->d_vel, 0, 0.0f, 0.9f, 0.999f,
+                     1e-8f, 1.0f, 1.0f);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess)
+    return KMPM_ERR_HIP;
   return KMPM_OK;
 }
 

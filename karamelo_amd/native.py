@@ -218,6 +218,7 @@ def _declare(lib):
         "kmpm_cutnet_tuples": (C.c_int, [vp, C.c_int64, C.c_uint64, _FP, _FP]),
         "kmpm_cutnet_eval": (C.c_int, [vp, _FP, C.c_int64, _FP]),
         "kmpm_cutnet_gradient": (C.c_int, [vp, _FP, _FP, C.c_int64, _FP, _DP]),
+        "kmpm_cutnet_gradient_noalloc": (C.c_int, [vp, vp, vp, C.c_int64, vp, vp]),
         "kmpm_cutnet_params": (C.c_int, [vp, _FP, C.c_int, C.c_int]),
         "kmpm_cutnet_maps": (C.c_int, [vp, _FP, _FP, _FP, _FP, C.c_int]),
         "kmpm_cutnet_envelope": (C.c_int, [vp, _DP]),
