@@ -47,38 +47,47 @@ METRIC = "particle-steps/s: mixed+surrogate vs plain FP64 GPU at ≤1e-3 rel L2 
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per particle of each timed phase, 3D, 8 particles per
-# cell (node data divided by 8).
+# cell (node data divided by 8): every state field a kernel must read or
+# write once (the particle state is x, v, F, sigma, strain_el, eps_p,
+# epsdot_p, damage_init, damage, mass, vol0, vol and the particle id and
+# material index), node data once.  The permutation (the sorted order a
+# kernel reads its particles in) and the re-reads of node blocks at tile
+# borders are overhead, not counted.
 #
-# plain FP64:
-#   g2p  fused USL G2P/F/stress: reads x3 v3 F9 sigma6 eps_p epsdot_p
-#        damage_init damage mass vol0 (27 doubles = 216 B), writes x v F sigma
-#        eps_p epsdot_p damage_init damage (25 doubles = 200 B), reads v_update
-#        and v of the grid (6 doubles per node -> 6 B)                 = 422 B
+# plain FP64 (34 doubles of state):
+#   g2p  fused USL G2P + F + stress: reads x3 v3 F9 sigma6 strain_el6 eps_p
+#        epsdot_p damage_init damage mass vol0 (33 doubles) and pid, mat
+#        (272 B); writes the same plus vol and the next cell key (284 B);
+#        reads v_update and v of the grid (6 doubles per node -> 6 B)  = 562 B
 #   p2g  reads x3 v3 mass vol sigma6 (14 doubles = 112 B), writes node mass,
 #        momentum and force (7 doubles per node -> 7 B)                 = 119 B
 #   grid reads node mass, momentum, force (7 doubles), writes mass, v,
 #        v_update (7 doubles): 14 doubles per node -> 14 B               = 14 B
 #   sort reads the cell key and particle id, writes the permutation
 #        (3 x 4 B)                                                      = 12 B
-# mixed (x and F in FP64, the rest FP32, int64 fixed-point node sums):
-#   g2p  reads x 24, F 72, v 12, sigma 24, 4 scalars 16, mass + vol0 8
-#        (156 B), writes x v F sigma and the 4 scalars (148 B), reads
-#        v_update and v (6 floats per node -> 3 B)                      = 307 B
+# mixed (x and F in FP64, the other 22 fields FP32, int64 fixed-point node
+# sums):
+#   g2p  reads x 24, F 72, v 12, sigma 24, strain_el 24, 4 scalars 16, mass
+#        + vol0 8, pid + mat 8 (188 B); writes those, vol and the key
+#        (196 B); reads v_update and v (6 floats per node -> 3 B)      = 387 B
 #   p2g  reads x 24, v 12, mass 4, vol 4, sigma 24 (68 B), writes 7 int64
 #        node sums (7 B)                                                = 75 B
 #   grid reads 7 int64, writes 7 floats per node                        = 10.5 B
 #   sort as plain                                                       = 12 B
+# mixed + a learned constitutive update (the surrogate's networks or the
+# cut network): the G2P kernel leaves the stress state to the constitutive
+# kernel and stages its inputs (D, pressure, wave speed: 8 floats)
+#   g2p  reads x 24, F 72, v 12, mass, vol0, damage, pid, mat 20 (128 B);
+#        writes x, F, v, mass, vol0, vol, key, pid, mat (132 B) and the
+#        staged 32 B; grid 3 B                                          = 295 B
+#   learned  reads sigma, strain_el, eps_p, epsdot_p, damage_init, damage
+#        (64 B) and the staged 32 B, writes the 64 B                   = 160 B
 ALG_BYTES_PER_PARTICLE = {
-    "plain-fp64": {"g2p": 422, "p2g": 119, "grid": 14, "sort": 12},
-    "mixed": {"g2p": 307, "p2g": 75, "grid": 10.5, "sort": 12},
+    "plain-fp64": {"g2p": 562, "p2g": 119, "grid": 14, "sort": 12},
+    "mixed": {"g2p": 387, "p2g": 75, "grid": 10.5, "sort": 12},
+    "mixed+cutnet": {"g2p": 295, "p2g": 75, "grid": 10.5, "sort": 12, "learned": 160},
 }
-ALG_BYTES_PER_PARTICLE["mixed+surrogate"] = ALG_BYTES_PER_PARTICLE["mixed"]
-# mixed+cutnet: the G2P kernel leaves the constitutive update to the cut
-# network's kernel and stages its inputs (D, pressure, wave speed: 8 floats
-# = 32 B per particle); the cut kernel reads sigma, strain, eps_p, epsdot_p,
-# damage_init, damage (64 B) and the staged 32 B, writes the same 64 B.
-ALG_BYTES_PER_PARTICLE["mixed+cutnet"] = {"g2p": 307 - 128 + 32, "p2g": 75, "grid": 10.5,
-                                          "sort": 12, "cut": 160}
+ALG_BYTES_PER_PARTICLE["mixed+surrogate"] = ALG_BYTES_PER_PARTICLE["mixed+cutnet"]
 # Matrix-core work of one cut network evaluation, algorithmic
 # (2 (18 x 64 + 4 x 64 x 64 + 64 x 9) flops), and per training sample
 # (forward, the input-gradient products of layers 2..6, the weight
@@ -91,10 +100,12 @@ CUTNET_FLOPS_PER_TRAIN_SAMPLE = CUTNET_FLOPS_PER_EVAL + 2 * (4 * 64 * 64 + 64 * 
 MFMA_FLOPS_PER_EVAL = 2 * 32 * 32
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 KERNEL_NAMES = {"g2p": "k_g2p_usl", "p2g": "k_p2g", "grid": "k_grid_update", "sort": "sort",
-                "cut": "k_stress_cutnet"}
-# Whole step, same counting (state once in, once out, grid once).
-STEP_ALG_BYTES_PER_PARTICLE = {"plain-fp64": 442, "mixed": 324, "mixed+surrogate": 324,
-                               "mixed+cutnet": 324}
+                "learned": "k_stress_learned"}
+# Whole step, same counting: the state read once and written once, node
+# data once per pass (FP64: 272 + 284 B of state, 7 + 14 + 6 doubles of node
+# traffic per node; mixed: 188 + 196 B, 7 + 7 int64 + 7 + 6 floats per node).
+STEP_ALG_BYTES_PER_PARTICLE = {"plain-fp64": 583, "mixed": 405, "mixed+surrogate": 405,
+                               "mixed+cutnet": 405}
 DTYPE = {"plain-fp64": "f64", "mixed": "f32+f64(x,F)+i64(grid sums)",
          "mixed+surrogate": "f32+f64(x,F)+i64(grid sums)+bf16(mfma, f32 acc)",
          "mixed+cutnet": "f32+f64(x,F)+i64(grid sums)+bf16(mfma, f32 acc)"}
@@ -501,7 +512,7 @@ def main():
             "autotune": rp["autotune"],
         }
         if path == "mixed+cutnet":
-            cut_ms, cut_n = rp["phases"]["cut"]
+            cut_ms, cut_n = rp["phases"]["learned"]
             ev = rp["surrogate_evals"]
             if ev < 0.5 * args.steps * npart:
                 raise RuntimeError("mixed+cutnet: %d network evaluations in %d timed steps of "
@@ -521,8 +532,8 @@ def main():
                     "note": "algorithmic network flops over the cut kernel's launch time"},
                 "training": cut_training}
         if path == "mixed+surrogate":
-            # surrogate evaluations happen inside the fused G2P launches
-            g2p_ms, g2p_n = rp["phases"]["g2p"]
+            # surrogate evaluations happen in the constitutive kernel after G2P
+            g2p_ms, g2p_n = rp["phases"]["learned"]
             ev = rp["surrogate_evals"]
             # a surrogate number is only a surrogate number if the networks ran
             if ev < 0.5 * args.steps * npart:
@@ -536,8 +547,9 @@ def main():
                 "mfma_roofline": {
                     "bound": "mfma", "flops_per_eval": MFMA_FLOPS_PER_EVAL, "achieved": mf,
                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": mf / MFMA_BF16_PEAK_TFLOPS,
-                    "note": "network flops over the time of the fused G2P launches"},
+                    "frac": mf / MFMA_BF16_PEAK_TFLOPS, "kernel": "k_stress_learned",
+                    "avg_launch_ms": g2p_ms / max(g2p_n, 1),
+                    "note": "network flops over the constitutive kernel's launch time"},
                 "training": training}
     accuracy = None
     if "plain-fp64" in states and head != "plain-fp64":

@@ -323,7 +323,8 @@ enum {
   KMPM_PHASE_GRID_B = 6,    /* MUSL velocities_to_grid (node velocities) */
   KMPM_PHASE_STRESS_B = 7,  /* MUSL rate of deformation + F + stress */
   KMPM_PHASE_DT = 8,        /* update_time + adjust_dt */
-  KMPM_PHASE_CUT = 9,       /* USL with the cut network: its constitutive update */
+  KMPM_PHASE_LEARNED = 9,   /* USL with the surrogate or the cut network: the
+                               constitutive kernel after G2P */
   KMPM_NPHASE = 10
 };
 /* enable != 0 starts recording (and clears totals); 0 stops. */
@@ -374,6 +375,37 @@ int kmpm_autotune(kmpm_ctx *ctx, int trial_steps, int *variant,
  * np*S, np*S, np*S*3. */
 int kmpm_debug_weights(kmpm_ctx *ctx, int64_t *node_tag, double *w, double *wd,
                        int64_t np);
+
+/* ---- on-disk formats ---------------------------------------------------------
+ * LAMMPS text dumps, as the reference's `dump(id, all, particle|grid, N,
+ * name, vars...)` commands write them: the '*' of the name becomes the
+ * step (dump_particle.cpp:54-68; a name without '*' becomes
+ * <name><step><name>, as the reference's unsigned test makes it), header
+ * "ITEM: TIMESTEP 0 ...", one line per particle / node "id type tag vars...",
+ * values in the stream's default %g format.  vars: names from
+ * dump_particle.h:32-43 / dump_grid.h:32-37 (an unknown name is
+ * KMPM_ERR_ARG, the reference's "output variable ... is unknown").
+ * Quantities this library does not carry print as the reference prints
+ * them without them: T, gamma, ienergy and rigid 0; the particle body force
+ * (set only by contact fixes) 0.  TLMPM particle stresses are rotated by R
+ * (dump_particle.cpp:100-103). */
+int kmpm_dump_particles(kmpm_ctx *ctx, const char *filename, const char *const *vars, int nvars);
+int kmpm_dump_grid(kmpm_ctx *ctx, const char *filename, const char *const *vars, int nvars);
+/* Restart file: the reference's update block (Update::write_restart,
+ * update.cpp:220-280) and one solid block per solid (Solid::write_restart,
+ * solid.cpp:2841-2889: Eigen matrices column-major, TLMPM vol0 PK1
+ * recomputed from F and sigma), framed by the magic "KMPMRST1", the
+ * dimension, the box and the solid count.  The regions, materials, groups
+ * and fixes the reference's file also holds stay with the caller (the
+ * input script re-creates them).  kmpm_read_restart checks the method,
+ * scheme, shape and FLIP blend against the context, uploads the particles
+ * (vol = J vol0, mass = rho0 vol0, as Solid::read_restart,
+ * solid.cpp:2931-2936) and restores dt, time and step. */
+int kmpm_write_restart(kmpm_ctx *ctx, const char *filename);
+int kmpm_read_restart(kmpm_ctx *ctx, const char *filename);
+/* Set dt, the elapsed time and the step count (what Update::read_restart
+ * restores, update.cpp:362-385). */
+int kmpm_set_time(kmpm_ctx *ctx, double dt, double time, int64_t step);
 
 /* ---- learned constitutive surrogate ----------------------------------------
  * Replaces, for a johnson_cook (+ damage_johnson_cook) material, the two

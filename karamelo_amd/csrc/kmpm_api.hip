@@ -847,9 +847,9 @@ int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
   tbegin(c, KMPM_PHASE_G2P);
   HIPCHK(c->a.ref_order ? launch_ref_g2p_usl(c) : launch_g2p_usl(c));
   tend(c);
-  if (c->a.cut_on) {
-    tbegin(c, KMPM_PHASE_CUT);
-    HIPCHK(launch_cutnet_stress(c));
+  if (learned_constitutive(c)) {
+    tbegin(c, KMPM_PHASE_LEARNED);
+    HIPCHK(launch_learned_stress(c));
     tend(c);
   }
   return after_g2p_state_swap(c);
@@ -921,7 +921,8 @@ int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
 // ---- batched instances ------------------------------------------------------
 int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
   if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
-  if (ninst > 0 && (c->a.tl || c->a.ref_order)) return KMPM_ERR_ARG;
+  if (ninst > 0 && (c->a.tl || c->a.ref_order || c->a.surr_on || c->a.cut_on))
+    return KMPM_ERR_ARG;
   for (int q = 0; q < c->a.nfv; q++)
     if (ninst > 0 && c->a.fv[q].expr) return KMPM_ERR_ARG;   // values of the context's time
   const TileGeom &g = c->a.tg;
@@ -1319,6 +1320,21 @@ int kmpm_timing_read(kmpm_ctx *c, double ms[KMPM_NPHASE], int64_t launches[KMPM_
   return KMPM_OK;
 }
 
+// The staging buffer of a learned constitutive update ([8][cap] floats: the
+// rate of deformation, the EOS pressure and the wave speed of each particle).
+static int ensure_learned_stage(kmpm_ctx *c) {
+  if (c->d_cut_stage && c->cut_stage_cap >= c->a.cap) return KMPM_OK;
+  if (c->d_cut_stage) (void)hipFree(c->d_cut_stage);
+  c->d_cut_stage = nullptr;
+  c->cut_stage_cap = 0;
+  c->a.cut_stage = nullptr;
+  const int cap = c->a.cap > 0 ? c->a.cap : 1;
+  HIPCHK(dalloc(&c->d_cut_stage, (size_t)8 * cap));
+  c->cut_stage_cap = cap;
+  c->a.cut_stage = c->d_cut_stage;
+  return KMPM_OK;
+}
+
 int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
   if (!c) return KMPM_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1328,12 +1344,17 @@ int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
     return KMPM_OK;
   }
   // one network pair serves the whole wave, so every material must be the
-  // surrogate's JC + shock material
-  if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK) return KMPM_ERR_ARG;
+  // surrogate's JC + shock material; one instance (the constitutive kernel
+  // reduces the context's CFL)
+  if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
+      c->a.ninst)
+    return KMPM_ERR_ARG;
   for (int i = 0; i < c->nmat; i++)
     if (!surrogate_matches(s, c->mats_host[i])) return KMPM_ERR_ARG;
   SurrogateDev img;
   int st = surrogate_device_image(s, &img);
+  if (st) return st;
+  st = ensure_learned_stage(c);
   if (st) return st;
   if (!c->d_surr) HIPCHK(dalloc(&c->d_surr, 1));
   HIPCHK(hipMemcpy(c->d_surr, &img, sizeof(img), hipMemcpyHostToDevice));
@@ -1357,17 +1378,11 @@ int kmpm_ctx_set_cutnet(kmpm_ctx *c, const kmpm_cutnet *s) {
       !cutnet_matches(s, c->mats_host[0]))
     return KMPM_ERR_ARG;
   if (!c->d_cut) HIPCHK(dalloc(&c->d_cut, 1));
-  if (c->cut_stage_cap < c->a.cap || !c->d_cut_stage) {
-    if (c->d_cut_stage) (void)hipFree(c->d_cut_stage);
-    c->d_cut_stage = nullptr;
-    c->cut_stage_cap = 0;
-    HIPCHK(dalloc(&c->d_cut_stage, (size_t)8 * (c->a.cap > 0 ? c->a.cap : 1)));
-    c->cut_stage_cap = c->a.cap > 0 ? c->a.cap : 1;
-  }
-  const int st = cutnet_device_image(s, c->d_cut, c->stream);
+  int st = ensure_learned_stage(c);
+  if (st) return st;
+  st = cutnet_device_image(s, c->d_cut, c->stream);
   if (st) return st;
   c->a.cut = c->d_cut;
-  c->a.cut_stage = c->d_cut_stage;
   c->a.cut_on = 1;
   return KMPM_OK;
 }

@@ -161,6 +161,14 @@ struct kmpm_ctx {
 };
 
 namespace kmpm {
+// The USL step's constitutive update is a learned one (the surrogate's
+// networks or the cut network, JC + shock class on the mixed ULMPM path),
+// run by k_stress_learned after the G2P kernel staged its inputs.
+static inline bool learned_constitutive(const kmpm_ctx *c) {
+  return (c->a.cut_on || c->a.surr_on) && c->a.mat_class == MC_JC_SHOCK &&
+         c->a.prec == KMPM_PRECISION_MIXED && !c->a.tl;
+}
+
 // launchers (kmpm_kernels.hip); all stream-ordered, return hipError_t
 hipError_t launch_keys(kmpm_ctx *c);
 hipError_t launch_sort(kmpm_ctx *c);
@@ -169,7 +177,7 @@ hipError_t launch_grid_update(kmpm_ctx *c, int remap);
 hipError_t launch_g2p_usl(kmpm_ctx *c);
 hipError_t launch_g2p_musl_a(kmpm_ctx *c);
 hipError_t launch_stress_musl_b(kmpm_ctx *c);
-hipError_t launch_cutnet_stress(kmpm_ctx *c);   // kmpm_cutnet.hip
+hipError_t launch_learned_stress(kmpm_ctx *c);   // kmpm_cutnet.hip
 hipError_t launch_adjust_dt(kmpm_ctx *c);
 hipError_t launch_grid_clean(kmpm_ctx *c);
 hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);

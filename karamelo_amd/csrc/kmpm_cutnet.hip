@@ -528,27 +528,37 @@ __global__ void k_cn_reduce(float *p, const float *partial, int nblocks, float *
   p[q] -= lr * (m * c1) / (sqrtf(v * c2) + eps);
 }
 
-// ---- the step's constitutive update ---------------------------------------------
+// ---- the step's learned constitutive update ------------------------------------
 //
 // Particles in sorted order j, 64 per wave; the state is read from the
-// current slot order[j] and written at j (the G2P kernel's permutation).
-// Lanes inside the envelope take the network; the others the plain FP64
-// functions.  One wave in judge_stride (rotating with the step) also runs
-// the plain strength for its network lanes and sums the squared deviatoric
-// error.
-__global__ __launch_bounds__(256, 2) void k_stress_cutnet(KArgs a) {
+// current slot order[j] and written at j (the G2P kernel's permutation),
+// with the rate of deformation, pressure and wave speed the G2P kernel
+// staged.  MODE 1: the JC material functions from the two small networks
+// (kmpm_surrogate.h); MODE 2: the cut network.  Lanes inside the envelope
+// take the networks, the others the plain FP64 functions.  MODE 2: one wave
+// in judge_stride (rotating with the step) also runs the plain strength for
+// its network lanes and sums the squared deviatoric error.
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void k_stress_learned(KArgs a) {
   constexpr int PREC = KMPM_PRECISION_MIXED;
   if (frozen(a)) return;   // the G2P kernel copied the whole state
-  __shared__ CutNetDev s_net;
-  __shared__ __attribute__((aligned(16))) uint16_t stage[4][64 * CN_KE];
-  cn_copy_to_lds(a.cut, &s_net);
+  constexpr size_t NETB = MODE == 2 ? sizeof(CutNetDev) : sizeof(SurrogateDev);
+  __shared__ __attribute__((aligned(16))) unsigned char s_net[NETB];
+  __shared__ __attribute__((aligned(16))) uint16_t stage[MODE == 2 ? 4 : 1][64 * CN_KE];
+  {
+    static_assert(NETB % 4 == 0, "4-byte copies");
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(
+        MODE == 2 ? (const void *)a.cut : (const void *)a.surr);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(s_net);
+    for (int q = threadIdx.x; q < (int)(NETB / 4); q += blockDim.x) dst[q] = src[q];
+  }
   __syncthreads();
-  const CutNetDev &N = s_net;
+  const CutNetDev &N = *reinterpret_cast<const CutNetDev *>(s_net);
+  const SurrogateDev &SD = *reinterpret_cast<const SurrogateDev *>(s_net);
   const int wave = threadIdx.x >> 6;
   const DevMat &m = a.mats[0];
   const double dt = a.sc->dt;
   const float dtf = (float)dt;
-  const float A = N.A, G = N.G, epsdot0 = N.epsdot0;
   const float tav = (float)(1000 * a.h / m.signal_velocity);          // solid.cpp:1324-1329
   const int64_t step = a.sc->step;
   const float *st = a.cut_stage;
@@ -571,90 +581,96 @@ __global__ __launch_bounds__(256, 2) void k_stress_cutnet(KArgs a) {
     float ep = (float)LDA(EP, slot), epdot = (float)LDA(EPDOT, slot);
     float dinit = (float)LDA(DINIT, slot), dmg = (float)LDA(DMG, slot);
     const float pH = st[6 * cs + j], cwave = st[7 * cs + j];
-    // the trial deviator (strength_jc.cpp:146-157) and the envelope
-    float t[6];
-    const float q = cn_trial<float>(G, dtf, dmg, sig, Dr, t);
-    float f[CN_NI];
-    cn_features<float>(A, G, epsdot0, dtf, t, q, Dr, ep, epdot, dmg, dinit, pH, f);
-    float dD[6];
-    for (int e = 0; e < 6; e++) dD[e] = f[6 + e];
-    const float inc = (float)SQRT_3_OVER_2 * norm_sym_t(dD);
-    const bool inside = ep <= N.ep_max && epdot <= N.ratio_max * epsdot0 && dmg < N.dmg_max &&
-                        dinit <= N.dinit_max && fabsf(pH) <= N.p_max * A && q <= N.q_max * A &&
-                        inc <= N.inc_max;
-    const bool use_net = live && inside;
-    const bool fb = live && !inside;
-    float y[CN_NO];
-    if (__ballot(use_net)) cn_eval_wave(N, f, y, stage[wave]);
-    float sdev[6], dps = 0.0f;
-    bool plastic = false;
-    if (use_net) {
-      plastic = y[6] > 0.0f;
-      for (int e = 0; e < 6; e++) sdev[e] = plastic ? A * y[e] : t[e];
-      dps = plastic ? A * y[6] / (3.0f * G) : 0.0f;
+    if constexpr (MODE == 1) {
+      update_stress_jc_surrogate_p(m, SD, live, dt, a.h, pH, Dr, sig, eel, ep, epdot, dinit, dmg,
+                                   cnt);
     } else {
-      double sig_c[6], D_c[6], out_c[6], dps_c = 0.0;
-      for (int e = 0; e < 6; e++) {
-        sig_c[e] = sig[e];
-        D_c[e] = Dr[e];
-      }
-      if (fb) jc_strength_fallback(m, sig_c, D_c, dt, ep, epdot, dmg, out_c, &dps_c);
-      for (int e = 0; e < 6; e++) sdev[e] = (float)out_c[e];
-      dps = (float)dps_c;
-    }
-    // the judge: the plain FP64 strength on one wave in judge_stride
-    const int64_t wid = base / 64 + wave;
-    if (N.judge_stride > 0 && (wid + step) % N.judge_stride == 0 && use_net) {
-      double sig_c[6], D_c[6], out_c[6], dps_c = 0.0;
-      for (int e = 0; e < 6; e++) {
-        sig_c[e] = sig[e];
-        D_c[e] = Dr[e];
-      }
-      jc_strength_fallback(m, sig_c, D_c, dt, ep, epdot, dmg, out_c, &dps_c);
-      double diff[6];
-      for (int e = 0; e < 6; e++) diff[e] = (double)sdev[e] - out_c[e];
-      const double nd = norm_sym(diff), nr = norm_sym(out_c);
-      je2 += nd * nd;
-      jr2 += nr * nr;
-      jn += 1;
-    }
-    ep += dps;                                                          // solid.cpp:1319
-    epdot -= __fdividef(epdot * dtf, tav);                              // solid.cpp:1324-1329
-    epdot += __fdividef(dps, tav);
-    epdot = epdot > 0.0f ? epdot : 0.0f;
-    // damage (damage_jc.cpp:91-143): the network's increment, the reset and
-    // the damage closed form applied exactly
-    if (m.damage == KMPM_DAMAGE_JOHNSON_COOK) {
+      const float A = N.A, G = N.G, epsdot0 = N.epsdot0;
+      // the trial deviator (strength_jc.cpp:146-157) and the envelope
+      float t[6];
+      const float q = cn_trial<float>(G, dtf, dmg, sig, Dr, t);
+      float f[CN_NI];
+      cn_features<float>(A, G, epsdot0, dtf, t, q, Dr, ep, epdot, dmg, dinit, pH, f);
+      float dD[6];
+      for (int e = 0; e < 6; e++) dD[e] = f[6 + e];
+      const float inc = (float)SQRT_3_OVER_2 * norm_sym_t(dD);
+      const bool inside = ep <= N.ep_max && epdot <= N.ratio_max * epsdot0 && dmg < N.dmg_max &&
+                          dinit <= N.dinit_max && fabsf(pH) <= N.p_max * A &&
+                          q <= N.q_max * A && inc <= N.inc_max;
+      const bool use_net = live && inside;
+      const bool fb = live && !inside;
+      float y[CN_NO];
+      if (__ballot(use_net)) cn_eval_wave(N, f, y, stage[wave]);
+      float sdev[6], dps = 0.0f;
+      bool plastic = false;
       if (use_net) {
-        if (plastic) dinit += fmaxf(A * y[7] / (3.0f * G), 0.0f);
-        const float vm = (float)SQRT_3_OVER_2 * norm_sym_t(sdev);
-        float triax = 0.0f;
-        if (pH != 0.0f && vm != 0.0f) triax = __fdividef(-pH, vm + 0.001f * fabsf(pH));
-        if (triax <= -3.0f) dinit = 0.0f;
-        if (dinit >= 1.0f) dmg = fminf(fmaxf(y[8], dmg), 1.0f);
-      } else if (fb) {
-        double di_c = dinit, dm_c = dmg, sdev_c[6];
-        for (int e = 0; e < 6; e++) sdev_c[e] = sdev[e];
-        jc_damage_fallback(m, di_c, dm_c, pH, sdev_c, epdot, dps);
-        dinit = (float)di_c;
-        dmg = (float)dm_c;
+        plastic = y[6] > 0.0f;
+        for (int e = 0; e < 6; e++) sdev[e] = plastic ? A * y[e] : t[e];
+        dps = plastic ? A * y[6] / (3.0f * G) : 0.0f;
+      } else {
+        double sig_c[6], D_c[6], out_c[6], dps_c = 0.0;
+        for (int e = 0; e < 6; e++) {
+          sig_c[e] = sig[e];
+          D_c[e] = Dr[e];
+        }
+        if (fb) jc_strength_fallback(m, sig_c, D_c, dt, ep, epdot, dmg, out_c, &dps_c);
+        for (int e = 0; e < 6; e++) sdev[e] = (float)out_c[e];
+        dps = (float)dps_c;
       }
+      // the judge: the plain FP64 strength on one wave in judge_stride
+      const int64_t wid = base / 64 + wave;
+      if (N.judge_stride > 0 && (wid + step) % N.judge_stride == 0 && use_net) {
+        double sig_c[6], D_c[6], out_c[6], dps_c = 0.0;
+        for (int e = 0; e < 6; e++) {
+          sig_c[e] = sig[e];
+          D_c[e] = Dr[e];
+        }
+        jc_strength_fallback(m, sig_c, D_c, dt, ep, epdot, dmg, out_c, &dps_c);
+        double diff[6];
+        for (int e = 0; e < 6; e++) diff[e] = (double)sdev[e] - out_c[e];
+        const double nd = norm_sym(diff), nr = norm_sym(out_c);
+        je2 += nd * nd;
+        jr2 += nr * nr;
+        jn += 1;
+      }
+      ep += dps;                                                        // solid.cpp:1319
+      epdot -= __fdividef(epdot * dtf, tav);                            // solid.cpp:1324-1329
+      epdot += __fdividef(dps, tav);
+      epdot = epdot > 0.0f ? epdot : 0.0f;
+      // damage (damage_jc.cpp:91-143): the network's increment, the reset and
+      // the damage closed form applied exactly
+      if (m.damage == KMPM_DAMAGE_JOHNSON_COOK) {
+        if (use_net) {
+          if (plastic) dinit += fmaxf(A * y[7] / (3.0f * G), 0.0f);
+          const float vm = (float)SQRT_3_OVER_2 * norm_sym_t(sdev);
+          float triax = 0.0f;
+          if (pH != 0.0f && vm != 0.0f) triax = __fdividef(-pH, vm + 0.001f * fabsf(pH));
+          if (triax <= -3.0f) dinit = 0.0f;
+          if (dinit >= 1.0f) dmg = fminf(fmaxf(y[8], dmg), 1.0f);
+        } else if (fb) {
+          double di_c = dinit, dm_c = dmg, sdev_c[6];
+          for (int e = 0; e < 6; e++) sdev_c[e] = sdev[e];
+          jc_damage_fallback(m, di_c, dm_c, pH, sdev_c, epdot, dps);
+          dinit = (float)di_c;
+          dmg = (float)dm_c;
+        }
+      }
+      surrogate_count(cnt, use_net, fb);
+      // assembly (solid.cpp:1353-1366)
+      const float trD = Dr[0] + Dr[1] + Dr[2];
+      const float trE = eel[0] + eel[1] + eel[2];
+      if (dmg == 0.0f || pH >= 0.0f) {
+        for (int e = 0; e < 3; e++) sig[e] = -pH * 1.0f + sdev[e];
+      } else {
+        const float pd = -pH * (1.0f - dmg);
+        for (int e = 0; e < 3; e++) sig[e] = pd * 1.0f + sdev[e];
+      }
+      for (int e = 3; e < 6; e++) sig[e] = sdev[e];
+      const float e3 = (dtf * trD + trE) / 3.0f;
+      const float Ge = dmg > 1e-10f ? (float)m.G * (1.0f - dmg) : (float)m.G;
+      for (int e = 0; e < 3; e++) eel[e] = e3 * 1.0f + __fdividef(sdev[e], Ge);
+      for (int e = 3; e < 6; e++) eel[e] = __fdividef(sdev[e], Ge);
     }
-    surrogate_count(cnt, use_net, fb);
-    // assembly (solid.cpp:1353-1366)
-    const float trD = Dr[0] + Dr[1] + Dr[2];
-    const float trE = eel[0] + eel[1] + eel[2];
-    if (dmg == 0.0f || pH >= 0.0f) {
-      for (int e = 0; e < 3; e++) sig[e] = -pH * 1.0f + sdev[e];
-    } else {
-      const float pd = -pH * (1.0f - dmg);
-      for (int e = 0; e < 3; e++) sig[e] = pd * 1.0f + sdev[e];
-    }
-    for (int e = 3; e < 6; e++) sig[e] = sdev[e];
-    const float e3 = (dtf * trD + trE) / 3.0f;
-    const float Ge = dmg > 1e-10f ? (float)m.G * (1.0f - dmg) : (float)m.G;
-    for (int e = 0; e < 3; e++) eel[e] = e3 * 1.0f + __fdividef(sdev[e], Ge);
-    for (int e = 3; e < 6; e++) eel[e] = __fdividef(sdev[e], Ge);
     if (!live) continue;
     // the wave speed of particles that are not fully damaged (solid.cpp:1376-1385)
     if (dmg < 1.0f) {
@@ -672,18 +688,21 @@ __global__ __launch_bounds__(256, 2) void k_stress_cutnet(KArgs a) {
   }
   finish_cfl<false>(a, cmax, 1.0);
   surrogate_count_flush(cnt, a.sc);
-  if (jn) {
+  if (MODE == 2 && jn) {
     atomicAdd(&a.sc->judge_n, jn);
     atomicAdd(&a.sc->judge_err2, je2);
     atomicAdd(&a.sc->judge_ref2, jr2);
   }
 }
 
-hipError_t launch_cutnet_stress(kmpm_ctx *c) {
+hipError_t launch_learned_stress(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
   int g = (c->a.np + 255) / 256;
   if (g > c->cu_count * 4) g = c->cu_count * 4;
-  hipLaunchKernelGGL(k_stress_cutnet, dim3(g), dim3(256), 0, c->stream, c->a);
+  if (c->a.cut_on)
+    hipLaunchKernelGGL(k_stress_learned<2>, dim3(g), dim3(256), 0, c->stream, c->a);
+  else
+    hipLaunchKernelGGL(k_stress_learned<1>, dim3(g), dim3(256), 0, c->stream, c->a);
   return hipGetLastError();
 }
 

@@ -1099,14 +1099,10 @@ __device__ inline void stress_update(const DevMat &m, double dt, double h, doubl
 // D = sym(L) (solid.cpp:1239), stress update (solid.cpp:1246-1374) and the
 // particle's CFL wave speed (:1376-1385); returns -1 for fully damaged
 // particles (skipped by the CFL maximum).
-// SURR: the JC + shock update with the learned material functions; every
-// lane of the wave calls it (live == false for lanes without a particle).
-template <int MC, int PREC, bool SURR = false, class R = typename Real<PREC>::T>
+template <int MC, int PREC, class R = typename Real<PREC>::T>
 __device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
                                           R *sig, R *eel, R &ep, R &epdot, R &dinit, R &dmg,
-                                          int mat, const double *v, int pid, bool live = true,
-                                          const SurrogateDev *sd = nullptr,
-                                          SurrCount *cnt = nullptr) {
+                                          int mat, const double *v, int pid) {
   const DevMat &m = a.mats[mat];
   const double rho = m.rho0 / J;
   double D[6];
@@ -1116,15 +1112,11 @@ __device__ inline double stress_and_speed(const KArgs &a, double dt, const doubl
   D[3] = 0.5 * (L[1] + L[3]);
   D[4] = 0.5 * (L[2] + L[6]);
   D[5] = 0.5 * (L[5] + L[7]);
-  if constexpr (SURR)
-    update_stress_jc_surrogate(m, *sd, live, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg,
-                               *cnt);
-  else
-    stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   if (dmg >= 1.0) return -1.0;
   const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
   const double c = sqrt(m.cfl_k43g / rho) + vm;
-  if (isnan(c) && live) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
+  if (isnan(c)) set_error(a.sc, KMPM_ERR_WAVE_SPEED, pid);
   return c;
 }
 
@@ -1138,21 +1130,19 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // ---------------------------------------------------------------------------
 // USL fused G2P: L(v_update), G2P, advance, F, stress, CFL, next key
 // ---------------------------------------------------------------------------
-// SURR: 0 the plain constitutive update, 1 the learned JC material
-// functions (kmpm_surrogate.h), 2 the cut network: the constitutive update
-// is left to k_stress_cutnet, and this kernel stages its inputs (the rate of
-// deformation, the EOS pressure and the wave speed) in a.cut_stage.
-template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, int SURR>
+// STAGE: a learned constitutive update (the surrogate's networks or the cut
+// network) follows in k_stress_learned; this kernel leaves the stress state
+// alone and stages its inputs (the rate of deformation, the EOS pressure and
+// the wave speed) in a.cut_stage.
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool STAGE>
 // Mixed linear and JC classes fit four waves per SIMD without spilling (the
-// fluid and generic classes would spill at that bound).  The surrogate
-// kernel spills some 40 registers at four waves and is still faster there
-// than at three or five (0.61 against 0.62 and 0.84 ms on the 4.27M bar);
-// the plain mixed kernel takes 0.42 ms at four waves and 0.57 at five.
-__global__ __launch_bounds__(256, SURR ? 4
-                                       : (PREC == KMPM_PRECISION_MIXED &&
-                                                  (MC == MC_LINEAR || MC == MC_JC_SHOCK)
-                                              ? 4
-                                              : 1)) void k_g2p_usl(KArgs a) {
+// fluid and generic classes would spill at that bound); the plain mixed
+// kernel takes 0.42 ms at four waves and 0.57 at five on the 4.27M bar.
+__global__ __launch_bounds__(256, STAGE ? 4
+                                        : (PREC == KMPM_PRECISION_MIXED &&
+                                                   (MC == MC_LINEAR || MC == MC_JC_SHOCK)
+                                               ? 4
+                                               : 1)) void k_g2p_usl(KArgs a) {
   if (frozen(a)) {
     frozen_copy(a);
     return;
@@ -1161,22 +1151,12 @@ __global__ __launch_bounds__(256, SURR ? 4
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   __shared__ R nb[2 * DIM * BN];
-  // SURR: the networks and envelope, staged in LDS once per block (read at
-  // each use instead of being held in registers across the particle loop)
-  __shared__ __attribute__((aligned(16))) unsigned char sbuf[SURR == 1 ? sizeof(SurrogateDev) : 16];
-  const SurrogateDev *sd = reinterpret_cast<const SurrogateDev *>(sbuf);
-  if constexpr (SURR == 1) {
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(a.surr);
-    uint32_t *dst = reinterpret_cast<uint32_t *>(sbuf);
-    for (int q = threadIdx.x; q < (int)(sizeof(SurrogateDev) / 4); q += blockDim.x) dst[q] = src[q];
-  }
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
   const double dt_ctx = a.sc->dt;
   const double alpha = a.flip;
   double cmax = 0, rmin = 1.0;
-  SurrCount scnt = {0, 0};
   for (int li = blockIdx.x; li < ntile; li += gridDim.x) {
     const int t = a.tile_list[li];
     const double dt = tile_dt(a, t, dt_ctx);
@@ -1187,10 +1167,7 @@ __global__ __launch_bounds__(256, SURR ? 4
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
-    // live == false: a lane without a particle that still takes part in the
-    // wave-wide surrogate evaluation (it computes on particle p1 - 1 and
-    // stores nothing)
-    auto particle = [&](const int j, const bool live) {
+    auto particle = [&](const int j) {
       const int slot = a.order[j];
       double x[3], v[3];
       for (int d = 0; d < 3; d++) {
@@ -1218,12 +1195,12 @@ __global__ __launch_bounds__(256, SURR ? 4
       const int pid = IFLD(a.ia, PID, slot);
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
       for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
-      if (live && outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+      if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
       for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
       // State is written in sorted order as soon as each part is final, so
       // it does not stay live across the constitutive update.
       const int mat = IFLD(a.ia, MATI, slot);
-      if (live) {
+      {
         for (int d = 0; d < 3; d++) {
           STB(PX + d, j, x[d]);
           STB(VX + d, j, v[d]);
@@ -1233,7 +1210,7 @@ __global__ __launch_bounds__(256, SURR ? 4
         IFLD(a.ib, MATI, j) = mat;
       }
       const double vol0 = LDA(VOL0, slot);
-      if (live) {
+      {
         STB(MASS, j, LDA(MASS, slot));
         STB(VOL0, j, vol0);
       }
@@ -1242,13 +1219,12 @@ __global__ __launch_bounds__(256, SURR ? 4
         double F[9];
         for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot);
         J = update_F(dt, L, F);
-        if (live)
-          for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
+        for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
       }
-      if (live) STB(VOL, j, J * vol0);
+      STB(VOL, j, J * vol0);
       R dmg = (R)LDA(DMG, slot);
-      if (live && J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
-      if constexpr (SURR == 2) {
+      if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
+      if constexpr (STAGE) {
         // the cut's inputs: D = sym(L) (solid.cpp:1239), the EOS pressure
         // (eos_shock.cpp:99-133, FP64 from J as on the plain mixed path) and
         // the wave speed before the damage test (solid.cpp:1376-1385)
@@ -1278,10 +1254,8 @@ __global__ __launch_bounds__(256, SURR ? 4
       }
       R ep = (R)LDA(EP, slot), epdot = (R)LDA(EPDOT, slot);
       R dinit = (R)LDA(DINIT, slot);
-      const double c = stress_and_speed<MC, PREC, SURR == 1>(a, dt, L, J, sig, eel, ep, epdot,
-                                                             dinit, dmg, mat, v, pid, live, sd,
-                                                             &scnt);
-      if (!live) return;
+      const double c = stress_and_speed<MC, PREC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
+                                                  mat, v, pid);
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
         STB(SXX + e, j, sig[e]);
@@ -1292,18 +1266,10 @@ __global__ __launch_bounds__(256, SURR ? 4
       STB(DINIT, j, dinit);
       STB(DMG, j, dmg);
     };
-    if constexpr (SURR == 1) {
-      for (int jb = p0; jb < p1; jb += blockDim.x) {
-        const int j = jb + (int)threadIdx.x;
-        particle(j < p1 ? j : p1 - 1, j < p1);
-      }
-    } else {
-      for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) particle(j, true);
-    }
+    for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) particle(j);
     if (a.ninst) flush_tile_cfl<false>(a, t, cmax, rmin);
   }
   finish_cfl<false>(a, cmax, rmin);
-  if constexpr (SURR == 1) surrogate_count_flush(scnt, a.sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1686,6 +1652,7 @@ template <> struct Tiles<2, 2, 1> { static constexpr int TX = 8, TY = 16, TZ = 1
 template <> struct Tiles<2, 2, 2> { static constexpr int TX = 16, TY = 16, TZ = 1, CH = 64; };
 
 static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
 
 static inline int persistent_grid(kmpm_ctx *c, int per_cu) {
   int64_t g = (int64_t)c->cu_count * per_cu;

@@ -64,20 +64,21 @@ __device__ inline void surrogate_count_flush(const SurrCount &c, DevScalars *sc)
 }
 
 // Solid::update_stress eos-strength branch (solid.cpp:1292-1374) for the
-// JC + shock class on the mixed path.  Every lane of the wave calls it;
-// `live` is false for lanes without a particle (their results are
-// discarded).  The pressure is formed in FP64 from J (a small compression
-// cancels in rho / rho0 - 1); the rest in FP32, the arithmetic of the mixed
-// state, except the plain-function fallbacks, which stay FP64.
-__device__ inline void update_stress_jc_surrogate(const DevMat &m, const SurrogateDev &sd,
-                                                  bool live, double dt, double h, double J,
-                                                  double rho, const double *D, float *sig,
-                                                  float *eel, float &ep, float &epdot,
-                                                  float &dinit, float &dmg, SurrCount &cnt) {
-  const float pH = (float)shock_pressure<double>(m, J, rho, dmg, D, h);
+// JC + shock class on the mixed path, from the EOS pressure pH (formed in
+// FP64 from J by the G2P kernel, since a small compression cancels in
+// rho / rho0 - 1, then rounded to FP32) and the rate of deformation Df
+// (FP32); `dt` is the step's.  Every lane of the wave calls it; `live` is
+// false for lanes without a particle (their results are discarded).  The
+// arithmetic is FP32, that of the mixed state, except the plain-function
+// fallbacks, which stay FP64.
+__device__ inline void update_stress_jc_surrogate_p(const DevMat &m, const SurrogateDev &sd,
+                                                    bool live, double dt, double h, float pH,
+                                                    const float *Df, float *sig, float *eel,
+                                                    float &ep, float &epdot, float &dinit,
+                                                    float &dmg, SurrCount &cnt) {
   const float dtf = (float)dt;
-  float Df[6];
-  for (int e = 0; e < 6; e++) Df[e] = (float)D[e];
+  double D[6];
+  for (int e = 0; e < 6; e++) D[e] = Df[e];
   float epf = ep, epdf = epdot, dif = dinit, dmf = dmg;
   // ---- strength (strength_jc.cpp:100-184) -------------------------------
   const float G_ = (float)m.strength_p[0], A = (float)m.strength_p[1],

@@ -72,19 +72,16 @@ hipError_t TileLaunch<DIM, S1, V>::g2p_usl(kmpm_ctx *c) {
     }));
     return hipGetLastError();
   }
-  const bool jc_mixed = c->a.mat_class == MC_JC_SHOCK && c->a.prec == KMPM_PRECISION_MIXED;
-  const bool cut = c->a.cut_on && jc_mixed;
-  const bool surr = !cut && c->a.surr_on && jc_mixed;
+  // a learned constitutive update (the surrogate's networks or the cut
+  // network) runs in its own kernel after this one, which stages its inputs
+  const bool learned = learned_constitutive(c);
   MC_DISPATCH(c, PREC_DISPATCH(c, {
     constexpr bool CAN_SURR = MC == MC_JC_SHOCK && PREC == KMPM_PRECISION_MIXED;
-    if (CAN_SURR && cut)
-      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, CAN_SURR ? 2 : 0>),
-                         dim3(g), dim3(256), 0, c->stream, c->a);
-    else if (CAN_SURR && surr)
-      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, CAN_SURR ? 1 : 0>),
+    if (CAN_SURR && learned)
+      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, CAN_SURR>),
                          dim3(g), dim3(256), 0, c->stream, c->a);
     else
-      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, 0>), dim3(g),
+      hipLaunchKernelGGL((k_g2p_usl<DIM, S1, TT::TX, TT::TY, TT::TZ, MC, PREC, false>), dim3(g),
                          dim3(256), 0, c->stream, c->a);
   }));
   return hipGetLastError();

@@ -28,7 +28,8 @@ EOS_NONE, EOS_LINEAR, EOS_SHOCK, EOS_FLUID = 0, 1, 2, 3
 STRENGTH_NONE, STRENGTH_LINEAR, STRENGTH_JOHNSON_COOK, STRENGTH_FLUID = 0, 1, 2, 3
 DAMAGE_NONE, DAMAGE_JOHNSON_COOK = 0, 1
 REGION_BLOCK, REGION_CYLINDER = 0, 1
-PHASES = ["sort", "p2g", "grid", "g2p", "g2p_a", "p2g_b", "grid_b", "stress_b", "dt", "cut"]
+PHASES = ["sort", "p2g", "grid", "g2p", "g2p_a", "p2g_b", "grid_b", "stress_b", "dt",
+          "learned"]
 NPHASE = len(PHASES)
 
 # style keys of the reference's plugin registries -> enum values
@@ -212,6 +213,11 @@ def _declare(lib):
         "kmpm_surrogate_envelope": (C.c_int, [vp, _DP, _DP]),
         "kmpm_ctx_set_surrogate": (C.c_int, [vp, vp]),
         "kmpm_surrogate_stats": (C.c_int, [vp, P(C.c_int64), P(C.c_int64)]),
+        "kmpm_dump_particles": (C.c_int, [vp, C.c_char_p, P(C.c_char_p), C.c_int]),
+        "kmpm_dump_grid": (C.c_int, [vp, C.c_char_p, P(C.c_char_p), C.c_int]),
+        "kmpm_write_restart": (C.c_int, [vp, C.c_char_p]),
+        "kmpm_read_restart": (C.c_int, [vp, C.c_char_p]),
+        "kmpm_set_time": (C.c_int, [vp, C.c_double, C.c_double, C.c_int64]),
         "kmpm_cutnet_create": (C.c_int, [P(Material), P(CutNetConfig), P(vp)]),
         "kmpm_cutnet_destroy": (C.c_int, [vp]),
         "kmpm_cutnet_train": (C.c_int, [vp, C.c_int64, _DP, _DP]),
@@ -453,6 +459,24 @@ class Context:
     def set_surrogate(self, surrogate):
         check(self.L.kmpm_ctx_set_surrogate(self.h, surrogate.h if surrogate else None),
               "set_surrogate")
+
+    def dump_particles(self, filename, variables):
+        v = (C.c_char_p * len(variables))(*[x.encode() for x in variables])
+        check(self.L.kmpm_dump_particles(self.h, filename.encode(), v, len(variables)),
+              "dump_particles")
+
+    def dump_grid(self, filename, variables):
+        v = (C.c_char_p * len(variables))(*[x.encode() for x in variables])
+        check(self.L.kmpm_dump_grid(self.h, filename.encode(), v, len(variables)), "dump_grid")
+
+    def write_restart(self, filename):
+        check(self.L.kmpm_write_restart(self.h, filename.encode()), "write_restart")
+
+    def read_restart(self, filename):
+        check(self.L.kmpm_read_restart(self.h, filename.encode()), "read_restart")
+
+    def set_time(self, dt, time, step):
+        check(self.L.kmpm_set_time(self.h, dt, time, step), "set_time")
 
     def set_cutnet(self, net):
         check(self.L.kmpm_ctx_set_cutnet(self.h, net.h if net else None), "set_cutnet")
