@@ -10,7 +10,8 @@
  * reference's Method class and the per-particle constitutive plugin calls:
  *   Method virtuals ............ src/method.h:33-48
  *   ULMPM implementation ....... src/ulmpm.cpp:88-563
- *   Scheme loops ............... src/usl.cpp:43-93, src/musl.cpp:43-94
+ *   Scheme loops ............... src/usl.cpp:43-93, src/musl.cpp:43-94,
+ *                                src/usf.cpp:34-94
  *   constitutive plugins ....... src/eos.h:40, src/strength.h:38-45,
  *                                src/damage.h:35-41
  *   plugin registry keys ....... src/material.cpp:40-66, src/style_*.h
@@ -52,8 +53,26 @@ enum {
 /* ---- options ----------------------------------------------------------- */
 /* method(ulmpm|tlmpm, ...) -> update.cpp:108-196, style_method.h */
 enum { KMPM_METHOD_ULMPM = 0, KMPM_METHOD_TLMPM = 1 };
-/* scheme(usl|musl) -> update.cpp:83-101, style_scheme.h */
-enum { KMPM_SCHEME_USL = 0, KMPM_SCHEME_MUSL = 1 };
+/* scheme(usl|musl|usf) -> update.cpp:83-101, style_scheme.h (usf.cpp:34-94) */
+enum { KMPM_SCHEME_USL = 0, KMPM_SCHEME_MUSL = 1, KMPM_SCHEME_USF = 2 };
+/* velocity-update sub-method, the second word of method(...) (update.cpp:
+ * 129-146; values as Update::SubMethodType, update.h:62-69).
+ *   PIC, FLIP  blend v by pic_flip (PIC is pic_flip 0)
+ *   APIC       affine P2G momentum v + C (x_I - x_p) with C = L (ULMPM) or
+ *              Fdot (TLMPM), rate of deformation from the affine moments
+ *              D^-1 sum v_I (x_I - x_p)^T w; pic_flip forced to 0
+ *              (ulmpm.cpp:79-85, tlmpm.cpp:82-84, solid.cpp:392-426,1006-1153)
+ *   AFLIP      APIC transfers with FLIP blending (ULMPM)
+ *   ASFLIP     AFLIP with x advanced by the blended v (solid.cpp:637-685,
+ *              791-796; ULMPM)
+ *   MLS        APIC with the MLS internal force -vol w sigma D^-1 (x_I - x_p)
+ *              (solid.cpp:524-574; ULMPM)
+ * TLMPM implements PIC/FLIP and APIC only; the other values run as FLIP
+ * there, as in the reference.  The APIC family needs cubic splines on
+ * ULMPM (solid.cpp:1448-1452) and, with linear shape functions on TLMPM,
+ * np_per_cell 1 or 2 (solid.cpp:1454-1462). */
+enum { KMPM_SUB_PIC = 0, KMPM_SUB_FLIP = 1, KMPM_SUB_APIC = 2, KMPM_SUB_AFLIP = 3,
+       KMPM_SUB_ASFLIP = 4, KMPM_SUB_MLS = 5 };
 /* shape functions -> basis_functions.h:21-104 */
 enum { KMPM_SHAPE_LINEAR = 0, KMPM_SHAPE_CUBIC_SPLINE = 1 };
 /* arithmetic of the device step: FP64 throughout, or mixed (FP32 state,
@@ -82,13 +101,17 @@ typedef struct kmpm_config {
   double pic_flip;        /* FLIP blending alpha (update.cpp:163-166) */
   double dt;              /* dt of the first step (1e-16 unless set_dt, update.cpp:38) */
   int    dt_constant;     /* nonzero after set_dt (update.cpp:63-70) */
-  int    reserved1;
+  int    sub_method;      /* KMPM_SUB_* (PIC and FLIP both blend by pic_flip) */
   double dt_factor;       /* dt_factor (default 0.9, update.cpp:40) */
   /* TLMPM: limits of the solid's region (Region::limits, see
    * kmpm_region_limits), the extent of the solid's own grid
    * (Grid::init(solidlo, solidhi), grid.cpp:91-97, solid.cpp:1833-1838,1927-1930).
    * Unused by ULMPM, whose grid covers the box. */
   double solidlo[3], solidhi[3];
+  /* particles per cell per axis of the solid (solid.cpp:2027), read only by
+   * the APIC inertia of linear shape functions (solid.cpp:1454-1462) */
+  int    np_per_cell;
+  int    reserved2;
 } kmpm_config;
 
 /* ---- materials: material(...) + eos/strength/damage styles ------------- */
@@ -267,11 +290,25 @@ int kmpm_grid_to_points(kmpm_ctx *ctx);
 /* MUSL: velocities_to_grid + post_velocities_to_grid (ulmpm.cpp:476-496). */
 int kmpm_velocities_to_grid(kmpm_ctx *ctx);
 /* MUSL: compute_rate_deformation_gradient(true) + update_deformation_gradient
- * + update_stress (musl.cpp:75-77). */
+ * + update_stress (musl.cpp:75-77); USF: the same from the first P2G's grid
+ * velocity (usf.cpp:58-60). */
 int kmpm_update_stress(kmpm_ctx *ctx);
+/* USF: particles_to_grid_USF_1 (node mass and velocity, ulmpm.cpp:380-404)
+ * + the post_update_grid_state fixes that follow it (usf.cpp:53-55). */
+int kmpm_particles_to_grid_usf_1(kmpm_ctx *ctx);
+/* USF: particles_to_grid_USF_2 (the forces of the updated stress,
+ * ulmpm.cpp:406-431) + post_particles_to_grid (usf.cpp:62-64); then
+ * kmpm_update_grid_state and kmpm_grid_to_points complete the step. */
+int kmpm_particles_to_grid_usf_2(kmpm_ctx *ctx);
+/* The affine matrix of the APIC transfers (L on ULMPM, Fdot on TLMPM; the
+ * state compute_velocity_nodes_APIC reads, solid.cpp:392-426), row-major,
+ * 9 per particle in upload order.  KMPM_ERR_ARG for contexts without the
+ * APIC transfers.  Waits for the stream. */
+int kmpm_download_affine(kmpm_ctx *ctx, double *C, int64_t np);
 /* update_time + adjust_dt (update.cpp:201-206, ulmpm.cpp:525-551). */
 int kmpm_adjust_dt(kmpm_ctx *ctx);
-/* nsteps complete steps of the configured scheme (USL::run / MUSL::run),
+/* nsteps complete steps of the configured scheme (USL::run / MUSL::run /
+ * USF::run),
  * stream-ordered, no host synchronisation inside. */
 int kmpm_step(kmpm_ctx *ctx, int64_t nsteps);
 /* RunTime::command (run_time.cpp:31-49) with the scheme loop's end test

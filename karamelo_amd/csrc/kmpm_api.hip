@@ -118,9 +118,9 @@ static void tend(kmpm_ctx *c) {
 }
 
 // Mixed layout <-> the FP64 field layout used on the host.
-static void split_mixed(const std::vector<double> &hd, int cap, bool tl,
+static void split_mixed(const std::vector<double> &hd, int cap, int lay,
                         std::vector<double> &dd, std::vector<float> &ff) {
-  for (int f = 0; f < (tl ? NDF_TL : NDF); f++)
+  for (int f = 0; f < state_nd(false, lay); f++)
     for (int i = 0; i < cap; i++) {
       const double v = hd[(size_t)f * cap + i];
       if (mx_is_double(f)) dd[(size_t)mx_dslot(f) * cap + i] = v;
@@ -128,8 +128,8 @@ static void split_mixed(const std::vector<double> &hd, int cap, bool tl,
     }
 }
 static void join_mixed(const std::vector<double> &dd, const std::vector<float> &ff, int cap,
-                       bool tl, std::vector<double> &hd) {
-  for (int f = 0; f < (tl ? NDF_TL : NDF); f++)
+                       int lay, std::vector<double> &hd) {
+  for (int f = 0; f < state_nd(false, lay); f++)
     for (int i = 0; i < cap; i++)
       hd[(size_t)f * cap + i] = mx_is_double(f) ? dd[(size_t)mx_dslot(f) * cap + i]
                                                 : (double)ff[(size_t)mx_fslot(f) * cap + i];
@@ -142,7 +142,7 @@ static void join_mixed(const std::vector<double> &dd, const std::vector<float> &
 //   momentum  m_max * v_ref,  v_ref = 4 max(|v|_max, max wave speed of the materials)
 //   force     vol_ref * sigma_ref / h,   vol_ref = 4 vol_max,
 //             sigma_ref = 4 max(|sigma|_max, max (K + 4/3 G))
-// (|grad w| <= 1/h for both shape families).
+// (|grad w| <= 1/h for both shape families; 8/h for the MLS force).
 // Power-of-two scales of the fixed-point node sums: one particle's
 // contribution stays below 2^FX_CONTRIB_BITS.  The P2G kernels scale in FP32,
 // so a scale must lie inside the float range (false otherwise).
@@ -163,7 +163,9 @@ static bool fixed_point_scales(const kmpm_ctx *c, const std::vector<double> &hd,
   }
   const double vref = 4 * std::max(vmax, cmax);
   const double sref = 4 * std::max(smax, modmax);
-  const double bound[FX_N] = {mmax, mmax * vref, 4 * volmax * sref * c->a.inv_h};
+  // the MLS force's factor Di (x_I - x_p) reaches 3/h^2 * 2h = 6/h
+  const double fmls = c->a.xf == 2 ? 8.0 : 1.0;
+  const double bound[FX_N] = {mmax, mmax * vref, 4 * volmax * sref * c->a.inv_h * fmls};
   for (int k = 0; k < FX_N; k++) {
     int e = 0;
     std::frexp(bound[k] > 0 && std::isfinite(bound[k]) ? bound[k] : 1.0, &e);   // bound < 2^e
@@ -253,6 +255,9 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   TileGeom g;
   tile_shape(a.dim, a.S1, variant, g.T);
   if (variant < 0 || variant >= KMPM_NTILE_VARIANTS || g.T[0] == 0) return KMPM_ERR_ARG;
+  // the APIC transfers and the USF step are compiled for the first tile
+  // shape of each dimension and shape function only
+  if (variant != 0 && (a.xf || a.scheme == KMPM_SCHEME_USF)) return KMPM_ERR_ARG;
   g.ntiles = 1;
   g.cpt = 1;
   g.bn = 1;
@@ -362,8 +367,25 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   if (!cfg || !out || nmat < 1 || nmat > KMPM_MAX_MATERIALS || !mats) return KMPM_ERR_ARG;
   *out = nullptr;
   if (cfg->method != KMPM_METHOD_ULMPM && cfg->method != KMPM_METHOD_TLMPM) return KMPM_ERR_ARG;
-  if (cfg->scheme != KMPM_SCHEME_USL && cfg->scheme != KMPM_SCHEME_MUSL) return KMPM_ERR_ARG;
+  if (cfg->scheme != KMPM_SCHEME_USL && cfg->scheme != KMPM_SCHEME_MUSL &&
+      cfg->scheme != KMPM_SCHEME_USF)
+    return KMPM_ERR_ARG;
   if (cfg->shape != KMPM_SHAPE_LINEAR && cfg->shape != KMPM_SHAPE_CUBIC_SPLINE) return KMPM_ERR_ARG;
+  if (cfg->sub_method < KMPM_SUB_PIC || cfg->sub_method > KMPM_SUB_MLS) return KMPM_ERR_ARG;
+  // velocity-update sub-method: ULMPM::setup (ulmpm.cpp:79-85), TLMPM::setup
+  // (tlmpm.cpp:82-84); the inertia tensor of Solid::compute_inertia_tensor
+  // (solid.cpp:1440-1470) needs cubic splines on ULMPM and 1 or 2 particles
+  // per cell with linear shape functions on TLMPM
+  const bool tl_cfg = cfg->method == KMPM_METHOD_TLMPM;
+  const int sub = cfg->sub_method;
+  const bool apic = tl_cfg ? sub == KMPM_SUB_APIC : sub >= KMPM_SUB_APIC;
+  if (apic && cfg->shape == KMPM_SHAPE_LINEAR &&
+      (!tl_cfg || (cfg->np_per_cell != 1 && cfg->np_per_cell != 2)))
+    return KMPM_ERR_ARG;
+  // the reference-order kernels implement the PIC/FLIP transfers of USL
+  // and MUSL
+  if (cfg->order == KMPM_ORDER_REFERENCE && (apic || cfg->scheme == KMPM_SCHEME_USF))
+    return KMPM_ERR_ARG;
   if (cfg->precision != KMPM_PRECISION_FP64 && cfg->precision != KMPM_PRECISION_MIXED)
     return KMPM_ERR_ARG;
   if (cfg->dimension != 2 && cfg->dimension != 3) return KMPM_ERR_ARG;
@@ -414,6 +436,16 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   a.h = cfg->cellsize;
   a.inv_h = 1.0 / cfg->cellsize;  // ulmpm.cpp:122
   a.flip = cfg->pic_flip;
+  if (sub == KMPM_SUB_APIC || (!a.tl && sub == KMPM_SUB_MLS)) a.flip = 0.0;   // ulmpm.cpp:81
+  a.xf = !apic ? 0 : (!a.tl && sub == KMPM_SUB_MLS) ? 2 : 1;
+  a.asflip = !a.tl && sub == KMPM_SUB_ASFLIP;
+  a.lay = apic ? LAY_AP : a.tl ? LAY_TL : LAY_UL;
+  if (apic) {
+    const double inv2 = 1.0 / (cfg->cellsize * cfg->cellsize);
+    a.di = cfg->shape == KMPM_SHAPE_CUBIC_SPLINE ? 3.0 * inv2
+           : cfg->np_per_cell == 1               ? 16.0 / 4.0 * inv2
+                                                 : 16.0 / 3.0 * inv2;
+  }
   a.dt_factor = cfg->dt_factor;
   a.dt_constant = cfg->dt_constant;
   a.nmat = nmat;
@@ -508,11 +540,13 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
       if (p->solid[i] != p->solid[0]) return KMPM_ERR_ARG;   // one solid: one TL grid
   const int cap = (int)std::max<int64_t>(np, 1);
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
-  const int ndf = state_nd(mixed, a.tl);
+  const int ndf = state_nd(mixed, a.lay);
 
   // Host images of the new state.  Everything that can reject the upload is
   // checked here, while the context still holds its previous particles.
-  std::vector<double> hd((size_t)(a.tl ? NDF_TL : NDF) * cap, 0.0);
+  // The APIC layout's affine matrices start at zero, as the reference's
+  // L and Fdot do after populate or a restart.
+  std::vector<double> hd((size_t)state_nd(false, a.lay) * cap, 0.0);
   std::vector<int> hi((size_t)NIF * cap, 0);
   auto D = [&](int f, int64_t i) -> double & { return hd[(size_t)f * cap + i]; };
   for (int64_t i = 0; i < np; i++) {
@@ -563,8 +597,8 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   HIPCHK(dalloc(&c->d_pa, (size_t)ndf * cap));
   HIPCHK(dalloc(&c->d_pb, (size_t)ndf * cap));
   if (mixed) {
-    HIPCHK(dalloc(&c->d_fa, (size_t)state_nf(true, a.tl) * cap));
-    HIPCHK(dalloc(&c->d_fb, (size_t)state_nf(true, a.tl) * cap));
+    HIPCHK(dalloc(&c->d_fa, (size_t)state_nf(true, a.lay) * cap));
+    HIPCHK(dalloc(&c->d_fb, (size_t)state_nf(true, a.lay) * cap));
   }
   HIPCHK(dalloc(&c->d_xo, (size_t)3 * cap));
   HIPCHK(dalloc(&c->d_ia, (size_t)NIF * cap));
@@ -616,9 +650,9 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
     return st;
   };
   if (mixed) {
-    std::vector<double> dd((size_t)state_nd(true, a.tl) * cap);
-    std::vector<float> ff((size_t)state_nf(true, a.tl) * cap);
-    split_mixed(hd, cap, a.tl, dd, ff);
+    std::vector<double> dd((size_t)state_nd(true, a.lay) * cap);
+    std::vector<float> ff((size_t)state_nf(true, a.lay) * cap);
+    split_mixed(hd, cap, a.lay, dd, ff);
     if (hipMemcpy(c->d_pa, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(c->d_fa, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice) !=
@@ -647,19 +681,18 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
   return KMPM_OK;
 }
 
-int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
-  if (!c || !p) return KMPM_ERR_ARG;
-  const int64_t np = c->a.np;
-  if (p->np < np) return KMPM_ERR_CAPACITY;
+// The current state in the FP64 field layout (hd) and the int fields (hi),
+// in storage order; waits for the stream.
+static int fetch_state(kmpm_ctx *c, std::vector<double> &hd, std::vector<int> &hi) {
   const int cap = c->a.cap;
-  std::vector<double> hd((size_t)(c->a.tl ? NDF_TL : NDF) * cap);
-  std::vector<int> hi((size_t)NIF * cap);
+  hd.assign((size_t)state_nd(false, c->a.lay) * cap, 0.0);
+  hi.assign((size_t)NIF * cap, 0);
   std::vector<double> dd;
   std::vector<float> ff;
   const bool mixed = c->a.prec == KMPM_PRECISION_MIXED;
   if (mixed) {
-    dd.resize((size_t)state_nd(true, c->a.tl) * cap);
-    ff.resize((size_t)state_nf(true, c->a.tl) * cap);
+    dd.resize((size_t)state_nd(true, c->a.lay) * cap);
+    ff.resize((size_t)state_nf(true, c->a.lay) * cap);
     HIPCHK(hipMemcpyAsync(dd.data(), c->d_pa, dd.size() * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     HIPCHK(hipMemcpyAsync(ff.data(), c->d_fa, ff.size() * sizeof(float), hipMemcpyDeviceToHost,
@@ -671,7 +704,35 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
   HIPCHK(hipMemcpyAsync(hi.data(), c->d_ia, hi.size() * sizeof(int), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (mixed) join_mixed(dd, ff, cap, c->a.tl, hd);
+  if (mixed) join_mixed(dd, ff, cap, c->a.lay, hd);
+  return KMPM_OK;
+}
+
+int kmpm_download_affine(kmpm_ctx *c, double *C, int64_t np) {
+  if (!c || !C || c->a.lay != LAY_AP || c->upload_failed) return KMPM_ERR_ARG;
+  if (np < c->a.np) return KMPM_ERR_CAPACITY;
+  std::vector<double> hd;
+  std::vector<int> hi;
+  const int st = fetch_state(c, hd, hi);
+  if (st) return st;
+  const int cap = c->a.cap;
+  for (int64_t i = 0; i < c->a.np; i++) {
+    const int64_t q = hi[(size_t)PID * cap + i];
+    if (q < 0 || q >= c->a.np) return KMPM_ERR_HIP;
+    for (int e = 0; e < 9; e++) C[9 * q + e] = hd[(size_t)(C00 + e) * cap + i];
+  }
+  return KMPM_OK;
+}
+
+int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
+  if (!c || !p) return KMPM_ERR_ARG;
+  const int64_t np = c->a.np;
+  if (p->np < np) return KMPM_ERR_CAPACITY;
+  const int cap = c->a.cap;
+  std::vector<double> hd;
+  std::vector<int> hi;
+  const int st = fetch_state(c, hd, hi);
+  if (st) return st;
   auto D = [&](int f, int64_t i) { return hd[(size_t)f * cap + i]; };
   for (int64_t i = 0; i < np; i++) {
     const int64_t q = hi[(size_t)PID * cap + i];  // output index = upload index
@@ -823,7 +884,37 @@ int kmpm_update_grid_state(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_GRID);
-  HIPCHK(c->a.ref_order ? launch_ref_grid(c, 0) : launch_grid_update(c, 0));
+  // USF: the first pass's mass and velocity with particles_to_grid_USF_2's forces
+  const int mode = c->cfg.scheme == KMPM_SCHEME_USF ? GM_USF2 : GM_STEP;
+  HIPCHK(c->a.ref_order ? launch_ref_grid(c, 0) : launch_grid_update(c, mode));
+  tend(c);
+  return KMPM_OK;
+}
+
+// USF (usf.cpp:53-55): particles_to_grid_USF_1 (node mass and velocity,
+// ulmpm.cpp:380-404) and the fixes' post_update_grid_state.  The P2G
+// kernel also sums the forces of the current stress, which this pass leaves
+// unused.
+int kmpm_particles_to_grid_usf_1(kmpm_ctx *c) {
+  if (!c || !c->sorted || c->cfg.scheme != KMPM_SCHEME_USF) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_P2G);
+  HIPCHK(launch_p2g(c, 0));
+  tend(c);
+  tbegin(c, KMPM_PHASE_GRID_B);
+  HIPCHK(launch_grid_update(c, GM_USF1));
+  tend(c);
+  return KMPM_OK;
+}
+
+// USF (usf.cpp:62-64): particles_to_grid_USF_2, the forces of the updated
+// stress (ulmpm.cpp:406-431), and post_particles_to_grid's body forces
+// (applied by the grid update, fix_body_force.cpp:117-146).
+int kmpm_particles_to_grid_usf_2(kmpm_ctx *c) {
+  if (!c || !c->sorted || c->cfg.scheme != KMPM_SCHEME_USF) return KMPM_ERR_ARG;
+  if (c->upload_failed) return KMPM_ERR_ARG;
+  tbegin(c, KMPM_PHASE_P2G_B);
+  HIPCHK(launch_p2g(c, 0));
   tend(c);
   return KMPM_OK;
 }
@@ -861,6 +952,8 @@ int kmpm_grid_to_points(kmpm_ctx *c) {
   tbegin(c, KMPM_PHASE_G2P_A);
   HIPCHK(c->a.ref_order ? launch_ref_g2p_musl_a(c) : launch_g2p_musl_a(c));
   tend(c);
+  // USF: the G2P carries the (already updated) state into the next buffer
+  if (c->cfg.scheme == KMPM_SCHEME_USF) return after_g2p_state_swap(c);
   return KMPM_OK;
 }
 
@@ -882,6 +975,8 @@ int kmpm_update_stress(kmpm_ctx *c) {
   tbegin(c, KMPM_PHASE_STRESS_B);
   HIPCHK(c->a.ref_order ? launch_ref_stress_musl_b(c) : launch_stress_musl_b(c));
   tend(c);
+  // USF updates the current state in place, before its forces and G2P
+  if (c->cfg.scheme == KMPM_SCHEME_USF) return KMPM_OK;
   return after_g2p_state_swap(c);
 }
 
@@ -899,6 +994,15 @@ int kmpm_step(kmpm_ctx *c, int64_t nsteps) {
   for (int64_t k = 0; k < nsteps; k++) {
     int s = kmpm_compute_grid_weight_functions_and_gradients(c);
     if (s) return s;
+    if (c->cfg.scheme == KMPM_SCHEME_USF) {   // USF::run usf.cpp:48-83
+      if ((s = kmpm_particles_to_grid_usf_1(c))) return s;
+      if ((s = kmpm_update_stress(c))) return s;
+      if ((s = kmpm_particles_to_grid_usf_2(c))) return s;
+      if ((s = kmpm_update_grid_state(c))) return s;
+      if ((s = kmpm_grid_to_points(c))) return s;
+      if ((s = kmpm_adjust_dt(c))) return s;
+      continue;
+    }
     if ((s = kmpm_particles_to_grid(c))) return s;
     if ((s = kmpm_update_grid_state(c))) return s;
     if (c->cfg.scheme == KMPM_SCHEME_USL) {
@@ -921,7 +1025,8 @@ int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
 // ---- batched instances ------------------------------------------------------
 int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
   if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
-  if (ninst > 0 && (c->a.tl || c->a.ref_order || c->a.surr_on || c->a.cut_on))
+  if (ninst > 0 && (c->a.tl || c->a.ref_order || c->a.surr_on || c->a.cut_on || c->a.xf ||
+                    c->a.scheme == KMPM_SCHEME_USF))
     return KMPM_ERR_ARG;
   for (int q = 0; q < c->a.nfv; q++)
     if (ninst > 0 && c->a.fv[q].expr) return KMPM_ERR_ARG;   // values of the context's time
@@ -1002,8 +1107,8 @@ struct SavedState {
 static int save_particles(kmpm_ctx *c, SavedState &s) {
   const KArgs &a = c->a;
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
-  s.nd = (size_t)state_nd(mixed, a.tl) * a.cap;
-  s.nf = (size_t)state_nf(mixed, a.tl) * a.cap;
+  s.nd = (size_t)state_nd(mixed, a.lay) * a.cap;
+  s.nf = (size_t)state_nf(mixed, a.lay) * a.cap;
   s.ni = (size_t)NIF * a.cap;
   HIPCHK(dalloc(&s.pa, s.nd));
   if (s.nf) HIPCHK(dalloc(&s.fa, s.nf));
@@ -1347,7 +1452,7 @@ int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
   // surrogate's JC + shock material; one instance (the constitutive kernel
   // reduces the context's CFL)
   if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
-      c->a.ninst)
+      c->a.ninst || c->a.xf)
     return KMPM_ERR_ARG;
   for (int i = 0; i < c->nmat; i++)
     if (!surrogate_matches(s, c->mats_host[i])) return KMPM_ERR_ARG;
@@ -1374,7 +1479,8 @@ int kmpm_ctx_set_cutnet(kmpm_ctx *c, const kmpm_cutnet *s) {
   // one network serves every particle of the wave: one material, the cut
   // network's; USL on one instance of the mixed ULMPM path
   if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
-      c->a.ref_order || c->a.ninst || c->cfg.scheme != KMPM_SCHEME_USL || c->nmat != 1 ||
+      c->a.ref_order || c->a.ninst || c->a.xf || c->cfg.scheme != KMPM_SCHEME_USL ||
+      c->nmat != 1 ||
       !cutnet_matches(s, c->mats_host[0]))
     return KMPM_ERR_ARG;
   if (!c->d_cut) HIPCHK(dalloc(&c->d_cut, 1));

@@ -36,12 +36,20 @@ struct KArgs {
   TileGeom tg;
   // particles
   int tl;           // TLMPM: weights from x0 on the solid's grid, F += dt Fdot
+  int lay;          // state Layout (LAY_UL, LAY_TL, LAY_AP)
+  // APIC family (kmpm_config.sub_method): xf = 1 APIC transfers (affine
+  // P2G momentum, rate of deformation from the affine moments), 2 APIC with
+  // the MLS force; asflip: x advanced by the blended velocity; di: the
+  // diagonal of the inertia tensor Di (solid.cpp:1440-1470) in the axes of
+  // the dimension
+  int xf, asflip;
+  double di;
   int ref_order;    // KMPM_ORDER_REFERENCE: the reference-order kernels
   const int *solid_of;   // solid index by pid
   int prec;         // KMPM_PRECISION_*
   int np, cap;
-  double *pa;       // current state: state_nd(mixed, tl) * cap doubles
-  float *fa;        // current FP32 state (mixed: state_nf(true, tl) * cap)
+  double *pa;       // current state: state_nd(mixed, lay) * cap doubles
+  float *fa;        // current FP32 state (mixed: state_nf(true, lay) * cap)
   int *ia;          // current int state (NIF * cap)
   double *pb;       // next state
   float *fb;
@@ -169,11 +177,17 @@ static inline bool learned_constitutive(const kmpm_ctx *c) {
          c->a.prec == KMPM_PRECISION_MIXED && !c->a.tl;
 }
 
+// Passes of the grid update kernel: the step's update (GM_STEP), the MUSL
+// re-map of velocities (GM_REMAP), and USF's two passes, node mass and
+// velocity after its first P2G (GM_USF1) and the update with the forces of
+// its second P2G (GM_USF2).
+enum GridMode { GM_STEP = 0, GM_REMAP = 1, GM_USF1 = 2, GM_USF2 = 3 };
+
 // launchers (kmpm_kernels.hip); all stream-ordered, return hipError_t
 hipError_t launch_keys(kmpm_ctx *c);
 hipError_t launch_sort(kmpm_ctx *c);
 hipError_t launch_p2g(kmpm_ctx *c, int momentum_only);
-hipError_t launch_grid_update(kmpm_ctx *c, int remap);
+hipError_t launch_grid_update(kmpm_ctx *c, int mode);   // GridMode (kmpm_kernels.h)
 hipError_t launch_g2p_usl(kmpm_ctx *c);
 hipError_t launch_g2p_musl_a(kmpm_ctx *c);
 hipError_t launch_stress_musl_b(kmpm_ctx *c);

@@ -33,6 +33,14 @@ enum TLField {
   P00, P01, P02, P10, P11, P12, P20, P21, P22,
   NDF_TL
 };
+// APIC family: the affine matrix C of the transfers (L for ULMPM, Fdot for
+// TLMPM, row-major), kept from one step to the next (solid.cpp:392-426).
+// The APIC layout holds every TLMPM field too, so one layout serves both
+// methods.
+enum ApicField {
+  C00 = NDF_TL, C01, C02, C10, C11, C12, C20, C21, C22,
+  NDF_AP
+};
 // Int fields: PID = index into the static (upload-order) arrays, MAT = material.
 enum IField { PID = 0, MATI, NIF };
 
@@ -41,11 +49,13 @@ enum IField { PID = 0, MATI, NIF };
 // other field is FP32 (a float block of NDF - 12 fields in DField order).
 // TLMPM adds the reference position to the double block (x0, slots 12-14;
 // the weights and cell keys are computed from it) and vol0 PK1 to the float
-// block (after the common fields).
+// block (after the common fields); the APIC layout adds C to the float block
+// after those.
 constexpr int MX_ND = 12;
 constexpr int MX_NF = NDF - MX_ND;
 constexpr int MX_ND_TL = MX_ND + 3;
 constexpr int MX_NF_TL = MX_NF + 9;
+constexpr int MX_NF_AP = MX_NF_TL + 9;
 __host__ __device__ constexpr bool mx_is_double(int f) {
   return f <= PZ || (f >= F00 && f <= F22) || (f >= X0X && f <= X0Z);
 }
@@ -56,12 +66,16 @@ __host__ __device__ constexpr int mx_fslot(int f) {
   return f < F00 ? f - VX : f < NDF ? f - SXX + 3 : f - P00 + MX_NF;
 }
 static_assert(mx_fslot(VOL) == MX_NF - 1 && mx_dslot(F22) == MX_ND - 1, "mixed layout");
-// Field counts of the double and float blocks per method and precision.
-__host__ __device__ constexpr int state_nd(bool mixed, bool tl) {
-  return mixed ? (tl ? MX_ND_TL : MX_ND) : (tl ? NDF_TL : NDF);
+static_assert(mx_fslot(C00) == MX_NF_TL && mx_fslot(C22) == MX_NF_AP - 1, "mixed APIC layout");
+// State layouts: ULMPM, TLMPM, and the APIC layout (TLMPM's fields and C).
+enum Layout { LAY_UL = 0, LAY_TL = 1, LAY_AP = 2 };
+// Field counts of the double and float blocks per layout and precision.
+__host__ __device__ constexpr int state_nd(bool mixed, int lay) {
+  return mixed ? (lay != LAY_UL ? MX_ND_TL : MX_ND)
+               : (lay == LAY_UL ? NDF : lay == LAY_TL ? NDF_TL : NDF_AP);
 }
-__host__ __device__ constexpr int state_nf(bool mixed, bool tl) {
-  return mixed ? (tl ? MX_NF_TL : MX_NF) : 0;
+__host__ __device__ constexpr int state_nf(bool mixed, int lay) {
+  return mixed ? (lay == LAY_UL ? MX_NF : lay == LAY_TL ? MX_NF_TL : MX_NF_AP) : 0;
 }
 
 // Real type of the bulk state and of the transfer arithmetic per precision.

@@ -37,6 +37,19 @@ std::string dump_name(const std::string &name, int64_t step) {
   return out;
 }
 
+// Scheme style key (update.cpp:83-101).
+const char *scheme_key(int scheme) {
+  return scheme == KMPM_SCHEME_USL ? "usl" : scheme == KMPM_SCHEME_MUSL ? "musl" : "usf";
+}
+
+// Update::sub_method_type as the restart stores it (update.h:62-69): the
+// configured value for the APIC family; PIC (0) or FLIP (1) otherwise, PIC
+// being FLIP with PIC_FLIP = 0 (update.cpp:135-136).
+int32_t restart_sub_method(const kmpm_config &cf) {
+  if (cf.sub_method >= KMPM_SUB_APIC) return cf.sub_method;
+  return cf.pic_flip == 0.0 ? KMPM_SUB_PIC : KMPM_SUB_FLIP;
+}
+
 // ostream << double with the default flags and precision 6 is %g.
 void put(std::string &s, double v) {
   char b[40];
@@ -335,11 +348,9 @@ int kmpm_write_restart(kmpm_ctx *c, const char *filename) {
   // ---- Update::write_restart (update.cpp:220-280) ----
   wr_str(o, tl ? "tlmpm" : "ulmpm");
   wr(o, (bool)false);                                  // method->temp
-  wr_str(o, cf.scheme == KMPM_SCHEME_USL ? "usl" : "musl");
-  // SubMethodType::FLIP = 1, PIC = 0 (update.h:62-69); PIC is FLIP with
-  // PIC_FLIP = 0 (update.cpp:135-136)
-  wr(o, (int32_t)(cf.pic_flip == 0.0 ? 0 : 1));
-  wr(o, cf.pic_flip);
+  wr_str(o, scheme_key(cf.scheme));
+  wr(o, restart_sub_method(cf));
+  wr(o, c->a.flip);                                    // PIC_FLIP after the method's setup
   wr(o, (int32_t)(cf.shape == KMPM_SHAPE_CUBIC_SPLINE ? 1 : 0));   // ShapeFunctions
   wr(o, (size_t)0);                                    // additional_args
   wr(o, time);
@@ -435,8 +446,8 @@ int kmpm_read_restart(kmpm_ctx *c, const char *filename) {
     return KMPM_ERR_ARG;
   // the file must describe this context's method, scheme and grid
   if (dim != cf.dimension || method != (tl ? "tlmpm" : "ulmpm") ||
-      scheme != (cf.scheme == KMPM_SCHEME_USL ? "usl" : "musl") || temp ||
-      shape != (cf.shape == KMPM_SHAPE_CUBIC_SPLINE ? 1 : 0) || flip != cf.pic_flip)
+      scheme != scheme_key(cf.scheme) || temp || sub != restart_sub_method(cf) ||
+      shape != (cf.shape == KMPM_SHAPE_CUBIC_SPLINE ? 1 : 0) || flip != c->a.flip)
     return KMPM_ERR_ARG;
   double box[6];
   for (double &b : box)

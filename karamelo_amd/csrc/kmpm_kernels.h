@@ -169,8 +169,8 @@ __device__ inline void flush_fix_forces(const KArgs &a, double (*fsum)[3]) {
 
 __device__ inline void frozen_copy(const KArgs &a) {
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
-  const int64_t nd = (int64_t)state_nd(mixed, a.tl) * a.cap;
-  const int64_t nf = (int64_t)state_nf(mixed, a.tl) * a.cap;
+  const int64_t nd = (int64_t)state_nd(mixed, a.lay) * a.cap;
+  const int64_t nf = (int64_t)state_nf(mixed, a.lay) * a.cap;
   const int64_t ni = (int64_t)NIF * a.cap;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -328,19 +328,30 @@ template <int DIM, bool TL> constexpr int p2g_nsig() {
   return TL ? (DIM == 3 ? 9 : 4) : (DIM == 3 ? 6 : 3);
 }
 
-template <int DIM, bool REMAP, bool TL, class R>
+// APIC (XF > 0): the affine term's position xa (x, or x0 on TLMPM; the
+// advanced x in the MUSL re-map, whose weights are those of the step's
+// start) and C, the affine matrix of the previous step (row-major, DIM x DIM).
+template <int DIM, bool REMAP, bool TL, class R, int XF = 0>
 struct P2GIn {
   double x[DIM];
   R v[DIM], m;
   R s[REMAP ? 1 : p2g_nsig<DIM, TL>()];
+  double xa[XF ? DIM : 1];
+  R c[XF ? DIM * DIM : 1];
 };
 
-template <int DIM, bool REMAP, bool TL, int PREC, class R>
-__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R> &in) {
+template <int DIM, bool REMAP, bool TL, int PREC, class R, int XF>
+__device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R, XF> &in) {
+  if constexpr (XF > 0) {
+    const int slot = a.order[j];
+    for (int r = 0; r < DIM; r++)
+      for (int q = 0; q < DIM; q++) in.c[r * DIM + q] = (R)LDA(C00 + 3 * r + q, slot);
+  }
   if (REMAP) {
     for (int d = 0; d < DIM; d++) {
       in.x[d] = TL ? LDB(X0X + d, j) : a.xo[(size_t)d * a.cap + j];
       in.v[d] = (R)LDB(VX + d, j);
+      if constexpr (XF > 0) in.xa[d] = TL ? in.x[d] : LDB(PX + d, j);
     }
     in.m = (R)LDB(MASS, j);
   } else if (TL) {
@@ -365,7 +376,8 @@ __device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R> 
       in.v[d] = (R)LDA(VX + d, slot);
     }
     in.m = (R)LDA(MASS, slot);
-    const R nvol = (R)-LDA(VOL, slot);
+    // MLS (solid.cpp:560-565): -vol sigma Di, Di = di I in the run's axes
+    const R nvol = XF == 2 ? (R)(-LDA(VOL, slot) * a.di) : (R)-LDA(VOL, slot);
     if (DIM == 3) {
       for (int e = 0; e < 6; e++) in.s[e] = nvol * (R)LDA(SXX + e, slot);
     } else {
@@ -374,6 +386,9 @@ __device__ inline void p2g_load(const KArgs &a, int j, P2GIn<DIM, REMAP, TL, R> 
       in.s[2] = nvol * (R)LDA(SXY, slot);
     }
   }
+  if constexpr (XF > 0)
+    if (!REMAP)
+      for (int d = 0; d < DIM; d++) in.xa[d] = in.x[d];
 }
 
 // Stress term S_de of a staged P2G payload: -vol sigma (symmetric, stored
@@ -434,7 +449,14 @@ __device__ inline unsigned long long fx_word_fast(float val, double scale) {
 // FP64: double arithmetic and LDS sums.  Mixed: float arithmetic, register
 // sums flushed into 64-bit fixed-point LDS sums (order-independent, so the
 // scatter is deterministic).
-template <int DIM, int S1, int TX, int TY, int TZ, bool REMAP, int PREC, bool TL>
+//
+// APIC (XF 1, solid.cpp:392-426): the momentum adds m w C (x_I - x_p).  Per
+// axis the stage also holds g = (x_I - x_p) w, so sum_b C_db (x_I - x_p)_b w
+// is a gradient-like contraction: the column product with one axis's w
+// replaced by its g.  MLS (XF 2, solid.cpp:524-574): the force
+// -vol w sigma Di (x_I - x_p) is the gradient form with dw replaced by g and
+// sigma scaled by Di, so the stage holds (w, g) in place of (w, dw).
+template <int DIM, int S1, int TX, int TY, int TZ, bool REMAP, int PREC, bool TL, int XF = 0>
 __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   if (frozen(a)) return;
   using R = typename Real<PREC>::T;
@@ -444,18 +466,23 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   constexpr int GL = DIM == 3 ? S1 * S1 : S1;   // lanes (stencil columns) per group
   constexpr int NGW = 64 / GL;                   // groups per wave
   constexpr int NG = 4 * NGW;                    // groups per block
-  constexpr int CHK = 128;                       // particles staged per chunk
+  constexpr int CHK = XF ? 64 : 128;             // particles staged per chunk
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int NSIG = p2g_nsig<DIM, TL>();
   constexpr int NF = REMAP ? (1 + DIM) : (1 + 2 * DIM);
-  constexpr int NPAY = 1 + DIM + (REMAP ? 0 : NSIG);      // m, m v, stress terms
+  constexpr int PC = 1 + DIM + (REMAP ? 0 : NSIG);         // payload offset of m C
+  constexpr int NPAY = PC + (XF ? DIM * DIM : 0);          // m, m v, stress terms, m C
+  // APIC with the gradient force needs dw and g: g in its own table
+  constexpr bool GT = XF == 1 && !REMAP;
   constexpr int PSB = DIM == 3 ? EZ : 1;   // node block strides (last axis contiguous)
   constexpr int PSA = DIM == 3 ? EY * EZ : EY;
   constexpr int PBN = EX * PSA;
   static_assert(64 % GL == 0, "stencil shape");
 
   __shared__ BT blk[NF * PBN];
-  __shared__ R2 stw[CHK][DIM * S1];     // (w, dw) per axis and stencil offset
+  __shared__ R2 stw[CHK][DIM * S1];     // (w, dw) per axis and stencil offset; (w, g) for
+                                        // the MLS force and the APIC re-map
+  __shared__ R stg[GT ? CHK : 1][GT ? DIM * S1 : 1];   // APIC: g per axis and offset
   __shared__ R spay[CHK][NPAY + 1];     // +1: rows of distinct particles in distinct banks
   __shared__ int scl[CHK];              // block offset of the particle's cell
   __shared__ int seg[CHK + 1];          // segment starts; seg[nseg] = chunk size
@@ -506,7 +533,7 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
       __syncthreads();   // the previous chunk's segments are done
       // ---- stage: one thread per particle ----
       if (tid < n) {
-        P2GIn<DIM, REMAP, TL, R> in;
+        P2GIn<DIM, REMAP, TL, R, XF> in;
         p2g_load<DIM, REMAP, TL, PREC>(a, c0 + tid, in);
         int cl[3] = {0, 0, 0};
 #pragma unroll
@@ -515,8 +542,22 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
           cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
           double w[S1], dw[S1];
           weights_1d<S1>(in.x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w, dw);
+          if constexpr (XF > 0) {
 #pragma unroll
-          for (int k = 0; k < S1; k++) stw[tid][d * S1 + k] = R2{(R)w[k], (R)dw[k]};
+            for (int k = 0; k < S1; k++) {
+              // node coordinate as the weights compute it
+              const double g = ((a.lo[d] + (0 + (c - off) + k) * a.h) - in.xa[d]) * w[k];
+              if constexpr (GT) {
+                stw[tid][d * S1 + k] = R2{(R)w[k], (R)dw[k]};
+                stg[tid][d * S1 + k] = (R)g;
+              } else {
+                stw[tid][d * S1 + k] = R2{(R)w[k], (R)g};
+              }
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < S1; k++) stw[tid][d * S1 + k] = R2{(R)w[k], (R)dw[k]};
+          }
         }
         const int cell = DIM == 3 ? cl[0] * PSA + cl[1] * PSB + cl[2] : cl[0] * PSA + cl[1];
         scl[tid] = cell;
@@ -524,6 +565,8 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
         for (int d = 0; d < DIM; d++) spay[tid][1 + d] = in.m * in.v[d];
         if (!REMAP)
           for (int e = 0; e < NSIG; e++) spay[tid][1 + DIM + e] = in.s[e];
+        if constexpr (XF > 0)
+          for (int e = 0; e < DIM * DIM; e++) spay[tid][PC + e] = in.m * in.c[e];
       }
       __syncthreads();
       // cells are sorted inside the tile: a segment starts where the cell changes
@@ -552,23 +595,41 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
           for (int f = 0; f < NF; f++) acc[k][f] = 0;
         for (int i = i0; i < i1; i++) {
           // column factors: c = product of the non-last-axis weights, ce[e]
-          // the same with axis e differentiated
-          R c, ce[DIM - 1];
+          // the same with axis e differentiated, cg[e] (APIC) with axis e's
+          // w replaced by its g
+          R c, ce[DIM - 1], cg[DIM - 1];
           if constexpr (DIM == 3) {
             const R2 px = stw[i][sa], py = stw[i][S1 + sb];
             c = px.x * py.x;
             ce[0] = px.y * py.x;
             ce[DIM - 2] = px.x * py.y;
+            if constexpr (XF > 0) {
+              cg[0] = (GT ? stg[i][sa] : px.y) * py.x;
+              cg[DIM - 2] = px.x * (GT ? stg[i][S1 + sb] : py.y);
+            }
           } else {
             const R2 px = stw[i][sa];
             c = px.x;
             ce[0] = px.y;
+            if constexpr (XF > 0) cg[0] = GT ? stg[i][sa] : px.y;
           }
           const R *p = spay[i];
           const R mc = p[0] * c;
-          R mvc[DIM], fa[DIM], fb[DIM];
+          R mvc[DIM], fa[DIM], fb[DIM], mvb[DIM];
 #pragma unroll
           for (int d = 0; d < DIM; d++) mvc[d] = p[1 + d] * c;
+          if constexpr (XF > 0) {
+            // m C (x_I - x_p) w: the non-last axes fold into the column's
+            // momentum, the last axis pairs with its g below
+            const R *mC = p + PC;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+              R u = mC[d * DIM] * cg[0];
+              if constexpr (DIM == 3) u = fma(mC[d * DIM + 1], cg[DIM - 2], u);
+              mvc[d] += u;
+              mvb[d] = mC[d * DIM + DIM - 1] * c;
+            }
+          }
           if (!REMAP) {
             const R *sg = p + 1 + DIM;
 #pragma unroll
@@ -585,6 +646,11 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
             acc[k][0] = fma(mc, wz.x, acc[k][0]);
 #pragma unroll
             for (int d = 0; d < DIM; d++) acc[k][1 + d] = fma(mvc[d], wz.x, acc[k][1 + d]);
+            if constexpr (XF > 0) {
+              const R gz = GT ? stg[i][(DIM - 1) * S1 + k] : wz.y;
+#pragma unroll
+              for (int d = 0; d < DIM; d++) acc[k][1 + d] = fma(mvb[d], gz, acc[k][1 + d]);
+            }
             if (!REMAP)
 #pragma unroll
               for (int d = 0; d < DIM; d++)
@@ -649,8 +715,72 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
 // grid: sum partial blocks, v = p/m, v_update, fixes (grid.cpp:448-466,
 // fix_body_force.cpp:117-146, fix_velocity_nodes.cpp:130-271)
 // ---------------------------------------------------------------------------
+// The node fields a grid pass (GridMode) reads from the partial blocks:
+// [f0, f1) of mass, momentum[3], force[3].
+__host__ __device__ constexpr int gm_f0(int mode) {
+  return mode == GM_REMAP ? 1 : mode == GM_USF2 ? 4 : 0;
+}
+__host__ __device__ constexpr int gm_f1(int mode) {
+  return mode == GM_REMAP || mode == GM_USF1 ? 4 : 7;
+}
+
+// One owned node of a grid pass, from its summed node fields `sum` (FP64):
+//   GM_STEP   v = p/m, v_update = v + dt (f + mb)/m (grid.cpp:448-466), then
+//             the velocity fixes' post_update_grid_state
+//   GM_REMAP  MUSL re-map (ulmpm.cpp:476-496): v = p/m with the step's mass,
+//             then post_velocities_to_grid, v <- value
+//             (fix_velocity_nodes.cpp:227-271)
+//   GM_USF1   USF after particles_to_grid_USF_1 (usf.cpp:53-55): m, v = p/m,
+//             then post_update_grid_state's v <- previous value (its
+//             v_update and reaction are replaced by GM_USF2's)
+//   GM_USF2   USF update_grid_state (usf.cpp:63-67): GM_STEP from GM_USF1's
+//             m and v with the forces of particles_to_grid_USF_2
+template <class R>
+__device__ inline void grid_node(const KArgs &a, int mode, int64_t sn, const double *sum,
+                                 uint32_t mask, double dt, R *gmass, R *gv, R *gvu,
+                                 const double (*fxv)[6], double (*fsum)[3]) {
+  if (mode == GM_REMAP) {
+    const double m = gmass[sn];
+    double v[3];
+    for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+    for (int fi = 0; fi < a.nfv; fi++)
+      if (mask & a.fv[fi].bit)
+        for (int d = 0; d < 3; d++)
+          if (a.fv[fi].set[d]) v[d] = fxv[fi][d];
+    for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
+    return;
+  }
+  if (mode == GM_USF1) {
+    const double m = sum[0];
+    double v[3];
+    for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
+    for (int fi = 0; fi < a.nfv; fi++)
+      if (mask & a.fv[fi].bit)
+        for (int d = 0; d < 3; d++)
+          if (a.fv[fi].set[d]) v[d] = fxv[fi][3 + d];
+    gmass[sn] = (R)m;
+    for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
+    return;
+  }
+  const bool usf = mode == GM_USF2;
+  const double m = usf ? (double)gmass[sn] : sum[0];
+  double v[3], vu[3], mb[3] = {0, 0, 0};
+  for (int d = 0; d < 3; d++)
+    v[d] = usf ? (double)gv[d * a.ns + sn] : m > 0 ? sum[1 + d] / m : 0.0;
+  for (int fi = 0; fi < a.nbf; fi++)
+    if (m > 0 && (mask & a.bf[fi].bit))
+      for (int d = 0; d < 3; d++)
+        if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
+  for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
+  apply_velocity_fixes(a, mask, m, 1.0 / dt, fxv, fsum, v, vu);
+  gmass[sn] = (R)m;
+  for (int d = 0; d < 3; d++) {
+    gv[d * a.ns + sn] = (R)v[d];
+    gvu[d * a.ns + sn] = (R)vu[d];
+  }
+}
 template <int DIM, int S1, int TX, int TY, int TZ, int PREC>
-__global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
+__global__ __launch_bounds__(256) void k_grid_update(KArgs a, int mode) {
   if (frozen(a)) return;
   using R = typename Real<PREC>::T;
   constexpr bool FX = PREC == KMPM_PRECISION_MIXED;
@@ -694,7 +824,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
     const int ndil = a.sc->dil_count;
     const double dt_ctx = a.sc->dt;
     const unsigned long long *scr = reinterpret_cast<const unsigned long long *>(a.scratch);
-    const int f0 = remap ? 1 : 0, f1 = remap ? 4 : SCR_FIELDS;
+    const int f0 = gm_f0(mode), f1 = gm_f1(mode);
     for (int l0 = blockIdx.x * TPB; l0 < ndil; l0 += gridDim.x * TPB) {
       const int li = l0 + grp;
       const bool have = li < ndil;
@@ -737,34 +867,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
         for (int f = 0; f < SCR_FIELDS; f++)
           sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
                       : (double)isum[f];
-        const uint32_t mask = a.gmask[sn];
-        if (remap) {
-          // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
-          // node mass, then FixVelocityNodes::post_velocities_to_grid.
-          const double m = gmass[sn];
-          double v[3];
-          for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
-          for (int fi = 0; fi < a.nfv; fi++)
-            if (mask & a.fv[fi].bit)
-              for (int d = 0; d < 3; d++)
-                if (a.fv[fi].set[d]) v[d] = fxv[fi][d];
-          for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
-          continue;
-        }
-        const double m = sum[0];
-        double v[3], vu[3], mb[3] = {0, 0, 0};
-        for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
-        for (int fi = 0; fi < a.nbf; fi++)
-          if (m > 0 && (mask & a.bf[fi].bit))
-            for (int d = 0; d < 3; d++)
-              if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
-        for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
-        apply_velocity_fixes(a, mask, m, 1.0 / dt, fxv, fsum, v, vu);
-        gmass[sn] = (R)m;
-        for (int d = 0; d < 3; d++) {
-          gv[d * a.ns + sn] = (R)v[d];
-          gvu[d * a.ns + sn] = (R)vu[d];
-        }
+        grid_node<R>(a, mode, sn, sum, a.gmask[sn], dt, gmass, gv, gvu, fxv, fsum);
       }
     }
   } else {
@@ -800,11 +903,7 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
             const int pos = CX::lo_b(kx) * EY * EZ + sx * CY::lo_b(ky) * EZ +
                             sx * sy * CZ::lo_b(kz) + (rx * sy + ry) * sz + rz;
             const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + pos;
-            if (remap) {
-              for (int f = 1; f <= 3; f++) isum[f] += src[f * BN];
-            } else {
-              for (int f = 0; f < SCR_FIELDS; f++) isum[f] += src[f * BN];
-            }
+            for (int f = gm_f0(mode); f < gm_f1(mode); f++) isum[f] += src[f * BN];
           }
         }
       }
@@ -812,37 +911,10 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int remap) {
       for (int f = 0; f < SCR_FIELDS; f++)
         sum[f] = FX ? (double)isum[f] * a.fxi[f == 0 ? FX_MASS : f <= 3 ? FX_MOM : FX_FORCE]
                     : (double)isum[f];
-      const uint32_t mask = a.gmask[sn];
-      if (remap) {
-        // ULMPM::velocities_to_grid -> compute_velocity_nodes with the step's
-        // node mass, then FixVelocityNodes::post_velocities_to_grid.
-        const double m = gmass[sn];
-        double v[3];
-        for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
-        for (int fi = 0; fi < a.nfv; fi++)
-          if (mask & a.fv[fi].bit)
-            for (int d = 0; d < 3; d++)
-              if (a.fv[fi].set[d]) v[d] = fxv[fi][d];
-        for (int d = 0; d < 3; d++) gv[d * a.ns + sn] = (R)v[d];
-        continue;
-      }
-      const double m = sum[0];
-      double v[3], vu[3], mb[3] = {0, 0, 0};
-      for (int d = 0; d < 3; d++) v[d] = m > 0 ? sum[1 + d] / m : 0.0;
-      for (int fi = 0; fi < a.nbf; fi++)
-        if (m > 0 && (mask & a.bf[fi].bit))
-          for (int d = 0; d < 3; d++)
-            if (a.bf[fi].set[d]) mb[d] += a.bf[fi].val[d] * m;
-      for (int d = 0; d < 3; d++) vu[d] = m != 0 ? v[d] + dt * (sum[4 + d] + mb[d]) / m : v[d];
-      apply_velocity_fixes(a, mask, m, 1.0 / dt, fxv, fsum, v, vu);
-      gmass[sn] = (R)m;
-      for (int d = 0; d < 3; d++) {
-        gv[d * a.ns + sn] = (R)v[d];
-        gvu[d * a.ns + sn] = (R)vu[d];
-      }
+      grid_node<R>(a, mode, sn, sum, a.gmask[sn], dt, gmass, gv, gvu, fxv, fsum);
     }
   }
-  flush_fix_forces(a, fsum);
+  if (mode != GM_USF1) flush_fix_forces(a, fsum);   // GM_USF2 records the step's reaction
 }
 
 // Grid state for download: nodes outside every touched tile hold zeros
@@ -1067,6 +1139,24 @@ __device__ inline void weights_1d_as(double x, double lo, double h, double inv_h
   }
 }
 
+// Per-axis weights of a G2P gather in the path's type R.  For the APIC
+// family (apic) the derivative slot holds Di (x_I - xa) w instead, so the
+// gather's gradient contraction yields the APIC rate of deformation
+// Di sum_I v_I (x_I - xa)^T w (solid.cpp:1006-1153), xa being the particle
+// position of that rate: x (USL, USF), the advanced x (MUSL) or x0 (TLMPM).
+template <int S1, class R>
+__device__ inline void g2p_weights_1d(const KArgs &a, int d, double x, int base, bool apic,
+                                      double xa, R *w, R *dw) {
+  double wd[S1], dwd[S1];
+  weights_1d<S1>(x, a.lo[d], a.h, a.inv_h, a.n[d], base, wd, dwd);
+#pragma unroll
+  for (int q = 0; q < S1; q++) {
+    w[q] = (R)wd[q];
+    // node coordinate as the weights compute it
+    dw[q] = apic ? (R)(a.di * (((a.lo[d] + (0 + base + q) * a.h) - xa) * wd[q])) : (R)dwd[q];
+  }
+}
+
 // F update (solid.cpp:1181-1216, UL): F = (I + dt L) F, J = det F.
 __device__ inline double update_F(double dt, const double *L, double *F) {
   double M[9], Fn[9];
@@ -1134,12 +1224,14 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
 // network) follows in k_stress_learned; this kernel leaves the stress state
 // alone and stages its inputs (the rate of deformation, the EOS pressure and
 // the wave speed) in a.cut_stage.
-template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool STAGE>
+// XF: the APIC family (the rate of deformation from the affine moments, L
+// kept for the next step's P2G; ASFLIP's position update when a.asflip).
+template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool STAGE, bool XF = false>
 // Mixed linear and JC classes fit four waves per SIMD without spilling (the
 // fluid and generic classes would spill at that bound); the plain mixed
 // kernel takes 0.42 ms at four waves and 0.57 at five on the 4.27M bar.
 __global__ __launch_bounds__(256, STAGE ? 4
-                                        : (PREC == KMPM_PRECISION_MIXED &&
+                                        : (PREC == KMPM_PRECISION_MIXED && !XF &&
                                                    (MC == MC_LINEAR || MC == MC_JC_SHOCK)
                                                ? 4
                                                : 1)) void k_g2p_usl(KArgs a) {
@@ -1180,7 +1272,10 @@ __global__ __launch_bounds__(256, STAGE ? 4
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
-        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        if constexpr (XF)
+          g2p_weights_1d<S1>(a, d, x[d], c - off, true, x[d], w[d], dw[d]);
+        else
+          weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
       }
       double vu[3], acc[3], L[9];
       {
@@ -1192,11 +1287,21 @@ __global__ __launch_bounds__(256, STAGE ? 4
         }
         for (int e = 0; e < 9; e++) L[e] = Lr[e];
       }
+      if constexpr (XF)
+        for (int e = 0; e < 9; e++) STB(C00 + e, j, L[e]);
       const int pid = IFLD(a.ia, PID, slot);
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
-      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
-      if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
-      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      if (XF && a.asflip) {
+        // ASFLIP (solid.cpp:637-685, 791-796): the box check sees x before
+        // the update, which then moves x by the blended velocity
+        if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+        for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+        for (int d = 0; d < DIM; d++) x[d] += dt * v[d];
+      } else {
+        for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
+        if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+        for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      }
       // State is written in sorted order as soon as each part is final, so
       // it does not stay live across the constitutive update.
       const int mat = IFLD(a.ia, MATI, slot);
@@ -1280,10 +1385,23 @@ __global__ __launch_bounds__(256, STAGE ? 4
 // damaged particles, which the CFL skips).  Writes slot j of the next state.
 // ---------------------------------------------------------------------------
 
+// Where a stress kernel writes a particle: the next state at its sorted
+// position, or (USF, whose stress update precedes the G2P that carries the
+// state over) the current state's own slot.
+struct OutSlot {
+  double *d;
+  float *f;
+  int i;
+};
+__device__ inline OutSlot out_slot(const KArgs &a, bool in_place, int slot, int j) {
+  return in_place ? OutSlot{a.pa, a.fa, slot} : OutSlot{a.pb, a.fb, j};
+}
+#define STO(fld, val) stp<PREC>(a, o.d, o.f, (fld), o.i, (val))
+
 template <int MC, int PREC>
-__device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot, int j,
-                                 int pid, int mat, const double *v, double &cmax,
-                                 double &rmin) {
+__device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot,
+                                 const OutSlot &o, int pid, int mat, const double *v,
+                                 double &cmax, double &rmin) {
   double F[9], Finv[9], R[9];
   for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot) + dt * Fdot[e];   // :1181-1182
   m3_inverse(F, Finv);                                                   // :1186
@@ -1308,19 +1426,19 @@ __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, 
   double P[9];
   pk1(vol0, J, R, S, Finv, P);
   for (int e = 0; e < 9; e++) {
-    STB(F00 + e, j, F[e]);
-    STB(P00 + e, j, P[e]);
+    STO(F00 + e, F[e]);
+    STO(P00 + e, P[e]);
   }
-  STB(VOL0, j, vol0);
-  STB(VOL, j, J * vol0);
+  STO(VOL0, vol0);
+  STO(VOL, J * vol0);
   for (int e = 0; e < 6; e++) {
-    STB(SXX + e, j, sig[e]);
-    STB(EXX + e, j, eel[e]);
+    STO(SXX + e, sig[e]);
+    STO(EXX + e, eel[e]);
   }
-  STB(EP, j, ep);
-  STB(EPDOT, j, epdot);
-  STB(DINIT, j, dinit);
-  STB(DMG, j, dmg);
+  STO(EP, ep);
+  STO(EPDOT, epdot);
+  STO(DINIT, dinit);
+  STO(DMG, dmg);
   if (dmg >= 1.0) return;                                                // :1378-1379
   const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
   const double c = sqrt(m.cfl_k43g / rho) + vm;
@@ -1373,7 +1491,7 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
-        weights_1d_as<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        g2p_weights_1d<S1>(a, d, x0[d], c - off, a.xf != 0, x0[d], w[d], dw[d]);
       }
       double vu[3], acc[3], Fdot[9];
       {
@@ -1385,6 +1503,8 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
         }
         for (int e = 0; e < 9; e++) Fdot[e] = Fr[e];
       }
+      if (a.xf)   // APIC: Fdot is the next step's affine matrix
+        for (int e = 0; e < 9; e++) STB(C00 + e, j, Fdot[e]);
       const int pid = IFLD(a.ia, PID, slot);
       const int mat = IFLD(a.ia, MATI, slot);
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
@@ -1399,7 +1519,7 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
       STB(MASS, j, LDA(MASS, slot));
-      tl_update<MC, PREC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+      tl_update<MC, PREC>(a, dt, Fdot, slot, OutSlot{a.pb, a.fb, j}, pid, mat, v, cmax, rmin);
     }
     if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }
@@ -1410,6 +1530,11 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
 // MUSL part A: G2P + advance (musl.cpp:61-67); keeps start-of-step x
 // ---------------------------------------------------------------------------
 // TL: weights of x0 on the solid's grid, no box check, x0 carried along.
+// USF (a.scheme): the G2P and advance of usf.cpp:69-75, after the stress
+// kernel updated the current state in place; every other field is carried
+// into the next state here.  ASFLIP (a.asflip): the box check sees x before
+// the update, which moves x by the blended velocity (solid.cpp:637-685,
+// 791-796).
 template <int DIM, int S1, int TX, int TY, int TZ, int PREC, bool TL>
 __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
   if (frozen(a)) return;
@@ -1466,14 +1591,27 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
         for (int d = 0; d < 3; d++) a.xo[(size_t)d * a.cap + j] = x[d];
       }
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
-      for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
-      if (!TL && outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
-      for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      if (!TL && a.asflip) {
+        if (outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+        for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+        for (int d = 0; d < DIM; d++) x[d] += dt * v[d];
+      } else {
+        for (int d = 0; d < DIM; d++) x[d] += dt * vu[d];
+        if (!TL && outside_box(a, x)) set_error(a.sc, KMPM_ERR_PARTICLE_LEFT_BOX, pid);
+        for (int d = 0; d < 3; d++) v[d] = (1 - alpha) * vu[d] + alpha * (v[d] + dt * acc[d]);
+      }
       for (int d = 0; d < 3; d++) {
         STB(PX + d, j, x[d]);
         STB(VX + d, j, v[d]);
       }
       STB(MASS, j, LDA(MASS, slot));
+      if (a.scheme == KMPM_SCHEME_USF) {
+        const int nfld = state_nd(false, a.lay);
+        for (int f = F00; f < nfld; f++)
+          if (f != MASS && !(TL && f >= X0X && f <= X0Z)) STB(f, j, LDA(f, slot));
+        IFLD(a.ib, PID, j) = pid;
+        IFLD(a.ib, MATI, j) = IFLD(a.ia, MATI, slot);
+      }
       // next step's cell: of the advanced x (ULMPM) or of x0 (TLMPM)
       const double *xk = TL ? xw : x;
       a.key[j] = particle_key(a, xk[0], xk[1], xk[2]);
@@ -1484,6 +1622,12 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
 // ---------------------------------------------------------------------------
 // MUSL part B: L from re-mapped v (old weights), F, stress, CFL (musl.cpp:75-77)
 // ---------------------------------------------------------------------------
+// USF (a.scheme): the rate of deformation from the first P2G's grid velocity
+// with the current positions, F and the stress (usf.cpp:58-60), written in
+// place: the USF forces then come from the updated stress and the G2P
+// kernel carries the state over.  APIC (a.xf): L from the affine moments
+// about the advanced x (MUSL) or the current x (USF), kept as the next
+// step's affine matrix.
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC>
 __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
   if (frozen(a)) {
@@ -1508,17 +1652,21 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
+    const bool usf = a.scheme == KMPM_SCHEME_USF;
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
-      double x[3];
-      for (int d = 0; d < 3; d++) x[d] = a.xo[(size_t)d * a.cap + j];
+      double x[3], xa[3];
+      for (int d = 0; d < 3; d++) {
+        x[d] = usf ? LDA(PX + d, slot) : a.xo[(size_t)d * a.cap + j];
+        xa[d] = usf || !a.xf ? x[d] : LDB(PX + d, j);
+      }
       R w[3][S1], dw[3][S1];
       int cl[3] = {0, 0, 0};
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
-        weights_1d_as<S1>(x[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        g2p_weights_1d<S1>(a, d, x[d], c - off, a.xf != 0, xa[d], w[d], dw[d]);
       }
       double L[9];
       {
@@ -1527,22 +1675,27 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
                                                                (R *)nullptr, Lr);
         for (int e = 0; e < 9; e++) L[e] = Lr[e];
       }
+      const OutSlot o = out_slot(a, usf, slot, j);
+      if (a.xf)
+        for (int e = 0; e < 9; e++) STO(C00 + e, L[e]);
       const int pid = IFLD(a.ia, PID, slot);
       double v[3];
-      for (int d = 0; d < 3; d++) v[d] = LDB(VX + d, j);
+      for (int d = 0; d < 3; d++) v[d] = usf ? LDA(VX + d, slot) : LDB(VX + d, j);
       const int mat = IFLD(a.ia, MATI, slot);
-      IFLD(a.ib, PID, j) = pid;
-      IFLD(a.ib, MATI, j) = mat;
+      if (!usf) {
+        IFLD(a.ib, PID, j) = pid;
+        IFLD(a.ib, MATI, j) = mat;
+      }
       const double vol0 = LDA(VOL0, slot);
-      STB(VOL0, j, vol0);
+      STO(VOL0, vol0);
       double J;
       {
         double F[9];
         for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot);
         J = update_F(dt, L, F);
-        for (int e = 0; e < 9; e++) STB(F00 + e, j, F[e]);
+        for (int e = 0; e < 9; e++) STO(F00 + e, F[e]);
       }
-      STB(VOL, j, J * vol0);
+      STO(VOL, J * vol0);
       R dmg = (R)LDA(DMG, slot);
       if (J <= 0.0 && dmg < 1.0) set_error(a.sc, KMPM_ERR_JACOBIAN, pid);
       R sig[6], eel[6];
@@ -1556,13 +1709,13 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
           stress_and_speed<MC, PREC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
       if (c > cmax) cmax = c;
       for (int e = 0; e < 6; e++) {
-        STB(SXX + e, j, sig[e]);
-        STB(EXX + e, j, eel[e]);
+        STO(SXX + e, sig[e]);
+        STO(EXX + e, eel[e]);
       }
-      STB(EP, j, ep);
-      STB(EPDOT, j, epdot);
-      STB(DINIT, j, dinit);
-      STB(DMG, j, dmg);
+      STO(EP, ep);
+      STO(EPDOT, epdot);
+      STO(DINIT, dinit);
+      STO(DMG, dmg);
     }
     if (a.ninst) flush_tile_cfl<false>(a, t, cmax, rmin);
   }
@@ -1596,6 +1749,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
     __syncthreads();
+    const bool usf = a.scheme == KMPM_SCHEME_USF;
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
       double x0[3];
@@ -1606,7 +1760,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
       for (int d = 0; d < DIM; d++) {
         const int c = cell_of_clamped(x0[d], a.lo[d], a.inv_h, a.n[d]);
         cl[d] = tile_local(c, org[d], d == 0 ? TX : d == 1 ? TY : TZ, a.sc);
-        weights_1d_as<S1>(x0[d], a.lo[d], a.h, a.inv_h, a.n[d], c - off, w[d], dw[d]);
+        g2p_weights_1d<S1>(a, d, x0[d], c - off, a.xf != 0, x0[d], w[d], dw[d]);
       }
       double Fdot[9];
       {
@@ -1617,11 +1771,16 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
       }
       const int pid = IFLD(a.ia, PID, slot);
       const int mat = IFLD(a.ia, MATI, slot);
-      IFLD(a.ib, PID, j) = pid;
-      IFLD(a.ib, MATI, j) = mat;
+      if (!usf) {
+        IFLD(a.ib, PID, j) = pid;
+        IFLD(a.ib, MATI, j) = mat;
+      }
       double v[3];
-      for (int d = 0; d < 3; d++) v[d] = LDB(VX + d, j);
-      tl_update<MC, PREC>(a, dt, Fdot, slot, j, pid, mat, v, cmax, rmin);
+      for (int d = 0; d < 3; d++) v[d] = usf ? LDA(VX + d, slot) : LDB(VX + d, j);
+      const OutSlot o = out_slot(a, usf, slot, j);
+      if (a.xf)
+        for (int e = 0; e < 9; e++) STO(C00 + e, Fdot[e]);
+      tl_update<MC, PREC>(a, dt, Fdot, slot, o, pid, mat, v, cmax, rmin);
     }
     if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }
@@ -1682,7 +1841,7 @@ static inline int persistent_grid(kmpm_ctx *c, int per_cu) {
 // (kmpm_tiles.hip), so the variants compile in parallel.
 template <int DIM, int S1, int V> struct TileLaunch {
   static hipError_t p2g(kmpm_ctx *c, int remap);
-  static hipError_t grid_update(kmpm_ctx *c, int remap);
+  static hipError_t grid_update(kmpm_ctx *c, int mode);   // GridMode
   static hipError_t grid_clean(kmpm_ctx *c);
   static hipError_t g2p_usl(kmpm_ctx *c);
   static hipError_t g2p_musl_a(kmpm_ctx *c);

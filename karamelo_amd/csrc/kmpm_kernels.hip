@@ -550,7 +550,7 @@ void tile_shape(int dim, int S1, int variant, int T[3]) {
   } while (0)
 
 hipError_t launch_p2g(kmpm_ctx *c, int remap) { TILE_CALL(c, p2g(c, remap)); }
-hipError_t launch_grid_update(kmpm_ctx *c, int remap) { TILE_CALL(c, grid_update(c, remap)); }
+hipError_t launch_grid_update(kmpm_ctx *c, int mode) { TILE_CALL(c, grid_update(c, mode)); }
 hipError_t launch_grid_clean(kmpm_ctx *c) { TILE_CALL(c, grid_clean(c)); }
 hipError_t launch_g2p_usl(kmpm_ctx *c) { TILE_CALL(c, g2p_usl(c)); }
 hipError_t launch_g2p_musl_a(kmpm_ctx *c) { TILE_CALL(c, g2p_musl_a(c)); }

@@ -19,7 +19,8 @@ ERR_FIXED_POINT, ERR_INTERNAL, ERR_INSTANCE = 15, 16, 17
 NTILE_VARIANTS = 3
 SURROGATE_NPARAM = 1185
 METHOD_ULMPM, METHOD_TLMPM = 0, 1
-SCHEME_USL, SCHEME_MUSL = 0, 1
+SCHEME_USL, SCHEME_MUSL, SCHEME_USF = 0, 1, 2
+SUB_PIC, SUB_FLIP, SUB_APIC, SUB_AFLIP, SUB_ASFLIP, SUB_MLS = 0, 1, 2, 3, 4, 5
 SHAPE_LINEAR, SHAPE_CUBIC_SPLINE = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 ORDER_FAST, ORDER_REFERENCE = 0, 1
@@ -37,7 +38,10 @@ EOS_KEYS = {"linear": EOS_LINEAR, "shock": EOS_SHOCK, "fluid": EOS_FLUID}
 STRENGTH_KEYS = {"linear": STRENGTH_LINEAR, "johnson_cook": STRENGTH_JOHNSON_COOK,
                  "fluid": STRENGTH_FLUID}
 DAMAGE_KEYS = {"damage_johnson_cook": DAMAGE_JOHNSON_COOK}
-SCHEME_KEYS = {"usl": SCHEME_USL, "musl": SCHEME_MUSL}
+SCHEME_KEYS = {"usl": SCHEME_USL, "musl": SCHEME_MUSL, "usf": SCHEME_USF}
+# method(<style>, <sub-method>, ...) words (update.h:78-83)
+SUB_METHOD_KEYS = {"PIC": SUB_PIC, "FLIP": SUB_FLIP, "APIC": SUB_APIC, "AFLIP": SUB_AFLIP,
+                   "ASFLIP": SUB_ASFLIP, "MLS": SUB_MLS}
 METHOD_KEYS = {"ulmpm": METHOD_ULMPM, "tlmpm": METHOD_TLMPM}
 SHAPE_KEYS = {"linear": SHAPE_LINEAR, "cubic-spline": SHAPE_CUBIC_SPLINE}
 
@@ -51,8 +55,9 @@ class Config(C.Structure):
                 ("shape", C.c_int), ("precision", C.c_int), ("order", C.c_int),
                 ("boxlo", D3), ("boxhi", D3), ("cellsize", C.c_double),
                 ("pic_flip", C.c_double), ("dt", C.c_double), ("dt_constant", C.c_int),
-                ("reserved1", C.c_int), ("dt_factor", C.c_double),
-                ("solidlo", D3), ("solidhi", D3)]
+                ("sub_method", C.c_int), ("dt_factor", C.c_double),
+                ("solidlo", D3), ("solidhi", D3), ("np_per_cell", C.c_int),
+                ("reserved2", C.c_int)]
 
 
 class Material(C.Structure):
@@ -181,6 +186,9 @@ def _declare(lib):
         "kmpm_grid_to_points": (C.c_int, [vp]),
         "kmpm_velocities_to_grid": (C.c_int, [vp]),
         "kmpm_update_stress": (C.c_int, [vp]),
+        "kmpm_particles_to_grid_usf_1": (C.c_int, [vp]),
+        "kmpm_particles_to_grid_usf_2": (C.c_int, [vp]),
+        "kmpm_download_affine": (C.c_int, [vp, C.POINTER(C.c_double), C.c_int64]),
         "kmpm_adjust_dt": (C.c_int, [vp]),
         "kmpm_step": (C.c_int, [vp, C.c_int64]),
         "kmpm_run_time": (C.c_int, [vp, C.c_double, C.c_int64, P(C.c_int64)]),
@@ -366,6 +374,13 @@ class Context:
 
     def phase(self, name):
         check(getattr(self.L, "kmpm_" + name)(self.h), name)
+
+    def affine(self):
+        """The APIC affine matrix per particle (L, or Fdot on TLMPM), (np, 3, 3)."""
+        out = np.zeros((self.np, 3, 3))
+        check(self.L.kmpm_download_affine(self.h, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                          self.np), "download_affine")
+        return out
 
     def step(self, n):
         check(self.L.kmpm_step(self.h, int(n)), "step")

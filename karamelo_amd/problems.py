@@ -51,9 +51,11 @@ class Problem:
 
 
 def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-16,
-                dt_constant=False, dt_factor=0.9, method="ulmpm", solid_region=None):
+                dt_constant=False, dt_factor=0.9, method="ulmpm", solid_region=None,
+                sub_method="FLIP", np_per_cell=0):
     """kmpm_config for dimension/method/scheme/shape commands; for TLMPM the
-    solid's region gives the limits of its grid (Region::limits)."""
+    solid's region gives the limits of its grid (Region::limits).  sub_method
+    is method()'s second word (PIC, FLIP, APIC, AFLIP, ASFLIP, MLS)."""
     c = N.Config()
     c.dimension = dim
     c.method = N.METHOD_KEYS[method]
@@ -68,6 +70,8 @@ def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-1
     c.dt = dt
     c.dt_constant = 1 if dt_constant else 0
     c.dt_factor = dt_factor
+    c.sub_method = N.SUB_METHOD_KEYS[sub_method]
+    c.np_per_cell = np_per_cell
     if solid_region is not None:
         lo_, hi_ = N.region_limits(solid_region)
         for d in range(3):

@@ -488,6 +488,11 @@ struct orc_sim {
   /* TLMPM: Fdot, F^-1, R (polar), vol0 PK1 (solid.h:50-97) */
   double *Fdot, *Finv, *R, *PK1;
   int tl, wf_done, mass_done;         /* update_wf, update_mass_nodes (tlmpm.cpp:89,345) */
+  /* velocity-update sub-method: apic (ULMPM: APIC/AFLIP/ASFLIP/MLS,
+   * ulmpm.cpp:79-85; TLMPM: APIC only, tlmpm.cpp:82-84), the MLS force and
+   * the ASFLIP position update; alpha = PIC_FLIP after the sub-method's
+   * override; Di the inertia tensor of compute_inertia_tensor (diagonal) */
+  int apic, mls, asflip; double alpha; double Di[9];
   /* neighbour lists: particle-major (neigh_pn) and node-major CSR (neigh_np) */
   int32_t *nn_pn; int64_t *neigh_pn; double *wf_pn, *wfd_pn;
   int64_t *np_start; int64_t *neigh_np; double *wf_np, *wfd_np; int64_t npairs_cap;
@@ -519,9 +524,42 @@ orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
   if ((c->method != KMPM_METHOD_ULMPM && c->method != KMPM_METHOD_TLMPM) ||
       (c->dimension != 2 && c->dimension != 3))
     return NULL;
+  if (c->scheme < KMPM_SCHEME_USL || c->scheme > KMPM_SCHEME_USF ||
+      c->sub_method < KMPM_SUB_PIC || c->sub_method > KMPM_SUB_MLS)
+    return NULL;
   orc_sim *s = (orc_sim *)xcalloc(1, sizeof(orc_sim));
   s->c = *c;
   s->tl = c->method == KMPM_METHOD_TLMPM;
+  /* Update::modify_method (update.cpp:129-146), ULMPM::setup
+   * (ulmpm.cpp:79-85), TLMPM::setup (tlmpm.cpp:82-84) */
+  {
+    int sub = c->sub_method;
+    s->alpha = c->pic_flip;
+    if (s->tl) {
+      s->apic = sub == KMPM_SUB_APIC;
+    } else {
+      s->apic = sub == KMPM_SUB_APIC || sub == KMPM_SUB_AFLIP || sub == KMPM_SUB_ASFLIP ||
+                sub == KMPM_SUB_MLS;
+      s->mls = sub == KMPM_SUB_MLS;
+      s->asflip = sub == KMPM_SUB_ASFLIP;
+    }
+    if (sub == KMPM_SUB_APIC || (!s->tl && sub == KMPM_SUB_MLS)) s->alpha = 0;
+    if (s->apic) {
+      /* Solid::compute_inertia_tensor solid.cpp:1440-1470 */
+      double inv2 = 1.0 / (c->cellsize * c->cellsize), di;
+      if (c->shape == KMPM_SHAPE_LINEAR) {
+        if (!s->tl || (c->np_per_cell != 1 && c->np_per_cell != 2)) {
+          free(s);
+          return NULL;
+        }
+        di = (c->np_per_cell == 1 ? 16.0 / 4.0 : 16.0 / 3.0) * inv2;
+      } else {
+        di = 3.0 * inv2;
+      }
+      for (int e = 0; e < 9; e++) s->Di[e] = e % 4 == 0 ? di : 0.0;
+      if (c->dimension == 2) s->Di[8] = 1;
+    }
+  }
   const double *glo = grid_lo(c);
   s->dim = c->dimension;
   s->S1 = c->shape == KMPM_SHAPE_LINEAR ? 2 : 4;
@@ -870,14 +908,14 @@ int orc_phase_weights(orc_sim *s) {
 /* ------------------------------------------------------------------------ */
 /* P2G: ULMPM::particles_to_grid ulmpm.cpp:339-378 ->                        */
 /*   Solid::compute_mass_nodes solid.cpp:317-335,                            */
-/*   Solid::compute_velocity_nodes :337-390,                                 */
+/*   Solid::compute_velocity_nodes :337-390 (APIC: _APIC :392-426),          */
 /*   Solid::compute_external_and_internal_forces_nodes_UL :482-522           */
+/*   (MLS: _UL_MLS :524-574)                                                 */
 /* then Modify::post_particles_to_grid -> FixBodyforce fix_body_force.cpp:106-179 */
+/* USF splits it: mass and velocity (particles_to_grid_USF_1,                */
+/* ulmpm.cpp:380-404), forces (particles_to_grid_USF_2, :406-431)            */
 /* ------------------------------------------------------------------------ */
-int orc_phase_p2g(orc_sim *s) {
-  int nsolid_max = 0;
-  for (int64_t ip = 0; ip < s->np; ip++)
-    if (s->solid[ip] + 1 > nsolid_max) nsolid_max = s->solid[ip] + 1;
+static void p2g_mass(orc_sim *s) {
   /* mass: solid 0 resets, every solid accumulates in ascending ip; TLMPM
    * computes it on the first step only (tlmpm.cpp:344-351) */
   if (!(s->tl && s->mass_done)) {
@@ -892,14 +930,18 @@ int orc_phase_p2g(orc_sim *s) {
     }
   }
   s->mass_done = 1;
+}
+
+/* node velocity, per solid (solid.cpp:357-378): v += (sum_solid (w m) v) /
+ * mass; APIC (solid.cpp:419-424): v + C (x0_I - x_p), C = L (ULMPM) or Fdot
+ * (TLMPM), x_p = x (ULMPM) or x0 (TLMPM) */
+static void p2g_velocity(orc_sim *s) {
+  const double *Cs = s->tl ? s->Fdot : s->L, *pos = s->tl ? s->x0 : s->x;
   OMP_FOR
   for (int64_t in = 0; in < s->nn; in++) {
     int64_t b = s->np_start[in], e = s->np_start[in + 1];
-    double *gv = &s->gv[3 * in], *gf = &s->gf[3 * in], *gmb = &s->gmb[3 * in];
+    double *gv = &s->gv[3 * in];
     gv[0] = gv[1] = gv[2] = 0;
-    gf[0] = gf[1] = gf[2] = 0;
-    gmb[0] = gmb[1] = gmb[2] = 0;
-    /* velocity, per solid (:357-378): v += (sum_solid (w m) v) / mass */
     int64_t o = b;
     while (o < e) {
       int sol = s->solid[s->neigh_np[o]];
@@ -908,7 +950,16 @@ int orc_phase_p2g(orc_sim *s) {
       for (; o2 < e && s->solid[s->neigh_np[o2]] == sol; o2++) {
         int64_t ip = s->neigh_np[o2];
         double wm = s->wf_np[o2] * s->mass[ip];
-        for (int d = 0; d < 3; d++) vt[d] += wm * s->v[3 * ip + d];
+        if (s->apic) {
+          const double *Cp = &Cs[9 * ip];
+          double dx[3];
+          for (int d = 0; d < 3; d++) dx[d] = s->gx0[3 * in + d] - pos[3 * ip + d];
+          for (int d = 0; d < 3; d++)
+            vt[d] += wm * (s->v[3 * ip + d] +
+                           (Cp[3 * d] * dx[0] + Cp[3 * d + 1] * dx[1] + Cp[3 * d + 2] * dx[2]));
+        } else {
+          for (int d = 0; d < 3; d++) vt[d] += wm * s->v[3 * ip + d];
+        }
       }
       if (s->gmass[in] > 0) {
         for (int d = 0; d < 3; d++) vt[d] /= s->gmass[in];
@@ -916,15 +967,36 @@ int orc_phase_p2g(orc_sim *s) {
       }
       o = o2;
     }
-    /* internal force: UL f -= vol * (sigma * wfd) (:510-515); TL
-     * f -= vol0PK1 * wfd0 (compute_internal_forces_nodes_TL :452-480) */
-    for (o = b; o < e; o++) {
+  }
+}
+
+/* internal force: UL f -= vol * (sigma * wfd) (:510-515); MLS f -= vol * w *
+ * (sigma * Di * (x0_I - x)) (:560-565); TL f -= vol0PK1 * wfd0
+ * (compute_internal_forces_nodes_TL :452-480); then the body force fix */
+static void p2g_forces(orc_sim *s) {
+  OMP_FOR
+  for (int64_t in = 0; in < s->nn; in++) {
+    int64_t b = s->np_start[in], e = s->np_start[in + 1];
+    double *gf = &s->gf[3 * in], *gmb = &s->gmb[3 * in];
+    gf[0] = gf[1] = gf[2] = 0;
+    gmb[0] = gmb[1] = gmb[2] = 0;
+    for (int64_t o = b; o < e; o++) {
       int64_t ip = s->neigh_np[o];
       const double *wd = &s->wfd_np[3 * o];
       if (s->tl) {
         const double *P = &s->PK1[9 * ip];
         for (int d = 0; d < 3; d++)
           gf[d] -= P[3 * d] * wd[0] + P[3 * d + 1] * wd[1] + P[3 * d + 2] * wd[2];
+      } else if (s->mls) {
+        const double *sg = &s->sig[9 * ip], *Di = s->Di;
+        double M[9], dx[3];
+        for (int i = 0; i < 3; i++)      /* sigma * Di, a 3x3 product */
+          for (int j = 0; j < 3; j++)
+            M[3 * i + j] = sg[3 * i] * Di[j] + sg[3 * i + 1] * Di[3 + j] + sg[3 * i + 2] * Di[6 + j];
+        for (int d = 0; d < 3; d++) dx[d] = s->gx0[3 * in + d] - s->x[3 * ip + d];
+        double vw = s->vol[ip] * s->wf_np[o];
+        for (int d = 0; d < 3; d++)
+          gf[d] -= vw * (M[3 * d] * dx[0] + M[3 * d + 1] * dx[1] + M[3 * d + 2] * dx[2]);
       } else {
         const double *sg = &s->sig[9 * ip];
         for (int d = 0; d < 3; d++) {
@@ -949,28 +1021,20 @@ int orc_phase_p2g(orc_sim *s) {
       }
     }
   }
-  (void)nsolid_max;
+}
+
+int orc_phase_p2g(orc_sim *s) {
+  p2g_mass(s);
+  p2g_velocity(s);
+  p2g_forces(s);
   return KMPM_OK;
 }
 
-/* ------------------------------------------------------------------------ */
-/* grid update: Grid::update_grid_velocities grid.cpp:448-466, then          */
-/* FixVelocityNodes::post_update_grid_state fix_velocity_nodes.cpp:130-225   */
-/* ------------------------------------------------------------------------ */
-int orc_phase_grid_update(orc_sim *s) {
+/* FixVelocityNodes::post_update_grid_state fix_velocity_nodes.cpp:130-225:
+ * fix after fix, nodes in order, v_update <- value, v <- previous value,
+ * with the reaction force sum (1/dt) m (value - v_update) */
+static void velocity_fixes_post_update(orc_sim *s) {
   double dt = s->dt;
-  OMP_FOR
-  for (int64_t in = 0; in < s->nn; in++) {
-    double m = s->gmass[in];
-    for (int d = 0; d < 3; d++) {
-      if (m != 0)
-        s->gvu[3 * in + d] = s->gv[3 * in + d] + dt * (s->gf[3 * in + d] + s->gmb[3 * in + d]) / m;
-      else
-        s->gvu[3 * in + d] = s->gv[3 * in + d];
-    }
-  }
-  /* FixVelocityNodes::post_update_grid_state, fix after fix, nodes in order,
-   * with the reaction force sum (1/dt) m (value - v_update) */
   double inv_dt = 1.0 / s->dt;
   for (int k = 0; k < s->nfv; k++) {
     fixvel_t *f = &s->fv[k];
@@ -998,35 +1062,34 @@ int orc_phase_grid_update(orc_sim *s) {
     }
     for (int d = 0; d < 3; d++) f->force[d] = ftot[d];
   }
+  (void)dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid update: Grid::update_grid_velocities grid.cpp:448-466, then          */
+/* FixVelocityNodes::post_update_grid_state fix_velocity_nodes.cpp:130-225   */
+/* ------------------------------------------------------------------------ */
+int orc_phase_grid_update(orc_sim *s) {
+  double dt = s->dt;
+  OMP_FOR
+  for (int64_t in = 0; in < s->nn; in++) {
+    double m = s->gmass[in];
+    for (int d = 0; d < 3; d++) {
+      if (m != 0)
+        s->gvu[3 * in + d] = s->gv[3 * in + d] + dt * (s->gf[3 * in + d] + s->gmb[3 * in + d]) / m;
+      else
+        s->gvu[3 * in + d] = s->gv[3 * in + d];
+    }
+  }
+  velocity_fixes_post_update(s);
   return KMPM_OK;
 }
 
 /* MUSL re-map: ULMPM::velocities_to_grid ulmpm.cpp:476-496 ->
- * compute_velocity_nodes (solid.cpp:337-390), then
+ * compute_velocity_nodes (solid.cpp:337-390; APIC :392-426), then
  * FixVelocityNodes::post_velocities_to_grid (fix_velocity_nodes.cpp:227-271) */
 static void phase_velocities_to_grid(orc_sim *s) {
-  OMP_FOR
-  for (int64_t in = 0; in < s->nn; in++) {
-    int64_t b = s->np_start[in], e = s->np_start[in + 1];
-    double *gv = &s->gv[3 * in];
-    gv[0] = gv[1] = gv[2] = 0;
-    int64_t o = b;
-    while (o < e) {
-      int sol = s->solid[s->neigh_np[o]];
-      double vt[3] = {0, 0, 0};
-      int64_t o2 = o;
-      for (; o2 < e && s->solid[s->neigh_np[o2]] == sol; o2++) {
-        int64_t ip = s->neigh_np[o2];
-        double wm = s->wf_np[o2] * s->mass[ip];
-        for (int d = 0; d < 3; d++) vt[d] += wm * s->v[3 * ip + d];
-      }
-      if (s->gmass[in] > 0) {
-        for (int d = 0; d < 3; d++) vt[d] /= s->gmass[in];
-        for (int d = 0; d < 3; d++) gv[d] += vt[d];
-      }
-      o = o2;
-    }
-  }
+  p2g_velocity(s);
   for (int k = 0; k < s->nfv; k++) {
     fixvel_t *f = &s->fv[k];
     OMP_FOR
@@ -1039,8 +1102,11 @@ static void phase_velocities_to_grid(orc_sim *s) {
 /* Solid::compute_rate_deformation_gradient_UL solid.cpp:860-936 */
 /* TLMPM: Solid::compute_rate_deformation_gradient_TL solid.cpp:798-858, the
  * same sums over the reference gradients into Fdot */
+/* APIC: compute_rate_deformation_gradient_UL_APIC solid.cpp:1081-1153
+ * (TL_APIC :1006-1079): L(a,b) += v_I[a] (x0_I - x_p)[b] w, then L *= Di */
 static void phase_rate_of_deformation(orc_sim *s, int doublemapping) {
   const double *vn = doublemapping ? s->gv : s->gvu;
+  const double *pos = s->tl ? s->x0 : s->x;
   OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
     double *L = s->tl ? &s->Fdot[9 * ip] : &s->L[9 * ip];
@@ -1048,7 +1114,13 @@ static void phase_rate_of_deformation(orc_sim *s, int doublemapping) {
     for (int c = 0; c < s->nn_pn[ip]; c++) {
       int64_t q = ip * s->S + c, in = s->neigh_pn[q];
       const double *wd = &s->wfd_pn[3 * q], *v = &vn[3 * in];
-      if (s->dim == 2) {
+      if (s->apic) {
+        double dx[3], w = s->wf_pn[q];
+        for (int d = 0; d < 3; d++) dx[d] = s->gx0[3 * in + d] - pos[3 * ip + d];
+        int nd = s->dim == 2 ? 2 : 3;
+        for (int a = 0; a < nd; a++)
+          for (int b = 0; b < nd; b++) L[3 * a + b] += v[a] * dx[b] * w;
+      } else if (s->dim == 2) {
         L[0] += v[0] * wd[0];
         L[1] += v[0] * wd[1];
         L[3] += v[1] * wd[0];
@@ -1058,14 +1130,21 @@ static void phase_rate_of_deformation(orc_sim *s, int doublemapping) {
           for (int b = 0; b < 3; b++) L[3 * a + b] += v[a] * wd[b];
       }
     }
+    if (s->apic) {                    /* L *= Di, a 3x3 product */
+      double T[9];
+      m3_mul(L, s->Di, T);
+      memcpy(L, T, sizeof(T));
+    }
   }
 }
 
 /* Solid::compute_particle_accelerations_velocities_and_positions
- * solid.cpp:576-635; Solid::update_particle_velocities :786-790 */
+ * solid.cpp:576-635 (ASFLIP: compute_particle_accelerations_velocities
+ * :637-685, no position update); Solid::update_particle_velocities :786-790
+ * (ASFLIP: update_particle_velocities_and_positions :791-796, x += dt v) */
 static int phase_g2p(orc_sim *s) {
   double inv_dt = 1.0 / s->dt;
-  double alpha = s->c.pic_flip;
+  double alpha = s->alpha;
   const double *lo = s->c.boxlo, *hi = s->c.boxhi;
   OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
@@ -1078,7 +1157,8 @@ static int phase_g2p(orc_sim *s) {
       for (int d = 0; d < 3; d++) a[d] += w * (s->gvu[3 * in + d] - s->gv[3 * in + d]);
     }
     for (int d = 0; d < 3; d++) a[d] *= inv_dt;
-    for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * vu[d];
+    if (!s->asflip)
+      for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * vu[d];
     const double *x = &s->x[3 * ip];
     if (!s->tl &&        /* the box check is UL only (solid.cpp:617-627) */
         !(x[0] >= lo[0] && x[0] <= hi[0] && x[1] >= lo[1] && x[1] <= hi[1] &&
@@ -1087,10 +1167,13 @@ static int phase_g2p(orc_sim *s) {
     }
   }
   OMP_FOR
-  for (int64_t ip = 0; ip < s->np; ip++)
+  for (int64_t ip = 0; ip < s->np; ip++) {
     for (int d = 0; d < 3; d++)
       s->v[3 * ip + d] = (1 - alpha) * s->vu[3 * ip + d] +
                          alpha * (s->v[3 * ip + d] + s->dt * s->a[3 * ip + d]);
+    if (s->asflip)
+      for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * s->v[3 * ip + d];
+  }
   return s->status;
 }
 
@@ -1346,12 +1429,25 @@ static int phase_time_and_dt(orc_sim *s) {
   return s->status;
 }
 
-/* USL::run usl.cpp:43-93 and MUSL::run musl.cpp:43-94, one step each */
+/* USL::run usl.cpp:43-93, MUSL::run musl.cpp:43-94 and USF::run
+ * usf.cpp:34-94, one step each */
 int orc_step(orc_sim *s, int64_t nsteps) {
   for (int64_t k = 0; k < nsteps && s->status == KMPM_OK; k++) {
     s->step++;                                                  /* update_timestep */
     orc_phase_weights(s);
     s->dtCFL = 1.0e22;                                          /* reset */
+    if (s->c.scheme == KMPM_SCHEME_USF) {
+      p2g_mass(s);                                              /* particles_to_grid_USF_1 */
+      p2g_velocity(s);
+      velocity_fixes_post_update(s);                            /* post_update_grid_state */
+      phase_rate_of_deformation(s, 1);
+      if (phase_F_and_stress(s)) break;
+      p2g_forces(s);                                            /* USF_2, post_particles_to_grid */
+      orc_phase_grid_update(s);
+      if (phase_g2p(s)) break;
+      phase_time_and_dt(s);
+      continue;
+    }
     orc_phase_p2g(s);
     orc_phase_grid_update(s);
     if (s->c.scheme == KMPM_SCHEME_USL) {
@@ -1366,6 +1462,23 @@ int orc_step(orc_sim *s, int64_t nsteps) {
     phase_time_and_dt(s);
   }
   return s->status;
+}
+
+/* set the affine matrix (L or Fdot) of every particle, as a state the
+ * reference reaches after a step (pins of the APIC transfers) */
+int orc_set_affine(orc_sim *s, const double *C, int64_t np) {
+  if (np != s->np) return KMPM_ERR_ARG;
+  memcpy(s->tl ? s->Fdot : s->L, C, (size_t)np * 9 * sizeof(double));
+  return KMPM_OK;
+}
+
+/* the affine matrix of the APIC transfers: L (ULMPM) or Fdot (TLMPM),
+ * row-major, particle order */
+int orc_get_affine(orc_sim *s, double *C, int64_t np) {
+  if (np < s->np) return KMPM_ERR_CAPACITY;
+  const double *src = s->tl ? s->Fdot : s->L;
+  memcpy(C, src, (size_t)s->np * 9 * sizeof(double));
+  return KMPM_OK;
 }
 
 int orc_get_time(orc_sim *s, double *dt, double *time, int64_t *step) {

@@ -55,6 +55,9 @@ int orc_var_eval(const char *text, int previous, double time, double dt, int nva
 /* reaction force of velocity fix `fix` in the last step */
 int orc_fix_velocity_force(orc_sim *s, int fix, double out[3]);
 int orc_step(orc_sim *s, int64_t nsteps);
+/* L (ULMPM) or Fdot (TLMPM) of the last step, row-major, 9 per particle */
+int orc_get_affine(orc_sim *s, double *C, int64_t np);
+int orc_set_affine(orc_sim *s, const double *C, int64_t np);
 int orc_get_time(orc_sim *s, double *dt, double *time, int64_t *step);
 int orc_energy(orc_sim *s, uint32_t groupbit, double *ek, double *es);
 

@@ -48,6 +48,8 @@ def load(variant="liboracle.so"):
             "orc_fix_velocity_nodes": (C.c_int, [vp, C.c_uint32, N.I3, N.D3, N.D3]),
             "orc_fix_body_force": (C.c_int, [vp, C.c_uint32, N.I3, N.D3]),
             "orc_step": (C.c_int, [vp, C.c_int64]),
+            "orc_get_affine": (C.c_int, [vp, D, C.c_int64]),
+            "orc_set_affine": (C.c_int, [vp, D, C.c_int64]),
             "orc_get_time": (C.c_int, [vp, D, D, P(C.c_int64)]),
             "orc_energy": (C.c_int, [vp, C.c_uint32, D, D]),
             "orc_phase_weights": (C.c_int, [vp]),
@@ -158,6 +160,16 @@ class Oracle:
 
     def step(self, n):
         return self.L.orc_step(self.h, int(n))
+
+    def affine(self):
+        """L (ULMPM) or Fdot (TLMPM) per particle, (np, 3, 3)."""
+        out = np.zeros((self.np, 3, 3))
+        assert self.L.orc_get_affine(self.h, _dp(out), self.np) == 0
+        return out
+
+    def set_affine(self, Cm):
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64).reshape(self.np, 3, 3)
+        assert self.L.orc_set_affine(self.h, _dp(Cm), self.np) == 0
 
     def phase(self, name):
         assert getattr(self.L, "orc_phase_" + name)(self.h) == 0
