@@ -188,6 +188,7 @@ hipError_t launch_keys(kmpm_ctx *c);
 hipError_t launch_sort(kmpm_ctx *c);
 hipError_t launch_p2g(kmpm_ctx *c, int momentum_only);
 hipError_t launch_grid_update(kmpm_ctx *c, int mode);   // GridMode (kmpm_kernels.h)
+hipError_t launch_fix_values(kmpm_ctx *c);   // expression-valued fixes, before a grid pass
 hipError_t launch_g2p_usl(kmpm_ctx *c);
 hipError_t launch_g2p_musl_a(kmpm_ctx *c);
 hipError_t launch_stress_musl_b(kmpm_ctx *c);

@@ -116,10 +116,9 @@ __device__ inline double expr_eval(const kmpm_expr &e, double t, double dt) {
 }
 
 // The values every velocity fix imposes in this step, into LDS (val[3],
-// prev[3] per fix), and the fix force accumulators cleared; every thread of
-// the block calls it (it synchronises).  Expressions see the time at the
-// start of the step and the script's dt: 0 in the first step unless set_dt
-// (input.cpp:55, update.cpp:69-76), the step's dt after that (ulmpm.cpp:550).
+// prev[3] per fix; expression values from k_fix_values), and the fix force
+// accumulators cleared; every thread of the block calls it (it
+// synchronises).
 __device__ inline void fix_values_to_lds(const KArgs &a, double (*fxv)[6], double (*fsum)[3]) {
   const int tid = threadIdx.x;
   if (tid < a.nfv * 6) {
@@ -127,13 +126,12 @@ __device__ inline void fix_values_to_lds(const KArgs &a, double (*fxv)[6], doubl
     const DevFixVel &f = a.fv[fi];
     double v;
     if (f.expr) {
-      const double dtv = (a.sc->step == 0 && !a.dt_constant) ? 0.0 : a.sc->dt;
-      v = expr_eval(a.fexpr[6 * fi + w], a.sc->time, dtv);
+      v = a.sc->fv_last[fi][w];
     } else {
       v = w < 3 ? f.val[w] : f.prev[w - 3];
+      if (blockIdx.x == 0) a.sc->fv_last[fi][w] = v;
     }
     fxv[fi][w] = v;
-    if (blockIdx.x == 0) a.sc->fv_last[fi][w] = v;
   }
   if (tid < a.nfv * 3) fsum[tid / 3][tid % 3] = 0.0;
   __syncthreads();
@@ -903,7 +901,10 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int mode) {
             const int pos = CX::lo_b(kx) * EY * EZ + sx * CY::lo_b(ky) * EZ +
                             sx * sy * CZ::lo_b(kz) + (rx * sy + ry) * sz + rz;
             const BT *src = reinterpret_cast<const BT *>(a.scratch) + (size_t)slot * SCR_FIELDS * BN + pos;
-            for (int f = gm_f0(mode); f < gm_f1(mode); f++) isum[f] += src[f * BN];
+            const int f0 = gm_f0(mode), f1 = gm_f1(mode);
+#pragma unroll
+            for (int f = 0; f < SCR_FIELDS; f++)
+              if (f >= f0 && f < f1) isum[f] += src[f * BN];
           }
         }
       }

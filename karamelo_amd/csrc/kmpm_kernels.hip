@@ -550,7 +550,40 @@ void tile_shape(int dim, int S1, int variant, int T[3]) {
   } while (0)
 
 hipError_t launch_p2g(kmpm_ctx *c, int remap) { TILE_CALL(c, p2g(c, remap)); }
-hipError_t launch_grid_update(kmpm_ctx *c, int mode) { TILE_CALL(c, grid_update(c, mode)); }
+// The expression-valued fixes' values of this step into sc->fv_last (one
+// block of 64 threads, launched before every grid pass of a context with
+// such fixes, so the interpreter's registers stay out of the grid kernels).
+// Expressions see the time at the start of the step and the script's dt: 0
+// in the first step unless set_dt (input.cpp:55, update.cpp:69-76), the
+// step's dt after that (ulmpm.cpp:550).
+__global__ __launch_bounds__(64) void k_fix_values(KArgs a) {
+  const int tid = threadIdx.x;
+  if (frozen(a) || tid >= a.nfv * 6) return;
+  const int fi = tid / 6, w = tid % 6;
+  if (!a.fv[fi].expr) return;
+  const double dtv = (a.sc->step == 0 && !a.dt_constant) ? 0.0 : a.sc->dt;
+  a.sc->fv_last[fi][w] = expr_eval(a.fexpr[6 * fi + w], a.sc->time, dtv);
+}
+
+static bool has_expr_fix(const kmpm_ctx *c) {
+  for (int fi = 0; fi < c->a.nfv; fi++)
+    if (c->a.fv[fi].expr) return true;
+  return false;
+}
+
+hipError_t launch_fix_values(kmpm_ctx *c) {
+  if (!has_expr_fix(c)) return hipSuccess;
+  hipLaunchKernelGGL(k_fix_values, dim3(1), dim3(64), 0, c->stream, c->a);
+  return hipGetLastError();
+}
+
+static hipError_t tile_grid_update(kmpm_ctx *c, int mode) { TILE_CALL(c, grid_update(c, mode)); }
+
+hipError_t launch_grid_update(kmpm_ctx *c, int mode) {
+  const hipError_t e = launch_fix_values(c);
+  if (e != hipSuccess) return e;
+  return tile_grid_update(c, mode);
+}
 hipError_t launch_grid_clean(kmpm_ctx *c) { TILE_CALL(c, grid_clean(c)); }
 hipError_t launch_g2p_usl(kmpm_ctx *c) { TILE_CALL(c, g2p_usl(c)); }
 hipError_t launch_g2p_musl_a(kmpm_ctx *c) { TILE_CALL(c, g2p_musl_a(c)); }

@@ -456,6 +456,8 @@ hipError_t launch_ref_p2g(kmpm_ctx *c, int remap) {
 }
 
 hipError_t launch_ref_grid(kmpm_ctx *c, int remap) {
+  const hipError_t e = launch_fix_values(c);
+  if (e != hipSuccess) return e;
   if (!remap)
     hipLaunchKernelGGL(k_ref_grid, dim3(ref_blocks(c->a.nn)), dim3(256), 0, c->stream, c->a);
   hipLaunchKernelGGL(k_ref_fixes, dim3(1), dim3(256), 0, c->stream, c->a, remap);
