@@ -138,3 +138,56 @@ def test_batched_256_instances_full_size_graph_equals_eager():
     assert len(np.unique(ta)) > 1 and np.all(ta > 0)
     eager.close()
     graph.close()
+
+
+def test_dam_break_256_cubed_2000_steps():
+    """The 256^3 dam break (17.2M particles, mixed, MUSL cubic) for its 2000
+    steps: no error, every particle inside the box, the column slumps, and
+    the kinetic energy stays below the potential energy released."""
+    say("dam break 256^3: 2000 steps")
+    prob = mixed(P.dam_break(n_cells=256, jitter=0.1))
+    m = prob.particles["mass"]
+    z0 = prob.particles["x"][:, 2].copy()
+    ctx = P.context_for(prob)
+    for k in range(4):
+        ctx.step(500)
+        st, tag = ctx.synchronize()
+        assert st == 0, (st, tag)
+        say("  %d steps, t = %.4g s" % (500 * (k + 1), ctx.time()[1]))
+    q = ctx.download()
+    ek = ctx.energy()[0]
+    ctx.close()
+    assert np.all((q["x"] >= 0) & (q["x"] <= 1))
+    dpe = 9.81 * float((m * (z0 - q["x"][:, 2])).sum())
+    say("  kinetic energy %.4g J, potential energy released %.4g J" % (ek, dpe))
+    assert dpe > 0 and q["x"][:, 2].mean() < z0.mean()
+    assert ek <= 1.05 * dpe
+
+
+def test_taylor_bar_full_size_80_microseconds():
+    """The 1.04M-particle TLMPM Taylor bar run to 80 us of simulated time
+    (run_time): no error, no particle past the wall, the kinetic energy
+    spent in plastic work, the impact end mushroomed."""
+    say("Taylor bar, 1.04M particles: run_time 80 us")
+    prob = P.taylor_bar()
+    r0 = np.hypot(prob.particles["x"][:, 1], prob.particles["x"][:, 2]).max()
+    ctx = P.context_for(prob)
+    ek0 = ctx.energy()[0]
+    n = ctx.run_time(80e-6, max_steps=20000)
+    st, tag = ctx.synchronize()
+    assert st == 0, (st, tag)
+    t = ctx.time()[1]
+    q = ctx.download()
+    ek = ctx.energy()[0]
+    ctx.close()
+    say("  %d steps to t = %.4g s; kinetic energy %.4g -> %.4g J" % (n, t, ek0, ek))
+    assert t >= 80e-6 - 1e-12 and n > 1000
+    wall = 0.5 * 32.4e-3
+    x = q["x"]
+    assert x[:, 0].max() <= wall + prob.cfg.cellsize
+    assert ek < 0.5 * ek0
+    near = x[:, 0] > wall - 2e-3
+    r_end = np.hypot(x[near, 1], x[near, 2]).max()
+    say("  radius at the impact end %.4g mm (initial %.4g mm)" % (r_end * 1e3, r0 * 1e3))
+    assert r_end > 1.1 * r0
+    assert q["eff_plastic_strain"].max() > 0.1
