@@ -111,7 +111,10 @@ typedef struct kmpm_config {
   /* particles per cell per axis of the solid (solid.cpp:2027), read only by
    * the APIC inertia of linear shape functions (solid.cpp:1454-1462) */
   int    np_per_cell;
-  int    reserved2;
+  /* method(..., thermo-mechanical) (update.cpp:168-180): particle and grid
+   * temperatures, heat sources and fluxes (solid.cpp:2743-2839,
+   * grid.cpp:1354-1362); FP64, one solid per context */
+  int    thermo;
 } kmpm_config;
 
 /* ---- materials: material(...) + eos/strength/damage styles ------------- */
@@ -124,6 +127,8 @@ enum { KMPM_STRENGTH_NONE = 0, KMPM_STRENGTH_LINEAR = 1,
        KMPM_STRENGTH_JOHNSON_COOK = 2, KMPM_STRENGTH_FLUID = 3 };
 /* damage(id, <key>, ...) keys: damage_johnson_cook (style_damage.h) */
 enum { KMPM_DAMAGE_NONE = 0, KMPM_DAMAGE_JOHNSON_COOK = 1 };
+/* temperature(id, <key>, ...) keys: plastic_work (style_temperature.h) */
+enum { KMPM_TEMP_NONE = 0, KMPM_TEMP_PLASTIC_WORK = 1 };
 
 typedef struct kmpm_material {
   int type;               /* KMPM_MAT_* */
@@ -144,6 +149,13 @@ typedef struct kmpm_material {
   double damage_p[8];
   /* Derived moduli, filled by kmpm_material_init (Mat ctors, material.cpp:720-769). */
   double rho0, K, G, E, nu, lambda, signal_velocity;
+  /* The material's temperature law, the last argument of material(...)
+   * (material.cpp:310-358); temp = KMPM_TEMP_* and
+   *   temperature plastic_work  temp_p = chi, cp, kappa, alpha, T0, Tm
+   *                             (temperature_plastic_work.cpp:50-55)
+   * A law makes the stress update temperature dependent (solid.cpp:1301-1351). */
+  int    temp, reserved_t;
+  double temp_p[6];
 } kmpm_material;
 
 /* ---- regions used to populate solids and groups ------------------------ */
@@ -161,7 +173,9 @@ typedef struct kmpm_region {
 /* ---- particle interchange (host side, reference layouts) --------------- */
 /* Vectors are [np][3], tensors [np][3][3] row-major, as Eigen::Vector3d /
  * Matrix3d per particle in solid.h:50-97.  On download any NULL pointer is
- * skipped.  Particles are returned in ascending ptag order. */
+ * skipped, so zero-initialise the struct and set the fields wanted (fields
+ * may be appended at its end).  Particles are returned in ascending ptag
+ * order. */
 typedef struct kmpm_particles {
   int64_t np;
   int64_t *ptag;
@@ -173,6 +187,11 @@ typedef struct kmpm_particles {
   double *mass, *vol0, *vol, *rho0, *rho, *J;
   double *eff_plastic_strain, *eff_plastic_strain_rate;
   double *damage, *damage_init;
+  /* thermo-mechanical contexts: temperature T (the solid's T0 at populate,
+   * solid.cpp:2305; kmpm_populate writes T0 = 0 and the caller sets it) and,
+   * on download, the heat source gamma (solid.cpp:1343-1351); either may be
+   * NULL; other contexts ignore T and download zeros */
+  double *T, *gamma;
 } kmpm_particles;
 
 /* ---- host-side set-up (no device needed) -------------------------------- */

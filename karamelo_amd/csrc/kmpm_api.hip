@@ -204,6 +204,15 @@ DevMat kmpm::make_devmat(const kmpm_material &m) {
   memcpy(d.eos_p, m.eos_p, sizeof(d.eos_p));
   memcpy(d.strength_p, m.strength_p, sizeof(d.strength_p));
   memcpy(d.damage_p, m.damage_p, sizeof(d.damage_p));
+  if (m.type == KMPM_MAT_EOS_STRENGTH && m.temp == KMPM_TEMP_PLASTIC_WORK && m.temp_p[1] != 0) {
+    d.temp_on = 1;
+    d.chi = m.temp_p[0];
+    d.invcp = 1.0 / m.temp_p[1];
+    d.kappa = m.temp_p[2];
+    d.talpha = m.temp_p[3];
+    d.T0 = m.temp_p[4];
+    d.Tmelt = m.temp_p[5];
+  }
   return d;
 }
 
@@ -257,7 +266,7 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   if (variant < 0 || variant >= KMPM_NTILE_VARIANTS || g.T[0] == 0) return KMPM_ERR_ARG;
   // the APIC transfers and the USF step are compiled for the first tile
   // shape of each dimension and shape function only
-  if (variant != 0 && (a.xf || a.scheme == KMPM_SCHEME_USF)) return KMPM_ERR_ARG;
+  if (variant != 0 && (a.xf || a.thermo || a.scheme == KMPM_SCHEME_USF)) return KMPM_ERR_ARG;
   g.ntiles = 1;
   g.cpt = 1;
   g.bn = 1;
@@ -297,7 +306,8 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   drop_graphs(c);
   void *old[] = {c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                  c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst,
-                 c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask};
+                 c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask, c->d_gT, c->d_gTu,
+                 c->d_scratch_th};
   for (void *p : old)
     if (p) (void)hipFree(p);
   c->d_tile_np = c->d_tile_p0 = c->d_tile_cur = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
@@ -305,6 +315,7 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   c->d_scratch = nullptr;
   c->d_tinst = nullptr;
   c->d_gmass = c->d_gv = c->d_gvu = nullptr;
+  c->d_gT = c->d_gTu = c->d_scratch_th = nullptr;
   c->d_gmask = nullptr;
   // the grid, tile-major over whole tiles
   const int64_t ns = ncells;
@@ -328,6 +339,13 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   // P2G partial blocks: one per active tile, at most min(tiles, particles)
   const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(g.ntiles, a.np));
   HIPCHK(dalloc(&c->d_scratch, (size_t)slots * 7 * g.bn));
+  if (a.thermo) {   // grid temperatures and the thermal P2G's partial blocks
+    HIPCHK(dalloc(&c->d_gT, ns));
+    HIPCHK(dalloc(&c->d_gTu, ns));
+    HIPCHK(hipMemset(c->d_gT, 0, ns * sizeof(double)));
+    HIPCHK(hipMemset(c->d_gTu, 0, ns * sizeof(double)));
+    HIPCHK(dalloc(&c->d_scratch_th, (size_t)slots * 3 * g.bn));
+  }
   if (ninst) {
     HIPCHK(dalloc(&c->d_tinst, g.ntiles));
     HIPCHK(hipMemcpy(c->d_tinst, map.data(), g.ntiles * sizeof(int), hipMemcpyHostToDevice));
@@ -353,6 +371,9 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   a.dil_list = c->d_dil_list;
   a.scratch = c->d_scratch;
   a.scratch_slots = (int)slots;
+  a.gT = c->d_gT;
+  a.gTu = c->d_gTu;
+  a.scratch_th = c->d_scratch_th;
   a.tinst = c->d_tinst;
   c->keys_valid = false;
   c->sorted = false;
@@ -385,6 +406,10 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   // the reference-order kernels implement the PIC/FLIP transfers of USL
   // and MUSL
   if (cfg->order == KMPM_ORDER_REFERENCE && (apic || cfg->scheme == KMPM_SCHEME_USF))
+    return KMPM_ERR_ARG;
+  // thermo-mechanical runs: FP64, the fast order, the PIC/FLIP transfers
+  if (cfg->thermo && (cfg->precision != KMPM_PRECISION_FP64 ||
+                      cfg->order != KMPM_ORDER_FAST || apic))
     return KMPM_ERR_ARG;
   if (cfg->precision != KMPM_PRECISION_FP64 && cfg->precision != KMPM_PRECISION_MIXED)
     return KMPM_ERR_ARG;
@@ -439,7 +464,8 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   if (sub == KMPM_SUB_APIC || (!a.tl && sub == KMPM_SUB_MLS)) a.flip = 0.0;   // ulmpm.cpp:81
   a.xf = !apic ? 0 : (!a.tl && sub == KMPM_SUB_MLS) ? 2 : 1;
   a.asflip = !a.tl && sub == KMPM_SUB_ASFLIP;
-  a.lay = apic ? LAY_AP : a.tl ? LAY_TL : LAY_UL;
+  a.thermo = cfg->thermo != 0;
+  a.lay = a.thermo ? LAY_TH : apic ? LAY_AP : a.tl ? LAY_TL : LAY_UL;
   if (apic) {
     const double inv2 = 1.0 / (cfg->cellsize * cfg->cellsize);
     a.di = cfg->shape == KMPM_SHAPE_CUBIC_SPLINE ? 3.0 * inv2
@@ -462,6 +488,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
                   m.strength == KMPM_STRENGTH_FLUID && m.damage == KMPM_DAMAGE_NONE;
     }
     a.mat_class = all_lin ? MC_LINEAR : all_jc ? MC_JC_SHOCK : all_fluid ? MC_FLUID : MC_GENERIC;
+    if (a.thermo) a.mat_class = MC_GENERIC;   // the temperature-aware update is the generic one
   }
   // material table
   std::vector<DevMat> dm(nmat);
@@ -506,7 +533,8 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
   void *ptrs[] = {c->d_mats, c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
                   c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst, c->d_red,
-                  c->d_fexpr, c->d_cut, c->d_cut_stage};
+                  c->d_fexpr, c->d_cut, c->d_cut_stage, c->d_gT, c->d_gTu,
+                  c->d_scratch_th};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   free(c->mats_host);
@@ -535,9 +563,9 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
       return KMPM_ERR_ARG;  // ascending tags, ordered by solid
   }
   KArgs &a = c->a;
-  if (a.tl)
+  if (a.tl || a.thermo)   // one solid: one TL grid; the thermal sums of one solid
     for (int64_t i = 1; i < np; i++)
-      if (p->solid[i] != p->solid[0]) return KMPM_ERR_ARG;   // one solid: one TL grid
+      if (p->solid[i] != p->solid[0]) return KMPM_ERR_ARG;
   const int cap = (int)std::max<int64_t>(np, 1);
   const bool mixed = a.prec == KMPM_PRECISION_MIXED;
   const int ndf = state_nd(mixed, a.lay);
@@ -579,6 +607,7 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
       pk1(p->vol0[i], m3_det(F), R, s, Finv, P);
       for (int e = 0; e < 9; e++) D(P00 + e, i) = P[e];
     }
+    if (a.thermo) D(TEMP, i) = p->T ? p->T[i] : 0.0;   // gamma = q = 0 (solid.cpp:2305-2309)
     hi[(size_t)PID * cap + i] = (int)i;
     hi[(size_t)MATI * cap + i] = p->mat[i];
   }
@@ -771,6 +800,8 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
     if (p->eff_plastic_strain_rate) p->eff_plastic_strain_rate[q] = D(EPDOT, i);
     if (p->damage) p->damage[q] = D(DMG, i);
     if (p->damage_init) p->damage_init[q] = D(DINIT, i);
+    if (p->T) p->T[q] = c->a.thermo ? D(TEMP, i) : 0.0;
+    if (p->gamma) p->gamma[q] = c->a.thermo ? D(GAMMA, i) : 0.0;
   }
   return KMPM_OK;
 }
@@ -876,6 +907,9 @@ int kmpm_particles_to_grid(kmpm_ctx *c) {
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G);
   HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 0) : launch_p2g(c, 0));
+  // the grid temperature's sums and the thermal driving forces
+  // (ulmpm.cpp:371-375, tlmpm.cpp:359-363)
+  if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 0));
   tend(c);
   return KMPM_OK;
 }
@@ -887,6 +921,9 @@ int kmpm_update_grid_state(kmpm_ctx *c) {
   // USF: the first pass's mass and velocity with particles_to_grid_USF_2's forces
   const int mode = c->cfg.scheme == KMPM_SCHEME_USF ? GM_USF2 : GM_STEP;
   HIPCHK(c->a.ref_order ? launch_ref_grid(c, 0) : launch_grid_update(c, mode));
+  // Grid::update_grid_temperature (grid.cpp:1354-1362), with the node mass
+  // of this pass
+  if (c->a.thermo) HIPCHK(launch_grid_thermal(c, TM_FULL));
   tend(c);
   return KMPM_OK;
 }
@@ -900,9 +937,11 @@ int kmpm_particles_to_grid_usf_1(kmpm_ctx *c) {
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G);
   HIPCHK(launch_p2g(c, 0));
+  if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 0));   // usf.cpp:53 (grid temperature)
   tend(c);
   tbegin(c, KMPM_PHASE_GRID_B);
   HIPCHK(launch_grid_update(c, GM_USF1));
+  if (c->a.thermo) HIPCHK(launch_grid_thermal(c, TM_T));
   tend(c);
   return KMPM_OK;
 }
@@ -915,6 +954,8 @@ int kmpm_particles_to_grid_usf_2(kmpm_ctx *c) {
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G_B);
   HIPCHK(launch_p2g(c, 0));
+  // the thermal driving forces of the updated heat source and flux
+  if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 0));
   tend(c);
   return KMPM_OK;
 }
@@ -962,9 +1003,13 @@ int kmpm_velocities_to_grid(kmpm_ctx *c) {
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G_B);
   HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 1) : launch_p2g(c, 1));
+  // the grid temperature of the updated particle temperatures
+  // (ulmpm.cpp:491-493)
+  if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 1));
   tend(c);
   tbegin(c, KMPM_PHASE_GRID_B);
   HIPCHK(c->a.ref_order ? launch_ref_grid(c, 1) : launch_grid_update(c, 1));
+  if (c->a.thermo) HIPCHK(launch_grid_thermal(c, TM_T));
   tend(c);
   return KMPM_OK;
 }
@@ -1026,7 +1071,7 @@ int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
 int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
   if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
   if (ninst > 0 && (c->a.tl || c->a.ref_order || c->a.surr_on || c->a.cut_on || c->a.xf ||
-                    c->a.scheme == KMPM_SCHEME_USF))
+                    c->a.thermo || c->a.scheme == KMPM_SCHEME_USF))
     return KMPM_ERR_ARG;
   for (int q = 0; q < c->a.nfv; q++)
     if (ninst > 0 && c->a.fv[q].expr) return KMPM_ERR_ARG;   // values of the context's time
@@ -1371,6 +1416,27 @@ int kmpm_download_grid(kmpm_ctx *c, double *mass, double *v, double *vu, int64_t
   return KMPM_OK;
 }
 
+}  // extern "C"
+
+// The grid temperature in tag order (thermo-mechanical contexts; zeros
+// otherwise), for the grid dump's T column.
+int kmpm::download_grid_temperature(kmpm_ctx *c, double *T, int64_t nn) {
+  if (!c || !T || nn != c->a.nn) return KMPM_ERR_ARG;
+  if (!c->a.thermo) {
+    std::fill(T, T + nn, 0.0);
+    return KMPM_OK;
+  }
+  HIPCHK(launch_grid_clean(c));
+  std::vector<double> raw(c->a.ns);
+  HIPCHK(hipMemcpyAsync(raw.data(), c->d_gT, c->a.ns * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < nn; i++) T[i] = raw[host_sidx(c->a, i)];
+  return KMPM_OK;
+}
+
+extern "C" {
+
 int kmpm_download_cells(kmpm_ctx *c, int32_t *cell_ijk, int64_t *sorted_ptags, int64_t np) {
   if (!c || np != c->a.np) return KMPM_ERR_ARG;
   if (cell_ijk) {
@@ -1452,7 +1518,7 @@ int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
   // surrogate's JC + shock material; one instance (the constitutive kernel
   // reduces the context's CFL)
   if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
-      c->a.ninst || c->a.xf)
+      c->a.ninst || c->a.xf || c->a.thermo)
     return KMPM_ERR_ARG;
   for (int i = 0; i < c->nmat; i++)
     if (!surrogate_matches(s, c->mats_host[i])) return KMPM_ERR_ARG;
@@ -1479,7 +1545,7 @@ int kmpm_ctx_set_cutnet(kmpm_ctx *c, const kmpm_cutnet *s) {
   // one network serves every particle of the wave: one material, the cut
   // network's; USL on one instance of the mixed ULMPM path
   if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
-      c->a.ref_order || c->a.ninst || c->a.xf || c->cfg.scheme != KMPM_SCHEME_USL ||
+      c->a.ref_order || c->a.ninst || c->a.xf || c->a.thermo || c->cfg.scheme != KMPM_SCHEME_USL ||
       c->nmat != 1 ||
       !cutnet_matches(s, c->mats_host[0]))
     return KMPM_ERR_ARG;

@@ -49,12 +49,27 @@ __device__ inline T norm_sym_t(const T *s) {
 template <class T>
 __device__ inline T pow_pos(T x, T y) { return m_exp(y * m_log(x)); }
 
+// Thermal inputs and output of one particle's stress update on a
+// thermo-mechanical context (FP64): the particle temperature T, whether the
+// method is thermo-mechanical (the damage then sees T, solid.cpp:1331-1341),
+// the volume that scales the heat source (vol, or vol0 on TLMPM) and the
+// heat source gamma (solid.cpp:1343-1351).
+struct ThermalPt {
+  double T;
+  int method_temp;
+  double volg;
+  double gamma;
+};
+
 // StrengthJohnsonCook::update_deviatoric_stress, strength_jc.cpp:100-184 (T = 0).
 // out = final deviatoric stress, *dps = plastic strain increment (untouched
 // when damage >= 1, the caller holds 0).
+// hot: the material's temperature law is on and Tt is the particle's
+// temperature (strength_jc.cpp:127-133 evaluated per particle).
 template <class T>
 __device__ inline void jc_strength(const DevMat &m, const T *sig, const T *D, T dt, T eps_p,
-                                   T epsdot, T damage, T *out, T *dps) {
+                                   T epsdot, T damage, T *out, T *dps, bool hot = false,
+                                   double Tt = 0.0) {
   const T G_ = (T)m.strength_p[0], A = (T)m.strength_p[1], B = (T)m.strength_p[2],
           n = (T)m.strength_p[3], epsdot0 = (T)m.strength_p[4], C = (T)m.strength_p[5];
   if (damage >= T(1.0)) {
@@ -69,7 +84,16 @@ __device__ inline void jc_strength(const DevMat &m, const T *sig, const T *D, T 
   if (C != T(0)) ys *= pow_pos(T(1.0) + epsdot_ratio, C);   // base >= 2
   // Thermal softening: with no temperature law T = 0 and the factor
   // (1 - ((T-Tr)/Tmr)^m, 1, or 0 above Tm) depends on the material only.
-  ys *= (T)m.jc_thermal;
+  if (hot) {
+    const double mm = m.strength_p[6], Tr = m.strength_p[7], Tm = m.strength_p[8];
+    if (Tt < Tm) {
+      if (mm != 0 && Tt >= Tr) ys *= (T)(1.0 - pow((Tt - Tr) / (Tm - Tr), mm));
+    } else {
+      ys = 0;
+    }
+  } else {
+    ys *= (T)m.jc_thermal;
+  }
   T Gd = G_;
   if (damage > T(0)) {
     Gd *= (T(1) - damage);
@@ -93,12 +117,14 @@ __device__ inline void jc_strength(const DevMat &m, const T *sig, const T *D, T 
   }
 }
 
-// EOSShock::compute_pressure, eos_shock.cpp:99-133 (T = 0).
+// EOSShock::compute_pressure, eos_shock.cpp:99-133 at temperature Tt (0
+// without a temperature law).
 template <class T>
-__device__ inline T shock_pressure(const DevMat &m, T J, T rho, T damage, const T *D, T cellsize) {
+__device__ inline T shock_pressure(const DevMat &m, T J, T rho, T damage, const T *D, T cellsize,
+                                   T Tt = 0) {
   const T rho0_ = (T)m.eos_p[0], c0 = (T)m.eos_p[2], S = (T)m.eos_p[3], Gamma = (T)m.eos_p[4],
           cv = (T)m.eos_p[5], Tr = (T)m.eos_p[6], Q1 = (T)m.eos_p[7], Q2 = (T)m.eos_p[8];
-  const T alpha = cv * rho0_, e0 = 0, Tt = 0;
+  const T alpha = cv * rho0_, e0 = 0;
   const T mu = m_div(rho, rho0_) - T(1.0);
   const T q1 = T(1.0) - (S - T(1.0)) * mu;
   const T pH = m_div(rho0_ * (c0 * c0) * mu * (T(1.0) + mu), q1 * q1);
@@ -121,14 +147,15 @@ __device__ inline T shock_pressure(const DevMat &m, T J, T rho, T damage, const 
   return p;
 }
 
-// DamageJohnsonCook::compute_damage, damage_jc.cpp:91-143 (T = 0).
+// DamageJohnsonCook::compute_damage, damage_jc.cpp:91-143 at temperature Tt
+// (0 unless the method is thermo-mechanical).
 template <class T>
 __device__ inline void jc_damage(const DevMat &m, T &damage_init, T &damage, T pH, const T *sdev,
-                                 T epsdot, T dps) {
+                                 T epsdot, T dps, T Tt = 0) {
   const T d1 = (T)m.damage_p[0], d2 = (T)m.damage_p[1], d3 = (T)m.damage_p[2],
           d4 = (T)m.damage_p[3], d5 = (T)m.damage_p[4], epsdot0 = (T)m.damage_p[5],
           Tr = (T)m.damage_p[6], Tm = (T)m.damage_p[7];
-  const T Tmr = Tm - Tr, Tt = 0;
+  const T Tmr = Tm - Tr;
   if (dps == T(0) && damage >= T(1.0)) return;
   const T vm = T(SQRT_3_OVER_2) * norm_sym_t(sdev);
   T triax = T(0.0);
@@ -157,10 +184,14 @@ __device__ inline void jc_damage(const DevMat &m, T &damage_init, T &damage, T p
 // stay FP64 and so does the pressure: mu = rho / rho0 - 1 and (rho /
 // rho0)^Gamma - 1 cancel, and FP32 would keep only a few digits of a small
 // compression.
+// th (thermo-mechanical contexts): the particle's temperature state; a
+// material with a temperature law evaluates its EOS and strength at T, adds
+// the thermal pressure alpha (T0 - T) and sets the heat source
+// (solid.cpp:1301-1351).
 template <int MC, class T = double>
 __device__ inline void update_stress_point(const DevMat &m, T dt, T h, double J, double rho,
                                            const T *D, T *sig, T *eel, T &ep, T &epdot, T &dinit,
-                                           T &dmg) {
+                                           T &dmg, ThermalPt *th = nullptr) {
   const bool linear = MC == MC_LINEAR || (MC == MC_GENERIC && m.type == KMPM_MAT_LINEAR);
   if (linear) {   // :1268-1280
     T si[6];
@@ -179,21 +210,24 @@ __device__ inline void update_stress_point(const DevMat &m, T dt, T h, double J,
   const bool efluid = MC == MC_FLUID || (MC == MC_GENERIC && m.eos == KMPM_EOS_FLUID);
   const bool jc = MC == MC_JC_SHOCK || (MC == MC_GENERIC && m.strength == KMPM_STRENGTH_JOHNSON_COOK);
   const bool sfluid = MC == MC_FLUID || (MC == MC_GENERIC && m.strength == KMPM_STRENGTH_FLUID);
+  const bool hot = th != nullptr && m.temp_on;
+  const double Tc = hot ? th->T : 0.0;
   double pHd;
   if (shock) {
     double Dd[6];
     for (int e = 0; e < 6; e++) Dd[e] = D[e];
-    pHd = shock_pressure<double>(m, J, rho, dmg, Dd, h);
+    pHd = shock_pressure<double>(m, J, rho, dmg, Dd, h, Tc);
   } else if (efluid) {                               // eos_fluid.cpp:68-73
     const double mu = rho / m.eos_p[0];
     pHd = m.eos_p[1] * (pow(mu, m.eos_p[2]) - 1.0);
   } else {                                           // eos_linear.cpp:71-74
     pHd = m.eos_p[1] * (1 - J) * (1 - (double)dmg);
   }
+  if (hot) pHd += m.talpha * (m.T0 - th->T);        // temperature_plastic_work.cpp:74-76
   const T pH = (T)pHd;
   T sdev[6], dps = 0;
   if (jc) {
-    jc_strength(m, sig, D, dt, ep, epdot, dmg, sdev, &dps);
+    jc_strength(m, sig, D, dt, ep, epdot, dmg, sdev, &dps, hot, Tc);
   } else if (sfluid) {                               // strength_fluid.cpp:46-58
     const T t3 = (D[0] + D[1] + D[2]) / T(3.0);
     const T c2 = T(2.0) * (T)m.strength_p[0];
@@ -217,7 +251,14 @@ __device__ inline void update_stress_point(const DevMat &m, T dt, T h, double J,
   epdot += m_div(dps, tav);
   epdot = epdot > T(0.0) ? epdot : T(0.0);
   if (MC != MC_FLUID && m.damage == KMPM_DAMAGE_JOHNSON_COOK)
-    jc_damage(m, dinit, dmg, pH, sdev, epdot, dps);
+    jc_damage(m, dinit, dmg, pH, sdev, epdot, dps,
+              (T)(th != nullptr && th->method_temp ? th->T : 0.0));
+  if (hot) {                                          // :1343-1351
+    const double flow = SQRT_3_OVER_2 * (double)norm_sym_t(sdev);
+    double g = th->T < m.Tmelt ? m.chi * flow * (double)epdot : 0.0;   // :65-71
+    g *= th->volg * m.invcp;
+    th->gamma = g;
+  }
   const T trD = D[0] + D[1] + D[2];
   const T trE = eel[0] + eel[1] + eel[2];
   if (dmg == T(0) || pH >= T(0)) {                  // :1353-1356

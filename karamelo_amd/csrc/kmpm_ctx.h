@@ -74,6 +74,10 @@ struct KArgs {
   // FP32 values in the same allocations
   int64_t ns;
   double *gmass, *gv, *gvu;   // gv/gvu: 3 * ns
+  // thermo-mechanical (FP64): grid T and T_update [ns], and the thermal P2G's
+  // partial blocks [slot][3][bn] (sum w m T, sum w gamma, sum grad w . q)
+  int thermo;
+  double *gT, *gTu, *scratch_th;
   uint32_t *gmask;
   // fixes
   int nfv, nbf;
@@ -131,6 +135,7 @@ struct kmpm_ctx {
   int *d_tile_list, *d_tile_slot, *d_dil_list;
   double *d_scratch;
   double *d_gmass, *d_gv, *d_gvu;
+  double *d_gT, *d_gTu, *d_scratch_th;
   uint32_t *d_gmask;
   uint32_t *d_pmask;     // particle group masks (by pid)
   kmpm_expr *d_fexpr;    // [MAX_FIX][6]
@@ -182,12 +187,18 @@ static inline bool learned_constitutive(const kmpm_ctx *c) {
 // velocity after its first P2G (GM_USF1) and the update with the forces of
 // its second P2G (GM_USF2).
 enum GridMode { GM_STEP = 0, GM_REMAP = 1, GM_USF1 = 2, GM_USF2 = 3 };
+// Passes of the grid temperature kernel (thermo-mechanical runs): T, the
+// driving forces and T_update (TM_FULL), or T alone (TM_T: the MUSL re-map
+// and USF's first pass).
+enum ThermalMode { TM_FULL = 0, TM_T = 1 };
 
 // launchers (kmpm_kernels.hip); all stream-ordered, return hipError_t
 hipError_t launch_keys(kmpm_ctx *c);
 hipError_t launch_sort(kmpm_ctx *c);
 hipError_t launch_p2g(kmpm_ctx *c, int momentum_only);
 hipError_t launch_grid_update(kmpm_ctx *c, int mode);   // GridMode (kmpm_kernels.h)
+hipError_t launch_p2g_thermal(kmpm_ctx *c, int remap);
+hipError_t launch_grid_thermal(kmpm_ctx *c, int mode);  // ThermalMode
 hipError_t launch_fix_values(kmpm_ctx *c);   // expression-valued fixes, before a grid pass
 hipError_t launch_g2p_usl(kmpm_ctx *c);
 hipError_t launch_g2p_musl_a(kmpm_ctx *c);
@@ -195,6 +206,7 @@ hipError_t launch_stress_musl_b(kmpm_ctx *c);
 hipError_t launch_learned_stress(kmpm_ctx *c);   // kmpm_cutnet.hip
 hipError_t launch_adjust_dt(kmpm_ctx *c);
 hipError_t launch_grid_clean(kmpm_ctx *c);
+int download_grid_temperature(kmpm_ctx *c, double *T, int64_t nn);   // tag order
 hipError_t launch_energy(kmpm_ctx *c, uint32_t bit, double *d_out2);
 hipError_t launch_debug_weights(kmpm_ctx *c, int64_t *d_tag, double *d_w, double *d_wd);
 hipError_t launch_cells(kmpm_ctx *c, int *d_ijk);

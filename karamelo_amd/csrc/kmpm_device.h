@@ -41,6 +41,9 @@ enum ApicField {
   C00 = NDF_TL, C01, C02, C10, C11, C12, C20, C21, C22,
   NDF_AP
 };
+// Thermo-mechanical contexts (FP64): temperature, heat source and heat flux
+// (solid.h: T, gamma, q), after every other field.
+enum ThermField { TEMP = NDF_AP, GAMMA, QX, QY, QZ, NDF_TH };
 // Int fields: PID = index into the static (upload-order) arrays, MAT = material.
 enum IField { PID = 0, MATI, NIF };
 
@@ -67,12 +70,14 @@ __host__ __device__ constexpr int mx_fslot(int f) {
 }
 static_assert(mx_fslot(VOL) == MX_NF - 1 && mx_dslot(F22) == MX_ND - 1, "mixed layout");
 static_assert(mx_fslot(C00) == MX_NF_TL && mx_fslot(C22) == MX_NF_AP - 1, "mixed APIC layout");
-// State layouts: ULMPM, TLMPM, and the APIC layout (TLMPM's fields and C).
-enum Layout { LAY_UL = 0, LAY_TL = 1, LAY_AP = 2 };
+// State layouts: ULMPM, TLMPM, the APIC layout (TLMPM's fields and C), and
+// the thermo-mechanical layout (all of those and the thermal fields; FP64
+// only).
+enum Layout { LAY_UL = 0, LAY_TL = 1, LAY_AP = 2, LAY_TH = 3 };
 // Field counts of the double and float blocks per layout and precision.
 __host__ __device__ constexpr int state_nd(bool mixed, int lay) {
   return mixed ? (lay != LAY_UL ? MX_ND_TL : MX_ND)
-               : (lay == LAY_UL ? NDF : lay == LAY_TL ? NDF_TL : NDF_AP);
+               : (lay == LAY_UL ? NDF : lay == LAY_TL ? NDF_TL : lay == LAY_AP ? NDF_AP : NDF_TH);
 }
 __host__ __device__ constexpr int state_nf(bool mixed, int lay) {
   return mixed ? (lay == LAY_UL ? MX_NF : lay == LAY_TL ? MX_NF_TL : MX_NF_AP) : 0;
@@ -100,6 +105,11 @@ struct DevMat {
   double eos_p[9];
   double strength_p[9];
   double damage_p[8];
+  // temperature law (material.cpp:726-742): temp_on when cp != 0, then
+  // chi, 1/cp, kappa, the thermal-pressure alpha, T0 and Tm of
+  // temperature_plastic_work.cpp:50-55
+  int temp_on, pad_t;
+  double chi, invcp, kappa, talpha, T0, Tmelt;
 };
 
 // Material classes the stress kernels are specialised for (chosen on the

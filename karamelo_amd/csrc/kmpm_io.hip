@@ -67,7 +67,7 @@ struct Host {
   std::vector<int32_t> solid, mat;
   std::vector<uint32_t> mask;
   std::vector<double> x0, x, v, F, sigma, strain_el, mass, vol0, vol, rho0, rho, J, ep, epdot,
-      damage, damage_init;
+      damage, damage_init, T, gamma;
   kmpm_particles p{};
   explicit Host(int64_t n) {
     auto z = [n](std::vector<double> &a, int c) { a.assign((size_t)(n * c), 0.0); };
@@ -77,7 +77,7 @@ struct Host {
     mask.assign(n, 0);
     z(x0, 3); z(x, 3); z(v, 3); z(F, 9); z(sigma, 9); z(strain_el, 9);
     z(mass, 1); z(vol0, 1); z(vol, 1); z(rho0, 1); z(rho, 1); z(J, 1);
-    z(ep, 1); z(epdot, 1); z(damage, 1); z(damage_init, 1);
+    z(ep, 1); z(epdot, 1); z(damage, 1); z(damage_init, 1); z(T, 1); z(gamma, 1);
     p.np = n;
     p.ptag = ptag.data(); p.solid = solid.data(); p.mat = mat.data(); p.mask = mask.data();
     p.x0 = x0.data(); p.x = x.data(); p.v = v.data(); p.F = F.data(); p.sigma = sigma.data();
@@ -85,6 +85,7 @@ struct Host {
     p.vol = vol.data(); p.rho0 = rho0.data(); p.rho = rho.data(); p.J = J.data();
     p.eff_plastic_strain = ep.data(); p.eff_plastic_strain_rate = epdot.data();
     p.damage = damage.data(); p.damage_init = damage_init.data();
+    p.T = T.data(); p.gamma = gamma.data();
   }
 };
 
@@ -257,9 +258,11 @@ int kmpm_dump_particles(kmpm_ctx *c, const char *filename, const char *const *va
         else if (v == "bx" || v == "by" || v == "bz") put(s, 0.0);
         else if (v == "ep") put(s, h.ep[i]);
         else if (v == "epdot") put(s, h.epdot[i]);
-        // no temperature law: T, gamma and the internal energy are 0
+        // T and gamma: 0 without a temperature law; the internal energy is
+        // not integrated (0, as the reference leaves it without one)
         else if (v == "ienergy") put(s, 0.0);
-        else if (v == "T" || v == "gamma") s += "0 ";
+        else if (v == "T") put(s, h.T[i]);
+        else if (v == "gamma") put(s, h.gamma[i]);
       }
       s += "\n";
     }
@@ -272,9 +275,10 @@ int kmpm_dump_grid(kmpm_ctx *c, const char *filename, const char *const *vars, i
     if (!vars[k] || !in_list(vars[k], GRID_VARS, (int)(sizeof GRID_VARS / sizeof *GRID_VARS)))
       return KMPM_ERR_ARG;   // dump_grid.cpp:30-42
   const int64_t nn = c->a.nn;
-  std::vector<double> mass(nn), v(3 * nn);
+  std::vector<double> mass(nn), v(3 * nn), T(nn);
   int st = kmpm_download_grid(c, mass.data(), v.data(), nullptr, nn);
   if (st) return st;
+  if ((st = download_grid_temperature(c, T.data(), nn))) return st;
   int64_t step = 0;
   if ((st = kmpm_get_time(c, nullptr, nullptr, &step))) return st;
   const bool tl = c->cfg.method == KMPM_METHOD_TLMPM;
@@ -316,7 +320,9 @@ int kmpm_dump_grid(kmpm_ctx *c, const char *filename, const char *const *vars, i
         put(s, mass[tag]);
       } else if (vv == "mask") {
         put_i(s, (int64_t)mask);
-      } else if (vv == "rigid" || vv == "T") {
+      } else if (vv == "T") {
+        put(s, T[tag]);
+      } else if (vv == "rigid") {
         s += "0 ";
       } else {   // ntype (grid.cpp:228-240)
         int64_t t = 0;
@@ -347,7 +353,7 @@ int kmpm_write_restart(kmpm_ctx *c, const char *filename) {
   wr(o, (int32_t)cf.dimension);
   // ---- Update::write_restart (update.cpp:220-280) ----
   wr_str(o, tl ? "tlmpm" : "ulmpm");
-  wr(o, (bool)false);                                  // method->temp
+  wr(o, (bool)(cf.thermo != 0));                       // method->temp
   wr_str(o, scheme_key(cf.scheme));
   wr(o, restart_sub_method(cf));
   wr(o, c->a.flip);                                    // PIC_FLIP after the method's setup
@@ -413,7 +419,8 @@ int kmpm_write_restart(kmpm_ctx *c, const char *filename) {
       wr(o, h.epdot[i]);
       wr(o, h.damage[i]);
       wr(o, h.damage_init[i]);
-      wr(o, 0.0);                                // ienergy (no temperature law)
+      if (cf.thermo) wr(o, h.T[i]);
+      wr(o, 0.0);                                // ienergy (not integrated)
       wr(o, (int32_t)h.mask[i]);
     }
   }
@@ -446,7 +453,8 @@ int kmpm_read_restart(kmpm_ctx *c, const char *filename) {
     return KMPM_ERR_ARG;
   // the file must describe this context's method, scheme and grid
   if (dim != cf.dimension || method != (tl ? "tlmpm" : "ulmpm") ||
-      scheme != scheme_key(cf.scheme) || temp || sub != restart_sub_method(cf) ||
+      scheme != scheme_key(cf.scheme) || temp != (cf.thermo != 0) ||
+      sub != restart_sub_method(cf) ||
       shape != (cf.shape == KMPM_SHAPE_CUBIC_SPLINE ? 1 : 0) || flip != c->a.flip)
     return KMPM_ERR_ARG;
   double box[6];
@@ -458,7 +466,7 @@ int kmpm_read_restart(kmpm_ctx *c, const char *filename) {
     int64_t ptag;
     int32_t solid, mat;
     uint32_t mask;
-    double x0[3], x[3], v[3], sig[9], eel[9], F[9], J, vol0, rho0, ep, epdot, dmg, dinit;
+    double x0[3], x[3], v[3], sig[9], eel[9], F[9], J, vol0, rho0, ep, epdot, dmg, dinit, T;
   };
   std::vector<Row> rows;
   for (int32_t is = 0; is < nsol; is++) {
@@ -485,8 +493,9 @@ int kmpm_read_restart(kmpm_ctx *c, const char *filename) {
       ok = ok && rd_mat(in, r.sig) && rd_mat(in, r.eel);
       if (tl) ok = ok && rd_mat(in, pk);
       ok = ok && rd_mat(in, r.F) && rd(in, r.J) && rd(in, r.vol0) && rd(in, r.rho0) &&
-           rd(in, r.ep) && rd(in, r.epdot) && rd(in, r.dmg) && rd(in, r.dinit) && rd(in, ien) &&
-           rd(in, mask);
+           rd(in, r.ep) && rd(in, r.epdot) && rd(in, r.dmg) && rd(in, r.dinit);
+      if (temp) ok = ok && rd(in, r.T);
+      ok = ok && rd(in, ien) && rd(in, mask);
       if (!ok) return KMPM_ERR_ARG;
       r.mask = (uint32_t)mask;
       rows.push_back(r);
@@ -519,6 +528,7 @@ int kmpm_read_restart(kmpm_ctx *c, const char *filename) {
     hb.epdot[i] = r.epdot;
     hb.damage[i] = r.dmg;
     hb.damage_init[i] = r.dinit;
+    hb.T[i] = r.T;   // gamma and the heat flux restart from 0
   }
   int st = kmpm_upload_particles(c, &hb.p);
   if (st) return st;

@@ -454,7 +454,15 @@ __device__ inline unsigned long long fx_word_fast(float val, double scale) {
 // replaced by its g.  MLS (XF 2, solid.cpp:524-574): the force
 // -vol w sigma Di (x_I - x_p) is the gradient form with dw replaced by g and
 // sigma scaled by Di, so the stage holds (w, g) in place of (w, dw).
-template <int DIM, int S1, int TX, int TY, int TZ, bool REMAP, int PREC, bool TL, int XF = 0>
+//
+// TH: the thermal transfer of a thermo-mechanical run (FP64, XF 0) into its
+// own partial blocks a.scratch_th [slot][3][bn]: sum w m T (the grid
+// temperature's numerator, solid.cpp:2743-2762), sum w gamma (the external
+// driving force, :2764-2779) and sum grad w . q (the internal one,
+// :2781-2797); the MUSL re-map (REMAP) sums w m T of the updated
+// temperatures only (ulmpm.cpp:491-493).
+template <int DIM, int S1, int TX, int TY, int TZ, bool REMAP, int PREC, bool TL, int XF = 0,
+          bool TH = false>
 __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   if (frozen(a)) return;
   using R = typename Real<PREC>::T;
@@ -467,9 +475,11 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   constexpr int CHK = XF ? 64 : 128;             // particles staged per chunk
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int NSIG = p2g_nsig<DIM, TL>();
-  constexpr int NF = REMAP ? (1 + DIM) : (1 + 2 * DIM);
+  static_assert(!TH || (XF == 0 && PREC == KMPM_PRECISION_FP64), "thermal transfer: FP64");
+  constexpr int NF = TH ? (REMAP ? 1 : 3) : REMAP ? (1 + DIM) : (1 + 2 * DIM);
   constexpr int PC = 1 + DIM + (REMAP ? 0 : NSIG);         // payload offset of m C
-  constexpr int NPAY = PC + (XF ? DIM * DIM : 0);          // m, m v, stress terms, m C
+  // m, m v, stress terms, m C; thermal: m T, gamma, q
+  constexpr int NPAY = TH ? (REMAP ? 1 : 2 + DIM) : PC + (XF ? DIM * DIM : 0);
   // APIC with the gradient force needs dw and g: g in its own table
   constexpr bool GT = XF == 1 && !REMAP;
   constexpr int PSB = DIM == 3 ? EZ : 1;   // node block strides (last axis contiguous)
@@ -532,7 +542,23 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
       // ---- stage: one thread per particle ----
       if (tid < n) {
         P2GIn<DIM, REMAP, TL, R, XF> in;
-        p2g_load<DIM, REMAP, TL, PREC>(a, c0 + tid, in);
+        double thp[TH ? 2 + DIM : 1];   // thermal payload: m T, gamma, q
+        if constexpr (TH) {
+          const int j = c0 + tid;
+          if (REMAP) {   // the updated temperatures at the step's first positions
+            for (int d = 0; d < DIM; d++)
+              in.x[d] = TL ? LDB(X0X + d, j) : a.xo[(size_t)d * a.cap + j];
+            thp[0] = LDB(MASS, j) * LDB(TEMP, j);
+          } else {
+            const int slot = a.order[j];
+            for (int d = 0; d < DIM; d++) in.x[d] = LDA((TL ? X0X : PX) + d, slot);
+            thp[0] = LDA(MASS, slot) * LDA(TEMP, slot);
+            thp[1] = LDA(GAMMA, slot);
+            for (int d = 0; d < DIM; d++) thp[2 + d] = LDA(QX + d, slot);
+          }
+        } else {
+          p2g_load<DIM, REMAP, TL, PREC>(a, c0 + tid, in);
+        }
         int cl[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
@@ -559,12 +585,16 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
         }
         const int cell = DIM == 3 ? cl[0] * PSA + cl[1] * PSB + cl[2] : cl[0] * PSA + cl[1];
         scl[tid] = cell;
-        spay[tid][0] = in.m;
-        for (int d = 0; d < DIM; d++) spay[tid][1 + d] = in.m * in.v[d];
-        if (!REMAP)
-          for (int e = 0; e < NSIG; e++) spay[tid][1 + DIM + e] = in.s[e];
-        if constexpr (XF > 0)
-          for (int e = 0; e < DIM * DIM; e++) spay[tid][PC + e] = in.m * in.c[e];
+        if constexpr (TH) {
+          for (int e = 0; e < NPAY; e++) spay[tid][e] = thp[e];
+        } else {
+          spay[tid][0] = in.m;
+          for (int d = 0; d < DIM; d++) spay[tid][1 + d] = in.m * in.v[d];
+          if (!REMAP)
+            for (int e = 0; e < NSIG; e++) spay[tid][1 + DIM + e] = in.s[e];
+          if constexpr (XF > 0)
+            for (int e = 0; e < DIM * DIM; e++) spay[tid][PC + e] = in.m * in.c[e];
+        }
       }
       __syncthreads();
       // cells are sorted inside the tile: a segment starts where the cell changes
@@ -612,47 +642,68 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
             if constexpr (XF > 0) cg[0] = GT ? stg[i][sa] : px.y;
           }
           const R *p = spay[i];
-          const R mc = p[0] * c;
-          R mvc[DIM], fa[DIM], fb[DIM], mvb[DIM];
-#pragma unroll
-          for (int d = 0; d < DIM; d++) mvc[d] = p[1 + d] * c;
-          if constexpr (XF > 0) {
-            // m C (x_I - x_p) w: the non-last axes fold into the column's
-            // momentum, the last axis pairs with its g below
-            const R *mC = p + PC;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-              R u = mC[d * DIM] * cg[0];
-              if constexpr (DIM == 3) u = fma(mC[d * DIM + 1], cg[DIM - 2], u);
-              mvc[d] += u;
-              mvb[d] = mC[d * DIM + DIM - 1] * c;
+          if constexpr (TH) {
+            // w m T, w gamma and grad w . q (the last axis' factor below)
+            const R mtc = p[0] * c;
+            R gc = 0, qa = 0, qb = 0;
+            if constexpr (!REMAP) {
+              gc = p[1] * c;
+              qa = p[2] * ce[0];
+              if constexpr (DIM == 3) qa = fma(p[3], ce[DIM - 2], qa);
+              qb = p[1 + DIM] * c;
             }
-          }
-          if (!REMAP) {
-            const R *sg = p + 1 + DIM;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-              R u = p2g_stress<DIM, TL>(sg, d, 0) * ce[0];
-              if constexpr (DIM == 3) u = fma(p2g_stress<DIM, TL>(sg, d, 1), ce[DIM - 2], u);
-              fa[d] = u;
-              fb[d] = p2g_stress<DIM, TL>(sg, d, DIM - 1) * c;
+            for (int k = 0; k < S1; k++) {
+              const R2 wz = stw[i][(DIM - 1) * S1 + k];
+              acc[k][0] = fma(mtc, wz.x, acc[k][0]);
+              if constexpr (!REMAP) {
+                acc[k][1] = fma(gc, wz.x, acc[k][1]);
+                acc[k][2] = fma(qb, wz.y, fma(qa, wz.x, acc[k][2]));
+              }
             }
-          }
+          } else {
+            const R mc = p[0] * c;
+            R mvc[DIM], fa[DIM], fb[DIM], mvb[DIM];
 #pragma unroll
-          for (int k = 0; k < S1; k++) {
-            const R2 wz = stw[i][(DIM - 1) * S1 + k];
-            acc[k][0] = fma(mc, wz.x, acc[k][0]);
-#pragma unroll
-            for (int d = 0; d < DIM; d++) acc[k][1 + d] = fma(mvc[d], wz.x, acc[k][1 + d]);
+            for (int d = 0; d < DIM; d++) mvc[d] = p[1 + d] * c;
             if constexpr (XF > 0) {
-              const R gz = GT ? stg[i][(DIM - 1) * S1 + k] : wz.y;
+              // m C (x_I - x_p) w: the non-last axes fold into the column's
+              // momentum, the last axis pairs with its g below
+              const R *mC = p + PC;
 #pragma unroll
-              for (int d = 0; d < DIM; d++) acc[k][1 + d] = fma(mvb[d], gz, acc[k][1 + d]);
+              for (int d = 0; d < DIM; d++) {
+                R u = mC[d * DIM] * cg[0];
+                if constexpr (DIM == 3) u = fma(mC[d * DIM + 1], cg[DIM - 2], u);
+                mvc[d] += u;
+                mvb[d] = mC[d * DIM + DIM - 1] * c;
+              }
             }
-            if (!REMAP)
+            if (!REMAP) {
+              const R *sg = p + 1 + DIM;
 #pragma unroll
-              for (int d = 0; d < DIM; d++)
-                acc[k][1 + DIM + d] = fma(fb[d], wz.y, fma(fa[d], wz.x, acc[k][1 + DIM + d]));
+              for (int d = 0; d < DIM; d++) {
+                R u = p2g_stress<DIM, TL>(sg, d, 0) * ce[0];
+                if constexpr (DIM == 3) u = fma(p2g_stress<DIM, TL>(sg, d, 1), ce[DIM - 2], u);
+                fa[d] = u;
+                fb[d] = p2g_stress<DIM, TL>(sg, d, DIM - 1) * c;
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < S1; k++) {
+              const R2 wz = stw[i][(DIM - 1) * S1 + k];
+              acc[k][0] = fma(mc, wz.x, acc[k][0]);
+#pragma unroll
+              for (int d = 0; d < DIM; d++) acc[k][1 + d] = fma(mvc[d], wz.x, acc[k][1 + d]);
+              if constexpr (XF > 0) {
+                const R gz = GT ? stg[i][(DIM - 1) * S1 + k] : wz.y;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) acc[k][1 + d] = fma(mvb[d], gz, acc[k][1 + d]);
+              }
+              if (!REMAP)
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                  acc[k][1 + DIM + d] = fma(fb[d], wz.y, fma(fa[d], wz.x, acc[k][1 + DIM + d]));
+            }
           }
         }
         const int base = scl[i0] + colofs;
@@ -693,6 +744,15 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
     }
     __syncthreads();
     constexpr int BN = EX * EY * EZ;
+    if constexpr (TH) {
+      double *tdst = a.scratch_th + (size_t)li * 3 * BN;
+      for (int qq = tid; qq < NF * BN; qq += 256) {
+        const int f = qq / BN, p = qq % BN;
+        tdst[qq] = blk[f * PBN + pinv[p]];
+      }
+      __syncthreads();
+      continue;
+    }
     BT *dst = reinterpret_cast<BT *>(a.scratch) + (size_t)li * SCR_FIELDS * BN;
     // blk fields: mass, mom[DIM], force[DIM] -> scratch fields mass, mom[3],
     // force[3]; components a 2D run does not have are written as zeros.
@@ -918,6 +978,84 @@ __global__ __launch_bounds__(256) void k_grid_update(KArgs a, int mode) {
   if (mode != GM_USF1) flush_fix_forces(a, fsum);   // GM_USF2 records the step's reaction
 }
 
+// The sub-blocks of the source tiles' partial blocks that hold node
+// (tile t = (tx, ty, tz), local (lx, ly, lz)): fn(slot, pos) per source,
+// pos the node's offset inside a field of the source's block.
+template <int DIM, int S1, int TX, int TY, int TZ, class Fn>
+__device__ inline void for_node_sources(const KArgs &a, int tx, int ty, int tz, int lx, int ly,
+                                        int lz, Fn &&fn) {
+  using CX = AxisCls<S1, TX>;
+  using CY = AxisCls<S1, TY>;
+  using CZ = AxisCls<S1, TZ, DIM == 2>;
+  constexpr int EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  const int nt0 = a.tg.nt[0], nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+#pragma unroll
+  for (int kx = 0; kx < CX::n; kx++) {
+    const int ux = tx - CX::toff(kx), rx = lx - CX::lo_l(kx);
+    if (rx < 0 || rx >= CX::size(kx) || ux < 0 || ux >= nt0) continue;
+#pragma unroll
+    for (int ky = 0; ky < CY::n; ky++) {
+      const int uy = ty - CY::toff(ky), ry = ly - CY::lo_l(ky);
+      if (ry < 0 || ry >= CY::size(ky) || uy < 0 || uy >= nt1) continue;
+#pragma unroll
+      for (int kz = 0; kz < CZ::n; kz++) {
+        const int uz = tz - CZ::toff(kz), rz = lz - CZ::lo_l(kz);
+        if (rz < 0 || rz >= CZ::size(kz) || uz < 0 || uz >= nt2) continue;
+        const int slot = a.tile_slot[(ux * nt1 + uy) * nt2 + uz];
+        if (slot < 0) continue;
+        const int sx = CX::size(kx), sy = CY::size(ky), sz = CZ::size(kz);
+        fn(slot, CX::lo_b(kx) * EY * EZ + sx * CY::lo_b(ky) * EZ + sx * sy * CZ::lo_b(kz) +
+                     (rx * sy + ry) * sz + rz);
+      }
+    }
+  }
+}
+
+// Grid temperature of a thermo-mechanical run (FP64), after the grid
+// update of the same P2G: the thermal partial blocks summed per owned node,
+// then
+//   T = sum w m T / m where m > 0 (solid.cpp:2743-2762),
+//   Qext = sum w gamma where m > 0 (:2764-2779), Qint = sum grad w . q
+//   (:2781-2797), and T_update = T + dt (Qint + Qext) / m, or T where
+//   m = 0 (Grid::update_grid_temperature grid.cpp:1354-1362);
+// TM_T: T alone (the MUSL re-map, ulmpm.cpp:491-493; USF's first pass).
+template <int DIM, int S1, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_grid_thermal(KArgs a, int mode) {
+  if (frozen(a)) return;
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  constexpr int TZZ = DIM == 3 ? TZ : 1;
+  constexpr int NO = TX * TY * TZZ;
+  const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
+  const int total = a.sc->dil_count * NO;
+  const double dt_ctx = a.sc->dt;
+  const bool full = mode == TM_FULL;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const int t = a.dil_list[q / NO], o = q % NO;
+    const int tz = t % nt2, ty = (t / nt2) % nt1, tx = t / (nt2 * nt1);
+    const int lz = o % TZZ, ly = (o / TZZ) % TY, lx = o / (TZZ * TY);
+    const int i = tx * TX + lx, j = ty * TY + ly, k = DIM == 3 ? tz * TZ + lz : 0;
+    if (i >= a.n[0] || j >= a.n[1] || k >= a.n[2]) continue;
+    const int64_t sn = (int64_t)t * NO + o;   // tile-major storage index
+    double s[3] = {0, 0, 0};
+    for_node_sources<DIM, S1, TX, TY, TZ>(a, tx, ty, tz, lx, ly, lz, [&](int slot, int pos) {
+      const double *src = a.scratch_th + (size_t)slot * 3 * BN + pos;
+      s[0] += src[0];
+      if (full) {
+        s[1] += src[BN];
+        s[2] += src[2 * BN];
+      }
+    });
+    const double m = a.gmass[sn];
+    const double T = m > 0 ? s[0] / m : 0.0;
+    a.gT[sn] = T;
+    if (!full) continue;
+    const double qext = m > 0 ? s[1] : 0.0;
+    const double dt = tile_dt(a, t, dt_ctx);
+    a.gTu[sn] = m != 0 ? T + dt * (s[2] + qext) / m : T;
+  }
+}
+
 // Grid state for download: nodes outside every touched tile hold zeros
 // (and their velocity-fix values), as the reference's reset leaves them.
 template <int DIM, int TX, int TY, int TZ, int PREC>
@@ -959,6 +1097,7 @@ __global__ void k_grid_clean(KArgs a) {
       gv[d * a.ns + sn] = (R)v[d];
       gvu[d * a.ns + sn] = (R)vu[d];
     }
+    if (a.thermo) a.gT[sn] = a.gTu[sn] = 0;
   }
 }
 
@@ -1046,6 +1185,59 @@ __device__ inline void load_node_block(const KArgs &a, typename Real<PREC>::T *n
         nb[d * BN + q] = ok ? gv[d * a.ns + sn] : (R)0;
       }
     }
+  }
+}
+
+// Load a scalar node field of the grid (FP64, tile-major [ns]: the grid
+// temperature) into the tile's LDS block [BN].
+template <int DIM, int S1, int TX, int TY, int TZ>
+__device__ inline void load_node_scalar(const KArgs &a, const double *g, double *nf, int ox, int oy,
+                                        int oz) {
+  constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
+  constexpr int BN = EX * EY * EZ;
+  const int off = S1 == 4 ? 1 : 0;
+  for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+    const int bz = q % EZ, by = (q / EZ) % EY, bx = q / (EZ * EY);
+    const int i = ox - off + bx, j = oy - off + by, k = DIM == 3 ? oz - off + bz : 0;
+    const bool ok = i >= 0 && j >= 0 && k >= 0 && i < a.n[0] && j < a.n[1] && k < a.n[2];
+    nf[q] = ok ? g[node_sidx<TX, TY, DIM == 3 ? TZ : 1>(a, i, j, k)] : 0.0;
+  }
+}
+
+// Gather of a scalar node field from its LDS block: s = sum w f and
+// g = sum grad w f, contracted one axis at a time as gather_stencil does.
+template <int DIM, int S1, int EY, int EZ, class R>
+__device__ inline void gather_scalar(const double *nf, const int *cl, const R (*w)[S1],
+                                     const R (*dw)[S1], double &s, double *g) {
+  s = 0;
+  g[0] = g[1] = g[2] = 0;
+#pragma unroll
+  for (int ia = 0; ia < S1; ia++) {
+    double uw = 0, ud1 = 0, ud2 = 0;
+#pragma unroll
+    for (int ib = 0; ib < S1; ib++) {
+      if (DIM == 3) {
+        const int row = ((cl[0] + ia) * EY + (cl[1] + ib)) * EZ + cl[2];
+        double tw = 0, tz = 0;
+#pragma unroll
+        for (int ic = 0; ic < S1; ic++) {
+          const double f = nf[row + ic];
+          tw = fma((double)w[2][ic], f, tw);
+          tz = fma((double)dw[2][ic], f, tz);
+        }
+        uw = fma((double)w[1][ib], tw, uw);
+        ud1 = fma((double)dw[1][ib], tw, ud1);
+        ud2 = fma((double)w[1][ib], tz, ud2);
+      } else {
+        const double f = nf[(cl[0] + ia) * EY + (cl[1] + ib)];
+        uw = fma((double)w[1][ib], f, uw);
+        ud1 = fma((double)dw[1][ib], f, ud1);
+      }
+    }
+    s = fma((double)w[0][ia], uw, s);
+    g[0] = fma((double)dw[0][ia], uw, g[0]);
+    g[1] = fma((double)w[0][ia], ud1, g[1]);
+    if (DIM == 3) g[2] = fma((double)w[0][ia], ud2, g[2]);
   }
 }
 
@@ -1173,15 +1365,19 @@ __device__ inline double update_F(double dt, const double *L, double *F) {
 // the mixed path (whose state is stored in FP32), with J and rho in FP64.
 // The stress state (sig, eel, ep, epdot, dinit, dmg) is held in the type of
 // the path's storage, R.
+// th: the thermal state of a thermo-mechanical context (FP64, the generic
+// material class), else null.
 template <int MC, int PREC, class R = typename Real<PREC>::T>
 __device__ inline void stress_update(const DevMat &m, double dt, double h, double J, double rho,
                                      const double *D, R *sig, R *eel, R &ep, R &epdot, R &dinit,
-                                     R &dmg) {
+                                     R &dmg, ThermalPt *th = nullptr) {
   if constexpr (PREC == KMPM_PRECISION_MIXED) {
     float Df[6];
     for (int e = 0; e < 6; e++) Df[e] = (float)D[e];
     update_stress_point<MC, float>(m, (float)dt, (float)h, J, rho, Df, sig, eel, ep, epdot, dinit,
                                    dmg);
+  } else if constexpr (MC == MC_GENERIC) {
+    update_stress_point<MC>(m, dt, h, J, rho, D, sig, eel, ep, epdot, dinit, dmg, th);
   } else {
     update_stress_point<MC>(m, dt, h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
   }
@@ -1193,7 +1389,8 @@ __device__ inline void stress_update(const DevMat &m, double dt, double h, doubl
 template <int MC, int PREC, class R = typename Real<PREC>::T>
 __device__ inline double stress_and_speed(const KArgs &a, double dt, const double *L, double J,
                                           R *sig, R *eel, R &ep, R &epdot, R &dinit, R &dmg,
-                                          int mat, const double *v, int pid) {
+                                          int mat, const double *v, int pid,
+                                          ThermalPt *th = nullptr) {
   const DevMat &m = a.mats[mat];
   const double rho = m.rho0 / J;
   double D[6];
@@ -1203,7 +1400,7 @@ __device__ inline double stress_and_speed(const KArgs &a, double dt, const doubl
   D[3] = 0.5 * (L[1] + L[3]);
   D[4] = 0.5 * (L[2] + L[6]);
   D[5] = 0.5 * (L[5] + L[7]);
-  stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg, th);
   if (dmg >= 1.0) return -1.0;
   const double vm = fmax(fmax(fabs(v[0]), fabs(v[1])), fabs(v[2]));
   const double c = sqrt(m.cfl_k43g / rho) + vm;
@@ -1216,6 +1413,31 @@ __device__ inline bool outside_box(const KArgs &a, const double *x) {
   for (int d = 0; d < a.dim; d++)
     if (!(x[d] >= a.lo[d] && x[d] <= a.hi[d])) return true;
   return false;
+}
+
+// Where a stress kernel writes a particle: the next state at its sorted
+// position, or (USF, whose stress update precedes the G2P that carries the
+// state over) the current state's own slot.
+struct OutSlot {
+  double *d;
+  float *f;
+  int i;
+};
+__device__ inline OutSlot out_slot(const KArgs &a, bool in_place, int slot, int j) {
+  return in_place ? OutSlot{a.pa, a.fa, slot} : OutSlot{a.pb, a.fb, j};
+}
+#define STO(fld, val) stp<PREC>(a, o.d, o.f, (fld), o.i, (val))
+
+// The heat source gamma and the heat flux q = -vol kappa / cp grad T_n of a
+// thermo-mechanical particle (solid.cpp:1343-1351, 2810-2839), vol being
+// the current volume (ULMPM) or vol0 (TLMPM).
+template <int PREC>
+__device__ inline void store_heat(const KArgs &a, const OutSlot &o, int mat, double gamma,
+                                  double vol, const double *gradT) {
+  const DevMat &m = a.mats[mat];
+  const double sc = vol * m.invcp * m.kappa;
+  STO(GAMMA, gamma);
+  for (int d = 0; d < 3; d++) STO(QX + d, -gradT[d] * sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1244,6 +1466,10 @@ __global__ __launch_bounds__(256, STAGE ? 4
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   __shared__ R nb[2 * DIM * BN];
+  // thermo-mechanical runs (FP64, generic class): T_update's node block
+  constexpr bool THK = MC == MC_GENERIC && PREC == KMPM_PRECISION_FP64 && !STAGE && !XF;
+  __shared__ double nT[THK ? BN : 1];
+  const bool th = THK && a.thermo;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -1259,6 +1485,7 @@ __global__ __launch_bounds__(256, STAGE ? 4
     const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
+    if (th) load_node_scalar<DIM, S1, TX, TY, TZ>(a, a.gTu, nT, org[0], org[1], org[2]);
     __syncthreads();
     auto particle = [&](const int j) {
       const int slot = a.order[j];
@@ -1290,6 +1517,13 @@ __global__ __launch_bounds__(256, STAGE ? 4
       }
       if constexpr (XF)
         for (int e = 0; e < 9; e++) STB(C00 + e, j, L[e]);
+      // the particle's temperature from T_update (solid.cpp:2799-2808) and
+      // the gradient the heat flux takes (:2810-2839)
+      double Tp = 0, gradT[3] = {0, 0, 0};
+      if (th) {
+        gather_scalar<DIM, S1, EY, EZ>(nT, cl, w, dw, Tp, gradT);
+        STB(TEMP, j, Tp);
+      }
       const int pid = IFLD(a.ia, PID, slot);
       for (int d = 0; d < 3; d++) acc[d] *= inv_dt;
       if (XF && a.asflip) {
@@ -1360,9 +1594,11 @@ __global__ __launch_bounds__(256, STAGE ? 4
       }
       R ep = (R)LDA(EP, slot), epdot = (R)LDA(EPDOT, slot);
       R dinit = (R)LDA(DINIT, slot);
+      ThermalPt tp{Tp, 1, J * vol0, th ? LDA(GAMMA, slot) : 0.0};
       const double c = stress_and_speed<MC, PREC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
-                                                  mat, v, pid);
+                                                  mat, v, pid, th ? &tp : nullptr);
       if (c > cmax) cmax = c;
+      if (th) store_heat<PREC>(a, OutSlot{a.pb, a.fb, j}, mat, tp.gamma, J * vol0, gradT);
       for (int e = 0; e < 6; e++) {
         STB(SXX + e, j, sig[e]);
         STB(EXX + e, j, eel[e]);
@@ -1386,23 +1622,11 @@ __global__ __launch_bounds__(256, STAGE ? 4
 // damaged particles, which the CFL skips).  Writes slot j of the next state.
 // ---------------------------------------------------------------------------
 
-// Where a stress kernel writes a particle: the next state at its sorted
-// position, or (USF, whose stress update precedes the G2P that carries the
-// state over) the current state's own slot.
-struct OutSlot {
-  double *d;
-  float *f;
-  int i;
-};
-__device__ inline OutSlot out_slot(const KArgs &a, bool in_place, int slot, int j) {
-  return in_place ? OutSlot{a.pa, a.fa, slot} : OutSlot{a.pb, a.fb, j};
-}
-#define STO(fld, val) stp<PREC>(a, o.d, o.f, (fld), o.i, (val))
 
 template <int MC, int PREC>
 __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot,
                                  const OutSlot &o, int pid, int mat, const double *v,
-                                 double &cmax, double &rmin) {
+                                 double &cmax, double &rmin, ThermalPt *th = nullptr) {
   double F[9], Finv[9], R[9];
   for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot) + dt * Fdot[e];   // :1181-1182
   m3_inverse(F, Finv);                                                   // :1186
@@ -1422,7 +1646,7 @@ __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, 
     eel[e] = (RS)LDA(EXX + e, slot);
   }
   RS ep = (RS)LDA(EP, slot), epdot = (RS)LDA(EPDOT, slot), dinit = (RS)LDA(DINIT, slot);
-  stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg);
+  stress_update<MC, PREC>(m, dt, a.h, J, rho, D, sig, eel, ep, epdot, dinit, dmg, th);
   double S[9] = {sig[0], sig[3], sig[4], sig[3], sig[1], sig[5], sig[4], sig[5], sig[2]};
   double P[9];
   pk1(vol0, J, R, S, Finv, P);
@@ -1462,6 +1686,9 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   __shared__ R nb[2 * DIM * BN];
+  constexpr bool THK = MC == MC_GENERIC && PREC == KMPM_PRECISION_FP64;
+  __shared__ double nT[THK ? BN : 1];   // T_update (thermo-mechanical runs)
+  const bool th = THK && a.thermo;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -1477,6 +1704,7 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
     const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
+    if (th) load_node_scalar<DIM, S1, TX, TY, TZ>(a, a.gTu, nT, org[0], org[1], org[2]);
     __syncthreads();
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
@@ -1520,7 +1748,16 @@ __global__ __launch_bounds__(256) void k_g2p_usl_tl(KArgs a) {
       IFLD(a.ib, PID, j) = pid;
       IFLD(a.ib, MATI, j) = mat;
       STB(MASS, j, LDA(MASS, slot));
-      tl_update<MC, PREC>(a, dt, Fdot, slot, OutSlot{a.pb, a.fb, j}, pid, mat, v, cmax, rmin);
+      double Tp = 0, gradT[3] = {0, 0, 0};
+      if (th) {   // the particle's temperature and its gradient, from T_update
+        gather_scalar<DIM, S1, EY, EZ>(nT, cl, w, dw, Tp, gradT);
+        STB(TEMP, j, Tp);
+      }
+      const double vol0 = LDA(VOL0, slot);
+      ThermalPt tp{Tp, 1, vol0, th ? LDA(GAMMA, slot) : 0.0};
+      const OutSlot o{a.pb, a.fb, j};
+      tl_update<MC, PREC>(a, dt, Fdot, slot, o, pid, mat, v, cmax, rmin, th ? &tp : nullptr);
+      if (th) store_heat<PREC>(a, o, mat, tp.gamma, vol0, gradT);
     }
     if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }
@@ -1543,6 +1780,9 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   __shared__ R nb[2 * DIM * BN];
+  constexpr bool THK = PREC == KMPM_PRECISION_FP64;
+  __shared__ double nT[THK ? BN : 1];   // T_update (thermo-mechanical runs)
+  const bool th = THK && a.thermo;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -1557,6 +1797,7 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
     const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 0, PREC>(a, nb, org[0], org[1], org[2]);
+    if (th) load_node_scalar<DIM, S1, TX, TY, TZ>(a, a.gTu, nT, org[0], org[1], org[2]);
     __syncthreads();
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
       const int slot = a.order[j];
@@ -1613,6 +1854,11 @@ __global__ __launch_bounds__(256) void k_g2p_musl_a(KArgs a) {
         IFLD(a.ib, PID, j) = pid;
         IFLD(a.ib, MATI, j) = IFLD(a.ia, MATI, slot);
       }
+      if (th) {   // update_particle_temperature (solid.cpp:2799-2808)
+        double Tp, gradT[3];
+        gather_scalar<DIM, S1, EY, EZ>(nT, cl, w, dw, Tp, gradT);
+        STB(TEMP, j, Tp);
+      }
       // next step's cell: of the advanced x (ULMPM) or of x0 (TLMPM)
       const double *xk = TL ? xw : x;
       a.key[j] = particle_key(a, xk[0], xk[1], xk[2]);
@@ -1639,6 +1885,11 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   __shared__ R nb[DIM * BN];
+  // thermo-mechanical runs (FP64, generic class): the re-mapped grid
+  // temperature (MUSL) or USF's first (ulmpm.cpp:515-521, doublemapping)
+  constexpr bool THK = MC == MC_GENERIC && PREC == KMPM_PRECISION_FP64;
+  __shared__ double nT[THK ? BN : 1];
+  const bool th = THK && a.thermo;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -1652,6 +1903,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
     const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
+    if (th) load_node_scalar<DIM, S1, TX, TY, TZ>(a, a.gT, nT, org[0], org[1], org[2]);
     __syncthreads();
     const bool usf = a.scheme == KMPM_SCHEME_USF;
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
@@ -1706,9 +1958,15 @@ __global__ __launch_bounds__(256) void k_stress_musl_b(KArgs a) {
       }
       R ep = (R)LDA(EP, slot), epdot = (R)LDA(EPDOT, slot);
       R dinit = (R)LDA(DINIT, slot);
-      const double c =
-          stress_and_speed<MC, PREC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg, mat, v, pid);
+      // the temperature: MUSL's updated one, USF's of the step's start
+      double Tn, gradT[3] = {0, 0, 0};
+      if (th) gather_scalar<DIM, S1, EY, EZ>(nT, cl, w, dw, Tn, gradT);
+      ThermalPt tp{th ? (usf ? LDA(TEMP, slot) : LDB(TEMP, j)) : 0.0, 1, J * vol0,
+                   th ? LDA(GAMMA, slot) : 0.0};
+      const double c = stress_and_speed<MC, PREC>(a, dt, L, J, sig, eel, ep, epdot, dinit, dmg,
+                                                  mat, v, pid, th ? &tp : nullptr);
       if (c > cmax) cmax = c;
+      if (th) store_heat<PREC>(a, o, mat, tp.gamma, J * vol0, gradT);
       for (int e = 0; e < 6; e++) {
         STO(SXX + e, sig[e]);
         STO(EXX + e, eel[e]);
@@ -1736,6 +1994,9 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
   constexpr int EX = TX + S1 - 1, EY = TY + S1 - 1, EZ = DIM == 3 ? TZ + S1 - 1 : 1;
   constexpr int BN = EX * EY * EZ;
   __shared__ R nb[DIM * BN];
+  constexpr bool THK = MC == MC_GENERIC && PREC == KMPM_PRECISION_FP64;
+  __shared__ double nT[THK ? BN : 1];   // the grid temperature (thermo-mechanical runs)
+  const bool th = THK && a.thermo;
   const int nt1 = a.tg.nt[1], nt2 = a.tg.nt[2];
   const int off = S1 == 4 ? 1 : 0;
   const int ntile = a.sc->tile_count;
@@ -1749,6 +2010,7 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
     const int p0 = a.tile_p0[t], p1 = a.tile_p0[t + 1];
     __syncthreads();
     load_node_block<DIM, S1, TX, TY, TZ, 1, PREC>(a, nb, org[0], org[1], org[2]);
+    if (th) load_node_scalar<DIM, S1, TX, TY, TZ>(a, a.gT, nT, org[0], org[1], org[2]);
     __syncthreads();
     const bool usf = a.scheme == KMPM_SCHEME_USF;
     for (int j = p0 + (int)threadIdx.x; j < p1; j += blockDim.x) {
@@ -1781,7 +2043,13 @@ __global__ __launch_bounds__(256) void k_stress_musl_b_tl(KArgs a) {
       const OutSlot o = out_slot(a, usf, slot, j);
       if (a.xf)
         for (int e = 0; e < 9; e++) STO(C00 + e, Fdot[e]);
-      tl_update<MC, PREC>(a, dt, Fdot, slot, o, pid, mat, v, cmax, rmin);
+      double Tn, gradT[3] = {0, 0, 0};
+      if (th) gather_scalar<DIM, S1, EY, EZ>(nT, cl, w, dw, Tn, gradT);
+      const double vol0 = LDA(VOL0, slot);
+      ThermalPt tp{th ? (usf ? LDA(TEMP, slot) : LDB(TEMP, j)) : 0.0, 1, vol0,
+                   th ? LDA(GAMMA, slot) : 0.0};
+      tl_update<MC, PREC>(a, dt, Fdot, slot, o, pid, mat, v, cmax, rmin, th ? &tp : nullptr);
+      if (th) store_heat<PREC>(a, o, mat, tp.gamma, vol0, gradT);
     }
     if (a.ninst) flush_tile_cfl<true>(a, t, cmax, rmin);
   }
@@ -1847,6 +2115,8 @@ template <int DIM, int S1, int V> struct TileLaunch {
   static hipError_t g2p_usl(kmpm_ctx *c);
   static hipError_t g2p_musl_a(kmpm_ctx *c);
   static hipError_t stress_musl_b(kmpm_ctx *c);
+  static hipError_t p2g_thermal(kmpm_ctx *c, int remap);   // variant 0 only
+  static hipError_t grid_thermal(kmpm_ctx *c, int mode);   // ThermalMode; variant 0 only
 };
 
 }  // namespace kmpm

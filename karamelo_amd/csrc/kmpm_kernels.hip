@@ -585,6 +585,8 @@ hipError_t launch_grid_update(kmpm_ctx *c, int mode) {
   return tile_grid_update(c, mode);
 }
 hipError_t launch_grid_clean(kmpm_ctx *c) { TILE_CALL(c, grid_clean(c)); }
+hipError_t launch_p2g_thermal(kmpm_ctx *c, int remap) { TILE_CALL(c, p2g_thermal(c, remap)); }
+hipError_t launch_grid_thermal(kmpm_ctx *c, int mode) { TILE_CALL(c, grid_thermal(c, mode)); }
 hipError_t launch_g2p_usl(kmpm_ctx *c) { TILE_CALL(c, g2p_usl(c)); }
 hipError_t launch_g2p_musl_a(kmpm_ctx *c) { TILE_CALL(c, g2p_musl_a(c)); }
 hipError_t launch_stress_musl_b(kmpm_ctx *c) { TILE_CALL(c, stress_musl_b(c)); }

@@ -143,4 +143,50 @@ hipError_t TileLaunch<DIM, S1, V>::stress_musl_b(kmpm_ctx *c) {
   return hipGetLastError();
 }
 
+// The thermal transfer and grid temperature (thermo-mechanical runs: FP64,
+// tile variant 0).
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::p2g_thermal(kmpm_ctx *c, int remap) {
+  using TT = Tiles<DIM, S1, V>;
+  if constexpr (V != 0) {
+    return hipErrorInvalidValue;
+  } else {
+    if (c->a.np == 0) return hipSuccess;
+    const int g = persistent_grid(c, 8);
+    constexpr int FP = KMPM_PRECISION_FP64;
+    if (c->a.tl) {
+      if (remap)
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, true, FP, true, 0, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+      else
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, false, FP, true, 0, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+    } else {
+      if (remap)
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, true, FP, false, 0, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+      else
+        hipLaunchKernelGGL((k_p2g<DIM, S1, TT::TX, TT::TY, TT::TZ, false, FP, false, 0, true>),
+                           dim3(g), dim3(256), 0, c->stream, c->a);
+    }
+    return hipGetLastError();
+  }
+}
+
+template <int DIM, int S1, int V>
+hipError_t TileLaunch<DIM, S1, V>::grid_thermal(kmpm_ctx *c, int mode) {
+  using TT = Tiles<DIM, S1, V>;
+  if constexpr (V != 0) {
+    return hipErrorInvalidValue;
+  } else {
+    int64_t total = (int64_t)c->a.tg.ntiles * c->a.tg.T[0] * c->a.tg.T[1] * c->a.tg.T[2];
+    int g = nblk(total, 256);
+    if (g > c->cu_count * 8) g = c->cu_count * 8;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL((k_grid_thermal<DIM, S1, TT::TX, TT::TY, TT::TZ>), dim3(g), dim3(256), 0,
+                       c->stream, c->a, mode);
+    return hipGetLastError();
+  }
+}
+
 }  // namespace kmpm

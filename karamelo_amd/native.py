@@ -38,6 +38,8 @@ EOS_KEYS = {"linear": EOS_LINEAR, "shock": EOS_SHOCK, "fluid": EOS_FLUID}
 STRENGTH_KEYS = {"linear": STRENGTH_LINEAR, "johnson_cook": STRENGTH_JOHNSON_COOK,
                  "fluid": STRENGTH_FLUID}
 DAMAGE_KEYS = {"damage_johnson_cook": DAMAGE_JOHNSON_COOK}
+TEMP_NONE, TEMP_PLASTIC_WORK = 0, 1
+TEMP_KEYS = {"plastic_work": TEMP_PLASTIC_WORK}
 SCHEME_KEYS = {"usl": SCHEME_USL, "musl": SCHEME_MUSL, "usf": SCHEME_USF}
 # method(<style>, <sub-method>, ...) words (update.h:78-83)
 SUB_METHOD_KEYS = {"PIC": SUB_PIC, "FLIP": SUB_FLIP, "APIC": SUB_APIC, "AFLIP": SUB_AFLIP,
@@ -57,7 +59,7 @@ class Config(C.Structure):
                 ("pic_flip", C.c_double), ("dt", C.c_double), ("dt_constant", C.c_int),
                 ("sub_method", C.c_int), ("dt_factor", C.c_double),
                 ("solidlo", D3), ("solidhi", D3), ("np_per_cell", C.c_int),
-                ("reserved2", C.c_int)]
+                ("thermo", C.c_int)]
 
 
 class Material(C.Structure):
@@ -65,7 +67,8 @@ class Material(C.Structure):
                 ("lin_p", C.c_double * 3), ("eos_p", C.c_double * 9),
                 ("strength_p", C.c_double * 9), ("damage_p", C.c_double * 8),
                 ("rho0", C.c_double), ("K", C.c_double), ("G", C.c_double), ("E", C.c_double),
-                ("nu", C.c_double), ("lambda_", C.c_double), ("signal_velocity", C.c_double)]
+                ("nu", C.c_double), ("lambda_", C.c_double), ("signal_velocity", C.c_double),
+                ("temp", C.c_int), ("reserved_t", C.c_int), ("temp_p", C.c_double * 6)]
 
 
 class Region(C.Structure):
@@ -102,7 +105,7 @@ class Particles(C.Structure):
                 ("F", _DP), ("sigma", _DP), ("strain_el", _DP), ("mass", _DP), ("vol0", _DP),
                 ("vol", _DP), ("rho0", _DP), ("rho", _DP), ("J", _DP),
                 ("eff_plastic_strain", _DP), ("eff_plastic_strain_rate", _DP),
-                ("damage", _DP), ("damage_init", _DP)]
+                ("damage", _DP), ("damage_init", _DP), ("T", _DP), ("gamma", _DP)]
 
 
 # particle arrays: name -> (dtype, components)
@@ -113,7 +116,7 @@ PARTICLE_FIELDS = {
     "vol0": (np.float64, 1), "vol": (np.float64, 1), "rho0": (np.float64, 1),
     "rho": (np.float64, 1), "J": (np.float64, 1), "eff_plastic_strain": (np.float64, 1),
     "eff_plastic_strain_rate": (np.float64, 1), "damage": (np.float64, 1),
-    "damage_init": (np.float64, 1),
+    "damage_init": (np.float64, 1), "T": (np.float64, 1), "gamma": (np.float64, 1),
 }
 
 

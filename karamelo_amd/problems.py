@@ -52,10 +52,11 @@ class Problem:
 
 def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-16,
                 dt_constant=False, dt_factor=0.9, method="ulmpm", solid_region=None,
-                sub_method="FLIP", np_per_cell=0):
+                sub_method="FLIP", np_per_cell=0, thermo=False):
     """kmpm_config for dimension/method/scheme/shape commands; for TLMPM the
     solid's region gives the limits of its grid (Region::limits).  sub_method
-    is method()'s second word (PIC, FLIP, APIC, AFLIP, ASFLIP, MLS)."""
+    is method()'s second word (PIC, FLIP, APIC, AFLIP, ASFLIP, MLS); thermo
+    its thermo-mechanical keyword."""
     c = N.Config()
     c.dimension = dim
     c.method = N.METHOD_KEYS[method]
@@ -72,6 +73,7 @@ def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-1
     c.dt_factor = dt_factor
     c.sub_method = N.SUB_METHOD_KEYS[sub_method]
     c.np_per_cell = np_per_cell
+    c.thermo = 1 if thermo else 0
     if solid_region is not None:
         lo_, hi_ = N.region_limits(solid_region)
         for d in range(3):
@@ -97,6 +99,16 @@ def eos_strength_material(eos, eos_p, strength, strength_p, damage=None, damage_
     m.strength_p[:len(strength_p)] = list(strength_p)
     m.damage_p[:len(damage_p)] = list(damage_p)
     return N.material_init(m)
+
+
+def with_temperature(mat, chi, cp, kappa, alpha, T0, Tm, law="plastic_work"):
+    """The material with a temperature law, the last argument of
+    material(...): temperature(id, plastic_work, chi, cp, kappa, alpha, T0,
+    Tm) (temperature_plastic_work.cpp:50-55)."""
+    m = N.Material.from_buffer_copy(mat)
+    m.temp = N.TEMP_KEYS[law]
+    m.temp_p[:] = [chi, cp, kappa, alpha, T0, Tm]
+    return m
 
 
 def block(xlo, xhi, ylo, yhi, zlo=0.0, zhi=0.0):

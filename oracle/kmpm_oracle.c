@@ -38,7 +38,7 @@
 #define OMP_ATOMIC_CAPTURE
 #endif
 
-#define SQRT_3_OVER_2 1.224744871                              /* strength_jc.cpp:29 */
+#define SQRT_3_OVER_2 1.224744871              /* strength_jc.cpp:29, solid.cpp:39 */
 #define FOUR_THIRD 1.333333333333333333333333333333333333333   /* solid.cpp:39 */
 #define MAXFIX 16
 
@@ -493,6 +493,12 @@ struct orc_sim {
    * the ASFLIP position update; alpha = PIC_FLIP after the sub-method's
    * override; Di the inertia tensor of compute_inertia_tensor (diagonal) */
   int apic, mls, asflip; double alpha; double Di[9];
+  /* thermo-mechanical (update.cpp:168-180): particle T, heat source gamma
+   * and heat flux q; grid T, T_update and the driving forces Qint, Qext
+   * (solid.h, grid.h) */
+  int temp;
+  double *T, *gamma, *q;
+  double *gT, *gTu, *gQint, *gQext;
   /* neighbour lists: particle-major (neigh_pn) and node-major CSR (neigh_np) */
   int32_t *nn_pn; int64_t *neigh_pn; double *wf_pn, *wfd_pn;
   int64_t *np_start; int64_t *neigh_np; double *wf_np, *wfd_np; int64_t npairs_cap;
@@ -576,6 +582,11 @@ orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
   s->gf = (double *)xcalloc(3 * s->nn, sizeof(double));
   s->gmb = (double *)xcalloc(3 * s->nn, sizeof(double));
   s->gmask = (uint32_t *)xcalloc(s->nn, sizeof(uint32_t));
+  s->temp = c->thermo != 0;
+  s->gT = (double *)xcalloc(s->nn, sizeof(double));
+  s->gTu = (double *)xcalloc(s->nn, sizeof(double));
+  s->gQint = (double *)xcalloc(s->nn, sizeof(double));
+  s->gQext = (double *)xcalloc(s->nn, sizeof(double));
   s->np_start = (int64_t *)xcalloc(s->nn + 1, sizeof(int64_t));
   int nx = s->n[0], ny = s->n[1], nz = s->n[2];
   int64_t l = 0;
@@ -609,7 +620,7 @@ static void free_particles(orc_sim *s) {
                   s->F, s->sig, s->eel, s->L, s->D, s->mass, s->vol0, s->vol, s->rho0,
                   s->rho, s->J, s->ep, s->epdot, s->dmg, s->dinit, s->nn_pn, s->neigh_pn,
                   s->wf_pn, s->wfd_pn, s->neigh_np, s->wf_np, s->wfd_np, s->cell_ijk,
-                  s->Fdot, s->Finv, s->R, s->PK1};
+                  s->Fdot, s->Finv, s->R, s->PK1, s->T, s->gamma, s->q};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
 }
 
@@ -618,6 +629,7 @@ void orc_destroy(orc_sim *s) {
   free_particles(s);
   free(s->gx0); free(s->gntype); free(s->gmass); free(s->gv); free(s->gvu); free(s->gf);
   free(s->gmb); free(s->gmask); free(s->np_start); free(s->mats);
+  free(s->gT); free(s->gTu); free(s->gQint); free(s->gQext);
   free(s);
 }
 
@@ -645,6 +657,7 @@ int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
   A(rho, double, 1); A(J, double, 1); A(ep, double, 1); A(epdot, double, 1);
   A(dmg, double, 1); A(dinit, double, 1);
   A(Fdot, double, 9); A(Finv, double, 9); A(R, double, 9); A(PK1, double, 9);
+  A(T, double, 1); A(gamma, double, 1); A(q, double, 3);
   A(nn_pn, int32_t, 1); A(neigh_pn, int64_t, S); A(wf_pn, double, S); A(wfd_pn, double, 3 * S);
   A(cell_ijk, int32_t, 3);
 #undef A
@@ -683,6 +696,7 @@ int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
     s->epdot[i] = p->eff_plastic_strain_rate[i];
     s->dmg[i] = p->damage[i];
     s->dinit[i] = p->damage_init[i];
+    s->T[i] = p->T ? p->T[i] : 0.0;       /* gamma = q = 0 (solid.cpp:2305-2309) */
     if (s->tl) tl_particle_pk1(s, i);
   }
   return KMPM_OK;
@@ -715,6 +729,8 @@ int orc_get_particles(orc_sim *s, kmpm_particles *p) {
     if (p->eff_plastic_strain_rate) p->eff_plastic_strain_rate[i] = s->epdot[i];
     if (p->damage) p->damage[i] = s->dmg[i];
     if (p->damage_init) p->damage_init[i] = s->dinit[i];
+    if (p->T) p->T[i] = s->temp ? s->T[i] : 0.0;
+    if (p->gamma) p->gamma[i] = s->temp ? s->gamma[i] : 0.0;
   }
   return KMPM_OK;
 }
@@ -1023,10 +1039,64 @@ static void p2g_forces(orc_sim *s) {
   }
 }
 
+/* grid temperature, per solid (Solid::compute_temperature_nodes
+ * solid.cpp:2743-2762): T += (sum_solid (w m) T) / mass where mass > 0 */
+static void p2g_temperature(orc_sim *s) {
+  OMP_FOR
+  for (int64_t in = 0; in < s->nn; in++) {
+    int64_t b = s->np_start[in], e = s->np_start[in + 1], o = b;
+    s->gT[in] = 0;
+    while (o < e) {
+      int sol = s->solid[s->neigh_np[o]];
+      double Tt = 0;
+      int64_t o2 = o;
+      for (; o2 < e && s->solid[s->neigh_np[o2]] == sol; o2++) {
+        int64_t ip = s->neigh_np[o2];
+        Tt += s->wf_np[o2] * s->mass[ip] * s->T[ip];
+      }
+      if (s->gmass[in] > 0) {
+        Tt /= s->gmass[in];
+        s->gT[in] += Tt;
+      }
+      o = o2;
+    }
+  }
+}
+
+/* thermal driving forces: Qext += sum w gamma where mass > 0
+ * (compute_external_temperature_driving_forces_nodes solid.cpp:2764-2779),
+ * Qint = sum wfd . q (compute_internal_temperature_driving_forces_nodes
+ * :2781-2797, which resets Qint in every solid's call: the last solid's sum
+ * remains) */
+static void p2g_thermal_forces(orc_sim *s) {
+  int last = 0;
+  for (int64_t ip = 0; ip < s->np; ip++)
+    if (s->solid[ip] > last) last = s->solid[ip];
+  OMP_FOR
+  for (int64_t in = 0; in < s->nn; in++) {
+    int64_t b = s->np_start[in], e = s->np_start[in + 1];
+    double qe = 0, qi = 0;
+    for (int64_t o = b; o < e; o++) {
+      int64_t ip = s->neigh_np[o];
+      if (s->gmass[in] > 0) qe += s->wf_np[o] * s->gamma[ip];
+      if (s->solid[ip] == last) {
+        const double *wd = &s->wfd_np[3 * o], *qp = &s->q[3 * ip];
+        qi += wd[0] * qp[0] + wd[1] * qp[1] + wd[2] * qp[2];
+      }
+    }
+    s->gQext[in] = qe;
+    s->gQint[in] = qi;
+  }
+}
+
 int orc_phase_p2g(orc_sim *s) {
   p2g_mass(s);
   p2g_velocity(s);
   p2g_forces(s);
+  if (s->temp) {                                  /* ulmpm.cpp:371-375, tlmpm.cpp:359-363 */
+    p2g_temperature(s);
+    p2g_thermal_forces(s);
+  }
   return KMPM_OK;
 }
 
@@ -1081,6 +1151,15 @@ int orc_phase_grid_update(orc_sim *s) {
         s->gvu[3 * in + d] = s->gv[3 * in + d];
     }
   }
+  if (s->temp) {                                  /* Grid::update_grid_temperature grid.cpp:1354-1362 */
+    OMP_FOR
+    for (int64_t in = 0; in < s->nn; in++) {
+      if (s->gmass[in] != 0)
+        s->gTu[in] = s->gT[in] + dt * (s->gQint[in] + s->gQext[in]) / s->gmass[in];
+      else
+        s->gTu[in] = s->gT[in];
+    }
+  }
   velocity_fixes_post_update(s);
   return KMPM_OK;
 }
@@ -1090,6 +1169,7 @@ int orc_phase_grid_update(orc_sim *s) {
  * FixVelocityNodes::post_velocities_to_grid (fix_velocity_nodes.cpp:227-271) */
 static void phase_velocities_to_grid(orc_sim *s) {
   p2g_velocity(s);
+  if (s->temp) p2g_temperature(s);                /* ulmpm.cpp:491-493 */
   for (int k = 0; k < s->nfv; k++) {
     fixvel_t *f = &s->fv[k];
     OMP_FOR
@@ -1174,6 +1254,16 @@ static int phase_g2p(orc_sim *s) {
     if (s->asflip)
       for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * s->v[3 * ip + d];
   }
+  if (s->temp) {                                  /* update_particle_temperature solid.cpp:2799-2808 */
+    OMP_FOR
+    for (int64_t ip = 0; ip < s->np; ip++) {
+      s->T[ip] = 0;
+      for (int c = 0; c < s->nn_pn[ip]; c++) {
+        int64_t q = ip * s->S + c;
+        s->T[ip] += s->wf_pn[q] * s->gTu[s->neigh_pn[q]];
+      }
+    }
+  }
   return s->status;
 }
 
@@ -1197,13 +1287,13 @@ static double det3(const double *m) {
 }
 
 /* StrengthJohnsonCook::update_deviatoric_stress strength_jc.cpp:100-184 (T = 0) */
-void orc_jc_strength(const kmpm_material *mt, const double sigma[9], const double D[9],
-                     double dt, double eps_p, double epsdot, double damage,
-                     double out[9], double *dps) {
+static void jc_strength_T(const kmpm_material *mt, const double sigma[9], const double D[9],
+                          double dt, double eps_p, double epsdot, double damage, double T,
+                          double out[9], double *dps) {
   const double *p = mt->strength_p;
   double G_ = p[0], A = p[1], B = p[2], n = p[3], epsdot0 = p[4], C = p[5], m = p[6],
          Tr = p[7], Tm = p[8];
-  double Tmr = Tm - Tr, T = 0;
+  double Tmr = Tm - Tr;
   if (damage >= 1.0) {
     for (int e = 0; e < 9; e++) out[e] = 0;
     return;                          /* *dps left untouched: caller holds 0 */
@@ -1232,13 +1322,21 @@ void orc_jc_strength(const kmpm_material *mt, const double sigma[9], const doubl
   }
 }
 
-/* EOSShock::compute_pressure eos_shock.cpp:99-133 (T = 0) */
-double orc_shock_pressure(const kmpm_material *mt, double J, double rho, double damage,
-                          const double D[9], double cellsize) {
+/* StrengthJohnsonCook::update_deviatoric_stress strength_jc.cpp:100-184,
+ * the call without a temperature (T = 0) */
+void orc_jc_strength(const kmpm_material *mt, const double sigma[9], const double D[9],
+                     double dt, double eps_p, double epsdot, double damage,
+                     double out[9], double *dps) {
+  jc_strength_T(mt, sigma, D, dt, eps_p, epsdot, damage, 0.0, out, dps);
+}
+
+/* EOSShock::compute_pressure eos_shock.cpp:99-133 at temperature T */
+static double shock_pressure_T(const kmpm_material *mt, double J, double rho, double damage,
+                               const double D[9], double cellsize, double T) {
   const double *p = mt->eos_p;
   double rho0_ = p[0], c0 = p[2], S = p[3], Gamma = p[4], cv = p[5], Tr = p[6], Q1 = p[7],
          Q2 = p[8];
-  double alpha = cv * rho0_, e0 = 0, T = 0, e;
+  double alpha = cv * rho0_, e0 = 0, e;
   double mu = rho / rho0_ - 1.0;
   double pH = rho0_ * square(c0) * mu * (1.0 + mu) / square(1.0 - (S - 1.0) * mu);
   if (T > Tr) e = alpha * (T - Tr);
@@ -1260,13 +1358,19 @@ double orc_shock_pressure(const kmpm_material *mt, double J, double rho, double 
   return pFinal;
 }
 
-/* DamageJohnsonCook::compute_damage damage_jc.cpp:91-143 (T = 0) */
-void orc_jc_damage(const kmpm_material *mt, double *damage_init, double *damage, double pH,
-                   const double sdev[9], double epsdot, double dps) {
+/* the call without a temperature (T = 0) */
+double orc_shock_pressure(const kmpm_material *mt, double J, double rho, double damage,
+                          const double D[9], double cellsize) {
+  return shock_pressure_T(mt, J, rho, damage, D, cellsize, 0.0);
+}
+
+/* DamageJohnsonCook::compute_damage damage_jc.cpp:91-143 at temperature T */
+static void jc_damage_T(const kmpm_material *mt, double *damage_init, double *damage, double pH,
+                        const double sdev[9], double epsdot, double dps, double T) {
   const double *p = mt->damage_p;
   double d1 = p[0], d2 = p[1], d3 = p[2], d4 = p[3], d5 = p[4], epsdot0 = p[5], Tr = p[6],
          Tm = p[7];
-  double Tmr = Tm - Tr, T = 0;
+  double Tmr = Tm - Tr;
   if (dps == 0 && *damage >= 1.0) return;
   double vm = SQRT_3_OVER_2 * norm9(sdev);
   double triax = 0.0;
@@ -1284,6 +1388,12 @@ void orc_jc_damage(const kmpm_material *mt, double *damage_init, double *damage,
   if (*damage_init >= 1.0) *damage = dmin((*damage_init - 1.0) * 10, 1.0);
 }
 
+/* the call without a temperature (T = 0) */
+void orc_jc_damage(const kmpm_material *mt, double *damage_init, double *damage, double pH,
+                   const double sdev[9], double epsdot, double dps) {
+  jc_damage_T(mt, damage_init, damage, pH, sdev, epsdot, dps, 0.0);
+}
+
 /* Solid::update_stress solid.cpp:1246-1374 for one particle: the LINEAR
  * material (:1268-1280), or EOS + strength (+ damage) with the eff. plastic
  * strain, its rate filter (:1319-1329), the tension rule (:1353-1356) and
@@ -1291,6 +1401,24 @@ void orc_jc_damage(const kmpm_material *mt, double *damage_init, double *damage,
 void orc_update_stress_point(const kmpm_material *mt, double h, double dt, double J, double rho,
                              const double D[9], double sg[9], double eel[9], double *ep,
                              double *epdot, double *dinit, double *dmg) {
+  double gamma = 0;
+  orc_update_stress_point_T(mt, h, dt, J, rho, D, sg, eel, ep, epdot, dinit, dmg, 0.0, 0, 0.0,
+                            &gamma);
+}
+
+/* The same with the particle's temperature T: a material with a temperature
+ * law (cp != 0) evaluates its EOS and strength at T, adds the thermal
+ * pressure alpha (T0 - T) (temperature_plastic_work.cpp:74-76) and sets the
+ * heat source gamma = chi sigma_flow epsdot_p (0 from Tm on,
+ * temperature_plastic_work.cpp:65-71) times vol / cp, vol being vol (ULMPM)
+ * or vol0 (TLMPM) (solid.cpp:1301-1351); the damage sees T on
+ * thermo-mechanical runs (method_temp, solid.cpp:1331-1341). */
+void orc_update_stress_point_T(const kmpm_material *mt, double h, double dt, double J,
+                               double rho, const double D[9], double sg[9], double eel[9],
+                               double *ep, double *epdot, double *dinit, double *dmg, double T,
+                               int method_temp, double vol, double *gamma) {
+  const int has_temp = mt->temp == KMPM_TEMP_PLASTIC_WORK && mt->temp_p[1] != 0;
+  const double Tc = has_temp ? T : 0.0;      /* the constitutive calls' temperature */
   if (mt->type == KMPM_MAT_LINEAR) {                           /* :1268-1280 */
     double si[9];
     for (int e = 0; e < 9; e++) si[e] = dt * D[e];
@@ -1303,16 +1431,17 @@ void orc_update_stress_point(const kmpm_material *mt, double h, double dt, doubl
   double pH = 0, dps = 0, sdev[9];                              /* :1292-1374 */
   /* EOS */
   if (mt->eos == KMPM_EOS_SHOCK) {
-    pH = orc_shock_pressure(mt, J, rho, *dmg, D, h);
+    pH = shock_pressure_T(mt, J, rho, *dmg, D, h, Tc);
   } else if (mt->eos == KMPM_EOS_FLUID) {                       /* eos_fluid.cpp:68-73 */
     double mu = rho / mt->eos_p[0];
     pH = mt->eos_p[1] * (pow(mu, mt->eos_p[2]) - 1.0);
   } else {                                                      /* eos_linear.cpp:71-74 */
     pH = mt->eos_p[1] * (1 - J) * (1 - *dmg);
   }
+  if (has_temp) pH += mt->temp_p[3] * (mt->temp_p[4] - T);      /* :1305 */
   /* strength */
   if (mt->strength == KMPM_STRENGTH_JOHNSON_COOK) {
-    orc_jc_strength(mt, sg, D, dt, *ep, *epdot, *dmg, sdev, &dps);
+    jc_strength_T(mt, sg, D, dt, *ep, *epdot, *dmg, Tc, sdev, &dps);
   } else if (mt->strength == KMPM_STRENGTH_FLUID) {             /* strength_fluid.cpp:46-58 */
     double dv[9];
     deviator(D, dv);
@@ -1330,7 +1459,13 @@ void orc_update_stress_point(const kmpm_material *mt, double h, double dt, doubl
   *epdot += dps / tav;
   *epdot = dmax(0.0, *epdot);
   if (mt->damage == KMPM_DAMAGE_JOHNSON_COOK)                   /* :1331-1341 */
-    orc_jc_damage(mt, dinit, dmg, pH, sdev, *epdot, dps);
+    jc_damage_T(mt, dinit, dmg, pH, sdev, *epdot, dps, method_temp ? T : 0.0);
+  if (has_temp) {                                               /* :1343-1351 */
+    const double chi = mt->temp_p[0], invcp = 1.0 / mt->temp_p[1], Tm = mt->temp_p[5];
+    const double flow_stress = SQRT_3_OVER_2 * norm9(sdev);
+    *gamma = T < Tm ? chi * flow_stress * *epdot : 0;
+    *gamma *= vol * invcp;
+  }
   double dm = *dmg;
   if (dm == 0 || pH >= 0) {                                     /* :1353-1356 */
     for (int e = 0; e < 9; e++) sg[e] = -pH * ((e % 4 == 0) ? 1.0 : 0.0) + sdev[e];
@@ -1344,7 +1479,28 @@ void orc_update_stress_point(const kmpm_material *mt, double h, double dt, doubl
 
 /* Solid::update_deformation_gradient solid.cpp:1155-1244 (UL),
  * Solid::update_stress solid.cpp:1246-1438 */
-static int phase_F_and_stress(orc_sim *s) {
+/* Solid::update_heat_flux solid.cpp:2810-2839: q = -(vol0 | vol) kappa / cp
+ * sum wfd T_n, T_n = T (doublemapping) or T_update */
+static void phase_heat_flux(orc_sim *s, int doublemapping) {
+  const double *Tn = doublemapping ? s->gT : s->gTu;
+  OMP_FOR
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    const kmpm_material *mt = &s->mats[s->mat[ip]];
+    const int has = mt->temp == KMPM_TEMP_PLASTIC_WORK && mt->temp_p[1] != 0;
+    const double invcp = has ? 1.0 / mt->temp_p[1] : 0.0, kappa = has ? mt->temp_p[2] : 0.0;
+    double *qp = &s->q[3 * ip];
+    qp[0] = qp[1] = qp[2] = 0;
+    for (int c = 0; c < s->nn_pn[ip]; c++) {
+      int64_t q = ip * s->S + c;
+      const double *wd = &s->wfd_pn[3 * q], t = Tn[s->neigh_pn[q]];
+      for (int d = 0; d < 3; d++) qp[d] -= wd[d] * t;
+    }
+    const double sc = (s->tl ? s->vol0[ip] : s->vol[ip]) * invcp * kappa;
+    for (int d = 0; d < 3; d++) qp[d] *= sc;
+  }
+}
+
+static int phase_F_and_stress(orc_sim *s, int doublemapping) {
   double dt = s->dt;
   double max_c = 0;
   OMP_FOR
@@ -1386,9 +1542,11 @@ static int phase_F_and_stress(orc_sim *s) {
   }
   OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++)
-    orc_update_stress_point(&s->mats[s->mat[ip]], s->h, dt, s->J[ip], s->rho[ip], &s->D[9 * ip],
-                            &s->sig[9 * ip], &s->eel[9 * ip], &s->ep[ip], &s->epdot[ip],
-                            &s->dinit[ip], &s->dmg[ip]);
+    orc_update_stress_point_T(&s->mats[s->mat[ip]], s->h, dt, s->J[ip], s->rho[ip],
+                              &s->D[9 * ip], &s->sig[9 * ip], &s->eel[9 * ip], &s->ep[ip],
+                              &s->epdot[ip], &s->dinit[ip], &s->dmg[ip], s->T[ip], s->temp,
+                              s->tl ? s->vol0[ip] : s->vol[ip], &s->gamma[ip]);
+  if (s->temp) phase_heat_flux(s, doublemapping);   /* ulmpm.cpp:515-521 */
   if (s->tl) {                                                  /* :1277-1280,1368-1372 */
     OMP_FOR
     for (int64_t ip = 0; ip < s->np; ip++)
@@ -1439,10 +1597,12 @@ int orc_step(orc_sim *s, int64_t nsteps) {
     if (s->c.scheme == KMPM_SCHEME_USF) {
       p2g_mass(s);                                              /* particles_to_grid_USF_1 */
       p2g_velocity(s);
+      if (s->temp) p2g_temperature(s);
       velocity_fixes_post_update(s);                            /* post_update_grid_state */
       phase_rate_of_deformation(s, 1);
-      if (phase_F_and_stress(s)) break;
+      if (phase_F_and_stress(s, 1)) break;
       p2g_forces(s);                                            /* USF_2, post_particles_to_grid */
+      if (s->temp) p2g_thermal_forces(s);
       orc_phase_grid_update(s);
       if (phase_g2p(s)) break;
       phase_time_and_dt(s);
@@ -1458,10 +1618,17 @@ int orc_step(orc_sim *s, int64_t nsteps) {
       phase_velocities_to_grid(s);
       phase_rate_of_deformation(s, 1);
     }
-    if (phase_F_and_stress(s)) break;
+    if (phase_F_and_stress(s, s->c.scheme != KMPM_SCHEME_USL)) break;
     phase_time_and_dt(s);
   }
   return s->status;
+}
+
+/* the heat flux q of every particle (3 per particle, particle order) */
+int orc_get_heat_flux(orc_sim *s, double *q, int64_t np) {
+  if (np < s->np) return KMPM_ERR_CAPACITY;
+  memcpy(q, s->q, (size_t)s->np * 3 * sizeof(double));
+  return KMPM_OK;
 }
 
 /* set the affine matrix (L or Fdot) of every particle, as a state the

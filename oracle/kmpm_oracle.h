@@ -58,6 +58,8 @@ int orc_step(orc_sim *s, int64_t nsteps);
 /* L (ULMPM) or Fdot (TLMPM) of the last step, row-major, 9 per particle */
 int orc_get_affine(orc_sim *s, double *C, int64_t np);
 int orc_set_affine(orc_sim *s, const double *C, int64_t np);
+/* thermo-mechanical runs: the heat flux q, 3 per particle */
+int orc_get_heat_flux(orc_sim *s, double *q, int64_t np);
 int orc_get_time(orc_sim *s, double *dt, double *time, int64_t *step);
 int orc_energy(orc_sim *s, uint32_t groupbit, double *ek, double *es);
 
@@ -94,6 +96,14 @@ double orc_basis(int shape, int derivative, double r, int ntype, double inv_h);
 void orc_update_stress_point(const kmpm_material *m, double cellsize, double dt, double J,
                              double rho, const double D[9], double sigma[9], double strain_el[9],
                              double *eps_p, double *epsdot, double *damage_init, double *damage);
+/* the same with the particle temperature T (a material with a temperature
+ * law), the thermo-mechanical flag of the method (the damage's T), the
+ * volume that scales the heat source, and the heat source out */
+void orc_update_stress_point_T(const kmpm_material *m, double cellsize, double dt, double J,
+                               double rho, const double D[9], double sigma[9],
+                               double strain_el[9], double *eps_p, double *epsdot,
+                               double *damage_init, double *damage, double T, int method_temp,
+                               double vol, double *gamma);
 /* TLMPM matrix functions: rotation of the polar decomposition of F (row-major
  * 3x3; returns 0 when det F <= 0) and the real parts of F's eigenvalues */
 int orc_polar_rotation(const double F[9], double R[9]);

@@ -47,7 +47,7 @@ struct Arrays {
     for (auto *a : {&mass, &vol0, &vol, &rho0, &rho, &J, &ep, &epdot, &dmg, &dinit}) a->assign(n, 0.0);
   }
   kmpm_particles view(size_t n, size_t off = 0) {
-    kmpm_particles p;
+    kmpm_particles p{};   // fields this driver does not use stay null
     p.np = (int64_t)n;
     p.ptag = ptag.data() + off; p.solid = solid.data() + off; p.mat = mat.data() + off;
     p.mask = mask.data() + off;

@@ -49,6 +49,10 @@ def load(variant="liboracle.so"):
             "orc_fix_body_force": (C.c_int, [vp, C.c_uint32, N.I3, N.D3]),
             "orc_step": (C.c_int, [vp, C.c_int64]),
             "orc_get_affine": (C.c_int, [vp, D, C.c_int64]),
+            "orc_get_heat_flux": (C.c_int, [vp, D, C.c_int64]),
+            "orc_update_stress_point_T": (None, [P(N.Material), C.c_double, C.c_double,
+                                                 C.c_double, C.c_double, D, D, D, D, D, D, D,
+                                                 C.c_double, C.c_int, C.c_double, D]),
             "orc_set_affine": (C.c_int, [vp, D, C.c_int64]),
             "orc_get_time": (C.c_int, [vp, D, D, P(C.c_int64)]),
             "orc_energy": (C.c_int, [vp, C.c_uint32, D, D]),
