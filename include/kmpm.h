@@ -45,14 +45,29 @@ enum {
   KMPM_ERR_INTERNAL = 16,           /* a particle was found outside the tile it
                                        was sorted into (an internal
                                        inconsistency; its access is clamped) */
-  KMPM_ERR_INSTANCE = 17            /* batched context: a particle's stencil
+  KMPM_ERR_INSTANCE = 17,           /* batched context: a particle's stencil
                                        reaches outside its instance's tiles
                                        (no counterpart in the reference) */
+  KMPM_ERR_DOMAIN = 18              /* CPDI: a particle domain spans more than
+                                       KMPM_CPDI_MAX_SPAN nodes on an axis (no
+                                       counterpart in the reference, whose
+                                       neighbour lists grow without bound) */
 };
 
 /* ---- options ----------------------------------------------------------- */
 /* method(ulmpm|tlmpm, ...) -> update.cpp:108-196, style_method.h */
-enum { KMPM_METHOD_ULMPM = 0, KMPM_METHOD_TLMPM = 1 };
+/* ULCPDI and TLCPDI: the CPDI methods (ulcpdi.cpp, tlcpdi.cpp; 2D, the
+ * style in kmpm_config.cpdi_style) */
+enum { KMPM_METHOD_ULMPM = 0, KMPM_METHOD_TLMPM = 1, KMPM_METHOD_ULCPDI = 2,
+       KMPM_METHOD_TLCPDI = 3 };
+/* CPDI styles, ulcpdi.h:59 known_styles {"R4", "Q4"}: R4 particle domains
+ * are parallelograms spanned by two vectors carried by F; Q4 domains are
+ * quadrilaterals whose corners move with the grid velocity */
+enum { KMPM_CPDI_R4 = 0, KMPM_CPDI_Q4 = 1 };
+/* Nodes per axis a CPDI particle domain's corner stencils may span (the
+ * union of the corners' stencils); wider domains stop the step with
+ * KMPM_ERR_DOMAIN */
+enum { KMPM_CPDI_MAX_SPAN = 8 };
 /* scheme(usl|musl|usf) -> update.cpp:83-101, style_scheme.h (usf.cpp:34-94) */
 enum { KMPM_SCHEME_USL = 0, KMPM_SCHEME_MUSL = 1, KMPM_SCHEME_USF = 2 };
 /* velocity-update sub-method, the second word of method(...) (update.cpp:
@@ -115,6 +130,8 @@ typedef struct kmpm_config {
    * temperatures, heat sources and fluxes (solid.cpp:2743-2839,
    * grid.cpp:1354-1362); FP64, one solid per context */
   int    thermo;
+  /* ULCPDI / TLCPDI: KMPM_CPDI_R4 or KMPM_CPDI_Q4 (ulcpdi.cpp:49-108) */
+  int    cpdi_style;
 } kmpm_config;
 
 /* ---- materials: material(...) + eos/strength/damage styles ------------- */
@@ -192,6 +209,12 @@ typedef struct kmpm_particles {
    * on download, the heat source gamma (solid.cpp:1343-1351); either may be
    * NULL; other contexts ignore T and download zeros */
   double *T, *gamma;
+  /* CPDI contexts (solid.h): R4 domain vectors rp0 (reference) and rp
+   * (current, F rp0 on ULCPDI, solid.cpp:2338-2352), [np][2][3]; Q4 corners
+   * xpc0 and xpc, [np][4][3], counter-clockwise from (-, -)
+   * (solid.cpp:2208-2222).  On upload a NULL current field starts from its
+   * reference one; other contexts ignore them */
+  double *rp0, *rp, *xpc0, *xpc;
 } kmpm_particles;
 
 /* ---- host-side set-up (no device needed) -------------------------------- */

@@ -26,7 +26,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 HIP_SOURCES = ["kmpm_api.hip", "kmpm_kernels.hip", "kmpm_surrogate.hip", "kmpm_reforder.hip",
-               "kmpm_cutnet.hip", "kmpm_io.hip"]
+               "kmpm_cutnet.hip", "kmpm_io.hip", "kmpm_cpdi.hip"]
 HOST_SOURCES = ["kmpm_host.cpp", "kmpm_expr.cpp"]
 # kmpm_tiles.hip is compiled once per (dimension, shape function, tile
 # variant); the variants build in parallel

@@ -307,9 +307,10 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   void *old[] = {c->d_tile_np, c->d_tile_p0, c->d_tile_cur, c->d_tile_list, c->d_tile_slot,
                  c->d_dil_list, c->d_scan_tmp, c->d_scratch, c->d_tinst,
                  c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask, c->d_gT, c->d_gTu,
-                 c->d_scratch_th};
+                 c->d_scratch_th, c->d_cacc};
   for (void *p : old)
     if (p) (void)hipFree(p);
+  c->d_cacc = nullptr;
   c->d_tile_np = c->d_tile_p0 = c->d_tile_cur = c->d_tile_list = c->d_tile_slot = c->d_dil_list = nullptr;
   c->d_scan_tmp = nullptr;
   c->d_scratch = nullptr;
@@ -346,6 +347,8 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
     HIPCHK(hipMemset(c->d_gTu, 0, ns * sizeof(double)));
     HIPCHK(dalloc(&c->d_scratch_th, (size_t)slots * 3 * g.bn));
   }
+  // CPDI node sums: mass, momentum[3], force[3]
+  if (a.cpdi) HIPCHK(dalloc(&c->d_cacc, (size_t)7 * ns));
   if (ninst) {
     HIPCHK(dalloc(&c->d_tinst, g.ntiles));
     HIPCHK(hipMemcpy(c->d_tinst, map.data(), g.ntiles * sizeof(int), hipMemcpyHostToDevice));
@@ -374,6 +377,7 @@ static int setup_tiles(kmpm_ctx *c, int variant) {
   a.gT = c->d_gT;
   a.gTu = c->d_gTu;
   a.scratch_th = c->d_scratch_th;
+  a.cacc = c->d_cacc;
   a.tinst = c->d_tinst;
   c->keys_valid = false;
   c->sorted = false;
@@ -387,7 +391,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
                     kmpm_ctx **out) {
   if (!cfg || !out || nmat < 1 || nmat > KMPM_MAX_MATERIALS || !mats) return KMPM_ERR_ARG;
   *out = nullptr;
-  if (cfg->method != KMPM_METHOD_ULMPM && cfg->method != KMPM_METHOD_TLMPM) return KMPM_ERR_ARG;
+  if (cfg->method < KMPM_METHOD_ULMPM || cfg->method > KMPM_METHOD_TLCPDI) return KMPM_ERR_ARG;
   if (cfg->scheme != KMPM_SCHEME_USL && cfg->scheme != KMPM_SCHEME_MUSL &&
       cfg->scheme != KMPM_SCHEME_USF)
     return KMPM_ERR_ARG;
@@ -397,8 +401,15 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   // (tlmpm.cpp:82-84); the inertia tensor of Solid::compute_inertia_tensor
   // (solid.cpp:1440-1470) needs cubic splines on ULMPM and 1 or 2 particles
   // per cell with linear shape functions on TLMPM
-  const bool tl_cfg = cfg->method == KMPM_METHOD_TLMPM;
+  const bool tl_cfg = cfg->method == KMPM_METHOD_TLMPM || cfg->method == KMPM_METHOD_TLCPDI;
+  const bool cpdi = cfg->method == KMPM_METHOD_ULCPDI || cfg->method == KMPM_METHOD_TLCPDI;
   const int sub = cfg->sub_method;
+  // CPDI (ulcpdi.cpp:115-118): 2D, FP64, the fast order, PIC/FLIP, one
+  // context per run (no batched instances or learned updates, below)
+  if (cpdi && (cfg->dimension != 2 || cfg->precision != KMPM_PRECISION_FP64 ||
+               cfg->order != KMPM_ORDER_FAST || sub >= KMPM_SUB_APIC || cfg->thermo ||
+               (cfg->cpdi_style != KMPM_CPDI_R4 && cfg->cpdi_style != KMPM_CPDI_Q4)))
+    return KMPM_ERR_ARG;
   const bool apic = tl_cfg ? sub == KMPM_SUB_APIC : sub >= KMPM_SUB_APIC;
   if (apic && cfg->shape == KMPM_SHAPE_LINEAR &&
       (!tl_cfg || (cfg->np_per_cell != 1 && cfg->np_per_cell != 2)))
@@ -445,17 +456,18 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   }
   KArgs &a = c->a;
   a.dim = cfg->dimension;
-  a.tl = cfg->method == KMPM_METHOD_TLMPM;
+  a.tl = tl_cfg;
+  a.cpdi = cpdi ? 1 + cfg->cpdi_style : 0;
   a.ref_order = cfg->order == KMPM_ORDER_REFERENCE;
   a.prec = cfg->precision;
   a.S1 = S1;
   a.scheme = cfg->scheme;
   for (int d = 0; d < 3; d++) {
     a.n[d] = (int)n3[d];
-    // the grid's bounds: the box (ULMPM) or the solid's limits (TLMPM)
-    const bool tl = cfg->method == KMPM_METHOD_TLMPM;
-    a.lo[d] = d < cfg->dimension ? (tl ? cfg->solidlo[d] : cfg->boxlo[d]) : 0.0;
-    a.hi[d] = d < cfg->dimension ? (tl ? cfg->solidhi[d] : cfg->boxhi[d]) : 0.0;
+    // the grid's bounds: the box (UL) or the solid's limits (TL)
+    a.lo[d] = d < cfg->dimension ? (tl_cfg ? cfg->solidlo[d] : cfg->boxlo[d]) : 0.0;
+    a.hi[d] = d < cfg->dimension ? (tl_cfg ? cfg->solidhi[d] : cfg->boxhi[d]) : 0.0;
+    a.blo[d] = d < cfg->dimension ? cfg->boxlo[d] : 0.0;
   }
   a.nn = n3[0] * n3[1] * n3[2];
   a.h = cfg->cellsize;
@@ -465,7 +477,7 @@ int kmpm_ctx_create(const kmpm_config *cfg, const kmpm_material *mats, int nmat,
   a.xf = !apic ? 0 : (!a.tl && sub == KMPM_SUB_MLS) ? 2 : 1;
   a.asflip = !a.tl && sub == KMPM_SUB_ASFLIP;
   a.thermo = cfg->thermo != 0;
-  a.lay = a.thermo ? LAY_TH : apic ? LAY_AP : a.tl ? LAY_TL : LAY_UL;
+  a.lay = a.cpdi ? LAY_CP : a.thermo ? LAY_TH : apic ? LAY_AP : a.tl ? LAY_TL : LAY_UL;
   if (apic) {
     const double inv2 = 1.0 / (cfg->cellsize * cfg->cellsize);
     a.di = cfg->shape == KMPM_SHAPE_CUBIC_SPLINE ? 3.0 * inv2
@@ -534,7 +546,7 @@ int kmpm_ctx_destroy(kmpm_ctx *c) {
                   c->d_dil_list, c->d_scratch, c->d_gmass, c->d_gv, c->d_gvu, c->d_gmask,
                   c->d_sc, c->d_scan_tmp, c->d_surr, c->d_inst, c->d_tinst, c->d_red,
                   c->d_fexpr, c->d_cut, c->d_cut_stage, c->d_gT, c->d_gTu,
-                  c->d_scratch_th};
+                  c->d_scratch_th, c->d_cacc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   free(c->mats_host);
@@ -608,6 +620,23 @@ int kmpm_upload_particles(kmpm_ctx *c, const kmpm_particles *p) {
       for (int e = 0; e < 9; e++) D(P00 + e, i) = P[e];
     }
     if (a.thermo) D(TEMP, i) = p->T ? p->T[i] : 0.0;   // gamma = q = 0 (solid.cpp:2305-2309)
+    if (a.cpdi == 1 + KMPM_CPDI_R4) {   // r1x, r1y, r2x, r2y of the [2][3] vectors
+      if (!p->rp0) return KMPM_ERR_ARG;
+      const double *r0 = &p->rp0[6 * i], *r = p->rp ? &p->rp[6 * i] : r0;
+      const int xy[4] = {0, 1, 3, 4};
+      for (int e = 0; e < 4; e++) {
+        D(REF0 + e, i) = r0[xy[e]];
+        D(DOM0 + e, i) = r[xy[e]];
+      }
+    } else if (a.cpdi == 1 + KMPM_CPDI_Q4) {   // x, y of the four [3] corners
+      if (!p->xpc0) return KMPM_ERR_ARG;
+      const double *c0 = &p->xpc0[12 * i], *cc = p->xpc ? &p->xpc[12 * i] : c0;
+      for (int k = 0; k < 4; k++)
+        for (int d = 0; d < 2; d++) {
+          D(REF0 + 2 * k + d, i) = c0[3 * k + d];
+          D(DOM0 + 2 * k + d, i) = cc[3 * k + d];
+        }
+    }
     hi[(size_t)PID * cap + i] = (int)i;
     hi[(size_t)MATI * cap + i] = p->mat[i];
   }
@@ -787,8 +816,11 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
     double Fl[9];
     for (int e = 0; e < 9; e++) Fl[e] = D(F00 + e, i);
     F = Fl;
-    const double J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-                     F[2] * (F[3] * F[7] - F[4] * F[6]);
+    // CPDI-Q4: J = vol / vol0 of the corners' polygon (solid.cpp:1188-1200)
+    const double J = c->a.cpdi == 1 + KMPM_CPDI_Q4
+                         ? D(VOL, i) / D(VOL0, i)
+                         : F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                               F[2] * (F[3] * F[7] - F[4] * F[6]);
     const double rho0 = c->mats_host[mat].rho0;
     if (p->mass) p->mass[q] = D(MASS, i);
     if (p->vol0) p->vol0[q] = D(VOL0, i);
@@ -802,6 +834,23 @@ int kmpm_download_particles(kmpm_ctx *c, kmpm_particles *p) {
     if (p->damage_init) p->damage_init[q] = D(DINIT, i);
     if (p->T) p->T[q] = c->a.thermo ? D(TEMP, i) : 0.0;
     if (p->gamma) p->gamma[q] = c->a.thermo ? D(GAMMA, i) : 0.0;
+    if (c->a.cpdi == 1 + KMPM_CPDI_R4) {
+      const int xy[4] = {0, 1, 3, 4};
+      for (int e = 0; e < 6; e++) {
+        if (p->rp0) p->rp0[6 * q + e] = 0.0;
+        if (p->rp) p->rp[6 * q + e] = 0.0;
+      }
+      for (int e = 0; e < 4; e++) {
+        if (p->rp0) p->rp0[6 * q + xy[e]] = D(REF0 + e, i);
+        if (p->rp) p->rp[6 * q + xy[e]] = D(DOM0 + e, i);
+      }
+    } else if (c->a.cpdi == 1 + KMPM_CPDI_Q4) {
+      for (int k = 0; k < 4; k++)
+        for (int d = 0; d < 3; d++) {
+          if (p->xpc0) p->xpc0[12 * q + 3 * k + d] = d < 2 ? D(REF0 + 2 * k + d, i) : 0.0;
+          if (p->xpc) p->xpc[12 * q + 3 * k + d] = d < 2 ? D(DOM0 + 2 * k + d, i) : 0.0;
+        }
+    }
   }
   return KMPM_OK;
 }
@@ -906,7 +955,10 @@ int kmpm_particles_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G);
-  HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 0) : launch_p2g(c, 0));
+  if (c->a.cpdi)
+    HIPCHK(launch_cpdi_p2g(c, 0, 0));
+  else
+    HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 0) : launch_p2g(c, 0));
   // the grid temperature's sums and the thermal driving forces
   // (ulmpm.cpp:371-375, tlmpm.cpp:359-363)
   if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 0));
@@ -920,7 +972,10 @@ int kmpm_update_grid_state(kmpm_ctx *c) {
   tbegin(c, KMPM_PHASE_GRID);
   // USF: the first pass's mass and velocity with particles_to_grid_USF_2's forces
   const int mode = c->cfg.scheme == KMPM_SCHEME_USF ? GM_USF2 : GM_STEP;
-  HIPCHK(c->a.ref_order ? launch_ref_grid(c, 0) : launch_grid_update(c, mode));
+  if (c->a.cpdi)
+    HIPCHK(launch_cpdi_grid(c, mode));
+  else
+    HIPCHK(c->a.ref_order ? launch_ref_grid(c, 0) : launch_grid_update(c, mode));
   // Grid::update_grid_temperature (grid.cpp:1354-1362), with the node mass
   // of this pass
   if (c->a.thermo) HIPCHK(launch_grid_thermal(c, TM_FULL));
@@ -936,11 +991,11 @@ int kmpm_particles_to_grid_usf_1(kmpm_ctx *c) {
   if (!c || !c->sorted || c->cfg.scheme != KMPM_SCHEME_USF) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G);
-  HIPCHK(launch_p2g(c, 0));
+  HIPCHK(c->a.cpdi ? launch_cpdi_p2g(c, 0, 0) : launch_p2g(c, 0));
   if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 0));   // usf.cpp:53 (grid temperature)
   tend(c);
   tbegin(c, KMPM_PHASE_GRID_B);
-  HIPCHK(launch_grid_update(c, GM_USF1));
+  HIPCHK(c->a.cpdi ? launch_cpdi_grid(c, GM_USF1) : launch_grid_update(c, GM_USF1));
   if (c->a.thermo) HIPCHK(launch_grid_thermal(c, TM_T));
   tend(c);
   return KMPM_OK;
@@ -953,7 +1008,7 @@ int kmpm_particles_to_grid_usf_2(kmpm_ctx *c) {
   if (!c || !c->sorted || c->cfg.scheme != KMPM_SCHEME_USF) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G_B);
-  HIPCHK(launch_p2g(c, 0));
+  HIPCHK(c->a.cpdi ? launch_cpdi_p2g(c, 0, 1) : launch_p2g(c, 0));
   // the thermal driving forces of the updated heat source and flux
   if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 0));
   tend(c);
@@ -977,7 +1032,10 @@ int kmpm_grid_to_points_and_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_G2P);
-  HIPCHK(c->a.ref_order ? launch_ref_g2p_usl(c) : launch_g2p_usl(c));
+  if (c->a.cpdi)
+    HIPCHK(launch_cpdi_g2p(c, CG_USL));
+  else
+    HIPCHK(c->a.ref_order ? launch_ref_g2p_usl(c) : launch_g2p_usl(c));
   tend(c);
   if (learned_constitutive(c)) {
     tbegin(c, KMPM_PHASE_LEARNED);
@@ -991,7 +1049,10 @@ int kmpm_grid_to_points(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_G2P_A);
-  HIPCHK(c->a.ref_order ? launch_ref_g2p_musl_a(c) : launch_g2p_musl_a(c));
+  if (c->a.cpdi)
+    HIPCHK(launch_cpdi_g2p(c, c->cfg.scheme == KMPM_SCHEME_USF ? CG_USF : CG_MUSL));
+  else
+    HIPCHK(c->a.ref_order ? launch_ref_g2p_musl_a(c) : launch_g2p_musl_a(c));
   tend(c);
   // USF: the G2P carries the (already updated) state into the next buffer
   if (c->cfg.scheme == KMPM_SCHEME_USF) return after_g2p_state_swap(c);
@@ -1002,13 +1063,19 @@ int kmpm_velocities_to_grid(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_P2G_B);
-  HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 1) : launch_p2g(c, 1));
+  if (c->a.cpdi)
+    HIPCHK(launch_cpdi_p2g(c, 1, 0));
+  else
+    HIPCHK(c->a.ref_order ? launch_ref_p2g(c, 1) : launch_p2g(c, 1));
   // the grid temperature of the updated particle temperatures
   // (ulmpm.cpp:491-493)
   if (c->a.thermo) HIPCHK(launch_p2g_thermal(c, 1));
   tend(c);
   tbegin(c, KMPM_PHASE_GRID_B);
-  HIPCHK(c->a.ref_order ? launch_ref_grid(c, 1) : launch_grid_update(c, 1));
+  if (c->a.cpdi)
+    HIPCHK(launch_cpdi_grid(c, GM_REMAP));
+  else
+    HIPCHK(c->a.ref_order ? launch_ref_grid(c, 1) : launch_grid_update(c, 1));
   if (c->a.thermo) HIPCHK(launch_grid_thermal(c, TM_T));
   tend(c);
   return KMPM_OK;
@@ -1018,7 +1085,10 @@ int kmpm_update_stress(kmpm_ctx *c) {
   if (!c || !c->sorted) return KMPM_ERR_ARG;
   if (c->upload_failed) return KMPM_ERR_ARG;
   tbegin(c, KMPM_PHASE_STRESS_B);
-  HIPCHK(c->a.ref_order ? launch_ref_stress_musl_b(c) : launch_stress_musl_b(c));
+  if (c->a.cpdi)
+    HIPCHK(launch_cpdi_stress(c));
+  else
+    HIPCHK(c->a.ref_order ? launch_ref_stress_musl_b(c) : launch_stress_musl_b(c));
   tend(c);
   // USF updates the current state in place, before its forces and G2P
   if (c->cfg.scheme == KMPM_SCHEME_USF) return KMPM_OK;
@@ -1071,7 +1141,7 @@ int kmpm_synchronize(kmpm_ctx *c, int64_t *ptag_out) {
 int kmpm_set_instances(kmpm_ctx *c, int ninst, const int32_t *clo, const int32_t *chi) {
   if (!c || ninst < 0 || (ninst > 0 && (!clo || !chi))) return KMPM_ERR_ARG;
   if (ninst > 0 && (c->a.tl || c->a.ref_order || c->a.surr_on || c->a.cut_on || c->a.xf ||
-                    c->a.thermo || c->a.scheme == KMPM_SCHEME_USF))
+                    c->a.thermo || c->a.cpdi || c->a.scheme == KMPM_SCHEME_USF))
     return KMPM_ERR_ARG;
   for (int q = 0; q < c->a.nfv; q++)
     if (ninst > 0 && c->a.fv[q].expr) return KMPM_ERR_ARG;   // values of the context's time
@@ -1387,7 +1457,8 @@ int kmpm_reduce_energy(kmpm_ctx *c, uint32_t bit, double *ek, double *es) {
 
 int kmpm_download_grid(kmpm_ctx *c, double *mass, double *v, double *vu, int64_t nn) {
   if (!c || nn != c->a.nn) return KMPM_ERR_ARG;
-  if (!c->a.ref_order) HIPCHK(launch_grid_clean(c));   // the reference order writes every node
+  // the reference order and CPDI write every node each step
+  if (!c->a.ref_order && !c->a.cpdi) HIPCHK(launch_grid_clean(c));
   HIPCHK(hipStreamSynchronize(c->stream));
   // device layout [comp][node] (FP32 values in the mixed path) -> reference
   // layout [node][comp]
@@ -1518,7 +1589,7 @@ int kmpm_ctx_set_surrogate(kmpm_ctx *c, const kmpm_surrogate *s) {
   // surrogate's JC + shock material; one instance (the constitutive kernel
   // reduces the context's CFL)
   if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
-      c->a.ninst || c->a.xf || c->a.thermo)
+      c->a.ninst || c->a.xf || c->a.thermo || c->a.cpdi)
     return KMPM_ERR_ARG;
   for (int i = 0; i < c->nmat; i++)
     if (!surrogate_matches(s, c->mats_host[i])) return KMPM_ERR_ARG;
@@ -1545,7 +1616,8 @@ int kmpm_ctx_set_cutnet(kmpm_ctx *c, const kmpm_cutnet *s) {
   // one network serves every particle of the wave: one material, the cut
   // network's; USL on one instance of the mixed ULMPM path
   if (c->a.prec != KMPM_PRECISION_MIXED || c->a.mat_class != MC_JC_SHOCK || c->a.tl ||
-      c->a.ref_order || c->a.ninst || c->a.xf || c->a.thermo || c->cfg.scheme != KMPM_SCHEME_USL ||
+      c->a.ref_order || c->a.ninst || c->a.xf || c->a.thermo || c->a.cpdi ||
+      c->cfg.scheme != KMPM_SCHEME_USL ||
       c->nmat != 1 ||
       !cutnet_matches(s, c->mats_host[0]))
     return KMPM_ERR_ARG;

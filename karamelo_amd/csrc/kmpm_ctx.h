@@ -78,6 +78,12 @@ struct KArgs {
   // partial blocks [slot][3][bn] (sum w m T, sum w gamma, sum grad w . q)
   int thermo;
   double *gT, *gTu, *scratch_th;
+  // CPDI (FP64, 2D): 1 + KMPM_CPDI_* or 0; the box's lower corner (the
+  // reference takes the candidate nodes from it on the solid's grid too);
+  // the node sums of the particle-major P2G [7][ns] (mass, momentum, force)
+  int cpdi;
+  double blo[3];
+  double *cacc;
   uint32_t *gmask;
   // fixes
   int nfv, nbf;
@@ -136,6 +142,7 @@ struct kmpm_ctx {
   double *d_scratch;
   double *d_gmass, *d_gv, *d_gvu;
   double *d_gT, *d_gTu, *d_scratch_th;
+  double *d_cacc;
   uint32_t *d_gmask;
   uint32_t *d_pmask;     // particle group masks (by pid)
   kmpm_expr *d_fexpr;    // [MAX_FIX][6]
@@ -191,6 +198,9 @@ enum GridMode { GM_STEP = 0, GM_REMAP = 1, GM_USF1 = 2, GM_USF2 = 3 };
 // driving forces and T_update (TM_FULL), or T alone (TM_T: the MUSL re-map
 // and USF's first pass).
 enum ThermalMode { TM_FULL = 0, TM_T = 1 };
+// CPDI G2P kernels: USL's fused G2P, F and stress; MUSL's part A (G2P and
+// advance); USF's G2P (after its in-place stress pass)
+enum CpdiG2P { CG_USL = 0, CG_MUSL = 1, CG_USF = 2 };
 
 // launchers (kmpm_kernels.hip); all stream-ordered, return hipError_t
 hipError_t launch_keys(kmpm_ctx *c);
@@ -199,6 +209,13 @@ hipError_t launch_p2g(kmpm_ctx *c, int momentum_only);
 hipError_t launch_grid_update(kmpm_ctx *c, int mode);   // GridMode (kmpm_kernels.h)
 hipError_t launch_p2g_thermal(kmpm_ctx *c, int remap);
 hipError_t launch_grid_thermal(kmpm_ctx *c, int mode);  // ThermalMode
+// CPDI (kmpm_cpdi.hip): P2G (remap: the MUSL momentum re-map; saved: USF's
+// second, whose weights divide by the volume its stress pass kept), the
+// grid pass (GridMode), the G2P (CpdiG2P) and the MUSL / USF stress pass
+hipError_t launch_cpdi_p2g(kmpm_ctx *c, int remap, int saved);
+hipError_t launch_cpdi_grid(kmpm_ctx *c, int mode);
+hipError_t launch_cpdi_g2p(kmpm_ctx *c, int kind);
+hipError_t launch_cpdi_stress(kmpm_ctx *c);
 hipError_t launch_fix_values(kmpm_ctx *c);   // expression-valued fixes, before a grid pass
 hipError_t launch_g2p_usl(kmpm_ctx *c);
 hipError_t launch_g2p_musl_a(kmpm_ctx *c);

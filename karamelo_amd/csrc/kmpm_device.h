@@ -44,6 +44,10 @@ enum ApicField {
 // Thermo-mechanical contexts (FP64): temperature, heat source and heat flux
 // (solid.h: T, gamma, q), after every other field.
 enum ThermField { TEMP = NDF_AP, GAMMA, QX, QY, QZ, NDF_TH };
+// CPDI contexts (FP64, 2D): the particle domain after the TLMPM fields.
+// DOM0..7 the current one (R4: the vectors r1, r2 as x, y pairs; Q4: the
+// four corners' x, y), REF0..7 the reference one (rp0 / xpc0), solid.h.
+enum CpdiField { DOM0 = NDF_TL, REF0 = DOM0 + 8, NDF_CP = REF0 + 8 };
 // Int fields: PID = index into the static (upload-order) arrays, MAT = material.
 enum IField { PID = 0, MATI, NIF };
 
@@ -70,14 +74,18 @@ __host__ __device__ constexpr int mx_fslot(int f) {
 }
 static_assert(mx_fslot(VOL) == MX_NF - 1 && mx_dslot(F22) == MX_ND - 1, "mixed layout");
 static_assert(mx_fslot(C00) == MX_NF_TL && mx_fslot(C22) == MX_NF_AP - 1, "mixed APIC layout");
-// State layouts: ULMPM, TLMPM, the APIC layout (TLMPM's fields and C), and
-// the thermo-mechanical layout (all of those and the thermal fields; FP64
-// only).
-enum Layout { LAY_UL = 0, LAY_TL = 1, LAY_AP = 2, LAY_TH = 3 };
+// State layouts: ULMPM, TLMPM, the APIC layout (TLMPM's fields and C), the
+// thermo-mechanical layout (all of those and the thermal fields; FP64
+// only) and the CPDI layout (TLMPM's fields and the domains; FP64 only).
+enum Layout { LAY_UL = 0, LAY_TL = 1, LAY_AP = 2, LAY_TH = 3, LAY_CP = 4 };
 // Field counts of the double and float blocks per layout and precision.
 __host__ __device__ constexpr int state_nd(bool mixed, int lay) {
   return mixed ? (lay != LAY_UL ? MX_ND_TL : MX_ND)
-               : (lay == LAY_UL ? NDF : lay == LAY_TL ? NDF_TL : lay == LAY_AP ? NDF_AP : NDF_TH);
+               : (lay == LAY_UL   ? NDF
+                  : lay == LAY_TL ? NDF_TL
+                  : lay == LAY_AP ? NDF_AP
+                  : lay == LAY_TH ? NDF_TH
+                                  : NDF_CP);
 }
 __host__ __device__ constexpr int state_nf(bool mixed, int lay) {
   return mixed ? (lay == LAY_UL ? MX_NF : lay == LAY_TL ? MX_NF_TL : MX_NF_AP) : 0;

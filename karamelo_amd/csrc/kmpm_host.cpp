@@ -106,13 +106,13 @@ int kmpm_material_init(kmpm_material *m) {
 int kmpm_grid_size(const kmpm_config *c, int64_t out[3]) {
   if (!c || !(c->cellsize > 0) || c->dimension < 2 || c->dimension > 3) return KMPM_ERR_ARG;
   const double h = c->cellsize;
-  const bool tl = c->method == KMPM_METHOD_TLMPM;
+  const bool tl = c->method == KMPM_METHOD_TLMPM || c->method == KMPM_METHOD_TLCPDI;
   for (int d = 0; d < 3; d++) {
     if (d >= c->dimension) {
       out[d] = 1;
       continue;
     }
-    // the bounds of the grid: the box (ULMPM) or the solid (TLMPM), grid.cpp:91-97
+    // the bounds of the grid: the box (UL) or the solid (TL), grid.cpp:91-97
     const double lo = tl ? c->solidlo[d] : c->boxlo[d];
     const double hi = tl ? c->solidhi[d] : c->boxhi[d];
     if (tl && (lo < c->boxlo[d] || hi > c->boxhi[d] || !(hi > lo))) return KMPM_ERR_ARG;
@@ -160,14 +160,22 @@ int kmpm_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
                   const kmpm_material *mat, int mat_index, int solid_index, int64_t ptag_offset,
                   kmpm_particles *out, int64_t capacity, int64_t *np_out) {
   if (!c || !region || !mat || !out || !np_out) return KMPM_ERR_ARG;
-  if (c->method != KMPM_METHOD_ULMPM && c->method != KMPM_METHOD_TLMPM) return KMPM_ERR_ARG;
+  if (c->method < KMPM_METHOD_ULMPM || c->method > KMPM_METHOD_TLCPDI) return KMPM_ERR_ARG;
+  const bool tl = c->method == KMPM_METHOD_TLMPM || c->method == KMPM_METHOD_TLCPDI;
+  const bool cpdi = c->method == KMPM_METHOD_ULCPDI || c->method == KMPM_METHOD_TLCPDI;
+  const bool q4 = cpdi && c->cpdi_style == KMPM_CPDI_Q4;
   const int dim = c->dimension;
   const double delta = c->cellsize;
+  // CPDI: the half-width of a particle's domain (solid.cpp:2029-2106)
+  double lp = delta;
+  if (ppc1d == 1) lp *= 0.5;
+  else if (ppc1d == 2) lp *= 0.25;
+  else lp *= 1.0 / (2 * ppc1d);
   // Lattice of cells over the bounds (Solid::populate, solid.cpp:1927-2002,
   // one rank: sub-domain == box, no neighbour adjustments).  The bounds are
   // the box for ULMPM and the region's limits for TLMPM.
   double blo[3], bhi[3];
-  if (c->method == KMPM_METHOD_TLMPM) {
+  if (tl) {
     kmpm_region_limits(region, blo, bhi);
   } else {
     for (int d = 0; d < 3; d++) {
@@ -254,6 +262,22 @@ int kmpm_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
             out->solid[p] = solid_index;
             out->mat[p] = mat_index;
             if (out->mask) out->mask[p] = 1;
+            if (cpdi && !q4) {   // R4 vectors (lp, 0, 0), (0, lp, 0), solid.cpp:2188-2205
+              for (int e = 0; e < 6; e++) {
+                const double r = e == 0 || e == 4 ? lp : 0.0;
+                if (out->rp0) out->rp0[6 * p + e] = r;
+                if (out->rp) out->rp[6 * p + e] = r;
+              }
+            } else if (q4) {     // Q4 corners from (-, -), counter-clockwise, :2208-2222
+              const double cx[4] = {x - lp, x + lp, x + lp, x - lp};
+              const double cy[4] = {y - lp, y - lp, y + lp, y + lp};
+              for (int k = 0; k < 4; k++)
+                for (int d = 0; d < 3; d++) {
+                  const double v = d == 0 ? cx[k] : d == 1 ? cy[k] : 0.0;
+                  if (out->xpc0) out->xpc0[12 * p + 3 * k + d] = v;
+                  if (out->xpc) out->xpc[12 * p + 3 * k + d] = v;
+                }
+            }
           }
           count++;
         }

@@ -186,7 +186,7 @@ int kmpm_dump_particles(kmpm_ctx *c, const char *filename, const char *const *va
   if ((st = kmpm_get_time(c, nullptr, nullptr, &step))) return st;
   std::string s;
   header(s, c, n, vars, nvars);
-  const bool tl = c->cfg.method == KMPM_METHOD_TLMPM;
+  const bool tl = (c->cfg.method == KMPM_METHOD_TLMPM || c->cfg.method == KMPM_METHOD_TLCPDI);
   // solids in order, each in its particles' order (the upload order)
   int32_t nsol = 0;
   for (int64_t i = 0; i < n; i++) nsol = std::max(nsol, h.solid[i] + 1);
@@ -281,7 +281,7 @@ int kmpm_dump_grid(kmpm_ctx *c, const char *filename, const char *const *vars, i
   if ((st = download_grid_temperature(c, T.data(), nn))) return st;
   int64_t step = 0;
   if ((st = kmpm_get_time(c, nullptr, nullptr, &step))) return st;
-  const bool tl = c->cfg.method == KMPM_METHOD_TLMPM;
+  const bool tl = (c->cfg.method == KMPM_METHOD_TLMPM || c->cfg.method == KMPM_METHOD_TLCPDI);
   int64_t n[3];
   if ((st = kmpm_grid_size(&c->cfg, n))) return st;
   const double *lo = tl ? c->cfg.solidlo : c->cfg.boxlo;
@@ -337,7 +337,8 @@ int kmpm_dump_grid(kmpm_ctx *c, const char *filename, const char *const *vars, i
 }
 
 int kmpm_write_restart(kmpm_ctx *c, const char *filename) {
-  if (!c || !filename) return KMPM_ERR_ARG;
+  // the reference's restart blocks hold no CPDI domains (solid.cpp:2841-2889)
+  if (!c || !filename || c->a.cpdi) return KMPM_ERR_ARG;
   const int64_t n = c->a.np;
   Host h(n);
   int st = kmpm_download_particles(c, &h.p);
@@ -348,7 +349,7 @@ int kmpm_write_restart(kmpm_ctx *c, const char *filename) {
   std::ofstream o(filename, std::ios::out | std::ios::binary);
   if (!o.is_open()) return KMPM_ERR_ARG;
   const kmpm_config &cf = c->cfg;
-  const bool tl = cf.method == KMPM_METHOD_TLMPM;
+  const bool tl = (cf.method == KMPM_METHOD_TLMPM || cf.method == KMPM_METHOD_TLCPDI);
   o.write(MAGIC, sizeof MAGIC);
   wr(o, (int32_t)cf.dimension);
   // ---- Update::write_restart (update.cpp:220-280) ----
@@ -429,13 +430,13 @@ int kmpm_write_restart(kmpm_ctx *c, const char *filename) {
 }
 
 int kmpm_read_restart(kmpm_ctx *c, const char *filename) {
-  if (!c || !filename) return KMPM_ERR_ARG;
+  if (!c || !filename || c->a.cpdi) return KMPM_ERR_ARG;
   std::ifstream in(filename, std::ios::in | std::ios::binary);
   if (!in.is_open()) return KMPM_ERR_ARG;
   char magic[8];
   if (!in.read(magic, 8) || std::memcmp(magic, MAGIC, 8) != 0) return KMPM_ERR_ARG;
   const kmpm_config &cf = c->cfg;
-  const bool tl = cf.method == KMPM_METHOD_TLMPM;
+  const bool tl = (cf.method == KMPM_METHOD_TLMPM || cf.method == KMPM_METHOD_TLCPDI);
   int32_t dim = 0, sub = 0, shape = 0;
   std::string method, scheme;
   bool temp = false, dtc = false;

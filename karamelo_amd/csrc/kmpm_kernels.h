@@ -1626,11 +1626,13 @@ __global__ __launch_bounds__(256, STAGE ? 4
 template <int MC, int PREC>
 __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, int slot,
                                  const OutSlot &o, int pid, int mat, const double *v,
-                                 double &cmax, double &rmin, ThermalPt *th = nullptr) {
+                                 double &cmax, double &rmin, ThermalPt *th = nullptr,
+                                 const double *jv = nullptr) {
   double F[9], Finv[9], R[9];
   for (int e = 0; e < 9; e++) F[e] = LDA(F00 + e, slot) + dt * Fdot[e];   // :1181-1182
   m3_inverse(F, Finv);                                                   // :1186
-  const double J = m3_det(F);                                            // :1203
+  // jv: J and vol given (CPDI-Q4: the corners' polygon, :1188-1200)
+  const double J = jv ? jv[0] : m3_det(F);                               // :1203
   const double vol0 = LDA(VOL0, slot);
   using RS = typename Real<PREC>::T;   // the stored state's type
   RS dmg = (RS)LDA(DMG, slot);
@@ -1655,7 +1657,7 @@ __device__ inline void tl_update(const KArgs &a, double dt, const double *Fdot, 
     STO(P00 + e, P[e]);
   }
   STO(VOL0, vol0);
-  STO(VOL, J * vol0);
+  STO(VOL, jv ? jv[1] : J * vol0);
   for (int e = 0; e < 6; e++) {
     STO(SXX + e, sig[e]);
     STO(EXX + e, eel[e]);

@@ -15,10 +15,12 @@ LIB_PATH = os.path.join(HERE, "libkmpm.so")
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NO_DEVICE, ERR_CAPACITY = 1, 2, 3, 4
 ERR_PARTICLE_LEFT_BOX, ERR_JACOBIAN, ERR_DTCFL, ERR_WAVE_SPEED, ERR_GRID = 10, 11, 12, 13, 14
-ERR_FIXED_POINT, ERR_INTERNAL, ERR_INSTANCE = 15, 16, 17
+ERR_FIXED_POINT, ERR_INTERNAL, ERR_INSTANCE, ERR_DOMAIN = 15, 16, 17, 18
 NTILE_VARIANTS = 3
 SURROGATE_NPARAM = 1185
-METHOD_ULMPM, METHOD_TLMPM = 0, 1
+METHOD_ULMPM, METHOD_TLMPM, METHOD_ULCPDI, METHOD_TLCPDI = 0, 1, 2, 3
+CPDI_R4, CPDI_Q4 = 0, 1
+CPDI_MAX_SPAN = 8
 SCHEME_USL, SCHEME_MUSL, SCHEME_USF = 0, 1, 2
 SUB_PIC, SUB_FLIP, SUB_APIC, SUB_AFLIP, SUB_ASFLIP, SUB_MLS = 0, 1, 2, 3, 4, 5
 SHAPE_LINEAR, SHAPE_CUBIC_SPLINE = 0, 1
@@ -44,7 +46,9 @@ SCHEME_KEYS = {"usl": SCHEME_USL, "musl": SCHEME_MUSL, "usf": SCHEME_USF}
 # method(<style>, <sub-method>, ...) words (update.h:78-83)
 SUB_METHOD_KEYS = {"PIC": SUB_PIC, "FLIP": SUB_FLIP, "APIC": SUB_APIC, "AFLIP": SUB_AFLIP,
                    "ASFLIP": SUB_ASFLIP, "MLS": SUB_MLS}
-METHOD_KEYS = {"ulmpm": METHOD_ULMPM, "tlmpm": METHOD_TLMPM}
+METHOD_KEYS = {"ulmpm": METHOD_ULMPM, "tlmpm": METHOD_TLMPM, "ulcpdi": METHOD_ULCPDI,
+               "tlcpdi": METHOD_TLCPDI}
+CPDI_STYLE_KEYS = {"R4": CPDI_R4, "Q4": CPDI_Q4}   # ulcpdi.h:59
 SHAPE_KEYS = {"linear": SHAPE_LINEAR, "cubic-spline": SHAPE_CUBIC_SPLINE}
 
 D3 = C.c_double * 3
@@ -59,7 +63,7 @@ class Config(C.Structure):
                 ("pic_flip", C.c_double), ("dt", C.c_double), ("dt_constant", C.c_int),
                 ("sub_method", C.c_int), ("dt_factor", C.c_double),
                 ("solidlo", D3), ("solidhi", D3), ("np_per_cell", C.c_int),
-                ("thermo", C.c_int)]
+                ("thermo", C.c_int), ("cpdi_style", C.c_int)]
 
 
 class Material(C.Structure):
@@ -105,7 +109,8 @@ class Particles(C.Structure):
                 ("F", _DP), ("sigma", _DP), ("strain_el", _DP), ("mass", _DP), ("vol0", _DP),
                 ("vol", _DP), ("rho0", _DP), ("rho", _DP), ("J", _DP),
                 ("eff_plastic_strain", _DP), ("eff_plastic_strain_rate", _DP),
-                ("damage", _DP), ("damage_init", _DP), ("T", _DP), ("gamma", _DP)]
+                ("damage", _DP), ("damage_init", _DP), ("T", _DP), ("gamma", _DP),
+                ("rp0", _DP), ("rp", _DP), ("xpc0", _DP), ("xpc", _DP)]
 
 
 # particle arrays: name -> (dtype, components)
@@ -118,12 +123,22 @@ PARTICLE_FIELDS = {
     "eff_plastic_strain_rate": (np.float64, 1), "damage": (np.float64, 1),
     "damage_init": (np.float64, 1), "T": (np.float64, 1), "gamma": (np.float64, 1),
 }
+# CPDI particle domains (solid.h): R4 vectors [np][2][3], Q4 corners [np][4][3];
+# present in a particle dict only for CPDI problems
+CPDI_FIELDS = {"rp0": (np.float64, 6), "rp": (np.float64, 6), "xpc0": (np.float64, 12),
+               "xpc": (np.float64, 12)}
 
 
-def empty_particles(n):
-    """Dict of zeroed particle arrays in the reference layouts."""
+def is_cpdi(cfg):
+    return cfg.method in (METHOD_ULCPDI, METHOD_TLCPDI)
+
+
+def empty_particles(n, cpdi=False):
+    """Dict of zeroed particle arrays in the reference layouts (with the CPDI
+    domain arrays when cpdi)."""
     out = {}
-    for k, (dt, c) in PARTICLE_FIELDS.items():
+    fields = dict(PARTICLE_FIELDS, **CPDI_FIELDS) if cpdi else PARTICLE_FIELDS
+    for k, (dt, c) in fields.items():
         out[k] = np.zeros((n, c) if c > 1 else (n,), dtype=dt)
     return out
 
@@ -135,7 +150,7 @@ def particles_struct(arrs, n=None):
     p.np = int(n if n is not None else len(arrs["ptag"]))
     ctmap = {np.int64: C.c_int64, np.int32: C.c_int32, np.uint32: C.c_uint32,
              np.float64: C.c_double}
-    for k, (dt, _) in PARTICLE_FIELDS.items():
+    for k, (dt, _) in dict(PARTICLE_FIELDS, **CPDI_FIELDS).items():
         a = arrs.get(k)
         if a is None:
             setattr(p, k, None)
@@ -146,7 +161,8 @@ def particles_struct(arrs, n=None):
 
 
 def concat_particles(parts):
-    return {k: np.concatenate([p[k] for p in parts]) for k in PARTICLE_FIELDS}
+    keys = [k for k in dict(PARTICLE_FIELDS, **CPDI_FIELDS) if all(k in p for p in parts)]
+    return {k: np.concatenate([p[k] for p in parts]) for k in keys}
 
 
 EXPR_MAX = 32
@@ -305,7 +321,7 @@ def populate(cfg, region, ppc1d, mat, mat_index, solid_index, ptag_offset, fn=No
     if st not in (OK, ERR_CAPACITY):
         raise KmpmError(st, "populate (count)")
     n = npo.value
-    arrs = empty_particles(n)
+    arrs = empty_particles(n, cpdi=is_cpdi(cfg))
     ps = particles_struct(arrs, n)
     check(fn(C.byref(cfg), C.byref(region), ppc1d, C.byref(mat), mat_index, solid_index,
              ptag_offset, C.byref(ps), n, C.byref(npo)), "populate")
@@ -344,7 +360,7 @@ class Context:
         self.np = ps.np
 
     def download(self):
-        arrs = empty_particles(self.np)
+        arrs = empty_particles(self.np, cpdi=is_cpdi(self.cfg))
         ps = particles_struct(arrs, self.np)
         check(self.L.kmpm_download_particles(self.h, C.byref(ps)), "download")
         return arrs

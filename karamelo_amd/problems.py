@@ -52,11 +52,13 @@ class Problem:
 
 def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-16,
                 dt_constant=False, dt_factor=0.9, method="ulmpm", solid_region=None,
-                sub_method="FLIP", np_per_cell=0, thermo=False):
-    """kmpm_config for dimension/method/scheme/shape commands; for TLMPM the
-    solid's region gives the limits of its grid (Region::limits).  sub_method
-    is method()'s second word (PIC, FLIP, APIC, AFLIP, ASFLIP, MLS); thermo
-    its thermo-mechanical keyword."""
+                sub_method="FLIP", np_per_cell=0, thermo=False, cpdi_style="R4"):
+    """kmpm_config for dimension/method/scheme/shape commands; for TLMPM and
+    TLCPDI the solid's region gives the limits of its grid (Region::limits).
+    method is method()'s first word (ulmpm, tlmpm, ulcpdi, tlcpdi),
+    sub_method its second (PIC, FLIP, APIC, AFLIP, ASFLIP, MLS); thermo its
+    thermo-mechanical keyword; cpdi_style the CPDI methods' style (R4,
+    Q4)."""
     c = N.Config()
     c.dimension = dim
     c.method = N.METHOD_KEYS[method]
@@ -74,6 +76,7 @@ def make_config(dim, lo, hi, h, scheme="usl", shape="linear", flip=0.99, dt=1e-1
     c.sub_method = N.SUB_METHOD_KEYS[sub_method]
     c.np_per_cell = np_per_cell
     c.thermo = 1 if thermo else 0
+    c.cpdi_style = N.CPDI_STYLE_KEYS[cpdi_style]
     if solid_region is not None:
         lo_, hi_ = N.region_limits(solid_region)
         for d in range(3):

@@ -243,12 +243,15 @@ int orc_material_init(kmpm_material *m) {
 /* ------------------------------------------------------------------------ */
 /* grid: Grid::init for one rank, grid.cpp:68-275                            */
 /* ------------------------------------------------------------------------ */
-/* bounds of the grid: the box (ULMPM) or the solid (TLMPM), grid.cpp:91-97 */
+/* total Lagrangian methods: TLMPM and TLCPDI (is_TL, tlcpdi.cpp:26) */
+static int method_tl(int m) { return m == KMPM_METHOD_TLMPM || m == KMPM_METHOD_TLCPDI; }
+static int method_cpdi(int m) { return m == KMPM_METHOD_ULCPDI || m == KMPM_METHOD_TLCPDI; }
+/* bounds of the grid: the box (UL) or the solid (TL), grid.cpp:91-97 */
 static const double *grid_lo(const kmpm_config *c) {
-  return c->method == KMPM_METHOD_TLMPM ? c->solidlo : c->boxlo;
+  return method_tl(c->method) ? c->solidlo : c->boxlo;
 }
 static const double *grid_hi(const kmpm_config *c) {
-  return c->method == KMPM_METHOD_TLMPM ? c->solidhi : c->boxhi;
+  return method_tl(c->method) ? c->solidhi : c->boxhi;
 }
 static void grid_global_counts(const kmpm_config *c, int n[3]) {
   double h = c->cellsize;
@@ -283,7 +286,7 @@ static void grid_local_counts(const kmpm_config *c, int n[3]) {
 int orc_grid_size(const kmpm_config *c, int64_t out[3]) {
   int g[3], l[3];
   if (!c || c->cellsize <= 0) return KMPM_ERR_ARG;
-  if (c->method == KMPM_METHOD_TLMPM)
+  if (method_tl(c->method))
     for (int d = 0; d < c->dimension; d++)
       if (c->solidlo[d] < c->boxlo[d] || c->solidhi[d] > c->boxhi[d] ||
           !(c->solidhi[d] > c->solidlo[d]))
@@ -356,11 +359,11 @@ int orc_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
                  int64_t *np_out) {
   double rlo[3], rhi[3];
   const double *boundlo = c->boxlo, *boundhi = c->boxhi;       /* :1927-1936 */
-  if (c->method == KMPM_METHOD_TLMPM) {
+  if (method_tl(c->method)) {
     orc_region_limits(region, rlo, rhi);                        /* :1831-1838 */
     boundlo = rlo;
     boundhi = rhi;
-  } else if (c->method != KMPM_METHOD_ULMPM) {
+  } else if (c->method != KMPM_METHOD_ULMPM && c->method != KMPM_METHOD_ULCPDI) {
     return KMPM_ERR_ARG;
   }
   const double *sublo = c->boxlo, *subhi = c->boxhi;
@@ -383,6 +386,12 @@ int orc_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
   }
   double vol_ = dim == 1 ? delta : (dim == 2 ? delta * delta : delta * delta * delta);
   double mass_ = mat->rho0 * vol_;                              /* :2019 */
+  /* CPDI half-width of a particle's domain (:2029-2106) */
+  double lp = delta;
+  if (ppc1d == 1) lp *= 0.5;
+  else if (ppc1d == 2) lp *= 0.25;
+  else lp *= 1.0 / (2 * ppc1d);
+  const int cpdi = method_cpdi(c->method), q4 = cpdi && c->cpdi_style == KMPM_CPDI_Q4;
   int nip = 1;
   double ip3[3 * 64];
   memset(ip3, 0, sizeof(ip3));
@@ -450,6 +459,22 @@ int orc_populate(const kmpm_config *c, const kmpm_region *region, int ppc1d,
             out->solid[q] = solid_index;
             out->mat[q] = mat_index;
             out->mask[q] = 1;
+            if (cpdi && !q4) {                                  /* R4, :2188-2205 */
+              for (int e = 0; e < 6; e++) {
+                const double r = e == 0 || e == 4 ? lp : 0.0;
+                if (out->rp0) out->rp0[6 * q + e] = r;
+                if (out->rp) out->rp[6 * q + e] = r;
+              }
+            } else if (q4) {                                    /* Q4, :2208-2222 */
+              const double cx[4] = {x - lp, x + lp, x + lp, x - lp};
+              const double cy[4] = {y - lp, y - lp, y + lp, y + lp};
+              for (int k = 0; k < 4; k++)
+                for (int d = 0; d < 3; d++) {
+                  const double v = d == 0 ? cx[k] : d == 1 ? cy[k] : 0.0;
+                  if (out->xpc0) out->xpc0[12 * q + 3 * k + d] = v;
+                  if (out->xpc) out->xpc[12 * q + 3 * k + d] = v;
+                }
+            }
           }
           l++;
         }
@@ -499,6 +524,11 @@ struct orc_sim {
   int temp;
   double *T, *gamma, *q;
   double *gT, *gTu, *gQint, *gQext;
+  /* CPDI (ulcpdi.cpp, tlcpdi.cpp; 2D): cpdi = 1 + KMPM_CPDI_*, or 0; R4
+   * domain vectors rp0, rp [np][2][3]; Q4 corners xpc0, xpc [np][4][3]; the
+   * corners' weights of each kept pair wfc_pn [np][S][4] (wf_pn_corners) */
+  int cpdi;
+  double *rp0, *rp, *xpc0, *xpc, *wfc_pn;
   /* neighbour lists: particle-major (neigh_pn) and node-major CSR (neigh_np) */
   int32_t *nn_pn; int64_t *neigh_pn; double *wf_pn, *wfd_pn;
   int64_t *np_start; int64_t *neigh_np; double *wf_np, *wfd_np; int64_t npairs_cap;
@@ -527,15 +557,22 @@ static void set_status(orc_sim *s, int code, int64_t tag) {
 orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
   int64_t n3[3];
   if (orc_grid_size(c, n3) != KMPM_OK) return NULL;
-  if ((c->method != KMPM_METHOD_ULMPM && c->method != KMPM_METHOD_TLMPM) ||
+  if (c->method < KMPM_METHOD_ULMPM || c->method > KMPM_METHOD_TLCPDI ||
       (c->dimension != 2 && c->dimension != 3))
+    return NULL;
+  const int cpdi = c->method == KMPM_METHOD_ULCPDI || c->method == KMPM_METHOD_TLCPDI;
+  /* the CPDI methods are 2D (ulcpdi.cpp:115-118) and take the PIC/FLIP
+   * transfers here */
+  if (cpdi && (c->dimension != 2 || c->sub_method >= KMPM_SUB_APIC ||
+               (c->cpdi_style != KMPM_CPDI_R4 && c->cpdi_style != KMPM_CPDI_Q4)))
     return NULL;
   if (c->scheme < KMPM_SCHEME_USL || c->scheme > KMPM_SCHEME_USF ||
       c->sub_method < KMPM_SUB_PIC || c->sub_method > KMPM_SUB_MLS)
     return NULL;
   orc_sim *s = (orc_sim *)xcalloc(1, sizeof(orc_sim));
   s->c = *c;
-  s->tl = c->method == KMPM_METHOD_TLMPM;
+  s->tl = c->method == KMPM_METHOD_TLMPM || c->method == KMPM_METHOD_TLCPDI;
+  s->cpdi = cpdi ? 1 + c->cpdi_style : 0;
   /* Update::modify_method (update.cpp:129-146), ULMPM::setup
    * (ulmpm.cpp:79-85), TLMPM::setup (tlmpm.cpp:82-84) */
   {
@@ -570,6 +607,7 @@ orc_sim *orc_create(const kmpm_config *c, const kmpm_material *mats, int nmat) {
   s->dim = c->dimension;
   s->S1 = c->shape == KMPM_SHAPE_LINEAR ? 2 : 4;
   s->S = s->dim == 3 ? s->S1 * s->S1 * s->S1 : s->S1 * s->S1;
+  if (s->cpdi) s->S = KMPM_CPDI_MAX_SPAN * KMPM_CPDI_MAX_SPAN;   /* the corners' union */
   s->h = c->cellsize;
   s->inv_h = 1.0 / s->h;                                        /* ulmpm.cpp:122 */
   for (int d = 0; d < 3; d++) s->n[d] = (int)n3[d];
@@ -620,7 +658,8 @@ static void free_particles(orc_sim *s) {
                   s->F, s->sig, s->eel, s->L, s->D, s->mass, s->vol0, s->vol, s->rho0,
                   s->rho, s->J, s->ep, s->epdot, s->dmg, s->dinit, s->nn_pn, s->neigh_pn,
                   s->wf_pn, s->wfd_pn, s->neigh_np, s->wf_np, s->wfd_np, s->cell_ijk,
-                  s->Fdot, s->Finv, s->R, s->PK1, s->T, s->gamma, s->q};
+                  s->Fdot, s->Finv, s->R, s->PK1, s->T, s->gamma, s->q,
+                  s->rp0, s->rp, s->xpc0, s->xpc, s->wfc_pn};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
 }
 
@@ -658,6 +697,8 @@ int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
   A(dmg, double, 1); A(dinit, double, 1);
   A(Fdot, double, 9); A(Finv, double, 9); A(R, double, 9); A(PK1, double, 9);
   A(T, double, 1); A(gamma, double, 1); A(q, double, 3);
+  A(rp0, double, 6); A(rp, double, 6); A(xpc0, double, 12); A(xpc, double, 12);
+  A(wfc_pn, double, s->cpdi ? 4 * S : 1);
   A(nn_pn, int32_t, 1); A(neigh_pn, int64_t, S); A(wf_pn, double, S); A(wfd_pn, double, 3 * S);
   A(cell_ijk, int32_t, 3);
 #undef A
@@ -697,6 +738,19 @@ int orc_set_particles(orc_sim *s, const kmpm_particles *p) {
     s->dmg[i] = p->damage[i];
     s->dinit[i] = p->damage_init[i];
     s->T[i] = p->T ? p->T[i] : 0.0;       /* gamma = q = 0 (solid.cpp:2305-2309) */
+    if (s->cpdi == 1 + KMPM_CPDI_R4) {
+      if (!p->rp0) return KMPM_ERR_ARG;
+      for (int e = 0; e < 6; e++) {
+        s->rp0[6 * i + e] = p->rp0[6 * i + e];
+        s->rp[6 * i + e] = p->rp ? p->rp[6 * i + e] : p->rp0[6 * i + e];
+      }
+    } else if (s->cpdi == 1 + KMPM_CPDI_Q4) {
+      if (!p->xpc0) return KMPM_ERR_ARG;
+      for (int e = 0; e < 12; e++) {
+        s->xpc0[12 * i + e] = p->xpc0[12 * i + e];
+        s->xpc[12 * i + e] = p->xpc ? p->xpc[12 * i + e] : p->xpc0[12 * i + e];
+      }
+    }
     if (s->tl) tl_particle_pk1(s, i);
   }
   return KMPM_OK;
@@ -731,6 +785,14 @@ int orc_get_particles(orc_sim *s, kmpm_particles *p) {
     if (p->damage_init) p->damage_init[i] = s->dinit[i];
     if (p->T) p->T[i] = s->temp ? s->T[i] : 0.0;
     if (p->gamma) p->gamma[i] = s->temp ? s->gamma[i] : 0.0;
+    for (int e = 0; e < 6; e++) {
+      if (p->rp0) p->rp0[6 * i + e] = s->rp0[6 * i + e];
+      if (p->rp) p->rp[6 * i + e] = s->rp[6 * i + e];
+    }
+    for (int e = 0; e < 12; e++) {
+      if (p->xpc0) p->xpc0[12 * i + e] = s->xpc0[12 * i + e];
+      if (p->xpc) p->xpc[12 * i + e] = s->xpc[12 * i + e];
+    }
   }
   return KMPM_OK;
 }
@@ -819,8 +881,173 @@ int orc_fix_body_force(orc_sim *s, uint32_t bit, const int set[3], const double 
 /* weights: ULMPM::compute_grid_weight_functions_and_gradients,              */
 /* ulmpm.cpp:88-337 (one rank: map_ntag is the identity on [0,nnodes))       */
 /* ------------------------------------------------------------------------ */
+/* The corners of particle ip's domain, 2D (ulcpdi.cpp:183-222,
+ * tlcpdi.cpp:170-196): R4 x -+ r1 -+ r2 from the current vectors (ULCPDI)
+ * or, TLCPDI, those of its one weights pass (the reference ones of a fresh
+ * run) about x0; Q4 the tracked corners (TLCPDI: their reference
+ * positions). */
+static void cpdi_corners(const orc_sim *s, int64_t ip, double xc[4][3]) {
+  if (s->cpdi == 1 + KMPM_CPDI_R4) {
+    const double *xp = s->tl ? &s->x0[3 * ip] : &s->x[3 * ip];
+    const double *r = s->tl ? &s->rp0[6 * ip] : &s->rp[6 * ip];
+    for (int d = 0; d < 3; d++) {
+      xc[0][d] = xp[d] - r[d] - r[3 + d];
+      xc[1][d] = xp[d] + r[d] - r[3 + d];
+      xc[2][d] = xp[d] + r[d] + r[3 + d];
+      xc[3][d] = xp[d] - r[d] + r[3 + d];
+    }
+  } else {
+    const double *c = s->tl ? &s->xpc0[12 * ip] : &s->xpc[12 * ip];
+    for (int k = 0; k < 4; k++)
+      for (int d = 0; d < 3; d++) xc[k][d] = c[3 * k + d];
+  }
+}
+
+static int cmp_i64(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+/* CPDI weights and gradients (ULCPDI::compute_grid_weight_functions_and_
+ * gradients ulcpdi.cpp:111-393, TLCPDI tlcpdi.cpp:96-329; 2D).  A
+ * particle's candidate nodes are the union of its corners' stencils, taken
+ * from (corner - boxlo) / h even on TLCPDI's solid grid, in ascending tag;
+ * per node the corners' tensor-product weights wfc give
+ *   R4: wf = 1/4 sum of the wfc above 1e-12, and
+ *       wfd = ((wfc0 - wfc2)(r1y - r2y) + (wfc1 - wfc3)(r1y + r2y),
+ *              (wfc0 - wfc2)(r2x - r1x) - (wfc1 - wfc3)(r1x + r2x)) / V;
+ *   Q4: wf = 0.0417/V sum (6V -+ a -+ b) wfc (the reference's rounded 1/24;
+ *       a, b from the corners) and wfd = 1/(2V) sum_c wfc_c (y_{c+1} -
+ *       y_{c-1}, x_{c-1} - x_{c+1});
+ * and the node is kept where wf > 1e-12. */
+static void cpdi_weights(orc_sim *s) {
+  const double *blo = s->c.boxlo;
+  const double inv_h = s->inv_h;
+  const int ny = s->n[1], lin = s->c.shape == KMPM_SHAPE_LINEAR, m = lin ? 2 : 4;
+  const int q4 = s->cpdi == 1 + KMPM_CPDI_Q4;
+  OMP_FOR
+  for (int64_t in = 0; in <= s->nn; in++) s->np_start[in] = 0;
+  OMP_FOR
+  for (int64_t ip = 0; ip < s->np; ip++) {
+    double xc[4][3];
+    cpdi_corners(s, ip, xc);
+    int64_t cand[64];
+    int nc = 0, span[2][2] = {{1 << 30, -(1 << 30)}, {1 << 30, -(1 << 30)}};
+    for (int c = 0; c < 4; c++) {                               /* :224-316 */
+      const int i0 = lin ? (int)((xc[c][0] - blo[0]) * inv_h) : (int)((xc[c][0] - blo[0]) * inv_h - 1);
+      const int j0 = lin ? (int)((xc[c][1] - blo[1]) * inv_h) : (int)((xc[c][1] - blo[1]) * inv_h - 1);
+      const int lo2[2] = {i0, j0};
+      for (int d = 0; d < 2; d++) {
+        const int a = lo2[d] < 0 ? 0 : lo2[d], b = lo2[d] + m - 1 > s->n[d] - 1 ? s->n[d] - 1 : lo2[d] + m - 1;
+        if (a < span[d][0]) span[d][0] = a;
+        if (b > span[d][1]) span[d][1] = b;
+      }
+      for (int i = i0; i < i0 + m; i++)
+        for (int j = j0; j < j0 + m; j++) {
+          const int64_t n = (int64_t)ny * i + j;
+          /* a candidate off the grid's axes names another node through the
+           * tag arithmetic, one far from the corners, whose weights are 0:
+           * it is dropped here */
+          if (n < s->nn && i >= 0 && i < s->n[0] && j >= 0 && j < ny) cand[nc++] = n;
+        }
+    }
+    /* the device kernels hold KMPM_CPDI_MAX_SPAN nodes per axis of the
+     * candidates' extent on the grid */
+    if (span[0][1] - span[0][0] >= KMPM_CPDI_MAX_SPAN ||
+        span[1][1] - span[1][0] >= KMPM_CPDI_MAX_SPAN) {
+      set_status(s, KMPM_ERR_DOMAIN, s->ptag[ip]);
+      s->nn_pn[ip] = 0;
+      continue;
+    }
+    qsort(cand, nc, sizeof(int64_t), cmp_i64);                 /* :318-321 */
+    int nu = 0;
+    for (int k = 0; k < nc; k++)
+      if (nu == 0 || cand[k] != cand[nu - 1]) cand[nu++] = cand[k];
+    const double Vp = s->vol[ip], inv_Vp = 1.0 / Vp;           /* :329 */
+    double qa = 0, qb = 0, alpha_over_Vp = 0, sixVp = 0;
+    if (q4) {                                                   /* :331-340 */
+      qa = (xc[3][0] - xc[0][0]) * (xc[1][1] - xc[2][1]) -
+           (xc[1][0] - xc[2][0]) * (xc[3][1] - xc[0][1]);
+      qb = (xc[2][0] - xc[3][0]) * (xc[0][1] - xc[1][1]) -
+           (xc[0][0] - xc[1][0]) * (xc[2][1] - xc[3][1]);
+      alpha_over_Vp = 0.0417 * inv_Vp;
+      sixVp = 6 * Vp;
+    }
+    const double *r = &s->rp[6 * ip];
+    if (s->tl) r = &s->rp0[6 * ip];
+    int cnt = 0;
+    for (int k = 0; k < nu; k++) {
+      const int64_t in = cand[k];
+      double wfc[4], wf = 0;
+      for (int c = 0; c < 4; c++) {                             /* :345-357 */
+        double rr[2];
+        for (int d = 0; d < 2; d++) rr[d] = (xc[c][d] - s->gx0[3 * in + d]) * inv_h;
+        const int *nt = &s->gntype[3 * in];
+        double p0 = orc_basis(s->c.shape, 0, rr[0], nt[0], inv_h);
+        double p1 = orc_basis(s->c.shape, 0, rr[1], nt[1], inv_h);
+        wfc[c] = p0 * p1 * 1;
+        if (!q4 && wfc[c] > 1.0e-12) wf += wfc[c];
+      }
+      if (!q4) wf *= 0.25;
+      else
+        wf = alpha_over_Vp * ((sixVp - qa - qb) * wfc[0] + (sixVp - qa + qb) * wfc[1] +
+                              (sixVp + qa + qb) * wfc[2] + (sixVp + qa - qb) * wfc[3]);
+      if (!(wf > 1.0e-12)) continue;
+      double wd[3];
+      if (!q4) {                                                /* :369-378 */
+        wd[0] = (wfc[0] - wfc[2]) * (r[1] - r[3 + 1]) + (wfc[1] - wfc[3]) * (r[1] + r[3 + 1]);
+        wd[1] = (wfc[0] - wfc[2]) * (r[3 + 0] - r[0]) - (wfc[1] - wfc[3]) * (r[0] + r[3 + 0]);
+        wd[2] = 0;
+        for (int d = 0; d < 3; d++) wd[d] *= inv_Vp;
+      } else {                                                  /* :380-394 */
+        wd[0] = wfc[0] * (xc[1][1] - xc[3][1]) + wfc[1] * (xc[2][1] - xc[0][1]) +
+                wfc[2] * (xc[3][1] - xc[1][1]) + wfc[3] * (xc[0][1] - xc[2][1]);
+        wd[1] = wfc[0] * (xc[3][0] - xc[1][0]) + wfc[1] * (xc[0][0] - xc[2][0]) +
+                wfc[2] * (xc[1][0] - xc[3][0]) + wfc[3] * (xc[2][0] - xc[0][0]);
+        wd[2] = 0;
+        for (int d = 0; d < 3; d++) wd[d] *= 0.5 * inv_Vp;
+      }
+      const int64_t q = ip * s->S + cnt;
+      s->neigh_pn[q] = in;
+      s->wf_pn[q] = wf;
+      for (int d = 0; d < 3; d++) s->wfd_pn[3 * q + d] = wd[d];
+      for (int c = 0; c < 4; c++) s->wfc_pn[4 * q + c] = wfc[c];
+      OMP_ATOMIC
+      s->np_start[in + 1]++;
+      cnt++;
+    }
+    s->nn_pn[ip] = cnt;
+    s->cell_ijk[3 * ip] = s->cell_ijk[3 * ip + 1] = s->cell_ijk[3 * ip + 2] = 0;
+  }
+}
+
+/* node-major lists (neigh_np) in ascending particle order, as the
+ * push_back at ulmpm.cpp:269-275 builds them (np_start holds the counts) */
+static int node_lists(orc_sim *s) {
+  for (int64_t in = 0; in < s->nn; in++) s->np_start[in + 1] += s->np_start[in];
+  int64_t *fill = (int64_t *)xcalloc(s->nn, sizeof(int64_t));
+  OMP_FOR
+  for (int64_t ip = 0; ip < s->np; ip++)
+    for (int c = 0; c < s->nn_pn[ip]; c++) {
+      int64_t q = ip * s->S + c, in = s->neigh_pn[q], k;
+      OMP_ATOMIC_CAPTURE
+      k = fill[in]++;
+      int64_t o = s->np_start[in] + k;
+      s->neigh_np[o] = ip;
+      s->wf_np[o] = s->wf_pn[q];
+      for (int d = 0; d < 3; d++) s->wfd_np[3 * o + d] = s->wfd_pn[3 * q + d];
+    }
+  free(fill);
+  s->wf_done = 1;
+  return s->status;
+}
+
 int orc_phase_weights(orc_sim *s) {
   if (s->tl && s->wf_done) return KMPM_OK;                      /* tlmpm.cpp:89,339 */
+  if (s->cpdi) {
+    cpdi_weights(s);
+    return node_lists(s);
+  }
   const double *lo = grid_lo(&s->c);
   double inv_h = s->inv_h;
   int nx = s->n[0], ny = s->n[1], nz = s->n[2];
@@ -901,24 +1128,7 @@ int orc_phase_weights(orc_sim *s) {
     }
     s->nn_pn[ip] = cnt;
   }
-  /* node-major lists (neigh_np) in ascending particle order, as the
-   * push_back at :269-275 builds them */
-  for (int64_t in = 0; in < s->nn; in++) s->np_start[in + 1] += s->np_start[in];
-  int64_t *fill = (int64_t *)xcalloc(s->nn, sizeof(int64_t));
-  OMP_FOR
-  for (int64_t ip = 0; ip < s->np; ip++)
-    for (int c = 0; c < s->nn_pn[ip]; c++) {
-      int64_t q = ip * s->S + c, in = s->neigh_pn[q], k;
-      OMP_ATOMIC_CAPTURE
-      k = fill[in]++;
-      int64_t o = s->np_start[in] + k;
-      s->neigh_np[o] = ip;
-      s->wf_np[o] = s->wf_pn[q];
-      for (int d = 0; d < 3; d++) s->wfd_np[3 * o + d] = s->wfd_pn[3 * q + d];
-    }
-  free(fill);
-  s->wf_done = 1;
-  return KMPM_OK;
+  return node_lists(s);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1229,15 +1439,28 @@ static int phase_g2p(orc_sim *s) {
   OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++) {
     double *vu = &s->vu[3 * ip], *a = &s->a[3 * ip];
+    double vc[4][3] = {{0}};
+    const int q4 = s->cpdi == 1 + KMPM_CPDI_Q4;
     for (int d = 0; d < 3; d++) vu[d] = a[d] = 0;
     for (int c = 0; c < s->nn_pn[ip]; c++) {
       int64_t q = ip * s->S + c, in = s->neigh_pn[q];
       double w = s->wf_pn[q];
       for (int d = 0; d < 3; d++) vu[d] += w * s->gvu[3 * in + d];
       for (int d = 0; d < 3; d++) a[d] += w * (s->gvu[3 * in + d] - s->gv[3 * in + d]);
+      /* CPDI (Solid::compute_particle_velocities_and_positions
+       * solid.cpp:696-764): x moves node by node, Q4 corners by their own
+       * weights */
+      if (s->cpdi)
+        for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * w * s->gvu[3 * in + d];
+      if (q4)
+        for (int k = 0; k < 4; k++)
+          for (int d = 0; d < 3; d++) vc[k][d] += s->wfc_pn[4 * q + k] * s->gvu[3 * in + d];
     }
     for (int d = 0; d < 3; d++) a[d] *= inv_dt;
-    if (!s->asflip)
+    if (q4)
+      for (int k = 0; k < 4; k++)
+        for (int d = 0; d < 3; d++) s->xpc[12 * ip + 3 * k + d] += s->dt * vc[k][d];
+    if (!s->asflip && !s->cpdi)
       for (int d = 0; d < 3; d++) s->x[3 * ip + d] += s->dt * vu[d];
     const double *x = &s->x[3 * ip];
     if (!s->tl &&        /* the box check is UL only (solid.cpp:617-627) */
@@ -1517,8 +1740,15 @@ static int phase_F_and_stress(orc_sim *s, int doublemapping) {
       memcpy(F, Fn, sizeof(Fn));
     }
     if (s->tl) m3_inverse(F, &s->Finv[9 * ip]);                 /* :1186 */
-    s->J[ip] = det3(F);
-    s->vol[ip] = s->J[ip] * s->vol0[ip];
+    if (s->cpdi == 1 + KMPM_CPDI_Q4) {                          /* :1188-1200 */
+      const double *c = &s->xpc[12 * ip];
+      s->vol[ip] = 0.5 * (c[0] * c[4] - c[3] * c[1] + c[3] * c[7] - c[6] * c[4] +
+                          c[6] * c[10] - c[9] * c[7] + c[9] * c[1] - c[0] * c[10]);
+      s->J[ip] = s->vol[ip] / s->vol0[ip];
+    } else {
+      s->J[ip] = det3(F);
+      s->vol[ip] = s->J[ip] * s->vol0[ip];
+    }
     if (s->J[ip] <= 0.0 && s->dmg[ip] < 1.0) {
       set_status(s, KMPM_ERR_JACOBIAN, s->ptag[ip]);
     }
@@ -1539,6 +1769,15 @@ static int phase_F_and_stress(orc_sim *s, int doublemapping) {
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) D[3 * i + j] = 0.5 * (L[3 * i + j] + L[3 * j + i]);
     }
+  }
+  if (s->cpdi == 1 + KMPM_CPDI_R4 && !s->tl) {    /* update_particle_domain solid.cpp:2338-2352 */
+    OMP_FOR
+    for (int64_t ip = 0; ip < s->np; ip++)
+      for (int k = 0; k < 2; k++) {
+        const double *F = &s->F[9 * ip], *r0 = &s->rp0[6 * ip + 3 * k];
+        for (int i = 0; i < 3; i++)
+          s->rp[6 * ip + 3 * k + i] = F[3 * i] * r0[0] + F[3 * i + 1] * r0[1] + F[3 * i + 2] * r0[2];
+      }
   }
   OMP_FOR
   for (int64_t ip = 0; ip < s->np; ip++)

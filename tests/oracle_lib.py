@@ -135,7 +135,7 @@ class Oracle:
         self.np = ps.np
 
     def download(self):
-        arrs = N.empty_particles(self.np)
+        arrs = N.empty_particles(self.np, cpdi=N.is_cpdi(self.cfg))
         ps = N.particles_struct(arrs, self.np)
         assert self.L.orc_get_particles(self.h, C.byref(ps)) == 0
         return arrs
@@ -198,7 +198,7 @@ class Oracle:
 
     def neighbours(self):
         S1 = 2 if self.cfg.shape == N.SHAPE_LINEAR else 4
-        S = S1 ** self.cfg.dimension
+        S = N.CPDI_MAX_SPAN ** 2 if N.is_cpdi(self.cfg) else S1 ** self.cfg.dimension
         nn = np.zeros(self.np, dtype=np.int32)
         tags = np.zeros((self.np, S), dtype=np.int64)
         w = np.zeros((self.np, S))
