@@ -538,6 +538,30 @@ __global__ void k_cn_reduce(float *p, const float *partial, int nblocks, float *
 // take the networks, the others the plain FP64 functions.  MODE 2: one wave
 // in judge_stride (rotating with the step) also runs the plain strength for
 // its network lanes and sums the squared deviatoric error.
+// One particle's inputs to the learned update: its stress state (from the
+// current slot order[j]) and what the G2P kernel staged at j.
+struct LearnedIn {
+  float sig[6], eel[6], Dr[6];
+  float ep, epdot, dinit, dmg, pH, cwave;
+};
+__device__ inline void learned_load(const KArgs &a, int j, LearnedIn &in) {
+  constexpr int PREC = KMPM_PRECISION_MIXED;
+  const int slot = a.order[j];
+  const float *st = a.cut_stage;
+  const size_t cs = (size_t)a.cap;
+  for (int e = 0; e < 6; e++) {
+    in.sig[e] = (float)LDA(SXX + e, slot);
+    in.eel[e] = (float)LDA(EXX + e, slot);
+    in.Dr[e] = st[e * cs + j];
+  }
+  in.ep = (float)LDA(EP, slot);
+  in.epdot = (float)LDA(EPDOT, slot);
+  in.dinit = (float)LDA(DINIT, slot);
+  in.dmg = (float)LDA(DMG, slot);
+  in.pH = st[6 * cs + j];
+  in.cwave = st[7 * cs + j];
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256, 2) void k_stress_learned(KArgs a) {
   constexpr int PREC = KMPM_PRECISION_MIXED;
@@ -561,26 +585,26 @@ __global__ __launch_bounds__(256, 2) void k_stress_learned(KArgs a) {
   const float dtf = (float)dt;
   const float tav = (float)(1000 * a.h / m.signal_velocity);          // solid.cpp:1324-1329
   const int64_t step = a.sc->step;
-  const float *st = a.cut_stage;
-  const size_t cs = (size_t)a.cap;
   double cmax = 0.0;
   SurrCount cnt = {0, 0};
   unsigned long long jn = 0;
   double je2 = 0.0, jr2 = 0.0;
-  for (int base = blockIdx.x * blockDim.x; base < a.np; base += gridDim.x * blockDim.x) {
+  // The next round's inputs are loaded before this round's updates, so
+  // their memory latency overlaps the networks (the state is read from the
+  // current buffers and written to the next ones: no load waits on a store).
+  const int stride = gridDim.x * blockDim.x;
+  LearnedIn nxt;
+  if ((int)(blockIdx.x * blockDim.x) < a.np)
+    learned_load(a, min((int)(blockIdx.x * blockDim.x + threadIdx.x), a.np - 1), nxt);
+  for (int base = blockIdx.x * blockDim.x; base < a.np; base += stride) {
     const int j0 = base + (int)threadIdx.x;
     const bool live = j0 < a.np;
     const int j = live ? j0 : a.np - 1;
-    const int slot = a.order[j];
-    float sig[6], eel[6], Dr[6];
-    for (int e = 0; e < 6; e++) {
-      sig[e] = (float)LDA(SXX + e, slot);
-      eel[e] = (float)LDA(EXX + e, slot);
-      Dr[e] = st[e * cs + j];
-    }
-    float ep = (float)LDA(EP, slot), epdot = (float)LDA(EPDOT, slot);
-    float dinit = (float)LDA(DINIT, slot), dmg = (float)LDA(DMG, slot);
-    const float pH = st[6 * cs + j], cwave = st[7 * cs + j];
+    LearnedIn in = nxt;
+    if (base + stride < a.np) learned_load(a, min(j0 + stride, a.np - 1), nxt);
+    float *sig = in.sig, *eel = in.eel, *Dr = in.Dr;
+    float ep = in.ep, epdot = in.epdot, dinit = in.dinit, dmg = in.dmg;
+    const float pH = in.pH, cwave = in.cwave;
     if constexpr (MODE == 1) {
       update_stress_jc_surrogate_p(m, SD, live, dt, a.h, pH, Dr, sig, eel, ep, epdot, dinit, dmg,
                                    cnt);
@@ -697,8 +721,21 @@ __global__ __launch_bounds__(256, 2) void k_stress_learned(KArgs a) {
 
 hipError_t launch_learned_stress(kmpm_ctx *c) {
   if (c->a.np == 0) return hipSuccess;
+  // One round of resident workgroups: a grid larger than what fits at once
+  // would run its last blocks at a fraction of the occupancy.
+  static int resident[2] = {0, 0};
+  const int mode = c->a.cut_on ? 1 : 0;
+  if (resident[mode] == 0) {
+    int n = 0;
+    const hipError_t e =
+        c->a.cut_on
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stress_learned<2>, 256, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stress_learned<1>, 256, 0);
+    if (e != hipSuccess) return e;
+    resident[mode] = n > 0 ? n : 1;
+  }
   int g = (c->a.np + 255) / 256;
-  if (g > c->cu_count * 4) g = c->cu_count * 4;
+  if (g > c->cu_count * resident[mode]) g = c->cu_count * resident[mode];
   if (c->a.cut_on)
     hipLaunchKernelGGL(k_stress_learned<2>, dim3(g), dim3(256), 0, c->stream, c->a);
   else

@@ -488,9 +488,11 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
   static_assert(64 % GL == 0, "stencil shape");
 
   __shared__ BT blk[NF * PBN];
-  __shared__ R2 stw[CHK][DIM * S1];     // (w, dw) per axis and stencil offset; (w, g) for
-                                        // the MLS force and the APIC re-map
-  __shared__ R stg[GT ? CHK : 1][GT ? DIM * S1 : 1];   // APIC: g per axis and offset
+  // Rows of the stage tables are padded to an odd number of elements, so
+  // the stage's per-particle row writes fall on distinct LDS banks.
+  __shared__ R2 stw[CHK][DIM * S1 + 1];   // (w, dw) per axis and stencil offset; (w, g) for
+                                          // the MLS force and the APIC re-map
+  __shared__ R stg[GT ? CHK : 1][GT ? DIM * S1 + 1 : 1];   // APIC: g per axis and offset
   __shared__ R spay[CHK][NPAY + 1];     // +1: rows of distinct particles in distinct banks
   __shared__ int scl[CHK];              // block offset of the particle's cell
   __shared__ int seg[CHK + 1];          // segment starts; seg[nseg] = chunk size

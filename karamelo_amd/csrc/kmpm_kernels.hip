@@ -16,7 +16,6 @@
 //            lanes = particles: gather, L, x/v update, F, stress, CFL, next
 //            cell key; state is written in sorted order (the permutation is
 //            applied for free)
-#include <hipcub/hipcub.hpp>
 
 #include "kmpm_kernels.h"
 
@@ -597,10 +596,76 @@ hipError_t launch_keys(kmpm_ctx *c) {
   return hipGetLastError();
 }
 
+// Exclusive scan of the per-tile counts (n = tiles + 1 values, the last
+// zero) into the tiles' first positions, in two passes with no waits between
+// workgroups: k_scan_sums writes the total of each chunk of SCAN_CHUNK
+// counts; k_scan_apply adds the totals of the chunks before its own and
+// scans its chunk (SCAN_PPT consecutive counts per thread).
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_PPT = 8;
+constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_PPT;
+
+__device__ inline int wave_sum_i(int v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Sum of v over the block (every thread gets it); ws holds 4 ints.
+__device__ inline int block_sum_i(int v, int *ws) {
+  v = wave_sum_i(v);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int s = ws[0] + ws[1] + ws[2] + ws[3];
+  __syncthreads();
+  return s;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(const int *cnt, int n, int *part) {
+  __shared__ int ws[4];
+  const int b0 = blockIdx.x * SCAN_CHUNK;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PPT; k++) {
+    const int i = b0 + k * SCAN_BLOCK + (int)threadIdx.x;
+    s += i < n ? cnt[i] : 0;
+  }
+  s = block_sum_i(s, ws);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const int *cnt, int n,
+                                                           const int *part, int *out) {
+  __shared__ int ws[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int o = 0;
+  for (int q = tid; q < (int)blockIdx.x; q += SCAN_BLOCK) o += part[q];
+  const int base = block_sum_i(o, ws);
+  const int i0 = blockIdx.x * SCAN_CHUNK + tid * SCAN_PPT;
+  int v[SCAN_PPT], t = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PPT; k++) {
+    v[k] = i0 + k < n ? cnt[i0 + k] : 0;
+    t += v[k];
+  }
+  // exclusive prefix of the thread totals: inside the wave, then the waves
+  int x = t;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) ws[wave] = x;
+  __syncthreads();
+  int ex = base + x - t;
+  for (int w = 0; w < wave; w++) ex += ws[w];
+#pragma unroll
+  for (int k = 0; k < SCAN_PPT; k++) {
+    if (i0 + k < n) out[i0 + k] = ex;
+    ex += v[k];
+  }
+}
+
 size_t scan_temp_bytes(int64_t n) {
-  size_t bytes = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int *)nullptr, (int *)nullptr, (int)n);
-  return bytes;
+  return (size_t)((n + SCAN_CHUNK - 1) / SCAN_CHUNK) * sizeof(int);
 }
 
 hipError_t launch_sort(kmpm_ctx *c) {
@@ -611,9 +676,13 @@ hipError_t launch_sort(kmpm_ctx *c) {
                        a);
   else
     hipLaunchKernelGGL(k_sort_begin, dim3(1), dim3(64), 0, c->stream, a);
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, c->scan_tmp_bytes, a.tile_np,
-                                                  a.tile_p0, (int)(nt + 1), c->stream);
-  if (e != hipSuccess) return e;
+  {
+    const int n = (int)(nt + 1), nb = (int)nblk(n, SCAN_CHUNK);
+    int *part = static_cast<int *>(c->d_scan_tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, a.tile_np, n, part);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, a.tile_np, n, part,
+                       a.tile_p0);
+  }
   hipLaunchKernelGGL(k_tiles, dim3(nblk(nt + 1, 256)), dim3(256), 0, c->stream, a);
   if (a.np > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblk(a.np, SORT_PER_BLOCK)), dim3(SORT_BLOCK), 0,

@@ -40,9 +40,13 @@ static_assert(sizeof(MlpParams) == MLP_NPARAM * sizeof(float), "flat parameter l
 // coefficients of 1, x0, x1, x0^2, x0 x1, x1^2).  The network learns the
 // residual over the baseline:
 //   target = base(x) + out_shift + out_scale * z.
+// W2's BF16 rows are padded to MLP_W2_LD values (80 bytes): the A-operand
+// reads of 16 lanes (16 bytes each, one row per lane) then fall on 16
+// distinct groups of 4 LDS banks.
+constexpr int MLP_W2_LD = MLP_H + 8;
 struct MlpNet {
   MlpParams p;
-  uint16_t w2_bf16[MLP_H][MLP_H];
+  uint16_t w2_bf16[MLP_H][MLP_W2_LD];
   float in_scale[MLP_IN], in_shift[MLP_IN];   // feature -> network input
   float out_scale, out_shift;                 // network output -> residual
   float base[6];
