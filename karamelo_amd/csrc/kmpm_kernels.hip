@@ -33,8 +33,8 @@ __global__ void k_keys(KArgs a) {
 }
 
 // Resets the step's tile-list counters when no particle runs k_count (which
-// resets them otherwise).  Per-tile counts are zero between sorts: k_tiles
-// clears them once the scan has consumed them.
+// resets them otherwise).  Per-tile counts are zero between sorts: k_scatter
+// clears them once k_tiles has consumed them.
 __global__ void k_sort_begin(KArgs a) {
   if (frozen(a)) return;
   if (threadIdx.x == 0) {
@@ -149,6 +149,9 @@ __device__ inline void scatter_rows(const KArgs &a, int i0, int lane) {
 // is settled by k_tile_sort.
 __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(KArgs a) {
   if (frozen(a)) return;
+  // the per-tile counts are zero between sorts (k_tiles has read them all)
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= a.tg.ntiles; t += gridDim.x * blockDim.x)
+    a.tile_np[t] = 0;
   const int lane = threadIdx.x & 63;
   const int w0 = blockIdx.x * SORT_PER_BLOCK + (threadIdx.x >> 6) * SORT_PER_WAVE;
   if (w0 >= a.np) return;
@@ -156,24 +159,67 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(KArgs a) {
   else scatter_rows<false>(a, w0 + lane, lane);
 }
 
-// Active tiles (particles in the tile) and the tiles whose owned nodes lie in
-// the node block of an active tile: the active tile itself and its
-// neighbours at -1 (and +1 for cubic) per axis.  Sets the scatter cursors
-// and clears the per-tile counts for the next sort (the scan has consumed
-// them).
-__global__ void k_tiles(KArgs a) {
+// Tile offsets: an exclusive scan of the per-tile counts (tiles + 1 values,
+// the last zero) in two passes with no waits between workgroups.
+// k_scan_sums writes the total of each chunk of SCAN_BLOCK counts; k_tiles
+// adds the totals of the chunks before its own to a scan of its chunk.
+constexpr int SCAN_BLOCK = 256;
+
+__device__ inline int wave_sum_i(int v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Sum of v over the block (every thread gets it); ws holds 4 ints.
+__device__ inline int block_sum_i(int v, int *ws) {
+  v = wave_sum_i(v);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int s = ws[0] + ws[1] + ws[2] + ws[3];
+  __syncthreads();
+  return s;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(const int *cnt, int n, int *part) {
+  __shared__ int ws[4];
+  const int i = blockIdx.x * SCAN_BLOCK + (int)threadIdx.x;
+  const int s = block_sum_i(i < n ? cnt[i] : 0, ws);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// One thread per tile: its first position (the scan), then the active tiles
+// (particles in the tile) and the tiles whose owned nodes lie in the node
+// block of an active tile: the active tile itself and its neighbours at -1
+// (and +1 for cubic) per axis, from the counts.  Sets the scatter cursors.
+// The counts are cleared by k_scatter, after every tile has read its
+// neighbours'.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_tiles(KArgs a, const int *part) {
   if (frozen(a)) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int ws[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int t = blockIdx.x * SCAN_BLOCK + tid;
   const TileGeom &g = a.tg;
+  int o = 0;
+  for (int q = tid; q < (int)blockIdx.x; q += SCAN_BLOCK) o += part[q];
+  const int base = block_sum_i(o, ws);
+  const int cnt = t <= g.ntiles ? a.tile_np[t] : 0;
+  int x = cnt;   // inclusive prefix inside the wave, then the waves before
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) ws[wave] = x;
+  __syncthreads();
+  int p0 = base + x - cnt;
+  for (int w = 0; w < wave; w++) p0 += ws[w];
+  if (t <= g.ntiles) a.tile_p0[t] = p0;
   const bool in = t < g.ntiles;
-  if (t <= g.ntiles) a.tile_np[t] = 0;
   bool active = false, hit = false;
   if (in) {
-    const int p0 = a.tile_p0[t];
     a.tile_cur[t] = p0;
-    active = a.tile_p0[t + 1] > p0;
+    active = cnt > 0;
     const int tz = t % g.nt[2], ty = (t / g.nt[2]) % g.nt[1], tx = t / (g.nt[2] * g.nt[1]);
-    // all neighbours' ranges in flight at once (no early exit)
+    // all neighbours' counts in flight at once (no early exit)
     const int up = a.S1 == 4 ? 1 : 0;
     const int zlo = a.dim == 3 ? -1 : 0, zhi = a.dim == 3 ? up : 0;
 #pragma unroll
@@ -186,22 +232,21 @@ __global__ void k_tiles(KArgs a) {
           const bool use = dx <= up && dy <= up && dz >= zlo && dz <= zhi && ux >= 0 &&
                            uy >= 0 && uz >= 0 && ux < g.nt[0] && uy < g.nt[1] && uz < g.nt[2];
           const int u = use ? (ux * g.nt[1] + uy) * g.nt[2] + uz : t;
-          hit |= a.tile_p0[u + 1] > a.tile_p0[u];
+          hit |= a.tile_np[u] > 0;
         }
   }
   // one atomic per wavefront appends to both lists (the counts are the low
   // and high halves of one 64-bit word, neither exceeds ntiles)
   const unsigned long long ma = __ballot(active), mh = __ballot(hit);
-  const int lane = threadIdx.x & 63;
   const int leader = __ffsll(ma | mh | 1ull) - 1;
-  unsigned long long base = 0;
+  unsigned long long lb = 0;
   if (lane == leader && (ma | mh))
-    base = atomicAdd((unsigned long long *)&a.sc->tile_count,
-                     (unsigned long long)__popcll(ma) | ((unsigned long long)__popcll(mh) << 32));
-  base = __shfl(base, leader, 64);
+    lb = atomicAdd((unsigned long long *)&a.sc->tile_count,
+                   (unsigned long long)__popcll(ma) | ((unsigned long long)__popcll(mh) << 32));
+  lb = __shfl(lb, leader, 64);
   const unsigned long long below = (1ull << lane) - 1ull;
-  const int aslot = (int)(base & 0xffffffffull) + __popcll(ma & below);
-  const int hslot = (int)(base >> 32) + __popcll(mh & below);
+  const int aslot = (int)(lb & 0xffffffffull) + __popcll(ma & below);
+  const int hslot = (int)(lb >> 32) + __popcll(mh & below);
   if (active) a.tile_list[aslot] = t;
   if (in) a.tile_slot[t] = active ? aslot : -1;
   if (hit) a.dil_list[hslot] = t;
@@ -596,76 +641,8 @@ hipError_t launch_keys(kmpm_ctx *c) {
   return hipGetLastError();
 }
 
-// Exclusive scan of the per-tile counts (n = tiles + 1 values, the last
-// zero) into the tiles' first positions, in two passes with no waits between
-// workgroups: k_scan_sums writes the total of each chunk of SCAN_CHUNK
-// counts; k_scan_apply adds the totals of the chunks before its own and
-// scans its chunk (SCAN_PPT consecutive counts per thread).
-constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_PPT = 8;
-constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_PPT;
-
-__device__ inline int wave_sum_i(int v) {
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
-
-// Sum of v over the block (every thread gets it); ws holds 4 ints.
-__device__ inline int block_sum_i(int v, int *ws) {
-  v = wave_sum_i(v);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const int s = ws[0] + ws[1] + ws[2] + ws[3];
-  __syncthreads();
-  return s;
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(const int *cnt, int n, int *part) {
-  __shared__ int ws[4];
-  const int b0 = blockIdx.x * SCAN_CHUNK;
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_PPT; k++) {
-    const int i = b0 + k * SCAN_BLOCK + (int)threadIdx.x;
-    s += i < n ? cnt[i] : 0;
-  }
-  s = block_sum_i(s, ws);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const int *cnt, int n,
-                                                           const int *part, int *out) {
-  __shared__ int ws[4];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int o = 0;
-  for (int q = tid; q < (int)blockIdx.x; q += SCAN_BLOCK) o += part[q];
-  const int base = block_sum_i(o, ws);
-  const int i0 = blockIdx.x * SCAN_CHUNK + tid * SCAN_PPT;
-  int v[SCAN_PPT], t = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_PPT; k++) {
-    v[k] = i0 + k < n ? cnt[i0 + k] : 0;
-    t += v[k];
-  }
-  // exclusive prefix of the thread totals: inside the wave, then the waves
-  int x = t;
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  if (lane == 63) ws[wave] = x;
-  __syncthreads();
-  int ex = base + x - t;
-  for (int w = 0; w < wave; w++) ex += ws[w];
-#pragma unroll
-  for (int k = 0; k < SCAN_PPT; k++) {
-    if (i0 + k < n) out[i0 + k] = ex;
-    ex += v[k];
-  }
-}
-
 size_t scan_temp_bytes(int64_t n) {
-  return (size_t)((n + SCAN_CHUNK - 1) / SCAN_CHUNK) * sizeof(int);
+  return (size_t)((n + SCAN_BLOCK - 1) / SCAN_BLOCK) * sizeof(int);
 }
 
 hipError_t launch_sort(kmpm_ctx *c) {
@@ -677,13 +654,12 @@ hipError_t launch_sort(kmpm_ctx *c) {
   else
     hipLaunchKernelGGL(k_sort_begin, dim3(1), dim3(64), 0, c->stream, a);
   {
-    const int n = (int)(nt + 1), nb = (int)nblk(n, SCAN_CHUNK);
+    const int nb = (int)nblk(nt + 1, SCAN_BLOCK);
     int *part = static_cast<int *>(c->d_scan_tmp);
-    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, a.tile_np, n, part);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, a.tile_np, n, part,
-                       a.tile_p0);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, a.tile_np,
+                       (int)(nt + 1), part);
+    hipLaunchKernelGGL(k_tiles, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, a, (const int *)part);
   }
-  hipLaunchKernelGGL(k_tiles, dim3(nblk(nt + 1, 256)), dim3(256), 0, c->stream, a);
   if (a.np > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblk(a.np, SORT_PER_BLOCK)), dim3(SORT_BLOCK), 0,
                        c->stream, a);
