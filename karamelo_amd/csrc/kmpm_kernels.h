@@ -712,13 +712,21 @@ __global__ __launch_bounds__(256) void k_p2g(KArgs a) {
         bool ok = true;
         if constexpr (FX) {
           // the three-instruction words unless a sum of the wave is beyond
-          // their range (then the exact split, which also reports overflow)
-          bool fast = true;
+          // their range (then the exact split, which also reports overflow).
+          // The largest magnitude per scale class, on the bit patterns of
+          // |sum|: they order as the values do, with infinities and NaNs
+          // above every finite value, so those take the exact split too.
+          uint32_t mb[3] = {0u, 0u, 0u};
 #pragma unroll
           for (int k = 0; k < S1; k++)
 #pragma unroll
-            for (int f = 0; f < NF; f++)
-              fast &= fabsf(acc[k][f]) < flim[f == 0 ? FX_MASS : f <= DIM ? FX_MOM : FX_FORCE];
+            for (int f = 0; f < NF; f++) {
+              const int c = f == 0 ? 0 : f <= DIM ? 1 : 2;
+              mb[c] = max(mb[c], __float_as_uint(acc[k][f]) & 0x7fffffffu);
+            }
+          const bool fast = mb[0] < __float_as_uint(flim[FX_MASS]) &&
+                            mb[1] < __float_as_uint(flim[FX_MOM]) &&
+                            mb[2] < __float_as_uint(flim[FX_FORCE]);
           if (__ballot(!fast) == 0) {
 #pragma unroll
             for (int k = 0; k < S1; k++)
