@@ -1462,8 +1462,11 @@ __device__ inline void store_heat(const KArgs &a, const OutSlot &o, int mat, dou
 template <int DIM, int S1, int TX, int TY, int TZ, int MC, int PREC, bool STAGE, bool XF = false>
 // Mixed linear and JC classes fit four waves per SIMD without spilling (the
 // fluid and generic classes would spill at that bound); the plain mixed
-// kernel takes 0.42 ms at four waves and 0.57 at five on the 4.27M bar.
-__global__ __launch_bounds__(256, STAGE ? 4
+// kernel takes 0.44 ms at four waves, 0.54 at three and 0.57 at five on the
+// 4.27M bar.  The staging kernel is bounded for three waves: it still fits
+// four (126 VGPRs), and the schedule made for three takes 0.31 ms against
+// 0.35 for the one made for four.
+__global__ __launch_bounds__(256, STAGE ? 3
                                         : (PREC == KMPM_PRECISION_MIXED && !XF &&
                                                    (MC == MC_LINEAR || MC == MC_JC_SHOCK)
                                                ? 4
